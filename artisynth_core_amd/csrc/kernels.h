@@ -1,0 +1,84 @@
+// Device kernels of the supernodal LDL^T solver (gfx950) and their launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace kkt {
+
+constexpr int NB = 64;    // panel width (pivot columns per panel step)
+constexpr int TILE = 64;  // assembly / Schur-update tile edge
+
+// Device copy of the supernodal structure (all arrays device pointers).
+struct DevSym {
+  const int* sn_first;       // first permuted column
+  const int* sn_k;           // pivot columns
+  const int* sn_m;           // front size k + r
+  const long long* sn_foff;  // front offset (elements), m*m column-major
+  const long long* sn_roff;  // offset into rows[] of the r below-diagonal rows
+  const long long* sn_woff;  // panel workspace offset (m * NB elements)
+  const long long* sn_uoff;  // solve update-vector offset (r elements)
+  const int* ch_ptr;         // children of s: ch_list[ch_ptr[s] .. ch_ptr[s+1])
+  const int* ch_list;
+  const int* rel;            // for child c: parent-front position of each CB row,
+                             // stored at rel[sn_roff[c] ...]
+  const int* rows;           // global permuted row index of each below-diagonal row
+  int* lperm;                // per permuted column: original local index in supernode
+  signed char* piv;          // 1 = 1x1, 2 = first of 2x2, 0 = second of 2x2
+};
+
+struct FactorCounters {  // device ints
+  int npos, nneg, npert, n2x2;
+};
+
+template <typename T>
+void launch_assemble(const int4* items, int nitems, const DevSym& d, T* F, hipStream_t st);
+template <typename T>
+void launch_scatter(const int* aidx, const long long* apos, int cnt, const double* vals, T* F,
+                    hipStream_t st);
+template <typename T>
+void launch_panel_diag(const int* items, int nitems, int p, const DevSym& d, T* F, double pert,
+                       const double* amax, FactorCounters* ctr, hipStream_t st);
+template <typename T>
+void launch_panel_trsm(const int2* items, int nitems, int p, const DevSym& d, T* F, T* W,
+                       hipStream_t st);
+template <typename T>
+void launch_panel_update(const int4* items, int nitems, int p, const DevSym& d, T* F,
+                         const T* W, hipStream_t st);
+
+// solves (vectors in the front precision T)
+template <typename T>
+void launch_fwd_level(const int* items, int nitems, const DevSym& d, const T* F, const T* bp,
+                      T* y, T* z, T* u, hipStream_t st);
+template <typename T>
+void launch_fwd_gemv(const int2* items, int nitems, const DevSym& d, const T* F, const T* y,
+                     T* u, hipStream_t st);
+template <typename T>
+void launch_bwd_gemv(const int2* items, int nitems, const DevSym& d, const T* F, const T* xp,
+                     T* t, hipStream_t st);
+template <typename T>
+void launch_bwd_level(const int* items, int nitems, const DevSym& d, const T* F, const T* z,
+                      const T* t, T* xp, hipStream_t st);
+
+// permutation / precision conversion: out[j] = (TO) in[perm[j]]  (gather)
+template <typename TI, typename TO>
+void launch_gather_perm(int n, const int* perm, const TI* in, TO* out, hipStream_t st);
+// out[perm[j]] = (TO) in[j]  (scatter)
+template <typename TI, typename TO>
+void launch_scatter_perm(int n, const int* perm, const TI* in, TO* out, hipStream_t st);
+
+// sparse symmetric matrix (full CSR copy) in FP64
+void launch_expand_vals(long long nnzf, const int* map, const double* vals, double* valsf,
+                        hipStream_t st);
+// y = A x (mode 0) or y = b - A x (mode 1)
+void launch_spmv(int n, const int* rowptr, const int* col, const double* val, const double* x,
+                 const double* b, double* y, int mode, hipStream_t st);
+void launch_absmax(long long nnz, const double* v, double* partial, double* out, hipStream_t st);
+// out[slot] = sum a.*b  (deterministic two-stage)
+void launch_dot(int n, const double* a, const double* b, double* partial, double* out,
+                hipStream_t st);
+// generic fused vector op: y = alpha*x + beta*y (+ gamma*z), scalars from device when ptrs given
+void launch_axpby(int n, double alpha, const double* x, double beta, double* y, hipStream_t st);
+void launch_axpbypcz(int n, double a, const double* x, double b, const double* y, double c,
+                     double* z, hipStream_t st);
+
+}  // namespace kkt

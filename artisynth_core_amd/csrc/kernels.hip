@@ -1,0 +1,791 @@
+// gfx950 kernels of the supernodal multifrontal LDL^T solver.
+//
+// Fronts: one dense m x m column-major array per supernode (lower triangle
+// used), resident in HBM.  The numeric factorization runs level by level over
+// the assembly tree; inside a level, all fronts advance together one 64-column
+// panel step at a time:
+//   assemble   : zero + extend-add of the children's contribution blocks
+//                (tile-owned gather, deterministic) + scatter of A's entries
+//   panel_diag : Bunch-Kaufman LDL^T of the 64x64 diagonal block (FP64 in LDS)
+//   panel_trsm : L21 = A21 L11^-T D^-1 for the rows below, W = L21 D
+//   panel_update: C -= W L^T on 64x64 tiles with MFMA (f64 or f32)
+// Solves run level by level (forward bottom-up, backward top-down) with the
+// update vectors of the multifrontal scheme, so no floating-point atomics are
+// used anywhere: results are bitwise reproducible run to run.
+#include "kernels.h"
+
+namespace kkt {
+
+// ------------------------------------------------------------------------
+// wave helpers (64-wide)
+// ------------------------------------------------------------------------
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+// max value, lowest index among equal maxima; result in all lanes
+__device__ inline void wave_argmax(double& v, int& idx) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double ov = __shfl_xor(v, off, 64);
+    int oi = __shfl_xor(idx, off, 64);
+    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+  }
+}
+template <typename T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ inline int lower_bound_i(const int* a, int n, int key) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// ------------------------------------------------------------------------
+// assembly: one work item = one 64x64 lower tile of a parent front
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_assemble(const int4* __restrict__ items, DevSym d,
+                                                  T* __restrict__ F) {
+  __shared__ T tile[TILE * (TILE + 1)];  // tile[c*65 + r]
+  const int4 it = items[blockIdx.x];
+  const int s = it.x, I = it.y, J = it.z;
+  const int m = d.sn_m[s];
+  const int r0 = I * TILE, c0 = J * TILE;
+  const int nr = min(TILE, m - r0), nc = min(TILE, m - c0);
+  for (int e = threadIdx.x; e < TILE * (TILE + 1); e += 256) tile[e] = (T)0;
+  __syncthreads();
+  for (int q = d.ch_ptr[s]; q < d.ch_ptr[s + 1]; q++) {
+    const int c = d.ch_list[q];
+    const int kc = d.sn_k[c], mc = d.sn_m[c], rc = mc - kc;
+    const int* rel = d.rel + d.sn_roff[c];
+    const int ia = lower_bound_i(rel, rc, r0), ib = lower_bound_i(rel, rc, r0 + nr);
+    const int ja = lower_bound_i(rel, rc, c0), jb = lower_bound_i(rel, rc, c0 + nc);
+    if (ia < ib && ja < jb) {
+      const T* CB = F + d.sn_foff[c] + (size_t)kc * mc + kc;
+      for (int j = ja + (threadIdx.x >> 6); j < jb; j += 4) {
+        const int cj = rel[j] - c0;
+        const T* col = CB + (size_t)j * mc;
+        for (int i = max(ia, j) + (threadIdx.x & 63); i < ib; i += 64) {
+          tile[cj * (TILE + 1) + (rel[i] - r0)] += col[i];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  T* Fs = F + d.sn_foff[s];
+  for (int e = threadIdx.x; e < TILE * TILE; e += 256) {
+    const int rr = e & (TILE - 1), cc = e / TILE;
+    if (rr < nr && cc < nc && r0 + rr >= c0 + cc)
+      Fs[(size_t)(c0 + cc) * m + r0 + rr] = tile[cc * (TILE + 1) + rr];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ aidx,
+                                                 const long long* __restrict__ apos, int cnt,
+                                                 const double* __restrict__ vals,
+                                                 T* __restrict__ F) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e < cnt) F[apos[e]] += (T)vals[aidx[e]];
+}
+
+// ------------------------------------------------------------------------
+// panel diagonal block: Bunch-Kaufman restricted to the block, FP64 in LDS
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items, int p, DevSym d,
+                                                   T* __restrict__ F, double pert,
+                                                   const double* __restrict__ amax,
+                                                   FactorCounters* __restrict__ ctr) {
+  __shared__ double A[64 * 65];  // A[i*65+j], lower part (i >= j)
+  __shared__ int lp[64];
+  __shared__ signed char pt[64];
+  const int s = items[blockIdx.x];
+  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s];
+  const int j0 = p * NB, w = min(NB, k - j0);
+  T* Fs = F + d.sn_foff[s];
+  const int t = threadIdx.x;
+  for (int j = 0; j < w; j++)
+    if (t >= j && t < w) A[t * 65 + j] = (double)Fs[(size_t)(j0 + j) * m + j0 + t];
+  lp[t] = t;
+  pt[t] = 1;
+  __syncthreads();
+  const double alpha = (1.0 + sqrt(17.0)) / 8.0;
+  const double tau = pert * amax[0];
+  int npos = 0, nneg = 0, npert = 0, n2 = 0;
+  int kc = 0;
+  while (kc < w) {
+    const double absakk = fabs(A[kc * 65 + kc]);
+    double cv = (t > kc && t < w) ? fabs(A[t * 65 + kc]) : -1.0;
+    int ci = t;
+    wave_argmax(cv, ci);
+    const double colmax = cv > 0.0 ? cv : 0.0;
+    const int imax = cv > 0.0 ? ci : -1;
+    int kstep = 1, kp = kc;
+    bool perturb = false;
+    if (fmax(absakk, colmax) < tau) {
+      perturb = true;
+    } else if (absakk >= alpha * colmax) {
+      kp = kc;
+    } else {
+      double rv = 0.0;
+      if (t >= kc && t < w && t != imax)
+        rv = (t < imax) ? fabs(A[imax * 65 + t]) : fabs(A[t * 65 + imax]);
+      const double rowmax = wave_max(rv);
+      if (absakk * rowmax >= alpha * colmax * colmax) kp = kc;
+      else if (fabs(A[imax * 65 + imax]) >= alpha * rowmax) kp = imax;
+      else { kp = imax; kstep = 2; }
+    }
+    const int kk = kc + kstep - 1;
+    __syncthreads();
+    if (kp != kk) {
+      const int a = kk, b = kp;
+      if (t == 0) {
+        double tmp = A[a * 65 + a]; A[a * 65 + a] = A[b * 65 + b]; A[b * 65 + b] = tmp;
+        int ti = lp[a]; lp[a] = lp[b]; lp[b] = ti;
+      }
+      if (t > a && t < b) { double tmp = A[t * 65 + a]; A[t * 65 + a] = A[b * 65 + t]; A[b * 65 + t] = tmp; }
+      if (t > b && t < w) { double tmp = A[t * 65 + a]; A[t * 65 + a] = A[t * 65 + b]; A[t * 65 + b] = tmp; }
+      if (t < a) { double tmp = A[a * 65 + t]; A[a * 65 + t] = A[b * 65 + t]; A[b * 65 + t] = tmp; }
+      for (int c = t; c < j0; c += 64) {
+        const size_t o = (size_t)c * m;
+        T tmp = Fs[o + j0 + a]; Fs[o + j0 + a] = Fs[o + j0 + b]; Fs[o + j0 + b] = tmp;
+      }
+      __syncthreads();
+    }
+    if (kstep == 1) {
+      double dd = A[kc * 65 + kc];
+      const bool pz = perturb || fabs(dd) < tau;
+      if (pz) dd = (dd >= 0.0) ? tau : -tau;
+      __syncthreads();
+      if (t == 0) {
+        A[kc * 65 + kc] = dd;
+        pt[kc] = 1;
+        if (pz) npert++;
+        if (dd > 0) npos++; else nneg++;
+      }
+      const double dinv = 1.0 / dd;
+      if (t > kc && t < w) {
+        const double aik = A[t * 65 + kc];
+        for (int j = kc + 1; j <= t; j++) A[t * 65 + j] -= aik * (A[j * 65 + kc] * dinv);
+      }
+      __syncthreads();
+      if (t > kc && t < w) A[t * 65 + kc] *= dinv;
+      __syncthreads();
+    } else {
+      const int g1 = kc, g2 = kc + 1;
+      const double d11 = A[g1 * 65 + g1], d21 = A[g2 * 65 + g1], d22 = A[g2 * 65 + g2];
+      const double det = d11 * d22 - d21 * d21;
+      const double i11 = d22 / det, i21 = -d21 / det, i22 = d11 / det;
+      if (t == 0) {
+        if (det < 0) { npos++; nneg++; }
+        else if (d11 + d22 > 0) npos += 2;
+        else nneg += 2;
+        n2++;
+        pt[g1] = 2;
+        pt[g2] = 0;
+      }
+      if (t > g2 && t < w) {
+        const double a1 = A[t * 65 + g1], a2 = A[t * 65 + g2];
+        for (int j = kc + 2; j <= t; j++) {
+          const double b1 = A[j * 65 + g1], b2 = A[j * 65 + g2];
+          const double l1 = b1 * i11 + b2 * i21, l2 = b1 * i21 + b2 * i22;
+          A[t * 65 + j] -= a1 * l1 + a2 * l2;
+        }
+      }
+      __syncthreads();
+      if (t > g2 && t < w) {
+        const double a1 = A[t * 65 + g1], a2 = A[t * 65 + g2];
+        A[t * 65 + g1] = a1 * i11 + a2 * i21;
+        A[t * 65 + g2] = a1 * i21 + a2 * i22;
+      }
+      __syncthreads();
+    }
+    kc += kstep;
+  }
+  __syncthreads();
+  // inverse of the unit lower L11 (entries (q+1,q) of 2x2 pivots hold D, not L):
+  // lane c forms column c of L11^-1 into the upper triangle, A[c][r] = Linv[r][c]
+  if (t < w) {
+    const int c = t;
+    for (int r = c + 1; r < w; r++) {
+      double acc = (r == c + 1 && pt[c] == 2) ? 0.0 : -A[r * 65 + c];
+      for (int l = c + 1; l < r; l++) {
+        const double lrl = (l == r - 1 && pt[l] == 2) ? 0.0 : A[r * 65 + l];
+        acc -= lrl * A[c * 65 + l];
+      }
+      A[c * 65 + r] = acc;
+    }
+  }
+  __syncthreads();
+  // lower: L and D; upper: L11^-1 (front positions above the diagonal are unused)
+  for (int j = 0; j < w; j++)
+    if (t < w) Fs[(size_t)(j0 + j) * m + j0 + t] = (T)A[t * 65 + j];
+  if (t < w) {
+    d.lperm[first + j0 + t] = j0 + lp[t];
+    d.piv[first + j0 + t] = pt[t];
+  }
+  if (t == 0) {
+    atomicAdd(&ctr->npos, npos);
+    atomicAdd(&ctr->nneg, nneg);
+    atomicAdd(&ctr->npert, npert);
+    atomicAdd(&ctr->n2x2, n2);
+  }
+}
+
+// ------------------------------------------------------------------------
+// MFMA 16x16x4 wrappers (f64 / f32).  Lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; the 16x16 result sits in lane l as C[4*(l/16)+v][l%16].
+// ------------------------------------------------------------------------
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 op(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 op(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// ------------------------------------------------------------------------
+// panel TRSM as a GEMM with the stored inverse: W = A21 L11^-T, L21 = W D^-1
+// item = (supernode, 64-row block below the diagonal block)
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_trsm(const int2* __restrict__ items, int p,
+                                                    DevSym d, T* __restrict__ F,
+                                                    T* __restrict__ Wb) {
+  constexpr int LD = 65;
+  __shared__ T As[64 * LD];  // As[r*LD + l]: row r, permuted panel column l
+  __shared__ T Li[64 * LD];  // Li[q*LD + l] = Linv[q][l]
+  __shared__ T Dg[64], Dsub[64];
+  __shared__ int sig[64];
+  __shared__ signed char pv[64];
+  typedef typename Mfma<T>::v4 v4;
+  const int2 it = items[blockIdx.x];
+  const int s = it.x, rb = it.y;
+  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s];
+  const int j0 = p * NB, w = min(NB, k - j0), r0 = j0 + w;
+  const int rbase = r0 + rb * 64, nr = min(64, m - rbase);
+  T* Fs = F + d.sn_foff[s];
+  if (threadIdx.x < 64) {
+    const int q = threadIdx.x;
+    sig[q] = 0; pv[q] = 1; Dg[q] = (T)1; Dsub[q] = (T)0;
+    if (q < w) {
+      sig[q] = d.lperm[first + j0 + q] - j0;
+      pv[q] = d.piv[first + j0 + q];
+      Dg[q] = Fs[(size_t)(j0 + q) * m + j0 + q];
+      if (q + 1 < w) Dsub[q] = Fs[(size_t)(j0 + q) * m + j0 + q + 1];
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int c = e >> 6, r = e & 63;  // column c of the source, row r
+    // Linv[c][r] for r < c lives at front (row j0+r, col j0+c)
+    T v = (T)0;
+    if (c < w && r < w) v = (r < c) ? Fs[(size_t)(j0 + c) * m + j0 + r] : (r == c ? (T)1 : (T)0);
+    Li[c * LD + r] = v;
+    As[r * LD + c] = (r < nr && c < w) ? Fs[(size_t)(j0 + sig[c]) * m + rbase + r] : (T)0;
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  v4 acc[2][2];
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++) acc[bi][bj] = (v4)((T)0);
+  const int kmax = (w + 3) & ~3;
+  for (int kk = 0; kk < kmax; kk += 4) {
+    const T a0 = As[(wr + lr) * LD + kk + lk];
+    const T a1 = As[(wr + 16 + lr) * LD + kk + lk];
+    const T b0 = Li[(wc + lr) * LD + kk + lk];
+    const T b1 = Li[(wc + 16 + lr) * LD + kk + lk];
+    acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
+    acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
+    acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+    acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        As[(wr + bi * 16 + 4 * lk + v) * LD + wc + bj * 16 + lr] = acc[bi][bj][v];
+  __syncthreads();
+  T* Ws = Wb + d.sn_woff[s];
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int q = e >> 6, r = e & 63;
+    if (r >= nr || q >= w) continue;
+    const T xq = As[r * LD + q];
+    T l;
+    const int pq = pv[q];
+    if (pq == 1) {
+      l = xq / Dg[q];
+    } else if (pq == 2) {
+      const T det = Dg[q] * Dg[q + 1] - Dsub[q] * Dsub[q];
+      l = (xq * Dg[q + 1] - As[r * LD + q + 1] * Dsub[q]) / det;
+    } else {
+      const T det = Dg[q - 1] * Dg[q] - Dsub[q - 1] * Dsub[q - 1];
+      l = (xq * Dg[q - 1] - As[r * LD + q - 1] * Dsub[q - 1]) / det;
+    }
+    Ws[(size_t)q * m + rbase + r] = xq;
+    Fs[(size_t)(j0 + q) * m + rbase + r] = l;
+  }
+}
+
+// ------------------------------------------------------------------------
+// Schur update on 64x64 tiles with MFMA: C -= W L^T
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_update(const int4* __restrict__ items, int p,
+                                                      DevSym d, T* __restrict__ F,
+                                                      const T* __restrict__ Wb) {
+  constexpr int LD = 65;
+  __shared__ T Ws[64 * LD];  // Ws[r*LD + t]
+  __shared__ T Ls[64 * LD];
+  typedef typename Mfma<T>::v4 v4;
+  const int4 it = items[blockIdx.x];
+  const int s = it.x, I = it.y, J = it.z;
+  const int k = d.sn_k[s], m = d.sn_m[s];
+  const int j0 = p * NB, w = min(NB, k - j0), r0 = j0 + w;
+  const int rI = r0 + I * TILE, rJ = r0 + J * TILE;
+  const int nI = min(TILE, m - rI), nJ = min(TILE, m - rJ);
+  const T* Wsrc = Wb + d.sn_woff[s];
+  T* Fs = F + d.sn_foff[s];
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int t = e >> 6, r = e & 63;
+    Ws[r * LD + t] = (r < nI && t < w) ? Wsrc[(size_t)t * m + rI + r] : (T)0;
+    Ls[r * LD + t] = (r < nJ && t < w) ? Fs[(size_t)(j0 + t) * m + rJ + r] : (T)0;
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  v4 acc[2][2];
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int row = wr + bi * 16 + 4 * lk + v, col = wc + bj * 16 + lr;
+        acc[bi][bj][v] = (row < nI && col < nJ) ? Fs[(size_t)(rJ + col) * m + rI + row] : (T)0;
+      }
+  const int kmax = (w + 3) & ~3;
+  for (int kk = 0; kk < kmax; kk += 4) {
+    const T a0 = -Ws[(wr + lr) * LD + kk + lk];
+    const T a1 = -Ws[(wr + 16 + lr) * LD + kk + lk];
+    const T b0 = Ls[(wc + lr) * LD + kk + lk];
+    const T b1 = Ls[(wc + 16 + lr) * LD + kk + lk];
+    acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
+    acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
+    acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+    acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+  }
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int row = wr + bi * 16 + 4 * lk + v, col = wc + bj * 16 + lr;
+        if (row < nI && col < nJ && rI + row >= rJ + col)
+          Fs[(size_t)(rJ + col) * m + rI + row] = acc[bi][bj][v];
+      }
+}
+
+// ------------------------------------------------------------------------
+// forward solve of one level: assemble update vectors, L11 solve, D^-1
+// ------------------------------------------------------------------------
+constexpr int KMAX = 1024;  // widest supernode (the symbolic analysis splits wider ones)
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_level(const int* __restrict__ items, DevSym d,
+                                                   const T* __restrict__ F,
+                                                   const T* __restrict__ bp, T* __restrict__ y,
+                                                   T* __restrict__ z, T* __restrict__ u) {
+  __shared__ T g[KMAX];
+  __shared__ T yv[KMAX];
+  __shared__ T Lb[64 * 65];  // Lb[c*65 + r] = Linv[r][c] of the current 64-column block
+  const int s = items[blockIdx.x];
+  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s], r = m - k;
+  const T* Fs = F + d.sn_foff[s];
+  T* us = u + d.sn_uoff[s];
+  const int tid = threadIdx.x;
+  for (int t = tid; t < k; t += 256) g[t] = bp[first + t];
+  for (int i = tid; i < r; i += 256) us[i] = (T)0;
+  __syncthreads();
+  for (int q = d.ch_ptr[s]; q < d.ch_ptr[s + 1]; q++) {
+    const int c = d.ch_list[q];
+    const int rc = d.sn_m[c] - d.sn_k[c];
+    const int* relc = d.rel + d.sn_roff[c];
+    const T* uc = u + d.sn_uoff[c];
+    for (int i = tid; i < rc; i += 256) {
+      const int pos = relc[i];
+      const T v = uc[i];
+      if (pos < k) g[pos] += v; else us[pos - k] += v;
+    }
+    __syncthreads();
+  }
+  for (int t = tid; t < k; t += 256) yv[t] = g[d.lperm[first + t]];
+  __syncthreads();
+  for (int c0 = 0; c0 < k; c0 += 64) {
+    const int c1 = min(k, c0 + 64), bw = c1 - c0;
+    // Lb[c*65 + r] = Linv[r][c] of this 64-column block (stored above the diagonal)
+    for (int e = tid; e < 64 * 64; e += 256) {
+      const int cc = e >> 6, rr = e & 63;
+      T v = (T)0;
+      if (cc < bw && rr < bw) v = (rr > cc) ? Fs[(size_t)(c0 + rr) * m + c0 + cc] : (rr == cc ? (T)1 : (T)0);
+      Lb[cc * 65 + rr] = v;
+    }
+    __syncthreads();
+    T ynew = (T)0;
+    if (tid < bw) {
+      for (int c = 0; c <= tid; c++) ynew += Lb[c * 65 + tid] * yv[c0 + c];
+    }
+    __syncthreads();
+    if (tid < bw) yv[c0 + tid] = ynew;
+    __syncthreads();
+    for (int i = c1 + tid; i < k; i += 256) {
+      T acc = yv[i];
+      for (int q = 0; q < bw; q++) acc -= Fs[(size_t)(c0 + q) * m + i] * yv[c0 + q];
+      yv[i] = acc;
+    }
+    __syncthreads();
+  }
+  for (int t = tid; t < k; t += 256) {
+    const int pq = d.piv[first + t];
+    T val;
+    if (pq == 1) {
+      val = yv[t] / Fs[(size_t)t * m + t];
+    } else if (pq == 2) {
+      const T d11 = Fs[(size_t)t * m + t], d21 = Fs[(size_t)t * m + t + 1],
+              d22 = Fs[(size_t)(t + 1) * m + t + 1];
+      const T det = d11 * d22 - d21 * d21;
+      val = (yv[t] * d22 - yv[t + 1] * d21) / det;
+    } else {
+      const T d11 = Fs[(size_t)(t - 1) * m + t - 1], d21 = Fs[(size_t)(t - 1) * m + t],
+              d22 = Fs[(size_t)t * m + t];
+      const T det = d11 * d22 - d21 * d21;
+      val = (yv[t] * d11 - yv[t - 1] * d21) / det;
+    }
+    z[first + t] = val;
+    y[first + t] = yv[t];
+  }
+}
+
+// u_s[i] -= sum_t L(k+i, t) y_t ; item = (s, 64-row block); 4 waves split columns
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_gemv(const int2* __restrict__ items, DevSym d,
+                                                  const T* __restrict__ F,
+                                                  const T* __restrict__ y, T* __restrict__ u) {
+  __shared__ T ys[KMAX];
+  __shared__ T part[4][64];
+  const int2 it = items[blockIdx.x];
+  const int s = it.x, rb = it.y;
+  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s], r = m - k;
+  const T* Fs = F + d.sn_foff[s];
+  for (int t = threadIdx.x; t < k; t += 256) ys[t] = y[first + t];
+  __syncthreads();
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = rb * 64 + lane;
+  T acc = (T)0;
+  if (i < r) {
+    const T* col = Fs + k + i;
+    for (int q = wv; q < k; q += 4) acc += col[(size_t)q * m] * ys[q];
+  }
+  part[wv][lane] = acc;
+  __syncthreads();
+  if (wv == 0 && i < r) {
+    T* us = u + d.sn_uoff[s];
+    us[i] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+  }
+}
+
+// t_s[q] = sum_i L(k+i, q) xp[rows_i] ; item = (s, block of 4 columns), wave per column
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_gemv(const int2* __restrict__ items, DevSym d,
+                                                  const T* __restrict__ F,
+                                                  const T* __restrict__ xp, T* __restrict__ tb) {
+  const int2 it = items[blockIdx.x];
+  const int s = it.x, cb = it.y;
+  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s], r = m - k;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int q = cb * 4 + wv;
+  if (q >= k) return;
+  const T* col = F + d.sn_foff[s] + (size_t)q * m + k;
+  const int* rows = d.rows + d.sn_roff[s];
+  T acc = (T)0;
+  for (int i = lane; i < r; i += 64) acc += col[i] * xp[rows[i]];
+  acc = wave_sum(acc);
+  if (lane == 0) tb[first + q] = acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_level(const int* __restrict__ items, DevSym d,
+                                                   const T* __restrict__ F,
+                                                   const T* __restrict__ z,
+                                                   const T* __restrict__ tb,
+                                                   T* __restrict__ xp) {
+  __shared__ T vv[KMAX];
+  __shared__ T Lb[64 * 65];  // Lb[c*65 + r] = Linv[r][c] of the current 64-column block
+  const int s = items[blockIdx.x];
+  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s], r = m - k;
+  const T* Fs = F + d.sn_foff[s];
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  for (int t = tid; t < k; t += 256) vv[t] = z[first + t] - (r > 0 ? tb[first + t] : (T)0);
+  __syncthreads();
+  int c1 = k;
+  while (c1 > 0) {
+    const int c0 = ((c1 - 1) / 64) * 64, bw = c1 - c0;
+    // columns of this block against the already solved part of the pivot block
+    if (c1 < k) {
+      for (int q = c0 + wv; q < c1; q += 4) {
+        const T* col = Fs + (size_t)q * m;
+        T acc = (T)0;
+        for (int i = c1 + lane; i < k; i += 64) acc += col[i] * vv[i];
+        acc = wave_sum(acc);
+        if (lane == 0) vv[q] -= acc;
+      }
+    }
+    for (int e = tid; e < 64 * 64; e += 256) {
+      const int cc = e >> 6, rr = e & 63;
+      T v = (T)0;
+      if (cc < bw && rr < bw) v = (rr > cc) ? Fs[(size_t)(c0 + rr) * m + c0 + cc] : (rr == cc ? (T)1 : (T)0);
+      Lb[cc * 65 + rr] = v;
+    }
+    __syncthreads();
+    T xnew = (T)0;
+    if (tid < bw) {
+      for (int r = tid; r < bw; r++) xnew += Lb[tid * 65 + r] * vv[c0 + r];
+    }
+    __syncthreads();
+    if (tid < bw) vv[c0 + tid] = xnew;
+    __syncthreads();
+    c1 = c0;
+  }
+  for (int t = tid; t < k; t += 256) xp[first + d.lperm[first + t]] = vv[t];
+}
+
+// ------------------------------------------------------------------------
+// vectors and SpMV
+// ------------------------------------------------------------------------
+template <typename TI, typename TO>
+__global__ void k_gather_perm(int n, const int* __restrict__ perm, const TI* __restrict__ in,
+                              TO* __restrict__ out) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j < n) out[j] = (TO)in[perm[j]];
+}
+template <typename TI, typename TO>
+__global__ void k_scatter_perm(int n, const int* __restrict__ perm, const TI* __restrict__ in,
+                               TO* __restrict__ out) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j < n) out[perm[j]] = (TO)in[j];
+}
+
+__global__ void k_expand_vals(long long nnzf, const int* __restrict__ map,
+                              const double* __restrict__ vals, double* __restrict__ valsf) {
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e < nnzf) valsf[e] = vals[map[e]];
+}
+
+// one wave per row
+__global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ rowptr,
+                                              const int* __restrict__ col,
+                                              const double* __restrict__ val,
+                                              const double* __restrict__ x,
+                                              const double* __restrict__ b,
+                                              double* __restrict__ y, int mode) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  double acc = 0.0;
+  for (int e = rowptr[row] + lane; e < rowptr[row + 1]; e += 64) acc += val[e] * x[col[e]];
+  acc = wave_sum(acc);
+  if (lane == 0) y[row] = mode ? (b[row] - acc) : acc;
+}
+
+constexpr int RED_BLOCKS = 512;
+
+__global__ __launch_bounds__(256) void k_dot_partial(int n, const double* __restrict__ a,
+                                                     const double* __restrict__ b,
+                                                     double* __restrict__ partial) {
+  __shared__ double sm[4];
+  double acc = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += RED_BLOCKS * 256) acc += a[i] * b[i];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+}
+__global__ __launch_bounds__(256) void k_reduce_final(const double* __restrict__ partial, int np,
+                                                      double* __restrict__ out, int is_max) {
+  __shared__ double sm[4];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) acc = is_max ? fmax(acc, partial[i]) : acc + partial[i];
+  if (is_max) acc = wave_max(acc); else acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    *out = is_max ? fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3])) : (sm[0] + sm[1]) + (sm[2] + sm[3]);
+}
+__global__ __launch_bounds__(256) void k_absmax_partial(long long n, const double* __restrict__ v,
+                                                        double* __restrict__ partial) {
+  __shared__ double sm[4];
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)RED_BLOCKS * 256)
+    acc = fmax(acc, fabs(v[i]));
+  acc = wave_max(acc);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+}
+
+__global__ void k_axpby(int n, double alpha, const double* __restrict__ x, double beta,
+                        double* __restrict__ y) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) y[i] = alpha * x[i] + beta * y[i];
+}
+__global__ void k_axpbypcz(int n, double a, const double* __restrict__ x, double b,
+                           const double* __restrict__ y, double c, double* __restrict__ z) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) z[i] = a * x[i] + b * y[i] + c * z[i];
+}
+
+// ------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------
+static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+template <typename T>
+void launch_assemble(const int4* items, int nitems, const DevSym& d, T* F, hipStream_t st) {
+  if (nitems > 0) hipLaunchKernelGGL(k_assemble<T>, dim3(nitems), dim3(256), 0, st, items, d, F);
+}
+template <typename T>
+void launch_scatter(const int* aidx, const long long* apos, int cnt, const double* vals, T* F,
+                    hipStream_t st) {
+  if (cnt > 0)
+    hipLaunchKernelGGL(k_scatter<T>, dim3(nblk(cnt, 256)), dim3(256), 0, st, aidx, apos, cnt, vals, F);
+}
+template <typename T>
+void launch_panel_diag(const int* items, int nitems, int p, const DevSym& d, T* F, double pert,
+                       const double* amax, FactorCounters* ctr, hipStream_t st) {
+  if (nitems > 0)
+    hipLaunchKernelGGL(k_panel_diag<T>, dim3(nitems), dim3(64), 0, st, items, p, d, F, pert, amax,
+                       ctr);
+}
+template <typename T>
+void launch_panel_trsm(const int2* items, int nitems, int p, const DevSym& d, T* F, T* W,
+                       hipStream_t st) {
+  if (nitems > 0)
+    hipLaunchKernelGGL(k_panel_trsm<T>, dim3(nitems), dim3(256), 0, st, items, p, d, F, W);
+}
+template <typename T>
+void launch_panel_update(const int4* items, int nitems, int p, const DevSym& d, T* F,
+                         const T* W, hipStream_t st) {
+  if (nitems > 0)
+    hipLaunchKernelGGL(k_panel_update<T>, dim3(nitems), dim3(256), 0, st, items, p, d, F, W);
+}
+template <typename T>
+void launch_fwd_level(const int* items, int nitems, const DevSym& d, const T* F, const T* bp,
+                      T* y, T* z, T* u, hipStream_t st) {
+  if (nitems > 0)
+    hipLaunchKernelGGL(k_fwd_level<T>, dim3(nitems), dim3(256), 0, st, items, d, F, bp, y, z, u);
+}
+template <typename T>
+void launch_fwd_gemv(const int2* items, int nitems, const DevSym& d, const T* F, const T* y,
+                     T* u, hipStream_t st) {
+  if (nitems > 0)
+    hipLaunchKernelGGL(k_fwd_gemv<T>, dim3(nitems), dim3(256), 0, st, items, d, F, y, u);
+}
+template <typename T>
+void launch_bwd_gemv(const int2* items, int nitems, const DevSym& d, const T* F, const T* xp,
+                     T* t, hipStream_t st) {
+  if (nitems > 0)
+    hipLaunchKernelGGL(k_bwd_gemv<T>, dim3(nitems), dim3(256), 0, st, items, d, F, xp, t);
+}
+template <typename T>
+void launch_bwd_level(const int* items, int nitems, const DevSym& d, const T* F, const T* z,
+                      const T* t, T* xp, hipStream_t st) {
+  if (nitems > 0)
+    hipLaunchKernelGGL(k_bwd_level<T>, dim3(nitems), dim3(256), 0, st, items, d, F, z, t, xp);
+}
+template <typename TI, typename TO>
+void launch_gather_perm(int n, const int* perm, const TI* in, TO* out, hipStream_t st) {
+  hipLaunchKernelGGL((k_gather_perm<TI, TO>), dim3(nblk(n, 256)), dim3(256), 0, st, n, perm, in, out);
+}
+template <typename TI, typename TO>
+void launch_scatter_perm(int n, const int* perm, const TI* in, TO* out, hipStream_t st) {
+  hipLaunchKernelGGL((k_scatter_perm<TI, TO>), dim3(nblk(n, 256)), dim3(256), 0, st, n, perm, in, out);
+}
+void launch_expand_vals(long long nnzf, const int* map, const double* vals, double* valsf,
+                        hipStream_t st) {
+  if (nnzf > 0)
+    hipLaunchKernelGGL(k_expand_vals, dim3(nblk(nnzf, 256)), dim3(256), 0, st, nnzf, map, vals, valsf);
+}
+void launch_spmv(int n, const int* rowptr, const int* col, const double* val, const double* x,
+                 const double* b, double* y, int mode, hipStream_t st) {
+  hipLaunchKernelGGL(k_spmv, dim3(nblk(n, 4)), dim3(256), 0, st, n, rowptr, col, val, x, b, y, mode);
+}
+void launch_absmax(long long nnz, const double* v, double* partial, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_absmax_partial, dim3(RED_BLOCKS), dim3(256), 0, st, nnz, v, partial);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, (const double*)partial, RED_BLOCKS, out, 1);
+}
+void launch_dot(int n, const double* a, const double* b, double* partial, double* out,
+                hipStream_t st) {
+  hipLaunchKernelGGL(k_dot_partial, dim3(RED_BLOCKS), dim3(256), 0, st, n, a, b, partial);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, (const double*)partial, RED_BLOCKS, out, 0);
+}
+void launch_axpby(int n, double alpha, const double* x, double beta, double* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpby, dim3(nblk(n, 256)), dim3(256), 0, st, n, alpha, x, beta, y);
+}
+void launch_axpbypcz(int n, double a, const double* x, double b, const double* y, double c,
+                     double* z, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpbypcz, dim3(nblk(n, 256)), dim3(256), 0, st, n, a, x, b, y, c, z);
+}
+
+#define KKT_INST(T)                                                                              \
+  template void launch_assemble<T>(const int4*, int, const DevSym&, T*, hipStream_t);            \
+  template void launch_scatter<T>(const int*, const long long*, int, const double*, T*,          \
+                                  hipStream_t);                                                  \
+  template void launch_panel_diag<T>(const int*, int, int, const DevSym&, T*, double,           \
+                                     const double*, FactorCounters*, hipStream_t);               \
+  template void launch_panel_trsm<T>(const int2*, int, int, const DevSym&, T*, T*, hipStream_t); \
+  template void launch_panel_update<T>(const int4*, int, int, const DevSym&, T*, const T*,       \
+                                       hipStream_t);                                             \
+  template void launch_fwd_level<T>(const int*, int, const DevSym&, const T*, const T*, T*, T*,  \
+                                    T*, hipStream_t);                                            \
+  template void launch_fwd_gemv<T>(const int2*, int, const DevSym&, const T*, const T*, T*,      \
+                                   hipStream_t);                                                 \
+  template void launch_bwd_gemv<T>(const int2*, int, const DevSym&, const T*, const T*, T*,      \
+                                   hipStream_t);                                                 \
+  template void launch_bwd_level<T>(const int*, int, const DevSym&, const T*, const T*,          \
+                                    const T*, T*, hipStream_t);
+KKT_INST(double)
+KKT_INST(float)
+template void launch_gather_perm<double, double>(int, const int*, const double*, double*, hipStream_t);
+template void launch_gather_perm<double, float>(int, const int*, const double*, float*, hipStream_t);
+template void launch_scatter_perm<double, double>(int, const int*, const double*, double*, hipStream_t);
+template void launch_scatter_perm<float, double>(int, const int*, const float*, double*, hipStream_t);
+
+}  // namespace kkt

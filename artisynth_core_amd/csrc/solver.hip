@@ -1,0 +1,1128 @@
+// Solver handle, plan construction and the C ABI (include/kkt_mi355x.h).
+//
+// Semantics follow the reference's cuDSS bridge (cudssBridge.cc):
+//   set_pattern -> analyze -> factor (first: full, later: numeric only)
+//   -> solve / iterative_solve, with the state machine UNSET -> ANALYZED ->
+//   FACTORED (CuDssSolver.java:57-59), errors as negative status codes and a
+//   per-handle last-error string.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "kkt_mi355x.h"
+#include "symbolic.h"
+
+using namespace kkt;
+
+namespace {
+
+bool g_timing = false;
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+struct Launch {
+  long long off = 0;
+  int cnt = 0;
+};
+struct Step {
+  Launch diag, trsm, upd;
+  double upd_flops = 0;
+};
+
+// timing classes
+enum { TC_ASM = 0, TC_DIAG, TC_TRSM, TC_UPD, TC_SLEVEL, TC_SGEMV, TC_SPMV, TC_VEC, TC_N };
+
+}  // namespace
+
+struct kkt_solver {
+  // ---- state ----
+  enum { UNSET = 0, PATTERN = 1, ANALYZED = 2, FACTORED = 3 };
+  int state = UNSET;
+  int n = 0, nnz = 0, mtype = KKT_MT_SYMMETRIC;
+  std::vector<int> rowOffs, colIdxs;
+  std::vector<int> diagPos;  // position of the diagonal entry of each row, -1 if absent
+  int ordering = 1;
+  int prec = KKT_PREC_FP64;   // requested precision of the next factor
+  int fprec = KKT_PREC_FP64;  // precision of the current factor
+  bool delay_known = false;
+  double refine_tol = 1e-10;
+  int refine_max = 10;
+  int ir_steps = 0;
+  int last_refine_steps = 0;
+  int fp64_fallbacks = 0;
+  double last_relres = 0.0;
+  std::string err;
+  bool has_err = false;
+  Symbolic S;
+  FactorCounters ctr{};
+
+  // ---- device ----
+  bool dev_ok = false;
+  hipStream_t st = nullptr;
+  std::vector<void*> allocs;  // structure/plan allocations (freed on re-analysis)
+  // pattern (full symmetric CSR for SpMV) and values
+  int *d_rowptr_f = nullptr, *d_col_f = nullptr, *d_map_f = nullptr;
+  long long nnzf = 0;
+  double *d_vals = nullptr, *d_valsf = nullptr;
+  // structure
+  DevSym ds{};
+  int* d_perm = nullptr;
+  int *d_lperm = nullptr;
+  signed char* d_piv = nullptr;
+  // plan
+  int4 *d_asm = nullptr, *d_upd = nullptr;
+  int *d_diag = nullptr, *d_lvlsn = nullptr, *d_aidx = nullptr;
+  int2 *d_trsm = nullptr, *d_fg = nullptr, *d_bg = nullptr;
+  long long* d_apos = nullptr;
+  std::vector<Launch> asm_l, scat_l, sn_l, fg_l, bg_l;
+  std::vector<std::vector<Step>> steps;
+  long long front_elems = 0, w_elems = 0, u_elems = 0;
+  // numeric buffers (allocated lazily per precision)
+  double* d_F64 = nullptr;
+  float* d_F32 = nullptr;
+  double* d_W64 = nullptr;
+  float* d_W32 = nullptr;
+  void* d_svec = nullptr;  // solve scratch: bp, y, z, tb, xp (n each) + u (u_elems), as T
+  size_t svec_bytes = 0;
+  // FP64 vectors for refinement / BiCGStab / host copies
+  double* d_vec = nullptr;  // 12 vectors of n
+  double* d_partial = nullptr;
+  double* d_scal = nullptr;  // 16 scalars
+  FactorCounters* d_ctr = nullptr;
+  std::vector<double> multiB;
+
+  // ---- timing ----
+  bool tm_on = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_class;
+  size_t ev_used = 0;
+  double tm_ms[TC_N] = {0};
+  long long tm_cnt[TC_N] = {0};
+  double tm_upd_flops = 0;
+
+  kkt_solver() {}
+  ~kkt_solver() { release_all(); }
+
+  int fail(int code, const std::string& msg) {
+    err = msg;
+    has_err = true;
+    return code;
+  }
+  void ok() { has_err = false; }
+
+  void free_list(std::vector<void*>& v) {
+    for (void* p : v)
+      if (p) (void)hipFree(p);
+    v.clear();
+  }
+  void release_numeric() {
+    if (d_F64) (void)hipFree(d_F64);
+    if (d_F32) (void)hipFree(d_F32);
+    if (d_W64) (void)hipFree(d_W64);
+    if (d_W32) (void)hipFree(d_W32);
+    if (d_svec) (void)hipFree(d_svec);
+    d_F64 = nullptr; d_F32 = nullptr; d_W64 = nullptr; d_W32 = nullptr; d_svec = nullptr;
+    svec_bytes = 0;
+  }
+  void release_all() {
+    if (dev_ok) {
+      if (st) (void)hipStreamSynchronize(st);
+      free_list(allocs);
+      release_numeric();
+      if (d_rowptr_f) (void)hipFree(d_rowptr_f);
+      if (d_col_f) (void)hipFree(d_col_f);
+      if (d_map_f) (void)hipFree(d_map_f);
+      if (d_vals) (void)hipFree(d_vals);
+      if (d_valsf) (void)hipFree(d_valsf);
+      if (d_vec) (void)hipFree(d_vec);
+      if (d_partial) (void)hipFree(d_partial);
+      if (d_scal) (void)hipFree(d_scal);
+      if (d_ctr) (void)hipFree(d_ctr);
+      for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+      ev_pool.clear();
+      if (st) (void)hipStreamDestroy(st);
+    }
+    d_rowptr_f = d_col_f = d_map_f = nullptr;
+    d_vals = d_valsf = d_vec = d_partial = d_scal = nullptr;
+    d_ctr = nullptr;
+    st = nullptr;
+    dev_ok = false;
+  }
+
+  int ensure_device() {
+    if (dev_ok) return KKT_OK;
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt <= 0)
+      return fail(KKT_ERR_DEVICE, "no HIP device available");
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+      return fail(KKT_ERR_INIT, "hipStreamCreate failed");
+    if (hipMalloc(&d_partial, 1024 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&d_scal, 16 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&d_ctr, sizeof(FactorCounters)) != hipSuccess)
+      return fail(KKT_ERR_ALLOC, "hipMalloc failed (scalars)");
+    dev_ok = true;
+    return KKT_OK;
+  }
+
+  template <typename T>
+  int upload(T*& dptr, const std::vector<T>& h) {
+    size_t bytes = std::max<size_t>(h.size(), 1) * sizeof(T);
+    if (hipMalloc(&dptr, bytes) != hipSuccess) return fail(KKT_ERR_ALLOC, "hipMalloc failed (plan)");
+    allocs.push_back(dptr);
+    if (!h.empty() && hipMemcpy(dptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(KKT_ERR_COPY, "hipMemcpy failed (plan)");
+    return KKT_OK;
+  }
+
+  // ---- timing helpers ----
+  void tbeg(int cls) {
+    if (!tm_on) return;
+    if (ev_used + 2 > ev_pool.size()) {
+      for (int q = 0; q < 256; q++) {
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        ev_pool.push_back(e);
+      }
+    }
+    (void)hipEventRecord(ev_pool[ev_used], st);
+    ev_class.push_back(cls);
+    ev_used++;
+  }
+  void tend() {
+    if (!tm_on) return;
+    (void)hipEventRecord(ev_pool[ev_used], st);
+    ev_used++;
+  }
+  void tflush() {  // after a stream sync
+    if (!tm_on) return;
+    for (size_t q = 0; q + 1 < ev_used; q += 2) {
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, ev_pool[q], ev_pool[q + 1]);
+      int c = ev_class[q / 2];
+      tm_ms[c] += ms;
+      tm_cnt[c]++;
+    }
+    ev_used = 0;
+    ev_class.clear();
+  }
+
+  // ------------------------------------------------------------------
+  int set_pattern(int n_, int nnz_, const int* ro, const int* ci, int mt) {
+    if (mt != KKT_MT_SYMMETRIC && mt != KKT_MT_SPD) {
+      if (mt == KKT_MT_GENERAL)
+        return fail(KKT_ERR_UNKNOWN_MTYPE, "general (unsymmetric) matrices are not supported; "
+                                           "use a symmetric type");
+      return fail(KKT_ERR_UNKNOWN_MTYPE, "unknown matrix type");
+    }
+    std::string e;
+    if (validate_pattern(n_, nnz_, ro, ci, e) != 0) return fail(KKT_ERR_PATTERN, e);
+    if (dev_ok) {
+      (void)hipStreamSynchronize(st);
+      free_list(allocs);
+      release_numeric();
+      if (d_rowptr_f) (void)hipFree(d_rowptr_f);
+      if (d_col_f) (void)hipFree(d_col_f);
+      if (d_map_f) (void)hipFree(d_map_f);
+      if (d_vals) (void)hipFree(d_vals);
+      if (d_valsf) (void)hipFree(d_valsf);
+      if (d_vec) (void)hipFree(d_vec);
+      d_rowptr_f = d_col_f = d_map_f = nullptr;
+      d_vals = d_valsf = d_vec = nullptr;
+    }
+    n = n_;
+    nnz = nnz_;
+    mtype = mt;
+    rowOffs.assign(ro, ro + n + 1);
+    colIdxs.assign(ci, ci + nnz);
+    diagPos.assign(n, -1);
+    for (int i = 0; i < n; i++)
+      for (int p = rowOffs[i]; p < rowOffs[i + 1]; p++)
+        if (colIdxs[p] == i) diagPos[i] = p;
+    state = PATTERN;
+    delay_known = false;
+    return KKT_OK;
+  }
+
+  int analyze_with_mask(const unsigned char* delay) {
+    SymOptions opt;
+    opt.ordering = ordering;
+    std::string e;
+    double t0 = now_ms();
+    int rc = symbolic_analyze(n, rowOffs.data(), colIdxs.data(), delay, opt, S, e);
+    if (rc != 0) return fail(KKT_ERR_ANALYSIS, "symbolic analysis failed: " + e);
+    double t1 = now_ms();
+    rc = ensure_device();
+    if (rc) return rc;
+    rc = build_plan();
+    if (rc) return rc;
+    if (g_timing)
+      fprintf(stderr, "[kkt-timing] analyze n=%d nnz=%d: symbolic %.2f ms, plan+upload %.2f ms, "
+              "nsuper=%d levels=%d nnzL=%lld flops=%.3g\n",
+              n, nnz, t1 - t0, now_ms() - t1, S.nsuper, S.nlevels, (long long)S.nnzL, S.flops);
+    state = ANALYZED;
+    return KKT_OK;
+  }
+
+  int analyze(const double* hvals) {
+    if (state == UNSET) return fail(KKT_ERR_STATE, "analyze called before setPattern");
+    if (hvals) {
+      std::vector<unsigned char> mask;
+      make_mask(hvals, mask);
+      delay_known = true;
+      return analyze_with_mask(mask.data());
+    }
+    delay_known = false;
+    return analyze_with_mask(nullptr);
+  }
+
+  // zero (or absent) diagonal -> constraint-like row ordered after its neighbours
+  bool make_mask(const double* hvals, std::vector<unsigned char>& mask) {
+    mask.assign(n, 0);
+    bool any = false;
+    for (int i = 0; i < n; i++) {
+      int p = diagPos[i];
+      if (p < 0 || hvals[p] == 0.0) { mask[i] = 1; any = true; }
+    }
+    return any;
+  }
+
+  // ------------------------------------------------------------------
+  int build_plan() {
+    (void)hipStreamSynchronize(st);
+    free_list(allocs);
+    release_numeric();
+    for (void* q : {(void*)d_rowptr_f, (void*)d_col_f, (void*)d_map_f, (void*)d_vals, (void*)d_valsf,
+                    (void*)d_vec})
+      if (q) (void)hipFree(q);
+    d_rowptr_f = d_col_f = d_map_f = nullptr;
+    d_vals = d_valsf = d_vec = nullptr;
+    const int ns = S.nsuper;
+    std::vector<int> first(ns), kk(ns), mm(ns);
+    std::vector<long long> foff(ns + 1), roff(ns + 1), woff(ns + 1), uoff(ns + 1);
+    foff[0] = roff[0] = woff[0] = uoff[0] = 0;
+    for (int s = 0; s < ns; s++) {
+      first[s] = S.sup_ptr[s];
+      kk[s] = S.sup_ptr[s + 1] - S.sup_ptr[s];
+      long long r = S.rows_ptr[s + 1] - S.rows_ptr[s];
+      mm[s] = kk[s] + (int)r;
+      foff[s + 1] = foff[s] + (long long)mm[s] * mm[s];
+      roff[s + 1] = roff[s] + r;
+      woff[s + 1] = woff[s] + (long long)mm[s] * NB;
+      uoff[s + 1] = uoff[s] + r;
+    }
+    for (int s = 0; s < ns; s++)
+      if (kk[s] > 1024) return fail(KKT_ERR_ANALYSIS, "internal: supernode wider than 1024 columns");
+    front_elems = foff[ns];
+    w_elems = woff[ns];
+    u_elems = uoff[ns];
+    // children (ascending)
+    std::vector<int> chp(ns + 1, 0), chl;
+    for (int s = 0; s < ns; s++)
+      if (S.sparent[s] >= 0) chp[S.sparent[s] + 1]++;
+    for (int s = 0; s < ns; s++) chp[s + 1] += chp[s];
+    chl.assign(std::max(chp[ns], 1), 0);
+    {
+      std::vector<int> pos(chp.begin(), chp.end() - 1);
+      for (int s = 0; s < ns; s++)
+        if (S.sparent[s] >= 0) chl[pos[S.sparent[s]]++] = s;
+    }
+    // relative positions of each child's rows in its parent front
+    std::vector<int> rel(std::max<long long>(roff[ns], 1), 0);
+    for (int c = 0; c < ns; c++) {
+      int p = S.sparent[c];
+      if (p < 0) continue;
+      int fp = first[p], kp = kk[p];
+      const int* prow = S.rows.data() + S.rows_ptr[p];
+      int rp = (int)(S.rows_ptr[p + 1] - S.rows_ptr[p]);
+      int q = 0;
+      for (long long i = S.rows_ptr[c]; i < S.rows_ptr[c + 1]; i++) {
+        int g = S.rows[i];
+        int pos;
+        if (g < fp + kp) pos = g - fp;
+        else {
+          while (q < rp && prow[q] < g) q++;
+          if (q >= rp || prow[q] != g) return fail(KKT_ERR_ANALYSIS, "internal: child row not in parent");
+          pos = kp + q;
+        }
+        rel[i] = pos;
+      }
+    }
+    // levels
+    int nl = S.nlevels;
+    std::vector<std::vector<int>> lv(nl);
+    for (int s = 0; s < ns; s++) lv[S.level[s]].push_back(s);
+    // A entries -> fronts, bucketed by level
+    std::vector<std::vector<int>> a_idx(nl);
+    std::vector<std::vector<long long>> a_pos(nl);
+    for (int i = 0; i < n; i++)
+      for (int p = rowOffs[i]; p < rowOffs[i + 1]; p++) {
+        int j = colIdxs[p];
+        if (j < i) continue;
+        int a = S.iperm[i], b = S.iperm[j];
+        int c = std::min(a, b), r = std::max(a, b);
+        int s = S.col2sup[c];
+        int fs = first[s], k = kk[s], m = mm[s];
+        int rp;
+        if (r < fs + k) rp = r - fs;
+        else {
+          const int* rw = S.rows.data() + S.rows_ptr[s];
+          int cnt = (int)(S.rows_ptr[s + 1] - S.rows_ptr[s]);
+          int q = (int)(std::lower_bound(rw, rw + cnt, r) - rw);
+          if (q >= cnt || rw[q] != r) return fail(KKT_ERR_ANALYSIS, "internal: entry outside structure");
+          rp = k + q;
+        }
+        int L = S.level[s];
+        a_idx[L].push_back(p);
+        a_pos[L].push_back(foff[s] + (long long)(c - fs) * m + rp);
+      }
+    // work items
+    std::vector<int4> asmv, updv;
+    std::vector<int> diagv, lvlsn, aidx;
+    std::vector<long long> apos;
+    std::vector<int2> trsmv, fgv, bgv;
+    asm_l.assign(nl, Launch());
+    scat_l.assign(nl, Launch());
+    sn_l.assign(nl, Launch());
+    fg_l.assign(nl, Launch());
+    bg_l.assign(nl, Launch());
+    steps.assign(nl, std::vector<Step>());
+    for (int L = 0; L < nl; L++) {
+      asm_l[L].off = (long long)asmv.size();
+      for (int s : lv[L]) {
+        int t = (mm[s] + TILE - 1) / TILE;
+        for (int I = 0; I < t; I++)
+          for (int J = 0; J <= I; J++) asmv.push_back(make_int4(s, I, J, 0));
+      }
+      asm_l[L].cnt = (int)(asmv.size() - asm_l[L].off);
+      scat_l[L].off = (long long)aidx.size();
+      aidx.insert(aidx.end(), a_idx[L].begin(), a_idx[L].end());
+      apos.insert(apos.end(), a_pos[L].begin(), a_pos[L].end());
+      scat_l[L].cnt = (int)a_idx[L].size();
+      sn_l[L].off = (long long)lvlsn.size();
+      lvlsn.insert(lvlsn.end(), lv[L].begin(), lv[L].end());
+      sn_l[L].cnt = (int)lv[L].size();
+      fg_l[L].off = (long long)fgv.size();
+      bg_l[L].off = (long long)bgv.size();
+      for (int s : lv[L]) {
+        int r = mm[s] - kk[s];
+        if (r <= 0) continue;
+        for (int rb = 0; rb < (r + 63) / 64; rb++) fgv.push_back(make_int2(s, rb));
+        for (int cb = 0; cb < (kk[s] + 3) / 4; cb++) bgv.push_back(make_int2(s, cb));
+      }
+      fg_l[L].cnt = (int)(fgv.size() - fg_l[L].off);
+      bg_l[L].cnt = (int)(bgv.size() - bg_l[L].off);
+      int P = 0;
+      for (int s : lv[L]) P = std::max(P, (kk[s] + NB - 1) / NB);
+      steps[L].resize(P);
+      for (int p = 0; p < P; p++) {
+        Step& sp = steps[L][p];
+        sp.diag.off = (long long)diagv.size();
+        sp.trsm.off = (long long)trsmv.size();
+        sp.upd.off = (long long)updv.size();
+        for (int s : lv[L]) {
+          int j0 = p * NB;
+          if (kk[s] <= j0) continue;
+          int w = std::min(NB, kk[s] - j0), r0 = j0 + w, mr = mm[s] - r0;
+          diagv.push_back(s);
+          for (int rb = 0; rb < (mr + 63) / 64; rb++) trsmv.push_back(make_int2(s, rb));
+          int t = (mr + TILE - 1) / TILE;
+          for (int I = 0; I < t; I++)
+            for (int J = 0; J <= I; J++) updv.push_back(make_int4(s, I, J, 0));
+          sp.upd_flops += (double)w * (double)mr * (double)(mr + 1);
+        }
+        sp.diag.cnt = (int)(diagv.size() - sp.diag.off);
+        sp.trsm.cnt = (int)(trsmv.size() - sp.trsm.off);
+        sp.upd.cnt = (int)(updv.size() - sp.upd.off);
+      }
+    }
+    // full symmetric CSR for SpMV (row-sorted), map to upper value index
+    std::vector<int> rp_f(n + 1, 0);
+    for (int i = 0; i < n; i++)
+      for (int p = rowOffs[i]; p < rowOffs[i + 1]; p++) {
+        int j = colIdxs[p];
+        if (j < i) continue;
+        rp_f[i + 1]++;
+        if (j != i) rp_f[j + 1]++;
+      }
+    for (int i = 0; i < n; i++) rp_f[i + 1] += rp_f[i];
+    nnzf = rp_f[n];
+    std::vector<int> col_f(std::max<long long>(nnzf, 1)), map_f(std::max<long long>(nnzf, 1));
+    {
+      std::vector<int> pos(rp_f.begin(), rp_f.end() - 1);
+      for (int i = 0; i < n; i++)
+        for (int p = rowOffs[i]; p < rowOffs[i + 1]; p++) {
+          int j = colIdxs[p];
+          if (j < i) continue;
+          col_f[pos[i]] = j; map_f[pos[i]++] = p;
+          if (j != i) { col_f[pos[j]] = i; map_f[pos[j]++] = p; }
+        }
+      std::vector<std::pair<int, int>> tmp;
+      for (int i = 0; i < n; i++) {
+        tmp.clear();
+        for (int q = rp_f[i]; q < rp_f[i + 1]; q++) tmp.push_back({col_f[q], map_f[q]});
+        std::sort(tmp.begin(), tmp.end());
+        for (int q = rp_f[i]; q < rp_f[i + 1]; q++) { col_f[q] = tmp[q - rp_f[i]].first; map_f[q] = tmp[q - rp_f[i]].second; }
+      }
+    }
+    // uploads
+    int rc = 0;
+    int *d_first, *d_k, *d_m, *d_chp, *d_chl, *d_rel, *d_rows;
+    long long *d_foff, *d_roff, *d_woff, *d_uoff;
+    if ((rc = upload(d_first, first)) || (rc = upload(d_k, kk)) || (rc = upload(d_m, mm)) ||
+        (rc = upload(d_foff, foff)) || (rc = upload(d_roff, roff)) || (rc = upload(d_woff, woff)) ||
+        (rc = upload(d_uoff, uoff)) || (rc = upload(d_chp, chp)) || (rc = upload(d_chl, chl)) ||
+        (rc = upload(d_rel, rel)) || (rc = upload(d_rows, S.rows)) || (rc = upload(d_perm, S.perm)) ||
+        (rc = upload(d_asm, asmv)) || (rc = upload(d_upd, updv)) || (rc = upload(d_diag, diagv)) ||
+        (rc = upload(d_lvlsn, lvlsn)) || (rc = upload(d_aidx, aidx)) || (rc = upload(d_apos, apos)) ||
+        (rc = upload(d_trsm, trsmv)) || (rc = upload(d_fg, fgv)) || (rc = upload(d_bg, bgv)))
+      return rc;
+    std::vector<int> lp0(n);
+    for (int s = 0; s < ns; s++)
+      for (int t = 0; t < kk[s]; t++) lp0[first[s] + t] = t;
+    std::vector<signed char> pv0(n, 1);
+    if ((rc = upload(d_lperm, lp0)) || (rc = upload(d_piv, pv0))) return rc;
+    ds.sn_first = d_first; ds.sn_k = d_k; ds.sn_m = d_m;
+    ds.sn_foff = d_foff; ds.sn_roff = d_roff; ds.sn_woff = d_woff; ds.sn_uoff = d_uoff;
+    ds.ch_ptr = d_chp; ds.ch_list = d_chl; ds.rel = d_rel; ds.rows = d_rows;
+    ds.lperm = d_lperm; ds.piv = d_piv;
+    // host copies of what was uploaded, for the structure-parity check
+    h_first = first; h_k = kk; h_m = mm; h_foff = foff; h_roff = roff; h_chp = chp; h_chl = chl;
+    h_rel = rel;
+    // pattern + values
+    if (hipMalloc(&d_rowptr_f, (n + 1) * sizeof(int)) != hipSuccess ||
+        hipMalloc(&d_col_f, col_f.size() * sizeof(int)) != hipSuccess ||
+        hipMalloc(&d_map_f, map_f.size() * sizeof(int)) != hipSuccess ||
+        hipMalloc(&d_vals, std::max(nnz, 1) * sizeof(double)) != hipSuccess ||
+        hipMalloc(&d_valsf, col_f.size() * sizeof(double)) != hipSuccess ||
+        hipMalloc(&d_vec, (size_t)12 * n * sizeof(double)) != hipSuccess)
+      return fail(KKT_ERR_ALLOC, "hipMalloc failed (pattern)");
+    if (hipMemcpy(d_rowptr_f, rp_f.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_col_f, col_f.data(), col_f.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_map_f, map_f.data(), map_f.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(KKT_ERR_COPY, "hipMemcpy failed (pattern)");
+    return KKT_OK;
+  }
+  std::vector<int> h_first, h_k, h_m, h_chp, h_chl, h_rel;
+  std::vector<long long> h_foff, h_roff;
+
+  int ensure_numeric(int p) {
+    size_t esz = (p == KKT_PREC_FP64) ? sizeof(double) : sizeof(float);
+    if (p == KKT_PREC_FP64 && !d_F64) {
+      if (hipMalloc(&d_F64, std::max<long long>(front_elems, 1) * sizeof(double)) != hipSuccess ||
+          hipMalloc(&d_W64, std::max<long long>(w_elems, 1) * sizeof(double)) != hipSuccess)
+        return fail(KKT_ERR_ALLOC, "hipMalloc failed (FP64 fronts)");
+    }
+    if (p == KKT_PREC_FP32_REFINE && !d_F32) {
+      if (hipMalloc(&d_F32, std::max<long long>(front_elems, 1) * sizeof(float)) != hipSuccess ||
+          hipMalloc(&d_W32, std::max<long long>(w_elems, 1) * sizeof(float)) != hipSuccess)
+        return fail(KKT_ERR_ALLOC, "hipMalloc failed (FP32 fronts)");
+    }
+    size_t need = ((size_t)5 * n + (size_t)u_elems + 16) * std::max(esz, sizeof(double));
+    if (svec_bytes < need) {
+      if (d_svec) (void)hipFree(d_svec);
+      if (hipMalloc(&d_svec, need) != hipSuccess) return fail(KKT_ERR_ALLOC, "hipMalloc failed (solve)");
+      svec_bytes = need;
+    }
+    return KKT_OK;
+  }
+
+  // ------------------------------------------------------------------
+  template <typename T>
+  void run_factor(T* F, T* W, double pert) {
+    tbeg(TC_VEC);
+    launch_absmax(nnz, d_vals, d_partial, d_scal + 15, st);
+    tend();
+    (void)hipMemsetAsync(d_ctr, 0, sizeof(FactorCounters), st);
+    const int nl = S.nlevels;
+    for (int L = 0; L < nl; L++) {
+      tbeg(TC_ASM);
+      launch_assemble<T>(d_asm + asm_l[L].off, asm_l[L].cnt, ds, F, st);
+      launch_scatter<T>(d_aidx + scat_l[L].off, d_apos + scat_l[L].off, scat_l[L].cnt, d_vals, F, st);
+      tend();
+      for (size_t p = 0; p < steps[L].size(); p++) {
+        const Step& sp = steps[L][p];
+        tbeg(TC_DIAG);
+        launch_panel_diag<T>(d_diag + sp.diag.off, sp.diag.cnt, (int)p, ds, F, pert, d_scal + 15, d_ctr, st);
+        tend();
+        tbeg(TC_TRSM);
+        launch_panel_trsm<T>(d_trsm + sp.trsm.off, sp.trsm.cnt, (int)p, ds, F, W, st);
+        tend();
+        tbeg(TC_UPD);
+        launch_panel_update<T>(d_upd + sp.upd.off, sp.upd.cnt, (int)p, ds, F, W, st);
+        tend();
+        if (tm_on) tm_upd_flops += sp.upd_flops;
+      }
+    }
+  }
+
+  int factor_device(const double* dv, bool from_host_copy) {
+    if (state < ANALYZED) return fail(KKT_ERR_STATE, "factor called before analyze");
+    if (!from_host_copy && dv != d_vals) {
+      if (hipMemcpyAsync(d_vals, dv, (size_t)nnz * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess)
+        return fail(KKT_ERR_COPY, "hipMemcpyAsync (vals D2D) failed");
+    }
+    if (!delay_known) {
+      // first factor after a pattern-only analysis: look at the diagonal once
+      std::vector<double> hv(nnz);
+      if (hipMemcpyAsync(hv.data(), d_vals, (size_t)nnz * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        return fail(KKT_ERR_COPY, "hipMemcpy (vals D2H) failed");
+      std::vector<unsigned char> mask;
+      delay_known = true;
+      if (make_mask(hv.data(), mask)) {
+        int rc = analyze_with_mask(mask.data());
+        if (rc) return rc;
+        if (hipMemcpy(d_vals, hv.data(), (size_t)nnz * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+          return fail(KKT_ERR_COPY, "hipMemcpy (vals) failed");
+      }
+    }
+    return do_factor(prec);
+  }
+
+  int do_factor(int p) {
+    int rc = ensure_numeric(p);
+    if (rc) return rc;
+    double t0 = g_timing ? now_ms() : 0;
+    launch_expand_vals(nnzf, d_map_f, d_vals, d_valsf, st);
+    if (p == KKT_PREC_FP64) run_factor<double>(d_F64, d_W64, 1e-8);
+    else run_factor<float>(d_F32, d_W32, 1e-6);
+    if (hipMemcpyAsync(&ctr, d_ctr, sizeof(FactorCounters), hipMemcpyDeviceToHost, st) != hipSuccess)
+      return fail(KKT_ERR_COPY, "hipMemcpyAsync (counters) failed");
+    if (hipStreamSynchronize(st) != hipSuccess) return fail(KKT_ERR_FACTORIZATION, "factor kernels failed");
+    tflush();
+    if (hipGetLastError() != hipSuccess) return fail(KKT_ERR_FACTORIZATION, "kernel launch failed");
+    fprec = p;
+    state = FACTORED;
+    if (g_timing)
+      fprintf(stderr, "[kkt-timing] factor(%s) n=%d: %.3f ms (inertia +%d/-%d, perturbed %d)\n",
+              p == KKT_PREC_FP64 ? "fp64" : "fp32", n, now_ms() - t0, ctr.npos, ctr.nneg, ctr.npert);
+    return KKT_OK;
+  }
+
+  // ------------------------------------------------------------------
+  template <typename T>
+  void run_tri_solve(const T* F, const double* d_b, double* d_x) {
+    T* sv = (T*)d_svec;
+    T *bp = sv, *y = sv + n, *z = sv + 2 * (size_t)n, *tb = sv + 3 * (size_t)n,
+      *xp = sv + 4 * (size_t)n, *u = sv + 5 * (size_t)n;
+    tbeg(TC_VEC);
+    launch_gather_perm<double, T>(n, d_perm, d_b, bp, st);
+    tend();
+    const int nl = S.nlevels;
+    for (int L = 0; L < nl; L++) {
+      tbeg(TC_SLEVEL);
+      launch_fwd_level<T>(d_lvlsn + sn_l[L].off, sn_l[L].cnt, ds, F, bp, y, z, u, st);
+      tend();
+      tbeg(TC_SGEMV);
+      launch_fwd_gemv<T>(d_fg + fg_l[L].off, fg_l[L].cnt, ds, F, y, u, st);
+      tend();
+    }
+    for (int L = nl - 1; L >= 0; L--) {
+      tbeg(TC_SGEMV);
+      launch_bwd_gemv<T>(d_bg + bg_l[L].off, bg_l[L].cnt, ds, F, xp, tb, st);
+      tend();
+      tbeg(TC_SLEVEL);
+      launch_bwd_level<T>(d_lvlsn + sn_l[L].off, sn_l[L].cnt, ds, F, z, tb, xp, st);
+      tend();
+    }
+    tbeg(TC_VEC);
+    launch_scatter_perm<T, double>(n, d_perm, xp, d_x, st);
+    tend();
+  }
+  void tri_solve(const double* d_b, double* d_x) {
+    if (fprec == KKT_PREC_FP64) run_tri_solve<double>(d_F64, d_b, d_x);
+    else run_tri_solve<float>(d_F32, d_b, d_x);
+  }
+
+  double* vec(int i) { return d_vec + (size_t)i * n; }
+
+  int dotv(const double* a, const double* b, double& out) {
+    tbeg(TC_VEC);
+    launch_dot(n, a, b, d_partial, d_scal, st);
+    tend();
+    if (hipMemcpyAsync(&out, d_scal, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return fail(KKT_ERR_REDUCTION, "reduction failed");
+    return KKT_OK;
+  }
+  void spmv(const double* x, const double* b, double* y, int mode) {
+    tbeg(TC_SPMV);
+    launch_spmv(n, d_rowptr_f, d_col_f, d_valsf, x, b, y, mode, st);
+    tend();
+  }
+
+  // refined solve on device vectors; d_b and d_x must not alias
+  int solve_device(const double* d_b, double* d_x) {
+    if (state != FACTORED) return fail(KKT_ERR_STATE, "solve called before factor");
+    double t0 = g_timing ? now_ms() : 0;
+    tri_solve(d_b, d_x);
+    last_refine_steps = 0;
+    int want = (fprec == KKT_PREC_FP32_REFINE) ? refine_max : ir_steps;
+    if (want > 0) {
+      double* r = vec(0);
+      double* dx = vec(1);
+      double bn2;
+      int rc = dotv(d_b, d_b, bn2);
+      if (rc) return rc;
+      double bn = std::sqrt(bn2);
+      if (bn == 0.0) bn = 1.0;
+      double prev = 0.0;
+      int stall = 0;
+      for (int step = 0;; step++) {
+        spmv(d_x, d_b, r, 1);
+        double rn2;
+        if ((rc = dotv(r, r, rn2))) return rc;
+        double rel = std::sqrt(rn2) / bn;
+        last_relres = rel;
+        if (fprec == KKT_PREC_FP32_REFINE) {
+          if (rel <= refine_tol) break;
+          if (step > 0 && rel > 0.5 * prev) stall++; else stall = 0;
+          if (step >= refine_max || stall >= 2 || !std::isfinite(rel)) {
+            // refinement stalled: FP64 refactor fallback with the same values
+            fp64_fallbacks++;
+            if ((rc = do_factor(KKT_PREC_FP64))) return rc;
+            tri_solve(d_b, d_x);
+            for (int s2 = 0; s2 < 4; s2++) {
+              spmv(d_x, d_b, r, 1);
+              if ((rc = dotv(r, r, rn2))) return rc;
+              last_relres = std::sqrt(rn2) / bn;
+              if (last_relres <= refine_tol) break;
+              tri_solve(r, dx);
+              launch_axpby(n, 1.0, dx, 1.0, d_x, st);
+            }
+            break;
+          }
+        } else if (step >= ir_steps) {
+          break;
+        }
+        prev = rel;
+        tri_solve(r, dx);
+        tbeg(TC_VEC);
+        launch_axpby(n, 1.0, dx, 1.0, d_x, st);
+        tend();
+        last_refine_steps++;
+      }
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) return fail(KKT_ERR_SOLVE, "solve kernels failed");
+    tflush();
+    if (g_timing)
+      fprintf(stderr, "[kkt-timing] solve n=%d: %.3f ms (refine steps %d, rel resid %.3g)\n", n,
+              now_ms() - t0, last_refine_steps, last_relres);
+    return KKT_OK;
+  }
+
+  int set_values_device(const double* dv) {
+    if (dv != d_vals &&
+        hipMemcpyAsync(d_vals, dv, (size_t)nnz * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess)
+      return fail(KKT_ERR_COPY, "hipMemcpyAsync (vals) failed");
+    launch_expand_vals(nnzf, d_map_f, d_vals, d_valsf, st);
+    return KKT_OK;
+  }
+
+  // BiCGStab preconditioned by the current (stale) factor: cudssBridge.cc:1705-2006
+  int bicgstab(const double* d_b, double* d_xout, double tolRel, int maxIter, int* iters) {
+    if (state != FACTORED) return fail(KKT_ERR_STATE, "iterativeSolve called before factor");
+    double t0 = g_timing ? now_ms() : 0;
+    double *r = vec(0), *rh = vec(1), *p = vec(2), *v = vec(3), *s = vec(4), *t = vec(5),
+           *y = vec(6), *z = vec(7), *x = vec(8);
+    tri_solve(d_b, x);
+    spmv(x, d_b, r, 1);
+    double bn2, rn2;
+    int rc;
+    if ((rc = dotv(d_b, d_b, bn2)) || (rc = dotv(r, r, rn2))) return rc;
+    double bn = std::sqrt(bn2), rn = std::sqrt(rn2);
+    if (bn == 0.0) bn = 1.0;
+    int it = 1;
+    if (rn / bn <= tolRel) {
+      it = 1;
+    } else {
+      (void)hipMemcpyAsync(rh, r, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st);
+      (void)hipMemsetAsync(p, 0, (size_t)n * sizeof(double), st);
+      (void)hipMemsetAsync(v, 0, (size_t)n * sizeof(double), st);
+      double rho_prev = 1.0, alpha = 1.0, omega = 1.0;
+      const double eps = 1e-30;
+      for (it = 1; it <= maxIter; it++) {
+        double rho;
+        if ((rc = dotv(rh, r, rho))) return rc;
+        if (std::fabs(rho) < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: rho ~ 0");
+        double beta = (rho / rho_prev) * (alpha / omega);
+        // p = r + beta * (p - omega v)
+        tbeg(TC_VEC);
+        launch_axpby(n, -omega, v, 1.0, p, st);
+        launch_axpby(n, 1.0, r, beta, p, st);
+        tend();
+        tri_solve(p, y);
+        spmv(y, nullptr, v, 0);
+        double rhv;
+        if ((rc = dotv(rh, v, rhv))) return rc;
+        if (std::fabs(rhv) < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: (r_hat,v) ~ 0");
+        alpha = rho / rhv;
+        tbeg(TC_VEC);
+        (void)hipMemcpyAsync(s, r, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st);
+        launch_axpby(n, -alpha, v, 1.0, s, st);
+        tend();
+        double sn2;
+        if ((rc = dotv(s, s, sn2))) return rc;
+        if (std::sqrt(sn2) / bn <= tolRel) {
+          launch_axpby(n, alpha, y, 1.0, x, st);
+          break;
+        }
+        tri_solve(s, z);
+        spmv(z, nullptr, t, 0);
+        double ts, tt;
+        if ((rc = dotv(t, s, ts)) || (rc = dotv(t, t, tt))) return rc;
+        if (tt < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: (t,t) ~ 0");
+        omega = ts / tt;
+        tbeg(TC_VEC);
+        launch_axpbypcz(n, alpha, y, omega, z, 1.0, x, st);
+        (void)hipMemcpyAsync(r, s, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st);
+        launch_axpby(n, -omega, t, 1.0, r, st);
+        tend();
+        if ((rc = dotv(r, r, rn2))) return rc;
+        if (std::sqrt(rn2) / bn <= tolRel) break;
+        if (std::fabs(omega) < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: omega ~ 0");
+        rho_prev = rho;
+      }
+      if (it > maxIter) return fail(KKT_ERR_ITER_NO_CONVERGE, "BiCGStab did not converge within iteration limit");
+    }
+    if (hipMemcpyAsync(d_xout, x, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return fail(KKT_ERR_SOLVE, "iterative solve failed");
+    tflush();
+    *iters = it;
+    if (g_timing)
+      fprintf(stderr, "[kkt-timing] bicgstab n=%d iters=%d: total=%.2fms (%.2fms/iter)\n", n, it,
+              now_ms() - t0, (now_ms() - t0) / std::max(1, it));
+    return KKT_OK;
+  }
+};
+
+// =====================================================================
+// C ABI
+// =====================================================================
+template <typename T>
+static int cmp_dev(const T* d, const std::vector<T>& hv) {
+  std::vector<T> tmp(hv.size());
+  if (!hv.empty() && hipMemcpy(tmp.data(), d, hv.size() * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) return 1;
+  return memcmp(tmp.data(), hv.data(), hv.size() * sizeof(T)) != 0;
+}
+
+#define CHECK_H(h) \
+  if (!(h)) return KKT_ERR_ARGUMENT
+
+extern "C" {
+
+kkt_solver* kkt_create(void) {
+  try {
+    return new kkt_solver();
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+void kkt_destroy(kkt_solver* h) { delete h; }
+
+int kkt_set_pattern(kkt_solver* h, int n, int nnz, const int* rowOffs, const int* colIdxs, int mtype) {
+  CHECK_H(h);
+  int rc = h->set_pattern(n, nnz, rowOffs, colIdxs, mtype);
+  if (!rc) h->ok();
+  return rc;
+}
+
+int kkt_analyze(kkt_solver* h) {
+  CHECK_H(h);
+  int rc = h->analyze(nullptr);
+  if (!rc) h->ok();
+  return rc;
+}
+
+int kkt_analyze_values(kkt_solver* h, const double* vals) {
+  CHECK_H(h);
+  if (!vals) return h->fail(KKT_ERR_ARGUMENT, "null values");
+  int rc = h->analyze(vals);
+  if (!rc) h->ok();
+  return rc;
+}
+
+int kkt_factor(kkt_solver* h, const double* vals) {
+  CHECK_H(h);
+  if (!vals) return h->fail(KKT_ERR_ARGUMENT, "null values");
+  if (h->state < kkt_solver::ANALYZED) return h->fail(KKT_ERR_STATE, "factor called before analyze");
+  if (hipMemcpyAsync(h->d_vals, vals, (size_t)h->nnz * sizeof(double), hipMemcpyHostToDevice, h->st) != hipSuccess)
+    return h->fail(KKT_ERR_COPY, "hipMemcpyAsync (vals H2D) failed");
+  int rc = h->factor_device(h->d_vals, true);
+  if (!rc) h->ok();
+  return rc;
+}
+
+int kkt_factor_device(kkt_solver* h, const double* d_vals) {
+  CHECK_H(h);
+  if (!d_vals) return h->fail(KKT_ERR_ARGUMENT, "null values");
+  int rc = h->factor_device(d_vals, false);
+  if (!rc) h->ok();
+  return rc;
+}
+
+int kkt_solve(kkt_solver* h, const double* b, double* x) {
+  CHECK_H(h);
+  if (!b || !x) return h->fail(KKT_ERR_ARGUMENT, "null vector");
+  if (h->state != kkt_solver::FACTORED) return h->fail(KKT_ERR_STATE, "solve called before factor");
+  double* db = h->vec(10);
+  double* dx = h->vec(11);
+  if (hipMemcpyAsync(db, b, (size_t)h->n * sizeof(double), hipMemcpyHostToDevice, h->st) != hipSuccess)
+    return h->fail(KKT_ERR_COPY, "hipMemcpyAsync (b) failed");
+  int rc = h->solve_device(db, dx);
+  if (rc) return rc;
+  if (hipMemcpyAsync(x, dx, (size_t)h->n * sizeof(double), hipMemcpyDeviceToHost, h->st) != hipSuccess ||
+      hipStreamSynchronize(h->st) != hipSuccess)
+    return h->fail(KKT_ERR_COPY, "hipMemcpyAsync (x) failed");
+  h->ok();
+  return KKT_OK;
+}
+
+int kkt_solve_device(kkt_solver* h, const double* d_b, double* d_x) {
+  CHECK_H(h);
+  if (!d_b || !d_x || d_b == d_x) return h->fail(KKT_ERR_ARGUMENT, "null or aliased vectors");
+  int rc = h->solve_device(d_b, d_x);
+  if (!rc) h->ok();
+  return rc;
+}
+
+int kkt_solve_multi(kkt_solver* h, int nrhs, const double* B, double* X) {
+  CHECK_H(h);
+  if (nrhs <= 0 || !B || !X) return h->fail(KKT_ERR_ARGUMENT, "nrhs must be positive");
+  for (int c = 0; c < nrhs; c++) {
+    int rc = kkt_solve(h, B + (size_t)c * h->n, X + (size_t)c * h->n);
+    if (rc) return rc;
+  }
+  return KKT_OK;
+}
+
+int kkt_multiply(kkt_solver* h, const double* x, double* y) {
+  CHECK_H(h);
+  if (!x || !y) return h->fail(KKT_ERR_ARGUMENT, "null vector");
+  if (h->state < kkt_solver::FACTORED) return h->fail(KKT_ERR_STATE, "multiply called before factor");
+  double* dx = h->vec(10);
+  double* dy = h->vec(11);
+  if (hipMemcpyAsync(dx, x, (size_t)h->n * sizeof(double), hipMemcpyHostToDevice, h->st) != hipSuccess)
+    return h->fail(KKT_ERR_COPY, "hipMemcpyAsync (x) failed");
+  h->spmv(dx, nullptr, dy, 0);
+  if (hipMemcpyAsync(y, dy, (size_t)h->n * sizeof(double), hipMemcpyDeviceToHost, h->st) != hipSuccess ||
+      hipStreamSynchronize(h->st) != hipSuccess)
+    return h->fail(KKT_ERR_SPMV, "multiply failed");
+  h->tflush();
+  h->ok();
+  return KKT_OK;
+}
+
+int kkt_iterative_solve_device(kkt_solver* h, const double* d_vals, const double* d_b, double* d_x,
+                               double tolRel, int maxIter) {
+  CHECK_H(h);
+  if (!d_vals || !d_b || !d_x) return h->fail(KKT_ERR_ARGUMENT, "null argument");
+  if (h->state != kkt_solver::FACTORED) return h->fail(KKT_ERR_STATE, "iterativeSolve called before factor");
+  if (maxIter <= 0) return h->fail(KKT_ERR_ARGUMENT, "maxIter must be positive");
+  int rc = h->set_values_device(d_vals);
+  if (rc) return rc;
+  int it = 0;
+  rc = h->bicgstab(d_b, d_x, tolRel, maxIter, &it);
+  if (rc) return rc;
+  h->ok();
+  return it;
+}
+
+int kkt_iterative_solve(kkt_solver* h, const double* vals, const double* b, double* x, double tolRel,
+                        int maxIter) {
+  CHECK_H(h);
+  if (!vals || !b || !x) return h->fail(KKT_ERR_ARGUMENT, "null argument");
+  if (h->state != kkt_solver::FACTORED) return h->fail(KKT_ERR_STATE, "iterativeSolve called before factor");
+  if (hipMemcpyAsync(h->d_vals, vals, (size_t)h->nnz * sizeof(double), hipMemcpyHostToDevice, h->st) != hipSuccess)
+    return h->fail(KKT_ERR_COPY, "hipMemcpyAsync (vals) failed");
+  double* db = h->vec(10);
+  double* dx = h->vec(11);
+  if (hipMemcpyAsync(db, b, (size_t)h->n * sizeof(double), hipMemcpyHostToDevice, h->st) != hipSuccess)
+    return h->fail(KKT_ERR_COPY, "hipMemcpyAsync (b) failed");
+  int it = kkt_iterative_solve_device(h, h->d_vals, db, dx, tolRel, maxIter);
+  if (it <= 0) return it;
+  if (hipMemcpyAsync(x, dx, (size_t)h->n * sizeof(double), hipMemcpyDeviceToHost, h->st) != hipSuccess ||
+      hipStreamSynchronize(h->st) != hipSuccess)
+    return h->fail(KKT_ERR_COPY, "hipMemcpyAsync (x) failed");
+  return it;
+}
+
+int kkt_set_ir_steps(kkt_solver* h, int nsteps) {
+  CHECK_H(h);
+  if (nsteps < 0) return h->fail(KKT_ERR_ARGUMENT, "negative refinement steps");
+  h->ir_steps = nsteps;
+  h->ok();
+  return KKT_OK;
+}
+
+const char* kkt_last_error(kkt_solver* h) {
+  if (!h || !h->has_err) return nullptr;
+  return h->err.c_str();
+}
+
+const char* kkt_version(void) { return "kkt_mi355x 0.1.0 (gfx950)"; }
+
+void kkt_set_timing_enabled(int on) { g_timing = on != 0; }
+
+int kkt_set_precision(kkt_solver* h, int prec) {
+  CHECK_H(h);
+  if (prec != KKT_PREC_FP64 && prec != KKT_PREC_FP32_REFINE)
+    return h->fail(KKT_ERR_ARGUMENT, "unknown precision");
+  h->prec = prec;
+  h->ok();
+  return KKT_OK;
+}
+
+int kkt_set_refine_tol(kkt_solver* h, double tol, int max_steps) {
+  CHECK_H(h);
+  if (!(tol > 0) || max_steps < 1) return h->fail(KKT_ERR_ARGUMENT, "bad refinement settings");
+  h->refine_tol = tol;
+  h->refine_max = max_steps;
+  h->ok();
+  return KKT_OK;
+}
+
+int kkt_set_ordering(kkt_solver* h, int ordering) {
+  CHECK_H(h);
+  if (ordering != 0 && ordering != 1) return h->fail(KKT_ERR_ARGUMENT, "unknown ordering");
+  h->ordering = ordering;
+  h->ok();
+  return KKT_OK;
+}
+
+static void fill_info(const Symbolic& S, kkt_info* o) {
+  o->n = S.n;
+  o->nsuper = S.nsuper;
+  o->nlevels = S.nlevels;
+  o->ndelayed = S.ndelayed;
+  o->nnzL = S.nnzL;
+  o->nnzL_exact = S.nnzL_exact;
+  o->factor_flops = S.flops;
+}
+
+int kkt_get_info(kkt_solver* h, kkt_info* out) {
+  CHECK_H(h);
+  if (!out) return KKT_ERR_ARGUMENT;
+  memset(out, 0, sizeof(*out));
+  if (h->state < kkt_solver::ANALYZED) return h->fail(KKT_ERR_STATE, "no analysis");
+  fill_info(h->S, out);
+  out->nnz = h->nnz;
+  out->front_bytes = h->front_elems * (long long)(h->fprec == KKT_PREC_FP64 ? 8 : 4);
+  out->npos = h->ctr.npos;
+  out->nneg = h->ctr.nneg;
+  out->nperturbed = h->ctr.npert;
+  out->n2x2 = h->ctr.n2x2;
+  out->precision = h->fprec;
+  out->refine_steps = h->last_refine_steps;
+  out->fp64_fallbacks = h->fp64_fallbacks;
+  out->last_rel_residual = h->last_relres;
+  return KKT_OK;
+}
+
+int kkt_get_symbolic(kkt_solver* h, int* perm, int* sup_ptr, int* sparent, long long* rows_ptr, int* rows) {
+  CHECK_H(h);
+  if (h->state < kkt_solver::ANALYZED) return h->fail(KKT_ERR_STATE, "no analysis");
+  const Symbolic& S = h->S;
+  if (perm) memcpy(perm, S.perm.data(), sizeof(int) * S.n);
+  if (sup_ptr) memcpy(sup_ptr, S.sup_ptr.data(), sizeof(int) * (S.nsuper + 1));
+  if (sparent) memcpy(sparent, S.sparent.data(), sizeof(int) * S.nsuper);
+  if (rows_ptr)
+    for (int s = 0; s <= S.nsuper; s++) rows_ptr[s] = S.rows_ptr[s];
+  if (rows && !S.rows.empty()) memcpy(rows, S.rows.data(), sizeof(int) * S.rows.size());
+  return KKT_OK;
+}
+
+int kkt_verify_device_structure(kkt_solver* h) {
+  CHECK_H(h);
+  if (h->state < kkt_solver::ANALYZED) return h->fail(KKT_ERR_STATE, "no analysis");
+  (void)hipStreamSynchronize(h->st);
+  int bad = 0;
+  bad |= cmp_dev(h->d_perm, h->S.perm);
+  bad |= cmp_dev(h->ds.sn_first, h->h_first);
+  bad |= cmp_dev(h->ds.sn_k, h->h_k);
+  bad |= cmp_dev(h->ds.sn_m, h->h_m);
+  bad |= cmp_dev(h->ds.sn_foff, h->h_foff);
+  bad |= cmp_dev(h->ds.sn_roff, h->h_roff);
+  bad |= cmp_dev(h->ds.ch_ptr, h->h_chp);
+  bad |= cmp_dev(h->ds.rows, h->S.rows);
+  bad |= cmp_dev(h->ds.rel, h->h_rel);
+  return bad ? 1 : 0;
+}
+
+void kkt_timing_reset(kkt_solver* h, int enable) {
+  if (!h) return;
+  h->tm_on = enable != 0;
+  for (int c = 0; c < TC_N; c++) { h->tm_ms[c] = 0; h->tm_cnt[c] = 0; }
+  h->tm_upd_flops = 0;
+  h->ev_used = 0;
+  h->ev_class.clear();
+}
+
+int kkt_timing_get(kkt_solver* h, double* ms, long long* launches, double* update_flops) {
+  CHECK_H(h);
+  for (int c = 0; c < TC_N; c++) {
+    if (ms) ms[c] = h->tm_ms[c];
+    if (launches) launches[c] = h->tm_cnt[c];
+  }
+  if (update_flops) *update_flops = h->tm_upd_flops;
+  return KKT_OK;
+}
+
+// ---- host-only symbolic analysis ----
+struct kkt_symbolic {
+  Symbolic S;
+};
+
+kkt_symbolic* kkt_symbolic_create(int n, int nnz, const int* rowOffs, const int* colIdxs,
+                                  const unsigned char* delay, int ordering) {
+  std::string e;
+  if (validate_pattern(n, nnz, rowOffs, colIdxs, e) != 0) return nullptr;
+  kkt_symbolic* s = new kkt_symbolic();
+  SymOptions opt;
+  opt.ordering = ordering;
+  if (symbolic_analyze(n, rowOffs, colIdxs, delay, opt, s->S, e) != 0) {
+    delete s;
+    return nullptr;
+  }
+  return s;
+}
+
+int kkt_symbolic_info(kkt_symbolic* s, kkt_info* out) {
+  if (!s || !out) return KKT_ERR_ARGUMENT;
+  memset(out, 0, sizeof(*out));
+  fill_info(s->S, out);
+  return KKT_OK;
+}
+
+int kkt_symbolic_get(kkt_symbolic* s, int* perm, int* parent, int* colcount, int* sup_ptr, int* sparent,
+                     long long* rows_ptr, int* rows, int* level) {
+  if (!s) return KKT_ERR_ARGUMENT;
+  const Symbolic& S = s->S;
+  if (perm) memcpy(perm, S.perm.data(), sizeof(int) * S.n);
+  if (parent) memcpy(parent, S.parent.data(), sizeof(int) * S.n);
+  if (colcount) memcpy(colcount, S.colcount.data(), sizeof(int) * S.n);
+  if (sup_ptr) memcpy(sup_ptr, S.sup_ptr.data(), sizeof(int) * (S.nsuper + 1));
+  if (sparent) memcpy(sparent, S.sparent.data(), sizeof(int) * S.nsuper);
+  if (rows_ptr)
+    for (int q = 0; q <= S.nsuper; q++) rows_ptr[q] = S.rows_ptr[q];
+  if (rows && !S.rows.empty()) memcpy(rows, S.rows.data(), sizeof(int) * S.rows.size());
+  if (level) memcpy(level, S.level.data(), sizeof(int) * S.nsuper);
+  return KKT_OK;
+}
+
+void kkt_symbolic_destroy(kkt_symbolic* s) { delete s; }
+
+}  // extern "C"

@@ -1,0 +1,258 @@
+"""Synthetic KKT systems of the benchmark configurations (host numpy/scipy).
+
+These stand in for ArtiSynth's model assembly (the caller of the solver):
+trilinear hex elements with 2x2x2 Gauss integration, lumped mass, Rayleigh
+stiffness damping, point-attachment constraints G^T = I3 blocks, and the
+ConstrainedBackwardEuler system S = M + h D + h^2 K (MechSystemSolver.java:
+2686-2692, 2900-3019) assembled into the KKT upper CSR of KKTSolver
+(KKTSolver.java:171-454), right-hand side b = M v + h f (:4851-4862).
+
+Materials restated from ArtiSynth:
+  LinearMaterial(E, nu)                         (LinearMaterial.java:65-69)
+  NeoHookeanMaterial: W = mu(I1-3)/2 - mu lnJ + lam lnJ^2/2
+                                                (NeoHookeanMaterial.java:113-156)
+"""
+import numpy as np
+import scipy.sparse as sp
+
+from .solvers import kkt_upper_csr
+
+GRAVITY = np.array([0.0, 0.0, -9.8])  # MechModel DEFAULT_GRAVITY
+
+
+def hex_grid(nx, ny, nz, lx, ly, lz):
+    """Nodes (N,3) and hex connectivity (E,8) of a box grid centred at the origin
+    (FemFactory.createHexGrid layout: x fastest)."""
+    xs = np.linspace(-lx / 2, lx / 2, nx + 1)
+    ys = np.linspace(-ly / 2, ly / 2, ny + 1)
+    zs = np.linspace(-lz / 2, lz / 2, nz + 1)
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    nodes = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+
+    def nid(i, j, k):
+        return (k * (ny + 1) + j) * (nx + 1) + i
+
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    elems = np.stack([nid(i, j, k), nid(i + 1, j, k), nid(i + 1, j + 1, k), nid(i, j + 1, k),
+                      nid(i, j, k + 1), nid(i + 1, j, k + 1), nid(i + 1, j + 1, k + 1),
+                      nid(i, j + 1, k + 1)], 1)
+    return nodes, elems
+
+
+_HEX_XI = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                    [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], float)
+
+
+def hex_gauss_grads(hx, hy, hz):
+    """dN_a/dX at the 8 Gauss points of an axis-aligned hx*hy*hz hex: (8,8,3),
+    and the quadrature weight * det J per point."""
+    g = 1.0 / np.sqrt(3.0)
+    gps = _HEX_XI * g
+    dN = np.zeros((8, 8, 3))
+    for q, (a, b, c) in enumerate(gps):
+        for n, (xa, ya, za) in enumerate(_HEX_XI):
+            dN[q, n, 0] = xa * (1 + ya * b) * (1 + za * c) / 8 * (2 / hx)
+            dN[q, n, 1] = ya * (1 + xa * a) * (1 + za * c) / 8 * (2 / hy)
+            dN[q, n, 2] = za * (1 + xa * a) * (1 + ya * b) / 8 * (2 / hz)
+    return dN, hx * hy * hz / 8.0
+
+
+def lame(E, nu):
+    return E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+
+
+def neo_hookean_PA(F, lam, mu):
+    """First Piola stress P and tangent A = dP/dF (A[...,i,J,k,L]) of
+    W = mu(I1-3)/2 - mu lnJ + lam lnJ^2/2, for F of shape (...,3,3)."""
+    J = np.linalg.det(F)
+    Fi = np.linalg.inv(F)  # Fi[J,i] = F^-1
+    FiT = np.swapaxes(Fi, -1, -2)
+    lnJ = np.log(J)
+    P = mu * (F - FiT) + (lam * lnJ)[..., None, None] * FiT
+    I3 = np.eye(3)
+    A = mu * np.einsum("ik,JL->iJkL", I3, I3)[None] * np.ones(F.shape[:-2] + (1, 1, 1, 1))
+    c1 = (mu - lam * lnJ)[..., None, None, None, None]
+    A = A + c1 * np.einsum("...Li,...Jk->...iJkL", Fi, Fi)
+    A = A + lam * np.einsum("...Ji,...Lk->...iJkL", Fi, Fi)
+    return P, A
+
+
+class HexBeam:
+    """Cantilever hex beam with its left face attached by point constraints."""
+
+    def __init__(self, nx, ny, nz, length=1.0, width=0.2, height=None, density=1000.0,
+                 material=("linear", 1e5, 0.33), stiffness_damping=0.002, h=0.01):
+        height = width if height is None else height
+        self.nodes, self.elems = hex_grid(nx, ny, nz, length, width, height)
+        self.nn = len(self.nodes)
+        self.sizeM = 3 * self.nn
+        self.h = h
+        self.beta = stiffness_damping
+        self.material = material
+        hx, hy, hz = length / nx, width / ny, height / nz
+        self.dN, self.wdet = hex_gauss_grads(hx, hy, hz)
+        # lumped mass
+        m_node = np.zeros(self.nn)
+        np.add.at(m_node, self.elems.ravel(), density * hx * hy * hz / 8.0)
+        self.mass = np.repeat(m_node, 3)
+        # attachments: nodes on the x = -length/2 face
+        xmin = self.nodes[:, 0].min()
+        self.attached = np.nonzero(np.isclose(self.nodes[:, 0], xmin))[0]
+        self.tip = np.nonzero(np.isclose(self.nodes[:, 0], self.nodes[:, 0].max()))[0]
+        nG = 3 * len(self.attached)
+        rows = (3 * self.attached[:, None] + np.arange(3)[None]).ravel()
+        self.GT = sp.csr_matrix((np.ones(nG), (rows, np.arange(nG))), shape=(self.sizeM, nG))
+        self.numG = nG
+        self._build_pattern()
+
+    def _build_pattern(self):
+        E = len(self.elems)
+        dofs = (3 * self.elems[:, :, None] + np.arange(3)[None, None]).reshape(E, 24)
+        r = np.repeat(dofs, 24, axis=1).ravel()
+        c = np.tile(dofs, (1, 24)).ravel()
+        n = self.sizeM
+        keep = r <= c
+        self._ekeep = keep
+        key = r[keep].astype(np.int64) * n + c[keep]
+        ukeys, inv = np.unique(key, return_inverse=True)
+        self._einv = inv
+        ur, uc = ukeys // n, ukeys % n
+        # make sure the diagonal is present
+        S = sp.csr_matrix((np.ones(len(ur)), (ur, uc)), shape=(n, n))
+        self.Spat = S
+        self._ukeys = ukeys
+        ro, ci, _ = kkt_upper_csr(S, self.GT, None)
+        self.rowOffs, self.colIdxs = ro, ci
+        # map from S-upper entry (ukeys order) to KKT value position
+        Sr = np.repeat(np.arange(n), np.diff(S.indptr))
+        skey = Sr.astype(np.int64) * n + S.indices
+        assert np.array_equal(skey, ukeys)
+        pos = np.empty(len(ukeys), np.int64)
+        kr = np.repeat(np.arange(len(ro) - 1), np.diff(ro))
+        kkey = kr.astype(np.int64) * (n + self.numG) + ci
+        tgt = Sr.astype(np.int64) * (n + self.numG) + S.indices
+        pos = np.searchsorted(kkey, tgt)
+        assert np.array_equal(kkey[pos], tgt)
+        self._kpos = pos
+        # positions of G^T entries and of the lambda diagonal
+        isG = (kr < n) & (ci >= n)
+        self._gpos = np.nonzero(isG)[0]
+        self._lampos = np.nonzero(kr >= n)[0]
+        self.nnz = len(ci)
+        self.n = n + self.numG
+        self._diagpos = None
+
+    def element_tangent(self, u):
+        """Per-element 24x24 tangent stiffness and 24 internal forces at displacement u."""
+        E = len(self.elems)
+        mat = self.material
+        if mat[0] == "linear":
+            lam, mu = lame(mat[1], mat[2])
+            F = np.broadcast_to(np.eye(3), (1, 8, 3, 3))
+            I3 = np.eye(3)
+            A = (mu * (np.einsum("ik,JL->iJkL", I3, I3) + np.einsum("iL,Jk->iJkL", I3, I3)) +
+                 lam * np.einsum("iJ,kL->iJkL", I3, I3))
+            Ke = np.einsum("gaJ,iJkL,gbL->aibk", self.dN, A, self.dN) * self.wdet
+            Ke = np.broadcast_to(Ke.reshape(1, 24, 24), (E, 24, 24))
+            ue = u[self.elems]  # (E,8,3)
+            fint = np.einsum("eaibk,ebk->eai", Ke.reshape(E, 8, 3, 8, 3), ue).reshape(E, 24)
+            return Ke, fint
+        if mat[0] == "neohookean":
+            mu, kappa = mat[1], mat[2]
+            lam = kappa - 2.0 * mu / 3.0
+            ue = u[self.elems]  # (E,8,3)
+            F = np.eye(3)[None, None] + np.einsum("eai,gaJ->egiJ", ue, self.dN)
+            P, A = neo_hookean_PA(F, lam, mu)
+            T1 = np.einsum("gaJ,egiJkL->egaikL", self.dN, A)
+            Ke = np.einsum("egaikL,gbL->eaibk", T1, self.dN).reshape(E, 24, 24) * self.wdet
+            fint = np.einsum("egiJ,gaJ->eai", P, self.dN).reshape(E, 24) * self.wdet
+            return Ke, fint
+        raise ValueError("unknown material %r" % (mat,))
+
+    def system(self, u, v, fext):
+        """KKT values (upper CSR) and right-hand side for one ConstrainedBackwardEuler
+        step at displacement u, velocity v (both (nn,3)), external nodal forces fext."""
+        E = len(self.elems)
+        Ke, fint_e = self.element_tangent(u)
+        n = self.sizeM
+        # K upper entries
+        kv = np.bincount(self._einv, weights=np.ascontiguousarray(Ke).reshape(-1)[self._ekeep.ravel()],
+                         minlength=len(self._ukeys))
+        fint = np.zeros(n)
+        dofs = (3 * self.elems[:, :, None] + np.arange(3)[None, None]).reshape(E, 24)
+        np.add.at(fint, dofs.ravel(), fint_e.ravel())
+        h, beta = self.h, self.beta
+        a = h * beta + h * h
+        sv = a * kv
+        diag = (self._ukeys // n) == (self._ukeys % n)
+        sv[diag] += self.mass
+        vals = np.zeros(self.nnz)
+        vals[self._kpos] = sv
+        vals[self._gpos] = 1.0
+        vals[self._lampos] = 0.0
+        # f = fext - fint - D v, D = beta K ; b = M v + h f
+        vv = v.reshape(-1)
+        Kv = self._kmul(kv, vv)
+        f = fext.reshape(-1) - fint - beta * Kv
+        bm = self.mass * vv + h * f
+        bg = np.zeros(self.numG)
+        return vals, np.concatenate([bm, bg])
+
+    def _kmul(self, kv, x):
+        n = self.sizeM
+        r = self._ukeys // n
+        c = self._ukeys % n
+        y = np.bincount(r, weights=kv * x[c], minlength=n)
+        off = r != c
+        y += np.bincount(c[off], weights=kv[off] * x[r[off]], minlength=n)
+        return y
+
+    def gravity_loads(self):
+        m_node = self.mass[0::3]
+        return m_node[:, None] * GRAVITY[None]
+
+
+def small_beam(nx=8, seed=0x1234):
+    """Config S: hex beam nx x 2 x 2, E=1e5, nu=0.33, rho=1000, left face attached,
+    one step h=0.01 from rest; loads = gravity + seeded uniform[-1,1] N per node."""
+    beam = HexBeam(nx, 2, 2, 1.0, 0.2, density=1000.0, material=("linear", 1e5, 0.33), h=0.01)
+    rng = np.random.default_rng(seed)
+    fext = beam.gravity_loads() + rng.uniform(-1, 1, size=(beam.nn, 3))
+    u = np.zeros((beam.nn, 3))
+    vals, b = beam.system(u, u, fext)
+    return beam, vals, b
+
+
+def large_beam_sequence(nsteps=20, seed=0x5EED, nx=100, ny=16, nz=16, length=1.0, width=0.16):
+    """Config L: hex beam nx x ny x nz, neo-Hookean mu=3.4e4, kappa=1.7e6, rho=1000,
+    h=0.005; the 289 left-face nodes attached (867 constraint rows).  The 20 step
+    states follow a prescribed, growing cantilever bending under seeded random tip
+    loads (quasi-static shape u_z ~ x^2 (3L - x)); each step's system is assembled at
+    its own state, so values change from step to step while the pattern is fixed.
+    Returns (beam, [vals_k], [b_k])."""
+    beam = HexBeam(nx, ny, nz, length, width, density=1000.0,
+                   material=("neohookean", 3.4e4, 1.7e6), h=0.005)
+    rng = np.random.default_rng(seed)
+    X = beam.nodes
+    s = (X[:, 0] - X[:, 0].min()) / length
+    shape = s * s * (3.0 - s) / 2.0
+    tipload = np.zeros(3)
+    grav = beam.gravity_loads()
+    u_prev = np.zeros((beam.nn, 3))
+    out_v, out_b = [], []
+    for k in range(nsteps):
+        tipload = tipload + rng.uniform(-1.0, 1.0, size=3) * np.array([0.2, 1.0, 1.0])
+        defl = 2e-4 * (k + 1) + 1e-4 * tipload  # tip deflection (m)
+        u = np.zeros((beam.nn, 3))
+        u[:, 1] = defl[1] * shape
+        u[:, 2] = -abs(defl[2]) * shape - 2e-4 * (k + 1) * shape
+        u[:, 0] = defl[0] * 0.01 * s
+        v = (u - u_prev) / beam.h
+        fext = grav.copy()
+        fext[beam.tip] += tipload[None, :] / len(beam.tip)
+        vals, b = beam.system(u, v, fext)
+        out_v.append(vals)
+        out_b.append(b)
+        u_prev = u
+    return beam, out_v, out_b

@@ -1,0 +1,160 @@
+/*
+ * kkt_mi355x.h -- C ABI of the MI355X sparse KKT direct solver.
+ *
+ * A drop-in native backend for ArtiSynth's maspack.solvers.DirectSolver
+ * contract (DirectSolver.java:26-254).  Each entry point below replaces one
+ * native method of the reference's cuDSS bridge (CuDssSolver.java:87-144,
+ * implemented in CuDssJNI.cc and cudssBridge.cc); the citation is given per
+ * function.  Plain pointers and sizes only; indices are 0-based; symmetric
+ * matrices are passed as the CSR of their UPPER triangle, exactly as
+ * KKTSolver builds them (KKTSolver.java:171-288, 386-454).
+ *
+ * Status codes: 0 = OK, negative = error (kkt_last_error() has the text).
+ * The first thirteen codes keep the meaning of CUDSS_BRIDGE_* in
+ * cudssBridge.h:39-52; codes below -13 are new.
+ */
+#ifndef KKT_MI355X_H
+#define KKT_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* matrix types, same values as CUDSS_BRIDGE_MT_* (cudssBridge.h:35-37) */
+#define KKT_MT_GENERAL 0
+#define KKT_MT_SYMMETRIC 1
+#define KKT_MT_SPD 2
+
+#define KKT_OK 0
+#define KKT_ERR_INIT -1
+#define KKT_ERR_ALLOC -2
+#define KKT_ERR_COPY -3
+#define KKT_ERR_PATTERN -4
+#define KKT_ERR_ANALYSIS -5
+#define KKT_ERR_FACTORIZATION -6
+#define KKT_ERR_SOLVE -7
+#define KKT_ERR_STATE -8
+#define KKT_ERR_UNKNOWN_MTYPE -9
+#define KKT_ERR_SPMV -10
+#define KKT_ERR_REDUCTION -11
+#define KKT_ERR_ITER_BREAKDOWN -12
+#define KKT_ERR_ITER_NO_CONVERGE -13
+#define KKT_ERR_ARGUMENT -14
+#define KKT_ERR_DEVICE -15
+#define KKT_ERR_REFINE_STALL -16
+
+/* factor precision */
+#define KKT_PREC_FP64 0        /* FP64 fronts, FP64 solves */
+#define KKT_PREC_FP32_REFINE 1 /* FP32 fronts on MFMA + FP64 iterative refinement */
+
+typedef struct kkt_solver kkt_solver;
+
+/* Statistics of the current symbolic analysis / factor. */
+typedef struct kkt_info {
+  int n;
+  int nsuper;
+  int nlevels;
+  int ndelayed;          /* zero-diagonal rows ordered after their neighbours */
+  long long nnz;         /* upper-triangle entries of A */
+  long long nnzL;        /* stored entries of L (with amalgamation zeros) */
+  long long nnzL_exact;  /* exact column-count sum */
+  double factor_flops;   /* sum over supernodes of k^3/3 + r k^2 + r^2 k */
+  long long front_bytes; /* HBM bytes held by the fronts */
+  int npos, nneg;        /* inertia of D (PARDISO iparm[21], iparm[22]) */
+  int nperturbed;        /* perturbed pivots (PARDISO iparm[13]) */
+  int n2x2;              /* 2x2 Bunch-Kaufman pivots */
+  int precision;         /* KKT_PREC_* of the current factor */
+  int refine_steps;      /* refinement steps of the last solve */
+  int fp64_fallbacks;    /* FP64 refactors forced by stalled refinement */
+  double last_rel_residual; /* ||b - A x|| / ||b|| of the last refined solve */
+} kkt_info;
+
+/* CuDssSolver.doInit (CuDssJNI.cc:11-23): creates a handle (NULL on failure). */
+kkt_solver* kkt_create(void);
+/* CuDssSolver.doDispose (CuDssJNI.cc:718-724; cudssBridge.cc:2008-2023).
+ * Releases all device memory; safe on NULL. */
+void kkt_destroy(kkt_solver* h);
+
+/* CuDssSolver.doSetPattern (CuDssJNI.cc:25-45; cudssBridge.cc:291-382).
+ * Copies the pattern; drops any previous analysis and factor. */
+int kkt_set_pattern(kkt_solver* h, int n, int nnz, const int* rowOffs, const int* colIdxs,
+                    int mtype);
+/* CuDssSolver.doAnalyze (CuDssJNI.cc:47-53; cudssBridge.cc:384-407).
+ * Pattern-only analysis; rows whose diagonal turns out to be zero at the
+ * first factor are re-ordered then (once per pattern). */
+int kkt_analyze(kkt_solver* h);
+/* PardisoSolver.analyze(vals,...) (PardisoSolver.java:1378-1432, phase 11 with
+ * values, pardisoMkl.cc:440-511): analysis that also sees the values, so
+ * zero-diagonal (constraint) rows are ordered after their neighbours at once. */
+int kkt_analyze_values(kkt_solver* h, const double* vals);
+/* CuDssSolver.doFactor (CuDssJNI.cc:55-70; cudssBridge.cc:409-455): host
+ * values (nnz doubles) -> numeric LDL^T on the GPU. */
+int kkt_factor(kkt_solver* h, const double* vals);
+/* Same with the values already resident in device memory (no H2D copy). */
+int kkt_factor_device(kkt_solver* h, const double* d_vals);
+/* CuDssSolver.doSolve (CuDssJNI.cc:640-651; cudssBridge.cc:1376-1417):
+ * x = A^-1 b (host arrays of n doubles), plus the configured refinement. */
+int kkt_solve(kkt_solver* h, const double* b, double* x);
+/* Same on device arrays. */
+int kkt_solve_device(kkt_solver* h, const double* d_b, double* d_x);
+/* CuDssSolver.doSolveMulti (cudssBridge.cc:1454-1523): column-major n x nrhs. */
+int kkt_solve_multi(kkt_solver* h, int nrhs, const double* B, double* X);
+/* CuDssSolver.doMultiply (cudssBridge.cc:1419-1452): y = A x with the values
+ * of the last factor / iterative solve. */
+int kkt_multiply(kkt_solver* h, const double* x, double* y);
+/* CuDssSolver.doIterativeSolve (CuDssJNI.cc:700-716; cudssBridge.cc:1705-2006):
+ * BiCGStab on the NEW values `vals`, preconditioned by the existing (stale)
+ * factor.  Returns iterations (>= 1) or a negative status. */
+int kkt_iterative_solve(kkt_solver* h, const double* vals, const double* b, double* x,
+                        double tolRel, int maxIter);
+/* Same on device arrays. */
+int kkt_iterative_solve_device(kkt_solver* h, const double* d_vals, const double* d_b,
+                               double* d_x, double tolRel, int maxIter);
+/* CuDssSolver.doSetIrSteps (cudssBridge.cc setIterativeRefinementSteps):
+ * refinement steps after each FP64 solve (default 0; FP32 factors always
+ * refine to the refinement tolerance). */
+int kkt_set_ir_steps(kkt_solver* h, int nsteps);
+/* CuDssSolver.doGetLastError (cudssBridge.h:187-189): NULL after success. */
+const char* kkt_last_error(kkt_solver* h);
+/* CuDssSolver.doGetVersion. */
+const char* kkt_version(void);
+/* CuDssSolver.doSetTimingEnabled (cudssBridge.cc:79-91): per-phase stderr lines. */
+void kkt_set_timing_enabled(int on);
+
+/* ---- extensions of this backend (no counterpart in the cuDSS bridge) ---- */
+/* KKT_PREC_FP64 or KKT_PREC_FP32_REFINE; applies from the next factor. */
+int kkt_set_precision(kkt_solver* h, int prec);
+/* relative-residual target of the FP64 refinement of FP32 factors (default 1e-10) */
+int kkt_set_refine_tol(kkt_solver* h, double tol, int max_steps);
+/* ordering: 0 natural, 1 nested dissection (default) */
+int kkt_set_ordering(kkt_solver* h, int ordering);
+int kkt_get_info(kkt_solver* h, kkt_info* out);
+/* Copies of the host symbolic structure (for structure-parity checks):
+ * perm (n), sup_ptr (nsuper+1), sparent (nsuper), rows_ptr (nsuper+1, int64),
+ * rows (rows_ptr[nsuper]).  Any pointer may be NULL. */
+int kkt_get_symbolic(kkt_solver* h, int* perm, int* sup_ptr, int* sparent, long long* rows_ptr,
+                     int* rows);
+/* Reads back the device-resident structure arrays and compares them with the
+ * host arrays byte for byte; returns 0 when identical, 1 when not. */
+int kkt_verify_device_structure(kkt_solver* h);
+/* Kernel timing (HIP events on the solver's stream) accumulated since the
+ * last reset, per kernel class: 0 assemble, 1 panel-diag, 2 panel-trsm,
+ * 3 panel-update (MFMA), 4 solve-level, 5 solve-gemv, 6 spmv, 7 vector ops.
+ * ms[c], launches[c]; `flops` = algorithmic flops of class-3 launches. */
+void kkt_timing_reset(kkt_solver* h, int enable);
+int kkt_timing_get(kkt_solver* h, double* ms, long long* launches, double* update_flops);
+
+/* Host-only symbolic analysis (no GPU needed): structure statistics and the
+ * same arrays kkt_get_symbolic returns, for pattern checks on the CPU. */
+typedef struct kkt_symbolic kkt_symbolic;
+kkt_symbolic* kkt_symbolic_create(int n, int nnz, const int* rowOffs, const int* colIdxs,
+                                  const unsigned char* delay, int ordering);
+int kkt_symbolic_info(kkt_symbolic* s, kkt_info* out);
+int kkt_symbolic_get(kkt_symbolic* s, int* perm, int* parent, int* colcount, int* sup_ptr,
+                     int* sparent, long long* rows_ptr, int* rows, int* level);
+void kkt_symbolic_destroy(kkt_symbolic* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,140 @@
+"""ctypes face of the CPU oracle (oracle/kkt_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product path never loads it.
+Reference citations per function are in kkt_oracle.h.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "libkkt_oracle.so")
+_lib = None
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle` (%s)" % _LIB_PATH)
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_spmv_upper.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
+        L.orc_spmv_upper.restype = None
+        L.orc_symbolic.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p, _i32p, _i32p]
+        L.orc_symbolic.restype = ctypes.c_int
+        L.orc_supernode_rows.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p, ctypes.c_int,
+                                         _i32p, _i64p, _i32p, ctypes.c_long]
+        L.orc_supernode_rows.restype = ctypes.c_long
+        L.orc_factor_create.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _i32p, ctypes.c_int,
+                                        _i32p, ctypes.c_int, ctypes.c_double]
+        L.orc_factor_create.restype = ctypes.c_void_p
+        L.orc_factor_free.argtypes = [ctypes.c_void_p]
+        L.orc_factor_free.restype = None
+        L.orc_factor_solve.argtypes = [ctypes.c_void_p, _f64p, _f64p]
+        L.orc_factor_solve.restype = None
+        L.orc_factor_info.argtypes = [ctypes.c_void_p, _i64p]
+        L.orc_factor_info.restype = None
+        L.orc_bicgstab.argtypes = [ctypes.c_void_p, ctypes.c_int, _i32p, _i32p, _f64p, _f64p,
+                                   _f64p, ctypes.c_double, ctypes.c_int]
+        L.orc_bicgstab.restype = ctypes.c_int
+        L.orc_refine.argtypes = [ctypes.c_void_p, ctypes.c_int, _i32p, _i32p, _f64p, _f64p,
+                                 _f64p, ctypes.c_double, ctypes.c_int]
+        L.orc_refine.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def spmv_upper(rowOffs, colIdxs, vals, x):
+    n = len(rowOffs) - 1
+    y = np.zeros(n)
+    lib().orc_spmv_upper(n, _i32(rowOffs), _i32(colIdxs), _f64(vals), _f64(x), y)
+    return y
+
+
+def symbolic(rowOffs, colIdxs, perm):
+    n = len(rowOffs) - 1
+    parent = np.zeros(n, np.int32)
+    cc = np.zeros(n, np.int32)
+    rc = lib().orc_symbolic(n, _i32(rowOffs), _i32(colIdxs), _i32(perm), parent, cc)
+    if rc != 0:
+        raise RuntimeError("orc_symbolic failed")
+    return parent, cc
+
+
+def supernode_rows(rowOffs, colIdxs, perm, sup_ptr):
+    n = len(rowOffs) - 1
+    ns = len(sup_ptr) - 1
+    cap = max(1024, 16 * n)
+    while True:
+        rp = np.zeros(ns + 1, np.int64)
+        rows = np.zeros(cap, np.int32)
+        t = lib().orc_supernode_rows(n, _i32(rowOffs), _i32(colIdxs), _i32(perm), ns,
+                                     _i32(sup_ptr), rp, rows, cap)
+        if t == -2:
+            cap *= 4
+            continue
+        if t < 0:
+            raise RuntimeError("orc_supernode_rows failed")
+        return rp, rows[:t].copy()
+
+
+class Factor:
+    """FP64 multifrontal LDL^T with panel-restricted Bunch-Kaufman pivoting."""
+
+    def __init__(self, rowOffs, colIdxs, vals, perm=None, sup_ptr=None, nb=64, pert=1e-8):
+        self.rowOffs = _i32(rowOffs)
+        self.colIdxs = _i32(colIdxs)
+        self.vals = _f64(vals)
+        self.n = n = len(rowOffs) - 1
+        if perm is None:
+            perm = np.arange(n, dtype=np.int32)
+        if sup_ptr is None:
+            sup_ptr = np.array([0, n], np.int32)
+        self._h = lib().orc_factor_create(n, self.rowOffs, self.colIdxs, self.vals, _i32(perm),
+                                          len(sup_ptr) - 1, _i32(sup_ptr), nb, pert)
+        if not self._h:
+            raise RuntimeError("oracle factorization failed")
+
+    def solve(self, b):
+        x = np.zeros(self.n)
+        lib().orc_factor_solve(self._h, _f64(b), x)
+        return x
+
+    def info(self):
+        a = np.zeros(5, np.int64)
+        lib().orc_factor_info(self._h, a)
+        return {"npos": int(a[0]), "nneg": int(a[1]), "nperturbed": int(a[2]),
+                "n2x2": int(a[3]), "nnzL": int(a[4])}
+
+    def bicgstab(self, b, tolRel, maxIter=32, vals=None):
+        x = np.zeros(self.n)
+        v = self.vals if vals is None else _f64(vals)
+        it = lib().orc_bicgstab(self._h, self.n, self.rowOffs, self.colIdxs, v, _f64(b), x,
+                                tolRel, maxIter)
+        return it, x
+
+    def refine(self, b, tol=1e-8, maxSteps=20, x0=None):
+        x = self.solve(b) if x0 is None else _f64(x0).copy()
+        steps = lib().orc_refine(self._h, self.n, self.rowOffs, self.colIdxs, self.vals,
+                                 _f64(b), x, tol, maxSteps)
+        return steps, x
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().orc_factor_free(h)
+            self._h = None

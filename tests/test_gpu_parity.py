@@ -1,0 +1,137 @@
+"""GPU parity tests: the HIP path through the C ABI against the CPU oracle and the
+reference's known answers.  Marked gpu; run with `pytest -m gpu`."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from artisynth_core_amd import KKT_PREC_FP32_REFINE, KKT_PREC_FP64
+from artisynth_core_amd import generators as G
+from artisynth_core_amd.solvers import ImproperStateException, MI355XSolver, Matrix, NumericalException
+from kkt_helpers import (IND3_COLS, IND3_ROWS, IND3_VALS, KKT5_B, KKT5_COLS, KKT5_ROWS, KKT5_VALS,
+                         P5_B, P5_COLS, P5_ROWS, P5_VALS, SPD5_COLS, SPD5_RHS, SPD5_ROWS, SPD5_VALS,
+                         SPD5_X, full_from_upper, rel_err, rel_residual)
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print("[gpu-case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("prec", [KKT_PREC_FP64, KKT_PREC_FP32_REFINE])
+def test_pardiso_5x5_known_answers(gpu, prec):
+    # PardisoSolverTest.java:407-543: inertia 2-/3+, 0 perturbed pivots, 15-digit x
+    _say("pardiso5x5 prec=%d" % prec)
+    s = MI355XSolver(precision=prec)
+    try:
+        s.analyzeArrays(P5_VALS[0][0], P5_COLS, P5_ROWS, 5, Matrix.SYMMETRIC)
+        for vals, xchk in P5_VALS:
+            s.factor(vals)
+            x = np.zeros(5)
+            s.solve(x, P5_B)
+            assert np.abs(x - xchk).max() < 1e-12, (x, xchk)
+        s.factor(P5_VALS[0][0])
+        assert s.getNumNegEigenvalues() == 2
+        assert s.getNumPosEigenvalues() == 3
+        assert s.getNumPerturbedPivots() == 0
+        # multiple right-hand sides, column-major (PardisoSolverTest.java:479-497)
+        B = np.concatenate([P5_B, 2 * P5_B, 4 * P5_B])
+        X = np.zeros(15)
+        s.solve(X, B, 3)
+        xc = P5_VALS[0][1]
+        assert np.abs(X - np.concatenate([xc, 2 * xc, 4 * xc])).max() < 1e-12
+    finally:
+        s.dispose()
+
+
+def test_cudss_fixtures(gpu):
+    _say("cudss fixtures")
+    s = MI355XSolver()
+    try:
+        s.analyzeArrays(SPD5_VALS, SPD5_COLS, SPD5_ROWS, 5, Matrix.SPD)
+        s.factor(SPD5_VALS)
+        x = np.zeros(5)
+        s.solve(x, SPD5_RHS)
+        assert np.abs(x - SPD5_X).max() < 1e-12
+        s.factor(SPD5_VALS * 2.5)  # refactor with scaled values (CuDssSolverTest.java:190-217)
+        s.solve(x, SPD5_RHS)
+        assert np.abs(x - SPD5_X / 2.5).max() < 1e-12
+        s.analyzeArrays(IND3_VALS, IND3_COLS, IND3_ROWS, 3, Matrix.SYMMETRIC)
+        s.factor(IND3_VALS)
+        b = np.array([1.0, 2.0, 3.0])
+        s.solve(x, b)
+        A = full_from_upper(IND3_ROWS, IND3_COLS, IND3_VALS).toarray()
+        assert np.abs(x[:3] - np.linalg.solve(A, b)).max() < 1e-13
+        assert s.getNumNegEigenvalues() == 1 and s.getNumPosEigenvalues() == 2
+        s.analyzeArrays(KKT5_VALS, KKT5_COLS, KKT5_ROWS, 5, Matrix.SYMMETRIC)
+        s.factor(KKT5_VALS)
+        x = np.zeros(5)
+        s.solve(x, KKT5_B)
+        assert rel_residual(KKT5_ROWS, KKT5_COLS, KKT5_VALS, x, KKT5_B) <= 1e-9
+    finally:
+        s.dispose()
+
+
+def test_state_errors(gpu):
+    _say("state errors")
+    s = MI355XSolver()
+    with pytest.raises(ImproperStateException):
+        s.solve(np.zeros(5), SPD5_RHS)
+    s.analyzeArrays(SPD5_VALS, SPD5_COLS, SPD5_ROWS, 5, Matrix.SPD)
+    with pytest.raises(ImproperStateException):
+        s.solve(np.zeros(5), SPD5_RHS)
+    assert s.iterativeSolve(SPD5_VALS, np.zeros(5), SPD5_RHS, 10) <= 0
+    s.dispose()
+    s.dispose()  # idempotent (CuDssSolverTest.java:333-338)
+
+
+@pytest.mark.parametrize("nx", [8, 26])
+@pytest.mark.parametrize("prec", [KKT_PREC_FP64, KKT_PREC_FP32_REFINE])
+def test_small_beam_vs_oracle(gpu, nx, prec):
+    _say("small beam nx=%d prec=%d" % (nx, prec))
+    beam, vals, b = G.small_beam(nx)
+    s = MI355XSolver(precision=prec)
+    try:
+        s.analyzeArrays(vals, beam.colIdxs, beam.rowOffs, beam.n, Matrix.SYMMETRIC)
+        assert s.verifyDeviceStructure()
+        s.factor(vals)
+        x = np.zeros(beam.n)
+        s.solve(x, b)
+        sym = s.symbolic()
+        ref = O.Factor(beam.rowOffs, beam.colIdxs, vals, perm=sym["perm"], sup_ptr=sym["sup_ptr"])
+        xo = ref.solve(b)
+        assert rel_residual(beam.rowOffs, beam.colIdxs, vals, x, b) <= 1e-8
+        assert rel_err(x, xo) <= 1e-8
+        inf = s.info()
+        oi = ref.info()
+        assert (inf["npos"], inf["nneg"]) == (oi["npos"], oi["nneg"])
+        assert inf["nnzL"] == oi["nnzL"]
+    finally:
+        s.dispose()
+
+
+def test_iterative_solve_stale_factor(gpu):
+    # CuDssSolverTest.java:993-1094 / PardisoSolverTest.java:668-688: values perturbed
+    # after the factor; BiCGStab with the stale factor reaches 1e-9
+    _say("iterative stale factor")
+    beam, vals, b = G.small_beam(26)
+    s = MI355XSolver()
+    try:
+        s.analyzeArrays(vals, beam.colIdxs, beam.rowOffs, beam.n, Matrix.SYMMETRIC)
+        s.factor(vals)
+        v2 = vals * 1.05
+        x = np.zeros(beam.n)
+        it = s.iterativeSolve(v2, x, b, 10)
+        assert it > 0
+        assert rel_residual(beam.rowOffs, beam.colIdxs, v2, x, b) <= 1e-9
+        sym = s.symbolic()
+        ref = O.Factor(beam.rowOffs, beam.colIdxs, vals, perm=sym["perm"], sup_ptr=sym["sup_ptr"])
+        it_o, xo = ref.bicgstab(b, 1e-10, 32, vals=v2)
+        assert it_o > 0
+        assert abs(it - it_o) <= 1
+        assert rel_err(x, xo) <= 1e-8
+    finally:
+        s.dispose()
