@@ -244,7 +244,8 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
 
 // ------------------------------------------------------------------------
 // MFMA 16x16x4 wrappers (f64 / f32).  Lane l supplies A[l%16][l/16] and
-// B[l/16][l%16]; the 16x16 result sits in lane l as C[4*(l/16)+v][l%16].
+// B[l/16][l%16]; the 16x16 result sits in lane l, element v, at column l%16 and
+// row Mfma<T>::row(l/16, v): 4*(l/16)+v for f32, l/16+4*v for f64 (gfx950).
 // ------------------------------------------------------------------------
 template <typename T>
 struct Mfma;
@@ -254,6 +255,8 @@ struct Mfma<double> {
   static __device__ inline v4 op(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // row of accumulator element v held by a lane in row group lk = lane/16
+  static __device__ inline int row(int lk, int v) { return lk + 4 * v; }
 };
 template <>
 struct Mfma<float> {
@@ -261,6 +264,7 @@ struct Mfma<float> {
   static __device__ inline v4 op(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ inline int row(int lk, int v) { return 4 * lk + v; }
 };
 
 // ------------------------------------------------------------------------
@@ -330,7 +334,7 @@ __global__ __launch_bounds__(256) void k_panel_trsm(const int2* __restrict__ ite
     for (int bj = 0; bj < 2; bj++)
 #pragma unroll
       for (int v = 0; v < 4; v++)
-        As[(wr + bi * 16 + 4 * lk + v) * LD + wc + bj * 16 + lr] = acc[bi][bj][v];
+        As[(wr + bi * 16 + Mfma<T>::row(lk, v)) * LD + wc + bj * 16 + lr] = acc[bi][bj][v];
   __syncthreads();
   T* Ws = Wb + d.sn_woff[s];
   for (int e = threadIdx.x; e < 64 * 64; e += 256) {
@@ -388,7 +392,7 @@ __global__ __launch_bounds__(256) void k_panel_update(const int4* __restrict__ i
     for (int bj = 0; bj < 2; bj++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int row = wr + bi * 16 + 4 * lk + v, col = wc + bj * 16 + lr;
+        const int row = wr + bi * 16 + Mfma<T>::row(lk, v), col = wc + bj * 16 + lr;
         acc[bi][bj][v] = (row < nI && col < nJ) ? Fs[(size_t)(rJ + col) * m + rI + row] : (T)0;
       }
   const int kmax = (w + 3) & ~3;
@@ -408,7 +412,7 @@ __global__ __launch_bounds__(256) void k_panel_update(const int4* __restrict__ i
     for (int bj = 0; bj < 2; bj++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int row = wr + bi * 16 + 4 * lk + v, col = wc + bj * 16 + lr;
+        const int row = wr + bi * 16 + Mfma<T>::row(lk, v), col = wc + bj * 16 + lr;
         if (row < nI && col < nJ && rI + row >= rJ + col)
           Fs[(size_t)(rJ + col) * m + rI + row] = acc[bi][bj][v];
       }

@@ -92,6 +92,12 @@ class HexBeam:
         self.material = material
         hx, hy, hz = length / nx, width / ny, height / nz
         self.dN, self.wdet = hex_gauss_grads(hx, hy, hz)
+        # B_g[(i,J), (a,k)] = delta_ik dN_a/dX_J at Gauss point g
+        self._Bmat = np.zeros((8, 9, 24))
+        for g in range(8):
+            for i in range(3):
+                for J in range(3):
+                    self._Bmat[g, 3 * i + J, i::3] = self.dN[g, :, J]
         # lumped mass
         m_node = np.zeros(self.nn)
         np.add.at(m_node, self.elems.ravel(), density * hx * hy * hz / 8.0)
@@ -164,8 +170,13 @@ class HexBeam:
             ue = u[self.elems]  # (E,8,3)
             F = np.eye(3)[None, None] + np.einsum("eai,gaJ->egiJ", ue, self.dN)
             P, A = neo_hookean_PA(F, lam, mu)
-            T1 = np.einsum("gaJ,egiJkL->egaikL", self.dN, A)
-            Ke = np.einsum("egaikL,gbL->eaibk", T1, self.dN).reshape(E, 24, 24) * self.wdet
+            A9 = np.ascontiguousarray(A.reshape(E, 8, 9, 9))
+            Ke = np.zeros((E, 24, 24))
+            for g in range(8):
+                Bg = self._Bmat[g]  # (9, 24): grad u (i,J) from nodal u (a,k)
+                C = (A9[:, g].reshape(E * 9, 9) @ Bg).reshape(E, 9, 24)
+                Ke += (Bg.T @ C.transpose(1, 0, 2).reshape(9, E * 24)).reshape(24, E, 24).transpose(1, 0, 2)
+            Ke *= self.wdet
             fint = np.einsum("egiJ,gaJ->eai", P, self.dN).reshape(E, 24) * self.wdet
             return Ke, fint
         raise ValueError("unknown material %r" % (mat,))

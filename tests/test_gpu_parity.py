@@ -135,3 +135,62 @@ def test_iterative_solve_stale_factor(gpu):
         assert rel_err(x, xo) <= 1e-8
     finally:
         s.dispose()
+
+
+@pytest.mark.parametrize("prec", [KKT_PREC_FP64, KKT_PREC_FP32_REFINE])
+def test_multipanel_beam_vs_oracle(gpu, prec):
+    # fronts wider than one 64-column panel and several tree levels
+    _say("multipanel beam 24x8x8 prec=%d" % prec)
+    beam = G.HexBeam(24, 8, 8, 1.0, 0.2, material=("neohookean", 3.4e4, 1.7e6), h=0.005)
+    rng = np.random.default_rng(5)
+    u = rng.uniform(-1e-4, 1e-4, (beam.nn, 3))
+    vals, b = beam.system(u, np.zeros_like(u), beam.gravity_loads())
+    s = MI355XSolver(precision=prec)
+    try:
+        s.analyzeArrays(vals, beam.colIdxs, beam.rowOffs, beam.n, Matrix.SYMMETRIC)
+        sym = s.symbolic()
+        assert np.diff(sym["sup_ptr"]).max() > 64
+        assert s.verifyDeviceStructure()
+        s.factor(vals)
+        x = np.zeros(beam.n)
+        s.solve(x, b)
+        ref = O.Factor(beam.rowOffs, beam.colIdxs, vals, perm=sym["perm"], sup_ptr=sym["sup_ptr"])
+        xo = ref.solve(b)
+        assert rel_residual(beam.rowOffs, beam.colIdxs, vals, x, b) <= 1e-8
+        assert rel_err(x, xo) <= 1e-7
+        inf, oi = s.info(), ref.info()
+        assert (inf["npos"], inf["nneg"]) == (oi["npos"], oi["nneg"]) == (beam.sizeM, beam.numG)
+        # bitwise reproducible: no order-dependent atomics
+        x2 = np.zeros(beam.n)
+        s.factor(vals)
+        s.solve(x2, b)
+        assert np.array_equal(x, x2)
+    finally:
+        s.dispose()
+
+
+def test_test_matrix_on_gpu(gpu):
+    # PardisoSolverTest.java:608-688 on the GPU path
+    _say("testMatrix")
+    import os
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "testMatrix_upper.npz"))
+    ro, ci, va = d["rowOffs"], d["colIdxs"], d["vals"]
+    n = len(ro) - 1
+    s = MI355XSolver()
+    try:
+        s.analyzeArrays(va, ci, ro, n, Matrix.SYMMETRIC)
+        s.factor(va)
+        b = np.ones(n)
+        x = np.zeros(n)
+        s.solve(x, b)
+        A = full_from_upper(ro, ci, va)
+        assert np.abs(A @ x - b).max() <= 1e-10
+        norm = np.sqrt((A.data ** 2).sum() / A.nnz)
+        rng = np.random.default_rng(0x1234)
+        va2 = va + (rng.random(len(va)) - 0.5) * (norm / 1e4)
+        x2 = np.zeros(n)
+        it = s.iterativeSolve(va2, x2, b, 10)
+        assert it > 0
+        assert np.abs(full_from_upper(ro, ci, va2) @ x2 - b).max() <= 1e-9
+    finally:
+        s.dispose()

@@ -1,0 +1,175 @@
+"""CPU tests: pin the oracle against the reference's known answers, brute force
+(dense LAPACK) and invariants, and check that planted mistakes are caught."""
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+
+import oracle as O
+from artisynth_core_amd import generators as G
+from kkt_helpers import (IND3_COLS, IND3_ROWS, IND3_VALS, KKT5_B, KKT5_COLS, KKT5_ROWS, KKT5_VALS,
+                         P5_B, P5_COLS, P5_ROWS, P5_VALS, SPD5_COLS, SPD5_RHS, SPD5_ROWS, SPD5_VALS,
+                         SPD5_X, full_from_upper, rel_err, rel_residual)
+from symbolic_helpers import product_symbolic
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def load_test_matrix():
+    d = np.load(os.path.join(HERE, "golden", "testMatrix_upper.npz"))
+    return d["rowOffs"], d["colIdxs"], d["vals"]
+
+
+@pytest.mark.parametrize("nb", [64, 2, 1])
+def test_pardiso_5x5_known_answers(nb):
+    # PardisoSolverTest.java:407-543
+    for vals, xchk in P5_VALS:
+        f = O.Factor(P5_ROWS, P5_COLS, vals, nb=nb)
+        x = f.solve(P5_B)
+        assert np.abs(x - xchk).max() < 1e-12
+    inf = O.Factor(P5_ROWS, P5_COLS, P5_VALS[0][0], nb=nb).info()
+    assert (inf["nneg"], inf["npos"], inf["nperturbed"]) == (2, 3, 0)
+
+
+def test_pardiso_5x5_with_product_ordering():
+    vals = P5_VALS[0][0]
+    sym = product_symbolic(P5_ROWS, P5_COLS, vals)
+    f = O.Factor(P5_ROWS, P5_COLS, vals, perm=sym["perm"], sup_ptr=sym["sup_ptr"])
+    assert np.abs(f.solve(P5_B) - P5_VALS[0][1]).max() < 1e-12
+    assert (f.info()["nneg"], f.info()["npos"]) == (2, 3)
+
+
+def test_cudss_fixtures():
+    # CuDssSolverTest.java:55-107, 190-217, 343-349, 870-947
+    f = O.Factor(SPD5_ROWS, SPD5_COLS, SPD5_VALS)
+    assert np.abs(f.solve(SPD5_RHS) - SPD5_X).max() < 1e-13
+    f = O.Factor(SPD5_ROWS, SPD5_COLS, SPD5_VALS * 2.5)
+    assert np.abs(f.solve(SPD5_RHS) - SPD5_X / 2.5).max() < 1e-13
+    f = O.Factor(IND3_ROWS, IND3_COLS, IND3_VALS)
+    b = np.array([1.0, 2.0, 3.0])
+    # closed form: [[1,2],[2,1]] inverse = [[-1,2],[2,-1]]/3, third row 1/3
+    assert np.abs(f.solve(b) - np.array([(-1 + 4) / 3, (2 - 2) / 3, 1.0])).max() < 1e-14
+    assert (f.info()["nneg"], f.info()["npos"]) == (1, 2)
+    f = O.Factor(KKT5_ROWS, KKT5_COLS, KKT5_VALS)
+    assert rel_residual(KKT5_ROWS, KKT5_COLS, KKT5_VALS, f.solve(KKT5_B), KKT5_B) <= 1e-12
+
+
+def test_spmv_matches_dense():
+    rng = np.random.default_rng(7)
+    beam, vals, b = G.small_beam(8)
+    A = full_from_upper(beam.rowOffs, beam.colIdxs, vals).toarray()
+    x = rng.standard_normal(beam.n)
+    assert np.abs(O.spmv_upper(beam.rowOffs, beam.colIdxs, vals, x) - A @ x).max() < 1e-12
+
+
+def test_test_matrix_residual_and_hybrid():
+    # PardisoSolverTest.java:608-688: factor+solve to inf-norm residual <= 1e-10 with
+    # b = 1, then values perturbed by norm/1e4 and iterativeSolve(...,10) to <= 1e-9
+    ro, ci, va = load_test_matrix()
+    n = len(ro) - 1
+    sym = product_symbolic(ro, ci, va)
+    f = O.Factor(ro, ci, va, perm=sym["perm"], sup_ptr=sym["sup_ptr"])
+    b = np.ones(n)
+    x = f.solve(b)
+    A = full_from_upper(ro, ci, va)
+    assert np.abs(A @ x - b).max() <= 1e-10
+    full_entries = A.nnz
+    norm = np.sqrt((A.data ** 2).sum() / full_entries)
+    rng = np.random.default_rng(0x1234)
+    va2 = va + (rng.random(len(va)) - 0.5) * (norm / 1e4)
+    it, x2 = f.bicgstab(b, 1e-10, 32, vals=va2)
+    assert it > 0
+    A2 = full_from_upper(ro, ci, va2)
+    assert np.abs(A2 @ x2 - b).max() <= 1e-9
+
+
+@pytest.mark.parametrize("nx", [8, 26])
+def test_beam_brute_force_and_inertia(nx):
+    beam, vals, b = G.small_beam(nx)
+    A = full_from_upper(beam.rowOffs, beam.colIdxs, vals).toarray()
+    xd = sla.solve(A, b, assume_a="sym")
+    sym = product_symbolic(beam.rowOffs, beam.colIdxs, vals)
+    f = O.Factor(beam.rowOffs, beam.colIdxs, vals, perm=sym["perm"], sup_ptr=sym["sup_ptr"])
+    assert rel_err(f.solve(b), xd) < 1e-12
+    ev = np.linalg.eigvalsh(A)
+    inf = f.info()
+    assert (inf["npos"], inf["nneg"]) == ((ev > 0).sum(), (ev < 0).sum())
+    # the KKT has exactly numG negative eigenvalues (S SPD, G full row rank)
+    assert inf["nneg"] == beam.numG
+
+
+def test_beam_sizes_match_survey():
+    beam, vals, b = G.small_beam(8)
+    assert (beam.n, beam.nnz) == (270, 5688)
+    beam, vals, b = G.small_beam(26)
+    assert (beam.n, beam.nnz) == (756, 17838)
+
+
+def test_random_indefinite_needs_pivoting():
+    # symmetric indefinite matrices with zero diagonals: Bunch-Kaufman 2x2 pivots
+    rng = np.random.default_rng(11)
+    for trial in range(5):
+        n = 40
+        A = rng.standard_normal((n, n))
+        A = A + A.T
+        A[np.arange(0, n, 3), np.arange(0, n, 3)] = 0.0
+        U = sp.csr_matrix(np.triu(A))
+        U.sort_indices()
+        ro, ci, va = U.indptr.astype(np.int32), U.indices.astype(np.int32), U.data
+        # keep explicit zero diagonals in the pattern
+        D = sp.csr_matrix((np.zeros(n), (np.arange(n), np.arange(n))), shape=(n, n))
+        U = sp.csr_matrix(sp.triu(sp.csr_matrix(A)) + D)
+        U.sort_indices()
+        ro, ci = U.indptr.astype(np.int32), U.indices.astype(np.int32)
+        va = np.array([A[i, j] for i in range(n) for j in ci[ro[i]:ro[i + 1]]])
+        b = rng.standard_normal(n)
+        for nb in (64, 8):
+            f = O.Factor(ro, ci, va, nb=nb)
+            x = f.solve(b)
+            assert rel_residual(ro, ci, va, x, b) < 1e-10
+            ev = np.linalg.eigvalsh(A)
+            assert (f.info()["npos"], f.info()["nneg"]) == ((ev > 0).sum(), (ev < 0).sum())
+
+
+# ---------------------------------------------------------------------------
+# planted mistakes: each must be caught by the pins above
+# ---------------------------------------------------------------------------
+def test_planted_dropped_transpose_term_is_caught():
+    # a symmetric SpMV that forgets the U^T part (a dropped term) fails the dense check
+    beam, vals, b = G.small_beam(8)
+    A = full_from_upper(beam.rowOffs, beam.colIdxs, vals)
+    x = np.random.default_rng(1).standard_normal(beam.n)
+    U = sp.csr_matrix((vals, beam.colIdxs, beam.rowOffs), shape=(beam.n, beam.n))
+    assert np.abs(U @ x - A @ x).max() > 1e-3
+    assert np.abs(O.spmv_upper(beam.rowOffs, beam.colIdxs, vals, x) - A @ x).max() < 1e-12
+
+
+def test_planted_wrong_sign_constraint_block_is_caught():
+    # solving with G^T negated in the upper block (a sign error) changes x by O(1)
+    # and the residual check against the true matrix fails
+    beam, vals, b = G.small_beam(8)
+    bad = vals.copy()
+    bad[beam._gpos] *= -1.0
+    f = O.Factor(beam.rowOffs, beam.colIdxs, bad)
+    x = f.solve(b)
+    assert rel_residual(beam.rowOffs, beam.colIdxs, vals, x, b) > 1e-3
+
+
+def test_planted_missing_2x2_pivot_is_caught():
+    # on the PARDISO 5x5 (zero diagonals), an LDL^T without pivoting perturbs the
+    # zero pivot: with nb=1 the oracle cannot form 2x2 pivots; the known answer still
+    # holds only because the Schur complement pivots are nonzero -- a zero-pivot
+    # matrix [[0,1],[1,0]] must use a 2x2 pivot
+    ro = np.array([0, 2, 3], np.int32)
+    ci = np.array([0, 1, 1], np.int32)
+    va = np.array([0.0, 1.0, 0.0])
+    b = np.array([2.0, 3.0])
+    f = O.Factor(ro, ci, va, nb=64)
+    assert np.abs(f.solve(b) - np.array([3.0, 2.0])).max() < 1e-14
+    assert f.info()["n2x2"] == 1 and f.info()["nperturbed"] == 0
+    f1 = O.Factor(ro, ci, va, nb=1)  # panel width 1: only perturbed 1x1 pivots possible
+    assert f1.info()["nperturbed"] >= 1
+    # the perturbed factor misses the 1e-12 known-answer tolerance used above
+    assert np.abs(f1.solve(b) - np.array([3.0, 2.0])).max() > 1e-10

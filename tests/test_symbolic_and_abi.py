@@ -1,0 +1,167 @@
+"""CPU tests: host symbolic analysis (bit-exact against the oracle's independent
+symbolic elimination + invariants), the C-ABI library surface, and the host logic
+of the DirectSolver / KKTSolver mirror."""
+import os
+import re
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle as O
+from artisynth_core_amd import generators as G
+from artisynth_core_amd import _lib
+from artisynth_core_amd.solvers import (KKTSolver, MI355XSolver, Matrix, NumericalException,
+                                        kkt_upper_csr)
+from kkt_helpers import P5_COLS, P5_ROWS, P5_VALS
+from symbolic_helpers import product_symbolic
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _cases():
+    beam, vals, b = G.small_beam(8)
+    yield "beam8", beam.rowOffs, beam.colIdxs, vals
+    beam, vals, b = G.small_beam(26)
+    yield "beam26", beam.rowOffs, beam.colIdxs, vals
+    d = np.load(os.path.join(ROOT, "tests", "golden", "testMatrix_upper.npz"))
+    yield "testMatrix", d["rowOffs"], d["colIdxs"], d["vals"]
+    yield "pardiso5", P5_ROWS, P5_COLS, P5_VALS[0][0]
+    beam = G.HexBeam(12, 5, 4, 1.0, 0.2)
+    vals, b = beam.system(np.zeros((beam.nn, 3)), np.zeros((beam.nn, 3)), beam.gravity_loads())
+    yield "beam12x5x4", beam.rowOffs, beam.colIdxs, vals
+
+
+@pytest.mark.parametrize("case", list(_cases()), ids=lambda c: c[0])
+@pytest.mark.parametrize("ordering", [1, 0])
+def test_symbolic_matches_oracle(case, ordering):
+    name, ro, ci, va = case
+    n = len(ro) - 1
+    S = product_symbolic(ro, ci, va, ordering=ordering)
+    perm = S["perm"]
+    # permutation is a bijection
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    # elimination tree and column counts: bit-exact against symbolic elimination
+    par, cc = O.symbolic(ro, ci, perm)
+    assert np.array_equal(par, S["parent"])
+    assert np.array_equal(cc, S["colcount"])
+    assert S["info"]["nnzL_exact"] == int(cc.sum())
+    assert np.all((S["parent"] == -1) | (S["parent"] > np.arange(n)))
+    # supernode row structures: bit-exact against the oracle's union construction
+    rp, rows = O.supernode_rows(ro, ci, perm, S["sup_ptr"])
+    assert np.array_equal(rp, S["rows_ptr"])
+    assert np.array_equal(rows, S["rows"])
+    sup = S["sup_ptr"]
+    ns = len(sup) - 1
+    assert sup[0] == 0 and sup[-1] == n and np.all(np.diff(sup) > 0) and np.all(np.diff(sup) <= 1024)
+    # nested structure: child rows outside the parent's columns lie in the parent's rows
+    for s in range(ns):
+        p = S["sparent"][s]
+        r = rows[rp[s]:rp[s + 1]]
+        if p < 0:
+            assert len(r) == 0
+            continue
+        assert p > s and sup[p] <= r[0] < sup[p + 1]
+        outside = r[r >= sup[p + 1]]
+        assert np.all(np.isin(outside, rows[rp[p]:rp[p + 1]]))
+        assert S["level"][p] > S["level"][s]
+
+
+def test_delayed_rows_follow_their_neighbours():
+    beam, vals, b = G.small_beam(26)
+    S = product_symbolic(beam.rowOffs, beam.colIdxs, vals)
+    assert S["info"]["ndelayed"] == beam.numG
+    iperm = np.empty(beam.n, np.int64)
+    iperm[S["perm"]] = np.arange(beam.n)
+    A = sp.csr_matrix((vals, beam.colIdxs, beam.rowOffs), shape=(beam.n, beam.n))
+    A = A + A.T
+    for lam in range(beam.sizeM, beam.n):
+        nbrs = A.indices[A.indptr[lam]:A.indptr[lam + 1]]
+        nbrs = nbrs[nbrs != lam]
+        assert np.all(iperm[nbrs] < iperm[lam])
+
+
+def test_nested_dissection_reduces_fill():
+    beam = G.HexBeam(20, 6, 6, 1.0, 0.2)
+    vals, b = beam.system(np.zeros((beam.nn, 3)), np.zeros((beam.nn, 3)), beam.gravity_loads())
+    nd = product_symbolic(beam.rowOffs, beam.colIdxs, vals, ordering=1)["info"]
+    nat = product_symbolic(beam.rowOffs, beam.colIdxs, vals, ordering=0)["info"]
+    assert nd["factor_flops"] < 0.5 * nat["factor_flops"]
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "kkt_mi355x.h")).read()
+    declared = set(re.findall(r"\b(kkt_[a-z0-9_]+)\s*\(", hdr))
+    L = _lib.lib()
+    missing = [s for s in sorted(declared) if not hasattr(L, s)]
+    assert not missing, missing
+    bound = {name for name, _, _ in _lib.SIGNATURES}
+    assert declared == bound
+    assert L.kkt_version().decode().startswith("kkt_mi355x")
+
+
+def test_abi_argument_and_state_errors_without_gpu():
+    L = _lib.lib()
+    h = L.kkt_create()
+    assert h
+    try:
+        assert L.kkt_analyze(h) == -8  # STATE: analyze before setPattern
+        assert b"setPattern" in L.kkt_last_error(h)
+        ro = np.array([0, 1, 2], np.int32)
+        ci = np.array([0, 3], np.int32)  # column out of range
+        rc = L.kkt_set_pattern(h, 2, 2, _lib.ptr(ro, _lib.ctypes.c_int), _lib.ptr(ci, _lib.ctypes.c_int), 1)
+        assert rc == -4
+        ci = np.array([0, 1], np.int32)
+        assert L.kkt_set_pattern(h, 2, 2, _lib.ptr(ro, _lib.ctypes.c_int),
+                                 _lib.ptr(ci, _lib.ctypes.c_int), 0) == -9  # GENERAL rejected
+        assert L.kkt_set_pattern(h, 2, 2, _lib.ptr(ro, _lib.ctypes.c_int),
+                                 _lib.ptr(ci, _lib.ctypes.c_int), 1) == 0
+        assert L.kkt_last_error(h) is None
+        assert L.kkt_factor(h, _lib.ptr(np.ones(2), _lib.ctypes.c_double)) == -8
+        assert L.kkt_set_precision(h, 7) == -14
+        assert L.kkt_set_refine_tol(h, -1.0, 3) == -14
+        assert L.kkt_set_ir_steps(h, -1) == -14
+    finally:
+        L.kkt_destroy(h)
+    L.kkt_destroy(None)
+
+
+def test_direct_solver_mirror_state_machine():
+    s = MI355XSolver()
+    from artisynth_core_amd.solvers import ImproperStateException
+    with pytest.raises(ImproperStateException):
+        s.factor(np.ones(3))
+    with pytest.raises(ImproperStateException):
+        s.solve(np.zeros(3), np.ones(3))
+    assert s.iterativeSolve(np.ones(3), np.zeros(3), np.ones(3), 10) == -1
+    assert s.hasAutoIterativeSolving()
+    s.dispose()
+    s.dispose()
+    with pytest.raises(ImproperStateException):
+        s.setIterativeRefinementSteps(1)
+
+
+def test_kkt_csr_layout_matches_kktsolver():
+    # KKTSolver.java:171-454: rows < sizeM hold M's upper entries then G^T's entries
+    # (column sizeM+j); row sizeM+i holds only -Rg[i]
+    M = sp.csr_matrix(np.array([[4.0, 1, 0], [1, 3, 2], [0, 2, 5]]))
+    GT = sp.csr_matrix(np.array([[1.0, 0], [0, 1], [1, 1]]))
+    Rg = np.array([0.1, 0.2])
+    ro, ci, va = kkt_upper_csr(M, GT, Rg)
+    assert list(ro) == [0, 3, 6, 9, 10, 11]
+    assert list(ci) == [0, 1, 3, 1, 2, 4, 2, 3, 4, 3, 4]
+    assert list(va) == [4, 1, 1, 3, 2, 1, 5, 1, 1, -0.1, -0.2]
+    ro, ci, va = kkt_upper_csr(M, GT, None)
+    assert va[-2:].tolist() == [0.0, 0.0]
+
+
+def test_estimate_optimal_count():
+    # KKTSolver.java:889-922
+    k = KKTSolver.__new__(KKTSolver)
+    k.directTimeMsec, k.directCnt = 100.0, 1
+    k.firstIterativeTimeMsec, k.iterativeTimeMsec, k.iterativeCnt = 10.0, 10.0, 1
+    assert k.estimateOptimalCount() == 1 + 19  # dt <= 0
+    k.iterativeTimeMsec, k.iterativeCnt = 10.0 + 14.0, 2  # it0=10, avg 12 -> dt = 2
+    assert k.estimateOptimalCount() == int(np.ceil(np.sqrt(2 * (100 - 10 + 2) / 2)))
+    k.iterativeTimeMsec, k.iterativeCnt = 300.0, 2
+    assert k.estimateOptimalCount() == 3
