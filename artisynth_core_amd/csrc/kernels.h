@@ -45,19 +45,28 @@ template <typename T>
 void launch_panel_update(const int4* items, int nitems, int p, const DevSym& d, T* F,
                          const T* W, hipStream_t st);
 
-// solves (vectors in the front precision T)
 template <typename T>
-void launch_fwd_level(const int* items, int nitems, const DevSym& d, const T* F, const T* bp,
-                      T* y, T* z, T* u, hipStream_t st);
+void launch_panel_xrow(const int2* items, int nitems, int p, const DevSym& d, T* F, hipStream_t st);
+
+// level-scheduled solves (vectors in the front precision T)
+template <typename T>
+void launch_fwd_gather(const int* items, int nitems, const DevSym& d, const T* bp, T* fp, T* u,
+                       hipStream_t st);
+template <typename T>
+void launch_fwd_xgemv(const int2* items, int nitems, const DevSym& d, const T* F, const T* fp, T* y,
+                      hipStream_t st);
 template <typename T>
 void launch_fwd_gemv(const int2* items, int nitems, const DevSym& d, const T* F, const T* y,
                      T* u, hipStream_t st);
 template <typename T>
-void launch_bwd_gemv(const int2* items, int nitems, const DevSym& d, const T* F, const T* xp,
-                     T* t, hipStream_t st);
+void launch_bwd_vgemv(const int2* items, int nitems, const DevSym& d, const T* F, const T* y,
+                      const T* xp, T* v, hipStream_t st);
 template <typename T>
-void launch_bwd_level(const int* items, int nitems, const DevSym& d, const T* F, const T* z,
-                      const T* t, T* xp, hipStream_t st);
+void launch_bwd_xpart(const int2* items, int nitems, const DevSym& d, const T* F, const T* v,
+                      const long long* poff, T* part, hipStream_t st);
+template <typename T>
+void launch_bwd_finish(const int* items, int nitems, const DevSym& d, const T* v,
+                       const long long* poff, const T* part, T* xp, hipStream_t st);
 
 // permutation / precision conversion: out[j] = (TO) in[perm[j]]  (gather)
 template <typename TI, typename TO>

@@ -106,7 +106,8 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
                                                    T* __restrict__ F, double pert,
                                                    const double* __restrict__ amax,
                                                    FactorCounters* __restrict__ ctr) {
-  __shared__ double A[64 * 65];  // A[i*65+j], lower part (i >= j)
+  // one wave per front; the 64x64 block lives in LDS as A[i*65+j] (lower part used)
+  __shared__ double A[64 * 65];
   __shared__ int lp[64];
   __shared__ signed char pt[64];
   const int s = items[blockIdx.x];
@@ -114,8 +115,12 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
   const int j0 = p * NB, w = min(NB, k - j0);
   T* Fs = F + d.sn_foff[s];
   const int t = threadIdx.x;
-  for (int j = 0; j < w; j++)
-    if (t >= j && t < w) A[t * 65 + j] = (double)Fs[(size_t)(j0 + j) * m + j0 + t];
+#pragma unroll
+  for (int j = 0; j < 64; j++) {
+    double v = 0.0;
+    if (j < w && t >= j && t < w) v = (double)Fs[(size_t)(j0 + j) * m + j0 + t];
+    A[t * 65 + j] = v;
+  }
   lp[t] = t;
   pt[t] = 1;
   __syncthreads();
@@ -146,8 +151,8 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
       else { kp = imax; kstep = 2; }
     }
     const int kk = kc + kstep - 1;
-    __syncthreads();
     if (kp != kk) {
+      __syncthreads();
       const int a = kk, b = kp;
       if (t == 0) {
         double tmp = A[a * 65 + a]; A[a * 65 + a] = A[b * 65 + b]; A[b * 65 + b] = tmp;
@@ -160,53 +165,60 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
         const size_t o = (size_t)c * m;
         T tmp = Fs[o + j0 + a]; Fs[o + j0 + a] = Fs[o + j0 + b]; Fs[o + j0 + b] = tmp;
       }
-      __syncthreads();
     }
+    __syncthreads();
     if (kstep == 1) {
       double dd = A[kc * 65 + kc];
       const bool pz = perturb || fabs(dd) < tau;
       if (pz) dd = (dd >= 0.0) ? tau : -tau;
+      if (pz) npert++;
+      if (dd > 0) npos++; else nneg++;
+      const double dinv = 1.0 / dd;
+      // column kc into registers first, so the row updates below pipeline
+      double col[64];
+#pragma unroll
+      for (int j = 0; j < 64; j++) col[j] = A[j * 65 + kc];
+      const double aik = A[(t & 63) * 65 + kc];
       __syncthreads();
+      if (t > kc && t < w) {
+#pragma unroll
+        for (int j = 0; j < 64; j++)
+          if (j > kc && j <= t) A[t * 65 + j] -= aik * (col[j] * dinv);
+        A[t * 65 + kc] = aik * dinv;
+      }
       if (t == 0) {
         A[kc * 65 + kc] = dd;
         pt[kc] = 1;
-        if (pz) npert++;
-        if (dd > 0) npos++; else nneg++;
       }
-      const double dinv = 1.0 / dd;
-      if (t > kc && t < w) {
-        const double aik = A[t * 65 + kc];
-        for (int j = kc + 1; j <= t; j++) A[t * 65 + j] -= aik * (A[j * 65 + kc] * dinv);
-      }
-      __syncthreads();
-      if (t > kc && t < w) A[t * 65 + kc] *= dinv;
       __syncthreads();
     } else {
       const int g1 = kc, g2 = kc + 1;
       const double d11 = A[g1 * 65 + g1], d21 = A[g2 * 65 + g1], d22 = A[g2 * 65 + g2];
       const double det = d11 * d22 - d21 * d21;
       const double i11 = d22 / det, i21 = -d21 / det, i22 = d11 / det;
-      if (t == 0) {
-        if (det < 0) { npos++; nneg++; }
-        else if (d11 + d22 > 0) npos += 2;
-        else nneg += 2;
-        n2++;
-        pt[g1] = 2;
-        pt[g2] = 0;
-      }
-      if (t > g2 && t < w) {
-        const double a1 = A[t * 65 + g1], a2 = A[t * 65 + g2];
-        for (int j = kc + 2; j <= t; j++) {
-          const double b1 = A[j * 65 + g1], b2 = A[j * 65 + g2];
-          const double l1 = b1 * i11 + b2 * i21, l2 = b1 * i21 + b2 * i22;
-          A[t * 65 + j] -= a1 * l1 + a2 * l2;
-        }
-      }
+      if (det < 0) { npos++; nneg++; }
+      else if (d11 + d22 > 0) npos += 2;
+      else nneg += 2;
+      n2++;
+      double c1[64], c2[64];
+#pragma unroll
+      for (int j = 0; j < 64; j++) { c1[j] = A[j * 65 + g1]; c2[j] = A[j * 65 + g2]; }
+      const double a1 = A[(t & 63) * 65 + g1], a2 = A[(t & 63) * 65 + g2];
       __syncthreads();
       if (t > g2 && t < w) {
-        const double a1 = A[t * 65 + g1], a2 = A[t * 65 + g2];
+#pragma unroll
+        for (int j = 0; j < 64; j++) {
+          if (j > g2 && j <= t) {
+            const double l1 = c1[j] * i11 + c2[j] * i21, l2 = c1[j] * i21 + c2[j] * i22;
+            A[t * 65 + j] -= a1 * l1 + a2 * l2;
+          }
+        }
         A[t * 65 + g1] = a1 * i11 + a2 * i21;
         A[t * 65 + g2] = a1 * i21 + a2 * i22;
+      }
+      if (t == 0) {
+        pt[g1] = 2;
+        pt[g2] = 0;
       }
       __syncthreads();
     }
@@ -214,22 +226,33 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
   }
   __syncthreads();
   // inverse of the unit lower L11 (entries (q+1,q) of 2x2 pivots hold D, not L):
-  // lane c forms column c of L11^-1 into the upper triangle, A[c][r] = Linv[r][c]
-  if (t < w) {
-    const int c = t;
-    for (int r = c + 1; r < w; r++) {
-      double acc = (r == c + 1 && pt[c] == 2) ? 0.0 : -A[r * 65 + c];
-      for (int l = c + 1; l < r; l++) {
-        const double lrl = (l == r - 1 && pt[l] == 2) ? 0.0 : A[r * 65 + l];
-        acc -= lrl * A[c * 65 + l];
-      }
-      A[c * 65 + r] = acc;
+  // lane c forms column c of L11^-1 in registers, xc[r] = Linv[r][c]
+  const unsigned long long two = __ballot(t < w && pt[t] == 2);
+  double xc[64];
+#pragma unroll
+  for (int l = 0; l < 64; l++) xc[l] = (l == t) ? 1.0 : 0.0;
+#pragma unroll
+  for (int r = 1; r < 64; r++) {
+    double acc = 0.0;
+#pragma unroll
+    for (int l = 0; l < r; l++) {
+      double lrl = A[r * 65 + l];
+      if (l == r - 1 && ((two >> l) & 1ULL)) lrl = 0.0;
+      acc -= lrl * xc[l];
+    }
+    if (r > t) xc[r] = acc;
+  }
+  // lower: L and D; upper (row t, column j0+r > j0+t): Linv[r][t]
+#pragma unroll
+  for (int j = 0; j < 64; j++) {
+    if (j < w && t < w) {
+      const double v = (t >= j) ? A[t * 65 + j] : 0.0;
+      if (t >= j) Fs[(size_t)(j0 + j) * m + j0 + t] = (T)v;
     }
   }
-  __syncthreads();
-  // lower: L and D; upper: L11^-1 (front positions above the diagonal are unused)
-  for (int j = 0; j < w; j++)
-    if (t < w) Fs[(size_t)(j0 + j) * m + j0 + t] = (T)A[t * 65 + j];
+#pragma unroll
+  for (int r = 0; r < 64; r++)
+    if (r < w && r > t) Fs[(size_t)(j0 + r) * m + j0 + t] = (T)xc[r];
   if (t < w) {
     d.lperm[first + j0 + t] = j0 + lp[t];
     d.piv[first + j0 + t] = pt[t];
@@ -419,21 +442,115 @@ __global__ __launch_bounds__(256) void k_panel_update(const int4* __restrict__ i
 }
 
 // ------------------------------------------------------------------------
-// forward solve of one level: assemble update vectors, L11 solve, D^-1
+// Inverse of a supernode's whole unit-lower pivot block, X = L11^-1, built one
+// 64-row block per panel step: X(P,CB) = -Linv_P * sum_{Q=CB}^{P-1} L(P,Q) X(Q,CB).
+// X(i,l), l < i, is kept above the diagonal at front position (row l, col i);
+// the 64x64 diagonal blocks of X are the per-panel inverses from k_panel_diag.
+// item = (supernode, column block cb < p)
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_xrow(const int2* __restrict__ items, int p,
+                                                    DevSym d, T* __restrict__ F) {
+  constexpr int LD = 65;
+  __shared__ T As[64 * LD];  // A operand, As[r*LD + kk]
+  __shared__ T Bn[64 * LD];  // B operand stored n-major, Bn[n*LD + kk]
+  typedef typename Mfma<T>::v4 v4;
+  const int2 it = items[blockIdx.x];
+  const int s = it.x, cb = it.y;
+  const int k = d.sn_k[s], m = d.sn_m[s];
+  const int j0 = p * NB, w = min(NB, k - j0), c0 = cb * 64;
+  T* Fs = F + d.sn_foff[s];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  v4 acc[2][2];
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++) acc[bi][bj] = (v4)((T)0);
+  for (int q = cb; q < p; q++) {
+    const int q0 = q * 64;
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+      const int cc = e >> 6, rr = e & 63;
+      // L(j0+rr, q0+cc): column q0+cc, rows contiguous
+      As[rr * LD + cc] = (rr < w) ? Fs[(size_t)(q0 + cc) * m + j0 + rr] : (T)0;
+      // X(q0+cc, c0+rr) as Bn[n=rr][kk=cc]
+      T xv;
+      if (q == cb) xv = (cc > rr) ? Fs[(size_t)(q0 + cc) * m + c0 + rr] : (cc == rr ? (T)1 : (T)0);
+      else xv = Fs[(size_t)(q0 + cc) * m + c0 + rr];
+      Bn[rr * LD + cc] = xv;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < 64; kk += 4) {
+      const T a0 = As[(wr + lr) * LD + kk + lk], a1 = As[(wr + 16 + lr) * LD + kk + lk];
+      const T b0 = Bn[(wc + lr) * LD + kk + lk], b1 = Bn[(wc + 16 + lr) * LD + kk + lk];
+      acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
+      acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
+      acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+      acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+    }
+    __syncthreads();
+  }
+  // R (64 x 64, rows of P, columns of CB) -> Bn[n][kk=r]; Linv_P -> As
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        Bn[(wc + bj * 16 + lr) * LD + wr + bi * 16 + Mfma<T>::row(lk, v)] = acc[bi][bj][v];
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int rr = e >> 6, cc = e & 63;  // Linv_P(rr, cc) at front (row j0+cc, col j0+rr)
+    T v = (T)0;
+    if (rr < w && cc < w) v = (cc < rr) ? Fs[(size_t)(j0 + rr) * m + j0 + cc] : (cc == rr ? (T)1 : (T)0);
+    As[rr * LD + cc] = -v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++) acc[bi][bj] = (v4)((T)0);
+#pragma unroll 4
+  for (int kk = 0; kk < 64; kk += 4) {
+    const T a0 = As[(wr + lr) * LD + kk + lk], a1 = As[(wr + 16 + lr) * LD + kk + lk];
+    const T b0 = Bn[(wc + lr) * LD + kk + lk], b1 = Bn[(wc + 16 + lr) * LD + kk + lk];
+    acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
+    acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
+    acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+    acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+  }
+  // X(j0+r, c0+n) at front (row c0+n, col j0+r)
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = wr + bi * 16 + Mfma<T>::row(lk, v), n = wc + bj * 16 + lr;
+        if (r < w) Fs[(size_t)(j0 + r) * m + c0 + n] = acc[bi][bj][v];
+      }
+}
+
+// ------------------------------------------------------------------------
+// Level-scheduled triangular solves.  Per level, forward:
+//   fwd_gather : f = b + children's update vectors; fp = f permuted (pivots)
+//   fwd_xgemv  : y = X fp          (X = L11^-1, wave per row)
+//   fwd_gemv   : u = f_cb - L21 y  (64-row blocks)
+// backward (levels top-down):
+//   bwd_vgemv  : v = D^-1 y - L21^T x_rows   (wave per column)
+//   bwd_xpart  : partial sums of X^T v over 64-column chunks
+//   bwd_finish : x = v + sum of partials, scattered through the pivot permutation
 // ------------------------------------------------------------------------
 constexpr int KMAX = 1024;  // widest supernode (the symbolic analysis splits wider ones)
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_level(const int* __restrict__ items, DevSym d,
-                                                   const T* __restrict__ F,
-                                                   const T* __restrict__ bp, T* __restrict__ y,
-                                                   T* __restrict__ z, T* __restrict__ u) {
+__global__ __launch_bounds__(256) void k_fwd_gather(const int* __restrict__ items, DevSym d,
+                                                    const T* __restrict__ bp, T* __restrict__ fp,
+                                                    T* __restrict__ u) {
   __shared__ T g[KMAX];
-  __shared__ T yv[KMAX];
-  __shared__ T Lb[64 * 65];  // Lb[c*65 + r] = Linv[r][c] of the current 64-column block
   const int s = items[blockIdx.x];
   const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s], r = m - k;
-  const T* Fs = F + d.sn_foff[s];
   T* us = u + d.sn_uoff[s];
   const int tid = threadIdx.x;
   for (int t = tid; t < k; t += 256) g[t] = bp[first + t];
@@ -451,51 +568,26 @@ __global__ __launch_bounds__(256) void k_fwd_level(const int* __restrict__ items
     }
     __syncthreads();
   }
-  for (int t = tid; t < k; t += 256) yv[t] = g[d.lperm[first + t]];
-  __syncthreads();
-  for (int c0 = 0; c0 < k; c0 += 64) {
-    const int c1 = min(k, c0 + 64), bw = c1 - c0;
-    // Lb[c*65 + r] = Linv[r][c] of this 64-column block (stored above the diagonal)
-    for (int e = tid; e < 64 * 64; e += 256) {
-      const int cc = e >> 6, rr = e & 63;
-      T v = (T)0;
-      if (cc < bw && rr < bw) v = (rr > cc) ? Fs[(size_t)(c0 + rr) * m + c0 + cc] : (rr == cc ? (T)1 : (T)0);
-      Lb[cc * 65 + rr] = v;
-    }
-    __syncthreads();
-    T ynew = (T)0;
-    if (tid < bw) {
-      for (int c = 0; c <= tid; c++) ynew += Lb[c * 65 + tid] * yv[c0 + c];
-    }
-    __syncthreads();
-    if (tid < bw) yv[c0 + tid] = ynew;
-    __syncthreads();
-    for (int i = c1 + tid; i < k; i += 256) {
-      T acc = yv[i];
-      for (int q = 0; q < bw; q++) acc -= Fs[(size_t)(c0 + q) * m + i] * yv[c0 + q];
-      yv[i] = acc;
-    }
-    __syncthreads();
-  }
-  for (int t = tid; t < k; t += 256) {
-    const int pq = d.piv[first + t];
-    T val;
-    if (pq == 1) {
-      val = yv[t] / Fs[(size_t)t * m + t];
-    } else if (pq == 2) {
-      const T d11 = Fs[(size_t)t * m + t], d21 = Fs[(size_t)t * m + t + 1],
-              d22 = Fs[(size_t)(t + 1) * m + t + 1];
-      const T det = d11 * d22 - d21 * d21;
-      val = (yv[t] * d22 - yv[t + 1] * d21) / det;
-    } else {
-      const T d11 = Fs[(size_t)(t - 1) * m + t - 1], d21 = Fs[(size_t)(t - 1) * m + t],
-              d22 = Fs[(size_t)t * m + t];
-      const T det = d11 * d22 - d21 * d21;
-      val = (yv[t] * d11 - yv[t - 1] * d21) / det;
-    }
-    z[first + t] = val;
-    y[first + t] = yv[t];
-  }
+  for (int t = tid; t < k; t += 256) fp[first + t] = g[d.lperm[first + t]];
+}
+
+// y_t = fp_t + sum_{l<t} X(t,l) fp_l ; X(t,l) at front (row l, col t); item = (s, 4 rows)
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_xgemv(const int2* __restrict__ items, DevSym d,
+                                                   const T* __restrict__ F,
+                                                   const T* __restrict__ fp, T* __restrict__ y) {
+  const int2 it = items[blockIdx.x];
+  const int s = it.x;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int t = it.y * 4 + wv;
+  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s];
+  if (t >= k) return;
+  const T* col = F + d.sn_foff[s] + (size_t)t * m;
+  const T* f = fp + first;
+  T acc = (T)0;
+  for (int l = lane; l < t; l += 64) acc += col[l] * f[l];
+  acc = wave_sum(acc);
+  if (lane == 0) y[first + t] = f[t] + acc;
 }
 
 // u_s[i] -= sum_t L(k+i, t) y_t ; item = (s, 64-row block); 4 waves split columns
@@ -513,12 +605,17 @@ __global__ __launch_bounds__(256) void k_fwd_gemv(const int2* __restrict__ items
   __syncthreads();
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = rb * 64 + lane;
-  T acc = (T)0;
+  T acc0 = (T)0, acc1 = (T)0;
   if (i < r) {
     const T* col = Fs + k + i;
-    for (int q = wv; q < k; q += 4) acc += col[(size_t)q * m] * ys[q];
+    int q = wv;
+    for (; q + 4 < k; q += 8) {
+      acc0 += col[(size_t)q * m] * ys[q];
+      acc1 += col[(size_t)(q + 4) * m] * ys[q + 4];
+    }
+    if (q < k) acc0 += col[(size_t)q * m] * ys[q];
   }
-  part[wv][lane] = acc;
+  part[wv][lane] = acc0 + acc1;
   __syncthreads();
   if (wv == 0 && i < r) {
     T* us = u + d.sn_uoff[s];
@@ -526,69 +623,82 @@ __global__ __launch_bounds__(256) void k_fwd_gemv(const int2* __restrict__ items
   }
 }
 
-// t_s[q] = sum_i L(k+i, q) xp[rows_i] ; item = (s, block of 4 columns), wave per column
+// v_q = (D^-1 y)_q - sum_i L(k+i, q) xp[rows_i] ; item = (s, 4 columns), wave per column
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_gemv(const int2* __restrict__ items, DevSym d,
-                                                  const T* __restrict__ F,
-                                                  const T* __restrict__ xp, T* __restrict__ tb) {
+__global__ __launch_bounds__(256) void k_bwd_vgemv(const int2* __restrict__ items, DevSym d,
+                                                   const T* __restrict__ F,
+                                                   const T* __restrict__ y,
+                                                   const T* __restrict__ xp, T* __restrict__ v) {
   const int2 it = items[blockIdx.x];
-  const int s = it.x, cb = it.y;
-  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s], r = m - k;
+  const int s = it.x;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int q = cb * 4 + wv;
+  const int q = it.y * 4 + wv;
+  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s], r = m - k;
   if (q >= k) return;
-  const T* col = F + d.sn_foff[s] + (size_t)q * m + k;
+  const T* Fs = F + d.sn_foff[s];
+  const T* col = Fs + (size_t)q * m + k;
   const int* rows = d.rows + d.sn_roff[s];
   T acc = (T)0;
   for (int i = lane; i < r; i += 64) acc += col[i] * xp[rows[i]];
   acc = wave_sum(acc);
-  if (lane == 0) tb[first + q] = acc;
+  if (lane == 0) {
+    const int pq = d.piv[first + q];
+    T z;
+    if (pq == 1) {
+      z = y[first + q] / Fs[(size_t)q * m + q];
+    } else if (pq == 2) {
+      const T d11 = Fs[(size_t)q * m + q], d21 = Fs[(size_t)q * m + q + 1],
+              d22 = Fs[(size_t)(q + 1) * m + q + 1];
+      const T det = d11 * d22 - d21 * d21;
+      z = (y[first + q] * d22 - y[first + q + 1] * d21) / det;
+    } else {
+      const T d11 = Fs[(size_t)(q - 1) * m + q - 1], d21 = Fs[(size_t)(q - 1) * m + q],
+              d22 = Fs[(size_t)q * m + q];
+      const T det = d11 * d22 - d21 * d21;
+      z = (y[first + q] * d11 - y[first + q - 1] * d21) / det;
+    }
+    v[first + q] = z - acc;
+  }
+}
+
+// partial[c][l] = sum_{i in chunk c, i > l} X(i,l) v_i ; item = (s, chunk of 64 columns)
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_xpart(const int2* __restrict__ items, DevSym d,
+                                                   const T* __restrict__ F,
+                                                   const T* __restrict__ v,
+                                                   const long long* __restrict__ poff,
+                                                   T* __restrict__ part) {
+  __shared__ T vs[64];
+  const int2 it = items[blockIdx.x];
+  const int s = it.x, c = it.y;
+  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s];
+  const int i0 = c * 64, i1 = min(k, i0 + 64);
+  const T* Fs = F + d.sn_foff[s];
+  if (threadIdx.x < 64) vs[threadIdx.x] = (i0 + (int)threadIdx.x < k) ? v[first + i0 + threadIdx.x] : (T)0;
+  __syncthreads();
+  T* pc = part + poff[s] + (size_t)c * k;
+  for (int l = threadIdx.x; l < i1; l += 256) {
+    T acc = (T)0;
+    for (int i = max(i0, l + 1); i < i1; i++) acc += Fs[(size_t)i * m + l] * vs[i - i0];
+    pc[l] = acc;
+  }
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_level(const int* __restrict__ items, DevSym d,
-                                                   const T* __restrict__ F,
-                                                   const T* __restrict__ z,
-                                                   const T* __restrict__ tb,
-                                                   T* __restrict__ xp) {
-  __shared__ T vv[KMAX];
-  __shared__ T Lb[64 * 65];  // Lb[c*65 + r] = Linv[r][c] of the current 64-column block
+__global__ __launch_bounds__(256) void k_bwd_finish(const int* __restrict__ items, DevSym d,
+                                                    const T* __restrict__ v,
+                                                    const long long* __restrict__ poff,
+                                                    const T* __restrict__ part,
+                                                    T* __restrict__ xp) {
   const int s = items[blockIdx.x];
-  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s], r = m - k;
-  const T* Fs = F + d.sn_foff[s];
-  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-  for (int t = tid; t < k; t += 256) vv[t] = z[first + t] - (r > 0 ? tb[first + t] : (T)0);
-  __syncthreads();
-  int c1 = k;
-  while (c1 > 0) {
-    const int c0 = ((c1 - 1) / 64) * 64, bw = c1 - c0;
-    // columns of this block against the already solved part of the pivot block
-    if (c1 < k) {
-      for (int q = c0 + wv; q < c1; q += 4) {
-        const T* col = Fs + (size_t)q * m;
-        T acc = (T)0;
-        for (int i = c1 + lane; i < k; i += 64) acc += col[i] * vv[i];
-        acc = wave_sum(acc);
-        if (lane == 0) vv[q] -= acc;
-      }
-    }
-    for (int e = tid; e < 64 * 64; e += 256) {
-      const int cc = e >> 6, rr = e & 63;
-      T v = (T)0;
-      if (cc < bw && rr < bw) v = (rr > cc) ? Fs[(size_t)(c0 + rr) * m + c0 + cc] : (rr == cc ? (T)1 : (T)0);
-      Lb[cc * 65 + rr] = v;
-    }
-    __syncthreads();
-    T xnew = (T)0;
-    if (tid < bw) {
-      for (int r = tid; r < bw; r++) xnew += Lb[tid * 65 + r] * vv[c0 + r];
-    }
-    __syncthreads();
-    if (tid < bw) vv[c0 + tid] = xnew;
-    __syncthreads();
-    c1 = c0;
+  const int k = d.sn_k[s], first = d.sn_first[s];
+  const int nc = (k + 63) / 64;
+  const T* ps = part + poff[s];
+  for (int l = threadIdx.x; l < k; l += 256) {
+    T x = v[first + l];
+    for (int c = l / 64; c < nc; c++) x += ps[(size_t)c * k + l];
+    xp[first + d.lperm[first + l]] = x;
   }
-  for (int t = tid; t < k; t += 256) xp[first + d.lperm[first + t]] = vv[t];
 }
 
 // ------------------------------------------------------------------------
@@ -711,28 +821,40 @@ void launch_panel_update(const int4* items, int nitems, int p, const DevSym& d, 
     hipLaunchKernelGGL(k_panel_update<T>, dim3(nitems), dim3(256), 0, st, items, p, d, F, W);
 }
 template <typename T>
-void launch_fwd_level(const int* items, int nitems, const DevSym& d, const T* F, const T* bp,
-                      T* y, T* z, T* u, hipStream_t st) {
-  if (nitems > 0)
-    hipLaunchKernelGGL(k_fwd_level<T>, dim3(nitems), dim3(256), 0, st, items, d, F, bp, y, z, u);
+void launch_panel_xrow(const int2* items, int nitems, int p, const DevSym& d, T* F, hipStream_t st) {
+  if (nitems > 0) hipLaunchKernelGGL(k_panel_xrow<T>, dim3(nitems), dim3(256), 0, st, items, p, d, F);
+}
+template <typename T>
+void launch_fwd_gather(const int* items, int nitems, const DevSym& d, const T* bp, T* fp, T* u,
+                       hipStream_t st) {
+  if (nitems > 0) hipLaunchKernelGGL(k_fwd_gather<T>, dim3(nitems), dim3(256), 0, st, items, d, bp, fp, u);
+}
+template <typename T>
+void launch_fwd_xgemv(const int2* items, int nitems, const DevSym& d, const T* F, const T* fp, T* y,
+                      hipStream_t st) {
+  if (nitems > 0) hipLaunchKernelGGL(k_fwd_xgemv<T>, dim3(nitems), dim3(256), 0, st, items, d, F, fp, y);
 }
 template <typename T>
 void launch_fwd_gemv(const int2* items, int nitems, const DevSym& d, const T* F, const T* y,
                      T* u, hipStream_t st) {
-  if (nitems > 0)
-    hipLaunchKernelGGL(k_fwd_gemv<T>, dim3(nitems), dim3(256), 0, st, items, d, F, y, u);
+  if (nitems > 0) hipLaunchKernelGGL(k_fwd_gemv<T>, dim3(nitems), dim3(256), 0, st, items, d, F, y, u);
 }
 template <typename T>
-void launch_bwd_gemv(const int2* items, int nitems, const DevSym& d, const T* F, const T* xp,
-                     T* t, hipStream_t st) {
-  if (nitems > 0)
-    hipLaunchKernelGGL(k_bwd_gemv<T>, dim3(nitems), dim3(256), 0, st, items, d, F, xp, t);
+void launch_bwd_vgemv(const int2* items, int nitems, const DevSym& d, const T* F, const T* y,
+                      const T* xp, T* v, hipStream_t st) {
+  if (nitems > 0) hipLaunchKernelGGL(k_bwd_vgemv<T>, dim3(nitems), dim3(256), 0, st, items, d, F, y, xp, v);
 }
 template <typename T>
-void launch_bwd_level(const int* items, int nitems, const DevSym& d, const T* F, const T* z,
-                      const T* t, T* xp, hipStream_t st) {
+void launch_bwd_xpart(const int2* items, int nitems, const DevSym& d, const T* F, const T* v,
+                      const long long* poff, T* part, hipStream_t st) {
   if (nitems > 0)
-    hipLaunchKernelGGL(k_bwd_level<T>, dim3(nitems), dim3(256), 0, st, items, d, F, z, t, xp);
+    hipLaunchKernelGGL(k_bwd_xpart<T>, dim3(nitems), dim3(256), 0, st, items, d, F, v, poff, part);
+}
+template <typename T>
+void launch_bwd_finish(const int* items, int nitems, const DevSym& d, const T* v,
+                       const long long* poff, const T* part, T* xp, hipStream_t st) {
+  if (nitems > 0)
+    hipLaunchKernelGGL(k_bwd_finish<T>, dim3(nitems), dim3(256), 0, st, items, d, v, poff, part, xp);
 }
 template <typename TI, typename TO>
 void launch_gather_perm(int n, const int* perm, const TI* in, TO* out, hipStream_t st) {
@@ -777,14 +899,19 @@ void launch_axpbypcz(int n, double a, const double* x, double b, const double* y
   template void launch_panel_trsm<T>(const int2*, int, int, const DevSym&, T*, T*, hipStream_t); \
   template void launch_panel_update<T>(const int4*, int, int, const DevSym&, T*, const T*,       \
                                        hipStream_t);                                             \
-  template void launch_fwd_level<T>(const int*, int, const DevSym&, const T*, const T*, T*, T*,  \
-                                    T*, hipStream_t);                                            \
+  template void launch_panel_xrow<T>(const int2*, int, int, const DevSym&, T*, hipStream_t);     \
+  template void launch_fwd_gather<T>(const int*, int, const DevSym&, const T*, T*, T*,           \
+                                     hipStream_t);                                               \
+  template void launch_fwd_xgemv<T>(const int2*, int, const DevSym&, const T*, const T*, T*,     \
+                                    hipStream_t);                                                \
   template void launch_fwd_gemv<T>(const int2*, int, const DevSym&, const T*, const T*, T*,      \
                                    hipStream_t);                                                 \
-  template void launch_bwd_gemv<T>(const int2*, int, const DevSym&, const T*, const T*, T*,      \
-                                   hipStream_t);                                                 \
-  template void launch_bwd_level<T>(const int*, int, const DevSym&, const T*, const T*,          \
-                                    const T*, T*, hipStream_t);
+  template void launch_bwd_vgemv<T>(const int2*, int, const DevSym&, const T*, const T*,         \
+                                    const T*, T*, hipStream_t);                                  \
+  template void launch_bwd_xpart<T>(const int2*, int, const DevSym&, const T*, const T*,         \
+                                    const long long*, T*, hipStream_t);                          \
+  template void launch_bwd_finish<T>(const int*, int, const DevSym&, const T*,                   \
+                                     const long long*, const T*, T*, hipStream_t);
 KKT_INST(double)
 KKT_INST(float)
 template void launch_gather_perm<double, double>(int, const int*, const double*, double*, hipStream_t);

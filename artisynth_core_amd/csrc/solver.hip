@@ -36,7 +36,7 @@ struct Launch {
   int cnt = 0;
 };
 struct Step {
-  Launch diag, trsm, upd;
+  Launch diag, xrow, trsm, upd;
   double upd_flops = 0;
 };
 
@@ -83,9 +83,11 @@ struct kkt_solver {
   // plan
   int4 *d_asm = nullptr, *d_upd = nullptr;
   int *d_diag = nullptr, *d_lvlsn = nullptr, *d_aidx = nullptr;
-  int2 *d_trsm = nullptr, *d_fg = nullptr, *d_bg = nullptr;
-  long long* d_apos = nullptr;
-  std::vector<Launch> asm_l, scat_l, sn_l, fg_l, bg_l;
+  int2 *d_trsm = nullptr, *d_fg = nullptr, *d_bg = nullptr, *d_xrow = nullptr, *d_xg = nullptr,
+       *d_xp = nullptr;
+  long long *d_apos = nullptr, *d_poff = nullptr;
+  std::vector<Launch> asm_l, scat_l, sn_l, fg_l, bg_l, xg_l, xp_l;
+  long long part_elems = 0;
   std::vector<std::vector<Step>> steps;
   long long front_elems = 0, w_elems = 0, u_elems = 0;
   // numeric buffers (allocated lazily per precision)
@@ -93,7 +95,7 @@ struct kkt_solver {
   float* d_F32 = nullptr;
   double* d_W64 = nullptr;
   float* d_W32 = nullptr;
-  void* d_svec = nullptr;  // solve scratch: bp, y, z, tb, xp (n each) + u (u_elems), as T
+  void* d_svec = nullptr;  // solve scratch: bp, fp, y, v, xp (n each), u (u_elems), partials, as T
   size_t svec_bytes = 0;
   // FP64 vectors for refinement / BiCGStab / host copies
   double* d_vec = nullptr;  // 12 vectors of n
@@ -390,7 +392,12 @@ struct kkt_solver {
     std::vector<int4> asmv, updv;
     std::vector<int> diagv, lvlsn, aidx;
     std::vector<long long> apos;
-    std::vector<int2> trsmv, fgv, bgv;
+    std::vector<int2> trsmv, fgv, bgv, xrowv, xgv, xpv;
+    std::vector<long long> poff(ns + 1, 0);
+    for (int s = 0; s < ns; s++) poff[s + 1] = poff[s] + (long long)((kk[s] + 63) / 64) * kk[s];
+    part_elems = poff[ns];
+    xg_l.assign(nl, Launch());
+    xp_l.assign(nl, Launch());
     asm_l.assign(nl, Launch());
     scat_l.assign(nl, Launch());
     sn_l.assign(nl, Launch());
@@ -414,20 +421,26 @@ struct kkt_solver {
       sn_l[L].cnt = (int)lv[L].size();
       fg_l[L].off = (long long)fgv.size();
       bg_l[L].off = (long long)bgv.size();
+      xg_l[L].off = (long long)xgv.size();
+      xp_l[L].off = (long long)xpv.size();
       for (int s : lv[L]) {
         int r = mm[s] - kk[s];
-        if (r <= 0) continue;
         for (int rb = 0; rb < (r + 63) / 64; rb++) fgv.push_back(make_int2(s, rb));
         for (int cb = 0; cb < (kk[s] + 3) / 4; cb++) bgv.push_back(make_int2(s, cb));
+        for (int cb = 0; cb < (kk[s] + 3) / 4; cb++) xgv.push_back(make_int2(s, cb));
+        for (int c = 0; c < (kk[s] + 63) / 64; c++) xpv.push_back(make_int2(s, c));
       }
       fg_l[L].cnt = (int)(fgv.size() - fg_l[L].off);
       bg_l[L].cnt = (int)(bgv.size() - bg_l[L].off);
+      xg_l[L].cnt = (int)(xgv.size() - xg_l[L].off);
+      xp_l[L].cnt = (int)(xpv.size() - xp_l[L].off);
       int P = 0;
       for (int s : lv[L]) P = std::max(P, (kk[s] + NB - 1) / NB);
       steps[L].resize(P);
       for (int p = 0; p < P; p++) {
         Step& sp = steps[L][p];
         sp.diag.off = (long long)diagv.size();
+        sp.xrow.off = (long long)xrowv.size();
         sp.trsm.off = (long long)trsmv.size();
         sp.upd.off = (long long)updv.size();
         for (int s : lv[L]) {
@@ -435,6 +448,7 @@ struct kkt_solver {
           if (kk[s] <= j0) continue;
           int w = std::min(NB, kk[s] - j0), r0 = j0 + w, mr = mm[s] - r0;
           diagv.push_back(s);
+          for (int cb = 0; cb < p; cb++) xrowv.push_back(make_int2(s, cb));
           for (int rb = 0; rb < (mr + 63) / 64; rb++) trsmv.push_back(make_int2(s, rb));
           int t = (mr + TILE - 1) / TILE;
           for (int I = 0; I < t; I++)
@@ -442,6 +456,7 @@ struct kkt_solver {
           sp.upd_flops += (double)w * (double)mr * (double)(mr + 1);
         }
         sp.diag.cnt = (int)(diagv.size() - sp.diag.off);
+        sp.xrow.cnt = (int)(xrowv.size() - sp.xrow.off);
         sp.trsm.cnt = (int)(trsmv.size() - sp.trsm.off);
         sp.upd.cnt = (int)(updv.size() - sp.upd.off);
       }
@@ -485,7 +500,9 @@ struct kkt_solver {
         (rc = upload(d_rel, rel)) || (rc = upload(d_rows, S.rows)) || (rc = upload(d_perm, S.perm)) ||
         (rc = upload(d_asm, asmv)) || (rc = upload(d_upd, updv)) || (rc = upload(d_diag, diagv)) ||
         (rc = upload(d_lvlsn, lvlsn)) || (rc = upload(d_aidx, aidx)) || (rc = upload(d_apos, apos)) ||
-        (rc = upload(d_trsm, trsmv)) || (rc = upload(d_fg, fgv)) || (rc = upload(d_bg, bgv)))
+        (rc = upload(d_trsm, trsmv)) || (rc = upload(d_fg, fgv)) || (rc = upload(d_bg, bgv)) ||
+        (rc = upload(d_xrow, xrowv)) || (rc = upload(d_xg, xgv)) || (rc = upload(d_xp, xpv)) ||
+        (rc = upload(d_poff, poff)))
       return rc;
     std::vector<int> lp0(n);
     for (int s = 0; s < ns; s++)
@@ -528,7 +545,7 @@ struct kkt_solver {
           hipMalloc(&d_W32, std::max<long long>(w_elems, 1) * sizeof(float)) != hipSuccess)
         return fail(KKT_ERR_ALLOC, "hipMalloc failed (FP32 fronts)");
     }
-    size_t need = ((size_t)5 * n + (size_t)u_elems + 16) * std::max(esz, sizeof(double));
+    size_t need = ((size_t)5 * n + (size_t)u_elems + (size_t)part_elems + 16) * std::max(esz, sizeof(double));
     if (svec_bytes < need) {
       if (d_svec) (void)hipFree(d_svec);
       if (hipMalloc(&d_svec, need) != hipSuccess) return fail(KKT_ERR_ALLOC, "hipMalloc failed (solve)");
@@ -554,6 +571,7 @@ struct kkt_solver {
         const Step& sp = steps[L][p];
         tbeg(TC_DIAG);
         launch_panel_diag<T>(d_diag + sp.diag.off, sp.diag.cnt, (int)p, ds, F, pert, d_scal + 15, d_ctr, st);
+        launch_panel_xrow<T>(d_xrow + sp.xrow.off, sp.xrow.cnt, (int)p, ds, F, st);
         tend();
         tbeg(TC_TRSM);
         launch_panel_trsm<T>(d_trsm + sp.trsm.off, sp.trsm.cnt, (int)p, ds, F, W, st);
@@ -614,26 +632,28 @@ struct kkt_solver {
   template <typename T>
   void run_tri_solve(const T* F, const double* d_b, double* d_x) {
     T* sv = (T*)d_svec;
-    T *bp = sv, *y = sv + n, *z = sv + 2 * (size_t)n, *tb = sv + 3 * (size_t)n,
-      *xp = sv + 4 * (size_t)n, *u = sv + 5 * (size_t)n;
+    T *bp = sv, *fp = sv + n, *y = sv + 2 * (size_t)n, *v = sv + 3 * (size_t)n,
+      *xp = sv + 4 * (size_t)n, *u = sv + 5 * (size_t)n, *part = u + u_elems;
     tbeg(TC_VEC);
     launch_gather_perm<double, T>(n, d_perm, d_b, bp, st);
     tend();
     const int nl = S.nlevels;
     for (int L = 0; L < nl; L++) {
       tbeg(TC_SLEVEL);
-      launch_fwd_level<T>(d_lvlsn + sn_l[L].off, sn_l[L].cnt, ds, F, bp, y, z, u, st);
+      launch_fwd_gather<T>(d_lvlsn + sn_l[L].off, sn_l[L].cnt, ds, bp, fp, u, st);
       tend();
       tbeg(TC_SGEMV);
+      launch_fwd_xgemv<T>(d_xg + xg_l[L].off, xg_l[L].cnt, ds, F, fp, y, st);
       launch_fwd_gemv<T>(d_fg + fg_l[L].off, fg_l[L].cnt, ds, F, y, u, st);
       tend();
     }
     for (int L = nl - 1; L >= 0; L--) {
       tbeg(TC_SGEMV);
-      launch_bwd_gemv<T>(d_bg + bg_l[L].off, bg_l[L].cnt, ds, F, xp, tb, st);
+      launch_bwd_vgemv<T>(d_bg + bg_l[L].off, bg_l[L].cnt, ds, F, y, xp, v, st);
+      launch_bwd_xpart<T>(d_xp + xp_l[L].off, xp_l[L].cnt, ds, F, v, d_poff, part, st);
       tend();
       tbeg(TC_SLEVEL);
-      launch_bwd_level<T>(d_lvlsn + sn_l[L].off, sn_l[L].cnt, ds, F, z, tb, xp, st);
+      launch_bwd_finish<T>(d_lvlsn + sn_l[L].off, sn_l[L].cnt, ds, v, d_poff, part, xp, st);
       tend();
     }
     tbeg(TC_VEC);

@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--nsys", type=int, default=20, help="systems in the step sequence")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--grid", default="100x16x16")
+    ap.add_argument("--mode", default="bench", choices=["bench", "pmc"],
+                    help="pmc: one FP32 factor + solve and a 256 MiB copy, for counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -92,6 +94,8 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
     nx, ny, nz = (int(v) for v in args.grid.split("x"))
+    if args.mode == "pmc":
+        return pmc_run(nx, ny, nz)
     t0 = time.time()
     beam, V, B = G.large_beam_sequence(nsteps=args.nsys, nx=nx, ny=ny, nz=nz,
                                        width=0.16 * ny / 16)
@@ -201,6 +205,28 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def pmc_run(nx, ny, nz):
+    """Counter-collection workload: one FP32 factor + solve of step 0, then a plain
+    256 MiB device copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters."""
+    import torch
+    from artisynth_core_amd import KKT_PREC_FP32_REFINE
+    from artisynth_core_amd import generators as G
+    from artisynth_core_amd.solvers import Matrix, MI355XSolver
+    beam, V, B = G.large_beam_sequence(nsteps=1, nx=nx, ny=ny, nz=nz, width=0.16 * ny / 16)
+    s = MI355XSolver(precision=KKT_PREC_FP32_REFINE)
+    s.analyzeArrays(V[0], beam.colIdxs, beam.rowOffs, beam.n, Matrix.SYMMETRIC)
+    s.factor(V[0])
+    x = np.zeros(beam.n)
+    s.solve(x, B[0])
+    a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+    b = torch.ones(256 << 20, dtype=torch.uint8, device="cuda")
+    a.copy_(b)
+    torch.cuda.synchronize()
+    print(json.dumps({"mode": "pmc", "copy_bytes": 256 << 20, "n": beam.n,
+                      "info": s.info()}), flush=True)
+    s.dispose()
 
 
 def cpu_baseline(beam, vals, b, x64, x32):
