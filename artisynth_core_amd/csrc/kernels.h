@@ -62,7 +62,7 @@ template <typename T>
 void launch_bwd_vgemv(const int2* items, int nitems, const DevSym& d, const T* F, const T* y,
                       const T* xp, T* v, hipStream_t st);
 template <typename T>
-void launch_bwd_xpart(const int2* items, int nitems, const DevSym& d, const T* F, const T* v,
+void launch_bwd_xpart(const int4* items, int nitems, const DevSym& d, const T* F, const T* v,
                       const long long* poff, T* part, hipStream_t st);
 template <typename T>
 void launch_bwd_finish(const int* items, int nitems, const DevSym& d, const T* v,
@@ -89,5 +89,15 @@ void launch_dot(int n, const double* a, const double* b, double* partial, double
 void launch_axpby(int n, double alpha, const double* x, double beta, double* y, hipStream_t st);
 void launch_axpbypcz(int n, double a, const double* x, double b, const double* y, double c,
                      double* z, hipStream_t st);
+
+// BiCGStab updates with device-resident scalars (see kernels.hip for the slots)
+void launch_bicg_p(int n, const double* sc, const double* r, const double* v, double* p, hipStream_t st);
+void launch_bicg_s(int n, const double* sc, const double* r, const double* v, double* s_, hipStream_t st);
+void launch_bicg_xr(int n, const double* sc, const double* y, const double* z, const double* s_,
+                    const double* t, double* x, double* r, hipStream_t st);
+void launch_bicg_xa(int n, const double* sc, const double* y, double* x, hipStream_t st);
+void launch_bicg_scalars(double* sc, int init, hipStream_t st);
+void launch_fill(long long n, double v, double* x, hipStream_t st);
+void launch_zero_counters(FactorCounters* c, hipStream_t st);
 
 }  // namespace kkt

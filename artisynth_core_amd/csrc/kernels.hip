@@ -106,8 +106,9 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
                                                    T* __restrict__ F, double pert,
                                                    const double* __restrict__ amax,
                                                    FactorCounters* __restrict__ ctr) {
-  // one wave per front; the 64x64 block lives in LDS as A[i*65+j] (lower part used)
+  // one wave per front; the 64x64 block lives in LDS as A[i*65+j] (lower part)
   __shared__ double A[64 * 65];
+  __shared__ double colb[64], colb2[64];  // scaled pivot column(s) of the current step
   __shared__ int lp[64];
   __shared__ signed char pt[64];
   const int s = items[blockIdx.x];
@@ -115,12 +116,8 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
   const int j0 = p * NB, w = min(NB, k - j0);
   T* Fs = F + d.sn_foff[s];
   const int t = threadIdx.x;
-#pragma unroll
-  for (int j = 0; j < 64; j++) {
-    double v = 0.0;
-    if (j < w && t >= j && t < w) v = (double)Fs[(size_t)(j0 + j) * m + j0 + t];
-    A[t * 65 + j] = v;
-  }
+  for (int j = 0; j < 64; j++)
+    A[t * 65 + j] = (j < w && t >= j && t < w) ? (double)Fs[(size_t)(j0 + j) * m + j0 + t] : 0.0;
   lp[t] = t;
   pt[t] = 1;
   __syncthreads();
@@ -152,6 +149,8 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
     }
     const int kk = kc + kstep - 1;
     if (kp != kk) {
+      // symmetric interchange of kk and kp inside the block (lower storage); the
+      // same rows of earlier panels are permuted once at the end (lp)
       __syncthreads();
       const int a = kk, b = kp;
       if (t == 0) {
@@ -161,10 +160,6 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
       if (t > a && t < b) { double tmp = A[t * 65 + a]; A[t * 65 + a] = A[b * 65 + t]; A[b * 65 + t] = tmp; }
       if (t > b && t < w) { double tmp = A[t * 65 + a]; A[t * 65 + a] = A[t * 65 + b]; A[t * 65 + b] = tmp; }
       if (t < a) { double tmp = A[a * 65 + t]; A[a * 65 + t] = A[b * 65 + t]; A[b * 65 + t] = tmp; }
-      for (int c = t; c < j0; c += 64) {
-        const size_t o = (size_t)c * m;
-        T tmp = Fs[o + j0 + a]; Fs[o + j0 + a] = Fs[o + j0 + b]; Fs[o + j0 + b] = tmp;
-      }
     }
     __syncthreads();
     if (kstep == 1) {
@@ -174,17 +169,14 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
       if (pz) npert++;
       if (dd > 0) npos++; else nneg++;
       const double dinv = 1.0 / dd;
-      // column kc into registers first, so the row updates below pipeline
-      double col[64];
-#pragma unroll
-      for (int j = 0; j < 64; j++) col[j] = A[j * 65 + kc];
-      const double aik = A[(t & 63) * 65 + kc];
+      const double aik = (t > kc && t < w) ? A[t * 65 + kc] : 0.0;
+      colb[t] = aik * dinv;
       __syncthreads();
       if (t > kc && t < w) {
-#pragma unroll
-        for (int j = 0; j < 64; j++)
-          if (j > kc && j <= t) A[t * 65 + j] -= aik * (col[j] * dinv);
-        A[t * 65 + kc] = aik * dinv;
+        double* row = A + t * 65;
+#pragma unroll 4
+        for (int j = kc + 1; j <= t; j++) row[j] -= aik * colb[j];
+        row[kc] = aik * dinv;
       }
       if (t == 0) {
         A[kc * 65 + kc] = dd;
@@ -200,21 +192,17 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
       else if (d11 + d22 > 0) npos += 2;
       else nneg += 2;
       n2++;
-      double c1[64], c2[64];
-#pragma unroll
-      for (int j = 0; j < 64; j++) { c1[j] = A[j * 65 + g1]; c2[j] = A[j * 65 + g2]; }
-      const double a1 = A[(t & 63) * 65 + g1], a2 = A[(t & 63) * 65 + g2];
+      const bool act = (t > g2 && t < w);
+      const double a1 = act ? A[t * 65 + g1] : 0.0, a2 = act ? A[t * 65 + g2] : 0.0;
+      colb[t] = a1 * i11 + a2 * i21;
+      colb2[t] = a1 * i21 + a2 * i22;
       __syncthreads();
-      if (t > g2 && t < w) {
-#pragma unroll
-        for (int j = 0; j < 64; j++) {
-          if (j > g2 && j <= t) {
-            const double l1 = c1[j] * i11 + c2[j] * i21, l2 = c1[j] * i21 + c2[j] * i22;
-            A[t * 65 + j] -= a1 * l1 + a2 * l2;
-          }
-        }
-        A[t * 65 + g1] = a1 * i11 + a2 * i21;
-        A[t * 65 + g2] = a1 * i21 + a2 * i22;
+      if (act) {
+        double* row = A + t * 65;
+#pragma unroll 4
+        for (int j = g2 + 1; j <= t; j++) row[j] -= a1 * colb[j] + a2 * colb2[j];
+        row[g1] = colb[t];
+        row[g2] = colb2[t];
       }
       if (t == 0) {
         pt[g1] = 2;
@@ -225,34 +213,56 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
     kc += kstep;
   }
   __syncthreads();
+  // rows j0..j0+w of the earlier panels' columns follow the interchanges (new row t
+  // = old row lp[t]); one wave: every lane loads before any lane stores
+  if (j0 > 0) {
+    bool moved = false;
+    for (int q = 0; q < w; q++) moved |= (lp[q] != q);
+    if (moved) {
+      const int src = (t < w) ? lp[t] : t;
+      int c = 0;
+      for (; c + 4 <= j0; c += 4) {
+        T v0 = (T)0, v1 = (T)0, v2 = (T)0, v3 = (T)0;
+        if (t < w) {
+          v0 = Fs[(size_t)c * m + j0 + src];
+          v1 = Fs[(size_t)(c + 1) * m + j0 + src];
+          v2 = Fs[(size_t)(c + 2) * m + j0 + src];
+          v3 = Fs[(size_t)(c + 3) * m + j0 + src];
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (t < w) {
+          Fs[(size_t)c * m + j0 + t] = v0;
+          Fs[(size_t)(c + 1) * m + j0 + t] = v1;
+          Fs[(size_t)(c + 2) * m + j0 + t] = v2;
+          Fs[(size_t)(c + 3) * m + j0 + t] = v3;
+        }
+      }
+      for (; c < j0; c++) {
+        T v0 = (t < w) ? Fs[(size_t)c * m + j0 + src] : (T)0;
+        __builtin_amdgcn_wave_barrier();
+        if (t < w) Fs[(size_t)c * m + j0 + t] = v0;
+      }
+    }
+  }
   // inverse of the unit lower L11 (entries (q+1,q) of 2x2 pivots hold D, not L):
-  // lane c forms column c of L11^-1 in registers, xc[r] = Linv[r][c]
+  // lane c forms column c, stored above the diagonal as A[c][r] = Linv[r][c]
   const unsigned long long two = __ballot(t < w && pt[t] == 2);
-  double xc[64];
-#pragma unroll
-  for (int l = 0; l < 64; l++) xc[l] = (l == t) ? 1.0 : 0.0;
-#pragma unroll
-  for (int r = 1; r < 64; r++) {
-    double acc = 0.0;
-#pragma unroll
-    for (int l = 0; l < r; l++) {
-      double lrl = A[r * 65 + l];
-      if (l == r - 1 && ((two >> l) & 1ULL)) lrl = 0.0;
-      acc -= lrl * xc[l];
-    }
-    if (r > t) xc[r] = acc;
-  }
-  // lower: L and D; upper (row t, column j0+r > j0+t): Linv[r][t]
-#pragma unroll
-  for (int j = 0; j < 64; j++) {
-    if (j < w && t < w) {
-      const double v = (t >= j) ? A[t * 65 + j] : 0.0;
-      if (t >= j) Fs[(size_t)(j0 + j) * m + j0 + t] = (T)v;
+  if (t < w) {
+    const int c = t;
+    for (int r = c + 1; r < w; r++) {
+      double acc = (r == c + 1 && ((two >> c) & 1ULL)) ? 0.0 : -A[r * 65 + c];
+#pragma unroll 4
+      for (int l = c + 1; l < r; l++) {
+        const double lrl = (l == r - 1 && ((two >> l) & 1ULL)) ? 0.0 : A[r * 65 + l];
+        acc -= lrl * A[c * 65 + l];
+      }
+      A[c * 65 + r] = acc;
     }
   }
-#pragma unroll
-  for (int r = 0; r < 64; r++)
-    if (r < w && r > t) Fs[(size_t)(j0 + r) * m + j0 + t] = (T)xc[r];
+  __syncthreads();
+  // lower: L and D; upper (row t, column j0+j > j0+t): Linv[j][t]
+  for (int j = 0; j < w; j++)
+    if (t < w) Fs[(size_t)(j0 + j) * m + j0 + t] = (T)A[t * 65 + j];
   if (t < w) {
     d.lperm[first + j0 + t] = j0 + lp[t];
     d.piv[first + j0 + t] = pt[t];
@@ -590,36 +600,40 @@ __global__ __launch_bounds__(256) void k_fwd_xgemv(const int2* __restrict__ item
   if (lane == 0) y[first + t] = f[t] + acc;
 }
 
-// u_s[i] -= sum_t L(k+i, t) y_t ; item = (s, 64-row block); 4 waves split columns
+// u_s[i] -= sum_t L(k+i, t) y_t ; item = (s, 32-row block); 8 column groups per row
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_gemv(const int2* __restrict__ items, DevSym d,
                                                   const T* __restrict__ F,
                                                   const T* __restrict__ y, T* __restrict__ u) {
   __shared__ T ys[KMAX];
-  __shared__ T part[4][64];
+  __shared__ T part[8][33];
   const int2 it = items[blockIdx.x];
   const int s = it.x, rb = it.y;
   const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s], r = m - k;
   const T* Fs = F + d.sn_foff[s];
   for (int t = threadIdx.x; t < k; t += 256) ys[t] = y[first + t];
   __syncthreads();
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i = rb * 64 + lane;
-  T acc0 = (T)0, acc1 = (T)0;
+  const int g = threadIdx.x >> 5, lr = threadIdx.x & 31;
+  const int i = rb * 32 + lr;
+  T a0 = (T)0, a1 = (T)0, a2 = (T)0, a3 = (T)0;
   if (i < r) {
     const T* col = Fs + k + i;
-    int q = wv;
-    for (; q + 4 < k; q += 8) {
-      acc0 += col[(size_t)q * m] * ys[q];
-      acc1 += col[(size_t)(q + 4) * m] * ys[q + 4];
+    int q = g;
+    for (; q + 24 < k; q += 32) {
+      a0 += col[(size_t)q * m] * ys[q];
+      a1 += col[(size_t)(q + 8) * m] * ys[q + 8];
+      a2 += col[(size_t)(q + 16) * m] * ys[q + 16];
+      a3 += col[(size_t)(q + 24) * m] * ys[q + 24];
     }
-    if (q < k) acc0 += col[(size_t)q * m] * ys[q];
+    for (; q < k; q += 8) a0 += col[(size_t)q * m] * ys[q];
   }
-  part[wv][lane] = acc0 + acc1;
+  part[g][lr] = (a0 + a1) + (a2 + a3);
   __syncthreads();
-  if (wv == 0 && i < r) {
-    T* us = u + d.sn_uoff[s];
-    us[i] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+  if (threadIdx.x < 32 && i < r) {
+    T acc = (T)0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) acc += part[q][lr];
+    u[d.sn_uoff[s] + i] -= acc;
   }
 }
 
@@ -661,27 +675,32 @@ __global__ __launch_bounds__(256) void k_bwd_vgemv(const int2* __restrict__ item
   }
 }
 
-// partial[c][l] = sum_{i in chunk c, i > l} X(i,l) v_i ; item = (s, chunk of 64 columns)
+// partial[c][l] = sum_{i in chunk c, i > l} X(i,l) v_i
+// item = (s, chunk of 64 columns c, block of 256 outputs l)  -> int4 {s, c, lb, 0}
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_xpart(const int2* __restrict__ items, DevSym d,
+__global__ __launch_bounds__(256) void k_bwd_xpart(const int4* __restrict__ items, DevSym d,
                                                    const T* __restrict__ F,
                                                    const T* __restrict__ v,
                                                    const long long* __restrict__ poff,
                                                    T* __restrict__ part) {
   __shared__ T vs[64];
-  const int2 it = items[blockIdx.x];
-  const int s = it.x, c = it.y;
+  const int4 it = items[blockIdx.x];
+  const int s = it.x, c = it.y, lb = it.z;
   const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s];
   const int i0 = c * 64, i1 = min(k, i0 + 64);
   const T* Fs = F + d.sn_foff[s];
   if (threadIdx.x < 64) vs[threadIdx.x] = (i0 + (int)threadIdx.x < k) ? v[first + i0 + threadIdx.x] : (T)0;
   __syncthreads();
-  T* pc = part + poff[s] + (size_t)c * k;
-  for (int l = threadIdx.x; l < i1; l += 256) {
-    T acc = (T)0;
-    for (int i = max(i0, l + 1); i < i1; i++) acc += Fs[(size_t)i * m + l] * vs[i - i0];
-    pc[l] = acc;
+  const int l = lb * 256 + (int)threadIdx.x;
+  if (l >= i1) return;
+  T a0 = (T)0, a1 = (T)0;
+  int i = max(i0, l + 1);
+  for (; i + 1 < i1; i += 2) {
+    a0 += Fs[(size_t)i * m + l] * vs[i - i0];
+    a1 += Fs[(size_t)(i + 1) * m + l] * vs[i + 1 - i0];
   }
+  if (i < i1) a0 += Fs[(size_t)i * m + l] * vs[i - i0];
+  part[poff[s] + (size_t)c * k + l] = a0 + a1;
 }
 
 template <typename T>
@@ -787,6 +806,60 @@ __global__ void k_axpbypcz(int n, double a, const double* __restrict__ x, double
 }
 
 // ------------------------------------------------------------------------
+// BiCGStab vector updates with device-resident scalars (no host round trips):
+// slots: 0 rho, 1 rho_prev, 2 alpha, 3 omega, 4 (r_hat,v), 5 |s|^2, 6 (t,s), 7 (t,t)
+// ------------------------------------------------------------------------
+// p = r + beta (p - omega v), beta = (rho/rho_prev)(alpha/omega)
+__global__ void k_bicg_p(int n, const double* __restrict__ sc, const double* __restrict__ r,
+                         const double* __restrict__ v, double* __restrict__ p) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double beta = (sc[0] / sc[1]) * (sc[2] / sc[3]);
+  p[i] = r[i] + beta * (p[i] - sc[3] * v[i]);
+}
+// s = r - alpha v with alpha = rho / (r_hat, v)
+__global__ void k_bicg_s(int n, const double* __restrict__ sc, const double* __restrict__ r,
+                         const double* __restrict__ v, double* __restrict__ s) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double alpha = sc[0] / sc[4];
+  s[i] = r[i] - alpha * v[i];
+}
+// x += alpha y + omega z ; r = s - omega t, omega = (t,s)/(t,t)
+__global__ void k_bicg_xr(int n, const double* __restrict__ sc, const double* __restrict__ y,
+                          const double* __restrict__ z, const double* __restrict__ s,
+                          const double* __restrict__ t, double* __restrict__ x,
+                          double* __restrict__ r) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double alpha = sc[0] / sc[4], omega = sc[6] / sc[7];
+  x[i] = x[i] + alpha * y[i] + omega * z[i];
+  r[i] = s[i] - omega * t[i];
+}
+// x += alpha y (early exit on small |s|)
+__global__ void k_bicg_xa(int n, const double* __restrict__ sc, const double* __restrict__ y,
+                          double* __restrict__ x) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  x[i] += (sc[0] / sc[4]) * y[i];
+}
+// end of an iteration: alpha, omega, rho_prev for the next one; or initial values
+__global__ void k_bicg_scalars(double* __restrict__ sc, int init) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (init) { sc[1] = 1.0; sc[2] = 1.0; sc[3] = 1.0; return; }
+  sc[2] = sc[0] / sc[4];
+  sc[3] = sc[6] / sc[7];
+  sc[1] = sc[0];
+}
+__global__ void k_fill(long long n, double v, double* __restrict__ x) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+__global__ void k_zero_counters(FactorCounters* c) {
+  if (threadIdx.x == 0) { c->npos = 0; c->nneg = 0; c->npert = 0; c->n2x2 = 0; }
+}
+
+// ------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------
 static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -845,7 +918,7 @@ void launch_bwd_vgemv(const int2* items, int nitems, const DevSym& d, const T* F
   if (nitems > 0) hipLaunchKernelGGL(k_bwd_vgemv<T>, dim3(nitems), dim3(256), 0, st, items, d, F, y, xp, v);
 }
 template <typename T>
-void launch_bwd_xpart(const int2* items, int nitems, const DevSym& d, const T* F, const T* v,
+void launch_bwd_xpart(const int4* items, int nitems, const DevSym& d, const T* F, const T* v,
                       const long long* poff, T* part, hipStream_t st) {
   if (nitems > 0)
     hipLaunchKernelGGL(k_bwd_xpart<T>, dim3(nitems), dim3(256), 0, st, items, d, F, v, poff, part);
@@ -890,6 +963,29 @@ void launch_axpbypcz(int n, double a, const double* x, double b, const double* y
   hipLaunchKernelGGL(k_axpbypcz, dim3(nblk(n, 256)), dim3(256), 0, st, n, a, x, b, y, c, z);
 }
 
+void launch_bicg_p(int n, const double* sc, const double* r, const double* v, double* p, hipStream_t st) {
+  hipLaunchKernelGGL(k_bicg_p, dim3(nblk(n, 256)), dim3(256), 0, st, n, sc, r, v, p);
+}
+void launch_bicg_s(int n, const double* sc, const double* r, const double* v, double* s_, hipStream_t st) {
+  hipLaunchKernelGGL(k_bicg_s, dim3(nblk(n, 256)), dim3(256), 0, st, n, sc, r, v, s_);
+}
+void launch_bicg_xr(int n, const double* sc, const double* y, const double* z, const double* s_,
+                    const double* t, double* x, double* r, hipStream_t st) {
+  hipLaunchKernelGGL(k_bicg_xr, dim3(nblk(n, 256)), dim3(256), 0, st, n, sc, y, z, s_, t, x, r);
+}
+void launch_bicg_xa(int n, const double* sc, const double* y, double* x, hipStream_t st) {
+  hipLaunchKernelGGL(k_bicg_xa, dim3(nblk(n, 256)), dim3(256), 0, st, n, sc, y, x);
+}
+void launch_bicg_scalars(double* sc, int init, hipStream_t st) {
+  hipLaunchKernelGGL(k_bicg_scalars, dim3(1), dim3(64), 0, st, sc, init);
+}
+void launch_fill(long long n, double v, double* x, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, st, n, v, x);
+}
+void launch_zero_counters(FactorCounters* c, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, c);
+}
+
 #define KKT_INST(T)                                                                              \
   template void launch_assemble<T>(const int4*, int, const DevSym&, T*, hipStream_t);            \
   template void launch_scatter<T>(const int*, const long long*, int, const double*, T*,          \
@@ -908,7 +1004,7 @@ void launch_axpbypcz(int n, double a, const double* x, double b, const double* y
                                    hipStream_t);                                                 \
   template void launch_bwd_vgemv<T>(const int2*, int, const DevSym&, const T*, const T*,         \
                                     const T*, T*, hipStream_t);                                  \
-  template void launch_bwd_xpart<T>(const int2*, int, const DevSym&, const T*, const T*,         \
+  template void launch_bwd_xpart<T>(const int4*, int, const DevSym&, const T*, const T*,         \
                                     const long long*, T*, hipStream_t);                          \
   template void launch_bwd_finish<T>(const int*, int, const DevSym&, const T*,                   \
                                      const long long*, const T*, T*, hipStream_t);

@@ -83,8 +83,9 @@ struct kkt_solver {
   // plan
   int4 *d_asm = nullptr, *d_upd = nullptr;
   int *d_diag = nullptr, *d_lvlsn = nullptr, *d_aidx = nullptr;
-  int2 *d_trsm = nullptr, *d_fg = nullptr, *d_bg = nullptr, *d_xrow = nullptr, *d_xg = nullptr,
-       *d_xp = nullptr;
+  int2 *d_trsm = nullptr, *d_fg = nullptr, *d_bg = nullptr, *d_xrow = nullptr, *d_xg = nullptr;
+  int4* d_xp = nullptr;
+  std::vector<Launch> xrow_l;  // inverse-row post-pass, one launch per panel index p >= 1
   long long *d_apos = nullptr, *d_poff = nullptr;
   std::vector<Launch> asm_l, scat_l, sn_l, fg_l, bg_l, xg_l, xp_l;
   long long part_elems = 0;
@@ -98,14 +99,23 @@ struct kkt_solver {
   void* d_svec = nullptr;  // solve scratch: bp, fp, y, v, xp (n each), u (u_elems), partials, as T
   size_t svec_bytes = 0;
   // FP64 vectors for refinement / BiCGStab / host copies
-  double* d_vec = nullptr;  // 12 vectors of n
+  double* d_vec = nullptr;  // 14 vectors of n (0-8 BiCGStab, 10-11 host copies, 12-13 graph I/O)
   double* d_partial = nullptr;
   double* d_scal = nullptr;  // 16 scalars
   FactorCounters* d_ctr = nullptr;
   std::vector<double> multiB;
+  // captured triangular-solve graphs (per factor precision), replayed per solve
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  bool graphs_on = true;
+  void drop_graphs() {
+    for (int q = 0; q < 2; q++)
+      if (gexec[q]) { (void)hipGraphExecDestroy(gexec[q]); gexec[q] = nullptr; }
+  }
 
   // ---- timing ----
   bool tm_on = false;
+  int tm_mask = 0xFF;  // kernel classes that get events
+  bool tm_open = false;
   std::vector<hipEvent_t> ev_pool;
   std::vector<int> ev_class;
   size_t ev_used = 0;
@@ -129,6 +139,7 @@ struct kkt_solver {
     v.clear();
   }
   void release_numeric() {
+    drop_graphs();
     if (d_F64) (void)hipFree(d_F64);
     if (d_F32) (void)hipFree(d_F32);
     if (d_W64) (void)hipFree(d_W64);
@@ -189,7 +200,9 @@ struct kkt_solver {
 
   // ---- timing helpers ----
   void tbeg(int cls) {
-    if (!tm_on) return;
+    tm_open = false;
+    if (!tm_on || !((tm_mask >> cls) & 1)) return;
+    tm_open = true;
     if (ev_used + 2 > ev_pool.size()) {
       for (int q = 0; q < 256; q++) {
         hipEvent_t e;
@@ -202,7 +215,8 @@ struct kkt_solver {
     ev_used++;
   }
   void tend() {
-    if (!tm_on) return;
+    if (!tm_open) return;
+    tm_open = false;
     (void)hipEventRecord(ev_pool[ev_used], st);
     ev_used++;
   }
@@ -392,7 +406,8 @@ struct kkt_solver {
     std::vector<int4> asmv, updv;
     std::vector<int> diagv, lvlsn, aidx;
     std::vector<long long> apos;
-    std::vector<int2> trsmv, fgv, bgv, xrowv, xgv, xpv;
+    std::vector<int2> trsmv, fgv, bgv, xrowv, xgv;
+    std::vector<int4> xpv;
     std::vector<long long> poff(ns + 1, 0);
     for (int s = 0; s < ns; s++) poff[s + 1] = poff[s] + (long long)((kk[s] + 63) / 64) * kk[s];
     part_elems = poff[ns];
@@ -425,10 +440,13 @@ struct kkt_solver {
       xp_l[L].off = (long long)xpv.size();
       for (int s : lv[L]) {
         int r = mm[s] - kk[s];
-        for (int rb = 0; rb < (r + 63) / 64; rb++) fgv.push_back(make_int2(s, rb));
+        for (int rb = 0; rb < (r + 31) / 32; rb++) fgv.push_back(make_int2(s, rb));
         for (int cb = 0; cb < (kk[s] + 3) / 4; cb++) bgv.push_back(make_int2(s, cb));
         for (int cb = 0; cb < (kk[s] + 3) / 4; cb++) xgv.push_back(make_int2(s, cb));
-        for (int c = 0; c < (kk[s] + 63) / 64; c++) xpv.push_back(make_int2(s, c));
+        for (int c = 0; c < (kk[s] + 63) / 64; c++) {
+          int i1 = std::min(kk[s], c * 64 + 64);
+          for (int lb = 0; lb < (i1 + 255) / 256; lb++) xpv.push_back(make_int4(s, c, lb, 0));
+        }
       }
       fg_l[L].cnt = (int)(fgv.size() - fg_l[L].off);
       bg_l[L].cnt = (int)(bgv.size() - bg_l[L].off);
@@ -448,7 +466,6 @@ struct kkt_solver {
           if (kk[s] <= j0) continue;
           int w = std::min(NB, kk[s] - j0), r0 = j0 + w, mr = mm[s] - r0;
           diagv.push_back(s);
-          for (int cb = 0; cb < p; cb++) xrowv.push_back(make_int2(s, cb));
           for (int rb = 0; rb < (mr + 63) / 64; rb++) trsmv.push_back(make_int2(s, rb));
           int t = (mr + TILE - 1) / TILE;
           for (int I = 0; I < t; I++)
@@ -459,6 +476,20 @@ struct kkt_solver {
         sp.xrow.cnt = (int)(xrowv.size() - sp.xrow.off);
         sp.trsm.cnt = (int)(trsmv.size() - sp.trsm.off);
         sp.upd.cnt = (int)(updv.size() - sp.upd.off);
+      }
+    }
+    // inverse rows X(P, :) of every pivot block, after the whole factorization:
+    // launch p covers block row p of all supernodes at once
+    {
+      int maxp = 0;
+      for (int s = 0; s < ns; s++) maxp = std::max(maxp, (kk[s] + NB - 1) / NB);
+      xrow_l.assign(std::max(maxp, 1), Launch());
+      for (int p = 1; p < maxp; p++) {
+        xrow_l[p].off = (long long)xrowv.size();
+        for (int s = 0; s < ns; s++)
+          if (kk[s] > p * NB)
+            for (int cb = 0; cb < p; cb++) xrowv.push_back(make_int2(s, cb));
+        xrow_l[p].cnt = (int)(xrowv.size() - xrow_l[p].off);
       }
     }
     // full symmetric CSR for SpMV (row-sorted), map to upper value index
@@ -522,7 +553,7 @@ struct kkt_solver {
         hipMalloc(&d_map_f, map_f.size() * sizeof(int)) != hipSuccess ||
         hipMalloc(&d_vals, std::max(nnz, 1) * sizeof(double)) != hipSuccess ||
         hipMalloc(&d_valsf, col_f.size() * sizeof(double)) != hipSuccess ||
-        hipMalloc(&d_vec, (size_t)12 * n * sizeof(double)) != hipSuccess)
+        hipMalloc(&d_vec, (size_t)14 * n * sizeof(double)) != hipSuccess)
       return fail(KKT_ERR_ALLOC, "hipMalloc failed (pattern)");
     if (hipMemcpy(d_rowptr_f, rp_f.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d_col_f, col_f.data(), col_f.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
@@ -547,6 +578,7 @@ struct kkt_solver {
     }
     size_t need = ((size_t)5 * n + (size_t)u_elems + (size_t)part_elems + 16) * std::max(esz, sizeof(double));
     if (svec_bytes < need) {
+      drop_graphs();
       if (d_svec) (void)hipFree(d_svec);
       if (hipMalloc(&d_svec, need) != hipSuccess) return fail(KKT_ERR_ALLOC, "hipMalloc failed (solve)");
       svec_bytes = need;
@@ -560,7 +592,7 @@ struct kkt_solver {
     tbeg(TC_VEC);
     launch_absmax(nnz, d_vals, d_partial, d_scal + 15, st);
     tend();
-    (void)hipMemsetAsync(d_ctr, 0, sizeof(FactorCounters), st);
+    launch_zero_counters(d_ctr, st);
     const int nl = S.nlevels;
     for (int L = 0; L < nl; L++) {
       tbeg(TC_ASM);
@@ -571,7 +603,6 @@ struct kkt_solver {
         const Step& sp = steps[L][p];
         tbeg(TC_DIAG);
         launch_panel_diag<T>(d_diag + sp.diag.off, sp.diag.cnt, (int)p, ds, F, pert, d_scal + 15, d_ctr, st);
-        launch_panel_xrow<T>(d_xrow + sp.xrow.off, sp.xrow.cnt, (int)p, ds, F, st);
         tend();
         tbeg(TC_TRSM);
         launch_panel_trsm<T>(d_trsm + sp.trsm.off, sp.trsm.cnt, (int)p, ds, F, W, st);
@@ -581,6 +612,12 @@ struct kkt_solver {
         tend();
         if (tm_on) tm_upd_flops += sp.upd_flops;
       }
+    }
+    // pivot-block inverses for the solves (independent across supernodes)
+    for (size_t p = 1; p < xrow_l.size(); p++) {
+      tbeg(TC_DIAG);
+      launch_panel_xrow<T>(d_xrow + xrow_l[p].off, xrow_l[p].cnt, (int)p, ds, F, st);
+      tend();
     }
   }
 
@@ -660,9 +697,40 @@ struct kkt_solver {
     launch_scatter_perm<T, double>(n, d_perm, xp, d_x, st);
     tend();
   }
-  void tri_solve(const double* d_b, double* d_x) {
+  void tri_solve_eager(const double* d_b, double* d_x) {
     if (fprec == KKT_PREC_FP64) run_tri_solve<double>(d_F64, d_b, d_x);
     else run_tri_solve<float>(d_F32, d_b, d_x);
+  }
+  // The launch sequence of a solve is fixed by the analysis, so it is captured
+  // once into a HIP graph (reading vec(12), writing vec(13)) and replayed.
+  void tri_solve(const double* d_b, double* d_x) {
+    const int solve_cls = (1 << TC_SLEVEL) | (1 << TC_SGEMV) | (1 << TC_VEC);
+    if (!graphs_on || (tm_on && (tm_mask & solve_cls))) { tri_solve_eager(d_b, d_x); return; }
+    const int q = (fprec == KKT_PREC_FP64) ? 0 : 1;
+    double* gin = vec(12);
+    double* gout = vec(13);
+    if (!gexec[q]) {
+      hipGraph_t g = nullptr;
+      if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        graphs_on = false;
+        tri_solve_eager(d_b, d_x);
+        return;
+      }
+      tri_solve_eager(gin, gout);
+      if (hipStreamEndCapture(st, &g) != hipSuccess || !g ||
+          hipGraphInstantiate(&gexec[q], g, nullptr, nullptr, 0) != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        gexec[q] = nullptr;
+        graphs_on = false;
+        (void)hipGetLastError();
+        tri_solve_eager(d_b, d_x);
+        return;
+      }
+      (void)hipGraphDestroy(g);
+    }
+    if (d_b != gin) (void)hipMemcpyAsync(gin, d_b, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st);
+    (void)hipGraphLaunch(gexec[q], st);
+    if (d_x != gout) (void)hipMemcpyAsync(d_x, gout, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st);
   }
 
   double* vec(int i) { return d_vec + (size_t)i * n; }
@@ -750,69 +818,79 @@ struct kkt_solver {
     return KKT_OK;
   }
 
+  // reads `cnt` device scalars starting at slot `first` (one stream sync)
+  int read_scalars(int first, int cnt, double* out) {
+    if (hipMemcpyAsync(out, d_scal + first, cnt * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return fail(KKT_ERR_REDUCTION, "reduction failed");
+    return KKT_OK;
+  }
+  void dot_to(const double* a, const double* b, int slot) {
+    tbeg(TC_VEC);
+    launch_dot(n, a, b, d_partial, d_scal + slot, st);
+    tend();
+  }
+
   // BiCGStab preconditioned by the current (stale) factor: cudssBridge.cc:1705-2006
+  // (x0 = M^-1 b, r_hat = r0, early exit on ||s||, breakdown threshold 1e-30).
+  // Scalars stay on the device; the host reads them twice per iteration for the
+  // convergence and breakdown tests.
   int bicgstab(const double* d_b, double* d_xout, double tolRel, int maxIter, int* iters) {
     if (state != FACTORED) return fail(KKT_ERR_STATE, "iterativeSolve called before factor");
     double t0 = g_timing ? now_ms() : 0;
     double *r = vec(0), *rh = vec(1), *p = vec(2), *v = vec(3), *s = vec(4), *t = vec(5),
            *y = vec(6), *z = vec(7), *x = vec(8);
+    double* sc = d_scal;
     tri_solve(d_b, x);
     spmv(x, d_b, r, 1);
-    double bn2, rn2;
+    dot_to(d_b, d_b, 9);
+    dot_to(r, r, 8);
+    double h2[2];
     int rc;
-    if ((rc = dotv(d_b, d_b, bn2)) || (rc = dotv(r, r, rn2))) return rc;
-    double bn = std::sqrt(bn2), rn = std::sqrt(rn2);
+    if ((rc = read_scalars(8, 2, h2))) return rc;
+    double bn = std::sqrt(h2[1]), rn = std::sqrt(h2[0]);
     if (bn == 0.0) bn = 1.0;
     int it = 1;
-    if (rn / bn <= tolRel) {
-      it = 1;
-    } else {
+    if (rn / bn > tolRel) {
       (void)hipMemcpyAsync(rh, r, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st);
-      (void)hipMemsetAsync(p, 0, (size_t)n * sizeof(double), st);
-      (void)hipMemsetAsync(v, 0, (size_t)n * sizeof(double), st);
-      double rho_prev = 1.0, alpha = 1.0, omega = 1.0;
+      launch_fill(n, 0.0, p, st);
+      launch_fill(n, 0.0, v, st);
+      launch_bicg_scalars(sc, 1, st);
       const double eps = 1e-30;
       for (it = 1; it <= maxIter; it++) {
-        double rho;
-        if ((rc = dotv(rh, r, rho))) return rc;
-        if (std::fabs(rho) < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: rho ~ 0");
-        double beta = (rho / rho_prev) * (alpha / omega);
-        // p = r + beta * (p - omega v)
+        dot_to(rh, r, 0);                      // rho
         tbeg(TC_VEC);
-        launch_axpby(n, -omega, v, 1.0, p, st);
-        launch_axpby(n, 1.0, r, beta, p, st);
+        launch_bicg_p(n, sc, r, v, p, st);     // p = r + beta (p - omega v)
         tend();
-        tri_solve(p, y);
-        spmv(y, nullptr, v, 0);
-        double rhv;
-        if ((rc = dotv(rh, v, rhv))) return rc;
-        if (std::fabs(rhv) < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: (r_hat,v) ~ 0");
-        alpha = rho / rhv;
+        tri_solve(p, y);                       // y = M^-1 p
+        spmv(y, nullptr, v, 0);                // v = A y
+        dot_to(rh, v, 4);                      // (r_hat, v)
         tbeg(TC_VEC);
-        (void)hipMemcpyAsync(s, r, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st);
-        launch_axpby(n, -alpha, v, 1.0, s, st);
+        launch_bicg_s(n, sc, r, v, s, st);     // s = r - alpha v
         tend();
-        double sn2;
-        if ((rc = dotv(s, s, sn2))) return rc;
-        if (std::sqrt(sn2) / bn <= tolRel) {
-          launch_axpby(n, alpha, y, 1.0, x, st);
+        dot_to(s, s, 5);
+        double h5[6];
+        if ((rc = read_scalars(0, 6, h5))) return rc;
+        if (std::fabs(h5[0]) < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: rho ~ 0");
+        if (std::fabs(h5[4]) < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: (r_hat,v) ~ 0");
+        if (std::sqrt(h5[5]) / bn <= tolRel) {
+          launch_bicg_xa(n, sc, y, x, st);     // x += alpha y
           break;
         }
-        tri_solve(s, z);
-        spmv(z, nullptr, t, 0);
-        double ts, tt;
-        if ((rc = dotv(t, s, ts)) || (rc = dotv(t, t, tt))) return rc;
-        if (tt < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: (t,t) ~ 0");
-        omega = ts / tt;
+        tri_solve(s, z);                       // z = M^-1 s
+        spmv(z, nullptr, t, 0);                // t = A z
+        dot_to(t, s, 6);
+        dot_to(t, t, 7);
         tbeg(TC_VEC);
-        launch_axpbypcz(n, alpha, y, omega, z, 1.0, x, st);
-        (void)hipMemcpyAsync(r, s, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st);
-        launch_axpby(n, -omega, t, 1.0, r, st);
+        launch_bicg_xr(n, sc, y, z, s, t, x, r, st);  // x += alpha y + omega z; r = s - omega t
         tend();
-        if ((rc = dotv(r, r, rn2))) return rc;
-        if (std::sqrt(rn2) / bn <= tolRel) break;
-        if (std::fabs(omega) < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: omega ~ 0");
-        rho_prev = rho;
+        dot_to(r, r, 8);
+        double h3[3];
+        if ((rc = read_scalars(6, 3, h3))) return rc;
+        if (h3[1] < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: (t,t) ~ 0");
+        if (std::sqrt(h3[2]) / bn <= tolRel) break;
+        if (std::fabs(h3[0] / h3[1]) < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: omega ~ 0");
+        launch_bicg_scalars(sc, 0, st);        // alpha, omega, rho_prev for the next iteration
       }
       if (it > maxIter) return fail(KKT_ERR_ITER_NO_CONVERGE, "BiCGStab did not converge within iteration limit");
     }
@@ -1085,6 +1163,7 @@ int kkt_verify_device_structure(kkt_solver* h) {
 void kkt_timing_reset(kkt_solver* h, int enable) {
   if (!h) return;
   h->tm_on = enable != 0;
+  h->tm_mask = (enable < 0) ? 0xFF : (enable & 0xFF);
   for (int c = 0; c < TC_N; c++) { h->tm_ms[c] = 0; h->tm_cnt[c] = 0; }
   h->tm_upd_flops = 0;
   h->ev_used = 0;

@@ -138,9 +138,11 @@ int kkt_get_symbolic(kkt_solver* h, int* perm, int* sup_ptr, int* sparent, long 
  * host arrays byte for byte; returns 0 when identical, 1 when not. */
 int kkt_verify_device_structure(kkt_solver* h);
 /* Kernel timing (HIP events on the solver's stream) accumulated since the
- * last reset, per kernel class: 0 assemble, 1 panel-diag, 2 panel-trsm,
- * 3 panel-update (MFMA), 4 solve-level, 5 solve-gemv, 6 spmv, 7 vector ops.
- * ms[c], launches[c]; `flops` = algorithmic flops of class-3 launches. */
+ * last reset, per kernel class: 0 assemble, 1 panel-diag (+ inverse rows),
+ * 2 panel-trsm, 3 panel-update (MFMA), 4 solve-level, 5 solve-gemv, 6 spmv,
+ * 7 vector ops.  enable: 0 off, -1 all classes, else a bit mask (1 << class).
+ * Solves replay a captured HIP graph unless a solve class is timed.
+ * ms[c], launches[c]; `update_flops` = algorithmic flops of class-3 launches. */
 void kkt_timing_reset(kkt_solver* h, int enable);
 int kkt_timing_get(kkt_solver* h, double* ms, long long* launches, double* update_flops);
 
