@@ -106,10 +106,29 @@ struct kkt_solver {
   std::vector<double> multiB;
   // captured triangular-solve graphs (per factor precision), replayed per solve
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  hipGraphExec_t fexec[2] = {nullptr, nullptr};  // captured factorizations
   bool graphs_on = true;
   void drop_graphs() {
-    for (int q = 0; q < 2; q++)
+    for (int q = 0; q < 2; q++) {
       if (gexec[q]) { (void)hipGraphExecDestroy(gexec[q]); gexec[q] = nullptr; }
+      if (fexec[q]) { (void)hipGraphExecDestroy(fexec[q]); fexec[q] = nullptr; }
+    }
+  }
+  // Captures `fn` (launches on st only) into exec, or returns false.
+  template <typename Fn>
+  bool capture(hipGraphExec_t& exec, Fn fn) {
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+    fn();
+    if (hipStreamEndCapture(st, &g) != hipSuccess || !g ||
+        hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      exec = nullptr;
+      (void)hipGetLastError();
+      return false;
+    }
+    (void)hipGraphDestroy(g);
+    return true;
   }
 
   // ---- timing ----
@@ -649,9 +668,20 @@ struct kkt_solver {
     int rc = ensure_numeric(p);
     if (rc) return rc;
     double t0 = g_timing ? now_ms() : 0;
-    launch_expand_vals(nnzf, d_map_f, d_vals, d_valsf, st);
-    if (p == KKT_PREC_FP64) run_factor<double>(d_F64, d_W64, 1e-8);
-    else run_factor<float>(d_F32, d_W32, 1e-6);
+    auto body = [&]() {
+      launch_expand_vals(nnzf, d_map_f, d_vals, d_valsf, st);
+      if (p == KKT_PREC_FP64) run_factor<double>(d_F64, d_W64, 1e-8);
+      else run_factor<float>(d_F32, d_W32, 1e-6);
+    };
+    // the factor's launch sequence is fixed by the analysis: one graph per precision,
+    // unless factor kernels are being timed with events
+    const int fcls = (1 << TC_ASM) | (1 << TC_DIAG) | (1 << TC_TRSM) | (1 << TC_UPD) | (1 << TC_VEC);
+    const int q = (p == KKT_PREC_FP64) ? 0 : 1;
+    if (graphs_on && !(tm_on && (tm_mask & fcls))) {
+      if (!fexec[q] && !capture(fexec[q], body)) graphs_on = false;
+    }
+    if (graphs_on && fexec[q] && !(tm_on && (tm_mask & fcls))) (void)hipGraphLaunch(fexec[q], st);
+    else body();
     if (hipMemcpyAsync(&ctr, d_ctr, sizeof(FactorCounters), hipMemcpyDeviceToHost, st) != hipSuccess)
       return fail(KKT_ERR_COPY, "hipMemcpyAsync (counters) failed");
     if (hipStreamSynchronize(st) != hipSuccess) return fail(KKT_ERR_FACTORIZATION, "factor kernels failed");
@@ -709,24 +739,10 @@ struct kkt_solver {
     const int q = (fprec == KKT_PREC_FP64) ? 0 : 1;
     double* gin = vec(12);
     double* gout = vec(13);
-    if (!gexec[q]) {
-      hipGraph_t g = nullptr;
-      if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        graphs_on = false;
-        tri_solve_eager(d_b, d_x);
-        return;
-      }
-      tri_solve_eager(gin, gout);
-      if (hipStreamEndCapture(st, &g) != hipSuccess || !g ||
-          hipGraphInstantiate(&gexec[q], g, nullptr, nullptr, 0) != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
-        gexec[q] = nullptr;
-        graphs_on = false;
-        (void)hipGetLastError();
-        tri_solve_eager(d_b, d_x);
-        return;
-      }
-      (void)hipGraphDestroy(g);
+    if (!gexec[q] && !capture(gexec[q], [&]() { tri_solve_eager(gin, gout); })) {
+      graphs_on = false;
+      tri_solve_eager(d_b, d_x);
+      return;
     }
     if (d_b != gin) (void)hipMemcpyAsync(gin, d_b, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st);
     (void)hipGraphLaunch(gexec[q], st);
@@ -1089,6 +1105,14 @@ int kkt_set_refine_tol(kkt_solver* h, double tol, int max_steps) {
   if (!(tol > 0) || max_steps < 1) return h->fail(KKT_ERR_ARGUMENT, "bad refinement settings");
   h->refine_tol = tol;
   h->refine_max = max_steps;
+  h->ok();
+  return KKT_OK;
+}
+
+int kkt_set_graphs(kkt_solver* h, int on) {
+  CHECK_H(h);
+  h->graphs_on = on != 0;
+  if (!h->graphs_on && h->dev_ok) h->drop_graphs();
   h->ok();
   return KKT_OK;
 }

@@ -267,3 +267,171 @@ def large_beam_sequence(nsteps=20, seed=0x5EED, nx=100, ny=16, nz=16, length=1.0
         out_b.append(b)
         u_prev = u
     return beam, out_v, out_b
+
+
+# ---------------------------------------------------------------------------
+# Linear tetrahedral FEM blocks, rigid bodies and contact constraints
+# ---------------------------------------------------------------------------
+def tet_grid(nx, ny, nz, lx, ly, lz, origin=(0.0, 0.0, 0.0)):
+    """Structured tet mesh: each cell of an nx*ny*nz box grid split into 6 tets
+    along its main diagonal (conforming across cells)."""
+    xs = np.linspace(0, lx, nx + 1) + origin[0]
+    ys = np.linspace(0, ly, ny + 1) + origin[1]
+    zs = np.linspace(0, lz, nz + 1) + origin[2]
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    nodes = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+
+    def nid(i, j, k):
+        return (k * (ny + 1) + j) * (nx + 1) + i
+
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    e = [(1, 0, 0), (0, 1, 0), (0, 0, 1)]
+    tets = []
+    for p in ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)):
+        a = (0, 0, 0)
+        b = tuple(a[q] + e[p[0]][q] for q in range(3))
+        c = tuple(b[q] + e[p[1]][q] for q in range(3))
+        tets.append(np.stack([nid(i, j, k), nid(i + b[0], j + b[1], k + b[2]),
+                              nid(i + c[0], j + c[1], k + c[2]), nid(i + 1, j + 1, k + 1)], 1))
+    return nodes, np.concatenate(tets, 0)
+
+
+def delaunay_box(npts_x, ny, nz, lx, ly, lz, seed, jitter=0.25):
+    """Seeded Delaunay tet mesh of a jittered lattice filling an lx*ly*lz box
+    (interior points jittered, boundary points on the box faces)."""
+    from scipy.spatial import Delaunay
+    rng = np.random.default_rng(seed)
+    xs = np.linspace(0, lx, npts_x)
+    ys = np.linspace(0, ly, ny)
+    zs = np.linspace(0, lz, nz)
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    h = np.array([lx / (npts_x - 1), ly / (ny - 1), lz / (nz - 1)])
+    inner = np.all((P > 1e-12) & (P < np.array([lx, ly, lz]) - 1e-12), axis=1)
+    P[inner] += rng.uniform(-jitter, jitter, size=(inner.sum(), 3)) * h
+    tri = Delaunay(P)
+    T = tri.simplices.astype(np.int64)
+    D = np.stack([P[T[:, 1]] - P[T[:, 0]], P[T[:, 2]] - P[T[:, 0]], P[T[:, 3]] - P[T[:, 0]]], 2)
+    vol = np.linalg.det(D) / 6.0
+    keep = np.abs(vol) > 1e-6 * np.abs(vol).mean()
+    T = T[keep]
+    flip = vol[keep] < 0
+    T[flip] = T[flip][:, [0, 2, 1, 3]]
+    return P, T
+
+
+def tet_linear_stiffness(nodes, tets, lam, mu):
+    """Element stiffness (E,12,12) and volumes of linear tets, isotropic material."""
+    P0 = nodes[tets[:, 0]]
+    Dm = np.stack([nodes[tets[:, 1]] - P0, nodes[tets[:, 2]] - P0, nodes[tets[:, 3]] - P0], 2)
+    vol = np.abs(np.linalg.det(Dm)) / 6.0
+    Dinv = np.linalg.inv(Dm)  # rows: gradients of N1..N3
+    g = np.zeros((len(tets), 4, 3))
+    g[:, 1:] = Dinv
+    g[:, 0] = -Dinv.sum(1)
+    I3 = np.eye(3)
+    # Ke[a,i,b,k] = V (lam g_a[i] g_b[k] + mu (g_a.g_b) delta_ik + mu g_a[k] g_b[i])
+    Ke = (lam * np.einsum("eai,ebk->eaibk", g, g) +
+          mu * np.einsum("eaj,ebj,ik->eaibk", g, g, I3) +
+          mu * np.einsum("eak,ebi->eaibk", g, g))
+    Ke = Ke * vol[:, None, None, None, None]
+    return Ke.reshape(len(tets), 12, 12), vol
+
+
+def assemble_tet(nodes, tets, lam, mu, density):
+    """Stiffness (scipy CSR, full) and lumped mass vector of a linear tet body."""
+    Ke, vol = tet_linear_stiffness(nodes, tets, lam, mu)
+    E = len(tets)
+    dofs = (3 * tets[:, :, None] + np.arange(3)[None, None]).reshape(E, 12)
+    r = np.repeat(dofs, 12, axis=1).ravel()
+    c = np.tile(dofs, (1, 12)).ravel()
+    n = 3 * len(nodes)
+    K = sp.csr_matrix((Ke.ravel(), (r, c)), shape=(n, n))
+    m = np.zeros(len(nodes))
+    np.add.at(m, tets.ravel(), np.repeat(density * vol / 4.0, 4))
+    return K, np.repeat(m, 3)
+
+
+def rigid_contact_system(ndof, seed, h=0.01):
+    """One system of the batching mix: 2-20 rigid bodies (6 DOF, random SPD spatial
+    inertia) plus a small linear tet FEM block, with 10-200 contact rows treated as
+    equalities (random unit normals; each row couples one rigid body to a distinct
+    FEM node, so G has full row rank).  Returns (M_or_S, GT, rhs_m, rhs_g)."""
+    rng = np.random.default_rng(seed)
+    nb = int(rng.integers(2, 21))
+    n_rigid = 6 * nb
+    n_fem_nodes = max(8, (ndof - n_rigid) // 3)
+    # tet box with about n_fem_nodes nodes
+    a = max(1, int(round((n_fem_nodes / 2.0) ** (1.0 / 3.0))))
+    dims = [2 * a - 1, a - 1, a - 1]
+    while (dims[0] + 1) * (dims[1] + 1) * (dims[2] + 1) < n_fem_nodes:
+        dims[int(np.argmin(dims))] += 1
+    dims = [max(1, d) for d in dims]
+    nodes, tets = tet_grid(dims[0], dims[1], dims[2], 0.02 * dims[0], 0.02 * dims[1], 0.02 * dims[2])
+    lam, mu = lame(rng.uniform(5e4, 5e5), 0.3)
+    K, mfem = assemble_tet(nodes, tets, lam, mu, 1000.0)
+    nfem = K.shape[0]
+    # rigid bodies: mass m I3 and a random SPD inertia tensor
+    blocks = []
+    for b in range(nb):
+        m = rng.uniform(0.1, 2.0)
+        Q = np.linalg.qr(rng.standard_normal((3, 3)))[0]
+        J = Q @ np.diag(rng.uniform(1e-3, 1e-2, 3)) @ Q.T
+        Mb = np.zeros((6, 6))
+        Mb[:3, :3] = m * np.eye(3)
+        Mb[3:, 3:] = J
+        blocks.append(sp.csr_matrix(Mb))
+    beta = 0.01
+    Sfem = sp.diags(mfem) + (h * beta + h * h) * K
+    S = sp.block_diag(blocks + [Sfem], format="csr")
+    sizeM = S.shape[0]
+    ncon = int(rng.integers(10, 201))
+    ncon = min(ncon, len(nodes))
+    fem_nodes = rng.choice(len(nodes), size=ncon, replace=False)
+    rows, cols, vals = [], [], []
+    centers = rng.uniform(-0.1, 0.1, size=(nb, 3))
+    for q in range(ncon):
+        nrm = rng.standard_normal(3)
+        nrm /= np.linalg.norm(nrm)
+        b = int(rng.integers(0, nb))
+        pnt = centers[b] + rng.uniform(-0.05, 0.05, 3)
+        ang = np.cross(pnt - centers[b], nrm)
+        for d in range(3):
+            rows.append(6 * b + d); cols.append(q); vals.append(nrm[d])
+            rows.append(6 * b + 3 + d); cols.append(q); vals.append(ang[d])
+            rows.append(n_rigid + 3 * fem_nodes[q] + d); cols.append(q); vals.append(-nrm[d])
+    GT = sp.csr_matrix((vals, (rows, cols)), shape=(sizeM, ncon))
+    bm = rng.uniform(-1, 1, sizeM)
+    bg = np.zeros(ncon)
+    return S, GT, bm, bg
+
+
+def batch_mix(nsys=256, seed=0xBA7C, dof_lo=300, dof_hi=4000):
+    """nsys independent KKT systems with DOF ~ U[dof_lo, dof_hi]."""
+    rng = np.random.default_rng(seed)
+    systems = []
+    for s in range(nsys):
+        ndof = int(rng.integers(dof_lo, dof_hi + 1))
+        S, GT, bm, bg = rigid_contact_system(ndof, seed * 1000 + s)
+        ro, ci, va = kkt_upper_csr(S, GT, None)
+        systems.append((ro, ci, va, np.concatenate([bm, bg])))
+    return systems
+
+
+def block_diag_csr(systems):
+    """Concatenates upper-CSR systems (rowOffs, colIdxs, vals, b) into one block-
+    diagonal upper CSR; returns (rowOffs, colIdxs, vals, b, row_offsets_per_system)."""
+    ros, cis, vas, bs, offs = [], [], [], [], [0]
+    nnz_off = 0
+    for ro, ci, va, b in systems:
+        n = len(ro) - 1
+        ros.append(ro[:-1] + nnz_off)
+        cis.append(ci + offs[-1])
+        vas.append(va)
+        bs.append(b)
+        nnz_off += int(ro[-1])
+        offs.append(offs[-1] + n)
+    ro = np.concatenate(ros + [np.array([nnz_off])]).astype(np.int32)
+    return (ro, np.concatenate(cis).astype(np.int32), np.concatenate(vas), np.concatenate(bs),
+            np.array(offs, np.int64))

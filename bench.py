@@ -77,8 +77,10 @@ def main():
     ap.add_argument("--nsys", type=int, default=20, help="systems in the step sequence")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--grid", default="100x16x16")
-    ap.add_argument("--mode", default="bench", choices=["bench", "pmc"],
-                    help="pmc: one FP32 factor + solve and a 256 MiB copy, for counter runs")
+    ap.add_argument("--mode", default="bench", choices=["bench", "pmc", "batch"],
+                    help="pmc: one FP32 factor + solve and a 256 MiB copy, for counter runs; "
+                         "batch: the 256-system mix, one batched launch vs per-system launches")
+    ap.add_argument("--nbatch", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -96,6 +98,8 @@ def main():
     nx, ny, nz = (int(v) for v in args.grid.split("x"))
     if args.mode == "pmc":
         return pmc_run(nx, ny, nz)
+    if args.mode == "batch":
+        return batch_run(args.nbatch, args.steps, args.warmup)
     t0 = time.time()
     beam, V, B = G.large_beam_sequence(nsteps=args.nsys, nx=nx, ny=ny, nz=nz,
                                        width=0.16 * ny / 16)
@@ -227,6 +231,95 @@ def pmc_run(nx, ny, nz):
     print(json.dumps({"mode": "pmc", "copy_bytes": 256 << 20, "n": beam.n,
                       "info": s.info()}), flush=True)
     s.dispose()
+
+
+def batch_run(nsys, reps, warmup):
+    """nsys independent KKT systems (DOF ~ U[300,4000], rigid bodies + tet
+    blocks + contact equalities, FP64).  One batched factor + solve (one captured graph
+    each) against the same systems factored and solved one handle at a time."""
+    import torch
+    from artisynth_core_amd import KKT_PREC_FP64
+    from artisynth_core_amd import generators as G
+    from artisynth_core_amd._lib import lib
+    from artisynth_core_amd._lib import ptr as nptr
+    from artisynth_core_amd.batch import BatchedKKT
+    t0 = time.time()
+    systems = G.batch_mix(nsys)
+    log("generated %d systems (n %d..%d) in %.1fs" % (nsys, min(len(s[0]) - 1 for s in systems),
+                                                    max(len(s[0]) - 1 for s in systems), time.time() - t0))
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    B = BatchedKKT(systems, precision=KKT_PREC_FP64, ir_steps=1)
+    t_an = time.time() - t0
+    dv = torch.from_numpy(B.vals).to(dev)
+    db = torch.from_numpy(B.b).to(dev)
+    dx = torch.zeros(B.n, dtype=torch.float64, device=dev)
+    for _ in range(max(1, warmup)):
+        B.factor_device(dv.data_ptr())
+        B.solve_device(db.data_ptr(), dx.data_ptr())
+    torch.cuda.synchronize()
+    lat = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        B.factor_device(dv.data_ptr())
+        B.solve_device(db.data_ptr(), dx.data_ptr())
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t0) * 1e3)
+    xb = dx.cpu().numpy()
+    res = B.per_system_residuals(xb, systems)
+    # per-system launches
+    L = lib()
+    hs, dvs, dbs, dxs = [], [], [], []
+    for ro, ci, va, b in systems:
+        h = L.kkt_create()
+        L.kkt_set_precision(h, KKT_PREC_FP64)
+        L.kkt_set_ir_steps(h, 1)
+        ro32 = np.ascontiguousarray(ro, np.int32)
+        ci32 = np.ascontiguousarray(ci, np.int32)
+        va64 = np.ascontiguousarray(va, np.float64)
+        assert L.kkt_set_pattern(h, len(ro) - 1, int(ro[-1]), nptr(ro32, ctypes.c_int),
+                                 nptr(ci32, ctypes.c_int), 1) == 0
+        assert L.kkt_analyze_values(h, nptr(va64, ctypes.c_double)) == 0
+        hs.append(h)
+        dvs.append(torch.from_numpy(va64).to(dev))
+        dbs.append(torch.from_numpy(np.ascontiguousarray(b, np.float64)).to(dev))
+        dxs.append(torch.zeros(len(b), dtype=torch.float64, device=dev))
+    for q, h in enumerate(hs):
+        assert L.kkt_factor_device(h, dvs[q].data_ptr()) == 0
+        assert L.kkt_solve_device(h, dbs[q].data_ptr(), dxs[q].data_ptr()) == 0
+    torch.cuda.synchronize()
+    lat1 = []
+    for _ in range(max(1, reps // 4)):
+        t0 = time.perf_counter()
+        for q, h in enumerate(hs):
+            L.kkt_factor_device(h, dvs[q].data_ptr())
+            L.kkt_solve_device(h, dbs[q].data_ptr(), dxs[q].data_ptr())
+        torch.cuda.synchronize()
+        lat1.append((time.perf_counter() - t0) * 1e3)
+    # same result alone and in the batch
+    diff = 0.0
+    for q in range(nsys):
+        xa = dxs[q].cpu().numpy()
+        xs = xb[B.offs[q]:B.offs[q + 1]]
+        diff = max(diff, float(np.linalg.norm(xa - xs) / max(np.linalg.norm(xa), 1e-300)))
+    for h in hs:
+        L.kkt_destroy(h)
+    from artisynth_core_amd._lib import KktInfo
+    inf = KktInfo()
+    L.kkt_get_info(B.h, ctypes.byref(inf))
+    t_b, t_1 = float(np.median(lat)), float(np.median(lat1))
+    line = {"mode": "batch", "nsys": nsys, "n_total": B.n, "nnz_total": B.nnz,
+            "batched_ms": t_b, "batched_p99_ms": float(np.percentile(lat, 99)),
+            "per_system_ms": t_1, "speedup": t_1 / t_b,
+            "batched_systems_per_s": nsys / (t_b * 1e-3),
+            "per_system_systems_per_s": nsys / (t_1 * 1e-3),
+            "max_rel_residual": max(res), "max_diff_alone_vs_batch": diff,
+            "factor_gflop": inf.factor_flops / 1e9, "nlevels": inf.nlevels,
+            "analyze_s": t_an, "dtype": "f64"}
+    log("batch: %.2f ms per batch of %d vs %.2f ms one by one (%.1fx), max resid %.2e" % (
+        t_b, nsys, t_1, t_1 / t_b, max(res)))
+    print(json.dumps(line), flush=True)
+    B.dispose()
 
 
 def cpu_baseline(beam, vals, b, x64, x32):

@@ -126,6 +126,9 @@ void kkt_set_timing_enabled(int on);
 int kkt_set_precision(kkt_solver* h, int prec);
 /* relative-residual target of the FP64 refinement of FP32 factors (default 1e-10) */
 int kkt_set_refine_tol(kkt_solver* h, double tol, int max_steps);
+/* 1 (default): the factor and triangular-solve launch sequences are captured once
+ * per analysis and precision into HIP graphs and replayed; 0: launched eagerly. */
+int kkt_set_graphs(kkt_solver* h, int on);
 /* ordering: 0 natural, 1 nested dissection (default) */
 int kkt_set_ordering(kkt_solver* h, int ordering);
 int kkt_get_info(kkt_solver* h, kkt_info* out);
