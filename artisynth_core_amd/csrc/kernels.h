@@ -90,6 +90,20 @@ void launch_axpby(int n, double alpha, const double* x, double beta, double* y, 
 void launch_axpbypcz(int n, double a, const double* x, double b, const double* y, double c,
                      double* z, hipStream_t st);
 
+// dependency-driven triangular solve (one persistent launch + a counter reset)
+struct DagCtl {
+  int* ctr;           // counter block: head, err, then 7 arrays of nsuper ints
+  const int* tchild;  // per supernode: number of u-row tasks of all its children
+  const int* nbx;     // per supernode: number of X^T partial tasks
+  const int* sparent; // assembly-tree parent
+  int nsuper;
+  int ntasks;
+};
+template <typename T>
+void launch_solve_dag(const int4* tasks, int nwg, const DagCtl& c, const DevSym& d, const T* F,
+                      const T* bp, T* fp, T* y, T* u, T* v, T* xp, T* part, const long long* poff,
+                      hipStream_t st);
+
 // BiCGStab updates with device-resident scalars (see kernels.hip for the slots)
 void launch_bicg_p(int n, const double* sc, const double* r, const double* v, double* p, hipStream_t st);
 void launch_bicg_s(int n, const double* sc, const double* r, const double* v, double* s_, hipStream_t st);

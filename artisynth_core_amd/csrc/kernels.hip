@@ -721,6 +721,203 @@ __global__ __launch_bounds__(256) void k_bwd_finish(const int* __restrict__ item
 }
 
 // ------------------------------------------------------------------------
+// Dependency-driven triangular solve: one persistent launch runs the whole
+// forward + backward sweep as a task list in topological order.  A workgroup
+// claims the next task index, waits (bounded) until the counters of the tasks
+// it depends on reach their targets, runs the task and bumps its own counter.
+// Every dependency of task t has a smaller index, and a claimed task belongs to
+// a running workgroup, so the oldest unfinished task can always proceed.
+// A wait that exceeds the spin bound sets *err and all workgroups drain.
+// Task kinds (int4 {kind, s, a, b}):
+//   0 FG   gather f of supernode s          needs child[s] >= tchild[s]
+//   1 FX   y rows 4a..4a+3                  needs fg[s] >= 1
+//   2 FU   u rows 32a..32a+31               needs fx[s] >= nfx(s); then child[parent]++
+//   3 BV   v columns 4a..4a+3               needs fx[s] >= nfx(s), fin[parent] >= 1
+//   4 BX   partial chunk a, outputs block b needs bv[s] >= nbv(s)
+//   5 BF   finish + scatter                 needs bx[s] >= nbx[s]
+// Counter block (ints): [0] head, [1] err, then 7 arrays of nsuper:
+//   child, fg, fx, bv, bx, fin, (spare)
+// ------------------------------------------------------------------------
+__device__ inline bool dag_wait(const int* p, int target, const int* err) {
+  for (int it = 0; it < 400000; it++) {
+    if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+    __builtin_amdgcn_s_sleep(2);
+  }
+  return false;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_solve_dag(const int4* __restrict__ tasks, DagCtl c,
+                                                   DevSym d, const T* __restrict__ F,
+                                                   const T* __restrict__ bp, T* __restrict__ fp,
+                                                   T* __restrict__ y, T* __restrict__ u,
+                                                   T* __restrict__ v, T* __restrict__ xp,
+                                                   T* __restrict__ part,
+                                                   const long long* __restrict__ poff) {
+  __shared__ T sh[KMAX];
+  __shared__ T prt[8][33];
+  __shared__ int s_task, s_ok;
+  int* head = c.ctr;
+  int* err = c.ctr + 1;
+  int* child = c.ctr + 2;
+  int* fg = child + c.nsuper;
+  int* fx = fg + c.nsuper;
+  int* bv = fx + c.nsuper;
+  int* bx = bv + c.nsuper;
+  int* fin = bx + c.nsuper;
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  for (;;) {
+    if (tid == 0) s_task = __hip_atomic_fetch_add(head, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int t = s_task;
+    if (t >= c.ntasks) break;
+    const int4 tk = tasks[t];
+    const int kind = tk.x, s = tk.y, a = tk.z, b = tk.w;
+    const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s], r = m - k;
+    const int par = c.sparent[s];
+    const int nfx = (k + 3) / 4;
+    if (tid == 0) {
+      bool ok = true;
+      if (kind == 0) ok = dag_wait(child + s, c.tchild[s], err);
+      else if (kind == 1) ok = dag_wait(fg + s, 1, err);
+      else if (kind == 2) ok = dag_wait(fx + s, nfx, err);
+      else if (kind == 3) ok = dag_wait(fx + s, nfx, err) && (par < 0 || dag_wait(fin + par, 1, err));
+      else if (kind == 4) ok = dag_wait(bv + s, nfx, err);
+      else ok = dag_wait(bx + s, c.nbx[s], err);
+      if (!ok) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_ok = ok ? 1 : 0;
+    }
+    __syncthreads();
+    if (!s_ok) break;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const T* Fs = F + d.sn_foff[s];
+    if (kind == 0) {
+      T* us = u + d.sn_uoff[s];
+      for (int q = tid; q < k; q += 256) sh[q] = bp[first + q];
+      for (int i = tid; i < r; i += 256) us[i] = (T)0;
+      __syncthreads();
+      for (int q = d.ch_ptr[s]; q < d.ch_ptr[s + 1]; q++) {
+        const int ch = d.ch_list[q];
+        const int rc = d.sn_m[ch] - d.sn_k[ch];
+        const int* relc = d.rel + d.sn_roff[ch];
+        const T* uc = u + d.sn_uoff[ch];
+        for (int i = tid; i < rc; i += 256) {
+          const int pos = relc[i];
+          const T val = uc[i];
+          if (pos < k) sh[pos] += val; else us[pos - k] += val;
+        }
+        __syncthreads();
+      }
+      for (int q = tid; q < k; q += 256) fp[first + q] = sh[d.lperm[first + q]];
+    } else if (kind == 1) {
+      const int row = a * 4 + wv;
+      if (row < k) {
+        const T* col = Fs + (size_t)row * m;
+        const T* f = fp + first;
+        T acc = (T)0;
+        for (int l = lane; l < row; l += 64) acc += col[l] * f[l];
+        acc = wave_sum(acc);
+        if (lane == 0) y[first + row] = f[row] + acc;
+      }
+    } else if (kind == 2) {
+      for (int q = tid; q < k; q += 256) sh[q] = y[first + q];
+      __syncthreads();
+      const int g = tid >> 5, lr = tid & 31;
+      const int i = a * 32 + lr;
+      T a0 = (T)0, a1 = (T)0, a2 = (T)0, a3 = (T)0;
+      if (i < r) {
+        const T* col = Fs + k + i;
+        int q = g;
+        for (; q + 24 < k; q += 32) {
+          a0 += col[(size_t)q * m] * sh[q];
+          a1 += col[(size_t)(q + 8) * m] * sh[q + 8];
+          a2 += col[(size_t)(q + 16) * m] * sh[q + 16];
+          a3 += col[(size_t)(q + 24) * m] * sh[q + 24];
+        }
+        for (; q < k; q += 8) a0 += col[(size_t)q * m] * sh[q];
+      }
+      prt[g][lr] = (a0 + a1) + (a2 + a3);
+      __syncthreads();
+      if (tid < 32 && i < r) {
+        T acc = (T)0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) acc += prt[q][lr];
+        u[d.sn_uoff[s] + i] -= acc;
+      }
+    } else if (kind == 3) {
+      const int q = a * 4 + wv;
+      if (q < k) {
+        const T* col = Fs + (size_t)q * m + k;
+        const int* rows = d.rows + d.sn_roff[s];
+        T acc = (T)0;
+        for (int i = lane; i < r; i += 64) acc += col[i] * xp[rows[i]];
+        acc = wave_sum(acc);
+        if (lane == 0) {
+          const int pq = d.piv[first + q];
+          T z;
+          if (pq == 1) {
+            z = y[first + q] / Fs[(size_t)q * m + q];
+          } else if (pq == 2) {
+            const T d11 = Fs[(size_t)q * m + q], d21 = Fs[(size_t)q * m + q + 1],
+                    d22 = Fs[(size_t)(q + 1) * m + q + 1];
+            const T det = d11 * d22 - d21 * d21;
+            z = (y[first + q] * d22 - y[first + q + 1] * d21) / det;
+          } else {
+            const T d11 = Fs[(size_t)(q - 1) * m + q - 1], d21 = Fs[(size_t)(q - 1) * m + q],
+                    d22 = Fs[(size_t)q * m + q];
+            const T det = d11 * d22 - d21 * d21;
+            z = (y[first + q] * d11 - y[first + q - 1] * d21) / det;
+          }
+          v[first + q] = z - acc;
+        }
+      }
+    } else if (kind == 4) {
+      const int i0 = a * 64, i1 = min(k, i0 + 64);
+      if (tid < 64) sh[tid] = (i0 + tid < k) ? v[first + i0 + tid] : (T)0;
+      __syncthreads();
+      const int l = b * 256 + tid;
+      if (l < i1) {
+        T a0 = (T)0, a1 = (T)0;
+        int i = max(i0, l + 1);
+        for (; i + 1 < i1; i += 2) {
+          a0 += Fs[(size_t)i * m + l] * sh[i - i0];
+          a1 += Fs[(size_t)(i + 1) * m + l] * sh[i + 1 - i0];
+        }
+        if (i < i1) a0 += Fs[(size_t)i * m + l] * sh[i - i0];
+        part[poff[s] + (size_t)a * k + l] = a0 + a1;
+      }
+    } else {
+      const int nc = (k + 63) / 64;
+      const T* ps = part + poff[s];
+      for (int l = tid; l < k; l += 256) {
+        T x = v[first + l];
+        for (int cc = l / 64; cc < nc; cc++) x += ps[(size_t)cc * k + l];
+        xp[first + d.lperm[first + l]] = x;
+      }
+    }
+    // publish: every wave releases its stores, then one thread bumps the counter
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (tid == 0) {
+      int* cnt = nullptr;
+      if (kind == 0) cnt = fg + s;
+      else if (kind == 1) cnt = fx + s;
+      else if (kind == 2) cnt = (par >= 0) ? child + par : nullptr;
+      else if (kind == 3) cnt = bv + s;
+      else if (kind == 4) cnt = bx + s;
+      else cnt = fin + s;
+      if (cnt) __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+__global__ void k_zero_ints(long long n, int* __restrict__ p) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// ------------------------------------------------------------------------
 // vectors and SpMV
 // ------------------------------------------------------------------------
 template <typename TI, typename TO>
@@ -962,6 +1159,22 @@ void launch_axpbypcz(int n, double a, const double* x, double b, const double* y
                      double* z, hipStream_t st) {
   hipLaunchKernelGGL(k_axpbypcz, dim3(nblk(n, 256)), dim3(256), 0, st, n, a, x, b, y, c, z);
 }
+
+template <typename T>
+void launch_solve_dag(const int4* tasks, int nwg, const DagCtl& c, const DevSym& d, const T* F,
+                      const T* bp, T* fp, T* y, T* u, T* v, T* xp, T* part, const long long* poff,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_ints, dim3(nblk(2 + 7LL * c.nsuper, 256)), dim3(256), 0, st,
+                     (long long)(2 + 7LL * c.nsuper), c.ctr);
+  hipLaunchKernelGGL(k_solve_dag<T>, dim3(nwg), dim3(256), 0, st, tasks, c, d, F, bp, fp, y, u, v,
+                     xp, part, poff);
+}
+template void launch_solve_dag<double>(const int4*, int, const DagCtl&, const DevSym&, const double*,
+                                       const double*, double*, double*, double*, double*, double*,
+                                       double*, const long long*, hipStream_t);
+template void launch_solve_dag<float>(const int4*, int, const DagCtl&, const DevSym&, const float*,
+                                      const float*, float*, float*, float*, float*, float*, float*,
+                                      const long long*, hipStream_t);
 
 void launch_bicg_p(int n, const double* sc, const double* r, const double* v, double* p, hipStream_t st) {
   hipLaunchKernelGGL(k_bicg_p, dim3(nblk(n, 256)), dim3(256), 0, st, n, sc, r, v, p);

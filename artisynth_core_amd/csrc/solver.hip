@@ -85,6 +85,11 @@ struct kkt_solver {
   int *d_diag = nullptr, *d_lvlsn = nullptr, *d_aidx = nullptr;
   int2 *d_trsm = nullptr, *d_fg = nullptr, *d_bg = nullptr, *d_xrow = nullptr, *d_xg = nullptr;
   int4* d_xp = nullptr;
+  // dependency-driven solve: task list, counters, per-supernode targets
+  int4* d_dag = nullptr;
+  int *d_dagctr = nullptr, *d_tchild = nullptr, *d_nbx = nullptr, *d_spar = nullptr;
+  int ndag = 0;
+  bool dag_on = true;
   std::vector<Launch> xrow_l;  // inverse-row post-pass, one launch per panel index p >= 1
   long long *d_apos = nullptr, *d_poff = nullptr;
   std::vector<Launch> asm_l, scat_l, sn_l, fg_l, bg_l, xg_l, xp_l;
@@ -497,6 +502,41 @@ struct kkt_solver {
         sp.upd.cnt = (int)(updv.size() - sp.upd.off);
       }
     }
+    // task list of the dependency-driven solve (topological order: forward by
+    // increasing level, then backward by decreasing level)
+    {
+      std::vector<int4> dag;
+      std::vector<int> tchild(ns, 0), nbxv(ns, 0);
+      for (int s = 0; s < ns; s++) {
+        int r = mm[s] - kk[s];
+        int p = S.sparent[s];
+        if (p >= 0) tchild[p] += (r + 31) / 32;
+        int cnt = 0;
+        for (int c = 0; c < (kk[s] + 63) / 64; c++) cnt += (std::min(kk[s], c * 64 + 64) + 255) / 256;
+        nbxv[s] = cnt;
+      }
+      for (int L = 0; L < nl; L++)
+        for (int s : lv[L]) {
+          dag.push_back(make_int4(0, s, 0, 0));
+          for (int a = 0; a < (kk[s] + 3) / 4; a++) dag.push_back(make_int4(1, s, a, 0));
+          for (int a = 0; a < (mm[s] - kk[s] + 31) / 32; a++) dag.push_back(make_int4(2, s, a, 0));
+        }
+      for (int L = nl - 1; L >= 0; L--)
+        for (int s : lv[L]) {
+          for (int a = 0; a < (kk[s] + 3) / 4; a++) dag.push_back(make_int4(3, s, a, 0));
+          for (int c = 0; c < (kk[s] + 63) / 64; c++) {
+            int i1 = std::min(kk[s], c * 64 + 64);
+            for (int lb = 0; lb < (i1 + 255) / 256; lb++) dag.push_back(make_int4(4, s, c, lb));
+          }
+          dag.push_back(make_int4(5, s, 0, 0));
+        }
+      ndag = (int)dag.size();
+      std::vector<int> zero(2 + 7 * (size_t)ns, 0);
+      int rc2;
+      if ((rc2 = upload(d_dag, dag)) || (rc2 = upload(d_dagctr, zero)) || (rc2 = upload(d_tchild, tchild)) ||
+          (rc2 = upload(d_nbx, nbxv)) || (rc2 = upload(d_spar, S.sparent)))
+        return rc2;
+    }
     // inverse rows X(P, :) of every pivot block, after the whole factorization:
     // launch p covers block row p of all supernodes at once
     {
@@ -704,6 +744,23 @@ struct kkt_solver {
     tbeg(TC_VEC);
     launch_gather_perm<double, T>(n, d_perm, d_b, bp, st);
     tend();
+    if (dag_on) {
+      DagCtl c;
+      c.ctr = d_dagctr;
+      c.tchild = d_tchild;
+      c.nbx = d_nbx;
+      c.sparent = d_spar;
+      c.nsuper = S.nsuper;
+      c.ntasks = ndag;
+      tbeg(TC_SLEVEL);
+      launch_solve_dag<T>(d_dag, std::max(1, std::min(ndag, 1024)), c, ds, F, bp, fp, y, u, v, xp, part,
+                          d_poff, st);
+      tend();
+      tbeg(TC_VEC);
+      launch_scatter_perm<T, double>(n, d_perm, xp, d_x, st);
+      tend();
+      return;
+    }
     const int nl = S.nlevels;
     for (int L = 0; L < nl; L++) {
       tbeg(TC_SLEVEL);
@@ -820,10 +877,24 @@ struct kkt_solver {
     }
     if (hipStreamSynchronize(st) != hipSuccess) return fail(KKT_ERR_SOLVE, "solve kernels failed");
     tflush();
+    if (dag_failed()) return solve_device(d_b, d_x);
     if (g_timing)
       fprintf(stderr, "[kkt-timing] solve n=%d: %.3f ms (refine steps %d, rel resid %.3g)\n", n,
               now_ms() - t0, last_refine_steps, last_relres);
     return KKT_OK;
+  }
+
+  // the dependency-driven solve flags a wait that exceeded its bound; the
+  // level-scheduled launches then take over for this handle
+  bool dag_failed() {
+    if (!dag_on || !d_dagctr) return false;
+    int e = 0;
+    if (hipMemcpy(&e, d_dagctr + 1, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return false;
+    if (e == 0) return false;
+    dag_on = false;
+    drop_graphs();
+    if (g_timing) fprintf(stderr, "[kkt-timing] solve task graph timed out; level launches enabled\n");
+    return true;
   }
 
   int set_values_device(const double* dv) {
@@ -914,6 +985,7 @@ struct kkt_solver {
         hipStreamSynchronize(st) != hipSuccess)
       return fail(KKT_ERR_SOLVE, "iterative solve failed");
     tflush();
+    if (dag_failed()) return bicgstab(d_b, d_xout, tolRel, maxIter, iters);
     *iters = it;
     if (g_timing)
       fprintf(stderr, "[kkt-timing] bicgstab n=%d iters=%d: total=%.2fms (%.2fms/iter)\n", n, it,
@@ -1105,6 +1177,15 @@ int kkt_set_refine_tol(kkt_solver* h, double tol, int max_steps) {
   if (!(tol > 0) || max_steps < 1) return h->fail(KKT_ERR_ARGUMENT, "bad refinement settings");
   h->refine_tol = tol;
   h->refine_max = max_steps;
+  h->ok();
+  return KKT_OK;
+}
+
+int kkt_set_solve_mode(kkt_solver* h, int mode) {
+  CHECK_H(h);
+  if (mode != 0 && mode != 1) return h->fail(KKT_ERR_ARGUMENT, "unknown solve mode");
+  h->dag_on = (mode == 1);
+  if (h->dev_ok) h->drop_graphs();
   h->ok();
   return KKT_OK;
 }

@@ -435,3 +435,66 @@ def block_diag_csr(systems):
     ro = np.concatenate(ros + [np.array([nnz_off])]).astype(np.int32)
     return (ro, np.concatenate(cis).astype(np.int32), np.concatenate(vas), np.concatenate(bs),
             np.array(offs, np.int64))
+
+
+def _beam_dims(ndof, aspect=5):
+    """Hex grid (nx, ny, nz) whose node count * 3 is close to ndof, aspect nx ~ 5 ny."""
+    nodes = max(8, ndof // 3)
+    a = max(1, int(round((nodes / aspect) ** (1.0 / 3.0))) - 1)
+    dims = [max(1, aspect * (a + 1) - 1), a, a]
+    while (dims[0] + 1) * (dims[1] + 1) * (dims[2] + 1) < nodes:
+        dims[0] += 1
+    return dims
+
+
+def serving_instance(kind, ndof, seed):
+    """One simulation instance of the serving mix: upper-CSR pattern, base values
+    and a right-hand side.  kind: 'beam' (linear hex beam, attached end), 'tet'
+    (linear tet box, attached end) or 'rigid' (rigid bodies + tet block + contacts)."""
+    rng = np.random.default_rng(seed)
+    if kind == "rigid":
+        S, GT, bm, bg = rigid_contact_system(min(ndof, 8000), seed)
+        ro, ci, va = kkt_upper_csr(S, GT, None)
+        return ro, ci, va, np.concatenate([bm, bg])
+    if kind == "beam":
+        nx, ny, nz = _beam_dims(ndof)
+        beam = HexBeam(nx, ny, nz, 1.0, 0.2 * ny / max(nx, 1) * 5, material=("linear", 1e5, 0.33),
+                       h=0.01)
+        fext = beam.gravity_loads() + rng.uniform(-1, 1, (beam.nn, 3))
+        u = np.zeros((beam.nn, 3))
+        vals, b = beam.system(u, u, fext)
+        return beam.rowOffs, beam.colIdxs, vals, b
+    # tet box with its x = 0 face attached by point constraints
+    nx, ny, nz = _beam_dims(ndof)
+    nodes, tets = tet_grid(nx, ny, nz, 0.01 * nx, 0.01 * ny, 0.01 * nz)
+    lam, mu = lame(1e5, 0.3)
+    K, m = assemble_tet(nodes, tets, lam, mu, 1000.0)
+    h = 0.01
+    S = sp.diags(m) + (h * 0.002 + h * h) * K
+    att = np.nonzero(np.isclose(nodes[:, 0], 0.0))[0]
+    nG = 3 * len(att)
+    rows = (3 * att[:, None] + np.arange(3)[None]).ravel()
+    GT = sp.csr_matrix((np.ones(nG), (rows, np.arange(nG))), shape=(S.shape[0], nG))
+    ro, ci, va = kkt_upper_csr(S, GT, None)
+    b = np.concatenate([rng.uniform(-1, 1, S.shape[0]) * 1e-2, np.zeros(nG)])
+    return ro, ci, va, b
+
+
+def serving_mix(ninst=64, seed=0x5E7E, dof_lo=1000, dof_hi=90000):
+    """ninst instances, DOF log-uniform in [dof_lo, dof_hi], kinds cycling
+    beam / tet / rigid."""
+    rng = np.random.default_rng(seed)
+    out = []
+    kinds = ("beam", "tet", "rigid")
+    for i in range(ninst):
+        ndof = int(np.exp(rng.uniform(np.log(dof_lo), np.log(dof_hi))))
+        kind = kinds[i % 3]
+        out.append((kind, ndof) + serving_instance(kind, ndof, seed * 100 + i))
+    return out
+
+
+def perturbed_values(vals, step, seed, eps=1e-3):
+    """Values of a later step of an instance: seeded relative perturbation of every
+    entry (zero entries stay zero, the pattern and symmetry are unchanged)."""
+    rng = np.random.default_rng(seed * 7919 + step)
+    return vals * (1.0 + eps * rng.uniform(-1, 1, len(vals)))

@@ -77,10 +77,15 @@ def main():
     ap.add_argument("--nsys", type=int, default=20, help="systems in the step sequence")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--grid", default="100x16x16")
-    ap.add_argument("--mode", default="bench", choices=["bench", "pmc", "batch"],
+    ap.add_argument("--mode", default="bench", choices=["bench", "pmc", "batch", "serve"],
                     help="pmc: one FP32 factor + solve and a 256 MiB copy, for counter runs; "
                          "batch: the 256-system mix, one batched launch vs per-system launches")
     ap.add_argument("--nbatch", type=int, default=256)
+    ap.add_argument("--ninst", type=int, default=64)
+    ap.add_argument("--nreq", type=int, default=2000)
+    ap.add_argument("--rate", type=float, default=2000.0, help="serving: offered requests/s")
+    ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -100,6 +105,8 @@ def main():
         return pmc_run(nx, ny, nz)
     if args.mode == "batch":
         return batch_run(args.nbatch, args.steps, args.warmup)
+    if args.mode == "serve":
+        return serve_run(args.ninst, args.nreq, args.rate, args.workers, args.budget_gb)
     t0 = time.time()
     beam, V, B = G.large_beam_sequence(nsteps=args.nsys, nx=nx, ny=ny, nz=nz,
                                        width=0.16 * ny / 16)
@@ -320,6 +327,73 @@ def batch_run(nsys, reps, warmup):
         t_b, nsys, t_1, t_1 / t_b, max(res)))
     print(json.dumps(line), flush=True)
     B.dispose()
+
+
+def serve_run(ninst, nreq, rate, nworkers, budget_gb):
+    """ninst instances (1k-90k DOF, beam/tet/rigid), a Poisson stream of
+    step requests at `rate` req/s, HBM-budget admission, FP32-MFMA + FP64 refinement
+    (1e-10 relative residual), p50/p99 latency from submission to completion."""
+    import scipy.sparse as sp
+    from artisynth_core_amd import generators as G
+    from artisynth_core_amd.serving import SolveServer, latency_stats
+    t0 = time.time()
+    mix = G.serving_mix(ninst)
+    log("generated %d instances (n %d..%d, total %d) in %.1fs" % (
+        ninst, min(len(m[2]) - 1 for m in mix), max(len(m[2]) - 1 for m in mix),
+        sum(len(m[2]) - 1 for m in mix), time.time() - t0))
+    srv = SolveServer(budget_bytes=int(budget_gb * (1 << 30)), nworkers=nworkers)
+    for i, (kind, nd, ro, ci, va, b) in enumerate(mix):
+        srv.register(i, ro, ci, va)
+    t0 = time.time()
+    warm = [srv.submit(i, mix[i][4], mix[i][5]) for i in range(ninst)]
+    for r in warm:
+        r.done.wait()
+    errs = [r.error for r in warm if r.error]
+    assert not errs, errs[:3]
+    log("admitted + first factor of %d instances in %.1fs (%.1f GB resident)" % (
+        ninst, time.time() - t0, srv.used / 2 ** 30))
+    rng = np.random.default_rng(2000)
+    gaps = rng.exponential(1.0 / rate, nreq)
+    who = rng.integers(0, ninst, nreq)
+    step = np.zeros(ninst, np.int64)
+    reqs = []
+    t_start = time.perf_counter()
+    t_next = t_start
+    for q in range(nreq):
+        t_next += gaps[q]
+        now = time.perf_counter()
+        if t_next > now:
+            time.sleep(t_next - now)
+        i = int(who[q])
+        step[i] += 1
+        kind, nd, ro, ci, va, b = mix[i]
+        reqs.append(srv.submit(i, G.perturbed_values(va, int(step[i]), i), b))
+    for r in reqs:
+        r.done.wait()
+    t_end = max(r.t_done for r in reqs)
+    errs = [r.error for r in reqs if r.error]
+    st = latency_stats(reqs)
+    # residual check on a sample of the results (host)
+    res = []
+    for r in reqs[:: max(1, nreq // 100)]:
+        kind, nd, ro, ci, va, b = mix[r.iid]
+        n = len(ro) - 1
+        U = sp.csr_matrix((r.vals, ci, ro), shape=(n, n))
+        A = U + sp.triu(U, 1).T
+        res.append(float(np.linalg.norm(A @ r.x - r.b) / np.linalg.norm(r.b)))
+    kinds = [r.kind for r in reqs]
+    line = {"mode": "serve", "ninst": ninst, "nreq": nreq, "offered_rate": rate,
+            "workers": nworkers, "budget_gb": budget_gb,
+            "throughput_req_s": nreq / (t_end - t_start), "p50_ms": st["p50_ms"],
+            "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"], "errors": len(errs),
+            "direct": kinds.count("direct"), "iterative": kinds.count("iterative"),
+            "max_rel_residual_sample": max(res), "resident_gb": srv.used / 2 ** 30,
+            "admissions": srv.admissions, "evictions": srv.evictions,
+            "total_dof": int(sum(len(m[2]) - 1 for m in mix))}
+    log("serve: %.0f req/s (offered %.0f), p50 %.2f ms, p99 %.2f ms, max resid %.2e" % (
+        line["throughput_req_s"], rate, st["p50_ms"], st["p99_ms"], max(res)))
+    print(json.dumps(line), flush=True)
+    srv.shutdown()
 
 
 def cpu_baseline(beam, vals, b, x64, x32):

@@ -129,6 +129,9 @@ int kkt_set_refine_tol(kkt_solver* h, double tol, int max_steps);
 /* 1 (default): the factor and triangular-solve launch sequences are captured once
  * per analysis and precision into HIP graphs and replayed; 0: launched eagerly. */
 int kkt_set_graphs(kkt_solver* h, int on);
+/* Triangular solves: 1 (default) one persistent dependency-driven launch per
+ * sweep; 0 level-by-level launches.  Both give bitwise-identical results. */
+int kkt_set_solve_mode(kkt_solver* h, int mode);
 /* ordering: 0 natural, 1 nested dissection (default) */
 int kkt_set_ordering(kkt_solver* h, int ordering);
 int kkt_get_info(kkt_solver* h, kkt_info* out);

@@ -213,3 +213,67 @@ def test_batched_systems_vs_oracle(gpu):
             assert rel_err(B.split(x)[q], xo) <= 1e-8
     finally:
         B.dispose()
+
+
+def test_serving_alone_equals_mix(gpu):
+    # each instance's results are the same alone and in a mix with eviction pressure
+    _say("serving alone vs mix")
+    from artisynth_core_amd.serving import SolveServer
+    mix = [("beam", 900), ("tet", 1200), ("rigid", 1500), ("beam", 2500)]
+    inst = [G.serving_instance(k, nd, 900 + i) for i, (k, nd) in enumerate(mix)]
+    steps = 3
+    seq = {i: [(G.perturbed_values(inst[i][2], s_, i), inst[i][3]) for s_ in range(steps)]
+           for i in range(len(inst))}
+    alone = {}
+    for i in range(len(inst)):
+        srv = SolveServer(nworkers=1)
+        srv.register(i, inst[i][0], inst[i][1], inst[i][2])
+        rs = [srv.submit(i, v, b) for v, b in seq[i]]
+        for r in rs:
+            r.done.wait()
+            assert r.error is None, r.error
+        alone[i] = [r.x for r in rs]
+        srv.shutdown()
+    srv = SolveServer(nworkers=3)
+    for i in range(len(inst)):
+        srv.register(i, inst[i][0], inst[i][1], inst[i][2])
+    rs = []
+    for s_ in range(steps):
+        for i in range(len(inst)):
+            rs.append((i, s_, srv.submit(i, *seq[i][s_])))
+    for i, s_, r in rs:
+        r.done.wait()
+        assert r.error is None, r.error
+        assert rel_err(r.x, alone[i][s_]) <= 1e-8
+        assert rel_residual(inst[i][0], inst[i][1], seq[i][s_][0], r.x, seq[i][s_][1]) <= 1e-8
+    srv.shutdown()
+
+
+@pytest.mark.parametrize("prec", [KKT_PREC_FP64, KKT_PREC_FP32_REFINE])
+def test_dependency_driven_solve_matches_level_solve(gpu, prec):
+    # the persistent task-queue sweep and the level-by-level launches compute the
+    # same arithmetic in the same order: bitwise-identical solutions
+    _say("dag vs level solve prec=%d" % prec)
+    import ctypes
+    from artisynth_core_amd._lib import ptr as nptr
+    beam = G.HexBeam(24, 8, 8, 1.0, 0.2, material=("neohookean", 3.4e4, 1.7e6), h=0.005)
+    u = np.random.default_rng(9).uniform(-1e-4, 1e-4, (beam.nn, 3))
+    vals, b = beam.system(u, np.zeros_like(u), beam.gravity_loads())
+    xs = []
+    for mode in (1, 0):
+        s = MI355XSolver(precision=prec)
+        try:
+            s._L.kkt_set_solve_mode(s._h, mode)
+            s.setIterativeRefinementSteps(0)
+            s.setRefinement(1e-10, 10)
+            s.analyzeArrays(vals, beam.colIdxs, beam.rowOffs, beam.n, Matrix.SYMMETRIC)
+            s.factor(vals)
+            x = np.zeros(beam.n)
+            s.solve(x, b)
+            it = s.iterativeSolve(vals * 1.01, np.zeros(beam.n), b, 10)
+            assert it > 0
+            xs.append(x)
+        finally:
+            s.dispose()
+    assert np.array_equal(xs[0], xs[1])
+    assert rel_residual(beam.rowOffs, beam.colIdxs, vals, xs[0], b) <= 1e-8
