@@ -39,6 +39,16 @@ __device__ inline T wave_sum(T v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
+// Lane 0's value in every lane.  Applied to values that are equal across the
+// wave (LDS broadcasts, wave max) so the compiler sees uniform branch and loop
+// conditions and emits scalar control flow around barriers.
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline double uni(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 
 __device__ inline int lower_bound_i(const int* a, int n, int key) {
   int lo = 0, hi = n;
@@ -126,10 +136,12 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
   int npos = 0, nneg = 0, npert = 0, n2 = 0;
   int kc = 0;
   while (kc < w) {
-    const double absakk = fabs(A[kc * 65 + kc]);
+    const double absakk = uni(fabs(A[kc * 65 + kc]));
     double cv = (t > kc && t < w) ? fabs(A[t * 65 + kc]) : -1.0;
     int ci = t;
     wave_argmax(cv, ci);
+    cv = uni(cv);
+    ci = uni(ci);
     const double colmax = cv > 0.0 ? cv : 0.0;
     const int imax = cv > 0.0 ? ci : -1;
     int kstep = 1, kp = kc;
@@ -142,9 +154,9 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
       double rv = 0.0;
       if (t >= kc && t < w && t != imax)
         rv = (t < imax) ? fabs(A[imax * 65 + t]) : fabs(A[t * 65 + imax]);
-      const double rowmax = wave_max(rv);
+      const double rowmax = uni(wave_max(rv));
       if (absakk * rowmax >= alpha * colmax * colmax) kp = kc;
-      else if (fabs(A[imax * 65 + imax]) >= alpha * rowmax) kp = imax;
+      else if (uni(fabs(A[imax * 65 + imax])) >= alpha * rowmax) kp = imax;
       else { kp = imax; kstep = 2; }
     }
     const int kk = kc + kstep - 1;
@@ -163,19 +175,28 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
     }
     __syncthreads();
     if (kstep == 1) {
-      double dd = A[kc * 65 + kc];
+      double dd = uni(A[kc * 65 + kc]);
       const bool pz = perturb || fabs(dd) < tau;
       if (pz) dd = (dd >= 0.0) ? tau : -tau;
       if (pz) npert++;
       if (dd > 0) npos++; else nneg++;
       const double dinv = 1.0 / dd;
       const double aik = (t > kc && t < w) ? A[t * 65 + kc] : 0.0;
-      colb[t] = aik * dinv;
+      colb[t] = aik;  // raw column; lanes >= w hold 0
       __syncthreads();
       if (t > kc && t < w) {
+        // whole trailing row, 8 entries per round trip: entries above the
+        // diagonal are scratch while the pivot loop runs
         double* row = A + t * 65;
-#pragma unroll 4
-        for (int j = kc + 1; j <= t; j++) row[j] -= aik * colb[j];
+        int j = kc + 1;
+        for (; j + 8 <= 64; j += 8) {
+          double rr[8], cc[8];
+#pragma unroll
+          for (int q = 0; q < 8; q++) { rr[q] = row[j + q]; cc[q] = colb[j + q]; }
+#pragma unroll
+          for (int q = 0; q < 8; q++) row[j + q] = rr[q] - aik * (cc[q] * dinv);
+        }
+        for (; j < 64; j++) row[j] -= aik * (colb[j] * dinv);
         row[kc] = aik * dinv;
       }
       if (t == 0) {
@@ -185,7 +206,7 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
       __syncthreads();
     } else {
       const int g1 = kc, g2 = kc + 1;
-      const double d11 = A[g1 * 65 + g1], d21 = A[g2 * 65 + g1], d22 = A[g2 * 65 + g2];
+      const double d11 = uni(A[g1 * 65 + g1]), d21 = uni(A[g2 * 65 + g1]), d22 = uni(A[g2 * 65 + g2]);
       const double det = d11 * d22 - d21 * d21;
       const double i11 = d22 / det, i21 = -d21 / det, i22 = d11 / det;
       if (det < 0) { npos++; nneg++; }
@@ -199,8 +220,15 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
       __syncthreads();
       if (act) {
         double* row = A + t * 65;
-#pragma unroll 4
-        for (int j = g2 + 1; j <= t; j++) row[j] -= a1 * colb[j] + a2 * colb2[j];
+        int j = g2 + 1;
+        for (; j + 8 <= 64; j += 8) {
+          double rr[8], c1[8], c2[8];
+#pragma unroll
+          for (int q = 0; q < 8; q++) { rr[q] = row[j + q]; c1[q] = colb[j + q]; c2[q] = colb2[j + q]; }
+#pragma unroll
+          for (int q = 0; q < 8; q++) row[j + q] = rr[q] - (a1 * c1[q] + a2 * c2[q]);
+        }
+        for (; j < 64; j++) row[j] -= a1 * colb[j] + a2 * colb2[j];
         row[g1] = colb[t];
         row[g2] = colb2[t];
       }
@@ -216,9 +244,9 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
   // rows j0..j0+w of the earlier panels' columns follow the interchanges (new row t
   // = old row lp[t]); one wave: every lane loads before any lane stores
   if (j0 > 0) {
-    bool moved = false;
+    int moved = 0;
     for (int q = 0; q < w; q++) moved |= (lp[q] != q);
-    if (moved) {
+    if (uni(moved)) {
       const int src = (t < w) ? lp[t] : t;
       int c = 0;
       for (; c + 4 <= j0; c += 4) {
@@ -247,18 +275,31 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
   // inverse of the unit lower L11 (entries (q+1,q) of 2x2 pivots hold D, not L):
   // lane c forms column c, stored above the diagonal as A[c][r] = Linv[r][c]
   const unsigned long long two = __ballot(t < w && pt[t] == 2);
+  // The 2x2 slots (q+1,q) are zeroed in a copy of row q+1 first.
+  if (t < w && t > 0 && ((two >> (t - 1)) & 1ULL)) colb[t] = A[t * 65 + t - 1];
+  __syncthreads();
+  if (t < w && t > 0 && ((two >> (t - 1)) & 1ULL)) A[t * 65 + t - 1] = 0.0;
+  __syncthreads();
   if (t < w) {
     const int c = t;
     for (int r = c + 1; r < w; r++) {
-      double acc = (r == c + 1 && ((two >> c) & 1ULL)) ? 0.0 : -A[r * 65 + c];
-#pragma unroll 4
-      for (int l = c + 1; l < r; l++) {
-        const double lrl = (l == r - 1 && ((two >> l) & 1ULL)) ? 0.0 : A[r * 65 + l];
-        acc -= lrl * A[c * 65 + l];
+      const double* Lr = A + r * 65;
+      const double* Xc = A + c * 65;
+      double acc0 = -Lr[c], acc1 = 0.0;
+      int l = c + 1;
+      for (; l + 8 <= r; l += 8) {
+        double lv[8], xv[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) { lv[q] = Lr[l + q]; xv[q] = Xc[l + q]; }
+#pragma unroll
+        for (int q = 0; q < 8; q += 2) { acc0 -= lv[q] * xv[q]; acc1 -= lv[q + 1] * xv[q + 1]; }
       }
-      A[c * 65 + r] = acc;
+      for (; l < r; l++) acc0 -= Lr[l] * Xc[l];
+      A[c * 65 + r] = acc0 + acc1;
     }
   }
+  __syncthreads();
+  if (t < w && t > 0 && ((two >> (t - 1)) & 1ULL)) A[t * 65 + t - 1] = colb[t];
   __syncthreads();
   // lower: L and D; upper (row t, column j0+j > j0+t): Linv[j][t]
   for (int j = 0; j < w; j++)
@@ -767,10 +808,14 @@ __global__ __launch_bounds__(256) void k_solve_dag(const int4* __restrict__ task
   int* bx = bv + c.nsuper;
   int* fin = bx + c.nsuper;
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  // Loop control is kept wave-uniform (readfirstlane) so that the compiler
+  // emits a uniform loop: with an LDS-read exit condition it treats the exit
+  // as divergent and may rotate the thread-0 claim out of the barrier loop,
+  // leaving the other threads re-running the previous task.
+  if (tid == 0) s_task = __hip_atomic_fetch_add(head, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
   for (;;) {
-    if (tid == 0) s_task = __hip_atomic_fetch_add(head, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const int t = s_task;
+    const int t = __builtin_amdgcn_readfirstlane(s_task);
     if (t >= c.ntasks) break;
     const int4 tk = tasks[t];
     const int kind = tk.x, s = tk.y, a = tk.z, b = tk.w;
@@ -789,7 +834,7 @@ __global__ __launch_bounds__(256) void k_solve_dag(const int4* __restrict__ task
       s_ok = ok ? 1 : 0;
     }
     __syncthreads();
-    if (!s_ok) break;
+    if (__builtin_amdgcn_readfirstlane(s_ok) == 0) break;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     const T* Fs = F + d.sn_foff[s];
     if (kind == 0) {
@@ -908,7 +953,9 @@ __global__ __launch_bounds__(256) void k_solve_dag(const int4* __restrict__ task
       else if (kind == 4) cnt = bx + s;
       else cnt = fin + s;
       if (cnt) __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      s_task = __hip_atomic_fetch_add(head, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    __syncthreads();
   }
 }
 

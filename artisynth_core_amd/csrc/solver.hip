@@ -12,6 +12,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
@@ -24,6 +26,18 @@ using namespace kkt;
 namespace {
 
 bool g_timing = false;
+
+// Handles may be driven from several host threads (the serving loop does).
+// Allocation, free and synchronous copies issued by one thread while another
+// thread has a stream capture open are rejected by HIP and can invalidate the
+// capture, so captures hold this lock exclusively and every allocation, free
+// and synchronous copy holds it shared.
+std::shared_mutex g_capture_mu;
+typedef std::shared_lock<std::shared_mutex> DevOpLock;
+
+std::string hip_msg(const char* what, hipError_t e) {
+  return std::string(what) + ": " + hipGetErrorString(e);
+}
 
 double now_ms() {
   return std::chrono::duration<double, std::milli>(
@@ -89,7 +103,7 @@ struct kkt_solver {
   int4* d_dag = nullptr;
   int *d_dagctr = nullptr, *d_tchild = nullptr, *d_nbx = nullptr, *d_spar = nullptr;
   int ndag = 0;
-  bool dag_on = true;
+  bool dag_on = false;
   std::vector<Launch> xrow_l;  // inverse-row post-pass, one launch per panel index p >= 1
   long long *d_apos = nullptr, *d_poff = nullptr;
   std::vector<Launch> asm_l, scat_l, sn_l, fg_l, bg_l, xg_l, xp_l;
@@ -123,6 +137,7 @@ struct kkt_solver {
   template <typename Fn>
   bool capture(hipGraphExec_t& exec, Fn fn) {
     hipGraph_t g = nullptr;
+    std::unique_lock<std::shared_mutex> lk(g_capture_mu);
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
     fn();
     if (hipStreamEndCapture(st, &g) != hipSuccess || !g ||
@@ -158,12 +173,14 @@ struct kkt_solver {
   void ok() { has_err = false; }
 
   void free_list(std::vector<void*>& v) {
+    DevOpLock lk(g_capture_mu);
     for (void* p : v)
       if (p) (void)hipFree(p);
     v.clear();
   }
   void release_numeric() {
     drop_graphs();
+    DevOpLock lk(g_capture_mu);
     if (d_F64) (void)hipFree(d_F64);
     if (d_F32) (void)hipFree(d_F32);
     if (d_W64) (void)hipFree(d_W64);
@@ -177,6 +194,7 @@ struct kkt_solver {
       if (st) (void)hipStreamSynchronize(st);
       free_list(allocs);
       release_numeric();
+      DevOpLock lk(g_capture_mu);
       if (d_rowptr_f) (void)hipFree(d_rowptr_f);
       if (d_col_f) (void)hipFree(d_col_f);
       if (d_map_f) (void)hipFree(d_map_f);
@@ -202,6 +220,7 @@ struct kkt_solver {
     int cnt = 0;
     if (hipGetDeviceCount(&cnt) != hipSuccess || cnt <= 0)
       return fail(KKT_ERR_DEVICE, "no HIP device available");
+    DevOpLock lk(g_capture_mu);
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
       return fail(KKT_ERR_INIT, "hipStreamCreate failed");
     if (hipMalloc(&d_partial, 1024 * sizeof(double)) != hipSuccess ||
@@ -215,10 +234,13 @@ struct kkt_solver {
   template <typename T>
   int upload(T*& dptr, const std::vector<T>& h) {
     size_t bytes = std::max<size_t>(h.size(), 1) * sizeof(T);
-    if (hipMalloc(&dptr, bytes) != hipSuccess) return fail(KKT_ERR_ALLOC, "hipMalloc failed (plan)");
+    DevOpLock lk(g_capture_mu);
+    hipError_t e = hipMalloc(&dptr, bytes);
+    if (e != hipSuccess) return fail(KKT_ERR_ALLOC, hip_msg("hipMalloc failed (plan)", e));
     allocs.push_back(dptr);
-    if (!h.empty() && hipMemcpy(dptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
-      return fail(KKT_ERR_COPY, "hipMemcpy failed (plan)");
+    if (!h.empty() &&
+        (e = hipMemcpy(dptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice)) != hipSuccess)
+      return fail(KKT_ERR_COPY, hip_msg("hipMemcpy failed (plan)", e));
     return KKT_OK;
   }
 
@@ -228,6 +250,7 @@ struct kkt_solver {
     if (!tm_on || !((tm_mask >> cls) & 1)) return;
     tm_open = true;
     if (ev_used + 2 > ev_pool.size()) {
+      DevOpLock lk(g_capture_mu);
       for (int q = 0; q < 256; q++) {
         hipEvent_t e;
         (void)hipEventCreate(&e);
@@ -271,6 +294,7 @@ struct kkt_solver {
       (void)hipStreamSynchronize(st);
       free_list(allocs);
       release_numeric();
+      DevOpLock lk(g_capture_mu);
       if (d_rowptr_f) (void)hipFree(d_rowptr_f);
       if (d_col_f) (void)hipFree(d_col_f);
       if (d_map_f) (void)hipFree(d_map_f);
@@ -342,9 +366,12 @@ struct kkt_solver {
     (void)hipStreamSynchronize(st);
     free_list(allocs);
     release_numeric();
-    for (void* q : {(void*)d_rowptr_f, (void*)d_col_f, (void*)d_map_f, (void*)d_vals, (void*)d_valsf,
-                    (void*)d_vec})
-      if (q) (void)hipFree(q);
+    {
+      DevOpLock lk(g_capture_mu);
+      for (void* q : {(void*)d_rowptr_f, (void*)d_col_f, (void*)d_map_f, (void*)d_vals, (void*)d_valsf,
+                      (void*)d_vec})
+        if (q) (void)hipFree(q);
+    }
     d_rowptr_f = d_col_f = d_map_f = nullptr;
     d_vals = d_valsf = d_vec = nullptr;
     const int ns = S.nsuper;
@@ -607,6 +634,7 @@ struct kkt_solver {
     h_first = first; h_k = kk; h_m = mm; h_foff = foff; h_roff = roff; h_chp = chp; h_chl = chl;
     h_rel = rel;
     // pattern + values
+    DevOpLock lk(g_capture_mu);
     if (hipMalloc(&d_rowptr_f, (n + 1) * sizeof(int)) != hipSuccess ||
         hipMalloc(&d_col_f, col_f.size() * sizeof(int)) != hipSuccess ||
         hipMalloc(&d_map_f, map_f.size() * sizeof(int)) != hipSuccess ||
@@ -625,6 +653,7 @@ struct kkt_solver {
 
   int ensure_numeric(int p) {
     size_t esz = (p == KKT_PREC_FP64) ? sizeof(double) : sizeof(float);
+    DevOpLock lk(g_capture_mu);
     if (p == KKT_PREC_FP64 && !d_F64) {
       if (hipMalloc(&d_F64, std::max<long long>(front_elems, 1) * sizeof(double)) != hipSuccess ||
           hipMalloc(&d_W64, std::max<long long>(w_elems, 1) * sizeof(double)) != hipSuccess)
@@ -697,6 +726,7 @@ struct kkt_solver {
       if (make_mask(hv.data(), mask)) {
         int rc = analyze_with_mask(mask.data());
         if (rc) return rc;
+        DevOpLock lk(g_capture_mu);
         if (hipMemcpy(d_vals, hv.data(), (size_t)nnz * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
           return fail(KKT_ERR_COPY, "hipMemcpy (vals) failed");
       }
@@ -722,9 +752,10 @@ struct kkt_solver {
     }
     if (graphs_on && fexec[q] && !(tm_on && (tm_mask & fcls))) (void)hipGraphLaunch(fexec[q], st);
     else body();
-    if (hipMemcpyAsync(&ctr, d_ctr, sizeof(FactorCounters), hipMemcpyDeviceToHost, st) != hipSuccess)
-      return fail(KKT_ERR_COPY, "hipMemcpyAsync (counters) failed");
-    if (hipStreamSynchronize(st) != hipSuccess) return fail(KKT_ERR_FACTORIZATION, "factor kernels failed");
+    hipError_t e = hipMemcpyAsync(&ctr, d_ctr, sizeof(FactorCounters), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return fail(KKT_ERR_COPY, hip_msg("hipMemcpyAsync (counters) failed", e));
+    if ((e = hipStreamSynchronize(st)) != hipSuccess)
+      return fail(KKT_ERR_FACTORIZATION, hip_msg("factor kernels failed", e));
     tflush();
     if (hipGetLastError() != hipSuccess) return fail(KKT_ERR_FACTORIZATION, "kernel launch failed");
     fprec = p;
@@ -889,7 +920,9 @@ struct kkt_solver {
   bool dag_failed() {
     if (!dag_on || !d_dagctr) return false;
     int e = 0;
-    if (hipMemcpy(&e, d_dagctr + 1, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return false;
+    if (hipMemcpyAsync(&e, d_dagctr + 1, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return false;
     if (e == 0) return false;
     dag_on = false;
     drop_graphs();
@@ -1000,6 +1033,7 @@ struct kkt_solver {
 template <typename T>
 static int cmp_dev(const T* d, const std::vector<T>& hv) {
   std::vector<T> tmp(hv.size());
+  DevOpLock lk(g_capture_mu);
   if (!hv.empty() && hipMemcpy(tmp.data(), d, hv.size() * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) return 1;
   return memcmp(tmp.data(), hv.data(), hv.size() * sizeof(T)) != 0;
 }

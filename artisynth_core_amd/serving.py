@@ -85,14 +85,6 @@ class SolveServer:
             self.instances[iid] = Instance(iid, rowOffs, colIdxs, vals0,
                                            self.precision if precision is None else precision)
 
-    def _estimate_bytes(self, inst, info):
-        esz = 4 if inst.precision == KKT_PREC_FP32_REFINE else 8
-        fronts = info.front_bytes if info.front_bytes else 0
-        work = inst.n * 64 * esz * 2  # panel workspace, rough
-        vecs = 14 * inst.n * 8 + 6 * inst.n * esz
-        pattern = inst.nnz * (8 + 4 + 4 + 8) * 2
-        return int(fronts + work + vecs + pattern + info.nnzL * 0)
-
     def _admit(self, inst):
         """Analyze and allocate an instance (called with the lock released)."""
         L = self.L

@@ -129,8 +129,11 @@ int kkt_set_refine_tol(kkt_solver* h, double tol, int max_steps);
 /* 1 (default): the factor and triangular-solve launch sequences are captured once
  * per analysis and precision into HIP graphs and replayed; 0: launched eagerly. */
 int kkt_set_graphs(kkt_solver* h, int on);
-/* Triangular solves: 1 (default) one persistent dependency-driven launch per
- * sweep; 0 level-by-level launches.  Both give bitwise-identical results. */
+/* Triangular solves: 0 (default) level-by-level launches; 1 one persistent
+ * dependency-driven launch for both sweeps.  Both give bitwise-identical
+ * results.  Mode 1 publishes every task with device-scope release/acquire,
+ * which on a multi-XCD GPU writes back and invalidates the issuing XCD's L2;
+ * on large systems it is slower than mode 0 and is kept for small ones. */
 int kkt_set_solve_mode(kkt_solver* h, int mode);
 /* ordering: 0 natural, 1 nested dissection (default) */
 int kkt_set_ordering(kkt_solver* h, int ordering);
