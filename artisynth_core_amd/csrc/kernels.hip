@@ -13,42 +13,9 @@
 // update vectors of the multifrontal scheme, so no floating-point atomics are
 // used anywhere: results are bitwise reproducible run to run.
 #include "kernels.h"
+#include "wave.h"
 
 namespace kkt {
-
-// ------------------------------------------------------------------------
-// wave helpers (64-wide)
-// ------------------------------------------------------------------------
-__device__ inline double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
-}
-// max value, lowest index among equal maxima; result in all lanes
-__device__ inline void wave_argmax(double& v, int& idx) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    double ov = __shfl_xor(v, off, 64);
-    int oi = __shfl_xor(idx, off, 64);
-    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
-  }
-}
-template <typename T>
-__device__ inline T wave_sum(T v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-// Lane 0's value in every lane.  Applied to values that are equal across the
-// wave (LDS broadcasts, wave max) so the compiler sees uniform branch and loop
-// conditions and emits scalar control flow around barriers.
-__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ inline double uni(double v) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
-  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
 
 __device__ inline int lower_bound_i(const int* a, int n, int key) {
   int lo = 0, hi = n;
@@ -110,38 +77,47 @@ __global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ aidx,
 
 // ------------------------------------------------------------------------
 // panel diagonal block: Bunch-Kaufman restricted to the block, FP64 in LDS
+//
+// Four waves per front.  Every wave runs the pivot search on the same LDS data
+// (DPP reductions, identical decisions), wave 0 applies interchanges, and each
+// wave applies the rank-1/rank-2 update to its 16 columns of every trailing row.
+// Pivot columns stay unscaled until the loop ends, so a step needs one barrier
+// (three with an interchange).  The inverse X = L11^-1 is then formed in 16x16
+// blocks: diagonal blocks by substitution, off-diagonal blocks as
+// X(I,J) = -X(I,I) sum_{J<=Q<I} L(I,Q) X(Q,J).
 // ------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items, int p, DevSym d,
-                                                   T* __restrict__ F, double pert,
-                                                   const double* __restrict__ amax,
-                                                   FactorCounters* __restrict__ ctr) {
-  // one wave per front; the 64x64 block lives in LDS as A[i*65+j] (lower part)
-  __shared__ double A[64 * 65];
-  __shared__ double colb[64], colb2[64];  // scaled pivot column(s) of the current step
+__global__ __launch_bounds__(256) void k_panel_diag(const int* __restrict__ items, int p, DevSym d,
+                                                    T* __restrict__ F, double pert,
+                                                    const double* __restrict__ amax,
+                                                    FactorCounters* __restrict__ ctr) {
+  __shared__ double A[64 * 65];     // A[row*65 + col]: lower part L/D, upper part X
+  __shared__ double dv[64][3];      // 1x1: {1/d, d}; 2x2 (first column): D^-1 {i11, i21, i22}
+  __shared__ double Tm[3][16 * 17];  // partial products of the blocked inverse
   __shared__ int lp[64];
   __shared__ signed char pt[64];
   const int s = items[blockIdx.x];
   const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s];
   const int j0 = p * NB, w = min(NB, k - j0);
   T* Fs = F + d.sn_foff[s];
-  const int t = threadIdx.x;
-  for (int j = 0; j < 64; j++)
+  const int tid = threadIdx.x, t = tid & 63, wv = tid >> 6;
+  for (int j = wv; j < 64; j += 4)
     A[t * 65 + j] = (j < w && t >= j && t < w) ? (double)Fs[(size_t)(j0 + j) * m + j0 + t] : 0.0;
-  lp[t] = t;
-  pt[t] = 1;
+  if (wv == 0) {
+    lp[t] = t;
+    pt[t] = 1;
+  }
   __syncthreads();
   const double alpha = (1.0 + sqrt(17.0)) / 8.0;
   const double tau = pert * amax[0];
   int npos = 0, nneg = 0, npert = 0, n2 = 0;
+  const int q0 = wv * 16;  // first of this wave's 16 update columns
   int kc = 0;
   while (kc < w) {
     const double absakk = uni(fabs(A[kc * 65 + kc]));
     double cv = (t > kc && t < w) ? fabs(A[t * 65 + kc]) : -1.0;
     int ci = t;
-    wave_argmax(cv, ci);
-    cv = uni(cv);
-    ci = uni(ci);
+    wave_argmax_u(cv, ci);
     const double colmax = cv > 0.0 ? cv : 0.0;
     const int imax = cv > 0.0 ? ci : -1;
     int kstep = 1, kp = kc;
@@ -154,7 +130,7 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
       double rv = 0.0;
       if (t >= kc && t < w && t != imax)
         rv = (t < imax) ? fabs(A[imax * 65 + t]) : fabs(A[t * 65 + imax]);
-      const double rowmax = uni(wave_max(rv));
+      const double rowmax = wave_max_u(rv);
       if (absakk * rowmax >= alpha * colmax * colmax) kp = kc;
       else if (uni(fabs(A[imax * 65 + imax])) >= alpha * rowmax) kp = imax;
       else { kp = imax; kstep = 2; }
@@ -163,47 +139,48 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
     if (kp != kk) {
       // symmetric interchange of kk and kp inside the block (lower storage); the
       // same rows of earlier panels are permuted once at the end (lp)
-      __syncthreads();
-      const int a = kk, b = kp;
-      if (t == 0) {
-        double tmp = A[a * 65 + a]; A[a * 65 + a] = A[b * 65 + b]; A[b * 65 + b] = tmp;
-        int ti = lp[a]; lp[a] = lp[b]; lp[b] = ti;
+      __syncthreads();  // every wave is done reading column kc and row imax
+      if (wv == 0) {
+        const int a = kk, b = kp;
+        if (t == 0) {
+          double tmp = A[a * 65 + a]; A[a * 65 + a] = A[b * 65 + b]; A[b * 65 + b] = tmp;
+          int ti = lp[a]; lp[a] = lp[b]; lp[b] = ti;
+        }
+        if (t > a && t < b) { double tmp = A[t * 65 + a]; A[t * 65 + a] = A[b * 65 + t]; A[b * 65 + t] = tmp; }
+        if (t > b && t < w) { double tmp = A[t * 65 + a]; A[t * 65 + a] = A[t * 65 + b]; A[t * 65 + b] = tmp; }
+        if (t < a) { double tmp = A[a * 65 + t]; A[a * 65 + t] = A[b * 65 + t]; A[b * 65 + t] = tmp; }
       }
-      if (t > a && t < b) { double tmp = A[t * 65 + a]; A[t * 65 + a] = A[b * 65 + t]; A[b * 65 + t] = tmp; }
-      if (t > b && t < w) { double tmp = A[t * 65 + a]; A[t * 65 + a] = A[t * 65 + b]; A[t * 65 + b] = tmp; }
-      if (t < a) { double tmp = A[a * 65 + t]; A[a * 65 + t] = A[b * 65 + t]; A[b * 65 + t] = tmp; }
+      __syncthreads();
     }
-    __syncthreads();
     if (kstep == 1) {
       double dd = uni(A[kc * 65 + kc]);
       const bool pz = perturb || fabs(dd) < tau;
-      if (pz) dd = (dd >= 0.0) ? tau : -tau;
-      if (pz) npert++;
+      if (pz) {
+        dd = (dd >= 0.0) ? tau : -tau;
+        npert++;
+      }
       if (dd > 0) npos++; else nneg++;
       const double dinv = 1.0 / dd;
-      const double aik = (t > kc && t < w) ? A[t * 65 + kc] : 0.0;
-      colb[t] = aik;  // raw column; lanes >= w hold 0
-      __syncthreads();
-      if (t > kc && t < w) {
-        // whole trailing row, 8 entries per round trip: entries above the
-        // diagonal are scratch while the pivot loop runs
-        double* row = A + t * 65;
-        int j = kc + 1;
-        for (; j + 8 <= 64; j += 8) {
-          double rr[8], cc[8];
+      // trailing rows: A(t, j) -= a(t) a(j) / d for j > kc; entries above the
+      // diagonal are scratch while the pivot loop runs
+      if (q0 + 16 > kc + 1 && t > kc && t < w) {
+        const double aik = A[t * 65 + kc];
+        double* row = A + t * 65 + q0;
+        double rr[16], cc[16];
 #pragma unroll
-          for (int q = 0; q < 8; q++) { rr[q] = row[j + q]; cc[q] = colb[j + q]; }
-#pragma unroll
-          for (int q = 0; q < 8; q++) row[j + q] = rr[q] - aik * (cc[q] * dinv);
+        for (int q = 0; q < 16; q++) {
+          rr[q] = row[q];
+          cc[q] = A[(q0 + q) * 65 + kc];
         }
-        for (; j < 64; j++) row[j] -= aik * (colb[j] * dinv);
-        row[kc] = aik * dinv;
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+          if (q0 + q > kc) row[q] = rr[q] - aik * (cc[q] * dinv);
       }
-      if (t == 0) {
-        A[kc * 65 + kc] = dd;
+      if (tid == 0) {
         pt[kc] = 1;
+        dv[kc][0] = dinv;
+        dv[kc][1] = dd;
       }
-      __syncthreads();
     } else {
       const int g1 = kc, g2 = kc + 1;
       const double d11 = uni(A[g1 * 65 + g1]), d21 = uni(A[g2 * 65 + g1]), d22 = uni(A[g2 * 65 + g2]);
@@ -213,43 +190,56 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
       else if (d11 + d22 > 0) npos += 2;
       else nneg += 2;
       n2++;
-      const bool act = (t > g2 && t < w);
-      const double a1 = act ? A[t * 65 + g1] : 0.0, a2 = act ? A[t * 65 + g2] : 0.0;
-      colb[t] = a1 * i11 + a2 * i21;
-      colb2[t] = a1 * i21 + a2 * i22;
-      __syncthreads();
-      if (act) {
-        double* row = A + t * 65;
-        int j = g2 + 1;
-        for (; j + 8 <= 64; j += 8) {
-          double rr[8], c1[8], c2[8];
+      if (q0 + 16 > g2 + 1 && t > g2 && t < w) {
+        const double a1 = A[t * 65 + g1], a2 = A[t * 65 + g2];
+        double* row = A + t * 65 + q0;
+        double rr[16], x1[16], x2[16];
 #pragma unroll
-          for (int q = 0; q < 8; q++) { rr[q] = row[j + q]; c1[q] = colb[j + q]; c2[q] = colb2[j + q]; }
-#pragma unroll
-          for (int q = 0; q < 8; q++) row[j + q] = rr[q] - (a1 * c1[q] + a2 * c2[q]);
+        for (int q = 0; q < 16; q++) {
+          rr[q] = row[q];
+          x1[q] = A[(q0 + q) * 65 + g1];
+          x2[q] = A[(q0 + q) * 65 + g2];
         }
-        for (; j < 64; j++) row[j] -= a1 * colb[j] + a2 * colb2[j];
-        row[g1] = colb[t];
-        row[g2] = colb2[t];
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+          const double c1 = x1[q] * i11 + x2[q] * i21, c2 = x1[q] * i21 + x2[q] * i22;
+          if (q0 + q > g2) row[q] = rr[q] - (a1 * c1 + a2 * c2);
+        }
       }
-      if (t == 0) {
+      if (tid == 0) {
         pt[g1] = 2;
         pt[g2] = 0;
+        dv[g1][0] = i11;
+        dv[g1][1] = i21;
+        dv[g1][2] = i22;
       }
-      __syncthreads();
     }
+    __syncthreads();
     kc += kstep;
   }
-  __syncthreads();
-  // rows j0..j0+w of the earlier panels' columns follow the interchanges (new row t
-  // = old row lp[t]); one wave: every lane loads before any lane stores
+  // scale the pivot columns: L(t,c) = a(t,c) / d_c, or [a1 a2] D^-1 for a 2x2 pivot
+  // (the wave of a pair's first column writes both columns)
+  for (int c = wv; c < w; c += 4) {
+    const int pc = pt[c];
+    if (pc == 1) {
+      if (t > c && t < w) A[t * 65 + c] *= dv[c][0];
+      if (t == c) A[c * 65 + c] = dv[c][1];
+    } else if (pc == 2) {
+      if (t > c + 1 && t < w) {
+        const double a1 = A[t * 65 + c], a2 = A[t * 65 + c + 1];
+        A[t * 65 + c] = a1 * dv[c][0] + a2 * dv[c][1];
+        A[t * 65 + c + 1] = a1 * dv[c][1] + a2 * dv[c][2];
+      }
+    }
+  }
+  // rows j0..j0+w of the earlier panels' columns follow the interchanges (new row
+  // t = old row lp[t]); each wave moves its own columns, loading before storing
   if (j0 > 0) {
     int moved = 0;
     for (int q = 0; q < w; q++) moved |= (lp[q] != q);
     if (uni(moved)) {
       const int src = (t < w) ? lp[t] : t;
-      int c = 0;
-      for (; c + 4 <= j0; c += 4) {
+      for (int c = 4 * wv; c < j0; c += 16) {
         T v0 = (T)0, v1 = (T)0, v2 = (T)0, v3 = (T)0;
         if (t < w) {
           v0 = Fs[(size_t)c * m + j0 + src];
@@ -265,50 +255,70 @@ __global__ __launch_bounds__(64) void k_panel_diag(const int* __restrict__ items
           Fs[(size_t)(c + 3) * m + j0 + t] = v3;
         }
       }
-      for (; c < j0; c++) {
-        T v0 = (t < w) ? Fs[(size_t)c * m + j0 + src] : (T)0;
-        __builtin_amdgcn_wave_barrier();
-        if (t < w) Fs[(size_t)c * m + j0 + t] = v0;
-      }
-    }
-  }
-  // inverse of the unit lower L11 (entries (q+1,q) of 2x2 pivots hold D, not L):
-  // lane c forms column c, stored above the diagonal as A[c][r] = Linv[r][c]
-  const unsigned long long two = __ballot(t < w && pt[t] == 2);
-  // The 2x2 slots (q+1,q) are zeroed in a copy of row q+1 first.
-  if (t < w && t > 0 && ((two >> (t - 1)) & 1ULL)) colb[t] = A[t * 65 + t - 1];
-  __syncthreads();
-  if (t < w && t > 0 && ((two >> (t - 1)) & 1ULL)) A[t * 65 + t - 1] = 0.0;
-  __syncthreads();
-  if (t < w) {
-    const int c = t;
-    for (int r = c + 1; r < w; r++) {
-      const double* Lr = A + r * 65;
-      const double* Xc = A + c * 65;
-      double acc0 = -Lr[c], acc1 = 0.0;
-      int l = c + 1;
-      for (; l + 8 <= r; l += 8) {
-        double lv[8], xv[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) { lv[q] = Lr[l + q]; xv[q] = Xc[l + q]; }
-#pragma unroll
-        for (int q = 0; q < 8; q += 2) { acc0 -= lv[q] * xv[q]; acc1 -= lv[q + 1] * xv[q + 1]; }
-      }
-      for (; l < r; l++) acc0 -= Lr[l] * Xc[l];
-      A[c * 65 + r] = acc0 + acc1;
     }
   }
   __syncthreads();
-  if (t < w && t > 0 && ((two >> (t - 1)) & 1ULL)) A[t * 65 + t - 1] = colb[t];
+  // X = L11^-1 (unit lower; the slot (c+1,c) of a 2x2 pivot holds D and counts as
+  // 0).  X(i,l), l < i, is stored at A[l*65 + i].
+  const unsigned long long two = __ballot(pt[t] == 2);
+  if (t < 16) {
+    const int b0 = wv * 16, c = t;
+    double x[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) x[r] = (r == c) ? 1.0 : 0.0;
+#pragma unroll
+    for (int r = 1; r < 16; r++) {
+      double acc = 0.0;
+#pragma unroll
+      for (int l = 0; l < r; l++) {
+        const double lv = (((two >> (b0 + l)) & 1ULL) && r == l + 1) ? 0.0 : A[(b0 + r) * 65 + b0 + l];
+        acc += lv * x[l];
+      }
+      x[r] = (r > c) ? -acc : x[r];
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+      if (r > c) A[(b0 + c) * 65 + b0 + r] = x[r];
+  }
   __syncthreads();
-  // lower: L and D; upper (row t, column j0+j > j0+t): Linv[j][t]
-  for (int j = 0; j < w; j++)
+  {
+    const int r = tid & 15, c = tid >> 4;
+    for (int bi = 1; bi < 4; bi++) {
+      for (int bj = 0; bj < bi; bj++) {
+        // T(r,c) = sum_{bj<=bl<bi} L(bi r, bl q) X(bl q, bj c)
+        double acc = 0.0;
+        for (int bl = bj; bl < bi; bl++) {
+#pragma unroll
+          for (int q = 0; q < 16; q++) {
+            const int R = 16 * bi + r, C = 16 * bl + q;
+            const double lv = (((two >> C) & 1ULL) && R == C + 1) ? 0.0 : A[R * 65 + C];
+            double xv = A[(16 * bj + c) * 65 + C];
+            if (bl == bj) xv = (q == c) ? 1.0 : (q > c ? xv : 0.0);
+            acc += lv * xv;
+          }
+        }
+        Tm[bj][r * 17 + c] = acc;
+      }
+      __syncthreads();
+      for (int bj = 0; bj < bi; bj++) {
+        // X(bi r, bj c) = -sum_{q<=r} X(bi r, bi q) T(q,c)
+        double acc = Tm[bj][r * 17 + c];
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+          if (q < r) acc += A[(16 * bi + q) * 65 + 16 * bi + r] * Tm[bj][q * 17 + c];
+        A[(16 * bj + c) * 65 + 16 * bi + r] = -acc;
+      }
+      __syncthreads();
+    }
+  }
+  // lower: L and D; upper (row t, column j > t): X(j, t)
+  for (int j = wv; j < w; j += 4)
     if (t < w) Fs[(size_t)(j0 + j) * m + j0 + t] = (T)A[t * 65 + j];
-  if (t < w) {
+  if (wv == 0 && t < w) {
     d.lperm[first + j0 + t] = j0 + lp[t];
     d.piv[first + j0 + t] = pt[t];
   }
-  if (t == 0) {
+  if (tid == 0) {
     atomicAdd(&ctr->npos, npos);
     atomicAdd(&ctr->nneg, nneg);
     atomicAdd(&ctr->npert, npert);
@@ -1122,7 +1132,7 @@ template <typename T>
 void launch_panel_diag(const int* items, int nitems, int p, const DevSym& d, T* F, double pert,
                        const double* amax, FactorCounters* ctr, hipStream_t st) {
   if (nitems > 0)
-    hipLaunchKernelGGL(k_panel_diag<T>, dim3(nitems), dim3(64), 0, st, items, p, d, F, pert, amax,
+    hipLaunchKernelGGL(k_panel_diag<T>, dim3(nitems), dim3(256), 0, st, items, p, d, F, pert, amax,
                        ctr);
 }
 template <typename T>

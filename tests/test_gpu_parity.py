@@ -109,6 +109,9 @@ def test_small_beam_vs_oracle(gpu, nx, prec):
         oi = ref.info()
         assert (inf["npos"], inf["nneg"]) == (oi["npos"], oi["nneg"])
         assert inf["nnzL"] == oi["nnzL"]
+        if prec == KKT_PREC_FP64:
+            # same panel-restricted Bunch-Kaufman decisions as the oracle
+            assert (inf["n2x2"], inf["nperturbed"]) == (oi["n2x2"], oi["nperturbed"])
     finally:
         s.dispose()
 
@@ -160,6 +163,8 @@ def test_multipanel_beam_vs_oracle(gpu, prec):
         assert rel_err(x, xo) <= 1e-7
         inf, oi = s.info(), ref.info()
         assert (inf["npos"], inf["nneg"]) == (oi["npos"], oi["nneg"]) == (beam.sizeM, beam.numG)
+        if prec == KKT_PREC_FP64:
+            assert (inf["n2x2"], inf["nperturbed"]) == (oi["n2x2"], oi["nperturbed"])
         # bitwise reproducible: no order-dependent atomics
         x2 = np.zeros(beam.n)
         s.factor(vals)
