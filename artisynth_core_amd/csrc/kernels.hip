@@ -101,8 +101,16 @@ __global__ __launch_bounds__(256) void k_panel_diag(const int* __restrict__ item
   const int j0 = p * NB, w = min(NB, k - j0);
   T* Fs = F + d.sn_foff[s];
   const int tid = threadIdx.x, t = tid & 63, wv = tid >> 6;
-  for (int j = wv; j < 64; j += 4)
-    A[t * 65 + j] = (j < w && t >= j && t < w) ? (double)Fs[(size_t)(j0 + j) * m + j0 + t] : 0.0;
+  {
+    T ld[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int j = wv + 4 * q;
+      ld[q] = (j < w && t >= j && t < w) ? Fs[(size_t)(j0 + j) * m + j0 + t] : (T)0;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) A[t * 65 + wv + 4 * q] = (double)ld[q];
+  }
   if (wv == 0) {
     lp[t] = t;
     pt[t] = 1;
@@ -161,10 +169,10 @@ __global__ __launch_bounds__(256) void k_panel_diag(const int* __restrict__ item
       }
       if (dd > 0) npos++; else nneg++;
       const double dinv = 1.0 / dd;
-      // trailing rows: A(t, j) -= a(t) a(j) / d for j > kc; entries above the
+      // trailing rows: A(t, j) -= (a(t) / d) a(j) for j > kc; entries above the
       // diagonal are scratch while the pivot loop runs
       if (q0 + 16 > kc + 1 && t > kc && t < w) {
-        const double aik = A[t * 65 + kc];
+        const double lt = A[t * 65 + kc] * dinv;
         double* row = A + t * 65 + q0;
         double rr[16], cc[16];
 #pragma unroll
@@ -172,9 +180,13 @@ __global__ __launch_bounds__(256) void k_panel_diag(const int* __restrict__ item
           rr[q] = row[q];
           cc[q] = A[(q0 + q) * 65 + kc];
         }
+        if (q0 > kc) {
 #pragma unroll
-        for (int q = 0; q < 16; q++)
-          if (q0 + q > kc) row[q] = rr[q] - aik * (cc[q] * dinv);
+          for (int q = 0; q < 16; q++) row[q] = fma(-lt, cc[q], rr[q]);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 16; q++) row[q] = (q0 + q > kc) ? fma(-lt, cc[q], rr[q]) : rr[q];
+        }
       }
       if (tid == 0) {
         pt[kc] = 1;
@@ -185,13 +197,16 @@ __global__ __launch_bounds__(256) void k_panel_diag(const int* __restrict__ item
       const int g1 = kc, g2 = kc + 1;
       const double d11 = uni(A[g1 * 65 + g1]), d21 = uni(A[g2 * 65 + g1]), d22 = uni(A[g2 * 65 + g2]);
       const double det = d11 * d22 - d21 * d21;
-      const double i11 = d22 / det, i21 = -d21 / det, i22 = d11 / det;
+      const double rdet = 1.0 / det;
+      const double i11 = d22 * rdet, i21 = -d21 * rdet, i22 = d11 * rdet;
       if (det < 0) { npos++; nneg++; }
       else if (d11 + d22 > 0) npos += 2;
       else nneg += 2;
       n2++;
+      // A(t, j) -= [a1(t) a2(t)] D^-1 [a1(j) a2(j)]^T for j > g2
       if (q0 + 16 > g2 + 1 && t > g2 && t < w) {
         const double a1 = A[t * 65 + g1], a2 = A[t * 65 + g2];
+        const double l1 = a1 * i11 + a2 * i21, l2 = a1 * i21 + a2 * i22;
         double* row = A + t * 65 + q0;
         double rr[16], x1[16], x2[16];
 #pragma unroll
@@ -200,10 +215,13 @@ __global__ __launch_bounds__(256) void k_panel_diag(const int* __restrict__ item
           x1[q] = A[(q0 + q) * 65 + g1];
           x2[q] = A[(q0 + q) * 65 + g2];
         }
+        if (q0 > g2) {
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-          const double c1 = x1[q] * i11 + x2[q] * i21, c2 = x1[q] * i21 + x2[q] * i22;
-          if (q0 + q > g2) row[q] = rr[q] - (a1 * c1 + a2 * c2);
+          for (int q = 0; q < 16; q++) row[q] = fma(-l2, x2[q], fma(-l1, x1[q], rr[q]));
+        } else {
+#pragma unroll
+          for (int q = 0; q < 16; q++)
+            row[q] = (q0 + q > g2) ? fma(-l2, x2[q], fma(-l1, x1[q], rr[q])) : rr[q];
         }
       }
       if (tid == 0) {

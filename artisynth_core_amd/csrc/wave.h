@@ -45,26 +45,6 @@ __device__ inline double dpp_d(double v) {
   const unsigned phi = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, RMASK, 0xF, false);
   return __longlong_as_double((long long)(((unsigned long long)phi << 32) | plo));
 }
-template <int CTRL, int RMASK>
-__device__ inline void argmax_step(double& v, int& i) {
-  const double pv = dpp_d<CTRL, RMASK>(v);
-  const int pi = __builtin_amdgcn_update_dpp(i, i, CTRL, RMASK, 0xF, false);
-  if (pv > v || (pv == v && pi < i)) {
-    v = pv;
-    i = pi;
-  }
-}
-// largest value, lowest lane index among equal maxima; uniform result
-__device__ inline void wave_argmax_u(double& v, int& i) {
-  argmax_step<0x111, 0xF>(v, i);
-  argmax_step<0x112, 0xF>(v, i);
-  argmax_step<0x114, 0xF>(v, i);
-  argmax_step<0x118, 0xF>(v, i);
-  argmax_step<0x142, 0xA>(v, i);
-  argmax_step<0x143, 0xC>(v, i);
-  v = readlane_d(v, 63);
-  i = __builtin_amdgcn_readlane(i, 63);
-}
 __device__ inline double wave_max_u(double v) {
   v = fmax(v, dpp_d<0x111, 0xF>(v));
   v = fmax(v, dpp_d<0x112, 0xF>(v));
@@ -73,6 +53,13 @@ __device__ inline double wave_max_u(double v) {
   v = fmax(v, dpp_d<0x142, 0xA>(v));
   v = fmax(v, dpp_d<0x143, 0xC>(v));
   return readlane_d(v, 63);
+}
+// largest value and the lowest lane holding it; uniform result
+__device__ inline void wave_argmax_u(double& v, int& i) {
+  const double mx = wave_max_u(v);
+  const unsigned long long hit = __ballot(v == mx);
+  v = mx;
+  i = __ffsll((unsigned long long)hit) - 1;
 }
 
 
