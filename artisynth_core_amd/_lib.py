@@ -64,7 +64,6 @@ SIGNATURES = [
     ("kkt_set_precision", _I, [_P, _I]),
     ("kkt_set_refine_tol", _I, [_P, _D, _I]),
     ("kkt_set_graphs", _I, [_P, _I]),
-    ("kkt_set_solve_mode", _I, [_P, _I]),
     ("kkt_set_ordering", _I, [_P, _I]),
     ("kkt_get_info", _I, [_P, ctypes.POINTER(KktInfo)]),
     ("kkt_get_symbolic", _I, [_P, _IP, _IP, _IP, _LLP, _IP]),

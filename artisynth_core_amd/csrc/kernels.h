@@ -36,8 +36,8 @@ template <typename T>
 void launch_scatter(const int* aidx, const long long* apos, int cnt, const double* vals, T* F,
                     hipStream_t st);
 template <typename T>
-void launch_panel_diag(const int* items, int nitems, int p, const DevSym& d, T* F, double pert,
-                       const double* amax, FactorCounters* ctr, hipStream_t st);
+void launch_panel_diag(const int* items, int nitems, int p, const DevSym& d, T* F, T* E0, T* E1,
+                       double pert, const double* amax, FactorCounters* ctr, hipStream_t st);
 template <typename T>
 void launch_panel_trsm(const int2* items, int nitems, int p, const DevSym& d, T* F, T* W,
                        hipStream_t st);
@@ -48,25 +48,25 @@ void launch_panel_update(const int4* items, int nitems, int p, const DevSym& d, 
 template <typename T>
 void launch_panel_xrow(const int2* items, int nitems, int p, const DevSym& d, T* F, hipStream_t st);
 
+// after the factor: Z^T = (L21 X)^T into the upper-right block, then the lower
+// part becomes G = [X; Z] (transposed copies of the upper part)
+template <typename T>
+void launch_zt(const int4* items, int nitems, const DevSym& d, T* F, hipStream_t st);
+template <typename T>
+void launch_gform(const int4* items, int nitems, const DevSym& d, T* F, hipStream_t st);
 // level-scheduled solves (vectors in the front precision T)
 template <typename T>
 void launch_fwd_gather(const int* items, int nitems, const DevSym& d, const T* bp, T* fp, T* u,
                        hipStream_t st);
 template <typename T>
-void launch_fwd_xgemv(const int2* items, int nitems, const DevSym& d, const T* F, const T* fp, T* y,
-                      hipStream_t st);
-template <typename T>
-void launch_fwd_gemv(const int2* items, int nitems, const DevSym& d, const T* F, const T* y,
+void launch_fwd_rows(const int2* items, int nitems, const DevSym& d, const T* F, const T* fp, T* y,
                      T* u, hipStream_t st);
 template <typename T>
-void launch_bwd_vgemv(const int2* items, int nitems, const DevSym& d, const T* F, const T* y,
-                      const T* xp, T* v, hipStream_t st);
+void launch_dsolve(int n, const T* E0, const T* E1, const signed char* piv, const T* y, T* z,
+                   hipStream_t st);
 template <typename T>
-void launch_bwd_xpart(const int4* items, int nitems, const DevSym& d, const T* F, const T* v,
-                      const long long* poff, T* part, hipStream_t st);
-template <typename T>
-void launch_bwd_finish(const int* items, int nitems, const DevSym& d, const T* v,
-                       const long long* poff, const T* part, T* xp, hipStream_t st);
+void launch_bwd_cols(const int2* items, int nitems, const DevSym& d, const T* F, const T* z, T* xp,
+                     hipStream_t st);
 
 // permutation / precision conversion: out[j] = (TO) in[perm[j]]  (gather)
 template <typename TI, typename TO>
@@ -90,19 +90,6 @@ void launch_axpby(int n, double alpha, const double* x, double beta, double* y, 
 void launch_axpbypcz(int n, double a, const double* x, double b, const double* y, double c,
                      double* z, hipStream_t st);
 
-// dependency-driven triangular solve (one persistent launch + a counter reset)
-struct DagCtl {
-  int* ctr;           // counter block: head, err, then 7 arrays of nsuper ints
-  const int* tchild;  // per supernode: number of u-row tasks of all its children
-  const int* nbx;     // per supernode: number of X^T partial tasks
-  const int* sparent; // assembly-tree parent
-  int nsuper;
-  int ntasks;
-};
-template <typename T>
-void launch_solve_dag(const int4* tasks, int nwg, const DagCtl& c, const DevSym& d, const T* F,
-                      const T* bp, T* fp, T* y, T* u, T* v, T* xp, T* part, const long long* poff,
-                      hipStream_t st);
 
 // BiCGStab updates with device-resident scalars (see kernels.hip for the slots)
 void launch_bicg_p(int n, const double* sc, const double* r, const double* v, double* p, hipStream_t st);

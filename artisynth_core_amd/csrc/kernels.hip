@@ -9,9 +9,11 @@
 //   panel_diag : Bunch-Kaufman LDL^T of the 64x64 diagonal block (FP64 in LDS)
 //   panel_trsm : L21 = A21 L11^-T D^-1 for the rows below, W = L21 D
 //   panel_update: C -= W L^T on 64x64 tiles with MFMA (f64 or f32)
-// Solves run level by level (forward bottom-up, backward top-down) with the
-// update vectors of the multifrontal scheme, so no floating-point atomics are
-// used anywhere: results are bitwise reproducible run to run.
+// After the last level, post-passes form X = L11^-1 (xrow), Z^T = (L21 X)^T
+// (zt, MFMA) and the lower copy G = [X; Z] (gform).  Solves then run one launch
+// per level and sweep (forward bottom-up, backward top-down) with the update
+// vectors of the multifrontal scheme.  No floating-point atomics are used
+// anywhere: results are bitwise reproducible run to run.
 #include "kernels.h"
 #include "wave.h"
 
@@ -88,7 +90,8 @@ __global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ aidx,
 // ------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_diag(const int* __restrict__ items, int p, DevSym d,
-                                                    T* __restrict__ F, double pert,
+                                                    T* __restrict__ F, T* __restrict__ E0,
+                                                    T* __restrict__ E1, double pert,
                                                     const double* __restrict__ amax,
                                                     FactorCounters* __restrict__ ctr) {
   __shared__ double A[64 * 65];     // A[row*65 + col]: lower part L/D, upper part X
@@ -335,6 +338,13 @@ __global__ __launch_bounds__(256) void k_panel_diag(const int* __restrict__ item
   if (wv == 0 && t < w) {
     d.lperm[first + j0 + t] = j0 + lp[t];
     d.piv[first + j0 + t] = pt[t];
+    // D^-1 for the solves: z_t = E0_t y_t + E1_t y_partner (partner t+1 / t-1
+    // for the first / second column of a 2x2 pivot)
+    const int pc = pt[t];
+    const double e0 = (pc == 1) ? dv[t][0] : (pc == 2 ? dv[t][0] : dv[t - 1][2]);
+    const double e1 = (pc == 1) ? 0.0 : (pc == 2 ? dv[t][1] : dv[t - 1][1]);
+    E0[first + j0 + t] = (T)e0;
+    E1[first + j0 + t] = (T)e1;
   }
   if (tid == 0) {
     atomicAdd(&ctr->npos, npos);
@@ -612,17 +622,118 @@ __global__ __launch_bounds__(256) void k_panel_xrow(const int2* __restrict__ ite
 }
 
 // ------------------------------------------------------------------------
-// Level-scheduled triangular solves.  Per level, forward:
-//   fwd_gather : f = b + children's update vectors; fp = f permuted (pivots)
-//   fwd_xgemv  : y = X fp          (X = L11^-1, wave per row)
-//   fwd_gemv   : u = f_cb - L21 y  (64-row blocks)
-// backward (levels top-down):
-//   bwd_vgemv  : v = D^-1 y - L21^T x_rows   (wave per column)
-//   bwd_xpart  : partial sums of X^T v over 64-column chunks
-//   bwd_finish : x = v + sum of partials, scattered through the pivot permutation
+// Solve form of the factor.  After the last panel step two post-passes turn
+// every front into
+//   lower part (rows > column):  G = [X; Z],  X = L11^-1, Z = L21 X
+//   upper part (row < column):   G^T (X^T in the pivot block, Z^T to its right)
+// with D^-1 kept in two vectors (E0, E1).  Then per supernode
+//   forward : gather f, then [y; u] = [f; f_cb] + [X - I; -Z] f
+//             (one wave per front row, reading column i of the upper copy)
+//   z = D^-1 y for all pivots at once
+//   backward: x = z + G^T [z; -x_rows]
+//             (one wave per pivot column, reading column c of the lower copy)
+// so every product streams contiguous columns, and a level costs two launches
+// forward and one backward.
 // ------------------------------------------------------------------------
 constexpr int KMAX = 1024;  // widest supernode (the symbolic analysis splits wider ones)
 
+// Z^T(c, j) = sum_{i >= c} X(i, c) L21(j, i) into front (row c, column k + j);
+// item = (supernode, 64-column block of Z, 64-row block of L21)
+template <typename T>
+__global__ __launch_bounds__(256) void k_zt(const int4* __restrict__ items, DevSym d,
+                                            T* __restrict__ F) {
+  constexpr int LD = 65;
+  __shared__ T As[64 * LD];  // As[c*LD + i] = X(i0 + i, c0 + c)
+  __shared__ T Bs[64 * LD];  // Bs[j*LD + i] = L21(jr0 + j, i0 + i)
+  typedef typename Mfma<T>::v4 v4;
+  const int4 it = items[blockIdx.x];
+  const int s = it.x, cb = it.y, jb = it.z;
+  const int k = d.sn_k[s], m = d.sn_m[s], r = m - k;
+  const int c0 = cb * 64, jr0 = jb * 64;
+  const int nc = min(64, k - c0), nj = min(64, r - jr0);
+  T* Fs = F + d.sn_foff[s];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  v4 acc[2][2];
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++) acc[bi][bj] = (v4)((T)0);
+  for (int i0 = c0; i0 < k; i0 += 64) {
+    const int ni = min(64, k - i0);
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+      const int ii = e >> 6, cc = e & 63;
+      T xv = (T)0, lv = (T)0;
+      if (ii < ni) {
+        const int gi = i0 + ii, gc = c0 + cc;
+        if (cc < nc) xv = (gi > gc) ? Fs[(size_t)gi * m + gc] : (gi == gc ? (T)1 : (T)0);
+        if (cc < nj) lv = Fs[(size_t)gi * m + k + jr0 + cc];
+      }
+      As[cc * LD + ii] = xv;
+      Bs[cc * LD + ii] = lv;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < 64; kk += 4) {
+      const T a0 = As[(wr + lr) * LD + kk + lk], a1 = As[(wr + 16 + lr) * LD + kk + lk];
+      const T b0 = Bs[(wc + lr) * LD + kk + lk], b1 = Bs[(wc + 16 + lr) * LD + kk + lk];
+      acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
+      acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
+      acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+      acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+    }
+    __syncthreads();
+  }
+  // C(c, j) -> As[j*LD + c], then columns k + jr0 + j are written contiguously in c
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        As[(wc + bj * 16 + lr) * LD + wr + bi * 16 + Mfma<T>::row(lk, v)] = acc[bi][bj][v];
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int jj = e >> 6, cc = e & 63;
+    if (jj < nj && cc < nc) Fs[(size_t)(k + jr0 + jj) * m + c0 + cc] = As[jj * LD + cc];
+  }
+}
+
+// lower part <- transpose of the upper part, one 64x64 tile per item:
+//   {s, ib, cb, 0}: pivot block, lower tile (ib, cb), ib >= cb, strictly below the diagonal
+//   {s, cb, jb, 1}: Z tile, rows k + 64 jb.., columns 64 cb..
+template <typename T>
+__global__ __launch_bounds__(256) void k_gform(const int4* __restrict__ items, DevSym d,
+                                               T* __restrict__ F) {
+  __shared__ T tile[64 * 65];  // tile[x*65 + y]: source column offset x, source row offset y
+  const int4 it = items[blockIdx.x];
+  const int s = it.x;
+  const int k = d.sn_k[s], m = d.sn_m[s];
+  T* Fs = F + d.sn_foff[s];
+  int srow0, scol0, nrow, ncol;  // source tile in the upper part: rows, columns
+  if (it.w == 0) {
+    srow0 = it.z * 64; scol0 = it.y * 64;
+    nrow = min(64, k - srow0); ncol = min(64, k - scol0);
+  } else {
+    srow0 = it.y * 64; scol0 = k + it.z * 64;
+    nrow = min(64, k - srow0); ncol = min(64, m - scol0);
+  }
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int x = e >> 6, y = e & 63;
+    if (x < ncol && y < nrow) tile[x * 65 + y] = Fs[(size_t)(scol0 + x) * m + srow0 + y];
+  }
+  __syncthreads();
+  // destination: row scol0 + x, column srow0 + y
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int y = e >> 6, x = e & 63;
+    if (x < ncol && y < nrow && scol0 + x > srow0 + y)
+      Fs[(size_t)(srow0 + y) * m + scol0 + x] = tile[x * 65 + y];
+  }
+}
+
+// forward gather of one supernode: f = b + the children's update vectors;
+// fp = f in pivot order, u_s = the children's sums on the contribution rows
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_gather(const int* __restrict__ items, DevSym d,
                                                     const T* __restrict__ bp, T* __restrict__ fp,
@@ -632,7 +743,7 @@ __global__ __launch_bounds__(256) void k_fwd_gather(const int* __restrict__ item
   const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s], r = m - k;
   T* us = u + d.sn_uoff[s];
   const int tid = threadIdx.x;
-  for (int t = tid; t < k; t += 256) g[t] = bp[first + t];
+  for (int q = tid; q < k; q += 256) g[q] = bp[first + q];
   for (int i = tid; i < r; i += 256) us[i] = (T)0;
   __syncthreads();
   for (int q = d.ch_ptr[s]; q < d.ch_ptr[s + 1]; q++) {
@@ -647,349 +758,87 @@ __global__ __launch_bounds__(256) void k_fwd_gather(const int* __restrict__ item
     }
     __syncthreads();
   }
-  for (int t = tid; t < k; t += 256) fp[first + t] = g[d.lperm[first + t]];
+  for (int q = tid; q < k; q += 256) fp[first + q] = g[d.lperm[first + q]];
 }
 
-// y_t = fp_t + sum_{l<t} X(t,l) fp_l ; X(t,l) at front (row l, col t); item = (s, 4 rows)
+// forward products, one wave per front row i, from the upper copy (column i,
+// rows l < min(i, k), contiguous): y_i = f_i + G(i,:) f (i < k) or
+// u_{i-k} -= G(i,:) f (i >= k); item = (s, 4 rows)
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_xgemv(const int2* __restrict__ items, DevSym d,
-                                                   const T* __restrict__ F,
-                                                   const T* __restrict__ fp, T* __restrict__ y) {
-  const int2 it = items[blockIdx.x];
-  const int s = it.x;
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int t = it.y * 4 + wv;
-  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s];
-  if (t >= k) return;
-  const T* col = F + d.sn_foff[s] + (size_t)t * m;
-  const T* f = fp + first;
-  T acc = (T)0;
-  for (int l = lane; l < t; l += 64) acc += col[l] * f[l];
-  acc = wave_sum(acc);
-  if (lane == 0) y[first + t] = f[t] + acc;
-}
-
-// u_s[i] -= sum_t L(k+i, t) y_t ; item = (s, 32-row block); 8 column groups per row
-template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_gemv(const int2* __restrict__ items, DevSym d,
+__global__ __launch_bounds__(256) void k_fwd_rows(const int2* __restrict__ items, DevSym d,
                                                   const T* __restrict__ F,
-                                                  const T* __restrict__ y, T* __restrict__ u) {
-  __shared__ T ys[KMAX];
-  __shared__ T part[8][33];
-  const int2 it = items[blockIdx.x];
-  const int s = it.x, rb = it.y;
-  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s], r = m - k;
-  const T* Fs = F + d.sn_foff[s];
-  for (int t = threadIdx.x; t < k; t += 256) ys[t] = y[first + t];
-  __syncthreads();
-  const int g = threadIdx.x >> 5, lr = threadIdx.x & 31;
-  const int i = rb * 32 + lr;
-  T a0 = (T)0, a1 = (T)0, a2 = (T)0, a3 = (T)0;
-  if (i < r) {
-    const T* col = Fs + k + i;
-    int q = g;
-    for (; q + 24 < k; q += 32) {
-      a0 += col[(size_t)q * m] * ys[q];
-      a1 += col[(size_t)(q + 8) * m] * ys[q + 8];
-      a2 += col[(size_t)(q + 16) * m] * ys[q + 16];
-      a3 += col[(size_t)(q + 24) * m] * ys[q + 24];
-    }
-    for (; q < k; q += 8) a0 += col[(size_t)q * m] * ys[q];
-  }
-  part[g][lr] = (a0 + a1) + (a2 + a3);
-  __syncthreads();
-  if (threadIdx.x < 32 && i < r) {
-    T acc = (T)0;
-#pragma unroll
-    for (int q = 0; q < 8; q++) acc += part[q][lr];
-    u[d.sn_uoff[s] + i] -= acc;
-  }
-}
-
-// v_q = (D^-1 y)_q - sum_i L(k+i, q) xp[rows_i] ; item = (s, 4 columns), wave per column
-template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_vgemv(const int2* __restrict__ items, DevSym d,
-                                                   const T* __restrict__ F,
-                                                   const T* __restrict__ y,
-                                                   const T* __restrict__ xp, T* __restrict__ v) {
+                                                  const T* __restrict__ fp, T* __restrict__ y,
+                                                  T* __restrict__ u) {
   const int2 it = items[blockIdx.x];
   const int s = it.x;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int q = it.y * 4 + wv;
-  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s], r = m - k;
-  if (q >= k) return;
-  const T* Fs = F + d.sn_foff[s];
-  const T* col = Fs + (size_t)q * m + k;
-  const int* rows = d.rows + d.sn_roff[s];
-  T acc = (T)0;
-  for (int i = lane; i < r; i += 64) acc += col[i] * xp[rows[i]];
-  acc = wave_sum(acc);
-  if (lane == 0) {
-    const int pq = d.piv[first + q];
-    T z;
-    if (pq == 1) {
-      z = y[first + q] / Fs[(size_t)q * m + q];
-    } else if (pq == 2) {
-      const T d11 = Fs[(size_t)q * m + q], d21 = Fs[(size_t)q * m + q + 1],
-              d22 = Fs[(size_t)(q + 1) * m + q + 1];
-      const T det = d11 * d22 - d21 * d21;
-      z = (y[first + q] * d22 - y[first + q + 1] * d21) / det;
-    } else {
-      const T d11 = Fs[(size_t)(q - 1) * m + q - 1], d21 = Fs[(size_t)(q - 1) * m + q],
-              d22 = Fs[(size_t)q * m + q];
-      const T det = d11 * d22 - d21 * d21;
-      z = (y[first + q] * d11 - y[first + q - 1] * d21) / det;
-    }
-    v[first + q] = z - acc;
-  }
-}
-
-// partial[c][l] = sum_{i in chunk c, i > l} X(i,l) v_i
-// item = (s, chunk of 64 columns c, block of 256 outputs l)  -> int4 {s, c, lb, 0}
-template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_xpart(const int4* __restrict__ items, DevSym d,
-                                                   const T* __restrict__ F,
-                                                   const T* __restrict__ v,
-                                                   const long long* __restrict__ poff,
-                                                   T* __restrict__ part) {
-  __shared__ T vs[64];
-  const int4 it = items[blockIdx.x];
-  const int s = it.x, c = it.y, lb = it.z;
+  const int i = it.y * 4 + wv;
   const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s];
-  const int i0 = c * 64, i1 = min(k, i0 + 64);
-  const T* Fs = F + d.sn_foff[s];
-  if (threadIdx.x < 64) vs[threadIdx.x] = (i0 + (int)threadIdx.x < k) ? v[first + i0 + threadIdx.x] : (T)0;
-  __syncthreads();
-  const int l = lb * 256 + (int)threadIdx.x;
-  if (l >= i1) return;
-  T a0 = (T)0, a1 = (T)0;
-  int i = max(i0, l + 1);
-  for (; i + 1 < i1; i += 2) {
-    a0 += Fs[(size_t)i * m + l] * vs[i - i0];
-    a1 += Fs[(size_t)(i + 1) * m + l] * vs[i + 1 - i0];
+  if (i >= m) return;
+  const T* col = F + d.sn_foff[s] + (size_t)i * m;
+  const T* f = fp + first;
+  const int len = min(i, k);
+  T a0 = (T)0, a1 = (T)0, a2 = (T)0, a3 = (T)0;
+  int l = lane;
+  for (; l + 192 < len; l += 256) {
+    a0 += col[l] * f[l];
+    a1 += col[l + 64] * f[l + 64];
+    a2 += col[l + 128] * f[l + 128];
+    a3 += col[l + 192] * f[l + 192];
   }
-  if (i < i1) a0 += Fs[(size_t)i * m + l] * vs[i - i0];
-  part[poff[s] + (size_t)c * k + l] = a0 + a1;
+  for (; l < len; l += 64) a0 += col[l] * f[l];
+  const T acc = wave_sum((a0 + a1) + (a2 + a3));
+  if (lane == 0) {
+    if (i < k) y[first + i] = f[i] + acc;
+    else u[d.sn_uoff[s] + i - k] -= acc;
+  }
 }
 
+// z = D^-1 y over all pivots: z_g = E0_g y_g + E1_g y_partner
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_finish(const int* __restrict__ items, DevSym d,
-                                                    const T* __restrict__ v,
-                                                    const long long* __restrict__ poff,
-                                                    const T* __restrict__ part,
-                                                    T* __restrict__ xp) {
-  const int s = items[blockIdx.x];
-  const int k = d.sn_k[s], first = d.sn_first[s];
-  const int nc = (k + 63) / 64;
-  const T* ps = part + poff[s];
-  for (int l = threadIdx.x; l < k; l += 256) {
-    T x = v[first + l];
-    for (int c = l / 64; c < nc; c++) x += ps[(size_t)c * k + l];
-    xp[first + d.lperm[first + l]] = x;
-  }
+__global__ __launch_bounds__(256) void k_dsolve(int n, const T* __restrict__ E0,
+                                                const T* __restrict__ E1,
+                                                const signed char* __restrict__ piv,
+                                                const T* __restrict__ y, T* __restrict__ z) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= n) return;
+  const int pc = piv[g];
+  const int pg = (pc == 2) ? g + 1 : (pc == 0 ? g - 1 : g);
+  z[g] = E0[g] * y[g] + E1[g] * y[pg];
 }
 
-// ------------------------------------------------------------------------
-// Dependency-driven triangular solve: one persistent launch runs the whole
-// forward + backward sweep as a task list in topological order.  A workgroup
-// claims the next task index, waits (bounded) until the counters of the tasks
-// it depends on reach their targets, runs the task and bumps its own counter.
-// Every dependency of task t has a smaller index, and a claimed task belongs to
-// a running workgroup, so the oldest unfinished task can always proceed.
-// A wait that exceeds the spin bound sets *err and all workgroups drain.
-// Task kinds (int4 {kind, s, a, b}):
-//   0 FG   gather f of supernode s          needs child[s] >= tchild[s]
-//   1 FX   y rows 4a..4a+3                  needs fg[s] >= 1
-//   2 FU   u rows 32a..32a+31               needs fx[s] >= nfx(s); then child[parent]++
-//   3 BV   v columns 4a..4a+3               needs fx[s] >= nfx(s), fin[parent] >= 1
-//   4 BX   partial chunk a, outputs block b needs bv[s] >= nbv(s)
-//   5 BF   finish + scatter                 needs bx[s] >= nbx[s]
-// Counter block (ints): [0] head, [1] err, then 7 arrays of nsuper:
-//   child, fg, fx, bv, bx, fin, (spare)
-// ------------------------------------------------------------------------
-__device__ inline bool dag_wait(const int* p, int target, const int* err) {
-  for (int it = 0; it < 400000; it++) {
-    if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
-    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
-    __builtin_amdgcn_s_sleep(2);
-  }
-  return false;
-}
-
+// backward, one wave per pivot column c, from the lower copy (column c, rows
+// below the diagonal, contiguous): x_c = z_c + sum_{c<i<k} X(i,c) z_i
+//   - sum_j Z(j,c) x[rows_j]; item = (s, 4 columns)
 template <typename T>
-__global__ __launch_bounds__(256) void k_solve_dag(const int4* __restrict__ tasks, DagCtl c,
-                                                   DevSym d, const T* __restrict__ F,
-                                                   const T* __restrict__ bp, T* __restrict__ fp,
-                                                   T* __restrict__ y, T* __restrict__ u,
-                                                   T* __restrict__ v, T* __restrict__ xp,
-                                                   T* __restrict__ part,
-                                                   const long long* __restrict__ poff) {
-  __shared__ T sh[KMAX];
-  __shared__ T prt[8][33];
-  __shared__ int s_task, s_ok;
-  int* head = c.ctr;
-  int* err = c.ctr + 1;
-  int* child = c.ctr + 2;
-  int* fg = child + c.nsuper;
-  int* fx = fg + c.nsuper;
-  int* bv = fx + c.nsuper;
-  int* bx = bv + c.nsuper;
-  int* fin = bx + c.nsuper;
-  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-  // Loop control is kept wave-uniform (readfirstlane) so that the compiler
-  // emits a uniform loop: with an LDS-read exit condition it treats the exit
-  // as divergent and may rotate the thread-0 claim out of the barrier loop,
-  // leaving the other threads re-running the previous task.
-  if (tid == 0) s_task = __hip_atomic_fetch_add(head, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  for (;;) {
-    const int t = __builtin_amdgcn_readfirstlane(s_task);
-    if (t >= c.ntasks) break;
-    const int4 tk = tasks[t];
-    const int kind = tk.x, s = tk.y, a = tk.z, b = tk.w;
-    const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s], r = m - k;
-    const int par = c.sparent[s];
-    const int nfx = (k + 3) / 4;
-    if (tid == 0) {
-      bool ok = true;
-      if (kind == 0) ok = dag_wait(child + s, c.tchild[s], err);
-      else if (kind == 1) ok = dag_wait(fg + s, 1, err);
-      else if (kind == 2) ok = dag_wait(fx + s, nfx, err);
-      else if (kind == 3) ok = dag_wait(fx + s, nfx, err) && (par < 0 || dag_wait(fin + par, 1, err));
-      else if (kind == 4) ok = dag_wait(bv + s, nfx, err);
-      else ok = dag_wait(bx + s, c.nbx[s], err);
-      if (!ok) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      s_ok = ok ? 1 : 0;
-    }
-    __syncthreads();
-    if (__builtin_amdgcn_readfirstlane(s_ok) == 0) break;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    const T* Fs = F + d.sn_foff[s];
-    if (kind == 0) {
-      T* us = u + d.sn_uoff[s];
-      for (int q = tid; q < k; q += 256) sh[q] = bp[first + q];
-      for (int i = tid; i < r; i += 256) us[i] = (T)0;
-      __syncthreads();
-      for (int q = d.ch_ptr[s]; q < d.ch_ptr[s + 1]; q++) {
-        const int ch = d.ch_list[q];
-        const int rc = d.sn_m[ch] - d.sn_k[ch];
-        const int* relc = d.rel + d.sn_roff[ch];
-        const T* uc = u + d.sn_uoff[ch];
-        for (int i = tid; i < rc; i += 256) {
-          const int pos = relc[i];
-          const T val = uc[i];
-          if (pos < k) sh[pos] += val; else us[pos - k] += val;
-        }
-        __syncthreads();
-      }
-      for (int q = tid; q < k; q += 256) fp[first + q] = sh[d.lperm[first + q]];
-    } else if (kind == 1) {
-      const int row = a * 4 + wv;
-      if (row < k) {
-        const T* col = Fs + (size_t)row * m;
-        const T* f = fp + first;
-        T acc = (T)0;
-        for (int l = lane; l < row; l += 64) acc += col[l] * f[l];
-        acc = wave_sum(acc);
-        if (lane == 0) y[first + row] = f[row] + acc;
-      }
-    } else if (kind == 2) {
-      for (int q = tid; q < k; q += 256) sh[q] = y[first + q];
-      __syncthreads();
-      const int g = tid >> 5, lr = tid & 31;
-      const int i = a * 32 + lr;
-      T a0 = (T)0, a1 = (T)0, a2 = (T)0, a3 = (T)0;
-      if (i < r) {
-        const T* col = Fs + k + i;
-        int q = g;
-        for (; q + 24 < k; q += 32) {
-          a0 += col[(size_t)q * m] * sh[q];
-          a1 += col[(size_t)(q + 8) * m] * sh[q + 8];
-          a2 += col[(size_t)(q + 16) * m] * sh[q + 16];
-          a3 += col[(size_t)(q + 24) * m] * sh[q + 24];
-        }
-        for (; q < k; q += 8) a0 += col[(size_t)q * m] * sh[q];
-      }
-      prt[g][lr] = (a0 + a1) + (a2 + a3);
-      __syncthreads();
-      if (tid < 32 && i < r) {
-        T acc = (T)0;
-#pragma unroll
-        for (int q = 0; q < 8; q++) acc += prt[q][lr];
-        u[d.sn_uoff[s] + i] -= acc;
-      }
-    } else if (kind == 3) {
-      const int q = a * 4 + wv;
-      if (q < k) {
-        const T* col = Fs + (size_t)q * m + k;
-        const int* rows = d.rows + d.sn_roff[s];
-        T acc = (T)0;
-        for (int i = lane; i < r; i += 64) acc += col[i] * xp[rows[i]];
-        acc = wave_sum(acc);
-        if (lane == 0) {
-          const int pq = d.piv[first + q];
-          T z;
-          if (pq == 1) {
-            z = y[first + q] / Fs[(size_t)q * m + q];
-          } else if (pq == 2) {
-            const T d11 = Fs[(size_t)q * m + q], d21 = Fs[(size_t)q * m + q + 1],
-                    d22 = Fs[(size_t)(q + 1) * m + q + 1];
-            const T det = d11 * d22 - d21 * d21;
-            z = (y[first + q] * d22 - y[first + q + 1] * d21) / det;
-          } else {
-            const T d11 = Fs[(size_t)(q - 1) * m + q - 1], d21 = Fs[(size_t)(q - 1) * m + q],
-                    d22 = Fs[(size_t)q * m + q];
-            const T det = d11 * d22 - d21 * d21;
-            z = (y[first + q] * d11 - y[first + q - 1] * d21) / det;
-          }
-          v[first + q] = z - acc;
-        }
-      }
-    } else if (kind == 4) {
-      const int i0 = a * 64, i1 = min(k, i0 + 64);
-      if (tid < 64) sh[tid] = (i0 + tid < k) ? v[first + i0 + tid] : (T)0;
-      __syncthreads();
-      const int l = b * 256 + tid;
-      if (l < i1) {
-        T a0 = (T)0, a1 = (T)0;
-        int i = max(i0, l + 1);
-        for (; i + 1 < i1; i += 2) {
-          a0 += Fs[(size_t)i * m + l] * sh[i - i0];
-          a1 += Fs[(size_t)(i + 1) * m + l] * sh[i + 1 - i0];
-        }
-        if (i < i1) a0 += Fs[(size_t)i * m + l] * sh[i - i0];
-        part[poff[s] + (size_t)a * k + l] = a0 + a1;
-      }
-    } else {
-      const int nc = (k + 63) / 64;
-      const T* ps = part + poff[s];
-      for (int l = tid; l < k; l += 256) {
-        T x = v[first + l];
-        for (int cc = l / 64; cc < nc; cc++) x += ps[(size_t)cc * k + l];
-        xp[first + d.lperm[first + l]] = x;
-      }
-    }
-    // publish: every wave releases its stores, then one thread bumps the counter
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (tid == 0) {
-      int* cnt = nullptr;
-      if (kind == 0) cnt = fg + s;
-      else if (kind == 1) cnt = fx + s;
-      else if (kind == 2) cnt = (par >= 0) ? child + par : nullptr;
-      else if (kind == 3) cnt = bv + s;
-      else if (kind == 4) cnt = bx + s;
-      else cnt = fin + s;
-      if (cnt) __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      s_task = __hip_atomic_fetch_add(head, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
+__global__ __launch_bounds__(256) void k_bwd_cols(const int2* __restrict__ items, DevSym d,
+                                                  const T* __restrict__ F,
+                                                  const T* __restrict__ z, T* __restrict__ xp) {
+  const int2 it = items[blockIdx.x];
+  const int s = it.x;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int c = it.y * 4 + wv;
+  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s], r = m - k;
+  if (c >= k) return;
+  const T* col = F + d.sn_foff[s] + (size_t)c * m;
+  const T* zs = z + first;
+  const int* rows = d.rows + d.sn_roff[s];
+  T a0 = (T)0, a1 = (T)0, a2 = (T)0, a3 = (T)0;
+  int i = c + 1 + lane;
+  for (; i + 64 < k; i += 128) {
+    a0 += col[i] * zs[i];
+    a1 += col[i + 64] * zs[i + 64];
   }
-}
-
-__global__ void k_zero_ints(long long n, int* __restrict__ p) {
-  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) p[i] = 0;
+  if (i < k) a0 += col[i] * zs[i];
+  const T* zc = col + k;
+  int j = lane;
+  for (; j + 64 < r; j += 128) {
+    a2 -= zc[j] * xp[rows[j]];
+    a3 -= zc[j + 64] * xp[rows[j + 64]];
+  }
+  if (j < r) a2 -= zc[j] * xp[rows[j]];
+  const T acc = wave_sum((a0 + a1) + (a2 + a3));
+  if (lane == 0) xp[first + d.lperm[first + c]] = zs[c] + acc;
 }
 
 // ------------------------------------------------------------------------
@@ -1147,11 +996,11 @@ void launch_scatter(const int* aidx, const long long* apos, int cnt, const doubl
     hipLaunchKernelGGL(k_scatter<T>, dim3(nblk(cnt, 256)), dim3(256), 0, st, aidx, apos, cnt, vals, F);
 }
 template <typename T>
-void launch_panel_diag(const int* items, int nitems, int p, const DevSym& d, T* F, double pert,
-                       const double* amax, FactorCounters* ctr, hipStream_t st) {
+void launch_panel_diag(const int* items, int nitems, int p, const DevSym& d, T* F, T* E0, T* E1,
+                       double pert, const double* amax, FactorCounters* ctr, hipStream_t st) {
   if (nitems > 0)
-    hipLaunchKernelGGL(k_panel_diag<T>, dim3(nitems), dim3(256), 0, st, items, p, d, F, pert, amax,
-                       ctr);
+    hipLaunchKernelGGL(k_panel_diag<T>, dim3(nitems), dim3(256), 0, st, items, p, d, F, E0, E1, pert,
+                       amax, ctr);
 }
 template <typename T>
 void launch_panel_trsm(const int2* items, int nitems, int p, const DevSym& d, T* F, T* W,
@@ -1170,36 +1019,33 @@ void launch_panel_xrow(const int2* items, int nitems, int p, const DevSym& d, T*
   if (nitems > 0) hipLaunchKernelGGL(k_panel_xrow<T>, dim3(nitems), dim3(256), 0, st, items, p, d, F);
 }
 template <typename T>
+void launch_zt(const int4* items, int nitems, const DevSym& d, T* F, hipStream_t st) {
+  if (nitems > 0) hipLaunchKernelGGL(k_zt<T>, dim3(nitems), dim3(256), 0, st, items, d, F);
+}
+template <typename T>
+void launch_gform(const int4* items, int nitems, const DevSym& d, T* F, hipStream_t st) {
+  if (nitems > 0) hipLaunchKernelGGL(k_gform<T>, dim3(nitems), dim3(256), 0, st, items, d, F);
+}
+template <typename T>
 void launch_fwd_gather(const int* items, int nitems, const DevSym& d, const T* bp, T* fp, T* u,
                        hipStream_t st) {
   if (nitems > 0) hipLaunchKernelGGL(k_fwd_gather<T>, dim3(nitems), dim3(256), 0, st, items, d, bp, fp, u);
 }
 template <typename T>
-void launch_fwd_xgemv(const int2* items, int nitems, const DevSym& d, const T* F, const T* fp, T* y,
-                      hipStream_t st) {
-  if (nitems > 0) hipLaunchKernelGGL(k_fwd_xgemv<T>, dim3(nitems), dim3(256), 0, st, items, d, F, fp, y);
-}
-template <typename T>
-void launch_fwd_gemv(const int2* items, int nitems, const DevSym& d, const T* F, const T* y,
+void launch_fwd_rows(const int2* items, int nitems, const DevSym& d, const T* F, const T* fp, T* y,
                      T* u, hipStream_t st) {
-  if (nitems > 0) hipLaunchKernelGGL(k_fwd_gemv<T>, dim3(nitems), dim3(256), 0, st, items, d, F, y, u);
-}
-template <typename T>
-void launch_bwd_vgemv(const int2* items, int nitems, const DevSym& d, const T* F, const T* y,
-                      const T* xp, T* v, hipStream_t st) {
-  if (nitems > 0) hipLaunchKernelGGL(k_bwd_vgemv<T>, dim3(nitems), dim3(256), 0, st, items, d, F, y, xp, v);
-}
-template <typename T>
-void launch_bwd_xpart(const int4* items, int nitems, const DevSym& d, const T* F, const T* v,
-                      const long long* poff, T* part, hipStream_t st) {
   if (nitems > 0)
-    hipLaunchKernelGGL(k_bwd_xpart<T>, dim3(nitems), dim3(256), 0, st, items, d, F, v, poff, part);
+    hipLaunchKernelGGL(k_fwd_rows<T>, dim3(nitems), dim3(256), 0, st, items, d, F, fp, y, u);
 }
 template <typename T>
-void launch_bwd_finish(const int* items, int nitems, const DevSym& d, const T* v,
-                       const long long* poff, const T* part, T* xp, hipStream_t st) {
-  if (nitems > 0)
-    hipLaunchKernelGGL(k_bwd_finish<T>, dim3(nitems), dim3(256), 0, st, items, d, v, poff, part, xp);
+void launch_dsolve(int n, const T* E0, const T* E1, const signed char* piv, const T* y, T* z,
+                   hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_dsolve<T>, dim3(nblk(n, 256)), dim3(256), 0, st, n, E0, E1, piv, y, z);
+}
+template <typename T>
+void launch_bwd_cols(const int2* items, int nitems, const DevSym& d, const T* F, const T* z, T* xp,
+                     hipStream_t st) {
+  if (nitems > 0) hipLaunchKernelGGL(k_bwd_cols<T>, dim3(nitems), dim3(256), 0, st, items, d, F, z, xp);
 }
 template <typename TI, typename TO>
 void launch_gather_perm(int n, const int* perm, const TI* in, TO* out, hipStream_t st) {
@@ -1235,22 +1081,6 @@ void launch_axpbypcz(int n, double a, const double* x, double b, const double* y
   hipLaunchKernelGGL(k_axpbypcz, dim3(nblk(n, 256)), dim3(256), 0, st, n, a, x, b, y, c, z);
 }
 
-template <typename T>
-void launch_solve_dag(const int4* tasks, int nwg, const DagCtl& c, const DevSym& d, const T* F,
-                      const T* bp, T* fp, T* y, T* u, T* v, T* xp, T* part, const long long* poff,
-                      hipStream_t st) {
-  hipLaunchKernelGGL(k_zero_ints, dim3(nblk(2 + 7LL * c.nsuper, 256)), dim3(256), 0, st,
-                     (long long)(2 + 7LL * c.nsuper), c.ctr);
-  hipLaunchKernelGGL(k_solve_dag<T>, dim3(nwg), dim3(256), 0, st, tasks, c, d, F, bp, fp, y, u, v,
-                     xp, part, poff);
-}
-template void launch_solve_dag<double>(const int4*, int, const DagCtl&, const DevSym&, const double*,
-                                       const double*, double*, double*, double*, double*, double*,
-                                       double*, const long long*, hipStream_t);
-template void launch_solve_dag<float>(const int4*, int, const DagCtl&, const DevSym&, const float*,
-                                      const float*, float*, float*, float*, float*, float*, float*,
-                                      const long long*, hipStream_t);
-
 void launch_bicg_p(int n, const double* sc, const double* r, const double* v, double* p, hipStream_t st) {
   hipLaunchKernelGGL(k_bicg_p, dim3(nblk(n, 256)), dim3(256), 0, st, n, sc, r, v, p);
 }
@@ -1278,24 +1108,22 @@ void launch_zero_counters(FactorCounters* c, hipStream_t st) {
   template void launch_assemble<T>(const int4*, int, const DevSym&, T*, hipStream_t);            \
   template void launch_scatter<T>(const int*, const long long*, int, const double*, T*,          \
                                   hipStream_t);                                                  \
-  template void launch_panel_diag<T>(const int*, int, int, const DevSym&, T*, double,           \
+  template void launch_panel_diag<T>(const int*, int, int, const DevSym&, T*, T*, T*, double,   \
                                      const double*, FactorCounters*, hipStream_t);               \
   template void launch_panel_trsm<T>(const int2*, int, int, const DevSym&, T*, T*, hipStream_t); \
   template void launch_panel_update<T>(const int4*, int, int, const DevSym&, T*, const T*,       \
                                        hipStream_t);                                             \
   template void launch_panel_xrow<T>(const int2*, int, int, const DevSym&, T*, hipStream_t);     \
+  template void launch_zt<T>(const int4*, int, const DevSym&, T*, hipStream_t);                \
+  template void launch_gform<T>(const int4*, int, const DevSym&, T*, hipStream_t);             \
   template void launch_fwd_gather<T>(const int*, int, const DevSym&, const T*, T*, T*,           \
                                      hipStream_t);                                               \
-  template void launch_fwd_xgemv<T>(const int2*, int, const DevSym&, const T*, const T*, T*,     \
-                                    hipStream_t);                                                \
-  template void launch_fwd_gemv<T>(const int2*, int, const DevSym&, const T*, const T*, T*,      \
+  template void launch_fwd_rows<T>(const int2*, int, const DevSym&, const T*, const T*, T*, T*,  \
                                    hipStream_t);                                                 \
-  template void launch_bwd_vgemv<T>(const int2*, int, const DevSym&, const T*, const T*,         \
-                                    const T*, T*, hipStream_t);                                  \
-  template void launch_bwd_xpart<T>(const int4*, int, const DevSym&, const T*, const T*,         \
-                                    const long long*, T*, hipStream_t);                          \
-  template void launch_bwd_finish<T>(const int*, int, const DevSym&, const T*,                   \
-                                     const long long*, const T*, T*, hipStream_t);
+  template void launch_dsolve<T>(int, const T*, const T*, const signed char*, const T*, T*,      \
+                                 hipStream_t);                                                   \
+  template void launch_bwd_cols<T>(const int2*, int, const DevSym&, const T*, const T*, T*,      \
+                                   hipStream_t);
 KKT_INST(double)
 KKT_INST(float)
 template void launch_gather_perm<double, double>(int, const int*, const double*, double*, hipStream_t);

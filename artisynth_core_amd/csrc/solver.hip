@@ -56,6 +56,7 @@ struct Step {
 
 // timing classes
 enum { TC_ASM = 0, TC_DIAG, TC_TRSM, TC_UPD, TC_SLEVEL, TC_SGEMV, TC_SPMV, TC_VEC, TC_N };
+// (TC_SLEVEL: forward solve launches, TC_SGEMV: backward solve launches)
 
 }  // namespace
 
@@ -96,18 +97,14 @@ struct kkt_solver {
   signed char* d_piv = nullptr;
   // plan
   int4 *d_asm = nullptr, *d_upd = nullptr;
-  int *d_diag = nullptr, *d_lvlsn = nullptr, *d_aidx = nullptr;
-  int2 *d_trsm = nullptr, *d_fg = nullptr, *d_bg = nullptr, *d_xrow = nullptr, *d_xg = nullptr;
-  int4* d_xp = nullptr;
-  // dependency-driven solve: task list, counters, per-supernode targets
-  int4* d_dag = nullptr;
-  int *d_dagctr = nullptr, *d_tchild = nullptr, *d_nbx = nullptr, *d_spar = nullptr;
-  int ndag = 0;
-  bool dag_on = false;
+  int *d_diag = nullptr, *d_aidx = nullptr, *d_fg = nullptr;
+  int2 *d_trsm = nullptr, *d_xrow = nullptr, *d_fr = nullptr, *d_bc = nullptr;
+  int4 *d_zt = nullptr, *d_gf = nullptr;
   std::vector<Launch> xrow_l;  // inverse-row post-pass, one launch per panel index p >= 1
-  long long *d_apos = nullptr, *d_poff = nullptr;
-  std::vector<Launch> asm_l, scat_l, sn_l, fg_l, bg_l, xg_l, xp_l;
-  long long part_elems = 0;
+  Launch zt_all, gf_all;       // Z^T and G-form post-passes, one launch each
+  long long* d_apos = nullptr;
+  // per level: assembly, scatter; forward gather / rows, backward columns
+  std::vector<Launch> asm_l, scat_l, fg_l, fr_l, bc_l;
   std::vector<std::vector<Step>> steps;
   long long front_elems = 0, w_elems = 0, u_elems = 0;
   // numeric buffers (allocated lazily per precision)
@@ -115,7 +112,7 @@ struct kkt_solver {
   float* d_F32 = nullptr;
   double* d_W64 = nullptr;
   float* d_W32 = nullptr;
-  void* d_svec = nullptr;  // solve scratch: bp, fp, y, v, xp (n each), u (u_elems), partials, as T
+  void* d_svec = nullptr;  // solve scratch: bp, fp, y, z, xp (n each), u (u_elems), as T
   size_t svec_bytes = 0;
   // FP64 vectors for refinement / BiCGStab / host copies
   double* d_vec = nullptr;  // 14 vectors of n (0-8 BiCGStab, 10-11 host copies, 12-13 graph I/O)
@@ -455,20 +452,16 @@ struct kkt_solver {
       }
     // work items
     std::vector<int4> asmv, updv;
-    std::vector<int> diagv, lvlsn, aidx;
+    std::vector<int> diagv, aidx;
     std::vector<long long> apos;
-    std::vector<int2> trsmv, fgv, bgv, xrowv, xgv;
-    std::vector<int4> xpv;
-    std::vector<long long> poff(ns + 1, 0);
-    for (int s = 0; s < ns; s++) poff[s + 1] = poff[s] + (long long)((kk[s] + 63) / 64) * kk[s];
-    part_elems = poff[ns];
-    xg_l.assign(nl, Launch());
-    xp_l.assign(nl, Launch());
+    std::vector<int2> trsmv, xrowv;
+    std::vector<int> fgv;
+    std::vector<int2> frv, bcv;
     asm_l.assign(nl, Launch());
     scat_l.assign(nl, Launch());
-    sn_l.assign(nl, Launch());
     fg_l.assign(nl, Launch());
-    bg_l.assign(nl, Launch());
+    fr_l.assign(nl, Launch());
+    bc_l.assign(nl, Launch());
     steps.assign(nl, std::vector<Step>());
     for (int L = 0; L < nl; L++) {
       asm_l[L].off = (long long)asmv.size();
@@ -482,27 +475,17 @@ struct kkt_solver {
       aidx.insert(aidx.end(), a_idx[L].begin(), a_idx[L].end());
       apos.insert(apos.end(), a_pos[L].begin(), a_pos[L].end());
       scat_l[L].cnt = (int)a_idx[L].size();
-      sn_l[L].off = (long long)lvlsn.size();
-      lvlsn.insert(lvlsn.end(), lv[L].begin(), lv[L].end());
-      sn_l[L].cnt = (int)lv[L].size();
       fg_l[L].off = (long long)fgv.size();
-      bg_l[L].off = (long long)bgv.size();
-      xg_l[L].off = (long long)xgv.size();
-      xp_l[L].off = (long long)xpv.size();
+      fr_l[L].off = (long long)frv.size();
+      bc_l[L].off = (long long)bcv.size();
       for (int s : lv[L]) {
-        int r = mm[s] - kk[s];
-        for (int rb = 0; rb < (r + 31) / 32; rb++) fgv.push_back(make_int2(s, rb));
-        for (int cb = 0; cb < (kk[s] + 3) / 4; cb++) bgv.push_back(make_int2(s, cb));
-        for (int cb = 0; cb < (kk[s] + 3) / 4; cb++) xgv.push_back(make_int2(s, cb));
-        for (int c = 0; c < (kk[s] + 63) / 64; c++) {
-          int i1 = std::min(kk[s], c * 64 + 64);
-          for (int lb = 0; lb < (i1 + 255) / 256; lb++) xpv.push_back(make_int4(s, c, lb, 0));
-        }
+        fgv.push_back(s);
+        for (int rb = 0; rb < (mm[s] + 3) / 4; rb++) frv.push_back(make_int2(s, rb));
+        for (int cb = 0; cb < (kk[s] + 3) / 4; cb++) bcv.push_back(make_int2(s, cb));
       }
       fg_l[L].cnt = (int)(fgv.size() - fg_l[L].off);
-      bg_l[L].cnt = (int)(bgv.size() - bg_l[L].off);
-      xg_l[L].cnt = (int)(xgv.size() - xg_l[L].off);
-      xp_l[L].cnt = (int)(xpv.size() - xp_l[L].off);
+      fr_l[L].cnt = (int)(frv.size() - fr_l[L].off);
+      bc_l[L].cnt = (int)(bcv.size() - bc_l[L].off);
       int P = 0;
       for (int s : lv[L]) P = std::max(P, (kk[s] + NB - 1) / NB);
       steps[L].resize(P);
@@ -529,41 +512,21 @@ struct kkt_solver {
         sp.upd.cnt = (int)(updv.size() - sp.upd.off);
       }
     }
-    // task list of the dependency-driven solve (topological order: forward by
-    // increasing level, then backward by decreasing level)
-    {
-      std::vector<int4> dag;
-      std::vector<int> tchild(ns, 0), nbxv(ns, 0);
-      for (int s = 0; s < ns; s++) {
-        int r = mm[s] - kk[s];
-        int p = S.sparent[s];
-        if (p >= 0) tchild[p] += (r + 31) / 32;
-        int cnt = 0;
-        for (int c = 0; c < (kk[s] + 63) / 64; c++) cnt += (std::min(kk[s], c * 64 + 64) + 255) / 256;
-        nbxv[s] = cnt;
+    // Z^T and G-form post-passes over all supernodes
+    std::vector<int4> ztv, gfv;
+    for (int s = 0; s < ns; s++) {
+      const int nc = (kk[s] + 63) / 64, nr = (mm[s] - kk[s] + 63) / 64;
+      for (int cb = 0; cb < nc; cb++) {
+        for (int jb = 0; jb < nr; jb++) {
+          ztv.push_back(make_int4(s, cb, jb, 0));
+          gfv.push_back(make_int4(s, cb, jb, 1));
+        }
+        for (int ib = cb; ib < nc; ib++) gfv.push_back(make_int4(s, ib, cb, 0));
       }
-      for (int L = 0; L < nl; L++)
-        for (int s : lv[L]) {
-          dag.push_back(make_int4(0, s, 0, 0));
-          for (int a = 0; a < (kk[s] + 3) / 4; a++) dag.push_back(make_int4(1, s, a, 0));
-          for (int a = 0; a < (mm[s] - kk[s] + 31) / 32; a++) dag.push_back(make_int4(2, s, a, 0));
-        }
-      for (int L = nl - 1; L >= 0; L--)
-        for (int s : lv[L]) {
-          for (int a = 0; a < (kk[s] + 3) / 4; a++) dag.push_back(make_int4(3, s, a, 0));
-          for (int c = 0; c < (kk[s] + 63) / 64; c++) {
-            int i1 = std::min(kk[s], c * 64 + 64);
-            for (int lb = 0; lb < (i1 + 255) / 256; lb++) dag.push_back(make_int4(4, s, c, lb));
-          }
-          dag.push_back(make_int4(5, s, 0, 0));
-        }
-      ndag = (int)dag.size();
-      std::vector<int> zero(2 + 7 * (size_t)ns, 0);
-      int rc2;
-      if ((rc2 = upload(d_dag, dag)) || (rc2 = upload(d_dagctr, zero)) || (rc2 = upload(d_tchild, tchild)) ||
-          (rc2 = upload(d_nbx, nbxv)) || (rc2 = upload(d_spar, S.sparent)))
-        return rc2;
     }
+    zt_all.off = gf_all.off = 0;
+    zt_all.cnt = (int)ztv.size();
+    gf_all.cnt = (int)gfv.size();
     // inverse rows X(P, :) of every pivot block, after the whole factorization:
     // launch p covers block row p of all supernodes at once
     {
@@ -616,10 +579,10 @@ struct kkt_solver {
         (rc = upload(d_uoff, uoff)) || (rc = upload(d_chp, chp)) || (rc = upload(d_chl, chl)) ||
         (rc = upload(d_rel, rel)) || (rc = upload(d_rows, S.rows)) || (rc = upload(d_perm, S.perm)) ||
         (rc = upload(d_asm, asmv)) || (rc = upload(d_upd, updv)) || (rc = upload(d_diag, diagv)) ||
-        (rc = upload(d_lvlsn, lvlsn)) || (rc = upload(d_aidx, aidx)) || (rc = upload(d_apos, apos)) ||
-        (rc = upload(d_trsm, trsmv)) || (rc = upload(d_fg, fgv)) || (rc = upload(d_bg, bgv)) ||
-        (rc = upload(d_xrow, xrowv)) || (rc = upload(d_xg, xgv)) || (rc = upload(d_xp, xpv)) ||
-        (rc = upload(d_poff, poff)))
+        (rc = upload(d_aidx, aidx)) || (rc = upload(d_apos, apos)) || (rc = upload(d_trsm, trsmv)) ||
+        (rc = upload(d_xrow, xrowv)) || (rc = upload(d_fg, fgv)) || (rc = upload(d_fr, frv)) ||
+        (rc = upload(d_bc, bcv)) ||
+        (rc = upload(d_zt, ztv)) || (rc = upload(d_gf, gfv)))
       return rc;
     std::vector<int> lp0(n);
     for (int s = 0; s < ns; s++)
@@ -655,16 +618,17 @@ struct kkt_solver {
     size_t esz = (p == KKT_PREC_FP64) ? sizeof(double) : sizeof(float);
     DevOpLock lk(g_capture_mu);
     if (p == KKT_PREC_FP64 && !d_F64) {
-      if (hipMalloc(&d_F64, std::max<long long>(front_elems, 1) * sizeof(double)) != hipSuccess ||
+      // fronts, then the two D^-1 vectors (E0, E1) of n entries each
+      if (hipMalloc(&d_F64, (front_elems + 2LL * n + 1) * sizeof(double)) != hipSuccess ||
           hipMalloc(&d_W64, std::max<long long>(w_elems, 1) * sizeof(double)) != hipSuccess)
         return fail(KKT_ERR_ALLOC, "hipMalloc failed (FP64 fronts)");
     }
     if (p == KKT_PREC_FP32_REFINE && !d_F32) {
-      if (hipMalloc(&d_F32, std::max<long long>(front_elems, 1) * sizeof(float)) != hipSuccess ||
+      if (hipMalloc(&d_F32, (front_elems + 2LL * n + 1) * sizeof(float)) != hipSuccess ||
           hipMalloc(&d_W32, std::max<long long>(w_elems, 1) * sizeof(float)) != hipSuccess)
         return fail(KKT_ERR_ALLOC, "hipMalloc failed (FP32 fronts)");
     }
-    size_t need = ((size_t)5 * n + (size_t)u_elems + (size_t)part_elems + 16) * std::max(esz, sizeof(double));
+    size_t need = ((size_t)5 * n + (size_t)u_elems + 16) * std::max(esz, sizeof(double));
     if (svec_bytes < need) {
       drop_graphs();
       if (d_svec) (void)hipFree(d_svec);
@@ -690,7 +654,8 @@ struct kkt_solver {
       for (size_t p = 0; p < steps[L].size(); p++) {
         const Step& sp = steps[L][p];
         tbeg(TC_DIAG);
-        launch_panel_diag<T>(d_diag + sp.diag.off, sp.diag.cnt, (int)p, ds, F, pert, d_scal + 15, d_ctr, st);
+        launch_panel_diag<T>(d_diag + sp.diag.off, sp.diag.cnt, (int)p, ds, F, F + front_elems,
+                             F + front_elems + n, pert, d_scal + 15, d_ctr, st);
         tend();
         tbeg(TC_TRSM);
         launch_panel_trsm<T>(d_trsm + sp.trsm.off, sp.trsm.cnt, (int)p, ds, F, W, st);
@@ -707,6 +672,11 @@ struct kkt_solver {
       launch_panel_xrow<T>(d_xrow + xrow_l[p].off, xrow_l[p].cnt, (int)p, ds, F, st);
       tend();
     }
+    // solve form: Z^T = (L21 X)^T beside the pivot block, then G = [X; Z] below
+    tbeg(TC_TRSM);
+    launch_zt<T>(d_zt, zt_all.cnt, ds, F, st);
+    launch_gform<T>(d_gf, gf_all.cnt, ds, F, st);
+    tend();
   }
 
   int factor_device(const double* dv, bool from_host_copy) {
@@ -770,47 +740,24 @@ struct kkt_solver {
   template <typename T>
   void run_tri_solve(const T* F, const double* d_b, double* d_x) {
     T* sv = (T*)d_svec;
-    T *bp = sv, *fp = sv + n, *y = sv + 2 * (size_t)n, *v = sv + 3 * (size_t)n,
-      *xp = sv + 4 * (size_t)n, *u = sv + 5 * (size_t)n, *part = u + u_elems;
+    T *bp = sv, *fp = sv + n, *y = sv + 2 * (size_t)n, *z = sv + 3 * (size_t)n,
+      *xp = sv + 4 * (size_t)n, *u = sv + 5 * (size_t)n;
+    const T* E0 = F + front_elems;
+    const T* E1 = E0 + n;
     tbeg(TC_VEC);
     launch_gather_perm<double, T>(n, d_perm, d_b, bp, st);
     tend();
-    if (dag_on) {
-      DagCtl c;
-      c.ctr = d_dagctr;
-      c.tchild = d_tchild;
-      c.nbx = d_nbx;
-      c.sparent = d_spar;
-      c.nsuper = S.nsuper;
-      c.ntasks = ndag;
-      tbeg(TC_SLEVEL);
-      launch_solve_dag<T>(d_dag, std::max(1, std::min(ndag, 1024)), c, ds, F, bp, fp, y, u, v, xp, part,
-                          d_poff, st);
-      tend();
-      tbeg(TC_VEC);
-      launch_scatter_perm<T, double>(n, d_perm, xp, d_x, st);
-      tend();
-      return;
-    }
     const int nl = S.nlevels;
+    tbeg(TC_SLEVEL);
     for (int L = 0; L < nl; L++) {
-      tbeg(TC_SLEVEL);
-      launch_fwd_gather<T>(d_lvlsn + sn_l[L].off, sn_l[L].cnt, ds, bp, fp, u, st);
-      tend();
-      tbeg(TC_SGEMV);
-      launch_fwd_xgemv<T>(d_xg + xg_l[L].off, xg_l[L].cnt, ds, F, fp, y, st);
-      launch_fwd_gemv<T>(d_fg + fg_l[L].off, fg_l[L].cnt, ds, F, y, u, st);
-      tend();
+      launch_fwd_gather<T>(d_fg + fg_l[L].off, fg_l[L].cnt, ds, bp, fp, u, st);
+      launch_fwd_rows<T>(d_fr + fr_l[L].off, fr_l[L].cnt, ds, F, fp, y, u, st);
     }
-    for (int L = nl - 1; L >= 0; L--) {
-      tbeg(TC_SGEMV);
-      launch_bwd_vgemv<T>(d_bg + bg_l[L].off, bg_l[L].cnt, ds, F, y, xp, v, st);
-      launch_bwd_xpart<T>(d_xp + xp_l[L].off, xp_l[L].cnt, ds, F, v, d_poff, part, st);
-      tend();
-      tbeg(TC_SLEVEL);
-      launch_bwd_finish<T>(d_lvlsn + sn_l[L].off, sn_l[L].cnt, ds, v, d_poff, part, xp, st);
-      tend();
-    }
+    launch_dsolve<T>(n, E0, E1, d_piv, y, z, st);
+    tend();
+    tbeg(TC_SGEMV);
+    for (int L = nl - 1; L >= 0; L--) launch_bwd_cols<T>(d_bc + bc_l[L].off, bc_l[L].cnt, ds, F, z, xp, st);
+    tend();
     tbeg(TC_VEC);
     launch_scatter_perm<T, double>(n, d_perm, xp, d_x, st);
     tend();
@@ -908,26 +855,10 @@ struct kkt_solver {
     }
     if (hipStreamSynchronize(st) != hipSuccess) return fail(KKT_ERR_SOLVE, "solve kernels failed");
     tflush();
-    if (dag_failed()) return solve_device(d_b, d_x);
     if (g_timing)
       fprintf(stderr, "[kkt-timing] solve n=%d: %.3f ms (refine steps %d, rel resid %.3g)\n", n,
               now_ms() - t0, last_refine_steps, last_relres);
     return KKT_OK;
-  }
-
-  // the dependency-driven solve flags a wait that exceeded its bound; the
-  // level-scheduled launches then take over for this handle
-  bool dag_failed() {
-    if (!dag_on || !d_dagctr) return false;
-    int e = 0;
-    if (hipMemcpyAsync(&e, d_dagctr + 1, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-      return false;
-    if (e == 0) return false;
-    dag_on = false;
-    drop_graphs();
-    if (g_timing) fprintf(stderr, "[kkt-timing] solve task graph timed out; level launches enabled\n");
-    return true;
   }
 
   int set_values_device(const double* dv) {
@@ -1018,7 +949,6 @@ struct kkt_solver {
         hipStreamSynchronize(st) != hipSuccess)
       return fail(KKT_ERR_SOLVE, "iterative solve failed");
     tflush();
-    if (dag_failed()) return bicgstab(d_b, d_xout, tolRel, maxIter, iters);
     *iters = it;
     if (g_timing)
       fprintf(stderr, "[kkt-timing] bicgstab n=%d iters=%d: total=%.2fms (%.2fms/iter)\n", n, it,
@@ -1211,15 +1141,6 @@ int kkt_set_refine_tol(kkt_solver* h, double tol, int max_steps) {
   if (!(tol > 0) || max_steps < 1) return h->fail(KKT_ERR_ARGUMENT, "bad refinement settings");
   h->refine_tol = tol;
   h->refine_max = max_steps;
-  h->ok();
-  return KKT_OK;
-}
-
-int kkt_set_solve_mode(kkt_solver* h, int mode) {
-  CHECK_H(h);
-  if (mode != 0 && mode != 1) return h->fail(KKT_ERR_ARGUMENT, "unknown solve mode");
-  h->dag_on = (mode == 1);
-  if (h->dev_ok) h->drop_graphs();
   h->ok();
   return KKT_OK;
 }

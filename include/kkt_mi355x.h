@@ -129,12 +129,6 @@ int kkt_set_refine_tol(kkt_solver* h, double tol, int max_steps);
 /* 1 (default): the factor and triangular-solve launch sequences are captured once
  * per analysis and precision into HIP graphs and replayed; 0: launched eagerly. */
 int kkt_set_graphs(kkt_solver* h, int on);
-/* Triangular solves: 0 (default) level-by-level launches; 1 one persistent
- * dependency-driven launch for both sweeps.  Both give bitwise-identical
- * results.  Mode 1 publishes every task with device-scope release/acquire,
- * which on a multi-XCD GPU writes back and invalidates the issuing XCD's L2;
- * on large systems it is slower than mode 0 and is kept for small ones. */
-int kkt_set_solve_mode(kkt_solver* h, int mode);
 /* ordering: 0 natural, 1 nested dissection (default) */
 int kkt_set_ordering(kkt_solver* h, int ordering);
 int kkt_get_info(kkt_solver* h, kkt_info* out);
@@ -148,8 +142,8 @@ int kkt_get_symbolic(kkt_solver* h, int* perm, int* sup_ptr, int* sparent, long 
 int kkt_verify_device_structure(kkt_solver* h);
 /* Kernel timing (HIP events on the solver's stream) accumulated since the
  * last reset, per kernel class: 0 assemble, 1 panel-diag (+ inverse rows),
- * 2 panel-trsm, 3 panel-update (MFMA), 4 solve-level, 5 solve-gemv, 6 spmv,
- * 7 vector ops.  enable: 0 off, -1 all classes, else a bit mask (1 << class).
+ * 2 panel-trsm (+ the solve-form post-passes), 3 panel-update (MFMA),
+ * 4 forward solve, 5 backward solve, 6 spmv, 7 vector ops.  enable: 0 off, -1 all classes, else a bit mask (1 << class).
  * Solves replay a captured HIP graph unless a solve class is timed.
  * ms[c], launches[c]; `update_flops` = algorithmic flops of class-3 launches. */
 void kkt_timing_reset(kkt_solver* h, int enable);

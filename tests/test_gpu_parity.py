@@ -255,30 +255,39 @@ def test_serving_alone_equals_mix(gpu):
 
 
 @pytest.mark.parametrize("prec", [KKT_PREC_FP64, KKT_PREC_FP32_REFINE])
-def test_dependency_driven_solve_matches_level_solve(gpu, prec):
-    # the persistent task-queue sweep and the level-by-level launches compute the
-    # same arithmetic in the same order: bitwise-identical solutions
-    _say("dag vs level solve prec=%d" % prec)
-    import ctypes
-    from artisynth_core_amd._lib import ptr as nptr
+def test_graph_replays_match_eager_launches(gpu, prec):
+    # the factor and solve launch sequences are captured once into HIP graphs and
+    # replayed; every replay (with eager work of another handle in between) must
+    # give the bitwise result of the eagerly launched kernels
+    _say("graph replays vs eager prec=%d" % prec)
     beam = G.HexBeam(24, 8, 8, 1.0, 0.2, material=("neohookean", 3.4e4, 1.7e6), h=0.005)
     u = np.random.default_rng(9).uniform(-1e-4, 1e-4, (beam.nn, 3))
     vals, b = beam.system(u, np.zeros_like(u), beam.gravity_loads())
-    xs = []
-    for mode in (1, 0):
+    rng = np.random.default_rng(10)
+    rhs = [b] + [rng.uniform(-1, 1, beam.n) for _ in range(3)]
+    solvers = []
+    for graphs in (0, 1):
         s = MI355XSolver(precision=prec)
-        try:
-            s._L.kkt_set_solve_mode(s._h, mode)
-            s.setIterativeRefinementSteps(0)
-            s.setRefinement(1e-10, 10)
-            s.analyzeArrays(vals, beam.colIdxs, beam.rowOffs, beam.n, Matrix.SYMMETRIC)
-            s.factor(vals)
-            x = np.zeros(beam.n)
-            s.solve(x, b)
-            it = s.iterativeSolve(vals * 1.01, np.zeros(beam.n), b, 10)
-            assert it > 0
-            xs.append(x)
-        finally:
+        s._L.kkt_set_graphs(s._h, graphs)
+        s.setIterativeRefinementSteps(0)
+        s.setRefinement(1e-10, 10)
+        s.analyzeArrays(vals, beam.colIdxs, beam.rowOffs, beam.n, Matrix.SYMMETRIC)
+        solvers.append(s)
+    try:
+        for rep in range(3):
+            v = vals * (1.0 + 0.01 * rep)
+            xs = []
+            for s in solvers:
+                s.factor(v)
+                out = []
+                for r in rhs:
+                    x = np.zeros(beam.n)
+                    s.solve(x, r)
+                    out.append(x)
+                xs.append(out)
+            for xe, xg in zip(*xs):
+                assert np.array_equal(xe, xg)
+            assert rel_residual(beam.rowOffs, beam.colIdxs, v, xs[1][0], b) <= 1e-8
+    finally:
+        for s in solvers:
             s.dispose()
-    assert np.array_equal(xs[0], xs[1])
-    assert rel_residual(beam.rowOffs, beam.colIdxs, vals, xs[0], b) <= 1e-8
