@@ -23,7 +23,13 @@ struct DevSym {
                              // stored at rel[sn_roff[c] ...]
   const int* rows;           // global permuted row index of each below-diagonal row
   int* lperm;                // per permuted column: original local index in supernode
+  int* ilperm;               // inverse of lperm (local index -> pivot position)
   signed char* piv;          // 1 = 1x1, 2 = first of 2x2, 0 = second of 2x2
+  // forward gather map: front position p of s receives u[gsrc[e]] for
+  // e in [gptr[sn_goff[s] + p], gptr[sn_goff[s] + p + 1]), children in ascending order
+  const long long* sn_goff;
+  const int* gptr;
+  const int* gsrc;
 };
 
 struct FactorCounters {  // device ints
@@ -56,16 +62,16 @@ template <typename T>
 void launch_gform(const int4* items, int nitems, const DevSym& d, T* F, hipStream_t st);
 // level-scheduled solves (vectors in the front precision T)
 template <typename T>
-void launch_fwd_gather(const int* items, int nitems, const DevSym& d, const T* bp, T* fp, T* u,
+void launch_fwd_gather(const int2* items, int nitems, const DevSym& d, const T* bp, T* fp, T* u,
                        hipStream_t st);
 template <typename T>
-void launch_fwd_rows(const int2* items, int nitems, const DevSym& d, const T* F, const T* fp, T* y,
+void launch_fwd_rows(const int4* items, int nitems, const DevSym& d, const T* F, const T* fp, T* y,
                      T* u, hipStream_t st);
 template <typename T>
 void launch_dsolve(int n, const T* E0, const T* E1, const signed char* piv, const T* y, T* z,
                    hipStream_t st);
 template <typename T>
-void launch_bwd_cols(const int2* items, int nitems, const DevSym& d, const T* F, const T* z, T* xp,
+void launch_bwd_cols(const int4* items, int nitems, const DevSym& d, const T* F, const T* z, T* xp,
                      hipStream_t st);
 
 // permutation / precision conversion: out[j] = (TO) in[perm[j]]  (gather)

@@ -337,6 +337,7 @@ __global__ __launch_bounds__(256) void k_panel_diag(const int* __restrict__ item
     if (t < w) Fs[(size_t)(j0 + j) * m + j0 + t] = (T)A[t * 65 + j];
   if (wv == 0 && t < w) {
     d.lperm[first + j0 + t] = j0 + lp[t];
+    d.ilperm[first + j0 + lp[t]] = j0 + t;
     d.piv[first + j0 + t] = pt[t];
     // D^-1 for the solves: z_t = E0_t y_t + E1_t y_partner (partner t+1 / t-1
     // for the first / second column of a 2x2 pivot)
@@ -732,63 +733,59 @@ __global__ __launch_bounds__(256) void k_gform(const int4* __restrict__ items, D
   }
 }
 
-// forward gather of one supernode: f = b + the children's update vectors;
-// fp = f in pivot order, u_s = the children's sums on the contribution rows
+// forward gather, one thread per front position p (children summed in
+// ascending order): f_p = b_p + sum of the children's update entries at p;
+// fp = f in pivot order (p < k), u_s = the sums on the contribution rows (p >= k);
+// item = (s, block of 256 positions)
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_gather(const int* __restrict__ items, DevSym d,
+__global__ __launch_bounds__(256) void k_fwd_gather(const int2* __restrict__ items, DevSym d,
                                                     const T* __restrict__ bp, T* __restrict__ fp,
                                                     T* __restrict__ u) {
-  __shared__ T g[KMAX];
-  const int s = items[blockIdx.x];
-  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s], r = m - k;
-  T* us = u + d.sn_uoff[s];
-  const int tid = threadIdx.x;
-  for (int q = tid; q < k; q += 256) g[q] = bp[first + q];
-  for (int i = tid; i < r; i += 256) us[i] = (T)0;
-  __syncthreads();
-  for (int q = d.ch_ptr[s]; q < d.ch_ptr[s + 1]; q++) {
-    const int c = d.ch_list[q];
-    const int rc = d.sn_m[c] - d.sn_k[c];
-    const int* relc = d.rel + d.sn_roff[c];
-    const T* uc = u + d.sn_uoff[c];
-    for (int i = tid; i < rc; i += 256) {
-      const int pos = relc[i];
-      const T v = uc[i];
-      if (pos < k) g[pos] += v; else us[pos - k] += v;
-    }
-    __syncthreads();
-  }
-  for (int q = tid; q < k; q += 256) fp[first + q] = g[d.lperm[first + q]];
+  const int2 it = items[blockIdx.x];
+  const int s = it.x, p = it.y * 256 + threadIdx.x;
+  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s];
+  if (p >= m) return;
+  const int* gp = d.gptr + d.sn_goff[s] + p;
+  T acc = (p < k) ? bp[first + p] : (T)0;
+  for (int e = gp[0]; e < gp[1]; e++) acc += u[d.gsrc[e]];
+  if (p < k) fp[first + d.ilperm[first + p]] = acc;
+  else u[d.sn_uoff[s] + p - k] = acc;
 }
 
-// forward products, one wave per front row i, from the upper copy (column i,
-// rows l < min(i, k), contiguous): y_i = f_i + G(i,:) f (i < k) or
-// u_{i-k} -= G(i,:) f (i >= k); item = (s, 4 rows)
+// sum over aligned segments of 2^lg lanes (every lane of a segment gets the sum)
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_rows(const int2* __restrict__ items, DevSym d,
+__device__ inline T seg_sum(T v, int lg) {
+  for (int off = (1 << lg) >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// forward products from the upper copy (column i, rows l < min(i, k),
+// contiguous): y_i = f_i + G(i,:) f (i < k) or u_{i-k} -= G(i,:) f (i >= k).
+// A segment of 2^lg lanes per row, 64 >> lg rows per wave;
+// item = (s, block of 4 * (64 >> lg) rows, lg)
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_rows(const int4* __restrict__ items, DevSym d,
                                                   const T* __restrict__ F,
                                                   const T* __restrict__ fp, T* __restrict__ y,
                                                   T* __restrict__ u) {
-  const int2 it = items[blockIdx.x];
-  const int s = it.x;
+  const int4 it = items[blockIdx.x];
+  const int s = it.x, lg = it.z;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i = it.y * 4 + wv;
+  const int seg = 1 << lg, sl = lane & (seg - 1);
+  const int i = (it.y * 4 + wv) * (64 >> lg) + (lane >> lg);
   const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s];
-  if (i >= m) return;
   const T* col = F + d.sn_foff[s] + (size_t)i * m;
   const T* f = fp + first;
-  const int len = min(i, k);
-  T a0 = (T)0, a1 = (T)0, a2 = (T)0, a3 = (T)0;
-  int l = lane;
-  for (; l + 192 < len; l += 256) {
+  const int len = (i < m) ? min(i, k) : 0;
+  T a0 = (T)0, a1 = (T)0;
+  int l = sl;
+  for (; l + seg < len; l += 2 * seg) {
     a0 += col[l] * f[l];
-    a1 += col[l + 64] * f[l + 64];
-    a2 += col[l + 128] * f[l + 128];
-    a3 += col[l + 192] * f[l + 192];
+    a1 += col[l + seg] * f[l + seg];
   }
-  for (; l < len; l += 64) a0 += col[l] * f[l];
-  const T acc = wave_sum((a0 + a1) + (a2 + a3));
-  if (lane == 0) {
+  if (l < len) a0 += col[l] * f[l];
+  const T acc = seg_sum(a0 + a1, lg);
+  if (sl == 0 && i < m) {
     if (i < k) y[first + i] = f[i] + acc;
     else u[d.sn_uoff[s] + i - k] -= acc;
   }
@@ -807,38 +804,40 @@ __global__ __launch_bounds__(256) void k_dsolve(int n, const T* __restrict__ E0,
   z[g] = E0[g] * y[g] + E1[g] * y[pg];
 }
 
-// backward, one wave per pivot column c, from the lower copy (column c, rows
-// below the diagonal, contiguous): x_c = z_c + sum_{c<i<k} X(i,c) z_i
-//   - sum_j Z(j,c) x[rows_j]; item = (s, 4 columns)
+// backward from the lower copy (column c, rows below the diagonal,
+// contiguous): x_c = z_c + sum_{c<i<k} X(i,c) z_i - sum_j Z(j,c) x[rows_j].
+// A segment of 2^lg lanes per column; item = (s, block of 4 * (64 >> lg) columns, lg)
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_cols(const int2* __restrict__ items, DevSym d,
+__global__ __launch_bounds__(256) void k_bwd_cols(const int4* __restrict__ items, DevSym d,
                                                   const T* __restrict__ F,
                                                   const T* __restrict__ z, T* __restrict__ xp) {
-  const int2 it = items[blockIdx.x];
-  const int s = it.x;
+  const int4 it = items[blockIdx.x];
+  const int s = it.x, lg = it.z;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int c = it.y * 4 + wv;
-  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s], r = m - k;
-  if (c >= k) return;
-  const T* col = F + d.sn_foff[s] + (size_t)c * m;
+  const int seg = 1 << lg, sl = lane & (seg - 1);
+  const int c = (it.y * 4 + wv) * (64 >> lg) + (lane >> lg);
+  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s];
+  const bool ok = c < k;
+  const int kx = ok ? k : 0, r = ok ? m - k : 0;
+  const T* col = F + d.sn_foff[s] + (size_t)(ok ? c : 0) * m;
   const T* zs = z + first;
   const int* rows = d.rows + d.sn_roff[s];
   T a0 = (T)0, a1 = (T)0, a2 = (T)0, a3 = (T)0;
-  int i = c + 1 + lane;
-  for (; i + 64 < k; i += 128) {
+  int i = c + 1 + sl;
+  for (; i + seg < kx; i += 2 * seg) {
     a0 += col[i] * zs[i];
-    a1 += col[i + 64] * zs[i + 64];
+    a1 += col[i + seg] * zs[i + seg];
   }
-  if (i < k) a0 += col[i] * zs[i];
+  if (i < kx) a0 += col[i] * zs[i];
   const T* zc = col + k;
-  int j = lane;
-  for (; j + 64 < r; j += 128) {
+  int j = sl;
+  for (; j + seg < r; j += 2 * seg) {
     a2 -= zc[j] * xp[rows[j]];
-    a3 -= zc[j + 64] * xp[rows[j + 64]];
+    a3 -= zc[j + seg] * xp[rows[j + seg]];
   }
   if (j < r) a2 -= zc[j] * xp[rows[j]];
-  const T acc = wave_sum((a0 + a1) + (a2 + a3));
-  if (lane == 0) xp[first + d.lperm[first + c]] = zs[c] + acc;
+  const T acc = seg_sum((a0 + a1) + (a2 + a3), lg);
+  if (sl == 0 && ok) xp[first + d.lperm[first + c]] = zs[c] + acc;
 }
 
 // ------------------------------------------------------------------------
@@ -1027,12 +1026,12 @@ void launch_gform(const int4* items, int nitems, const DevSym& d, T* F, hipStrea
   if (nitems > 0) hipLaunchKernelGGL(k_gform<T>, dim3(nitems), dim3(256), 0, st, items, d, F);
 }
 template <typename T>
-void launch_fwd_gather(const int* items, int nitems, const DevSym& d, const T* bp, T* fp, T* u,
+void launch_fwd_gather(const int2* items, int nitems, const DevSym& d, const T* bp, T* fp, T* u,
                        hipStream_t st) {
   if (nitems > 0) hipLaunchKernelGGL(k_fwd_gather<T>, dim3(nitems), dim3(256), 0, st, items, d, bp, fp, u);
 }
 template <typename T>
-void launch_fwd_rows(const int2* items, int nitems, const DevSym& d, const T* F, const T* fp, T* y,
+void launch_fwd_rows(const int4* items, int nitems, const DevSym& d, const T* F, const T* fp, T* y,
                      T* u, hipStream_t st) {
   if (nitems > 0)
     hipLaunchKernelGGL(k_fwd_rows<T>, dim3(nitems), dim3(256), 0, st, items, d, F, fp, y, u);
@@ -1043,7 +1042,7 @@ void launch_dsolve(int n, const T* E0, const T* E1, const signed char* piv, cons
   if (n > 0) hipLaunchKernelGGL(k_dsolve<T>, dim3(nblk(n, 256)), dim3(256), 0, st, n, E0, E1, piv, y, z);
 }
 template <typename T>
-void launch_bwd_cols(const int2* items, int nitems, const DevSym& d, const T* F, const T* z, T* xp,
+void launch_bwd_cols(const int4* items, int nitems, const DevSym& d, const T* F, const T* z, T* xp,
                      hipStream_t st) {
   if (nitems > 0) hipLaunchKernelGGL(k_bwd_cols<T>, dim3(nitems), dim3(256), 0, st, items, d, F, z, xp);
 }
@@ -1116,13 +1115,13 @@ void launch_zero_counters(FactorCounters* c, hipStream_t st) {
   template void launch_panel_xrow<T>(const int2*, int, int, const DevSym&, T*, hipStream_t);     \
   template void launch_zt<T>(const int4*, int, const DevSym&, T*, hipStream_t);                \
   template void launch_gform<T>(const int4*, int, const DevSym&, T*, hipStream_t);             \
-  template void launch_fwd_gather<T>(const int*, int, const DevSym&, const T*, T*, T*,           \
+  template void launch_fwd_gather<T>(const int2*, int, const DevSym&, const T*, T*, T*,           \
                                      hipStream_t);                                               \
-  template void launch_fwd_rows<T>(const int2*, int, const DevSym&, const T*, const T*, T*, T*,  \
+  template void launch_fwd_rows<T>(const int4*, int, const DevSym&, const T*, const T*, T*, T*,  \
                                    hipStream_t);                                                 \
   template void launch_dsolve<T>(int, const T*, const T*, const signed char*, const T*, T*,      \
                                  hipStream_t);                                                   \
-  template void launch_bwd_cols<T>(const int2*, int, const DevSym&, const T*, const T*, T*,      \
+  template void launch_bwd_cols<T>(const int4*, int, const DevSym&, const T*, const T*, T*,      \
                                    hipStream_t);
 KKT_INST(double)
 KKT_INST(float)

@@ -97,9 +97,11 @@ struct kkt_solver {
   signed char* d_piv = nullptr;
   // plan
   int4 *d_asm = nullptr, *d_upd = nullptr;
-  int *d_diag = nullptr, *d_aidx = nullptr, *d_fg = nullptr;
-  int2 *d_trsm = nullptr, *d_xrow = nullptr, *d_fr = nullptr, *d_bc = nullptr;
-  int4 *d_zt = nullptr, *d_gf = nullptr;
+  int *d_diag = nullptr, *d_aidx = nullptr;
+  int2 *d_trsm = nullptr, *d_xrow = nullptr, *d_fg = nullptr;
+  int *d_ilperm = nullptr, *d_gptr = nullptr, *d_gsrc = nullptr;
+  long long* d_goff = nullptr;
+  int4 *d_zt = nullptr, *d_gf = nullptr, *d_fr = nullptr, *d_bc = nullptr;
   std::vector<Launch> xrow_l;  // inverse-row post-pass, one launch per panel index p >= 1
   Launch zt_all, gf_all;       // Z^T and G-form post-passes, one launch each
   long long* d_apos = nullptr;
@@ -422,6 +424,30 @@ struct kkt_solver {
         rel[i] = pos;
       }
     }
+    // forward gather map: front position -> the children's update entries, in
+    // ascending child order (the summation order of the forward sweep)
+    std::vector<long long> goff(ns + 1, 0);
+    for (int s = 0; s < ns; s++) goff[s + 1] = goff[s] + mm[s];
+    std::vector<int> gptr(goff[ns] + 1, 0), gsrc;
+    for (int s = 0; s < ns; s++)
+      for (int q = chp[s]; q < chp[s + 1]; q++) {
+        const int c = chl[q];
+        for (long long i = roff[c]; i < roff[c + 1]; i++) gptr[goff[s] + rel[i] + 1]++;
+      }
+    for (long long g = 0; g < goff[ns]; g++) {
+      if ((long long)gptr[g] + gptr[g + 1] > INT32_MAX) return fail(KKT_ERR_ANALYSIS, "gather map too large");
+      gptr[g + 1] += gptr[g];
+    }
+    gsrc.assign(std::max(gptr[goff[ns]], 1), 0);
+    {
+      std::vector<int> fill(gptr.begin(), gptr.end() - 1);
+      for (int s = 0; s < ns; s++)
+        for (int q = chp[s]; q < chp[s + 1]; q++) {
+          const int c = chl[q];
+          for (long long i = roff[c]; i < roff[c + 1]; i++)
+            gsrc[fill[goff[s] + rel[i]]++] = (int)(uoff[c] + (i - roff[c]));
+        }
+    }
     // levels
     int nl = S.nlevels;
     std::vector<std::vector<int>> lv(nl);
@@ -455,8 +481,8 @@ struct kkt_solver {
     std::vector<int> diagv, aidx;
     std::vector<long long> apos;
     std::vector<int2> trsmv, xrowv;
-    std::vector<int> fgv;
-    std::vector<int2> frv, bcv;
+    std::vector<int2> fgv;
+    std::vector<int4> frv, bcv;
     asm_l.assign(nl, Launch());
     scat_l.assign(nl, Launch());
     fg_l.assign(nl, Launch());
@@ -478,10 +504,18 @@ struct kkt_solver {
       fg_l[L].off = (long long)fgv.size();
       fr_l[L].off = (long long)frv.size();
       bc_l[L].off = (long long)bcv.size();
+      // lanes per row / column: the dot length rounded up to a power of two (4..64)
+      auto seg_lg = [](int len) {
+        int lg = 2;
+        while (lg < 6 && (1 << lg) < len) lg++;
+        return lg;
+      };
       for (int s : lv[L]) {
-        fgv.push_back(s);
-        for (int rb = 0; rb < (mm[s] + 3) / 4; rb++) frv.push_back(make_int2(s, rb));
-        for (int cb = 0; cb < (kk[s] + 3) / 4; cb++) bcv.push_back(make_int2(s, cb));
+        for (int gb = 0; gb < (mm[s] + 255) / 256; gb++) fgv.push_back(make_int2(s, gb));
+        const int lf = seg_lg(kk[s]), rf = 4 * (64 >> lf);
+        for (int rb = 0; rb < (mm[s] + rf - 1) / rf; rb++) frv.push_back(make_int4(s, rb, lf, 0));
+        const int lb = seg_lg(mm[s]), rbw = 4 * (64 >> lb);
+        for (int cb = 0; cb < (kk[s] + rbw - 1) / rbw; cb++) bcv.push_back(make_int4(s, cb, lb, 0));
       }
       fg_l[L].cnt = (int)(fgv.size() - fg_l[L].off);
       fr_l[L].cnt = (int)(frv.size() - fr_l[L].off);
@@ -588,11 +622,14 @@ struct kkt_solver {
     for (int s = 0; s < ns; s++)
       for (int t = 0; t < kk[s]; t++) lp0[first[s] + t] = t;
     std::vector<signed char> pv0(n, 1);
-    if ((rc = upload(d_lperm, lp0)) || (rc = upload(d_piv, pv0))) return rc;
+    if ((rc = upload(d_lperm, lp0)) || (rc = upload(d_ilperm, lp0)) || (rc = upload(d_piv, pv0)) ||
+        (rc = upload(d_goff, goff)) || (rc = upload(d_gptr, gptr)) || (rc = upload(d_gsrc, gsrc)))
+      return rc;
     ds.sn_first = d_first; ds.sn_k = d_k; ds.sn_m = d_m;
     ds.sn_foff = d_foff; ds.sn_roff = d_roff; ds.sn_woff = d_woff; ds.sn_uoff = d_uoff;
     ds.ch_ptr = d_chp; ds.ch_list = d_chl; ds.rel = d_rel; ds.rows = d_rows;
-    ds.lperm = d_lperm; ds.piv = d_piv;
+    ds.lperm = d_lperm; ds.ilperm = d_ilperm; ds.piv = d_piv;
+    ds.sn_goff = d_goff; ds.gptr = d_gptr; ds.gsrc = d_gsrc;
     // host copies of what was uploaded, for the structure-parity check
     h_first = first; h_k = kk; h_m = mm; h_foff = foff; h_roff = roff; h_chp = chp; h_chl = chl;
     h_rel = rel;
