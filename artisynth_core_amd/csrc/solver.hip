@@ -504,10 +504,11 @@ struct kkt_solver {
       fg_l[L].off = (long long)fgv.size();
       fr_l[L].off = (long long)frv.size();
       bc_l[L].off = (long long)bcv.size();
-      // lanes per row / column: the dot length rounded up to a power of two (4..64)
+      // lanes per row / column: about four loads per lane, a power of two (1..64),
+      // so short dot products pack many rows into a wave
       auto seg_lg = [](int len) {
-        int lg = 2;
-        while (lg < 6 && (1 << lg) < len) lg++;
+        int lg = 0;
+        while (lg < 6 && 4 * (1 << lg) < len) lg++;
         return lg;
       };
       for (int s : lv[L]) {
