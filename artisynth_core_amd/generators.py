@@ -113,10 +113,11 @@ class HexBeam:
         self._build_pattern()
 
     def _build_pattern(self):
-        E = len(self.elems)
-        dofs = (3 * self.elems[:, :, None] + np.arange(3)[None, None]).reshape(E, 24)
-        r = np.repeat(dofs, 24, axis=1).ravel()
-        c = np.tile(dofs, (1, 24)).ravel()
+        E, npe = self.elems.shape
+        nd = 3 * npe
+        dofs = (3 * self.elems[:, :, None] + np.arange(3)[None, None]).reshape(E, nd)
+        r = np.repeat(dofs, nd, axis=1).ravel()
+        c = np.tile(dofs, (1, nd)).ravel()
         n = self.sizeM
         keep = r <= c
         self._ekeep = keep
@@ -191,7 +192,8 @@ class HexBeam:
         kv = np.bincount(self._einv, weights=np.ascontiguousarray(Ke).reshape(-1)[self._ekeep.ravel()],
                          minlength=len(self._ukeys))
         fint = np.zeros(n)
-        dofs = (3 * self.elems[:, :, None] + np.arange(3)[None, None]).reshape(E, 24)
+        nd = 3 * self.elems.shape[1]
+        dofs = (3 * self.elems[:, :, None] + np.arange(3)[None, None]).reshape(E, nd)
         np.add.at(fint, dofs.ravel(), fint_e.ravel())
         h, beta = self.h, self.beta
         a = h * beta + h * h
@@ -351,6 +353,109 @@ def assemble_tet(nodes, tets, lam, mu, density):
     m = np.zeros(len(nodes))
     np.add.at(m, tets.ravel(), np.repeat(density * vol / 4.0, 4))
     return K, np.repeat(m, 3)
+
+
+def mooney_rivlin_PA(F, c10, kappa):
+    """First Piola stress P and tangent A = dP/dF (A[...,i,J,k,L]) of the compressible
+    Mooney-Rivlin material with c01 = 0: W = c10 (J^-2/3 I1 - 3) + kappa/2 (J - 1)^2.
+    With H = F^-T and Jm = J^-2/3:
+      P = 2 c10 Jm (F - I1/3 H) + kappa (J - 1) J H
+      A = 2 c10 Jm [d_ik d_JL - 2/3 (F_iJ H_kL + H_iJ F_kL) + 2/9 I1 H_iJ H_kL
+                    + I1/3 H_iL H_kJ] + kappa [(2J - 1) J H_iJ H_kL - (J - 1) J H_iL H_kJ]
+    At F = I this is linear elasticity with mu = 2 c10, lambda = kappa - 2 mu / 3."""
+    J = np.linalg.det(F)
+    H = np.swapaxes(np.linalg.inv(F), -1, -2)
+    I1 = np.einsum("...iJ,...iJ->...", F, F)
+    Jm = J ** (-2.0 / 3.0)
+    a = (2.0 * c10 * Jm)[..., None, None]
+    P = a * (F - (I1 / 3.0)[..., None, None] * H) + (kappa * (J - 1.0) * J)[..., None, None] * H
+    I3 = np.eye(3)
+    a4 = a[..., None, None]
+    A = a4 * (np.einsum("ik,JL->iJkL", I3, I3)
+              - (2.0 / 3.0) * (np.einsum("...iJ,...kL->...iJkL", F, H) + np.einsum("...iJ,...kL->...iJkL", H, F))
+              + (2.0 / 9.0) * I1[..., None, None, None, None] * np.einsum("...iJ,...kL->...iJkL", H, H)
+              + (I1 / 3.0)[..., None, None, None, None] * np.einsum("...iL,...kJ->...iJkL", H, H))
+    A = A + (kappa * (2.0 * J - 1.0) * J)[..., None, None, None, None] * np.einsum("...iJ,...kL->...iJkL", H, H)
+    A = A - (kappa * (J - 1.0) * J)[..., None, None, None, None] * np.einsum("...iL,...kJ->...iJkL", H, H)
+    return P, A
+
+
+class TetBody(HexBeam):
+    """Linear-tet body (4 nodes, constant strain) with its x = min face attached by
+    point constraints, compressible Mooney-Rivlin material (c10, c01 = 0, kappa);
+    the KKT assembly (pattern, slot map, ConstrainedBackwardEuler values) is the
+    hex beam's."""
+
+    def __init__(self, nodes, tets, c10, kappa, density=1000.0, stiffness_damping=0.002, h=0.01):
+        self.nodes = np.asarray(nodes, np.float64)
+        self.elems = np.asarray(tets, np.int64)
+        self.nn = len(self.nodes)
+        self.sizeM = 3 * self.nn
+        self.h = h
+        self.beta = stiffness_damping
+        self.material = ("mooney-rivlin", c10, kappa)
+        P0 = self.nodes[self.elems[:, 0]]
+        Dm = np.stack([self.nodes[self.elems[:, q]] - P0 for q in (1, 2, 3)], 2)
+        self.vol = np.abs(np.linalg.det(Dm)) / 6.0
+        Dinv = np.linalg.inv(Dm)
+        g = np.zeros((len(self.elems), 4, 3))
+        g[:, 1:] = Dinv
+        g[:, 0] = -Dinv.sum(1)
+        self.grads = g  # reference shape-function gradients, (E, 4, 3)
+        m_node = np.zeros(self.nn)
+        np.add.at(m_node, self.elems.ravel(), np.repeat(density * self.vol / 4.0, 4))
+        self.mass = np.repeat(m_node, 3)
+        xmin = self.nodes[:, 0].min()
+        self.attached = np.nonzero(np.isclose(self.nodes[:, 0], xmin))[0]
+        self.tip = np.nonzero(np.isclose(self.nodes[:, 0], self.nodes[:, 0].max()))[0]
+        nG = 3 * len(self.attached)
+        rows = (3 * self.attached[:, None] + np.arange(3)[None]).ravel()
+        self.GT = sp.csr_matrix((np.ones(nG), (rows, np.arange(nG))), shape=(self.sizeM, nG))
+        self.numG = nG
+        self._build_pattern()
+
+    def element_tangent(self, u):
+        c10, kappa = self.material[1], self.material[2]
+        g = self.grads
+        F = np.eye(3)[None] + np.einsum("eai,eaJ->eiJ", u[self.elems], g)
+        P, A = mooney_rivlin_PA(F, c10, kappa)
+        E = len(self.elems)
+        C = np.einsum("eiJkL,ebL->eiJbk", A, g)
+        Ke = np.einsum("eaJ,eiJbk->eaibk", g, C).reshape(E, 12, 12) * self.vol[:, None, None]
+        fint = np.einsum("eiJ,eaJ->eai", P, g).reshape(E, 12) * self.vol[:, None]
+        return Ke, fint
+
+
+def tet_precision_sequence(nsteps=4, dims=(121, 24, 24), seed=0x7E7, c10=1e3, kappa=1e5,
+                           size=(0.1, 0.02, 0.02), h=0.01):
+    """Precision workload: seeded Delaunay tet box (jittered lattice of dims points,
+    size 0.1 x 0.02 x 0.02 m; (121, 24, 24) gives 69,696 nodes = 209,088 DOF),
+    Mooney-Rivlin c10 = 1e3, c01 = 0, kappa = 1e5 (nu ~ 0.49), rho = 1000, its x = 0
+    face attached.  The nsteps states follow a prescribed, growing bending of the box
+    (tip deflection up to a few percent of its length) under seeded tip loads; each
+    step's KKT system is assembled at its state.  Returns (body, [vals_k], [b_k])."""
+    nodes, tets = delaunay_box(dims[0], dims[1], dims[2], size[0], size[1], size[2], seed)
+    body = TetBody(nodes, tets, c10, kappa, density=1000.0, h=h)
+    rng = np.random.default_rng(seed + 1)
+    X = body.nodes
+    s = (X[:, 0] - X[:, 0].min()) / size[0]
+    shape = s * s * (3.0 - s) / 2.0
+    grav = body.gravity_loads()
+    u_prev = np.zeros((body.nn, 3))
+    out_v, out_b = [], []
+    for k in range(nsteps):
+        defl = size[0] * 0.01 * (k + 1) * (1.0 + 0.2 * rng.uniform(-1, 1, 3))
+        u = np.zeros((body.nn, 3))
+        u[:, 1] = 0.3 * defl[1] * shape
+        u[:, 2] = -defl[2] * shape
+        v = (u - u_prev) / h
+        fext = grav.copy()
+        fext[body.tip] += rng.uniform(-1e-2, 1e-2, 3)[None, :] / len(body.tip)
+        vals, b = body.system(u, v, fext)
+        out_v.append(vals)
+        out_b.append(b)
+        u_prev = u
+    return body, out_v, out_b
 
 
 def rigid_contact_system(ndof, seed, h=0.01):

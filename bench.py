@@ -77,9 +77,14 @@ def main():
     ap.add_argument("--nsys", type=int, default=20, help="systems in the step sequence")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--grid", default="100x16x16")
-    ap.add_argument("--mode", default="bench", choices=["bench", "pmc", "batch", "serve"],
+    ap.add_argument("--mode", default="bench", choices=["bench", "pmc", "batch", "serve", "precision"],
                     help="pmc: one FP32 factor + solve and a 256 MiB copy, for counter runs; "
-                         "batch: the 256-system mix, one batched launch vs per-system launches")
+                         "batch: the 256-system mix, one batched launch vs per-system launches; "
+                         "precision: the ~210k-DOF Mooney-Rivlin tet box, direct FP32-MFMA+refine "
+                         "vs FP64 solves")
+    ap.add_argument("--tet", default="121x24x24", help="precision: Delaunay lattice points")
+    ap.add_argument("--oracle", action="store_true",
+                    help="precision: also run the CPU oracle on step 0 (minutes at full size)")
     ap.add_argument("--nbatch", type=int, default=256)
     ap.add_argument("--ninst", type=int, default=64)
     ap.add_argument("--nreq", type=int, default=2000)
@@ -107,6 +112,9 @@ def main():
         return batch_run(args.nbatch, args.steps, args.warmup)
     if args.mode == "serve":
         return serve_run(args.ninst, args.nreq, args.rate, args.workers, args.budget_gb)
+    if args.mode == "precision":
+        return precision_run(tuple(int(v) for v in args.tet.split("x")), min(args.nsys, 4),
+                             args.warmup, args.oracle)
     t0 = time.time()
     beam, V, B = G.large_beam_sequence(nsteps=args.nsys, nx=nx, ny=ny, nz=nz,
                                        width=0.16 * ny / 16)
@@ -394,6 +402,108 @@ def serve_run(ninst, nreq, rate, nworkers, budget_gb):
         line["throughput_req_s"], rate, st["p50_ms"], st["p99_ms"], max(res)))
     print(json.dumps(line), flush=True)
     srv.shutdown()
+
+
+def precision_run(dims, nsteps, warmup, with_oracle):
+    """seeded Delaunay tet box (0.1 x 0.02 x 0.02 m, ~210k DOF at the
+    default lattice), Mooney-Rivlin c10 = 1e3, c01 = 0, kappa = 1e5, attached end.
+    Every step is a direct solve (factor + solve + refinement to 1e-10): FP32 fronts
+    on MFMA with FP64 refinement against FP64 fronts, inputs resident in HBM.  Per
+    solve: relative residual (host FP64 SpMV), forward error of the FP32 path against
+    the FP64 path, refinement steps, FP64 fallbacks."""
+    import torch
+    import scipy.sparse as sp
+    from artisynth_core_amd import KKT_PREC_FP32_REFINE, KKT_PREC_FP64
+    from artisynth_core_amd import generators as G
+    from artisynth_core_amd._lib import KktInfo, lib
+    from artisynth_core_amd._lib import ptr as nptr
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    body, V, B = G.tet_precision_sequence(nsteps=nsteps, dims=dims)
+    n, nnz = body.n, body.nnz
+    log("generated %d tet systems n=%d nnz=%d (%d tets) in %.1fs" % (
+        len(V), n, nnz, len(body.elems), time.time() - t0))
+    ro = np.ascontiguousarray(body.rowOffs, np.int32)
+    ci = np.ascontiguousarray(body.colIdxs, np.int32)
+    vals_d = [torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in V]
+    b_d = [torch.from_numpy(np.ascontiguousarray(b)).to(dev) for b in B]
+    x_d = torch.zeros(n, dtype=torch.float64, device=dev)
+    L = lib()
+    out = {}
+    for prec, name in ((KKT_PREC_FP32_REFINE, "fp32_mfma_refine"), (KKT_PREC_FP64, "fp64")):
+        h = L.kkt_create()
+        L.kkt_set_precision(h, prec)
+        L.kkt_set_refine_tol(h, 1e-10, 10)
+        if prec == KKT_PREC_FP64:
+            L.kkt_set_ir_steps(h, 1)
+        ta = time.time()
+        assert L.kkt_set_pattern(h, n, nnz, nptr(ro, ctypes.c_int), nptr(ci, ctypes.c_int), 1) == 0
+        v0 = np.ascontiguousarray(V[0])
+        assert L.kkt_analyze_values(h, nptr(v0, ctypes.c_double)) == 0, L.kkt_last_error(h)
+        t_an = time.time() - ta
+        for _ in range(max(warmup, 1)):
+            assert L.kkt_factor_device(h, vals_d[0].data_ptr()) == 0, L.kkt_last_error(h)
+            assert L.kkt_solve_device(h, b_d[0].data_ptr(), x_d.data_ptr()) == 0
+        torch.cuda.synchronize()
+        tf, ts, xs, info = [], [], [], []
+        for k in range(len(V)):
+            t1 = time.perf_counter()
+            assert L.kkt_factor_device(h, vals_d[k].data_ptr()) == 0, L.kkt_last_error(h)
+            t2 = time.perf_counter()
+            assert L.kkt_solve_device(h, b_d[k].data_ptr(), x_d.data_ptr()) == 0, L.kkt_last_error(h)
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+            tf.append(t2 - t1)
+            ts.append(t3 - t2)
+            xs.append(x_d.cpu().numpy().copy())
+            inf = KktInfo()
+            L.kkt_get_info(h, ctypes.byref(inf))
+            info.append(inf.as_dict())
+        res = []
+        for k in range(len(V)):
+            U = sp.csr_matrix((V[k], ci, ro), shape=(n, n))
+            A = U + sp.triu(U, 1).T
+            res.append(float(np.linalg.norm(A @ xs[k] - B[k]) / np.linalg.norm(B[k])))
+        fl = info[0]["factor_flops"]
+        out[name] = {"factor_ms": [1e3 * t for t in tf], "solve_ms": [1e3 * t for t in ts],
+                     "solves_per_s": len(V) / (sum(tf) + sum(ts)),
+                     "factor_tflops": fl / (np.median(tf)) / 1e12,
+                     "rel_residual": res, "refine_steps": [i["refine_steps"] for i in info],
+                     "fp64_fallbacks": info[-1]["fp64_fallbacks"], "analyze_s": t_an,
+                     "npos": info[0]["npos"], "nneg": info[0]["nneg"],
+                     "nperturbed": info[0]["nperturbed"], "n2x2": info[0]["n2x2"], "x": xs}
+        log("%s: %.2f direct solves/s, factor %.1f ms (%.1f TFLOP/s), solve+refine %.1f ms, "
+            "max rel resid %.2e, refine steps %s" % (
+                name, out[name]["solves_per_s"], 1e3 * np.median(tf), out[name]["factor_tflops"],
+                1e3 * np.median(ts), max(res), out[name]["refine_steps"]))
+        L.kkt_destroy(h)
+    r32, r64 = out["fp32_mfma_refine"], out["fp64"]
+    fwd = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(r32["x"], r64["x"])]
+    line = {"mode": "precision", "workload": "Delaunay tet box %s lattice, n=%d, nnz_upper=%d, "
+                                            "%d tets, Mooney-Rivlin c10=1e3 kappa=1e5" % (
+                                                "x".join(map(str, dims)), n, nnz, len(body.elems)),
+            "n": n, "nnz_upper": nnz, "steps": len(V),
+            "factor_gflop": info[0]["factor_flops"] / 1e9, "nnzL": info[0]["nnzL"],
+            "fp32": {k: v for k, v in r32.items() if k != "x"},
+            "fp64": {k: v for k, v in r64.items() if k != "x"},
+            "speedup_vs_fp64": r32["solves_per_s"] / r64["solves_per_s"],
+            "fwd_err_fp32_vs_fp64": fwd}
+    if with_oracle:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle as O
+        from symbolic_helpers import product_symbolic
+        S = product_symbolic(ro, ci, V[0])
+        t1 = time.perf_counter()
+        f = O.Factor(ro, ci, V[0], perm=S["perm"], sup_ptr=S["sup_ptr"])
+        _, xo = f.refine(B[0], tol=1e-10, maxSteps=1)
+        line["oracle_step0"] = {
+            "seconds": time.perf_counter() - t1, "cores": 1,
+            "fp64_err": float(np.linalg.norm(r64["x"][0] - xo) / np.linalg.norm(xo)),
+            "fp32_err": float(np.linalg.norm(r32["x"][0] - xo) / np.linalg.norm(xo))}
+    log("precision: FP32-MFMA+refine %.2fx FP64, fwd err vs FP64 %s" % (
+        line["speedup_vs_fp64"], ["%.1e" % e for e in fwd]))
+    print(json.dumps(line), flush=True)
 
 
 def cpu_baseline(beam, vals, b, x64, x32):

@@ -116,6 +116,31 @@ def test_small_beam_vs_oracle(gpu, nx, prec):
         s.dispose()
 
 
+@pytest.mark.parametrize("prec", [KKT_PREC_FP64, KKT_PREC_FP32_REFINE])
+def test_delaunay_tet_kkt_vs_oracle(gpu, prec):
+    # unstructured mesh (the precision workload's generator at a small lattice):
+    # Mooney-Rivlin tet box in a bent state, attached end
+    _say("delaunay tet 21x6x6 prec=%d" % prec)
+    body, V, B = G.tet_precision_sequence(nsteps=2, dims=(21, 6, 6))
+    vals, b = V[1], B[1]
+    s = MI355XSolver(precision=prec)
+    try:
+        s.analyzeArrays(vals, body.colIdxs, body.rowOffs, body.n, Matrix.SYMMETRIC)
+        assert s.verifyDeviceStructure()
+        s.factor(vals)
+        x = np.zeros(body.n)
+        s.solve(x, b)
+        sym = s.symbolic()
+        ref = O.Factor(body.rowOffs, body.colIdxs, vals, perm=sym["perm"], sup_ptr=sym["sup_ptr"])
+        xo = ref.solve(b)
+        assert rel_residual(body.rowOffs, body.colIdxs, vals, x, b) <= 1e-8
+        assert rel_err(x, xo) <= 1e-8
+        inf, oi = s.info(), ref.info()
+        assert (inf["npos"], inf["nneg"]) == (oi["npos"], oi["nneg"]) == (body.sizeM, body.numG)
+    finally:
+        s.dispose()
+
+
 def test_iterative_solve_stale_factor(gpu):
     # CuDssSolverTest.java:993-1094 / PardisoSolverTest.java:668-688: values perturbed
     # after the factor; BiCGStab with the stale factor reaches 1e-9

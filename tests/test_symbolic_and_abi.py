@@ -165,3 +165,28 @@ def test_estimate_optimal_count():
     assert k.estimateOptimalCount() == int(np.ceil(np.sqrt(2 * (100 - 10 + 2) / 2)))
     k.iterativeTimeMsec, k.iterativeCnt = 300.0, 2
     assert k.estimateOptimalCount() == 3
+
+
+def test_mooney_rivlin_tangent_is_the_stress_derivative():
+    # the precision workload's material: A = dP/dF by central differences, symmetric
+    # in (iJ)<->(kL), and linear elasticity (mu = 2 c10, lambda = kappa - 2mu/3) at F = I
+    rng = np.random.default_rng(3)
+    F = np.eye(3) + 0.1 * rng.standard_normal((6, 3, 3))
+    c10, kappa = 1e3, 1e5
+    P, A = G.mooney_rivlin_PA(F, c10, kappa)
+    eps = 1e-6
+    for k in range(3):
+        for L in range(3):
+            d = np.zeros((3, 3))
+            d[k, L] = eps
+            Pp, _ = G.mooney_rivlin_PA(F + d, c10, kappa)
+            Pm, _ = G.mooney_rivlin_PA(F - d, c10, kappa)
+            fd = (Pp - Pm) / (2 * eps)
+            assert np.abs(fd - A[:, :, :, k, L]).max() <= 1e-7 * np.abs(A).max()
+    assert np.allclose(A, np.transpose(A, (0, 3, 4, 1, 2)), rtol=0, atol=1e-9 * np.abs(A).max())
+    P0, A0 = G.mooney_rivlin_PA(np.eye(3)[None], c10, kappa)
+    mu, lam = 2 * c10, kappa - 2 * (2 * c10) / 3
+    I3 = np.eye(3)
+    lin = (mu * (np.einsum("ik,JL->iJkL", I3, I3) + np.einsum("iL,Jk->iJkL", I3, I3)) +
+           lam * np.einsum("iJ,kL->iJkL", I3, I3))
+    assert np.abs(P0).max() < 1e-9 and np.allclose(A0[0], lin, rtol=0, atol=1e-9 * kappa)
