@@ -7,6 +7,10 @@ namespace kkt {
 
 constexpr int NB = 64;    // panel width (pivot columns per panel step)
 constexpr int TILE = 64;  // assembly / Schur-update tile edge
+// fronts with k <= FWD_SMALL_K pivot columns and m <= FWD_SMALL_M rows run their
+// whole forward step (gather + rows) in one workgroup
+constexpr int FWD_SMALL_K = 32;
+constexpr int FWD_SMALL_M = 256;
 
 // Device copy of the supernodal structure (all arrays device pointers).
 struct DevSym {
@@ -62,8 +66,8 @@ template <typename T>
 void launch_gform(const int4* items, int nitems, const DevSym& d, T* F, hipStream_t st);
 // level-scheduled solves (vectors in the front precision T)
 template <typename T>
-void launch_fwd_gather(const int2* items, int nitems, const DevSym& d, const T* bp, T* fp, T* u,
-                       hipStream_t st);
+void launch_fwd_gather(const int4* items, int nitems, const DevSym& d, const T* F, const T* bp,
+                       T* fp, T* y, T* u, hipStream_t st);
 template <typename T>
 void launch_fwd_rows(const int4* items, int nitems, const DevSym& d, const T* F, const T* fp, T* y,
                      T* u, hipStream_t st);

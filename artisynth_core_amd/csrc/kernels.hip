@@ -733,23 +733,54 @@ __global__ __launch_bounds__(256) void k_gform(const int4* __restrict__ items, D
   }
 }
 
-// forward gather, one thread per front position p (children summed in
-// ascending order): f_p = b_p + sum of the children's update entries at p;
-// fp = f in pivot order (p < k), u_s = the sums on the contribution rows (p >= k);
-// item = (s, block of 256 positions)
+// forward, first launch of a level.  item = (s, block, kind):
+//   kind 1: gather block of a large front, one thread per position p (children
+//           summed in ascending order): f_p = b_p + the children's update entries
+//           at p; fp = f in pivot order (p < k), u_s = the sums on the
+//           contribution rows (p >= k)
+//   kind 0: a whole small front (k <= FWD_SMALL_K, m <= FWD_SMALL_M): the same gather into
+//           LDS, then one thread per row: y_i = f_i + G(i,:) f (i < k),
+//           u_{i-k} = f_i - G(i,:) f (i >= k), G(i,:) read from column i of the
+//           upper copy
+static_assert(FWD_SMALL_M == 256, "one thread per row of a small front");
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_gather(const int2* __restrict__ items, DevSym d,
+__global__ __launch_bounds__(256) void k_fwd_gather(const int4* __restrict__ items, DevSym d,
+                                                    const T* __restrict__ F,
                                                     const T* __restrict__ bp, T* __restrict__ fp,
-                                                    T* __restrict__ u) {
-  const int2 it = items[blockIdx.x];
+                                                    T* __restrict__ y, T* __restrict__ u) {
+  __shared__ T g[256];
+  __shared__ T fs[FWD_SMALL_K];
+  const int4 it = items[blockIdx.x];
   const int s = it.x, p = it.y * 256 + threadIdx.x;
   const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s];
-  if (p >= m) return;
-  const int* gp = d.gptr + d.sn_goff[s] + p;
-  T acc = (p < k) ? bp[first + p] : (T)0;
-  for (int e = gp[0]; e < gp[1]; e++) acc += u[d.gsrc[e]];
-  if (p < k) fp[first + d.ilperm[first + p]] = acc;
-  else u[d.sn_uoff[s] + p - k] = acc;
+  T acc = (T)0;
+  if (p < m) {
+    const int* gp = d.gptr + d.sn_goff[s] + p;
+    acc = (p < k) ? bp[first + p] : (T)0;
+    for (int e = gp[0]; e < gp[1]; e++) acc += u[d.gsrc[e]];
+  }
+  if (it.z == 1) {
+    if (p < k) fp[first + d.ilperm[first + p]] = acc;
+    else if (p < m) u[d.sn_uoff[s] + p - k] = acc;
+    return;
+  }
+  g[threadIdx.x] = acc;
+  __syncthreads();
+  if (p < k) fs[d.ilperm[first + p]] = acc;
+  __syncthreads();
+  if (p < m) {
+    const T* col = F + d.sn_foff[s] + (size_t)p * m;
+    const int len = min(p, k);
+    T a0 = (T)0, a1 = (T)0;
+    int l = 0;
+    for (; l + 1 < len; l += 2) {
+      a0 += col[l] * fs[l];
+      a1 += col[l + 1] * fs[l + 1];
+    }
+    if (l < len) a0 += col[l] * fs[l];
+    if (p < k) y[first + p] = fs[p] + (a0 + a1);
+    else u[d.sn_uoff[s] + p - k] = g[threadIdx.x] - (a0 + a1);
+  }
 }
 
 // sum over aligned segments of 2^lg lanes (every lane of a segment gets the sum)
@@ -1026,9 +1057,10 @@ void launch_gform(const int4* items, int nitems, const DevSym& d, T* F, hipStrea
   if (nitems > 0) hipLaunchKernelGGL(k_gform<T>, dim3(nitems), dim3(256), 0, st, items, d, F);
 }
 template <typename T>
-void launch_fwd_gather(const int2* items, int nitems, const DevSym& d, const T* bp, T* fp, T* u,
-                       hipStream_t st) {
-  if (nitems > 0) hipLaunchKernelGGL(k_fwd_gather<T>, dim3(nitems), dim3(256), 0, st, items, d, bp, fp, u);
+void launch_fwd_gather(const int4* items, int nitems, const DevSym& d, const T* F, const T* bp,
+                       T* fp, T* y, T* u, hipStream_t st) {
+  if (nitems > 0)
+    hipLaunchKernelGGL(k_fwd_gather<T>, dim3(nitems), dim3(256), 0, st, items, d, F, bp, fp, y, u);
 }
 template <typename T>
 void launch_fwd_rows(const int4* items, int nitems, const DevSym& d, const T* F, const T* fp, T* y,
@@ -1115,8 +1147,8 @@ void launch_zero_counters(FactorCounters* c, hipStream_t st) {
   template void launch_panel_xrow<T>(const int2*, int, int, const DevSym&, T*, hipStream_t);     \
   template void launch_zt<T>(const int4*, int, const DevSym&, T*, hipStream_t);                \
   template void launch_gform<T>(const int4*, int, const DevSym&, T*, hipStream_t);             \
-  template void launch_fwd_gather<T>(const int2*, int, const DevSym&, const T*, T*, T*,           \
-                                     hipStream_t);                                               \
+  template void launch_fwd_gather<T>(const int4*, int, const DevSym&, const T*, const T*, T*, T*, \
+                                     T*, hipStream_t);                                           \
   template void launch_fwd_rows<T>(const int4*, int, const DevSym&, const T*, const T*, T*, T*,  \
                                    hipStream_t);                                                 \
   template void launch_dsolve<T>(int, const T*, const T*, const signed char*, const T*, T*,      \

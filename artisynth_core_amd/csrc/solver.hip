@@ -98,7 +98,8 @@ struct kkt_solver {
   // plan
   int4 *d_asm = nullptr, *d_upd = nullptr;
   int *d_diag = nullptr, *d_aidx = nullptr;
-  int2 *d_trsm = nullptr, *d_xrow = nullptr, *d_fg = nullptr;
+  int2 *d_trsm = nullptr, *d_xrow = nullptr;
+  int4* d_fg = nullptr;
   int *d_ilperm = nullptr, *d_gptr = nullptr, *d_gsrc = nullptr;
   long long* d_goff = nullptr;
   int4 *d_zt = nullptr, *d_gf = nullptr, *d_fr = nullptr, *d_bc = nullptr;
@@ -120,16 +121,21 @@ struct kkt_solver {
   double* d_vec = nullptr;  // 14 vectors of n (0-8 BiCGStab, 10-11 host copies, 12-13 graph I/O)
   double* d_partial = nullptr;
   double* d_scal = nullptr;  // 16 scalars
+  double* h_pin = nullptr;   // pinned host copy of the 16 scalars
   FactorCounters* d_ctr = nullptr;
   std::vector<double> multiB;
   // captured triangular-solve graphs (per factor precision), replayed per solve
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   hipGraphExec_t fexec[2] = {nullptr, nullptr};  // captured factorizations
+  // captured BiCGStab half-iterations per precision: 0 first head, 1 head, 2 tail
+  hipGraphExec_t bexec[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   bool graphs_on = true;
   void drop_graphs() {
     for (int q = 0; q < 2; q++) {
       if (gexec[q]) { (void)hipGraphExecDestroy(gexec[q]); gexec[q] = nullptr; }
       if (fexec[q]) { (void)hipGraphExecDestroy(fexec[q]); fexec[q] = nullptr; }
+      for (int w = 0; w < 3; w++)
+        if (bexec[q][w]) { (void)hipGraphExecDestroy(bexec[q][w]); bexec[q][w] = nullptr; }
     }
   }
   // Captures `fn` (launches on st only) into exec, or returns false.
@@ -203,6 +209,8 @@ struct kkt_solver {
       if (d_partial) (void)hipFree(d_partial);
       if (d_scal) (void)hipFree(d_scal);
       if (d_ctr) (void)hipFree(d_ctr);
+      if (h_pin) (void)hipHostFree(h_pin);
+      h_pin = nullptr;
       for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
       ev_pool.clear();
       if (st) (void)hipStreamDestroy(st);
@@ -224,7 +232,8 @@ struct kkt_solver {
       return fail(KKT_ERR_INIT, "hipStreamCreate failed");
     if (hipMalloc(&d_partial, 1024 * sizeof(double)) != hipSuccess ||
         hipMalloc(&d_scal, 16 * sizeof(double)) != hipSuccess ||
-        hipMalloc(&d_ctr, sizeof(FactorCounters)) != hipSuccess)
+        hipMalloc(&d_ctr, sizeof(FactorCounters)) != hipSuccess ||
+        hipHostMalloc(&h_pin, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess)
       return fail(KKT_ERR_ALLOC, "hipMalloc failed (scalars)");
     dev_ok = true;
     return KKT_OK;
@@ -481,8 +490,7 @@ struct kkt_solver {
     std::vector<int> diagv, aidx;
     std::vector<long long> apos;
     std::vector<int2> trsmv, xrowv;
-    std::vector<int2> fgv;
-    std::vector<int4> frv, bcv;
+    std::vector<int4> fgv, frv, bcv;
     asm_l.assign(nl, Launch());
     scat_l.assign(nl, Launch());
     fg_l.assign(nl, Launch());
@@ -512,9 +520,14 @@ struct kkt_solver {
         return lg;
       };
       for (int s : lv[L]) {
-        for (int gb = 0; gb < (mm[s] + 255) / 256; gb++) fgv.push_back(make_int2(s, gb));
-        const int lf = seg_lg(kk[s]), rf = 4 * (64 >> lf);
-        for (int rb = 0; rb < (mm[s] + rf - 1) / rf; rb++) frv.push_back(make_int4(s, rb, lf, 0));
+        if (kk[s] <= FWD_SMALL_K && mm[s] <= FWD_SMALL_M) {
+          // small front: gather and rows in one workgroup (the first forward launch)
+          fgv.push_back(make_int4(s, 0, 0, 0));
+        } else {
+          for (int gb = 0; gb < (mm[s] + 255) / 256; gb++) fgv.push_back(make_int4(s, gb, 1, 0));
+          const int lf = seg_lg(kk[s]), rf = 4 * (64 >> lf);
+          for (int rb = 0; rb < (mm[s] + rf - 1) / rf; rb++) frv.push_back(make_int4(s, rb, lf, 0));
+        }
         const int lb = seg_lg(mm[s]), rbw = 4 * (64 >> lb);
         for (int cb = 0; cb < (kk[s] + rbw - 1) / rbw; cb++) bcv.push_back(make_int4(s, cb, lb, 0));
       }
@@ -788,7 +801,7 @@ struct kkt_solver {
     const int nl = S.nlevels;
     tbeg(TC_SLEVEL);
     for (int L = 0; L < nl; L++) {
-      launch_fwd_gather<T>(d_fg + fg_l[L].off, fg_l[L].cnt, ds, bp, fp, u, st);
+      launch_fwd_gather<T>(d_fg + fg_l[L].off, fg_l[L].cnt, ds, F, bp, fp, y, u, st);
       launch_fwd_rows<T>(d_fr + fr_l[L].off, fr_l[L].cnt, ds, F, fp, y, u, st);
     }
     launch_dsolve<T>(n, E0, E1, d_piv, y, z, st);
@@ -946,40 +959,58 @@ struct kkt_solver {
       launch_fill(n, 0.0, v, st);
       launch_bicg_scalars(sc, 1, st);
       const double eps = 1e-30;
-      for (it = 1; it <= maxIter; it++) {
-        dot_to(rh, r, 0);                      // rho
+      // Each half-iteration ends with its scalars copied to pinned host memory;
+      // both halves are captured once into HIP graphs (fixed buffers), so an
+      // iteration costs two graph launches and two host synchronizations.
+      auto head = [&](bool first) {
+        if (!first) launch_bicg_scalars(sc, 0, st);  // alpha, omega, rho_prev of the last iteration
+        dot_to(rh, r, 0);                             // rho
         tbeg(TC_VEC);
-        launch_bicg_p(n, sc, r, v, p, st);     // p = r + beta (p - omega v)
+        launch_bicg_p(n, sc, r, v, p, st);            // p = r + beta (p - omega v)
         tend();
-        tri_solve(p, y);                       // y = M^-1 p
-        spmv(y, nullptr, v, 0);                // v = A y
-        dot_to(rh, v, 4);                      // (r_hat, v)
+        tri_solve_eager(p, y);                        // y = M^-1 p
+        spmv(y, nullptr, v, 0);                       // v = A y
+        dot_to(rh, v, 4);                             // (r_hat, v)
         tbeg(TC_VEC);
-        launch_bicg_s(n, sc, r, v, s, st);     // s = r - alpha v
+        launch_bicg_s(n, sc, r, v, s, st);            // s = r - alpha v
         tend();
         dot_to(s, s, 5);
-        double h5[6];
-        if ((rc = read_scalars(0, 6, h5))) return rc;
-        if (std::fabs(h5[0]) < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: rho ~ 0");
-        if (std::fabs(h5[4]) < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: (r_hat,v) ~ 0");
-        if (std::sqrt(h5[5]) / bn <= tolRel) {
-          launch_bicg_xa(n, sc, y, x, st);     // x += alpha y
-          break;
-        }
-        tri_solve(s, z);                       // z = M^-1 s
-        spmv(z, nullptr, t, 0);                // t = A z
+        (void)hipMemcpyAsync(h_pin, sc, 6 * sizeof(double), hipMemcpyDeviceToHost, st);
+      };
+      auto tail = [&]() {
+        tri_solve_eager(s, z);                        // z = M^-1 s
+        spmv(z, nullptr, t, 0);                       // t = A z
         dot_to(t, s, 6);
         dot_to(t, t, 7);
         tbeg(TC_VEC);
         launch_bicg_xr(n, sc, y, z, s, t, x, r, st);  // x += alpha y + omega z; r = s - omega t
         tend();
         dot_to(r, r, 8);
-        double h3[3];
-        if ((rc = read_scalars(6, 3, h3))) return rc;
+        (void)hipMemcpyAsync(h_pin + 6, sc + 6, 3 * sizeof(double), hipMemcpyDeviceToHost, st);
+      };
+      const int q = (fprec == KKT_PREC_FP64) ? 0 : 1;
+      const int cls = (1 << TC_SLEVEL) | (1 << TC_SGEMV) | (1 << TC_VEC) | (1 << TC_SPMV);
+      auto run = [&](int w, auto fn) -> int {
+        const bool use = graphs_on && !(tm_on && (tm_mask & cls));
+        if (use && (bexec[q][w] || capture(bexec[q][w], fn))) (void)hipGraphLaunch(bexec[q][w], st);
+        else fn();
+        if (hipStreamSynchronize(st) != hipSuccess) return fail(KKT_ERR_REDUCTION, "reduction failed");
+        return KKT_OK;
+      };
+      for (it = 1; it <= maxIter; it++) {
+        if ((rc = (it == 1) ? run(0, [&]() { head(true); }) : run(1, [&]() { head(false); }))) return rc;
+        const double* h5 = h_pin;
+        if (std::fabs(h5[0]) < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: rho ~ 0");
+        if (std::fabs(h5[4]) < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: (r_hat,v) ~ 0");
+        if (std::sqrt(h5[5]) / bn <= tolRel) {
+          launch_bicg_xa(n, sc, y, x, st);            // x += alpha y
+          break;
+        }
+        if ((rc = run(2, tail))) return rc;
+        const double* h3 = h_pin + 6;
         if (h3[1] < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: (t,t) ~ 0");
         if (std::sqrt(h3[2]) / bn <= tolRel) break;
         if (std::fabs(h3[0] / h3[1]) < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: omega ~ 0");
-        launch_bicg_scalars(sc, 0, st);        // alpha, omega, rho_prev for the next iteration
       }
       if (it > maxIter) return fail(KKT_ERR_ITER_NO_CONVERGE, "BiCGStab did not converge within iteration limit");
     }

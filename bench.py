@@ -27,8 +27,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAKS = {"hbm_GBs": 8000.0, "fp64_TFLOPs": 78.6, "fp32_TFLOPs": 157.3}
-TIMING_CLASSES = ["assemble", "panel_diag", "panel_trsm", "panel_update", "solve_level",
-                  "solve_gemv", "spmv", "vector"]
+TIMING_CLASSES = ["assemble", "panel_diag", "panel_trsm", "panel_update", "solve_forward",
+                  "solve_backward", "spmv", "vector"]
 
 
 def log(msg):
@@ -407,10 +407,11 @@ def serve_run(ninst, nreq, rate, nworkers, budget_gb):
 def precision_run(dims, nsteps, warmup, with_oracle):
     """seeded Delaunay tet box (0.1 x 0.02 x 0.02 m, ~210k DOF at the
     default lattice), Mooney-Rivlin c10 = 1e3, c01 = 0, kappa = 1e5, attached end.
-    Every step is a direct solve (factor + solve + refinement to 1e-10): FP32 fronts
-    on MFMA with FP64 refinement against FP64 fronts, inputs resident in HBM.  Per
-    solve: relative residual (host FP64 SpMV), forward error of the FP32 path against
-    the FP64 path, refinement steps, FP64 fallbacks."""
+    Every step is a direct solve: FP32 fronts on MFMA with FP64 refinement to the
+    metric's 1e-8 relative residual, against FP64 fronts with one refinement step;
+    inputs resident in HBM.  Per solve: relative residual (host FP64 SpMV), forward
+    error of the FP32 path against the FP64 path, refinement steps, FP64 fallbacks.
+    One extra factor per precision is timed per kernel class."""
     import torch
     import scipy.sparse as sp
     from artisynth_core_amd import KKT_PREC_FP32_REFINE, KKT_PREC_FP64
@@ -433,7 +434,7 @@ def precision_run(dims, nsteps, warmup, with_oracle):
     for prec, name in ((KKT_PREC_FP32_REFINE, "fp32_mfma_refine"), (KKT_PREC_FP64, "fp64")):
         h = L.kkt_create()
         L.kkt_set_precision(h, prec)
-        L.kkt_set_refine_tol(h, 1e-10, 10)
+        L.kkt_set_refine_tol(h, 1e-8, 10)
         if prec == KKT_PREC_FP64:
             L.kkt_set_ir_steps(h, 1)
         ta = time.time()
@@ -459,6 +460,15 @@ def precision_run(dims, nsteps, warmup, with_oracle):
             inf = KktInfo()
             L.kkt_get_info(h, ctypes.byref(inf))
             info.append(inf.as_dict())
+        # one more factor with per-class kernel timing (eager launches)
+        L.kkt_timing_reset(h, 0x0F)
+        assert L.kkt_factor_device(h, vals_d[0].data_ptr()) == 0
+        ms = (ctypes.c_double * 8)()
+        cnt = (ctypes.c_longlong * 8)()
+        uflops = ctypes.c_double()
+        L.kkt_timing_get(h, ms, cnt, ctypes.byref(uflops))
+        L.kkt_timing_reset(h, 0)
+        kern = {TIMING_CLASSES[c]: ms[c] for c in range(4)}
         res = []
         for k in range(len(V)):
             U = sp.csr_matrix((V[k], ci, ro), shape=(n, n))
@@ -471,7 +481,10 @@ def precision_run(dims, nsteps, warmup, with_oracle):
                      "rel_residual": res, "refine_steps": [i["refine_steps"] for i in info],
                      "fp64_fallbacks": info[-1]["fp64_fallbacks"], "analyze_s": t_an,
                      "npos": info[0]["npos"], "nneg": info[0]["nneg"],
-                     "nperturbed": info[0]["nperturbed"], "n2x2": info[0]["n2x2"], "x": xs}
+                     "nperturbed": info[0]["nperturbed"], "n2x2": info[0]["n2x2"],
+                     "factor_kernel_ms": kern,
+                     "update_tflops": uflops.value / (kern["panel_update"] * 1e-3) / 1e12
+                     if kern["panel_update"] > 0 else None, "x": xs}
         log("%s: %.2f direct solves/s, factor %.1f ms (%.1f TFLOP/s), solve+refine %.1f ms, "
             "max rel resid %.2e, refine steps %s" % (
                 name, out[name]["solves_per_s"], 1e3 * np.median(tf), out[name]["factor_tflops"],

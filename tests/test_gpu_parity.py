@@ -281,9 +281,9 @@ def test_serving_alone_equals_mix(gpu):
 
 @pytest.mark.parametrize("prec", [KKT_PREC_FP64, KKT_PREC_FP32_REFINE])
 def test_graph_replays_match_eager_launches(gpu, prec):
-    # the factor and solve launch sequences are captured once into HIP graphs and
-    # replayed; every replay (with eager work of another handle in between) must
-    # give the bitwise result of the eagerly launched kernels
+    # the factor, solve and BiCGStab launch sequences are captured once into HIP
+    # graphs and replayed; every replay (with eager work of another handle in
+    # between) must give the bitwise result of the eagerly launched kernels
     _say("graph replays vs eager prec=%d" % prec)
     beam = G.HexBeam(24, 8, 8, 1.0, 0.2, material=("neohookean", 3.4e4, 1.7e6), h=0.005)
     u = np.random.default_rng(9).uniform(-1e-4, 1e-4, (beam.nn, 3))
@@ -313,6 +313,16 @@ def test_graph_replays_match_eager_launches(gpu, prec):
             for xe, xg in zip(*xs):
                 assert np.array_equal(xe, xg)
             assert rel_residual(beam.rowOffs, beam.colIdxs, v, xs[1][0], b) <= 1e-8
+            # stale-factor BiCGStab: captured half-iterations vs eager launches
+            v2 = v * (1.0 + 0.002 * (rep + 1))
+            its, xi = [], []
+            for s in solvers:
+                x = np.zeros(beam.n)
+                its.append(s.iterativeSolve(v2, x, b, 10))
+                xi.append(x)
+            assert its[0] == its[1] > 0
+            assert np.array_equal(xi[0], xi[1])
+            assert rel_residual(beam.rowOffs, beam.colIdxs, v2, xi[1], b) <= 1e-9
     finally:
         for s in solvers:
             s.dispose()
