@@ -403,6 +403,8 @@ class KKTSolver:
         self.sizeM = sizeM
         self.numG = 0 if GT is None else GT.shape[1]
         self._ci = ci
+        self._ro = ro
+        self._va = va
         kktType = Matrix.SYMMETRIC if (typeM & Matrix.SYMMETRIC) else Matrix.INDEFINITE
         if not (kktType & Matrix.SYMMETRIC):
             raise ValueError("MI355X backend handles symmetric KKT systems only")
@@ -421,8 +423,22 @@ class KKTSolver:
 
     def factor(self, M, GT, Rg):
         """factorMG (KKTSolver.java:1958-1994)."""
-        self.solver.factor(self._vals(M, GT, Rg))
+        self._va = self._vals(M, GT, Rg)
+        self.solver.factor(self._va)
         self._state = "FACTORED"
+
+    def printLinearProblem(self, path, bf, bg, fmtStr="%.17g", omitLowerRightDiagonal=False):
+        """KKTSolver.printLinearProblem (KKTSolver.java:1746-1788): the current KKT
+        values (upper CSR, one-based on disk) and the right-hand side [bf; bg]."""
+        from .kktio import write_linear_problem
+        write_linear_problem(path, self._ro, self._ci, self._va, bf, bg, symmetric=True,
+                             fmt=fmtStr, omit_lower_right_diagonal=omitLowerRightDiagonal,
+                             num_g=self.numG)
+
+    def writeLinearProblem(self, fileName, bf, bg):
+        """KKTSolver.writeLinearProblem (:1790-1800), at full precision ("%.17g";
+        the reference's "%g" keeps 6 digits)."""
+        self.printLinearProblem(fileName, bf, bg)
 
     def solve(self, vel, lam, bm, bg):
         """solveMG (KKTSolver.java:1912-1934): solves with [bm; bg] and negates lam."""
