@@ -51,6 +51,13 @@ SIGNATURES = [
     ("kkt_analyze_values", _I, [_P, _DP]),
     ("kkt_factor", _I, [_P, _DP]),
     ("kkt_factor_device", _I, [_P, _P]),
+    ("kkt_clear_device_values", _I, [_P]),
+    ("kkt_get_device_values", _I, [_P, _DP]),
+    ("kkt_add_device_values", _I, [_P, _IP, _DP, _I, _D]),
+    ("kkt_add_scaled_diagonal3_device_values", _I, [_P, _IP, _DP, _I, _D]),
+    ("kkt_add_scaled_block3_device_values", _I, [_P, _IP, _DP, _DP, _I, _D]),
+    ("kkt_add_material_stiffness3_device_values", _I, [_P, _IP, _DP, _DP, _DP, _DP, _DP, _I, _D]),
+    ("kkt_factor_device_values", _I, [_P]),
     ("kkt_solve", _I, [_P, _DP, _DP]),
     ("kkt_solve_device", _I, [_P, _P, _P]),
     ("kkt_solve_multi", _I, [_P, _I, _DP, _DP]),

@@ -102,6 +102,15 @@ void launch_axpbypcz(int n, double a, const double* x, double b, const double* y
 
 
 // BiCGStab updates with device-resident scalars (see kernels.hip for the slots)
+// device-side value assembly: contributions into scratch c, then the ordered
+// per-slot scatter vals[uslot[u]] += scale * sum_{e in segment u} c[order[e]]
+void launch_contrib_block3(int nblocks, const double* bv, const double* bs, double* c, hipStream_t st);
+void launch_contrib_diag3(int nblocks, const double* m, double* c, hipStream_t st);
+void launch_contrib_matstiff3(int nblocks, const double* gi, const double* gj, const double* D,
+                              const double* sg, const double* dv, double* c, hipStream_t st);
+void launch_seg_scatter(int nuniq, const int* useg, const int* uslot, const int* order,
+                        const double* c, double scale, double* vals, hipStream_t st);
+
 void launch_bicg_p(int n, const double* sc, const double* r, const double* v, double* p, hipStream_t st);
 void launch_bicg_s(int n, const double* sc, const double* r, const double* v, double* s_, hipStream_t st);
 void launch_bicg_xr(int n, const double* sc, const double* y, const double* z, const double* s_,

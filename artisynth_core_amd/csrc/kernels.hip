@@ -872,6 +872,75 @@ __global__ __launch_bounds__(256) void k_bwd_cols(const int4* __restrict__ items
 }
 
 // ------------------------------------------------------------------------
+// Device-side value assembly (CuDssSolver.*DeviceValues, cudssBridgeKernels.cu:57-380).
+// Each call computes its contributions into a scratch array, then one thread
+// per distinct CSR slot adds them in a fixed order (the plan lists the
+// contributions of every slot, ascending by contribution index): no atomics,
+// bitwise reproducible.
+// ------------------------------------------------------------------------
+__global__ void k_contrib_block3(int nblocks, const double* __restrict__ blockVals,
+                                 const double* __restrict__ blockScales, double* __restrict__ c) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e < 9 * nblocks) c[e] = blockScales[e / 9] * blockVals[e];
+}
+__global__ void k_contrib_diag3(int nblocks, const double* __restrict__ masses, double* __restrict__ c) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e < 3 * nblocks) c[e] = masses[e / 3];
+}
+// K = dv (B_i^T D B_j + (g_i^T sigma g_j) I), with B the 6x3 Voigt strain-
+// displacement matrix of gradient g (strain order xx, yy, zz, xy, yz, xz,
+// engineering shears), D 6x6 row-major, sigma = [s00, s11, s22, s01, s12, s02]
+// (FemUtilities.addMaterialStiffness)
+__global__ void k_contrib_matstiff3(int nblocks, const double* __restrict__ gis,
+                                    const double* __restrict__ gjs, const double* __restrict__ Ds,
+                                    const double* __restrict__ sigmas,
+                                    const double* __restrict__ dvs, double* __restrict__ c) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= nblocks) return;
+  const double* D = Ds + 36 * (size_t)b;
+  const double* sg = sigmas + 6 * (size_t)b;
+  const double gi[3] = {gis[3 * b], gis[3 * b + 1], gis[3 * b + 2]};
+  const double gj[3] = {gjs[3 * b], gjs[3 * b + 1], gjs[3 * b + 2]};
+  const double dv = dvs[b];
+  // Voigt rows of B(g) for displacement component k: bv(g, k)[r]
+  auto bv = [](const double* g, int k, double* out) {
+    for (int r = 0; r < 6; r++) out[r] = 0.0;
+    out[k] = g[k];
+    if (k == 0) { out[3] = g[1]; out[5] = g[2]; }
+    else if (k == 1) { out[3] = g[0]; out[4] = g[2]; }
+    else { out[4] = g[1]; out[5] = g[0]; }
+  };
+  double bi[3][6], bj[3][6];
+  for (int k = 0; k < 3; k++) {
+    bv(gi, k, bi[k]);
+    bv(gj, k, bj[k]);
+  }
+  const double S[3][3] = {{sg[0], sg[3], sg[5]}, {sg[3], sg[1], sg[4]}, {sg[5], sg[4], sg[2]}};
+  double geo = 0.0;
+  for (int a = 0; a < 3; a++)
+    for (int q = 0; q < 3; q++) geo += gi[a] * S[a][q] * gj[q];
+  for (int r = 0; r < 3; r++)
+    for (int q = 0; q < 3; q++) {
+      double acc = 0.0;
+      for (int u = 0; u < 6; u++) {
+        double db = 0.0;
+        for (int v = 0; v < 6; v++) db += D[6 * u + v] * bj[q][v];
+        acc += bi[r][u] * db;
+      }
+      c[9 * (size_t)b + 3 * r + q] = dv * (acc + (r == q ? geo : 0.0));
+    }
+}
+__global__ void k_seg_scatter(int nuniq, const int* __restrict__ useg, const int* __restrict__ uslot,
+                              const int* __restrict__ order, const double* __restrict__ c,
+                              double scale, double* __restrict__ vals) {
+  const int u = blockIdx.x * 256 + threadIdx.x;
+  if (u >= nuniq) return;
+  double acc = 0.0;
+  for (int e = useg[u]; e < useg[u + 1]; e++) acc += c[order[e]];
+  vals[uslot[u]] += scale * acc;
+}
+
+// ------------------------------------------------------------------------
 // vectors and SpMV
 // ------------------------------------------------------------------------
 template <typename TI, typename TO>
@@ -1112,6 +1181,26 @@ void launch_axpbypcz(int n, double a, const double* x, double b, const double* y
   hipLaunchKernelGGL(k_axpbypcz, dim3(nblk(n, 256)), dim3(256), 0, st, n, a, x, b, y, c, z);
 }
 
+void launch_contrib_block3(int nblocks, const double* bv, const double* bs, double* c, hipStream_t st) {
+  if (nblocks > 0)
+    hipLaunchKernelGGL(k_contrib_block3, dim3(nblk(9LL * nblocks, 256)), dim3(256), 0, st, nblocks, bv, bs, c);
+}
+void launch_contrib_diag3(int nblocks, const double* m, double* c, hipStream_t st) {
+  if (nblocks > 0)
+    hipLaunchKernelGGL(k_contrib_diag3, dim3(nblk(3LL * nblocks, 256)), dim3(256), 0, st, nblocks, m, c);
+}
+void launch_contrib_matstiff3(int nblocks, const double* gi, const double* gj, const double* D,
+                              const double* sg, const double* dv, double* c, hipStream_t st) {
+  if (nblocks > 0)
+    hipLaunchKernelGGL(k_contrib_matstiff3, dim3(nblk(nblocks, 256)), dim3(256), 0, st, nblocks, gi, gj,
+                       D, sg, dv, c);
+}
+void launch_seg_scatter(int nuniq, const int* useg, const int* uslot, const int* order,
+                        const double* c, double scale, double* vals, hipStream_t st) {
+  if (nuniq > 0)
+    hipLaunchKernelGGL(k_seg_scatter, dim3(nblk(nuniq, 256)), dim3(256), 0, st, nuniq, useg, uslot, order,
+                       c, scale, vals);
+}
 void launch_bicg_p(int n, const double* sc, const double* r, const double* v, double* p, hipStream_t st) {
   hipLaunchKernelGGL(k_bicg_p, dim3(nblk(n, 256)), dim3(256), 0, st, n, sc, r, v, p);
 }

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <initializer_list>
+#include <utility>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -122,6 +124,19 @@ struct kkt_solver {
   double* d_partial = nullptr;
   double* d_scal = nullptr;  // 16 scalars
   double* h_pin = nullptr;   // pinned host copy of the 16 scalars
+  // device-side value assembly: scatter plans keyed by the caller's slot list
+  struct AddPlan {
+    std::vector<int> slots;  // the slot list the plan was built for
+    int nuniq = 0;           // distinct valid slots
+    int *d_order = nullptr, *d_useg = nullptr, *d_uslot = nullptr;
+    unsigned long long used = 0;
+  };
+  std::vector<AddPlan> add_plans;
+  unsigned long long add_clock = 0;
+  double* d_contrib = nullptr;  // per-call contributions
+  size_t contrib_cap = 0;       // elements
+  char* d_stage = nullptr;      // per-call uploaded inputs
+  size_t stage_cap = 0;         // bytes
   FactorCounters* d_ctr = nullptr;
   std::vector<double> multiB;
   // captured triangular-solve graphs (per factor precision), replayed per solve
@@ -194,9 +209,27 @@ struct kkt_solver {
     d_F64 = nullptr; d_F32 = nullptr; d_W64 = nullptr; d_W32 = nullptr; d_svec = nullptr;
     svec_bytes = 0;
   }
+  void free_plan(AddPlan& p) {
+    DevOpLock lk(g_capture_mu);
+    if (p.d_order) (void)hipFree(p.d_order);
+    if (p.d_useg) (void)hipFree(p.d_useg);
+    if (p.d_uslot) (void)hipFree(p.d_uslot);
+    p.d_order = p.d_useg = p.d_uslot = nullptr;
+  }
+  void clear_add_state() {
+    for (AddPlan& p : add_plans) free_plan(p);
+    add_plans.clear();
+    DevOpLock lk(g_capture_mu);
+    if (d_contrib) (void)hipFree(d_contrib);
+    if (d_stage) (void)hipFree(d_stage);
+    d_contrib = nullptr;
+    d_stage = nullptr;
+    contrib_cap = stage_cap = 0;
+  }
   void release_all() {
     if (dev_ok) {
       if (st) (void)hipStreamSynchronize(st);
+      clear_add_state();
       free_list(allocs);
       release_numeric();
       DevOpLock lk(g_capture_mu);
@@ -372,6 +405,7 @@ struct kkt_solver {
   // ------------------------------------------------------------------
   int build_plan() {
     (void)hipStreamSynchronize(st);
+    clear_add_state();
     free_list(allocs);
     release_numeric();
     {
@@ -920,6 +954,110 @@ struct kkt_solver {
     return KKT_OK;
   }
 
+  // ---- device-side value assembly (CuDssSolver.java:421-788) ----
+  // The plan of a slot list: its valid contributions (slot >= 0) stably sorted by
+  // slot, grouped per distinct slot.  Plans are cached by the exact slot list.
+  int plan_for(const int* slots, long long cnt, AddPlan*& out) {
+    for (AddPlan& p : add_plans)
+      if ((long long)p.slots.size() == cnt && std::equal(slots, slots + cnt, p.slots.begin())) {
+        p.used = ++add_clock;
+        out = &p;
+        return KKT_OK;
+      }
+    std::vector<int> idx;
+    idx.reserve((size_t)cnt);
+    for (long long e = 0; e < cnt; e++) {
+      const int sl = slots[e];
+      if (sl < 0) continue;
+      if (sl >= nnz) return fail(KKT_ERR_ARGUMENT, "slot beyond the analyzed pattern's nnz");
+      idx.push_back((int)e);
+    }
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return slots[a] < slots[b]; });
+    std::vector<int> useg, uslot;
+    for (size_t q = 0; q < idx.size(); q++)
+      if (q == 0 || slots[idx[q]] != slots[idx[q - 1]]) {
+        useg.push_back((int)q);
+        uslot.push_back(slots[idx[q]]);
+      }
+    useg.push_back((int)idx.size());
+    if (add_plans.size() >= 16) {
+      size_t lru = 0;
+      for (size_t q = 1; q < add_plans.size(); q++)
+        if (add_plans[q].used < add_plans[lru].used) lru = q;
+      free_plan(add_plans[lru]);
+      add_plans.erase(add_plans.begin() + (long)lru);
+    }
+    add_plans.emplace_back();
+    AddPlan& p = add_plans.back();
+    p.slots.assign(slots, slots + cnt);
+    p.nuniq = (int)uslot.size();
+    p.used = ++add_clock;
+    bool okp;
+    {
+      DevOpLock lk(g_capture_mu);
+      okp = hipMalloc(&p.d_order, std::max<size_t>(idx.size(), 1) * sizeof(int)) == hipSuccess &&
+            hipMalloc(&p.d_useg, useg.size() * sizeof(int)) == hipSuccess &&
+            hipMalloc(&p.d_uslot, std::max<size_t>(uslot.size(), 1) * sizeof(int)) == hipSuccess &&
+            (idx.empty() || hipMemcpy(p.d_order, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess) &&
+            hipMemcpy(p.d_useg, useg.data(), useg.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+            (uslot.empty() || hipMemcpy(p.d_uslot, uslot.data(), uslot.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess);
+    }
+    if (!okp) {
+      free_plan(p);
+      add_plans.pop_back();
+      return fail(KKT_ERR_ALLOC, "device allocation or copy failed (assembly plan)");
+    }
+    out = &p;
+    return KKT_OK;
+  }
+  // uploads the listed host arrays into one staging buffer; returns device pointers
+  int stage_inputs(std::initializer_list<std::pair<const void*, size_t>> in, std::vector<const double*>& dp) {
+    size_t total = 0;
+    for (auto& a : in) total += (a.second + 255) & ~(size_t)255;
+    if (total > stage_cap) {
+      DevOpLock lk(g_capture_mu);
+      if (d_stage) (void)hipFree(d_stage);
+      d_stage = nullptr;
+      stage_cap = 0;
+      if (hipMalloc(&d_stage, std::max<size_t>(total, 256)) != hipSuccess)
+        return fail(KKT_ERR_ALLOC, "hipMalloc failed (assembly inputs)");
+      stage_cap = std::max<size_t>(total, 256);
+    }
+    size_t off = 0;
+    dp.clear();
+    for (auto& a : in) {
+      if (a.second && hipMemcpyAsync(d_stage + off, a.first, a.second, hipMemcpyHostToDevice, st) != hipSuccess)
+        return fail(KKT_ERR_COPY, "hipMemcpyAsync (assembly inputs) failed");
+      dp.push_back((const double*)(d_stage + off));
+      off += (a.second + 255) & ~(size_t)255;
+    }
+    return KKT_OK;
+  }
+  int ensure_contrib(size_t cnt) {
+    if (cnt <= contrib_cap) return KKT_OK;
+    DevOpLock lk(g_capture_mu);
+    if (d_contrib) (void)hipFree(d_contrib);
+    d_contrib = nullptr;
+    contrib_cap = 0;
+    if (hipMalloc(&d_contrib, std::max<size_t>(cnt, 1) * sizeof(double)) != hipSuccess)
+      return fail(KKT_ERR_ALLOC, "hipMalloc failed (assembly contributions)");
+    contrib_cap = std::max<size_t>(cnt, 1);
+    return KKT_OK;
+  }
+  // common tail: plan, contributions (produce(d_contrib)), ordered scatter, sync
+  template <typename Fn>
+  int device_add(const int* slots, long long cnt, double scale, Fn produce) {
+    if (state < ANALYZED) return fail(KKT_ERR_STATE, "analyze() not previously called");
+    AddPlan* pl = nullptr;
+    int rc;
+    if ((rc = plan_for(slots, cnt, pl))) return rc;
+    if ((rc = ensure_contrib((size_t)cnt))) return rc;
+    if ((rc = produce(d_contrib))) return rc;
+    launch_seg_scatter(pl->nuniq, pl->d_useg, pl->d_uslot, pl->d_order, d_contrib, scale, d_vals, st);
+    if (hipStreamSynchronize(st) != hipSuccess) return fail(KKT_ERR_DEVICE, "device value assembly failed");
+    return KKT_OK;
+  }
+
   // reads `cnt` device scalars starting at slot `first` (one stream sync)
   int read_scalars(int first, int cnt, double* out) {
     if (hipMemcpyAsync(out, d_scal + first, cnt * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -1093,6 +1231,97 @@ int kkt_factor_device(kkt_solver* h, const double* d_vals) {
   return rc;
 }
 
+int kkt_clear_device_values(kkt_solver* h) {
+  CHECK_H(h);
+  if (h->state < kkt_solver::ANALYZED) return h->fail(KKT_ERR_STATE, "analyze() not previously called");
+  launch_fill(h->nnz, 0.0, h->d_vals, h->st);
+  if (hipStreamSynchronize(h->st) != hipSuccess) return h->fail(KKT_ERR_DEVICE, "clear failed");
+  h->ok();
+  return KKT_OK;
+}
+
+int kkt_get_device_values(kkt_solver* h, double* vals) {
+  CHECK_H(h);
+  if (!vals) return h->fail(KKT_ERR_ARGUMENT, "null values");
+  if (h->state < kkt_solver::ANALYZED) return h->fail(KKT_ERR_STATE, "analyze() not previously called");
+  if (hipMemcpyAsync(vals, h->d_vals, (size_t)h->nnz * sizeof(double), hipMemcpyDeviceToHost, h->st) != hipSuccess ||
+      hipStreamSynchronize(h->st) != hipSuccess)
+    return h->fail(KKT_ERR_COPY, "hipMemcpyAsync (device values) failed");
+  h->ok();
+  return KKT_OK;
+}
+
+int kkt_add_device_values(kkt_solver* h, const int* slots, const double* vals, int nvals, double scale) {
+  CHECK_H(h);
+  if (nvals < 0 || (nvals > 0 && (!slots || !vals))) return h->fail(KKT_ERR_ARGUMENT, "bad slot/value arrays");
+  std::vector<const double*> dp;
+  int rc = h->device_add(slots, nvals, scale, [&](double* c) -> int {
+    if (nvals > 0 && hipMemcpyAsync(c, vals, (size_t)nvals * sizeof(double), hipMemcpyHostToDevice, h->st) != hipSuccess)
+      return h->fail(KKT_ERR_COPY, "hipMemcpyAsync (values) failed");
+    return KKT_OK;
+  });
+  if (!rc) h->ok();
+  return rc;
+}
+
+int kkt_add_scaled_diagonal3_device_values(kkt_solver* h, const int* diagSlots, const double* masses,
+                                           int nblocks, double scale) {
+  CHECK_H(h);
+  if (nblocks < 0 || (nblocks > 0 && (!diagSlots || !masses))) return h->fail(KKT_ERR_ARGUMENT, "bad block arrays");
+  std::vector<const double*> dp;
+  int rc = h->device_add(diagSlots, 3LL * nblocks, scale, [&](double* c) -> int {
+    int r = h->stage_inputs({{masses, (size_t)nblocks * sizeof(double)}}, dp);
+    if (r) return r;
+    launch_contrib_diag3(nblocks, dp[0], c, h->st);
+    return KKT_OK;
+  });
+  if (!rc) h->ok();
+  return rc;
+}
+
+int kkt_add_scaled_block3_device_values(kkt_solver* h, const int* blockSlots, const double* blockVals,
+                                        const double* blockScales, int nblocks, double scale) {
+  CHECK_H(h);
+  if (nblocks < 0 || (nblocks > 0 && (!blockSlots || !blockVals || !blockScales)))
+    return h->fail(KKT_ERR_ARGUMENT, "bad block arrays");
+  std::vector<const double*> dp;
+  int rc = h->device_add(blockSlots, 9LL * nblocks, scale, [&](double* c) -> int {
+    int r = h->stage_inputs({{blockVals, (size_t)nblocks * 9 * sizeof(double)},
+                             {blockScales, (size_t)nblocks * sizeof(double)}}, dp);
+    if (r) return r;
+    launch_contrib_block3(nblocks, dp[0], dp[1], c, h->st);
+    return KKT_OK;
+  });
+  if (!rc) h->ok();
+  return rc;
+}
+
+int kkt_add_material_stiffness3_device_values(kkt_solver* h, const int* blockSlots, const double* gis,
+                                              const double* gjs, const double* Ds, const double* sigmas,
+                                              const double* dvs, int nblocks, double scale) {
+  CHECK_H(h);
+  if (nblocks < 0 || (nblocks > 0 && (!blockSlots || !gis || !gjs || !Ds || !sigmas || !dvs)))
+    return h->fail(KKT_ERR_ARGUMENT, "bad descriptor arrays");
+  std::vector<const double*> dp;
+  int rc = h->device_add(blockSlots, 9LL * nblocks, scale, [&](double* c) -> int {
+    const size_t nb = (size_t)nblocks * sizeof(double);
+    int r = h->stage_inputs({{gis, 3 * nb}, {gjs, 3 * nb}, {Ds, 36 * nb}, {sigmas, 6 * nb}, {dvs, nb}}, dp);
+    if (r) return r;
+    launch_contrib_matstiff3(nblocks, dp[0], dp[1], dp[2], dp[3], dp[4], c, h->st);
+    return KKT_OK;
+  });
+  if (!rc) h->ok();
+  return rc;
+}
+
+int kkt_factor_device_values(kkt_solver* h) {
+  CHECK_H(h);
+  if (h->state < kkt_solver::ANALYZED) return h->fail(KKT_ERR_STATE, "analyze() not previously called");
+  int rc = h->factor_device(h->d_vals, true);
+  if (!rc) h->ok();
+  return rc;
+}
+
 int kkt_solve(kkt_solver* h, const double* b, double* x) {
   CHECK_H(h);
   if (!b || !x) return h->fail(KKT_ERR_ARGUMENT, "null vector");
@@ -1131,11 +1360,14 @@ int kkt_solve_multi(kkt_solver* h, int nrhs, const double* B, double* X) {
 int kkt_multiply(kkt_solver* h, const double* x, double* y) {
   CHECK_H(h);
   if (!x || !y) return h->fail(KKT_ERR_ARGUMENT, "null vector");
-  if (h->state < kkt_solver::FACTORED) return h->fail(KKT_ERR_STATE, "multiply called before factor");
+  if (h->state < kkt_solver::ANALYZED) return h->fail(KKT_ERR_STATE, "multiply called before analyze");
   double* dx = h->vec(10);
   double* dy = h->vec(11);
   if (hipMemcpyAsync(dx, x, (size_t)h->n * sizeof(double), hipMemcpyHostToDevice, h->st) != hipSuccess)
     return h->fail(KKT_ERR_COPY, "hipMemcpyAsync (x) failed");
+  // A is the matrix in the device value buffer (factor values or device assembly)
+  int rc = h->set_values_device(h->d_vals);
+  if (rc) return rc;
   h->spmv(dx, nullptr, dy, 0);
   if (hipMemcpyAsync(y, dy, (size_t)h->n * sizeof(double), hipMemcpyDeviceToHost, h->st) != hipSuccess ||
       hipStreamSynchronize(h->st) != hipSuccess)

@@ -288,6 +288,66 @@ class MI355XSolver:
         x[:n * nrhs] = XX
         return x
 
+    # ---- device-side value assembly (CuDssSolver.java:421-788) ----
+    def _need_analyze(self):
+        self._alive()
+        if self._state == self.UNSET:
+            raise ImproperStateException("analyze() not previously called")
+
+    def clearDeviceValues(self):
+        self._need_analyze()
+        self._check(self._L.kkt_clear_device_values(self._h), "clearDeviceValues")
+
+    def getDeviceValues(self, vals):
+        self._need_analyze()
+        out = np.zeros(self._numVals)
+        self._check(self._L.kkt_get_device_values(self._h, ptr(out, ctypes.c_double)), "getDeviceValues")
+        vals[:self._numVals] = out
+        return vals
+
+    def addDeviceValues(self, slots, vals, nvals, scale):
+        self._need_analyze()
+        sl, va = _i32(slots), _f64(vals)
+        if nvals < 0 or nvals > sl.size or nvals > va.size:
+            raise ValueError("nvals exceeds slot/value array length")
+        self._check(self._L.kkt_add_device_values(self._h, ptr(sl, ctypes.c_int), ptr(va, ctypes.c_double),
+                                                  int(nvals), float(scale)), "addDeviceValues")
+
+    def addScaledDiagonal3DeviceValues(self, diagSlots, masses, nblocks, scale):
+        self._need_analyze()
+        sl, m = _i32(diagSlots), _f64(masses)
+        if nblocks < 0 or nblocks > sl.size // 3 or nblocks > m.size:
+            raise ValueError("nblocks exceeds diagonal-slot/mass array length")
+        self._check(self._L.kkt_add_scaled_diagonal3_device_values(
+            self._h, ptr(sl, ctypes.c_int), ptr(m, ctypes.c_double), int(nblocks), float(scale)),
+            "addScaledDiagonal3DeviceValues")
+
+    def addScaledBlock3DeviceValues(self, blockSlots, blockVals, blockScales, nblocks, scale):
+        self._need_analyze()
+        sl, bv, bs = _i32(blockSlots), _f64(blockVals), _f64(blockScales)
+        if nblocks < 0 or nblocks > sl.size // 9 or nblocks > bv.size // 9 or nblocks > bs.size:
+            raise ValueError("nblocks exceeds block slot/value/scale array length")
+        self._check(self._L.kkt_add_scaled_block3_device_values(
+            self._h, ptr(sl, ctypes.c_int), ptr(bv, ctypes.c_double), ptr(bs, ctypes.c_double),
+            int(nblocks), float(scale)), "addScaledBlock3DeviceValues")
+
+    def addMaterialStiffness3DeviceValues(self, blockSlots, gis, gjs, Ds, sigmas, dvs, nblocks, scale):
+        self._need_analyze()
+        arr = [_i32(blockSlots), _f64(gis), _f64(gjs), _f64(Ds), _f64(sigmas), _f64(dvs)]
+        per = [9, 3, 3, 36, 6, 1]
+        if nblocks < 0 or any(nblocks > a.size // k for a, k in zip(arr, per)):
+            raise ValueError("nblocks exceeds material stiffness descriptor array length")
+        sl, gi, gj, D, sg, dv = arr
+        self._check(self._L.kkt_add_material_stiffness3_device_values(
+            self._h, ptr(sl, ctypes.c_int), ptr(gi, ctypes.c_double), ptr(gj, ctypes.c_double),
+            ptr(D, ctypes.c_double), ptr(sg, ctypes.c_double), ptr(dv, ctypes.c_double),
+            int(nblocks), float(scale)), "addMaterialStiffness3DeviceValues")
+
+    def factorDeviceValues(self):
+        self._need_analyze()
+        self._check(self._L.kkt_factor_device_values(self._h), "factorDeviceValues")
+        self._state = self.FACTORED
+
     def multiply(self, x, y):
         self._alive()
         xx = _f64(x)

@@ -92,6 +92,46 @@ int kkt_analyze_values(kkt_solver* h, const double* vals);
 int kkt_factor(kkt_solver* h, const double* vals);
 /* Same with the values already resident in device memory (no H2D copy). */
 int kkt_factor_device(kkt_solver* h, const double* d_vals);
+
+/* Device-side value assembly (the step before the factor).  The handle keeps
+ * one device CSR value buffer per analyzed pattern (nnz doubles, the array
+ * kkt_factor copies into); these calls edit it in place and
+ * kkt_factor_device_values factors it.  All require a previous analyze.
+ * Slots are 0-based positions in the CSR value array; negative slots are
+ * skipped.  Contributions to one slot are summed in a fixed order (no atomics),
+ * so results are bitwise reproducible.  Slot lists are planned once and cached
+ * by content (up to 16 per handle). */
+/* CuDssSolver.doClearDeviceValues (CuDssJNI.cc:66-71; cudssBridge.h:88): zero. */
+int kkt_clear_device_values(kkt_solver* h);
+/* CuDssSolver.doGetDeviceValues (CuDssJNI.cc:73-84; cudssBridge.h:135): copy
+ * the buffer to vals (nnz doubles). */
+int kkt_get_device_values(kkt_solver* h, double* vals);
+/* CuDssSolver.doAddDeviceValues (CuDssJNI.cc:86-104; cudssBridge.h:89):
+ * buf[slots[i]] += scale * vals[i], i < nvals. */
+int kkt_add_device_values(kkt_solver* h, const int* slots, const double* vals, int nvals,
+                          double scale);
+/* CuDssSolver.doAddScaledDiagonal3DeviceValues (CuDssJNI.cc:106-129):
+ * buf[diagSlots[3i+k]] += scale * masses[i], i < nblocks, k < 3. */
+int kkt_add_scaled_diagonal3_device_values(kkt_solver* h, const int* diagSlots,
+                                           const double* masses, int nblocks, double scale);
+/* CuDssSolver.doAddScaledBlock3DeviceValues (CuDssJNI.cc:131-162):
+ * buf[blockSlots[9i+j]] += scale * blockScales[i] * blockVals[9i+j]. */
+int kkt_add_scaled_block3_device_values(kkt_solver* h, const int* blockSlots,
+                                        const double* blockVals, const double* blockScales,
+                                        int nblocks, double scale);
+/* CuDssSolver.doAddMaterialStiffness3DeviceValues (CuDssJNI.cc:164-202;
+ * FemUtilities.addMaterialStiffness): per block i the 3x3 matrix
+ * K = dv_i (B(gi)^T D_i B(gj) + (gi^T sigma_i gj) I), B the 6x3 Voigt
+ * strain-displacement matrix (order xx,yy,zz,xy,yz,xz), D_i 6x6 row-major,
+ * sigma_i = [s00,s11,s22,s01,s12,s02]; buf[blockSlots[9i+j]] += scale * K[j]
+ * (row-major). */
+int kkt_add_material_stiffness3_device_values(kkt_solver* h, const int* blockSlots,
+                                              const double* gis, const double* gjs,
+                                              const double* Ds, const double* sigmas,
+                                              const double* dvs, int nblocks, double scale);
+/* CuDssSolver.doFactorDeviceValues (CuDssJNI.cc:629-652; cudssBridge.h:130):
+ * factor the device value buffer (no host copy). */
+int kkt_factor_device_values(kkt_solver* h);
 /* CuDssSolver.doSolve (CuDssJNI.cc:640-651; cudssBridge.cc:1376-1417):
  * x = A^-1 b (host arrays of n doubles), plus the configured refinement. */
 int kkt_solve(kkt_solver* h, const double* b, double* x);
@@ -99,8 +139,10 @@ int kkt_solve(kkt_solver* h, const double* b, double* x);
 int kkt_solve_device(kkt_solver* h, const double* d_b, double* d_x);
 /* CuDssSolver.doSolveMulti (cudssBridge.cc:1454-1523): column-major n x nrhs. */
 int kkt_solve_multi(kkt_solver* h, int nrhs, const double* B, double* X);
-/* CuDssSolver.doMultiply (cudssBridge.cc:1419-1452): y = A x with the values
- * of the last factor / iterative solve. */
+/* CuDssSolver.doMultiply (CuDssJNI.cc:654-675; cudssBridge.cc:1419-1452;
+ * CuDssSolver.java:443-465): y = A x with the matrix in the device value buffer
+ * (the last factor's or iterative solve's values, or device assembly).  Needs a
+ * previous analyze. */
 int kkt_multiply(kkt_solver* h, const double* x, double* y);
 /* CuDssSolver.doIterativeSolve (CuDssJNI.cc:700-716; cudssBridge.cc:1705-2006):
  * BiCGStab on the NEW values `vals`, preconditioned by the existing (stale)

@@ -138,3 +138,43 @@ class Factor:
         if h:
             lib().orc_factor_free(h)
             self._h = None
+
+
+# ---------------------------------------------------------------------------
+# Device-side value assembly (test restatement)
+# ---------------------------------------------------------------------------
+def voigt_B(g):
+    """6x3 strain-displacement matrix of a shape-function gradient g, Voigt strain
+    order xx, yy, zz, xy, yz, xz with engineering shears (FemUtilities.java:78-124)."""
+    gx, gy, gz = g
+    return np.array([[gx, 0, 0], [0, gy, 0], [0, 0, gz],
+                     [gy, gx, 0], [0, gz, gy], [gz, 0, gx]], float)
+
+
+def material_stiffness3(gi, gj, D, sig, dv):
+    """FemUtilities.addMaterialStiffness(K, gi, D, sig, gj, dv)
+    (FemUtilities.java:61-66, 78-124, 131-146): K = dv (Bi^T D Bj + (gi^T S gj) I),
+    D 6x6 row-major, sig = [s00, s11, s22, s01, s12, s02]."""
+    D = np.asarray(D, float).reshape(6, 6)
+    s = np.asarray(sig, float)
+    S = np.array([[s[0], s[3], s[5]], [s[3], s[1], s[4]], [s[5], s[4], s[2]]])
+    gi = np.asarray(gi, float)
+    gj = np.asarray(gj, float)
+    return dv * (voigt_B(gi).T @ D @ voigt_B(gj) + (gi @ S @ gj) * np.eye(3))
+
+
+def isotropic_D(lam, mu):
+    """Linear isotropic tangent in the same Voigt convention (engineering shears)."""
+    D = np.zeros((6, 6))
+    D[:3, :3] = lam
+    D[0, 0] = D[1, 1] = D[2, 2] = lam + 2 * mu
+    D[3, 3] = D[4, 4] = D[5, 5] = mu
+    return D
+
+
+def scatter_add(vals, slots, contrib, scale):
+    """vals[slots[e]] += scale * contrib[e] for slots >= 0, in index order."""
+    slots = np.asarray(slots)
+    keep = slots >= 0
+    np.add.at(vals, slots[keep], scale * np.asarray(contrib, float)[keep])
+    return vals

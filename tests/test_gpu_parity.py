@@ -326,3 +326,126 @@ def test_graph_replays_match_eager_launches(gpu, prec):
     finally:
         for s in solvers:
             s.dispose()
+
+
+def _csr_slots(ro, ci, rows, cols):
+    """Positions of (row, col) entries in an upper CSR (-1 below the diagonal)."""
+    n = len(ro) - 1
+    kr = np.repeat(np.arange(n), np.diff(ro)).astype(np.int64)
+    keys = kr * n + np.asarray(ci, np.int64)
+    rows = np.asarray(rows, np.int64)
+    cols = np.asarray(cols, np.int64)
+    q = rows * n + cols
+    pos = np.searchsorted(keys, q)
+    pos = np.minimum(pos, len(keys) - 1)
+    ok = (keys[pos] == q) & (rows <= cols)
+    return np.where(ok, pos, -1).astype(np.int32)
+
+
+def test_device_value_assembly_ops(gpu):
+    # CuDssSolver.*DeviceValues (CuDssSolver.java:421-788) against the oracle's
+    # restatement: duplicate and negative slots, 3-DOF diagonals, scaled 3x3 blocks,
+    # material stiffness blocks, and multiply with the assembled buffer
+    _say("device value assembly ops")
+    beam, vals, b = G.small_beam(8)
+    rng = np.random.default_rng(21)
+    s = MI355XSolver()
+    try:
+        s.analyzeArrays(vals, beam.colIdxs, beam.rowOffs, beam.n, Matrix.SYMMETRIC)
+        nnz = beam.nnz
+        s.clearDeviceValues()
+        got = s.getDeviceValues(np.ones(nnz))
+        assert not got.any()
+        ref = np.zeros(nnz)
+        slots = rng.integers(-3, nnz, 5000).astype(np.int32)
+        contrib = rng.standard_normal(5000)
+        s.addDeviceValues(slots, contrib, 5000, 0.5)
+        O.scatter_add(ref, slots, contrib, 0.5)
+        s.addDeviceValues(slots, contrib, 4000, -2.0)  # prefix: a second plan
+        O.scatter_add(ref, slots[:4000], contrib[:4000], -2.0)
+        ds = rng.integers(0, nnz, 3 * 300).astype(np.int32)
+        ds[::7] = -1
+        ms = rng.uniform(1, 2, 300)
+        s.addScaledDiagonal3DeviceValues(ds, ms, 300, 1.5)
+        O.scatter_add(ref, ds, np.repeat(ms, 3), 1.5)
+        bsl = rng.integers(-1, nnz, 9 * 200).astype(np.int32)
+        bv = rng.standard_normal(9 * 200)
+        bs = rng.uniform(0.5, 2, 200)
+        s.addScaledBlock3DeviceValues(bsl, bv, bs, 200, 0.25)
+        O.scatter_add(ref, bsl, bv * np.repeat(bs, 9), 0.25)
+        nb = 150
+        msl = rng.integers(-1, nnz, 9 * nb).astype(np.int32)
+        gis, gjs = rng.standard_normal((nb, 3)), rng.standard_normal((nb, 3))
+        Ds = rng.standard_normal((nb, 6, 6))
+        sig, dvs = rng.standard_normal((nb, 6)), rng.uniform(0.1, 1, nb)
+        s.addMaterialStiffness3DeviceValues(msl, gis, gjs, Ds, sig, dvs, nb, 0.75)
+        K = np.stack([O.material_stiffness3(gis[q], gjs[q], Ds[q], sig[q], dvs[q]) for q in range(nb)])
+        O.scatter_add(ref, msl, K.reshape(-1), 0.75)
+        got = s.getDeviceValues(np.zeros(nnz))
+        assert np.allclose(got, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+        # the same calls again reuse the cached plans and add the same amounts
+        s.addDeviceValues(slots, contrib, 5000, 0.5)
+        O.scatter_add(ref, slots, contrib, 0.5)
+        got = s.getDeviceValues(np.zeros(nnz))
+        assert np.allclose(got, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+        # multiply uses the device buffer
+        x = rng.standard_normal(beam.n)
+        y = s.multiply(x, np.zeros(beam.n))
+        assert np.allclose(y, full_from_upper(beam.rowOffs, beam.colIdxs, ref) @ x, rtol=1e-12,
+                           atol=1e-12 * np.abs(y).max())
+        with pytest.raises(NumericalException):
+            s.addDeviceValues(np.array([nnz], np.int32), np.ones(1), 1, 1.0)
+    finally:
+        s.dispose()
+
+
+@pytest.mark.parametrize("prec", [KKT_PREC_FP64, KKT_PREC_FP32_REFINE])
+def test_device_assembled_tet_kkt_solves(gpu, prec):
+    # the KKT of a tet body assembled on the device (lumped masses, material
+    # stiffness blocks from shape-function gradients, constraint entries), then
+    # factored from the device buffer: equals the host-assembled system
+    _say("device-assembled tet KKT prec=%d" % prec)
+    body, V, B = G.tet_precision_sequence(nsteps=1, dims=(21, 6, 6))
+    u0 = np.zeros((body.nn, 3))
+    fext = body.gravity_loads() + np.random.default_rng(22).uniform(-1e-2, 1e-2, (body.nn, 3))
+    vals, b = body.system(u0, u0, fext)  # undeformed: linear isotropic tangent
+    n, ro, ci = body.n, body.rowOffs, body.colIdxs
+    c10, kappa = body.material[1], body.material[2]
+    mu = 2 * c10
+    D = O.isotropic_D(kappa - 2 * mu / 3, mu)
+    E = len(body.elems)
+    g = body.grads
+    a_idx = np.repeat(np.arange(4), 4)
+    b_idx = np.tile(np.arange(4), 4)
+    na = body.elems[:, a_idx]  # (E, 16)
+    nb_ = body.elems[:, b_idx]
+    rows = (3 * na[:, :, None, None] + np.arange(3)[None, None, :, None]) + 0 * np.arange(3)[None, None, None, :]
+    cols = (3 * nb_[:, :, None, None] + np.arange(3)[None, None, None, :]) + 0 * np.arange(3)[None, None, :, None]
+    bslots = _csr_slots(ro, ci, rows.ravel(), cols.ravel())
+    gis = g[:, a_idx].reshape(-1, 3)
+    gjs = g[:, b_idx].reshape(-1, 3)
+    nblk = E * 16
+    Ds = np.broadcast_to(D, (nblk, 6, 6))
+    sig = np.zeros((nblk, 6))
+    dvs = np.repeat(body.vol, 16)
+    nodes = np.arange(body.nn)
+    dslots = _csr_slots(ro, ci, (3 * nodes[:, None] + np.arange(3)).ravel(),
+                        (3 * nodes[:, None] + np.arange(3)).ravel())
+    s = MI355XSolver(precision=prec)
+    try:
+        s.analyzeArrays(vals, ci, ro, n, Matrix.SYMMETRIC)
+        s.clearDeviceValues()
+        s.addScaledDiagonal3DeviceValues(dslots, body.mass[0::3], body.nn, 1.0)
+        s.addMaterialStiffness3DeviceValues(bslots, gis, gjs, Ds, sig, dvs, nblk,
+                                            body.h * body.beta + body.h * body.h)
+        s.addDeviceValues(body._gpos.astype(np.int32), np.ones(len(body._gpos)), len(body._gpos), 1.0)
+        got = s.getDeviceValues(np.zeros(body.nnz))
+        assert np.allclose(got, vals, rtol=1e-12, atol=1e-12 * np.abs(vals).max())
+        s.factorDeviceValues()
+        x = np.zeros(n)
+        s.solve(x, b)
+        assert rel_residual(ro, ci, vals, x, b) <= 1e-8
+        inf = s.info()
+        assert (inf["npos"], inf["nneg"]) == (body.sizeM, body.numG)
+    finally:
+        s.dispose()

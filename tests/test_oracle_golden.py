@@ -173,3 +173,34 @@ def test_planted_missing_2x2_pivot_is_caught():
     assert f1.info()["nperturbed"] >= 1
     # the perturbed factor misses the 1e-12 known-answer tolerance used above
     assert np.abs(f1.solve(b) - np.array([3.0, 2.0])).max() > 1e-10
+
+
+def test_material_stiffness_restatement_closed_forms():
+    # isotropic D and zero stress: K = dv (lam gi gj^T + mu gj gi^T + mu (gi.gj) I);
+    # a stress adds (gi^T S gj) dv I; the tet element stiffness of the generators is
+    # reproduced block by block
+    rng = np.random.default_rng(11)
+    lam, mu, dv = 3.7, 1.3, 0.25
+    D = O.isotropic_D(lam, mu)
+    for _ in range(5):
+        gi, gj = rng.standard_normal(3), rng.standard_normal(3)
+        K = O.material_stiffness3(gi, gj, D, np.zeros(6), dv)
+        ref = dv * (lam * np.outer(gi, gj) + mu * np.outer(gj, gi) + mu * (gi @ gj) * np.eye(3))
+        assert np.allclose(K, ref, rtol=0, atol=1e-14)
+        sig = rng.standard_normal(6)
+        S = np.array([[sig[0], sig[3], sig[5]], [sig[3], sig[1], sig[4]], [sig[5], sig[4], sig[2]]])
+        K2 = O.material_stiffness3(gi, gj, D, sig, dv)
+        assert np.allclose(K2 - K, dv * (gi @ S @ gj) * np.eye(3), rtol=0, atol=1e-14)
+        # a transposed operand is caught: K(gi, gj) = K(gj, gi)^T, not K(gj, gi)
+        assert np.allclose(O.material_stiffness3(gj, gi, D, np.zeros(6), dv), K.T, atol=1e-14)
+    nodes, tets = G.tet_grid(1, 1, 1, 1.0, 1.0, 1.0)
+    Ke, vol = G.tet_linear_stiffness(nodes, tets, lam, mu)
+    P0 = nodes[tets[:, 0]]
+    Dm = np.stack([nodes[tets[:, q]] - P0 for q in (1, 2, 3)], 2)
+    Dinv = np.linalg.inv(Dm)
+    for e in range(len(tets)):
+        g = np.vstack([-Dinv[e].sum(0), Dinv[e]])
+        for a in range(4):
+            for b in range(4):
+                blk = O.material_stiffness3(g[a], g[b], D, np.zeros(6), vol[e])
+                assert np.allclose(blk, Ke[e, 3 * a:3 * a + 3, 3 * b:3 * b + 3], rtol=0, atol=1e-12)
