@@ -833,16 +833,28 @@ struct kkt_solver {
     launch_gather_perm<double, T>(n, d_perm, d_b, bp, st);
     tend();
     const int nl = S.nlevels;
-    tbeg(TC_SLEVEL);
+    // timed per launch: class TC_SLEVEL = forward launches, TC_SGEMV = backward
     for (int L = 0; L < nl; L++) {
-      launch_fwd_gather<T>(d_fg + fg_l[L].off, fg_l[L].cnt, ds, F, bp, fp, y, u, st);
-      launch_fwd_rows<T>(d_fr + fr_l[L].off, fr_l[L].cnt, ds, F, fp, y, u, st);
+      if (fg_l[L].cnt) {
+        tbeg(TC_SLEVEL);
+        launch_fwd_gather<T>(d_fg + fg_l[L].off, fg_l[L].cnt, ds, F, bp, fp, y, u, st);
+        tend();
+      }
+      if (fr_l[L].cnt) {
+        tbeg(TC_SLEVEL);
+        launch_fwd_rows<T>(d_fr + fr_l[L].off, fr_l[L].cnt, ds, F, fp, y, u, st);
+        tend();
+      }
     }
+    tbeg(TC_SLEVEL);
     launch_dsolve<T>(n, E0, E1, d_piv, y, z, st);
     tend();
-    tbeg(TC_SGEMV);
-    for (int L = nl - 1; L >= 0; L--) launch_bwd_cols<T>(d_bc + bc_l[L].off, bc_l[L].cnt, ds, F, z, xp, st);
-    tend();
+    for (int L = nl - 1; L >= 0; L--) {
+      if (!bc_l[L].cnt) continue;
+      tbeg(TC_SGEMV);
+      launch_bwd_cols<T>(d_bc + bc_l[L].off, bc_l[L].cnt, ds, F, z, xp, st);
+      tend();
+    }
     tbeg(TC_VEC);
     launch_scatter_perm<T, double>(n, d_perm, xp, d_x, st);
     tend();

@@ -35,6 +35,38 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+def bwd_sweep_bytes(L, h, n, w):
+    """Algorithmic HBM bytes of one backward sweep (all k_bwd_cols launches of a
+    triangular solve): per supernode the lower copy of G once (k(k-1)/2 + k r
+    entries), z and x for its k pivots, the gathered x of its r rows, and the row /
+    permutation indices; w = bytes per entry of the front precision."""
+    from artisynth_core_amd._lib import KktInfo
+    from artisynth_core_amd._lib import ptr as nptr
+    inf = KktInfo()
+    L.kkt_get_info(h, ctypes.byref(inf))
+    ns = inf.nsuper
+    sup = np.zeros(ns + 1, np.int32)
+    rp = np.zeros(ns + 1, np.int64)
+    L.kkt_get_symbolic(h, None, nptr(sup, ctypes.c_int), None, nptr(rp, ctypes.c_longlong), None)
+    k = np.diff(sup).astype(np.float64)
+    r = np.diff(rp).astype(np.float64)
+    return float(np.sum((k * (k - 1) / 2 + k * r + 2 * k + r) * w + (r + k) * 4)), inf.nlevels
+
+
+def time_bwd_sweeps(L, h, b_ptr, x_ptr, nsolves=6):
+    """Per-launch HIP-event time of the backward-sweep kernel (timing class 5, eager
+    launches on the solver's stream) over nsolves refined solves."""
+    L.kkt_timing_reset(h, 1 << 5)
+    for _ in range(nsolves):
+        assert L.kkt_solve_device(h, b_ptr, x_ptr) == 0
+    ms = (ctypes.c_double * 8)()
+    cnt = (ctypes.c_longlong * 8)()
+    uf = ctypes.c_double()
+    L.kkt_timing_get(h, ms, cnt, ctypes.byref(uf))
+    L.kkt_timing_reset(h, 0)
+    return ms[5], int(cnt[5])
+
+
 def run_sequence(L, h, vals_d, b_d, x_d, steps, nsys, tol_exp=10, max_iter=30, record=None):
     """Runs `steps` KKT solves over the systems 0..nsys-1 (cyclic) with the hybrid
     policy of KKTSolver.factorAndSolve; returns per-step latencies (ms) and kinds."""
@@ -156,6 +188,10 @@ def main():
         L.kkt_timing_get(h, ms, cnt, ctypes.byref(uflops))
         inf = KktInfo()
         L.kkt_get_info(h, ctypes.byref(inf))
+        # after the timed region: live per-launch timing of the backward sweep kernel
+        x_t = torch.zeros(n, dtype=torch.float64, device=dev)  # not one of the verified solutions
+        bwd_ms, bwd_launches = time_bwd_sweeps(L, h, b_d[0].data_ptr(), x_t.data_ptr())
+        sweep_bytes, nlev = bwd_sweep_bytes(L, h, n, 4 if prec == KKT_PREC_FP32_REFINE else 8)
         # verify every step's solution (host, after timing)
         res = []
         for i in range(args.steps):
@@ -171,6 +207,8 @@ def main():
             "kernel_ms": {TIMING_CLASSES[c]: ms[c] for c in range(8)},
             "kernel_launches": {TIMING_CLASSES[c]: cnt[c] for c in range(8)},
             "update_flops": uflops.value, "info": inf.as_dict(),
+            "bwd_ms": bwd_ms, "bwd_launches": bwd_launches, "bwd_sweep_bytes": sweep_bytes,
+            "nlevels": nlev,
             "x": [x_d[i].cpu().numpy() for i in range(min(args.steps, len(V)))],
         }
         log("%s: %.2f solves/s, p50 %.2f ms, p99 %.2f ms, kinds %s, max rel resid %.2e" % (
@@ -181,19 +219,29 @@ def main():
     r32, r64 = results["fp32_mfma_refine"], results["fp64"]
     # forward error of each FP32-path solution against the FP64 path (same step)
     fwd = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(r32["x"], r64["x"])]
-    # dominant kernel: the MFMA Schur update of the FP32 factor
-    upd_ms = r32["kernel_ms"]["panel_update"]
-    upd_n = max(r32["kernel_launches"]["panel_update"], 1)
-    achieved = r32["update_flops"] / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
+    # dominant kernel over the sequence: the backward triangular sweep of the FP32
+    # path (k_bwd_cols<float>, HBM bound by design); algorithmic bytes per launch =
+    # the sweep's bytes / its launches (one per tree level)
+    nb = max(r32["bwd_launches"], 1)  # one timed launch per tree level and sweep
+    sweeps = nb / r32["nlevels"]
+    bpl = r32["bwd_sweep_bytes"] / r32["nlevels"]
+    achieved = r32["bwd_sweep_bytes"] * sweeps / (r32["bwd_ms"] * 1e-3) / 1e9 if r32["bwd_ms"] > 0 else 0.0
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_panel_update_fp32.json")
+    tf = os.path.join(ROOT, "profiles", "traffic_bwd_cols_fp32_hexbeam.json")
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get("bytes_per_launch")
-    roofline = {"bound": "mfma", "kernel": "k_panel_update<float>", "achieved": achieved,
-                "peak": PEAKS["fp32_TFLOPs"], "unit": "TFLOP/s",
-                "frac": achieved / PEAKS["fp32_TFLOPs"], "traffic": traffic,
-                "avg_launch_ms": upd_ms / upd_n, "launches": upd_n,
-                "flops_per_launch": r32["update_flops"] / upd_n}
+    roofline = {"bound": "hbm", "kernel": "k_bwd_cols<float>", "achieved": achieved,
+                "peak": PEAKS["hbm_GBs"], "unit": "GB/s", "frac": achieved / PEAKS["hbm_GBs"],
+                "traffic": traffic, "avg_launch_ms": r32["bwd_ms"] / nb, "launches": nb,
+                "bytes_per_launch": bpl}
+    # the factor's MFMA Schur update, for reference
+    upd_ms = r32["kernel_ms"]["panel_update"]
+    upd_n = max(r32["kernel_launches"]["panel_update"], 1)
+    upd_tf = r32["update_flops"] / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
+    roofline_update = {"bound": "mfma", "kernel": "k_panel_update<float>", "achieved": upd_tf,
+                       "peak": PEAKS["fp32_TFLOPs"], "unit": "TFLOP/s",
+                       "frac": upd_tf / PEAKS["fp32_TFLOPs"], "avg_launch_ms": upd_ms / upd_n,
+                       "launches": upd_n, "flops_per_launch": r32["update_flops"] / upd_n}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -221,7 +269,7 @@ def main():
         "bicgstab_iters": r32["iters"],
         "factor_gflop": r32["info"]["factor_flops"] / 1e9, "nnzL": r32["info"]["nnzL"],
         "kernel_ms_fp32": r32["kernel_ms"], "kernel_ms_fp64": r64["kernel_ms"],
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "roofline_factor_update": roofline_update, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 
