@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void k_panel_diag(const int* __restrict__ item
   const int j0 = p * NB, w = min(NB, k - j0);
   T* Fs = F + d.sn_foff[s];
   const int tid = threadIdx.x, t = tid & 63, wv = tid >> 6;
-  {
+  auto load_block = [&]() {
     T ld[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) {
@@ -113,17 +113,78 @@ __global__ __launch_bounds__(256) void k_panel_diag(const int* __restrict__ item
     }
 #pragma unroll
     for (int q = 0; q < 16; q++) A[t * 65 + wv + 4 * q] = (double)ld[q];
-  }
-  if (wv == 0) {
-    lp[t] = t;
-    pt[t] = 1;
-  }
+    if (wv == 0) {
+      lp[t] = t;
+      pt[t] = 1;
+    }
+  };
+  load_block();
   __syncthreads();
   const double alpha = (1.0 + sqrt(17.0)) / 8.0;
   const double tau = pert * amax[0];
   int npos = 0, nneg = 0, npert = 0, n2 = 0;
+  // Fast path: Bunch-Kaufman takes the 1x1 pivot without interchange whenever
+  // |a_kk| >= alpha * colmax (or perturbs a pivot below tau).  Columns are taken
+  // in 16-wide slabs: rank-1 updates inside the slab, one rank-16 update of the
+  // rest of the block after it.  The first column that needs anything else sends
+  // the whole block back to the full pivoting loop below, from the original data.
+  // Either way the lower part ends with the unscaled pivot columns and dv set.
+  bool fast = true;
+  for (int c0 = 0; c0 < w; c0 += 16) {
+    const int c1 = min(c0 + 16, w);
+    for (int c = c0; c < c1; c++) {
+      const double absakk = uni(fabs(A[c * 65 + c]));
+      const double cv = (t > c && t < w) ? fabs(A[t * 65 + c]) : -1.0;
+      const double colmax = fmax(wave_max_u(cv), 0.0);
+      const bool pz = fmax(absakk, colmax) < tau;
+      if (!pz && !(absakk >= alpha * colmax)) {
+        fast = false;
+        break;
+      }
+      double dd = uni(A[c * 65 + c]);
+      if (pz || fabs(dd) < tau) {
+        dd = (dd >= 0.0) ? tau : -tau;
+        npert++;
+      }
+      if (dd > 0) npos++; else nneg++;
+      const double dinv = 1.0 / dd;
+      if (t > c && t < w) {
+        const double lt = A[t * 65 + c] * dinv;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int j = c0 + 4 * wv + q;
+          if (j > c && j < c1) A[t * 65 + j] = fma(-lt, A[j * 65 + c], A[t * 65 + j]);
+        }
+      }
+      if (tid == 0) {
+        dv[c][0] = dinv;
+        dv[c][1] = dd;
+      }
+      __syncthreads();
+    }
+    if (!fast) break;
+    // A(t, j) -= sum_{q in slab} (a_tq / d_q) a_jq for the columns after the slab
+    if (t >= c1 && t < w) {
+      double lq[16];
+#pragma unroll
+      for (int q = 0; q < 16; q++) lq[q] = (c0 + q < c1) ? A[t * 65 + c0 + q] * dv[c0 + q][0] : 0.0;
+      for (int j = c1 + wv; j < w; j += 4) {
+        double acc = A[t * 65 + j];
+#pragma unroll
+        for (int q = 0; q < 16; q++) acc = fma(-lq[q], (c0 + q < c1) ? A[j * 65 + c0 + q] : 0.0, acc);
+        A[t * 65 + j] = acc;
+      }
+    }
+    __syncthreads();
+  }
+  if (!fast) {
+    __syncthreads();  // every wave has finished reading the block before it is reloaded
+    load_block();
+    npos = nneg = npert = 0;
+    __syncthreads();
+  }
   const int q0 = wv * 16;  // first of this wave's 16 update columns
-  int kc = 0;
+  int kc = fast ? w : 0;
   while (kc < w) {
     const double absakk = uni(fabs(A[kc * 65 + kc]));
     double cv = (t > kc && t < w) ? fabs(A[t * 65 + kc]) : -1.0;
