@@ -28,7 +28,7 @@ class KktInfo(ctypes.Structure):
         ("front_bytes", ctypes.c_longlong), ("npos", ctypes.c_int), ("nneg", ctypes.c_int),
         ("nperturbed", ctypes.c_int), ("n2x2", ctypes.c_int), ("precision", ctypes.c_int),
         ("refine_steps", ctypes.c_int), ("fp64_fallbacks", ctypes.c_int),
-        ("last_rel_residual", ctypes.c_double),
+        ("last_rel_residual", ctypes.c_double), ("pivoted_blocks", ctypes.c_int),
     ]
 
     def as_dict(self):

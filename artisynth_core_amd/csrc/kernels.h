@@ -38,6 +38,7 @@ struct DevSym {
 
 struct FactorCounters {  // device ints
   int npos, nneg, npert, n2x2;
+  int npivoted;  // diagonal blocks that needed the full pivoting loop
 };
 
 template <typename T>

@@ -29,6 +29,31 @@ __device__ inline int lower_bound_i(const int* a, int n, int key) {
 }
 
 // ------------------------------------------------------------------------
+// MFMA 16x16x4 wrappers (f64 / f32).  Lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; the 16x16 result sits in lane l, element v, at column l%16 and
+// row Mfma<T>::row(l/16, v): 4*(l/16)+v for f32, l/16+4*v for f64 (gfx950).
+// ------------------------------------------------------------------------
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 op(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // row of accumulator element v held by a lane in row group lk = lane/16
+  static __device__ inline int row(int lk, int v) { return lk + 4 * v; }
+};
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 op(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lk, int v) { return 4 * lk + v; }
+};
+
+// ------------------------------------------------------------------------
 // assembly: one work item = one 64x64 lower tile of a parent front
 // ------------------------------------------------------------------------
 template <typename T>
@@ -77,26 +102,41 @@ __global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ aidx,
   if (e < cnt) F[apos[e]] += (T)vals[aidx[e]];
 }
 
+// 1/d from the hardware reciprocal and two Newton steps (within an ulp; the pivots
+// are bounded away from 0 and from overflow by the static perturbation)
+__device__ inline double rcp_newton(double d) {
+  double y = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-d, y, 1.0);
+  return fma(y, e, y);
+}
+
 // ------------------------------------------------------------------------
 // panel diagonal block: Bunch-Kaufman restricted to the block, FP64 in LDS
 //
-// Four waves per front.  Every wave runs the pivot search on the same LDS data
-// (DPP reductions, identical decisions), wave 0 applies interchanges, and each
-// wave applies the rank-1/rank-2 update to its 16 columns of every trailing row.
-// Pivot columns stay unscaled until the loop ends, so a step needs one barrier
-// (three with an interchange).  The inverse X = L11^-1 is then formed in 16x16
-// blocks: diagonal blocks by substitution, off-diagonal blocks as
-// X(I,J) = -X(I,I) sum_{J<=Q<I} L(I,Q) X(Q,J).
+// Four waves per front.  A block whose every column passes the no-interchange
+// test is factored slab by slab (one wave pivots 16 columns in registers, all
+// four apply the rank-16 update).  Otherwise the full pivoting loop runs: every
+// wave does the pivot search on the same LDS data (DPP reductions, identical
+// decisions), wave 0 applies interchanges, and each wave applies the rank-1/2
+// update to its 16 columns of every trailing row; pivot columns stay unscaled
+// until the loop ends, so a step needs one barrier (three with an interchange).
+// The inverse X = L11^-1 is then formed in 16x16 blocks: diagonal blocks by
+// substitution, off-diagonal blocks as X(I,J) = -X(I,I) sum_{J<=Q<I} L(I,Q) X(Q,J)
+// with f64 MFMA.
 // ------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel_diag(const int* __restrict__ items, int p, DevSym d,
+// three workgroups per CU (VGPRs <= 168 per lane, 44 KB of LDS each)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_panel_diag(const int* __restrict__ items, int p, DevSym d,
                                                     T* __restrict__ F, T* __restrict__ E0,
                                                     T* __restrict__ E1, double pert,
                                                     const double* __restrict__ amax,
                                                     FactorCounters* __restrict__ ctr) {
   __shared__ double A[64 * 65];     // A[row*65 + col]: lower part L/D, upper part X
   __shared__ double dv[64][3];      // 1x1: {1/d, d}; 2x2 (first column): D^-1 {i11, i21, i22}
-  __shared__ double Tm[3][16 * 17];  // partial products of the blocked inverse
+  __shared__ double Sb[64 * 17];    // fast path: Sb[q*64 + t] = unscaled a(t, c0 + q) of the slab
+  __shared__ int sfail;
   __shared__ int lp[64];
   __shared__ signed char pt[64];
   const int s = items[blockIdx.x];
@@ -105,73 +145,107 @@ __global__ __launch_bounds__(256) void k_panel_diag(const int* __restrict__ item
   T* Fs = F + d.sn_foff[s];
   const int tid = threadIdx.x, t = tid & 63, wv = tid >> 6;
   auto load_block = [&]() {
+    // addresses clamped into the block so the 16 loads issue back to back
     T ld[16];
+    const int tc = min(t, w - 1);
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int jc = min(wv + 4 * q, w - 1);
+      ld[q] = Fs[(size_t)(j0 + jc) * m + j0 + tc];
+    }
 #pragma unroll
     for (int q = 0; q < 16; q++) {
       const int j = wv + 4 * q;
-      ld[q] = (j < w && t >= j && t < w) ? Fs[(size_t)(j0 + j) * m + j0 + t] : (T)0;
+      A[t * 65 + j] = (j < w && t >= j && t < w) ? (double)ld[q] : 0.0;
     }
-#pragma unroll
-    for (int q = 0; q < 16; q++) A[t * 65 + wv + 4 * q] = (double)ld[q];
     if (wv == 0) {
       lp[t] = t;
       pt[t] = 1;
     }
   };
   load_block();
+  if (tid == 0) sfail = 0;
   __syncthreads();
   const double alpha = (1.0 + sqrt(17.0)) / 8.0;
   const double tau = pert * amax[0];
   int npos = 0, nneg = 0, npert = 0, n2 = 0;
   // Fast path: Bunch-Kaufman takes the 1x1 pivot without interchange whenever
-  // |a_kk| >= alpha * colmax (or perturbs a pivot below tau).  Columns are taken
-  // in 16-wide slabs: rank-1 updates inside the slab, one rank-16 update of the
-  // rest of the block after it.  The first column that needs anything else sends
-  // the whole block back to the full pivoting loop below, from the original data.
-  // Either way the lower part ends with the unscaled pivot columns and dv set.
+  // |a_kk| >= alpha * colmax (or perturbs a pivot below tau).  Wave 0 factors the
+  // block in 16-column slabs held in registers (lane t: row t of the slab; each
+  // pivot column goes to LDS and its entries come back as broadcasts while the
+  // pivot test and reciprocal run), then all four waves apply
+  // the slab's rank-16 update to the rest of the block.  The first column that
+  // needs anything else sends the whole block back to the full pivoting loop
+  // below, from the original data.  On success the lower part holds L with D on
+  // the diagonal (what the pivoting loop and its scaling pass produce).
   bool fast = true;
   for (int c0 = 0; c0 < w; c0 += 16) {
     const int c1 = min(c0 + 16, w);
-    for (int c = c0; c < c1; c++) {
-      const double absakk = uni(fabs(A[c * 65 + c]));
-      const double cv = (t > c && t < w) ? fabs(A[t * 65 + c]) : -1.0;
-      const double colmax = fmax(wave_max_u(cv), 0.0);
-      const bool pz = fmax(absakk, colmax) < tau;
-      if (!pz && !(absakk >= alpha * colmax)) {
-        fast = false;
-        break;
-      }
-      double dd = uni(A[c * 65 + c]);
-      if (pz || fabs(dd) < tau) {
-        dd = (dd >= 0.0) ? tau : -tau;
-        npert++;
-      }
-      if (dd > 0) npos++; else nneg++;
-      const double dinv = 1.0 / dd;
-      if (t > c && t < w) {
-        const double lt = A[t * 65 + c] * dinv;
+    if (wv == 0) {
+      double r[16];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const int j = c0 + 4 * wv + q;
-          if (j > c && j < c1) A[t * 65 + j] = fma(-lt, A[j * 65 + c], A[t * 65 + j]);
+      for (int q = 0; q < 16; q++) r[q] = A[t * 65 + c0 + q];
+      bool ok = true;
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int c = c0 + q;
+        if (ok && c < c1) {
+          const double akk = readlane_d(r[q], c);
+          Sb[q * 64 + t] = r[q];
+          __builtin_amdgcn_wave_barrier();
+          const double absakk = fabs(akk);
+          const bool below = t > c && t < w;
+          const double av = fabs(r[q]);
+          // colmax < tau and |a_kk| < tau: perturbed 1x1 pivot; otherwise the
+          // test |a_kk| >= alpha colmax must hold in every row below
+          const unsigned long long big = __builtin_amdgcn_ballot_w64(below && av >= tau);
+          const unsigned long long bad = __builtin_amdgcn_ballot_w64(below && absakk < alpha * av);
+          const bool pz = (absakk < tau) & (big == 0ULL);
+          if (!pz & (bad != 0ULL)) {
+            ok = false;
+          } else {
+            double dd = akk;
+            if (pz || absakk < tau) {
+              dd = (dd >= 0.0) ? tau : -tau;
+              npert++;
+            }
+            if (dd > 0) npos++; else nneg++;
+            const double dinv = rcp_newton(dd);
+            // rows on or above c take lt = 0 and keep their values (a(c0 + j, c)
+            // is finite), so the update needs no select
+            const double lt = below ? r[q] * dinv : 0.0;
+#pragma unroll
+            for (int j = q + 1; j < 16; j++) r[j] = fma(-lt, Sb[q * 64 + c0 + j], r[j]);
+            r[q] = below ? lt : (t == c ? dd : r[q]);
+            if (t == 0) {
+              dv[c][0] = dinv;
+              dv[c][1] = dd;
+            }
+          }
         }
       }
-      if (tid == 0) {
-        dv[c][0] = dinv;
-        dv[c][1] = dd;
+      if (!ok) {
+        if (t == 0) sfail = 1;
+      } else {
+#pragma unroll
+        for (int q = 0; q < 16; q++) A[t * 65 + c0 + q] = r[q];
       }
-      __syncthreads();
     }
-    if (!fast) break;
-    // A(t, j) -= sum_{q in slab} (a_tq / d_q) a_jq for the columns after the slab
+    __syncthreads();
+    if (uni(sfail)) {
+      fast = false;
+      break;
+    }
+    // A(t, j) -= sum_{q in slab} L(t, q) a(j, q) for the columns after the slab
+    // (a full slab whenever rows remain below it)
     if (t >= c1 && t < w) {
       double lq[16];
 #pragma unroll
-      for (int q = 0; q < 16; q++) lq[q] = (c0 + q < c1) ? A[t * 65 + c0 + q] * dv[c0 + q][0] : 0.0;
+      for (int q = 0; q < 16; q++) lq[q] = A[t * 65 + c0 + q];
       for (int j = c1 + wv; j < w; j += 4) {
         double acc = A[t * 65 + j];
 #pragma unroll
-        for (int q = 0; q < 16; q++) acc = fma(-lq[q], (c0 + q < c1) ? A[j * 65 + c0 + q] : 0.0, acc);
+        for (int q = 0; q < 16; q++) acc = fma(-lq[q], Sb[q * 64 + j], acc);
         A[t * 65 + j] = acc;
       }
     }
@@ -301,7 +375,7 @@ __global__ __launch_bounds__(256) void k_panel_diag(const int* __restrict__ item
   }
   // scale the pivot columns: L(t,c) = a(t,c) / d_c, or [a1 a2] D^-1 for a 2x2 pivot
   // (the wave of a pair's first column writes both columns)
-  for (int c = wv; c < w; c += 4) {
+  for (int c = wv; c < w && !fast; c += 4) {
     const int pc = pt[c];
     if (pc == 1) {
       if (t > c && t < w) A[t * 65 + c] *= dv[c][0];
@@ -316,7 +390,7 @@ __global__ __launch_bounds__(256) void k_panel_diag(const int* __restrict__ item
   }
   // rows j0..j0+w of the earlier panels' columns follow the interchanges (new row
   // t = old row lp[t]); each wave moves its own columns, loading before storing
-  if (j0 > 0) {
+  if (j0 > 0 && !fast) {
     int moved = 0;
     for (int q = 0; q < w; q++) moved |= (lp[q] != q);
     if (uni(moved)) {
@@ -340,59 +414,71 @@ __global__ __launch_bounds__(256) void k_panel_diag(const int* __restrict__ item
     }
   }
   __syncthreads();
-  // X = L11^-1 (unit lower; the slot (c+1,c) of a 2x2 pivot holds D and counts as
-  // 0).  X(i,l), l < i, is stored at A[l*65 + i].
-  const unsigned long long two = __ballot(pt[t] == 2);
-  if (t < 16) {
-    const int b0 = wv * 16, c = t;
-    double x[16];
-#pragma unroll
-    for (int r = 0; r < 16; r++) x[r] = (r == c) ? 1.0 : 0.0;
-#pragma unroll
-    for (int r = 1; r < 16; r++) {
-      double acc = 0.0;
-#pragma unroll
-      for (int l = 0; l < r; l++) {
-        const double lv = (((two >> (b0 + l)) & 1ULL) && r == l + 1) ? 0.0 : A[(b0 + r) * 65 + b0 + l];
-        acc += lv * x[l];
-      }
-      x[r] = (r > c) ? -acc : x[r];
-    }
-#pragma unroll
-    for (int r = 0; r < 16; r++)
-      if (r > c) A[(b0 + c) * 65 + b0 + r] = x[r];
+  // X = L11^-1, unit lower.  The slot (c+1,c) of a 2x2 pivot holds d21 and counts
+  // as 0 in L11: it is zeroed while X is formed and restored afterwards.
+  // X(i,l), l < i, is stored at A[l*65 + i].
+  const bool slot = (wv == 0) && t < w && pt[t] == 2;
+  double d21 = 0.0;
+  if (slot) {
+    d21 = A[(t + 1) * 65 + t];
+    A[(t + 1) * 65 + t] = 0.0;
   }
   __syncthreads();
+  // diagonal 16x16 blocks: wave b inverts block b, lane c < 16 forms column c by
+  // substitution, X(r,c) = -sum_{l<r} L(r,l) X(l,c), with the column kept in Sb
+  // as X(l,c) = 0 (l < c), 1 (l = c), so every lane runs the same loop
+  if (t < 16) {
+    const int b0 = wv * 16, c = t;
+    double* xc = Sb + (b0 + c) * 17;
+#pragma unroll
+    for (int l = 0; l < 16; l++) xc[l] = (l == c) ? 1.0 : 0.0;
+#pragma unroll 1
+    for (int r = 1; r < 16; r++) {
+      const double* lrow = A + (b0 + r) * 65 + b0;
+      double acc = 0.0;
+      for (int l = 0; l < r; l++) acc = fma(lrow[l], xc[l], acc);
+      if (r > c) xc[r] = -acc;
+    }
+#pragma unroll
+    for (int r = 1; r < 16; r++)
+      if (r > c) A[(b0 + c) * 65 + b0 + r] = xc[r];
+  }
+  __syncthreads();
+  // off-diagonal 16x16 blocks with f64 MFMA, block row bi after block row bi-1;
+  // wave bj forms T = sum_{bj<=bl<bi} L(bi,bl) X(bl,bj), then X(bi,bj) = -X(bi,bi) T.
+  // T's accumulator element v is exactly the B operand of the second product's
+  // k-step v, so T stays in registers.
   {
-    const int r = tid & 15, c = tid >> 4;
+    typedef Mfma<double>::v4 v4;
+    const int lr = t & 15, lk = t >> 4;
     for (int bi = 1; bi < 4; bi++) {
-      for (int bj = 0; bj < bi; bj++) {
-        // T(r,c) = sum_{bj<=bl<bi} L(bi r, bl q) X(bl q, bj c)
-        double acc = 0.0;
+      if (wv < bi) {
+        const int bj = wv;
+        v4 acc = (v4)(0.0);
         for (int bl = bj; bl < bi; bl++) {
 #pragma unroll
-          for (int q = 0; q < 16; q++) {
-            const int R = 16 * bi + r, C = 16 * bl + q;
-            const double lv = (((two >> C) & 1ULL) && R == C + 1) ? 0.0 : A[R * 65 + C];
-            double xv = A[(16 * bj + c) * 65 + C];
-            if (bl == bj) xv = (q == c) ? 1.0 : (q > c ? xv : 0.0);
-            acc += lv * xv;
+          for (int kk = 0; kk < 4; kk++) {
+            const int R = 16 * bi + lr, C = 16 * bl + 4 * kk + lk;
+            const int XC = 16 * bj + lr;  // X(C, XC)
+            const double b = (C > XC) ? A[XC * 65 + C] : (C == XC ? 1.0 : 0.0);
+            acc = Mfma<double>::op(A[R * 65 + C], b, acc);
           }
         }
-        Tm[bj][r * 17 + c] = acc;
-      }
-      __syncthreads();
-      for (int bj = 0; bj < bi; bj++) {
-        // X(bi r, bj c) = -sum_{q<=r} X(bi r, bi q) T(q,c)
-        double acc = Tm[bj][r * 17 + c];
+        v4 res = (v4)(0.0);
 #pragma unroll
-        for (int q = 0; q < 16; q++)
-          if (q < r) acc += A[(16 * bi + q) * 65 + 16 * bi + r] * Tm[bj][q * 17 + c];
-        A[(16 * bj + c) * 65 + 16 * bi + r] = -acc;
+        for (int kk = 0; kk < 4; kk++) {
+          const int XR = 16 * bi + lr, XC = 16 * bi + 4 * kk + lk;  // X(XR, XC)
+          const double a = (XR > XC) ? A[XC * 65 + XR] : (XR == XC ? 1.0 : 0.0);
+          res = Mfma<double>::op(a, acc[kk], res);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) A[(16 * bj + lr) * 65 + 16 * bi + Mfma<double>::row(lk, v)] = -res[v];
       }
       __syncthreads();
     }
   }
+  if (slot) A[(t + 1) * 65 + t] = d21;
+  __syncthreads();
   // lower: L and D; upper (row t, column j > t): X(j, t)
   for (int j = wv; j < w; j += 4)
     if (t < w) Fs[(size_t)(j0 + j) * m + j0 + t] = (T)A[t * 65 + j];
@@ -413,33 +499,9 @@ __global__ __launch_bounds__(256) void k_panel_diag(const int* __restrict__ item
     atomicAdd(&ctr->nneg, nneg);
     atomicAdd(&ctr->npert, npert);
     atomicAdd(&ctr->n2x2, n2);
+    if (!fast) atomicAdd(&ctr->npivoted, 1);
   }
 }
-
-// ------------------------------------------------------------------------
-// MFMA 16x16x4 wrappers (f64 / f32).  Lane l supplies A[l%16][l/16] and
-// B[l/16][l%16]; the 16x16 result sits in lane l, element v, at column l%16 and
-// row Mfma<T>::row(l/16, v): 4*(l/16)+v for f32, l/16+4*v for f64 (gfx950).
-// ------------------------------------------------------------------------
-template <typename T>
-struct Mfma;
-template <>
-struct Mfma<double> {
-  typedef double v4 __attribute__((ext_vector_type(4)));
-  static __device__ inline v4 op(double a, double b, v4 c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  // row of accumulator element v held by a lane in row group lk = lane/16
-  static __device__ inline int row(int lk, int v) { return lk + 4 * v; }
-};
-template <>
-struct Mfma<float> {
-  typedef float v4 __attribute__((ext_vector_type(4)));
-  static __device__ inline v4 op(float a, float b, v4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static __device__ inline int row(int lk, int v) { return 4 * lk + v; }
-};
 
 // ------------------------------------------------------------------------
 // panel TRSM as a GEMM with the stored inverse: W = A21 L11^-T, L21 = W D^-1
@@ -1137,7 +1199,7 @@ __global__ void k_fill(long long n, double v, double* __restrict__ x) {
   if (i < n) x[i] = v;
 }
 __global__ void k_zero_counters(FactorCounters* c) {
-  if (threadIdx.x == 0) { c->npos = 0; c->nneg = 0; c->npert = 0; c->n2x2 = 0; }
+  if (threadIdx.x == 0) { c->npos = 0; c->nneg = 0; c->npert = 0; c->n2x2 = 0; c->npivoted = 0; }
 }
 
 // ------------------------------------------------------------------------

@@ -1496,6 +1496,7 @@ int kkt_get_info(kkt_solver* h, kkt_info* out) {
   out->nneg = h->ctr.nneg;
   out->nperturbed = h->ctr.npert;
   out->n2x2 = h->ctr.n2x2;
+  out->pivoted_blocks = h->ctr.npivoted;
   out->precision = h->fprec;
   out->refine_steps = h->last_refine_steps;
   out->fp64_fallbacks = h->fp64_fallbacks;

@@ -268,6 +268,7 @@ def main():
         "step_kinds": "".join("D" if k == "direct" else "i" for k in r32["kinds"]),
         "bicgstab_iters": r32["iters"],
         "factor_gflop": r32["info"]["factor_flops"] / 1e9, "nnzL": r32["info"]["nnzL"],
+        "n2x2": r32["info"]["n2x2"], "pivoted_blocks": r32["info"]["pivoted_blocks"],
         "kernel_ms_fp32": r32["kernel_ms"], "kernel_ms_fp64": r64["kernel_ms"],
         "roofline": roofline, "roofline_factor_update": roofline_update, "cpu_baseline": cpu,
     }
@@ -530,7 +531,7 @@ def precision_run(dims, nsteps, warmup, with_oracle):
                      "fp64_fallbacks": info[-1]["fp64_fallbacks"], "analyze_s": t_an,
                      "npos": info[0]["npos"], "nneg": info[0]["nneg"],
                      "nperturbed": info[0]["nperturbed"], "n2x2": info[0]["n2x2"],
-                     "factor_kernel_ms": kern,
+                     "pivoted_blocks": info[0]["pivoted_blocks"], "factor_kernel_ms": kern,
                      "update_tflops": uflops.value / (kern["panel_update"] * 1e-3) / 1e12
                      if kern["panel_update"] > 0 else None, "x": xs}
         log("%s: %.2f direct solves/s, factor %.1f ms (%.1f TFLOP/s), solve+refine %.1f ms, "

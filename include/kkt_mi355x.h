@@ -67,6 +67,9 @@ typedef struct kkt_info {
   int refine_steps;      /* refinement steps of the last solve */
   int fp64_fallbacks;    /* FP64 refactors forced by stalled refinement */
   double last_rel_residual; /* ||b - A x|| / ||b|| of the last refined solve */
+  int pivoted_blocks;    /* 64-column diagonal blocks that needed Bunch-Kaufman
+                            interchanges or 2x2 pivots (the rest took 1x1 pivots
+                            in order) */
 } kkt_info;
 
 /* CuDssSolver.doInit (CuDssJNI.cc:11-23): creates a handle (NULL on failure). */
