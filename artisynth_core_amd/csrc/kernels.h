@@ -36,6 +36,20 @@ struct DevSym {
   const int* gsrc;
 };
 
+// Work item of the solve kernels: one workgroup's share of a supernode, with the
+// supernode's structure inlined so a kernel issues its data loads right away
+struct alignas(16) SolveItem {
+  int s;     // supernode
+  int blk;   // block of positions / rows / columns of the supernode
+  int lg;    // 2^lg lanes per row or column (rows / columns kernels)
+  int kind;  // forward gather: 1 = gather block of a large front, 0 = whole small front
+  int k, m, first, pad;
+  long long foff;  // front offset (elements)
+  long long uoff;  // update-vector offset
+  long long roff;  // offset of the below-diagonal rows in rows[]
+  long long goff;  // offset of the front positions in the gather map
+};
+
 struct FactorCounters {  // device ints
   int npos, nneg, npert, n2x2;
   int npivoted;  // diagonal blocks that needed the full pivoting loop
@@ -67,17 +81,17 @@ template <typename T>
 void launch_gform(const int4* items, int nitems, const DevSym& d, T* F, hipStream_t st);
 // level-scheduled solves (vectors in the front precision T)
 template <typename T>
-void launch_fwd_gather(const int4* items, int nitems, const DevSym& d, const T* F, const T* bp,
-                       T* fp, T* y, T* u, hipStream_t st);
+void launch_fwd_gather(const SolveItem* items, int nitems, const DevSym& d, const T* F,
+                       const T* bp, T* fp, T* y, T* u, hipStream_t st);
 template <typename T>
-void launch_fwd_rows(const int4* items, int nitems, const DevSym& d, const T* F, const T* fp, T* y,
-                     T* u, hipStream_t st);
+void launch_fwd_rows(const SolveItem* items, int nitems, const DevSym& d, const T* F, const T* fp,
+                     T* y, T* u, hipStream_t st);
 template <typename T>
 void launch_dsolve(int n, const T* E0, const T* E1, const signed char* piv, const T* y, T* z,
                    hipStream_t st);
 template <typename T>
-void launch_bwd_cols(const int4* items, int nitems, const DevSym& d, const T* F, const T* z, T* xp,
-                     hipStream_t st);
+void launch_bwd_cols(const SolveItem* items, int nitems, const DevSym& d, const T* F, const T* z,
+                     T* xp, hipStream_t st);
 
 // permutation / precision conversion: out[j] = (TO) in[perm[j]]  (gather)
 template <typename TI, typename TO>

@@ -856,7 +856,73 @@ __global__ __launch_bounds__(256) void k_gform(const int4* __restrict__ items, D
   }
 }
 
-// forward, first launch of a level.  item = (s, block, kind):
+// 16-byte vectors of T.  Global loads need only element (dword) alignment on
+// this target, so any contiguous run of a front column or vector loads as
+// 4 floats / 2 doubles per instruction.
+template <typename T>
+struct V16;
+template <>
+struct V16<float> {
+  typedef float v __attribute__((ext_vector_type(4)));
+};
+template <>
+struct V16<double> {
+  typedef double v __attribute__((ext_vector_type(2)));
+};
+template <typename T>
+__device__ inline typename V16<T>::v ldv(const T* p) {
+  typename V16<T>::v r;
+  __builtin_memcpy(&r, p, 16);
+  return r;
+}
+template <typename T>
+__device__ inline T vdot(typename V16<T>::v a, typename V16<T>::v b, T acc) {
+#pragma unroll
+  for (int w = 0; w < (int)(16 / sizeof(T)); w++) acc = fma(a[w], b[w], acc);
+  return acc;
+}
+// Partial sum_{e < len} a[e] b[e] of lane sl of a segment of seg lanes.  The lane
+// takes the 16-byte chunks sl, sl + seg, ... in groups of G whose loads all issue
+// before the first use; the last chunk may run up to W-1 elements past len (the
+// callers' arrays are followed by mapped memory), and every element at or past
+// len is dropped by a select, so whatever it holds never reaches the sum.
+template <typename T, int G = 4>
+__device__ inline T seg_dot(const T* __restrict__ a, const T* __restrict__ b, int len, int sl,
+                            int seg) {
+  constexpr int W = 16 / sizeof(T);
+  typedef typename V16<T>::v vt;
+  const int nc = (len + W - 1) / W;
+  T acc = (T)0;
+  for (int q0 = sl; q0 < nc; q0 += G * seg) {
+    vt x[G], y[G];
+#pragma unroll
+    for (int u = 0; u < G; u++) {
+      const int q = q0 + u * seg, qc = q < nc ? q : 0;
+      x[u] = ldv(a + qc * W);
+      y[u] = ldv(b + qc * W);
+    }
+#pragma unroll
+    for (int u = 0; u < G; u++)
+#pragma unroll
+      for (int w = 0; w < W; w++) {
+        const T t = fma(x[u][w], y[u][w], acc);
+        acc = ((q0 + u * seg) * W + w < len) ? t : acc;
+      }
+  }
+  return acc;
+}
+
+template <typename T>
+struct VI16;  // the index vector that goes with V16<T>
+template <>
+struct VI16<float> {
+  typedef int v __attribute__((ext_vector_type(4)));
+};
+template <>
+struct VI16<double> {
+  typedef int v __attribute__((ext_vector_type(2)));
+};
+// forward, first launch of a level.  item = (s, block, kind) with the structure:
 //   kind 1: gather block of a large front, one thread per position p (children
 //           summed in ascending order): f_p = b_p + the children's update entries
 //           at p; fp = f in pivot order (p < k), u_s = the sums on the
@@ -867,42 +933,50 @@ __global__ __launch_bounds__(256) void k_gform(const int4* __restrict__ items, D
 //           upper copy
 static_assert(FWD_SMALL_M == 256, "one thread per row of a small front");
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_gather(const int4* __restrict__ items, DevSym d,
+__global__ __launch_bounds__(256) void k_fwd_gather(const SolveItem* __restrict__ items, DevSym d,
                                                     const T* __restrict__ F,
                                                     const T* __restrict__ bp, T* __restrict__ fp,
                                                     T* __restrict__ y, T* __restrict__ u) {
   __shared__ T g[256];
-  __shared__ T fs[FWD_SMALL_K];
-  const int4 it = items[blockIdx.x];
-  const int s = it.x, p = it.y * 256 + threadIdx.x;
-  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s];
+  __shared__ __align__(16) T fs[FWD_SMALL_K + 4];  // chunk loads may run 3 past k
+  const SolveItem it = items[blockIdx.x];
+  const int k = it.k, m = it.m, first = it.first;
+  const int p = it.blk * 256 + threadIdx.x;
   T acc = (T)0;
+  int ip = 0;
   if (p < m) {
-    const int* gp = d.gptr + d.sn_goff[s] + p;
-    acc = (p < k) ? bp[first + p] : (T)0;
-    for (int e = gp[0]; e < gp[1]; e++) acc += u[d.gsrc[e]];
+    const int* gp = d.gptr + it.goff + p;
+    const int e0 = gp[0], e1 = gp[1];
+    if (p < k) {
+      acc = bp[first + p];
+      ip = d.ilperm[first + p];
+    }
+    // children in ascending order; the index loads of four entries issue together
+    int e = e0;
+    for (; e + 3 < e1; e += 4) {
+      const int i0 = d.gsrc[e], i1 = d.gsrc[e + 1], i2 = d.gsrc[e + 2], i3 = d.gsrc[e + 3];
+      const T v0 = u[i0], v1 = u[i1], v2 = u[i2], v3 = u[i3];
+      acc += v0;
+      acc += v1;
+      acc += v2;
+      acc += v3;
+    }
+    for (; e < e1; e++) acc += u[d.gsrc[e]];
   }
-  if (it.z == 1) {
-    if (p < k) fp[first + d.ilperm[first + p]] = acc;
-    else if (p < m) u[d.sn_uoff[s] + p - k] = acc;
+  if (it.kind == 1) {
+    if (p < k) fp[first + ip] = acc;
+    else if (p < m) u[it.uoff + p - k] = acc;
     return;
   }
   g[threadIdx.x] = acc;
   __syncthreads();
-  if (p < k) fs[d.ilperm[first + p]] = acc;
+  if (p < k) fs[ip] = acc;
   __syncthreads();
   if (p < m) {
-    const T* col = F + d.sn_foff[s] + (size_t)p * m;
-    const int len = min(p, k);
-    T a0 = (T)0, a1 = (T)0;
-    int l = 0;
-    for (; l + 1 < len; l += 2) {
-      a0 += col[l] * fs[l];
-      a1 += col[l + 1] * fs[l + 1];
-    }
-    if (l < len) a0 += col[l] * fs[l];
-    if (p < k) y[first + p] = fs[p] + (a0 + a1);
-    else u[d.sn_uoff[s] + p - k] = g[threadIdx.x] - (a0 + a1);
+    const T* col = F + it.foff + (size_t)p * m;
+    const T acc = seg_dot<T, 8>(col, fs, min(p, k), 0, 1);  // k <= 32: one group
+    if (p < k) y[first + p] = fs[p] + acc;
+    else u[it.uoff + p - k] = g[threadIdx.x] - acc;
   }
 }
 
@@ -918,30 +992,27 @@ __device__ inline T seg_sum(T v, int lg) {
 // A segment of 2^lg lanes per row, 64 >> lg rows per wave;
 // item = (s, block of 4 * (64 >> lg) rows, lg)
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_rows(const int4* __restrict__ items, DevSym d,
+__global__ __launch_bounds__(256) void k_fwd_rows(const SolveItem* __restrict__ items, DevSym d,
                                                   const T* __restrict__ F,
                                                   const T* __restrict__ fp, T* __restrict__ y,
                                                   T* __restrict__ u) {
-  const int4 it = items[blockIdx.x];
-  const int s = it.x, lg = it.z;
+  const SolveItem it = items[blockIdx.x];
+  const int lg = it.lg;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int seg = 1 << lg, sl = lane & (seg - 1);
-  const int i = (it.y * 4 + wv) * (64 >> lg) + (lane >> lg);
-  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s];
-  const T* col = F + d.sn_foff[s] + (size_t)i * m;
+  const int i = (it.blk * 4 + wv) * (64 >> lg) + (lane >> lg);
+  const int k = it.k, m = it.m, first = it.first;
+  const T* col = F + it.foff + (size_t)(i < m ? i : 0) * m;
   const T* f = fp + first;
+  const bool own = sl == 0 && i < m;
+  // the value this row's result combines with, loaded before the dot product
+  T base = (T)0;
+  if (own) base = (i < k) ? f[i] : u[it.uoff + i - k];
   const int len = (i < m) ? min(i, k) : 0;
-  T a0 = (T)0, a1 = (T)0;
-  int l = sl;
-  for (; l + seg < len; l += 2 * seg) {
-    a0 += col[l] * f[l];
-    a1 += col[l + seg] * f[l + seg];
-  }
-  if (l < len) a0 += col[l] * f[l];
-  const T acc = seg_sum(a0 + a1, lg);
-  if (sl == 0 && i < m) {
-    if (i < k) y[first + i] = f[i] + acc;
-    else u[d.sn_uoff[s] + i - k] -= acc;
+  const T acc = seg_sum(seg_dot<T>(col, f, len, sl, seg), lg);
+  if (own) {
+    if (i < k) y[first + i] = base + acc;
+    else u[it.uoff + i - k] = base - acc;
   }
 }
 
@@ -960,38 +1031,75 @@ __global__ __launch_bounds__(256) void k_dsolve(int n, const T* __restrict__ E0,
 
 // backward from the lower copy (column c, rows below the diagonal,
 // contiguous): x_c = z_c + sum_{c<i<k} X(i,c) z_i - sum_j Z(j,c) x[rows_j].
-// A segment of 2^lg lanes per column; item = (s, block of 4 * (64 >> lg) columns, lg)
+// A segment of 2^lg lanes per column (lg <= 8: a long column takes two or four
+// waves); item = (s, block of 256 >> lg columns, lg)
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_cols(const int4* __restrict__ items, DevSym d,
+__global__ __launch_bounds__(256) void k_bwd_cols(const SolveItem* __restrict__ items, DevSym d,
                                                   const T* __restrict__ F,
                                                   const T* __restrict__ z, T* __restrict__ xp) {
-  const int4 it = items[blockIdx.x];
-  const int s = it.x, lg = it.z;
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int seg = 1 << lg, sl = lane & (seg - 1);
-  const int c = (it.y * 4 + wv) * (64 >> lg) + (lane >> lg);
-  const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s];
+  constexpr int W = 16 / sizeof(T);
+  typedef typename V16<T>::v vt;
+  typedef typename VI16<T>::v iv;
+  __shared__ T part[4];
+  const SolveItem it = items[blockIdx.x];
+  const int lg = it.lg;  // 2^lg lanes per column, up to the whole workgroup (lg = 8)
+  const int seg = 1 << lg, sl = threadIdx.x & (seg - 1);
+  const int c = it.blk * (256 >> lg) + (threadIdx.x >> lg);
+  const int k = it.k, m = it.m, first = it.first, r = m - k;
   const bool ok = c < k;
-  const int kx = ok ? k : 0, r = ok ? m - k : 0;
-  const T* col = F + d.sn_foff[s] + (size_t)(ok ? c : 0) * m;
-  const T* zs = z + first;
-  const int* rows = d.rows + d.sn_roff[s];
-  T a0 = (T)0, a1 = (T)0, a2 = (T)0, a3 = (T)0;
-  int i = c + 1 + sl;
-  for (; i + seg < kx; i += 2 * seg) {
-    a0 += col[i] * zs[i];
-    a1 += col[i + seg] * zs[i + seg];
+  const bool own = sl == 0 && ok;
+  int dst = 0;
+  T zc = (T)0;
+  if (own) {
+    dst = d.lperm[first + c];
+    zc = z[first + c];
   }
-  if (i < kx) a0 += col[i] * zs[i];
-  const T* zc = col + k;
-  int j = sl;
-  for (; j + seg < r; j += 2 * seg) {
-    a2 -= zc[j] * xp[rows[j]];
-    a3 -= zc[j + seg] * xp[rows[j + seg]];
+  // pivot-block part X(i,c) z_i (i = c+1..k-1) and contribution part Z(j,c) x[rows_j]
+  // in one pass: every chunk load of a group of four issues at once, then the x
+  // gathers, then the products (the same chunk / select scheme as seg_dot)
+  const int lx = ok ? k - c - 1 : 0, lz = ok ? r : 0;
+  const T* colx = F + it.foff + (size_t)(ok ? c : 0) * m + c + 1;
+  const T* colz = colx + (k - c - 1);
+  const T* zx = z + first + c + 1;
+  const int* rows = d.rows + it.roff;
+  const int ncx = (lx + W - 1) / W, ncz = (lz + W - 1) / W, nc = max(ncx, ncz);
+  T ax = (T)0, az = (T)0;
+  for (int q0 = sl; q0 < nc; q0 += 4 * seg) {
+    vt vx[4], bx[4], vz[4];
+    iv iz[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int q = q0 + u * seg, qx = q < ncx ? q : 0, qz = q < ncz ? q : 0;
+      vx[u] = ldv(colx + qx * W);
+      bx[u] = ldv(zx + qx * W);
+      vz[u] = ldv(colz + qz * W);
+      __builtin_memcpy(&iz[u], rows + qz * W, sizeof(iv));
+    }
+    T gz[4][W];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int w = 0; w < W; w++) gz[u][w] = xp[iz[u][w]];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int w = 0; w < W; w++) {
+        const int e = (q0 + u * seg) * W + w;
+        const T tx = fma(vx[u][w], bx[u][w], ax), tz = fma(vz[u][w], gz[u][w], az);
+        ax = (e < lx) ? tx : ax;
+        az = (e < lz) ? tz : az;
+      }
   }
-  if (j < r) a2 -= zc[j] * xp[rows[j]];
-  const T acc = seg_sum((a0 + a1) + (a2 + a3), lg);
-  if (sl == 0 && ok) xp[first + d.lperm[first + c]] = zs[c] + acc;
+  T acc = seg_sum(ax - az, lg < 6 ? lg : 6);
+  if (lg > 6) {  // columns spanning 2 or 4 waves: combine the wave sums in order
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (own) {
+      acc = part[threadIdx.x >> 6];
+      for (int q = 1; q < (seg >> 6); q++) acc += part[(threadIdx.x >> 6) + q];
+    }
+  }
+  if (own) xp[first + dst] = zc + acc;
 }
 
 // ------------------------------------------------------------------------
@@ -1249,13 +1357,13 @@ void launch_gform(const int4* items, int nitems, const DevSym& d, T* F, hipStrea
   if (nitems > 0) hipLaunchKernelGGL(k_gform<T>, dim3(nitems), dim3(256), 0, st, items, d, F);
 }
 template <typename T>
-void launch_fwd_gather(const int4* items, int nitems, const DevSym& d, const T* F, const T* bp,
+void launch_fwd_gather(const SolveItem* items, int nitems, const DevSym& d, const T* F, const T* bp,
                        T* fp, T* y, T* u, hipStream_t st) {
   if (nitems > 0)
     hipLaunchKernelGGL(k_fwd_gather<T>, dim3(nitems), dim3(256), 0, st, items, d, F, bp, fp, y, u);
 }
 template <typename T>
-void launch_fwd_rows(const int4* items, int nitems, const DevSym& d, const T* F, const T* fp, T* y,
+void launch_fwd_rows(const SolveItem* items, int nitems, const DevSym& d, const T* F, const T* fp, T* y,
                      T* u, hipStream_t st) {
   if (nitems > 0)
     hipLaunchKernelGGL(k_fwd_rows<T>, dim3(nitems), dim3(256), 0, st, items, d, F, fp, y, u);
@@ -1266,7 +1374,7 @@ void launch_dsolve(int n, const T* E0, const T* E1, const signed char* piv, cons
   if (n > 0) hipLaunchKernelGGL(k_dsolve<T>, dim3(nblk(n, 256)), dim3(256), 0, st, n, E0, E1, piv, y, z);
 }
 template <typename T>
-void launch_bwd_cols(const int4* items, int nitems, const DevSym& d, const T* F, const T* z, T* xp,
+void launch_bwd_cols(const SolveItem* items, int nitems, const DevSym& d, const T* F, const T* z, T* xp,
                      hipStream_t st) {
   if (nitems > 0) hipLaunchKernelGGL(k_bwd_cols<T>, dim3(nitems), dim3(256), 0, st, items, d, F, z, xp);
 }
@@ -1359,13 +1467,13 @@ void launch_zero_counters(FactorCounters* c, hipStream_t st) {
   template void launch_panel_xrow<T>(const int2*, int, int, const DevSym&, T*, hipStream_t);     \
   template void launch_zt<T>(const int4*, int, const DevSym&, T*, hipStream_t);                \
   template void launch_gform<T>(const int4*, int, const DevSym&, T*, hipStream_t);             \
-  template void launch_fwd_gather<T>(const int4*, int, const DevSym&, const T*, const T*, T*, T*, \
+  template void launch_fwd_gather<T>(const SolveItem*, int, const DevSym&, const T*, const T*, T*, T*, \
                                      T*, hipStream_t);                                           \
-  template void launch_fwd_rows<T>(const int4*, int, const DevSym&, const T*, const T*, T*, T*,  \
+  template void launch_fwd_rows<T>(const SolveItem*, int, const DevSym&, const T*, const T*, T*, T*,  \
                                    hipStream_t);                                                 \
   template void launch_dsolve<T>(int, const T*, const T*, const signed char*, const T*, T*,      \
                                  hipStream_t);                                                   \
-  template void launch_bwd_cols<T>(const int4*, int, const DevSym&, const T*, const T*, T*,      \
+  template void launch_bwd_cols<T>(const SolveItem*, int, const DevSym&, const T*, const T*, T*,      \
                                    hipStream_t);
 KKT_INST(double)
 KKT_INST(float)

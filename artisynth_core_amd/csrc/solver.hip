@@ -101,10 +101,10 @@ struct kkt_solver {
   int4 *d_asm = nullptr, *d_upd = nullptr;
   int *d_diag = nullptr, *d_aidx = nullptr;
   int2 *d_trsm = nullptr, *d_xrow = nullptr;
-  int4* d_fg = nullptr;
+  SolveItem *d_fg = nullptr, *d_fr = nullptr, *d_bc = nullptr;
   int *d_ilperm = nullptr, *d_gptr = nullptr, *d_gsrc = nullptr;
   long long* d_goff = nullptr;
-  int4 *d_zt = nullptr, *d_gf = nullptr, *d_fr = nullptr, *d_bc = nullptr;
+  int4 *d_zt = nullptr, *d_gf = nullptr;
   std::vector<Launch> xrow_l;  // inverse-row post-pass, one launch per panel index p >= 1
   Launch zt_all, gf_all;       // Z^T and G-form post-passes, one launch each
   long long* d_apos = nullptr;
@@ -524,7 +524,14 @@ struct kkt_solver {
     std::vector<int> diagv, aidx;
     std::vector<long long> apos;
     std::vector<int2> trsmv, xrowv;
-    std::vector<int4> fgv, frv, bcv;
+    std::vector<SolveItem> fgv, frv, bcv;
+    auto item = [&](int s, int blk, int lg, int kind) {
+      SolveItem it;
+      it.s = s; it.blk = blk; it.lg = lg; it.kind = kind;
+      it.k = kk[s]; it.m = mm[s]; it.first = first[s]; it.pad = 0;
+      it.foff = foff[s]; it.uoff = uoff[s]; it.roff = roff[s]; it.goff = goff[s];
+      return it;
+    };
     asm_l.assign(nl, Launch());
     scat_l.assign(nl, Launch());
     fg_l.assign(nl, Launch());
@@ -546,24 +553,28 @@ struct kkt_solver {
       fg_l[L].off = (long long)fgv.size();
       fr_l[L].off = (long long)frv.size();
       bc_l[L].off = (long long)bcv.size();
-      // lanes per row / column: about four loads per lane, a power of two (1..64),
-      // so short dot products pack many rows into a wave
+      // lanes per row / column: about sixteen elements (one group of four 16-byte
+      // chunks in FP32) per lane, a power of two (1..64), so short dot products
+      // pack many rows into a wave
       auto seg_lg = [](int len) {
         int lg = 0;
-        while (lg < 6 && 4 * (1 << lg) < len) lg++;
+        while (lg < 6 && 16 * (1 << lg) < len) lg++;
         return lg;
       };
       for (int s : lv[L]) {
         if (kk[s] <= FWD_SMALL_K && mm[s] <= FWD_SMALL_M) {
           // small front: gather and rows in one workgroup (the first forward launch)
-          fgv.push_back(make_int4(s, 0, 0, 0));
+          fgv.push_back(item(s, 0, 0, 0));
         } else {
-          for (int gb = 0; gb < (mm[s] + 255) / 256; gb++) fgv.push_back(make_int4(s, gb, 1, 0));
+          for (int gb = 0; gb < (mm[s] + 255) / 256; gb++) fgv.push_back(item(s, gb, 0, 1));
           const int lf = seg_lg(kk[s]), rf = 4 * (64 >> lf);
-          for (int rb = 0; rb < (mm[s] + rf - 1) / rf; rb++) frv.push_back(make_int4(s, rb, lf, 0));
+          for (int rb = 0; rb < (mm[s] + rf - 1) / rf; rb++) frv.push_back(item(s, rb, lf, 0));
         }
-        const int lb = seg_lg(mm[s]), rbw = 4 * (64 >> lb);
-        for (int cb = 0; cb < (kk[s] + rbw - 1) / rbw; cb++) bcv.push_back(make_int4(s, cb, lb, 0));
+        // backward columns: up to the whole workgroup (2^8 lanes) on one column
+        int lb = seg_lg(mm[s]);
+        while (lb < 8 && 16 * (1 << lb) < mm[s]) lb++;
+        const int rbw = 256 >> lb;
+        for (int cb = 0; cb < (kk[s] + rbw - 1) / rbw; cb++) bcv.push_back(item(s, cb, lb, 0));
       }
       fg_l[L].cnt = (int)(fgv.size() - fg_l[L].off);
       fr_l[L].cnt = (int)(frv.size() - fr_l[L].off);
@@ -652,14 +663,17 @@ struct kkt_solver {
         for (int q = rp_f[i]; q < rp_f[i + 1]; q++) { col_f[q] = tmp[q - rp_f[i]].first; map_f[q] = tmp[q - rp_f[i]].second; }
       }
     }
-    // uploads
+    // uploads (rows padded: the solve kernels' chunk loads may run 3 entries past
+    // a supernode's rows)
+    std::vector<int> rows_pad(S.rows);
+    rows_pad.resize(S.rows.size() + 4, 0);
     int rc = 0;
     int *d_first, *d_k, *d_m, *d_chp, *d_chl, *d_rel, *d_rows;
     long long *d_foff, *d_roff, *d_woff, *d_uoff;
     if ((rc = upload(d_first, first)) || (rc = upload(d_k, kk)) || (rc = upload(d_m, mm)) ||
         (rc = upload(d_foff, foff)) || (rc = upload(d_roff, roff)) || (rc = upload(d_woff, woff)) ||
         (rc = upload(d_uoff, uoff)) || (rc = upload(d_chp, chp)) || (rc = upload(d_chl, chl)) ||
-        (rc = upload(d_rel, rel)) || (rc = upload(d_rows, S.rows)) || (rc = upload(d_perm, S.perm)) ||
+        (rc = upload(d_rel, rel)) || (rc = upload(d_rows, rows_pad)) || (rc = upload(d_perm, S.perm)) ||
         (rc = upload(d_asm, asmv)) || (rc = upload(d_upd, updv)) || (rc = upload(d_diag, diagv)) ||
         (rc = upload(d_aidx, aidx)) || (rc = upload(d_apos, apos)) || (rc = upload(d_trsm, trsmv)) ||
         (rc = upload(d_xrow, xrowv)) || (rc = upload(d_fg, fgv)) || (rc = upload(d_fr, frv)) ||
@@ -703,13 +717,14 @@ struct kkt_solver {
     size_t esz = (p == KKT_PREC_FP64) ? sizeof(double) : sizeof(float);
     DevOpLock lk(g_capture_mu);
     if (p == KKT_PREC_FP64 && !d_F64) {
-      // fronts, then the two D^-1 vectors (E0, E1) of n entries each
-      if (hipMalloc(&d_F64, (front_elems + 2LL * n + 1) * sizeof(double)) != hipSuccess ||
+      // fronts, then the two D^-1 vectors (E0, E1) of n entries each, then 4 spare
+      // entries (the solve kernels' 16-byte chunk loads may run 3 past a column)
+      if (hipMalloc(&d_F64, (front_elems + 2LL * n + 4) * sizeof(double)) != hipSuccess ||
           hipMalloc(&d_W64, std::max<long long>(w_elems, 1) * sizeof(double)) != hipSuccess)
         return fail(KKT_ERR_ALLOC, "hipMalloc failed (FP64 fronts)");
     }
     if (p == KKT_PREC_FP32_REFINE && !d_F32) {
-      if (hipMalloc(&d_F32, (front_elems + 2LL * n + 1) * sizeof(float)) != hipSuccess ||
+      if (hipMalloc(&d_F32, (front_elems + 2LL * n + 4) * sizeof(float)) != hipSuccess ||
           hipMalloc(&d_W32, std::max<long long>(w_elems, 1) * sizeof(float)) != hipSuccess)
         return fail(KKT_ERR_ALLOC, "hipMalloc failed (FP32 fronts)");
     }
