@@ -57,6 +57,14 @@ SIGNATURES = [
     ("kkt_add_scaled_diagonal3_device_values", _I, [_P, _IP, _DP, _I, _D]),
     ("kkt_add_scaled_block3_device_values", _I, [_P, _IP, _DP, _DP, _I, _D]),
     ("kkt_add_material_stiffness3_device_values", _I, [_P, _IP, _DP, _DP, _DP, _DP, _DP, _I, _D]),
+    ("kkt_add_material_stiffness3_element_device_values", _I,
+     [_P, _IP, _IP, _IP, _IP, _IP, _IP, _IP, _IP, _DP, _DP, _DP, _DP, _DP, _I, _D]),
+    ("kkt_add_linear_elastic_stiffness3_element_device_values", _I,
+     [_P, _IP, _IP, _IP, _IP, _IP, _IP, _DP, _DP, _DP, _I, _D]),
+    ("kkt_add_linear_elastic_stiffness3_element_geometry_device_values", _I,
+     [_P, _IP, _IP, _IP, _IP, _IP, _IP, _IP, _IP, _DP, _DP, _DP, _DP, _DP, _I, _D]),
+    ("kkt_add_dilational_stiffness3_element_device_values", _I,
+     [_P, _IP, _IP, _IP, _IP, _IP, _IP, _IP, _IP, _IP, _DP, _DP, _DP, _I, _D]),
     ("kkt_factor_device_values", _I, [_P]),
     ("kkt_solve", _I, [_P, _DP, _DP]),
     ("kkt_solve_device", _I, [_P, _P, _P]),

@@ -123,6 +123,25 @@ void launch_contrib_block3(int nblocks, const double* bv, const double* bs, doub
 void launch_contrib_diag3(int nblocks, const double* m, double* c, hipStream_t st);
 void launch_contrib_matstiff3(int nblocks, const double* gi, const double* gj, const double* D,
                               const double* sg, const double* dv, double* c, hipStream_t st);
+// element-batched forms: contributions at c[36 p + r * coldim + q] of node pair p
+// (c[9 p + ...] for the linear elastic form without node reduction)
+void launch_contrib_matstiff3_elem(int nelems, const int* nodeCnt, const int* nodeOff, const int* pairOff,
+                                   const int* ipOff, const int* gradOff, const int* pairNodes,
+                                   const int* nodeDims, const double* T, const double* grads,
+                                   const double* Ds, const double* sigmas, const double* dvs, double* c,
+                                   hipStream_t st);
+void launch_contrib_linelastic3_elem(int nelems, const int* nodeCnt, const int* pairOff, const int* ipOff,
+                                     const int* gradOff, const int* pairNodes, const double* params,
+                                     const double* grads, const double* dvs, double* c, hipStream_t st);
+void launch_contrib_linelastic3_geom_elem(int nelems, const int* nodeCnt, const int* nodeOff,
+                                          const int* pairOff, const int* ipOff, const int* ngOff,
+                                          const int* pairNodes, const int* nodeDims, const double* T,
+                                          const double* params, const double* pos, const double* ngrads,
+                                          const double* w, double* c, hipStream_t st);
+void launch_contrib_dilational3_elem(int nelems, const int* nodeOff, const int* presCnt, const int* pairOff,
+                                     const int* consOff, const int* rinvOff, const int* pairNodes,
+                                     const int* nodeDims, const double* T, const double* cons,
+                                     const double* rinv, double* c, hipStream_t st);
 void launch_seg_scatter(int nuniq, const int* useg, const int* uslot, const int* order,
                         const double* c, double scale, double* vals, hipStream_t st);
 

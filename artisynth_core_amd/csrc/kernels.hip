@@ -1121,18 +1121,9 @@ __global__ void k_contrib_diag3(int nblocks, const double* __restrict__ masses, 
 // K = dv (B_i^T D B_j + (g_i^T sigma g_j) I), with B the 6x3 Voigt strain-
 // displacement matrix of gradient g (strain order xx, yy, zz, xy, yz, xz,
 // engineering shears), D 6x6 row-major, sigma = [s00, s11, s22, s01, s12, s02]
-// (FemUtilities.addMaterialStiffness)
-__global__ void k_contrib_matstiff3(int nblocks, const double* __restrict__ gis,
-                                    const double* __restrict__ gjs, const double* __restrict__ Ds,
-                                    const double* __restrict__ sigmas,
-                                    const double* __restrict__ dvs, double* __restrict__ c) {
-  const int b = blockIdx.x * 256 + threadIdx.x;
-  if (b >= nblocks) return;
-  const double* D = Ds + 36 * (size_t)b;
-  const double* sg = sigmas + 6 * (size_t)b;
-  const double gi[3] = {gis[3 * b], gis[3 * b + 1], gis[3 * b + 2]};
-  const double gj[3] = {gjs[3 * b], gjs[3 * b + 1], gjs[3 * b + 2]};
-  const double dv = dvs[b];
+// (FemUtilities.addMaterialStiffness); K row-major 3x3
+__device__ void matstiff3(const double* gi, const double* gj, const double* D, const double* sg,
+                          double dv, double* K) {
   // Voigt rows of B(g) for displacement component k: bv(g, k)[r]
   auto bv = [](const double* g, int k, double* out) {
     for (int r = 0; r < 6; r++) out[r] = 0.0;
@@ -1158,8 +1149,163 @@ __global__ void k_contrib_matstiff3(int nblocks, const double* __restrict__ gis,
         for (int v = 0; v < 6; v++) db += D[6 * u + v] * bj[q][v];
         acc += bi[r][u] * db;
       }
-      c[9 * (size_t)b + 3 * r + q] = dv * (acc + (r == q ? geo : 0.0));
+      K[3 * r + q] = dv * (acc + (r == q ? geo : 0.0));
     }
+}
+__global__ void k_contrib_matstiff3(int nblocks, const double* __restrict__ gis,
+                                    const double* __restrict__ gjs, const double* __restrict__ Ds,
+                                    const double* __restrict__ sigmas,
+                                    const double* __restrict__ dvs, double* __restrict__ c) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= nblocks) return;
+  const double gi[3] = {gis[3 * b], gis[3 * b + 1], gis[3 * b + 2]};
+  const double gj[3] = {gjs[3 * b], gjs[3 * b + 1], gjs[3 * b + 2]};
+  matstiff3(gi, gj, Ds + 36 * (size_t)b, sigmas + 6 * (size_t)b, dvs[b], c + 9 * (size_t)b);
+}
+
+// ---- element-batched forms (cudssBridgeKernels.cu:336-739) ----
+// One wave per element, one lane per node pair (looping); the integration
+// points of a pair are summed in order, then the pair's 3x3 block K is reduced
+// to its target block T_i K T_j^T (T_i row-major dim x 3 in an 18-double slot:
+// I for a free node, the 6x3 attachment block of a frame-attached one) and
+// stored at c[36 p + r * coldim + q].  Contributions are written only where the
+// slot list has entries, which the ordered scatter then adds.
+__device__ void reduce3_store(const double* K, const double* Ti, int rdim, const double* Tj,
+                              int cdim, double* c) {
+  for (int r = 0; r < rdim; r++) {
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;  // (T_i K)(r, :)
+    for (int k = 0; k < 3; k++) {
+      t0 += Ti[3 * r + k] * K[3 * k];
+      t1 += Ti[3 * r + k] * K[3 * k + 1];
+      t2 += Ti[3 * r + k] * K[3 * k + 2];
+    }
+    for (int q = 0; q < cdim; q++) c[r * cdim + q] = t0 * Tj[3 * q] + t1 * Tj[3 * q + 1] + t2 * Tj[3 * q + 2];
+  }
+}
+// D of isotropic linear elasticity from (E, nu), Voigt order with engineering
+// shears (cudssBridgeKernels.cu:451-468)
+__device__ void linear_elastic_D(double E, double nu, double* D) {
+  for (int q = 0; q < 36; q++) D[q] = 0.0;
+  const double a = E / (1.0 + nu);
+  const double dia = (1.0 - nu) / (1.0 - 2.0 * nu) * a, off = nu / (1.0 - 2.0 * nu) * a;
+  for (int r = 0; r < 3; r++)
+    for (int q = 0; q < 3; q++) D[6 * r + q] = (r == q) ? dia : off;
+  D[21] = D[28] = D[35] = 0.5 * a;
+}
+__global__ __launch_bounds__(64) void k_contrib_matstiff3_elem(
+    const int* __restrict__ nodeCnt, const int* __restrict__ nodeOff, const int* __restrict__ pairOff,
+    const int* __restrict__ ipOff, const int* __restrict__ gradOff, const int* __restrict__ pairNodes,
+    const int* __restrict__ nodeDims, const double* __restrict__ T, const double* __restrict__ grads,
+    const double* __restrict__ Ds, const double* __restrict__ sigmas, const double* __restrict__ dvs,
+    double* __restrict__ c) {
+  const int e = blockIdx.x;
+  const int nn = nodeCnt[e], n0 = nodeOff[e], p0 = pairOff[e], np = pairOff[e + 1] - p0;
+  const int i0 = ipOff[e], ni = ipOff[e + 1] - i0, g0 = gradOff[e];
+  for (int pr = threadIdx.x; pr < np; pr += 64) {
+    const int pidx = p0 + pr, i = pairNodes[2 * pidx], j = pairNodes[2 * pidx + 1];
+    double K[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Kq[9];
+    for (int ip = 0; ip < ni; ip++) {
+      const double* gb = grads + 3 * (size_t)(g0 + ip * nn);
+      matstiff3(gb + 3 * i, gb + 3 * j, Ds + 36 * (size_t)(i0 + ip), sigmas + 6 * (size_t)(i0 + ip),
+                dvs[i0 + ip], Kq);
+      for (int q = 0; q < 9; q++) K[q] += Kq[q];
+    }
+    reduce3_store(K, T + 18 * (size_t)(n0 + i), nodeDims[n0 + i], T + 18 * (size_t)(n0 + j),
+                  nodeDims[n0 + j], c + 36 * (size_t)pidx);
+  }
+}
+__global__ __launch_bounds__(64) void k_contrib_linelastic3_elem(
+    const int* __restrict__ nodeCnt, const int* __restrict__ pairOff, const int* __restrict__ ipOff,
+    const int* __restrict__ gradOff, const int* __restrict__ pairNodes,
+    const double* __restrict__ params, const double* __restrict__ grads,
+    const double* __restrict__ dvs, double* __restrict__ c) {
+  const int e = blockIdx.x;
+  const int nn = nodeCnt[e], p0 = pairOff[e], np = pairOff[e + 1] - p0;
+  const int i0 = ipOff[e], ni = ipOff[e + 1] - i0, g0 = gradOff[e];
+  double D[36];
+  linear_elastic_D(params[2 * e], params[2 * e + 1], D);
+  const double sg[6] = {0, 0, 0, 0, 0, 0};
+  for (int pr = threadIdx.x; pr < np; pr += 64) {
+    const int pidx = p0 + pr, i = pairNodes[2 * pidx], j = pairNodes[2 * pidx + 1];
+    double K[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Kq[9];
+    for (int ip = 0; ip < ni; ip++) {
+      const double* gb = grads + 3 * (size_t)(g0 + ip * nn);
+      matstiff3(gb + 3 * i, gb + 3 * j, D, sg, dvs[i0 + ip], Kq);
+      for (int q = 0; q < 9; q++) K[q] += Kq[q];
+    }
+    for (int q = 0; q < 9; q++) c[9 * (size_t)pidx + q] = K[q];
+  }
+}
+// spatial gradients from the Jacobian J = sum_n x_n (dN_n/dxi)^T of each
+// integration point: g = J^-T dN/dxi, dv = det(J) w (cudssBridgeKernels.cu:531-640)
+__global__ __launch_bounds__(64) void k_contrib_linelastic3_geom_elem(
+    const int* __restrict__ nodeCnt, const int* __restrict__ nodeOff, const int* __restrict__ pairOff,
+    const int* __restrict__ ipOff, const int* __restrict__ ngOff, const int* __restrict__ pairNodes,
+    const int* __restrict__ nodeDims, const double* __restrict__ T, const double* __restrict__ params,
+    const double* __restrict__ pos, const double* __restrict__ ngrads, const double* __restrict__ w,
+    double* __restrict__ c) {
+  const int e = blockIdx.x;
+  const int nn = nodeCnt[e], n0 = nodeOff[e], p0 = pairOff[e], np = pairOff[e + 1] - p0;
+  const int i0 = ipOff[e], ni = ipOff[e + 1] - i0, g0 = ngOff[e];
+  double D[36];
+  linear_elastic_D(params[2 * e], params[2 * e + 1], D);
+  const double sg[6] = {0, 0, 0, 0, 0, 0};
+  for (int pr = threadIdx.x; pr < np; pr += 64) {
+    const int pidx = p0 + pr, i = pairNodes[2 * pidx], j = pairNodes[2 * pidx + 1];
+    double K[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Kq[9];
+    for (int ip = 0; ip < ni; ip++) {
+      const double* gb = ngrads + 3 * (size_t)(g0 + ip * nn);
+      double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int n = 0; n < nn; n++) {
+        const double* x = pos + 3 * (size_t)(n0 + n);
+        for (int r = 0; r < 3; r++)
+          for (int q = 0; q < 3; q++) J[3 * r + q] += x[r] * gb[3 * n + q];
+      }
+      // inverse by cofactors
+      const double c00 = J[4] * J[8] - J[5] * J[7], c01 = J[5] * J[6] - J[3] * J[8],
+                   c02 = J[3] * J[7] - J[4] * J[6];
+      const double det = J[0] * c00 + J[1] * c01 + J[2] * c02, id = 1.0 / det;
+      const double Ji[9] = {c00 * id, (J[2] * J[7] - J[1] * J[8]) * id, (J[1] * J[5] - J[2] * J[4]) * id,
+                            c01 * id, (J[0] * J[8] - J[2] * J[6]) * id, (J[2] * J[3] - J[0] * J[5]) * id,
+                            c02 * id, (J[1] * J[6] - J[0] * J[7]) * id, (J[0] * J[4] - J[1] * J[3]) * id};
+      double gi[3], gj[3];
+      for (int q = 0; q < 3; q++) {
+        gi[q] = Ji[q] * gb[3 * i] + Ji[3 + q] * gb[3 * i + 1] + Ji[6 + q] * gb[3 * i + 2];
+        gj[q] = Ji[q] * gb[3 * j] + Ji[3 + q] * gb[3 * j + 1] + Ji[6 + q] * gb[3 * j + 2];
+      }
+      matstiff3(gi, gj, D, sg, det * w[i0 + ip], Kq);
+      for (int q = 0; q < 9; q++) K[q] += Kq[q];
+    }
+    reduce3_store(K, T + 18 * (size_t)(n0 + i), nodeDims[n0 + i], T + 18 * (size_t)(n0 + j),
+                  nodeDims[n0 + j], c + 36 * (size_t)pidx);
+  }
+}
+// K_ij = sum_{a,b} c_{i,a} Rinv(a,b) c_{j,b}^T over the element's np pressure
+// degrees of freedom (c_{i,a}: 3-vector); elements with np <= 0 contribute zeros
+// (cudssBridgeKernels.cu:658-717)
+__global__ __launch_bounds__(64) void k_contrib_dilational3_elem(
+    const int* __restrict__ nodeOff, const int* __restrict__ presCnt, const int* __restrict__ pairOff,
+    const int* __restrict__ consOff, const int* __restrict__ rinvOff, const int* __restrict__ pairNodes,
+    const int* __restrict__ nodeDims, const double* __restrict__ T, const double* __restrict__ cons,
+    const double* __restrict__ rinv, double* __restrict__ c) {
+  const int e = blockIdx.x;
+  const int npr = presCnt[e], n0 = nodeOff[e], p0 = pairOff[e], np = pairOff[e + 1] - p0;
+  const int c0 = consOff[e], r0 = rinvOff[e];
+  for (int pr = threadIdx.x; pr < np; pr += 64) {
+    const int pidx = p0 + pr, i = pairNodes[2 * pidx], j = pairNodes[2 * pidx + 1];
+    double K[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int a = 0; a < npr; a++) {
+      const double* ci = cons + 3 * (size_t)(c0 + i * npr + a);
+      for (int b = 0; b < npr; b++) {
+        const double rv = rinv[r0 + a * npr + b];
+        const double* cj = cons + 3 * (size_t)(c0 + j * npr + b);
+        for (int r = 0; r < 3; r++)
+          for (int q = 0; q < 3; q++) K[3 * r + q] += ci[r] * rv * cj[q];
+      }
+    }
+    reduce3_store(K, T + 18 * (size_t)(n0 + i), nodeDims[n0 + i], T + 18 * (size_t)(n0 + j),
+                  nodeDims[n0 + j], c + 36 * (size_t)pidx);
+  }
 }
 __global__ void k_seg_scatter(int nuniq, const int* __restrict__ useg, const int* __restrict__ uslot,
                               const int* __restrict__ order, const double* __restrict__ c,
@@ -1425,6 +1571,39 @@ void launch_contrib_matstiff3(int nblocks, const double* gi, const double* gj, c
   if (nblocks > 0)
     hipLaunchKernelGGL(k_contrib_matstiff3, dim3(nblk(nblocks, 256)), dim3(256), 0, st, nblocks, gi, gj,
                        D, sg, dv, c);
+}
+void launch_contrib_matstiff3_elem(int nelems, const int* nodeCnt, const int* nodeOff, const int* pairOff,
+                                   const int* ipOff, const int* gradOff, const int* pairNodes,
+                                   const int* nodeDims, const double* T, const double* grads,
+                                   const double* Ds, const double* sigmas, const double* dvs, double* c,
+                                   hipStream_t st) {
+  if (nelems > 0)
+    hipLaunchKernelGGL(k_contrib_matstiff3_elem, dim3(nelems), dim3(64), 0, st, nodeCnt, nodeOff, pairOff,
+                       ipOff, gradOff, pairNodes, nodeDims, T, grads, Ds, sigmas, dvs, c);
+}
+void launch_contrib_linelastic3_elem(int nelems, const int* nodeCnt, const int* pairOff, const int* ipOff,
+                                     const int* gradOff, const int* pairNodes, const double* params,
+                                     const double* grads, const double* dvs, double* c, hipStream_t st) {
+  if (nelems > 0)
+    hipLaunchKernelGGL(k_contrib_linelastic3_elem, dim3(nelems), dim3(64), 0, st, nodeCnt, pairOff, ipOff,
+                       gradOff, pairNodes, params, grads, dvs, c);
+}
+void launch_contrib_linelastic3_geom_elem(int nelems, const int* nodeCnt, const int* nodeOff,
+                                          const int* pairOff, const int* ipOff, const int* ngOff,
+                                          const int* pairNodes, const int* nodeDims, const double* T,
+                                          const double* params, const double* pos, const double* ngrads,
+                                          const double* w, double* c, hipStream_t st) {
+  if (nelems > 0)
+    hipLaunchKernelGGL(k_contrib_linelastic3_geom_elem, dim3(nelems), dim3(64), 0, st, nodeCnt, nodeOff,
+                       pairOff, ipOff, ngOff, pairNodes, nodeDims, T, params, pos, ngrads, w, c);
+}
+void launch_contrib_dilational3_elem(int nelems, const int* nodeOff, const int* presCnt, const int* pairOff,
+                                     const int* consOff, const int* rinvOff, const int* pairNodes,
+                                     const int* nodeDims, const double* T, const double* cons,
+                                     const double* rinv, double* c, hipStream_t st) {
+  if (nelems > 0)
+    hipLaunchKernelGGL(k_contrib_dilational3_elem, dim3(nelems), dim3(64), 0, st, nodeOff, presCnt, pairOff,
+                       consOff, rinvOff, pairNodes, nodeDims, T, cons, rinv, c);
 }
 void launch_seg_scatter(int nuniq, const int* useg, const int* uslot, const int* order,
                         const double* c, double scale, double* vals, hipStream_t st) {

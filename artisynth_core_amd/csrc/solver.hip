@@ -1341,6 +1341,227 @@ int kkt_add_material_stiffness3_device_values(kkt_solver* h, const int* blockSlo
   return rc;
 }
 
+// ---- element-batched forms (CuDssJNI.cc / cudssBridgeKernels.cu:336-739) ----
+// Shared checks of the compact element descriptors: offsets start at 0 and do not
+// decrease, pair node indices lie inside their element, node target dimensions
+// are 1..6, and each element's integration-point / gradient ranges fit.
+namespace {
+bool offsets_ok(const int* off, int nelems) {
+  if (off[0] != 0) return false;
+  for (int e = 0; e < nelems; e++)
+    if (off[e + 1] < off[e]) return false;
+  return true;
+}
+bool pairs_ok(const int* nodeCnt, const int* pairOff, const int* pairNodes, int nelems) {
+  for (int e = 0; e < nelems; e++)
+    for (int p = pairOff[e]; p < pairOff[e + 1]; p++) {
+      const int i = pairNodes[2 * p], j = pairNodes[2 * p + 1];
+      if (i < 0 || j < 0 || i >= nodeCnt[e] || j >= nodeCnt[e]) return false;
+    }
+  return true;
+}
+bool nodes_ok(const int* nodeCnt, const int* nodeOff, const int* nodeDims, int nelems) {
+  for (int e = 0; e < nelems; e++)
+    if (nodeCnt[e] < 0 || nodeOff[e + 1] - nodeOff[e] < nodeCnt[e]) return false;
+  for (int q = 0; q < nodeOff[nelems]; q++)
+    if (nodeDims[q] < 1 || nodeDims[q] > 6) return false;
+  return true;
+}
+// the 36-slot block list with the entries past rowdim x coldim of every pair
+// masked (they are not part of the target block)
+std::vector<int> masked_slots36(const int* nodeOff, const int* pairOff, const int* pairNodes,
+                                const int* blockSlots, const int* nodeDims, int nelems) {
+  std::vector<int> m((size_t)36 * pairOff[nelems], -1);
+  for (int e = 0; e < nelems; e++)
+    for (int p = pairOff[e]; p < pairOff[e + 1]; p++) {
+      const int n = nodeDims[nodeOff[e] + pairNodes[2 * p]] * nodeDims[nodeOff[e] + pairNodes[2 * p + 1]];
+      for (int t = 0; t < n; t++) m[36 * (size_t)p + t] = blockSlots[36 * (size_t)p + t];
+    }
+  return m;
+}
+}  // namespace
+
+int kkt_add_material_stiffness3_element_device_values(
+    kkt_solver* h, const int* elemNodeCounts, const int* elemNodeOffsets, const int* elemPairOffsets,
+    const int* elemIpOffsets, const int* elemGradOffsets, const int* pairNodeIdxs, const int* blockSlots,
+    const int* nodeDims, const double* nodeTransforms, const double* grads, const double* Ds,
+    const double* sigmas, const double* dvs, int nelems, double scale) {
+  CHECK_H(h);
+  if (h->state < kkt_solver::ANALYZED) return h->fail(KKT_ERR_STATE, "analyze() not previously called");
+  if (nelems < 0 || (nelems > 0 && (!elemNodeCounts || !elemNodeOffsets || !elemPairOffsets ||
+                                    !elemIpOffsets || !elemGradOffsets)))
+    return h->fail(KKT_ERR_ARGUMENT, "bad element arrays");
+  if (nelems == 0) { h->ok(); return KKT_OK; }
+  if (!offsets_ok(elemNodeOffsets, nelems) || !offsets_ok(elemPairOffsets, nelems) ||
+      !offsets_ok(elemIpOffsets, nelems) || !offsets_ok(elemGradOffsets, nelems))
+    return h->fail(KKT_ERR_ARGUMENT, "compact material stiffness descriptor arrays are inconsistent");
+  const long long nn = elemNodeOffsets[nelems], np = elemPairOffsets[nelems], ni = elemIpOffsets[nelems],
+                  ng = elemGradOffsets[nelems];
+  if ((np > 0 && (!pairNodeIdxs || !blockSlots)) || (nn > 0 && (!nodeDims || !nodeTransforms)) ||
+      (ni > 0 && (!Ds || !sigmas || !dvs)) || (ng > 0 && !grads))
+    return h->fail(KKT_ERR_ARGUMENT, "bad descriptor arrays");
+  if (!nodes_ok(elemNodeCounts, elemNodeOffsets, nodeDims, nelems) ||
+      !pairs_ok(elemNodeCounts, elemPairOffsets, pairNodeIdxs, nelems))
+    return h->fail(KKT_ERR_ARGUMENT, "compact material stiffness descriptor arrays are inconsistent");
+  for (int e = 0; e < nelems; e++)
+    if (elemGradOffsets[e + 1] - elemGradOffsets[e] <
+        (long long)(elemIpOffsets[e + 1] - elemIpOffsets[e]) * elemNodeCounts[e])
+      return h->fail(KKT_ERR_ARGUMENT, "fewer gradients than integration points x nodes");
+  const std::vector<int> slots =
+      masked_slots36(elemNodeOffsets, elemPairOffsets, pairNodeIdxs, blockSlots, nodeDims, nelems);
+  std::vector<const double*> dp;
+  const size_t I = sizeof(int), D = sizeof(double), ne = (size_t)nelems;
+  int rc = h->device_add(slots.data(), 36 * np, scale, [&](double* c) -> int {
+    int r = h->stage_inputs({{elemNodeCounts, ne * I}, {elemNodeOffsets, (ne + 1) * I},
+                             {elemPairOffsets, (ne + 1) * I}, {elemIpOffsets, (ne + 1) * I},
+                             {elemGradOffsets, (ne + 1) * I}, {pairNodeIdxs, 2 * np * I},
+                             {nodeDims, nn * I}, {nodeTransforms, 18 * nn * D}, {grads, 3 * ng * D},
+                             {Ds, 36 * ni * D}, {sigmas, 6 * ni * D}, {dvs, ni * D}}, dp);
+    if (r) return r;
+    launch_contrib_matstiff3_elem(nelems, (const int*)dp[0], (const int*)dp[1], (const int*)dp[2],
+                                  (const int*)dp[3], (const int*)dp[4], (const int*)dp[5],
+                                  (const int*)dp[6], dp[7], dp[8], dp[9], dp[10], dp[11], c, h->st);
+    return KKT_OK;
+  });
+  if (!rc) h->ok();
+  return rc;
+}
+
+int kkt_add_linear_elastic_stiffness3_element_device_values(
+    kkt_solver* h, const int* elemNodeCounts, const int* elemPairOffsets, const int* elemIpOffsets,
+    const int* elemGradOffsets, const int* pairNodeIdxs, const int* blockSlots, const double* elemParams,
+    const double* grads, const double* dvs, int nelems, double scale) {
+  CHECK_H(h);
+  if (h->state < kkt_solver::ANALYZED) return h->fail(KKT_ERR_STATE, "analyze() not previously called");
+  if (nelems < 0 || (nelems > 0 && (!elemNodeCounts || !elemPairOffsets || !elemIpOffsets ||
+                                    !elemGradOffsets || !elemParams)))
+    return h->fail(KKT_ERR_ARGUMENT, "bad element arrays");
+  if (nelems == 0) { h->ok(); return KKT_OK; }
+  if (!offsets_ok(elemPairOffsets, nelems) || !offsets_ok(elemIpOffsets, nelems) ||
+      !offsets_ok(elemGradOffsets, nelems))
+    return h->fail(KKT_ERR_ARGUMENT, "compact linear elastic descriptor arrays are inconsistent");
+  const long long np = elemPairOffsets[nelems], ni = elemIpOffsets[nelems], ng = elemGradOffsets[nelems];
+  if ((np > 0 && (!pairNodeIdxs || !blockSlots)) || (ni > 0 && !dvs) || (ng > 0 && !grads))
+    return h->fail(KKT_ERR_ARGUMENT, "bad descriptor arrays");
+  for (int e = 0; e < nelems; e++)
+    if (elemNodeCounts[e] < 0 || elemGradOffsets[e + 1] - elemGradOffsets[e] <
+                                     (long long)(elemIpOffsets[e + 1] - elemIpOffsets[e]) * elemNodeCounts[e])
+      return h->fail(KKT_ERR_ARGUMENT, "fewer gradients than integration points x nodes");
+  if (!pairs_ok(elemNodeCounts, elemPairOffsets, pairNodeIdxs, nelems))
+    return h->fail(KKT_ERR_ARGUMENT, "compact linear elastic descriptor arrays are inconsistent");
+  std::vector<const double*> dp;
+  const size_t I = sizeof(int), D = sizeof(double), ne = (size_t)nelems;
+  int rc = h->device_add(blockSlots, 9 * np, scale, [&](double* c) -> int {
+    int r = h->stage_inputs({{elemNodeCounts, ne * I}, {elemPairOffsets, (ne + 1) * I},
+                             {elemIpOffsets, (ne + 1) * I}, {elemGradOffsets, (ne + 1) * I},
+                             {pairNodeIdxs, 2 * np * I}, {elemParams, 2 * ne * D}, {grads, 3 * ng * D},
+                             {dvs, ni * D}}, dp);
+    if (r) return r;
+    launch_contrib_linelastic3_elem(nelems, (const int*)dp[0], (const int*)dp[1], (const int*)dp[2],
+                                    (const int*)dp[3], (const int*)dp[4], dp[5], dp[6], dp[7], c, h->st);
+    return KKT_OK;
+  });
+  if (!rc) h->ok();
+  return rc;
+}
+
+int kkt_add_linear_elastic_stiffness3_element_geometry_device_values(
+    kkt_solver* h, const int* elemNodeCounts, const int* elemNodeOffsets, const int* elemPairOffsets,
+    const int* elemIpOffsets, const int* elemNaturalGradOffsets, const int* pairNodeIdxs,
+    const int* blockSlots, const int* nodeDims, const double* nodeTransforms, const double* elemParams,
+    const double* elemNodePositions, const double* naturalGrads, const double* ipWeights, int nelems,
+    double scale) {
+  CHECK_H(h);
+  if (h->state < kkt_solver::ANALYZED) return h->fail(KKT_ERR_STATE, "analyze() not previously called");
+  if (nelems < 0 || (nelems > 0 && (!elemNodeCounts || !elemNodeOffsets || !elemPairOffsets ||
+                                    !elemIpOffsets || !elemNaturalGradOffsets || !elemParams)))
+    return h->fail(KKT_ERR_ARGUMENT, "bad element arrays");
+  if (nelems == 0) { h->ok(); return KKT_OK; }
+  if (!offsets_ok(elemNodeOffsets, nelems) || !offsets_ok(elemPairOffsets, nelems) ||
+      !offsets_ok(elemIpOffsets, nelems) || !offsets_ok(elemNaturalGradOffsets, nelems))
+    return h->fail(KKT_ERR_ARGUMENT, "compact linear elastic geometry descriptor arrays are inconsistent");
+  const long long nn = elemNodeOffsets[nelems], np = elemPairOffsets[nelems], ni = elemIpOffsets[nelems],
+                  ng = elemNaturalGradOffsets[nelems];
+  if ((np > 0 && (!pairNodeIdxs || !blockSlots)) ||
+      (nn > 0 && (!nodeDims || !nodeTransforms || !elemNodePositions)) || (ni > 0 && !ipWeights) ||
+      (ng > 0 && !naturalGrads))
+    return h->fail(KKT_ERR_ARGUMENT, "bad descriptor arrays");
+  if (!nodes_ok(elemNodeCounts, elemNodeOffsets, nodeDims, nelems) ||
+      !pairs_ok(elemNodeCounts, elemPairOffsets, pairNodeIdxs, nelems))
+    return h->fail(KKT_ERR_ARGUMENT, "compact linear elastic geometry descriptor arrays are inconsistent");
+  for (int e = 0; e < nelems; e++)
+    if (elemNaturalGradOffsets[e + 1] - elemNaturalGradOffsets[e] <
+        (long long)(elemIpOffsets[e + 1] - elemIpOffsets[e]) * elemNodeCounts[e])
+      return h->fail(KKT_ERR_ARGUMENT, "fewer natural gradients than integration points x nodes");
+  const std::vector<int> slots =
+      masked_slots36(elemNodeOffsets, elemPairOffsets, pairNodeIdxs, blockSlots, nodeDims, nelems);
+  std::vector<const double*> dp;
+  const size_t I = sizeof(int), D = sizeof(double), ne = (size_t)nelems;
+  int rc = h->device_add(slots.data(), 36 * np, scale, [&](double* c) -> int {
+    int r = h->stage_inputs({{elemNodeCounts, ne * I}, {elemNodeOffsets, (ne + 1) * I},
+                             {elemPairOffsets, (ne + 1) * I}, {elemIpOffsets, (ne + 1) * I},
+                             {elemNaturalGradOffsets, (ne + 1) * I}, {pairNodeIdxs, 2 * np * I},
+                             {nodeDims, nn * I}, {nodeTransforms, 18 * nn * D}, {elemParams, 2 * ne * D},
+                             {elemNodePositions, 3 * nn * D}, {naturalGrads, 3 * ng * D},
+                             {ipWeights, ni * D}}, dp);
+    if (r) return r;
+    launch_contrib_linelastic3_geom_elem(nelems, (const int*)dp[0], (const int*)dp[1], (const int*)dp[2],
+                                         (const int*)dp[3], (const int*)dp[4], (const int*)dp[5],
+                                         (const int*)dp[6], dp[7], dp[8], dp[9], dp[10], dp[11], c, h->st);
+    return KKT_OK;
+  });
+  if (!rc) h->ok();
+  return rc;
+}
+
+int kkt_add_dilational_stiffness3_element_device_values(
+    kkt_solver* h, const int* elemNodeCounts, const int* elemNodeOffsets, const int* elemPressureCounts,
+    const int* elemPairOffsets, const int* elemConstraintOffsets, const int* elemRinvOffsets,
+    const int* pairNodeIdxs, const int* blockSlots, const int* nodeDims, const double* nodeTransforms,
+    const double* constraints, const double* rinvs, int nelems, double scale) {
+  CHECK_H(h);
+  if (h->state < kkt_solver::ANALYZED) return h->fail(KKT_ERR_STATE, "analyze() not previously called");
+  if (nelems < 0 || (nelems > 0 && (!elemNodeCounts || !elemNodeOffsets || !elemPressureCounts ||
+                                    !elemPairOffsets || !elemConstraintOffsets || !elemRinvOffsets)))
+    return h->fail(KKT_ERR_ARGUMENT, "bad element arrays");
+  if (nelems == 0) { h->ok(); return KKT_OK; }
+  if (!offsets_ok(elemNodeOffsets, nelems) || !offsets_ok(elemPairOffsets, nelems) ||
+      !offsets_ok(elemConstraintOffsets, nelems) || !offsets_ok(elemRinvOffsets, nelems))
+    return h->fail(KKT_ERR_ARGUMENT, "compact dilational descriptor arrays are inconsistent");
+  const long long nn = elemNodeOffsets[nelems], np = elemPairOffsets[nelems],
+                  nc = elemConstraintOffsets[nelems], nr = elemRinvOffsets[nelems];
+  if ((np > 0 && (!pairNodeIdxs || !blockSlots)) || (nn > 0 && (!nodeDims || !nodeTransforms)) ||
+      (nc > 0 && !constraints) || (nr > 0 && !rinvs))
+    return h->fail(KKT_ERR_ARGUMENT, "bad descriptor arrays");
+  if (!nodes_ok(elemNodeCounts, elemNodeOffsets, nodeDims, nelems) ||
+      !pairs_ok(elemNodeCounts, elemPairOffsets, pairNodeIdxs, nelems))
+    return h->fail(KKT_ERR_ARGUMENT, "compact dilational descriptor arrays are inconsistent");
+  for (int e = 0; e < nelems; e++) {
+    const long long p = elemPressureCounts[e];
+    if (p > 0 && (elemConstraintOffsets[e + 1] - elemConstraintOffsets[e] < p * elemNodeCounts[e] ||
+                  elemRinvOffsets[e + 1] - elemRinvOffsets[e] < p * p))
+      return h->fail(KKT_ERR_ARGUMENT, "constraint or Rinv range shorter than the pressure count needs");
+  }
+  const std::vector<int> slots =
+      masked_slots36(elemNodeOffsets, elemPairOffsets, pairNodeIdxs, blockSlots, nodeDims, nelems);
+  std::vector<const double*> dp;
+  const size_t I = sizeof(int), D = sizeof(double), ne = (size_t)nelems;
+  int rc = h->device_add(slots.data(), 36 * np, scale, [&](double* c) -> int {
+    int r = h->stage_inputs({{elemNodeOffsets, (ne + 1) * I}, {elemPressureCounts, ne * I},
+                             {elemPairOffsets, (ne + 1) * I}, {elemConstraintOffsets, (ne + 1) * I},
+                             {elemRinvOffsets, (ne + 1) * I}, {pairNodeIdxs, 2 * np * I},
+                             {nodeDims, nn * I}, {nodeTransforms, 18 * nn * D}, {constraints, 3 * nc * D},
+                             {rinvs, nr * D}}, dp);
+    if (r) return r;
+    launch_contrib_dilational3_elem(nelems, (const int*)dp[0], (const int*)dp[1], (const int*)dp[2],
+                                    (const int*)dp[3], (const int*)dp[4], (const int*)dp[5],
+                                    (const int*)dp[6], dp[7], dp[8], dp[9], c, h->st);
+    return KKT_OK;
+  });
+  if (!rc) h->ok();
+  return rc;
+}
+
 int kkt_factor_device_values(kkt_solver* h) {
   CHECK_H(h);
   if (h->state < kkt_solver::ANALYZED) return h->fail(KKT_ERR_STATE, "analyze() not previously called");

@@ -343,6 +343,94 @@ class MI355XSolver:
             ptr(D, ctypes.c_double), ptr(sg, ctypes.c_double), ptr(dv, ctypes.c_double),
             int(nblocks), float(scale)), "addMaterialStiffness3DeviceValues")
 
+    # element-batched forms (CuDssSolver.java:580-775): the same argument checks
+    # as the reference, then the native call
+    def addMaterialStiffness3ElementDeviceValues(self, elemNodeCounts, elemNodeOffsets,
+                                                 elemPairOffsets, elemIpOffsets, elemGradOffsets,
+                                                 pairNodeIdxs, blockSlots, nodeDims, nodeTransforms,
+                                                 grads, Ds, sigmas, dvs, nelems, scale):
+        self._need_analyze()
+        nc, no, po, io, go, pn, bs, nd = (_i32(a) for a in (
+            elemNodeCounts, elemNodeOffsets, elemPairOffsets, elemIpOffsets, elemGradOffsets,
+            pairNodeIdxs, blockSlots, nodeDims))
+        T, g, D, sg, dv = (_f64(a) for a in (nodeTransforms, grads, Ds, sigmas, dvs))
+        if nelems < 0 or nelems > nc.size or any(nelems + 1 > a.size for a in (no, po, io, go)):
+            raise ValueError("nelems exceeds compact material stiffness element arrays")
+        nn, npr, nip, ng = int(no[nelems]), int(po[nelems]), int(io[nelems]), int(go[nelems])
+        if (min(nn, npr, nip, ng) < 0 or npr > pn.size // 2 or npr > bs.size // 36 or nn > nd.size
+                or nn > T.size // 18 or nip > D.size // 36 or nip > sg.size // 6 or nip > dv.size
+                or ng > g.size // 3):
+            raise ValueError("compact material stiffness descriptor arrays are inconsistent")
+        I, F = ctypes.c_int, ctypes.c_double
+        self._check(self._L.kkt_add_material_stiffness3_element_device_values(
+            self._h, ptr(nc, I), ptr(no, I), ptr(po, I), ptr(io, I), ptr(go, I), ptr(pn, I), ptr(bs, I),
+            ptr(nd, I), ptr(T, F), ptr(g, F), ptr(D, F), ptr(sg, F), ptr(dv, F), int(nelems),
+            float(scale)), "addMaterialStiffness3ElementDeviceValues")
+
+    def addLinearElasticStiffness3ElementDeviceValues(self, elemNodeCounts, elemPairOffsets,
+                                                      elemIpOffsets, elemGradOffsets, pairNodeIdxs,
+                                                      blockSlots, elemParams, grads, dvs, nelems, scale):
+        self._need_analyze()
+        nc, po, io, go, pn, bs = (_i32(a) for a in (
+            elemNodeCounts, elemPairOffsets, elemIpOffsets, elemGradOffsets, pairNodeIdxs, blockSlots))
+        ep, g, dv = (_f64(a) for a in (elemParams, grads, dvs))
+        if nelems < 0 or nelems > nc.size or any(nelems + 1 > a.size for a in (po, io, go)):
+            raise ValueError("nelems exceeds compact linear elastic element arrays")
+        npr, nip, ng = int(po[nelems]), int(io[nelems]), int(go[nelems])
+        if (min(npr, nip, ng) < 0 or npr > pn.size // 2 or npr > bs.size // 9
+                or nelems > ep.size // 2 or nip > dv.size or ng > g.size // 3):
+            raise ValueError("compact linear elastic descriptor arrays are inconsistent")
+        I, F = ctypes.c_int, ctypes.c_double
+        self._check(self._L.kkt_add_linear_elastic_stiffness3_element_device_values(
+            self._h, ptr(nc, I), ptr(po, I), ptr(io, I), ptr(go, I), ptr(pn, I), ptr(bs, I),
+            ptr(ep, F), ptr(g, F), ptr(dv, F), int(nelems), float(scale)),
+            "addLinearElasticStiffness3ElementDeviceValues")
+
+    def addLinearElasticStiffness3ElementGeometryDeviceValues(
+            self, elemNodeCounts, elemNodeOffsets, elemPairOffsets, elemIpOffsets,
+            elemNaturalGradOffsets, pairNodeIdxs, blockSlots, nodeDims, nodeTransforms, elemParams,
+            elemNodePositions, naturalGrads, ipWeights, nelems, scale):
+        self._need_analyze()
+        nc, no, po, io, go, pn, bs, nd = (_i32(a) for a in (
+            elemNodeCounts, elemNodeOffsets, elemPairOffsets, elemIpOffsets, elemNaturalGradOffsets,
+            pairNodeIdxs, blockSlots, nodeDims))
+        T, ep, X, g, w = (_f64(a) for a in (nodeTransforms, elemParams, elemNodePositions,
+                                             naturalGrads, ipWeights))
+        if nelems < 0 or nelems > nc.size or any(nelems + 1 > a.size for a in (no, po, io, go)):
+            raise ValueError("nelems exceeds compact linear elastic geometry arrays")
+        nn, npr, nip, ng = int(no[nelems]), int(po[nelems]), int(io[nelems]), int(go[nelems])
+        if (min(nn, npr, nip, ng) < 0 or npr > pn.size // 2 or npr > bs.size // 36 or nn > nd.size
+                or nn > T.size // 18 or nelems > ep.size // 2 or nn > X.size // 3
+                or ng > g.size // 3 or nip > w.size):
+            raise ValueError("compact linear elastic geometry descriptor arrays are inconsistent")
+        I, F = ctypes.c_int, ctypes.c_double
+        self._check(self._L.kkt_add_linear_elastic_stiffness3_element_geometry_device_values(
+            self._h, ptr(nc, I), ptr(no, I), ptr(po, I), ptr(io, I), ptr(go, I), ptr(pn, I), ptr(bs, I),
+            ptr(nd, I), ptr(T, F), ptr(ep, F), ptr(X, F), ptr(g, F), ptr(w, F), int(nelems),
+            float(scale)), "addLinearElasticStiffness3ElementGeometryDeviceValues")
+
+    def addDilationalStiffness3ElementDeviceValues(
+            self, elemNodeCounts, elemNodeOffsets, elemPressureCounts, elemPairOffsets,
+            elemConstraintOffsets, elemRinvOffsets, pairNodeIdxs, blockSlots, nodeDims,
+            nodeTransforms, constraints, rinvs, nelems, scale):
+        self._need_analyze()
+        nc, no, pc, po, co, ro, pn, bs, nd = (_i32(a) for a in (
+            elemNodeCounts, elemNodeOffsets, elemPressureCounts, elemPairOffsets,
+            elemConstraintOffsets, elemRinvOffsets, pairNodeIdxs, blockSlots, nodeDims))
+        T, C, R = (_f64(a) for a in (nodeTransforms, constraints, rinvs))
+        if (nelems < 0 or nelems > nc.size or nelems > pc.size
+                or any(nelems + 1 > a.size for a in (no, po, co, ro))):
+            raise ValueError("nelems exceeds compact dilational element arrays")
+        nn, npr, ncon, nr = int(no[nelems]), int(po[nelems]), int(co[nelems]), int(ro[nelems])
+        if (min(nn, npr, ncon, nr) < 0 or npr > pn.size // 2 or npr > bs.size // 36 or nn > nd.size
+                or nn > T.size // 18 or ncon > C.size // 3 or nr > R.size):
+            raise ValueError("compact dilational descriptor arrays are inconsistent")
+        I, F = ctypes.c_int, ctypes.c_double
+        self._check(self._L.kkt_add_dilational_stiffness3_element_device_values(
+            self._h, ptr(nc, I), ptr(no, I), ptr(pc, I), ptr(po, I), ptr(co, I), ptr(ro, I),
+            ptr(pn, I), ptr(bs, I), ptr(nd, I), ptr(T, F), ptr(C, F), ptr(R, F), int(nelems),
+            float(scale)), "addDilationalStiffness3ElementDeviceValues")
+
     def factorDeviceValues(self):
         self._need_analyze()
         self._check(self._L.kkt_factor_device_values(self._h), "factorDeviceValues")

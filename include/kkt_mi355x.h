@@ -132,6 +132,53 @@ int kkt_add_material_stiffness3_device_values(kkt_solver* h, const int* blockSlo
                                               const double* gis, const double* gjs,
                                               const double* Ds, const double* sigmas,
                                               const double* dvs, int nblocks, double scale);
+/* Element-batched forms.  Per element e: elemNodeCounts[e] nodes; node pairs
+ * p in [elemPairOffsets[e], elemPairOffsets[e+1]) with element-local node
+ * indices pairNodeIdxs[2p], pairNodeIdxs[2p+1]; integration points in
+ * [elemIpOffsets[e], elemIpOffsets[e+1]), each with elemNodeCounts[e] gradients
+ * starting at the element's gradient offset.  Each element node q (global index
+ * elemNodeOffsets[e] + local) has a target dimension nodeDims[q] (3 free, 6
+ * frame-attached) and an 18-double row-major (dim x 3) transform T_q; the pair
+ * block K_ij (summed over integration points, in order) is added as
+ * T_i K_ij T_j^T at blockSlots[36p + r * dim_j + c] (row-major
+ * dim_i x dim_j; the rest of the 36 are unused).  Negative slots are skipped. */
+/* CuDssSolver.doAddMaterialStiffness3ElementDeviceValues (CuDssJNI.cc:203-311;
+ * cudssBridgeKernels.cu:381-449): K_ij = the material stiffness block above with
+ * the integration point's D (6x6), sigma (6) and dv, and gradients g_i, g_j. */
+int kkt_add_material_stiffness3_element_device_values(
+    kkt_solver* h, const int* elemNodeCounts, const int* elemNodeOffsets, const int* elemPairOffsets,
+    const int* elemIpOffsets, const int* elemGradOffsets, const int* pairNodeIdxs, const int* blockSlots,
+    const int* nodeDims, const double* nodeTransforms, const double* grads, const double* Ds,
+    const double* sigmas, const double* dvs, int nelems, double scale);
+/* CuDssSolver.doAddLinearElasticStiffness3ElementDeviceValues (CuDssJNI.cc:312-391;
+ * cudssBridgeKernels.cu:451-512): isotropic linear elasticity from
+ * elemParams[2e..2e+1] = (E, nu), sigma = 0, no node reduction: 9 slots per
+ * pair at blockSlots[9p + 3r + c]. */
+int kkt_add_linear_elastic_stiffness3_element_device_values(
+    kkt_solver* h, const int* elemNodeCounts, const int* elemPairOffsets, const int* elemIpOffsets,
+    const int* elemGradOffsets, const int* pairNodeIdxs, const int* blockSlots, const double* elemParams,
+    const double* grads, const double* dvs, int nelems, double scale);
+/* CuDssSolver.doAddLinearElasticStiffness3ElementGeometryDeviceValues
+ * (CuDssJNI.cc:392-514; cudssBridgeKernels.cu:531-640): as the linear elastic
+ * form, with each integration point's Jacobian J = sum_n x_n (dN_n/dxi)^T from
+ * elemNodePositions (3 per element node) and naturalGrads, spatial gradients
+ * J^-T dN/dxi and dv = det(J) ipWeights[ip]; node reduction as above. */
+int kkt_add_linear_elastic_stiffness3_element_geometry_device_values(
+    kkt_solver* h, const int* elemNodeCounts, const int* elemNodeOffsets, const int* elemPairOffsets,
+    const int* elemIpOffsets, const int* elemNaturalGradOffsets, const int* pairNodeIdxs,
+    const int* blockSlots, const int* nodeDims, const double* nodeTransforms, const double* elemParams,
+    const double* elemNodePositions, const double* naturalGrads, const double* ipWeights, int nelems,
+    double scale);
+/* CuDssSolver.doAddDilationalStiffness3ElementDeviceValues (CuDssJNI.cc:515-628;
+ * cudssBridgeKernels.cu:658-717): K_ij = sum_{a,b < np} c_{i,a} Rinv(a,b) c_{j,b}^T
+ * with np = elemPressureCounts[e] (elements with np <= 0 add nothing), the
+ * 3-vectors c_{i,a} = constraints[3 (elemConstraintOffsets[e] + i np + a)] and
+ * Rinv row-major at rinvs[elemRinvOffsets[e]]; node reduction as above. */
+int kkt_add_dilational_stiffness3_element_device_values(
+    kkt_solver* h, const int* elemNodeCounts, const int* elemNodeOffsets, const int* elemPressureCounts,
+    const int* elemPairOffsets, const int* elemConstraintOffsets, const int* elemRinvOffsets,
+    const int* pairNodeIdxs, const int* blockSlots, const int* nodeDims, const double* nodeTransforms,
+    const double* constraints, const double* rinvs, int nelems, double scale);
 /* CuDssSolver.doFactorDeviceValues (CuDssJNI.cc:629-652; cudssBridge.h:130):
  * factor the device value buffer (no host copy). */
 int kkt_factor_device_values(kkt_solver* h);

@@ -178,3 +178,125 @@ def scatter_add(vals, slots, contrib, scale):
     keep = slots >= 0
     np.add.at(vals, slots[keep], scale * np.asarray(contrib, float)[keep])
     return vals
+
+
+# ---- element-batched assembly forms (cudssBridgeKernels.cu:336-739) ----
+# Each returns (slots, contrib): one entry per target-block position of every
+# (integration point, node pair), in element / point / pair / row-major order,
+# which scatter_add then applies.  The reference adds them with atomicAdd.
+
+def reduce_block3(K, Ti, rdim, Tj, cdim):
+    """T_i K T_j^T with T row-major (dim x 3) in an 18-entry slot
+    (scatterReduced3Block, cudssBridgeKernels.cu:336-367)."""
+    A = np.asarray(Ti, float)[:3 * rdim].reshape(rdim, 3)
+    B = np.asarray(Tj, float)[:3 * cdim].reshape(cdim, 3)
+    return A @ np.asarray(K, float).reshape(3, 3) @ B.T
+
+
+def linear_elastic_D(E, nu):
+    """fillLinearElasticD (cudssBridgeKernels.cu:451-468)."""
+    a = E / (1.0 + nu)
+    D = np.zeros((6, 6))
+    D[:3, :3] = nu / (1.0 - 2.0 * nu) * a
+    D[0, 0] = D[1, 1] = D[2, 2] = (1.0 - nu) / (1.0 - 2.0 * nu) * a
+    D[3, 3] = D[4, 4] = D[5, 5] = 0.5 * a
+    return D
+
+
+def _emit36(out_s, out_c, blockSlots, pidx, R):
+    rd, cd = R.shape
+    for r in range(rd):
+        for c in range(cd):
+            out_s.append(int(blockSlots[36 * pidx + r * cd + c]))
+            out_c.append(R[r, c])
+
+
+def material_stiffness3_element(elemNodeCounts, elemNodeOffsets, elemPairOffsets, elemIpOffsets,
+                                elemGradOffsets, pairNodeIdxs, blockSlots, nodeDims, nodeTransforms,
+                                grads, Ds, sigmas, dvs, nelems):
+    """addMaterialStiffness3ElementKernel (cudssBridgeKernels.cu:381-449)."""
+    S, C = [], []
+    for e in range(nelems):
+        nn, n0 = elemNodeCounts[e], elemNodeOffsets[e]
+        for ip in range(elemIpOffsets[e], elemIpOffsets[e + 1]):
+            gb = elemGradOffsets[e] + (ip - elemIpOffsets[e]) * nn
+            for p in range(elemPairOffsets[e], elemPairOffsets[e + 1]):
+                i, j = pairNodeIdxs[2 * p], pairNodeIdxs[2 * p + 1]
+                K = material_stiffness3(grads[3 * (gb + i):3 * (gb + i) + 3],
+                                        grads[3 * (gb + j):3 * (gb + j) + 3],
+                                        Ds[36 * ip:36 * ip + 36], sigmas[6 * ip:6 * ip + 6], dvs[ip])
+                R = reduce_block3(K, nodeTransforms[18 * (n0 + i):18 * (n0 + i) + 18], nodeDims[n0 + i],
+                                  nodeTransforms[18 * (n0 + j):18 * (n0 + j) + 18], nodeDims[n0 + j])
+                _emit36(S, C, blockSlots, p, R)
+    return np.array(S, np.int64), np.array(C, float)
+
+
+def linear_elastic_stiffness3_element(elemNodeCounts, elemPairOffsets, elemIpOffsets, elemGradOffsets,
+                                      pairNodeIdxs, blockSlots, elemParams, grads, dvs, nelems):
+    """addLinearElasticStiffness3ElementKernel (cudssBridgeKernels.cu:470-512):
+    9 slots per pair, no node reduction."""
+    S, C = [], []
+    z = np.zeros(6)
+    for e in range(nelems):
+        nn = elemNodeCounts[e]
+        D = linear_elastic_D(elemParams[2 * e], elemParams[2 * e + 1])
+        for ip in range(elemIpOffsets[e], elemIpOffsets[e + 1]):
+            gb = elemGradOffsets[e] + (ip - elemIpOffsets[e]) * nn
+            for p in range(elemPairOffsets[e], elemPairOffsets[e + 1]):
+                i, j = pairNodeIdxs[2 * p], pairNodeIdxs[2 * p + 1]
+                K = material_stiffness3(grads[3 * (gb + i):3 * (gb + i) + 3],
+                                        grads[3 * (gb + j):3 * (gb + j) + 3], D, z, dvs[ip])
+                S.extend(int(s) for s in blockSlots[9 * p:9 * p + 9])
+                C.extend(K.ravel())
+    return np.array(S, np.int64), np.array(C, float)
+
+
+def linear_elastic_stiffness3_element_geometry(elemNodeCounts, elemNodeOffsets, elemPairOffsets,
+                                               elemIpOffsets, elemNaturalGradOffsets, pairNodeIdxs,
+                                               blockSlots, nodeDims, nodeTransforms, elemParams,
+                                               elemNodePositions, naturalGrads, ipWeights, nelems):
+    """addLinearElasticStiffness3ElementGeometryKernel (cudssBridgeKernels.cu:531-640):
+    J = sum_n x_n (dN_n/dxi)^T, g = J^-T dN/dxi, dv = det(J) w."""
+    S, C = [], []
+    z = np.zeros(6)
+    X = np.asarray(elemNodePositions, float).reshape(-1, 3)
+    NG = np.asarray(naturalGrads, float).reshape(-1, 3)
+    for e in range(nelems):
+        nn, n0 = elemNodeCounts[e], elemNodeOffsets[e]
+        D = linear_elastic_D(elemParams[2 * e], elemParams[2 * e + 1])
+        for ip in range(elemIpOffsets[e], elemIpOffsets[e + 1]):
+            gb = elemNaturalGradOffsets[e] + (ip - elemIpOffsets[e]) * nn
+            J = X[n0:n0 + nn].T @ NG[gb:gb + nn]
+            Jit = np.linalg.inv(J).T
+            dv = np.linalg.det(J) * ipWeights[ip]
+            for p in range(elemPairOffsets[e], elemPairOffsets[e + 1]):
+                i, j = pairNodeIdxs[2 * p], pairNodeIdxs[2 * p + 1]
+                K = material_stiffness3(Jit @ NG[gb + i], Jit @ NG[gb + j], D, z, dv)
+                R = reduce_block3(K, nodeTransforms[18 * (n0 + i):18 * (n0 + i) + 18], nodeDims[n0 + i],
+                                  nodeTransforms[18 * (n0 + j):18 * (n0 + j) + 18], nodeDims[n0 + j])
+                _emit36(S, C, blockSlots, p, R)
+    return np.array(S, np.int64), np.array(C, float)
+
+
+def dilational_stiffness3_element(elemNodeCounts, elemNodeOffsets, elemPressureCounts, elemPairOffsets,
+                                  elemConstraintOffsets, elemRinvOffsets, pairNodeIdxs, blockSlots,
+                                  nodeDims, nodeTransforms, constraints, rinvs, nelems):
+    """addDilationalStiffness3ElementKernel (cudssBridgeKernels.cu:658-717):
+    K_ij = sum_ab c_{i,a} Rinv(a,b) c_{j,b}^T; elements with np <= 0 add nothing."""
+    S, C = [], []
+    CV = np.asarray(constraints, float).reshape(-1, 3)
+    for e in range(nelems):
+        npr, n0 = elemPressureCounts[e], elemNodeOffsets[e]
+        if npr <= 0:
+            continue
+        c0, r0 = elemConstraintOffsets[e], elemRinvOffsets[e]
+        Rinv = np.asarray(rinvs[r0:r0 + npr * npr], float).reshape(npr, npr)
+        for p in range(elemPairOffsets[e], elemPairOffsets[e + 1]):
+            i, j = pairNodeIdxs[2 * p], pairNodeIdxs[2 * p + 1]
+            Ci = CV[c0 + i * npr:c0 + i * npr + npr]  # (np, 3)
+            Cj = CV[c0 + j * npr:c0 + j * npr + npr]
+            K = Ci.T @ Rinv @ Cj
+            R = reduce_block3(K, nodeTransforms[18 * (n0 + i):18 * (n0 + i) + 18], nodeDims[n0 + i],
+                              nodeTransforms[18 * (n0 + j):18 * (n0 + j) + 18], nodeDims[n0 + j])
+            _emit36(S, C, blockSlots, p, R)
+    return np.array(S, np.int64), np.array(C, float)
