@@ -66,6 +66,8 @@ SIGNATURES = [
     ("kkt_add_dilational_stiffness3_element_device_values", _I,
      [_P, _IP, _IP, _IP, _IP, _IP, _IP, _IP, _IP, _IP, _DP, _DP, _DP, _I, _D]),
     ("kkt_factor_device_values", _I, [_P]),
+    ("kkt_autotune_solve", _I, [_P, _I]),
+    ("kkt_get_solve_shapes", _I, [_P, _IP, _IP]),
     ("kkt_solve", _I, [_P, _DP, _DP]),
     ("kkt_solve_device", _I, [_P, _P, _P]),
     ("kkt_solve_multi", _I, [_P, _I, _DP, _DP]),

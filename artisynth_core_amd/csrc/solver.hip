@@ -246,6 +246,8 @@ struct kkt_solver {
       h_pin = nullptr;
       for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
       ev_pool.clear();
+      for (hipEvent_t e : lev_ev) (void)hipEventDestroy(e);
+      lev_ev.clear();
       if (st) (void)hipStreamDestroy(st);
     }
     d_rowptr_f = d_col_f = d_map_f = nullptr;
@@ -402,6 +404,74 @@ struct kkt_solver {
     return any;
   }
 
+  // ---- solve work items ----
+  // Per level, the forward-rows and backward-columns kernels give each row or
+  // column 2^lg lanes, the smallest power of two with epl * 2^lg >= its length
+  // (rows: up to 64 lanes; columns: up to 256, several waves).  epl (elements
+  // per lane) is the shape the autotuner picks per level; the gather items do
+  // not depend on it.
+  std::vector<int> sv_k, sv_m, sv_first;
+  std::vector<long long> sv_foff, sv_uoff, sv_roff, sv_goff;
+  std::vector<std::vector<int>> sv_lv;
+  std::vector<int> fr_epl, bc_epl;
+  int replace_upload(SolveItem*& dptr, const std::vector<SolveItem>& v) {
+    // a pointer no longer in allocs was already freed with them (new analysis)
+    if (dptr) {
+      DevOpLock lk(g_capture_mu);
+      auto it = std::find(allocs.begin(), allocs.end(), (void*)dptr);
+      if (it != allocs.end()) {
+        allocs.erase(it);
+        (void)hipFree(dptr);
+      }
+      dptr = nullptr;
+    }
+    return upload(dptr, v);
+  }
+  int build_solve_items() {
+    const int nl = (int)sv_lv.size();
+    std::vector<SolveItem> fgv, frv, bcv;
+    auto item = [&](int s, int blk, int lg, int kind) {
+      SolveItem it;
+      it.s = s; it.blk = blk; it.lg = lg; it.kind = kind;
+      it.k = sv_k[s]; it.m = sv_m[s]; it.first = sv_first[s]; it.pad = 0;
+      it.foff = sv_foff[s]; it.uoff = sv_uoff[s]; it.roff = sv_roff[s]; it.goff = sv_goff[s];
+      return it;
+    };
+    auto seg_lg = [](int len, int epl, int cap) {
+      int lg = 0;
+      while (lg < cap && epl * (1 << lg) < len) lg++;
+      return lg;
+    };
+    fg_l.assign(nl, Launch());
+    fr_l.assign(nl, Launch());
+    bc_l.assign(nl, Launch());
+    for (int L = 0; L < nl; L++) {
+      fg_l[L].off = (long long)fgv.size();
+      fr_l[L].off = (long long)frv.size();
+      bc_l[L].off = (long long)bcv.size();
+      for (int s : sv_lv[L]) {
+        const int k = sv_k[s], m = sv_m[s];
+        if (k <= FWD_SMALL_K && m <= FWD_SMALL_M) {
+          // small front: gather and rows in one workgroup (the first forward launch)
+          fgv.push_back(item(s, 0, 0, 0));
+        } else {
+          for (int gb = 0; gb < (m + 255) / 256; gb++) fgv.push_back(item(s, gb, 0, 1));
+          const int lf = seg_lg(k, fr_epl[L], 6), rf = 4 * (64 >> lf);
+          for (int rb = 0; rb < (m + rf - 1) / rf; rb++) frv.push_back(item(s, rb, lf, 0));
+        }
+        const int lb = seg_lg(m, bc_epl[L], 8), rbw = 256 >> lb;
+        for (int cb = 0; cb < (k + rbw - 1) / rbw; cb++) bcv.push_back(item(s, cb, lb, 0));
+      }
+      fg_l[L].cnt = (int)(fgv.size() - fg_l[L].off);
+      fr_l[L].cnt = (int)(frv.size() - fr_l[L].off);
+      bc_l[L].cnt = (int)(bcv.size() - bc_l[L].off);
+    }
+    int rc;
+    if ((rc = replace_upload(d_fg, fgv)) || (rc = replace_upload(d_fr, frv)) || (rc = replace_upload(d_bc, bcv)))
+      return rc;
+    return KKT_OK;
+  }
+
   // ------------------------------------------------------------------
   int build_plan() {
     (void)hipStreamSynchronize(st);
@@ -524,19 +594,8 @@ struct kkt_solver {
     std::vector<int> diagv, aidx;
     std::vector<long long> apos;
     std::vector<int2> trsmv, xrowv;
-    std::vector<SolveItem> fgv, frv, bcv;
-    auto item = [&](int s, int blk, int lg, int kind) {
-      SolveItem it;
-      it.s = s; it.blk = blk; it.lg = lg; it.kind = kind;
-      it.k = kk[s]; it.m = mm[s]; it.first = first[s]; it.pad = 0;
-      it.foff = foff[s]; it.uoff = uoff[s]; it.roff = roff[s]; it.goff = goff[s];
-      return it;
-    };
     asm_l.assign(nl, Launch());
     scat_l.assign(nl, Launch());
-    fg_l.assign(nl, Launch());
-    fr_l.assign(nl, Launch());
-    bc_l.assign(nl, Launch());
     steps.assign(nl, std::vector<Step>());
     for (int L = 0; L < nl; L++) {
       asm_l[L].off = (long long)asmv.size();
@@ -550,35 +609,6 @@ struct kkt_solver {
       aidx.insert(aidx.end(), a_idx[L].begin(), a_idx[L].end());
       apos.insert(apos.end(), a_pos[L].begin(), a_pos[L].end());
       scat_l[L].cnt = (int)a_idx[L].size();
-      fg_l[L].off = (long long)fgv.size();
-      fr_l[L].off = (long long)frv.size();
-      bc_l[L].off = (long long)bcv.size();
-      // lanes per row / column: about sixteen elements (one group of four 16-byte
-      // chunks in FP32) per lane, a power of two (1..64), so short dot products
-      // pack many rows into a wave
-      auto seg_lg = [](int len) {
-        int lg = 0;
-        while (lg < 6 && 16 * (1 << lg) < len) lg++;
-        return lg;
-      };
-      for (int s : lv[L]) {
-        if (kk[s] <= FWD_SMALL_K && mm[s] <= FWD_SMALL_M) {
-          // small front: gather and rows in one workgroup (the first forward launch)
-          fgv.push_back(item(s, 0, 0, 0));
-        } else {
-          for (int gb = 0; gb < (mm[s] + 255) / 256; gb++) fgv.push_back(item(s, gb, 0, 1));
-          const int lf = seg_lg(kk[s]), rf = 4 * (64 >> lf);
-          for (int rb = 0; rb < (mm[s] + rf - 1) / rf; rb++) frv.push_back(item(s, rb, lf, 0));
-        }
-        // backward columns: up to the whole workgroup (2^8 lanes) on one column
-        int lb = seg_lg(mm[s]);
-        while (lb < 8 && 16 * (1 << lb) < mm[s]) lb++;
-        const int rbw = 256 >> lb;
-        for (int cb = 0; cb < (kk[s] + rbw - 1) / rbw; cb++) bcv.push_back(item(s, cb, lb, 0));
-      }
-      fg_l[L].cnt = (int)(fgv.size() - fg_l[L].off);
-      fr_l[L].cnt = (int)(frv.size() - fr_l[L].off);
-      bc_l[L].cnt = (int)(bcv.size() - bc_l[L].off);
       int P = 0;
       for (int s : lv[L]) P = std::max(P, (kk[s] + NB - 1) / NB);
       steps[L].resize(P);
@@ -676,10 +706,14 @@ struct kkt_solver {
         (rc = upload(d_rel, rel)) || (rc = upload(d_rows, rows_pad)) || (rc = upload(d_perm, S.perm)) ||
         (rc = upload(d_asm, asmv)) || (rc = upload(d_upd, updv)) || (rc = upload(d_diag, diagv)) ||
         (rc = upload(d_aidx, aidx)) || (rc = upload(d_apos, apos)) || (rc = upload(d_trsm, trsmv)) ||
-        (rc = upload(d_xrow, xrowv)) || (rc = upload(d_fg, fgv)) || (rc = upload(d_fr, frv)) ||
-        (rc = upload(d_bc, bcv)) ||
-        (rc = upload(d_zt, ztv)) || (rc = upload(d_gf, gfv)))
+        (rc = upload(d_xrow, xrowv)) || (rc = upload(d_zt, ztv)) || (rc = upload(d_gf, gfv)))
       return rc;
+    // solve work items, default shapes (sixteen elements per lane)
+    sv_k = kk; sv_m = mm; sv_first = first; sv_foff = foff; sv_uoff = uoff; sv_roff = roff;
+    sv_goff = goff; sv_lv = lv;
+    fr_epl.assign(nl, 16);
+    bc_epl.assign(nl, 16);
+    if ((rc = build_solve_items())) return rc;
     std::vector<int> lp0(n);
     for (int s = 0; s < ns; s++)
       for (int t = 0; t < kk[s]; t++) lp0[first[s] + t] = t;
@@ -857,7 +891,9 @@ struct kkt_solver {
       }
       if (fr_l[L].cnt) {
         tbeg(TC_SLEVEL);
+        if (lev_timing) (void)hipEventRecord(lev_ev[4 * L], st);
         launch_fwd_rows<T>(d_fr + fr_l[L].off, fr_l[L].cnt, ds, F, fp, y, u, st);
+        if (lev_timing) (void)hipEventRecord(lev_ev[4 * L + 1], st);
         tend();
       }
     }
@@ -867,12 +903,71 @@ struct kkt_solver {
     for (int L = nl - 1; L >= 0; L--) {
       if (!bc_l[L].cnt) continue;
       tbeg(TC_SGEMV);
+      if (lev_timing) (void)hipEventRecord(lev_ev[4 * L + 2], st);
       launch_bwd_cols<T>(d_bc + bc_l[L].off, bc_l[L].cnt, ds, F, z, xp, st);
+      if (lev_timing) (void)hipEventRecord(lev_ev[4 * L + 3], st);
       tend();
     }
     tbeg(TC_VEC);
     launch_scatter_perm<T, double>(n, d_perm, xp, d_x, st);
     tend();
+  }
+  // ---- solve-shape autotuner ----
+  // Times every level's forward-rows and backward-columns launch (HIP events,
+  // eager launches, best of `reps`) for each candidate elements-per-lane shape,
+  // keeps the fastest shape per level and kernel, rebuilds the work items and
+  // drops the captured graphs (they are captured again on the next solve).
+  bool lev_timing = false;
+  std::vector<hipEvent_t> lev_ev;
+  int autotune_solve(int reps) {
+    if (state < FACTORED) return fail(KKT_ERR_STATE, "factor() not previously called");
+    reps = std::max(1, std::min(reps, 20));
+    const int nl = (int)sv_lv.size();
+    if ((int)lev_ev.size() < 4 * nl) {
+      DevOpLock lk(g_capture_mu);
+      while ((int)lev_ev.size() < 4 * nl) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return fail(KKT_ERR_DEVICE, "hipEventCreate failed");
+        lev_ev.push_back(e);
+      }
+    }
+    static const int cand[] = {4, 8, 16, 32};
+    std::vector<float> bfr(nl, 1e30f), bbc(nl, 1e30f);
+    std::vector<int> cfr(nl, 16), cbc(nl, 16);
+    double* b = d_vec + 12 * (size_t)n;
+    double* x = d_vec + 13 * (size_t)n;
+    launch_fill(n, 1.0, b, st);
+    int rc;
+    for (int c : cand) {
+      fr_epl.assign(nl, c);
+      bc_epl.assign(nl, c);
+      if ((rc = build_solve_items())) return rc;
+      tri_solve_eager(b, x);
+      std::vector<float> tfr(nl, 1e30f), tbc(nl, 1e30f);
+      for (int r = 0; r < reps; r++) {
+        lev_timing = true;
+        tri_solve_eager(b, x);
+        lev_timing = false;
+        if (hipStreamSynchronize(st) != hipSuccess) return fail(KKT_ERR_DEVICE, "autotune solve failed");
+        for (int L = 0; L < nl; L++) {
+          float ms;
+          if (fr_l[L].cnt && hipEventElapsedTime(&ms, lev_ev[4 * L], lev_ev[4 * L + 1]) == hipSuccess)
+            tfr[L] = std::min(tfr[L], ms);
+          if (bc_l[L].cnt && hipEventElapsedTime(&ms, lev_ev[4 * L + 2], lev_ev[4 * L + 3]) == hipSuccess)
+            tbc[L] = std::min(tbc[L], ms);
+        }
+      }
+      for (int L = 0; L < nl; L++) {
+        if (tfr[L] < bfr[L]) { bfr[L] = tfr[L]; cfr[L] = c; }
+        if (tbc[L] < bbc[L]) { bbc[L] = tbc[L]; cbc[L] = c; }
+      }
+    }
+    fr_epl = cfr;
+    bc_epl = cbc;
+    if ((rc = build_solve_items())) return rc;
+    drop_graphs();
+    tflush();
+    return KKT_OK;
   }
   void tri_solve_eager(const double* d_b, double* d_x) {
     if (fprec == KKT_PREC_FP64) run_tri_solve<double>(d_F64, d_b, d_x);
@@ -1778,6 +1873,24 @@ void kkt_timing_reset(kkt_solver* h, int enable) {
   h->tm_upd_flops = 0;
   h->ev_used = 0;
   h->ev_class.clear();
+}
+
+int kkt_autotune_solve(kkt_solver* h, int reps) {
+  CHECK_H(h);
+  int rc = h->autotune_solve(reps);
+  if (!rc) h->ok();
+  return rc;
+}
+
+int kkt_get_solve_shapes(kkt_solver* h, int* fwd_elems, int* bwd_elems) {
+  CHECK_H(h);
+  if (h->state < kkt_solver::ANALYZED) return h->fail(KKT_ERR_STATE, "no analysis");
+  for (size_t L = 0; L < h->fr_epl.size(); L++) {
+    if (fwd_elems) fwd_elems[L] = h->fr_epl[L];
+    if (bwd_elems) bwd_elems[L] = h->bc_epl[L];
+  }
+  h->ok();
+  return KKT_OK;
 }
 
 int kkt_timing_get(kkt_solver* h, double* ms, long long* launches, double* update_flops) {

@@ -431,6 +431,21 @@ class MI355XSolver:
             ptr(pn, I), ptr(bs, I), ptr(nd, I), ptr(T, F), ptr(C, F), ptr(R, F), int(nelems),
             float(scale)), "addDilationalStiffness3ElementDeviceValues")
 
+    def autotuneSolve(self, reps=3):
+        """Picks the solve kernels' per-level lane shapes by timing them on the
+        current factor (kkt_autotune_solve); returns getSolveShapes()."""
+        self._check(self._L.kkt_autotune_solve(self._h, int(reps)), "autotuneSolve")
+        return self.getSolveShapes()
+
+    def getSolveShapes(self):
+        """(forward, backward) elements per lane of every tree level."""
+        nl = self.info()["nlevels"]
+        f = np.zeros(max(nl, 1), np.int32)
+        b = np.zeros(max(nl, 1), np.int32)
+        self._check(self._L.kkt_get_solve_shapes(self._h, ptr(f, ctypes.c_int), ptr(b, ctypes.c_int)),
+                    "getSolveShapes")
+        return f[:nl], b[:nl]
+
     def factorDeviceValues(self):
         self._need_analyze()
         self._check(self._L.kkt_factor_device_values(self._h), "factorDeviceValues")

@@ -174,8 +174,20 @@ def main():
         rc = L.kkt_analyze_values(h, nptr(v0, ctypes.c_double))
         assert rc == 0, L.kkt_last_error(h)
         t_an = time.time() - t0
-        # warmup (untimed), then the timed sequence from a fresh factor
+        # warmup (untimed) and the solve-shape autotuner on the warmup's factor, then
+        # the timed sequence from a fresh factor
         run_sequence(L, h, vals_d, b_d, x_d, args.warmup, len(V))
+        inf0 = KktInfo()
+        L.kkt_get_info(h, ctypes.byref(inf0))
+        t_tune = time.time()
+        rc = L.kkt_autotune_solve(h, 3)
+        assert rc == 0, L.kkt_last_error(h)
+        t_tune = time.time() - t_tune
+        shp_f = np.zeros(inf0.nlevels, np.int32)
+        shp_b = np.zeros(inf0.nlevels, np.int32)
+        L.kkt_get_solve_shapes(h, nptr(shp_f, ctypes.c_int), nptr(shp_b, ctypes.c_int))
+        log("%s: autotuned solve shapes (elements per lane, levels 0..%d) fwd %s bwd %s in %.2f s"
+            % (name, inf0.nlevels - 1, shp_f.tolist(), shp_b.tolist(), t_tune))
         L.kkt_timing_reset(h, 0x0F)  # factor kernel classes; solves stay on their HIP graph
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -208,7 +220,8 @@ def main():
             "kernel_launches": {TIMING_CLASSES[c]: cnt[c] for c in range(8)},
             "update_flops": uflops.value, "info": inf.as_dict(),
             "bwd_ms": bwd_ms, "bwd_launches": bwd_launches, "bwd_sweep_bytes": sweep_bytes,
-            "nlevels": nlev,
+            "nlevels": nlev, "solve_shapes": {"fwd": shp_f.tolist(), "bwd": shp_b.tolist(),
+                                             "tune_s": t_tune},
             "x": [x_d[i].cpu().numpy() for i in range(min(args.steps, len(V)))],
         }
         log("%s: %.2f solves/s, p50 %.2f ms, p99 %.2f ms, kinds %s, max rel resid %.2e" % (
@@ -269,6 +282,7 @@ def main():
         "bicgstab_iters": r32["iters"],
         "factor_gflop": r32["info"]["factor_flops"] / 1e9, "nnzL": r32["info"]["nnzL"],
         "n2x2": r32["info"]["n2x2"], "pivoted_blocks": r32["info"]["pivoted_blocks"],
+        "solve_shapes_fp32": r32["solve_shapes"],
         "kernel_ms_fp32": r32["kernel_ms"], "kernel_ms_fp64": r64["kernel_ms"],
         "roofline": roofline, "roofline_factor_update": roofline_update, "cpu_baseline": cpu,
     }

@@ -239,6 +239,18 @@ int kkt_verify_device_structure(kkt_solver* h);
  * Solves replay a captured HIP graph unless a solve class is timed.
  * ms[c], launches[c]; `update_flops` = algorithmic flops of class-3 launches. */
 void kkt_timing_reset(kkt_solver* h, int enable);
+/* Solve-shape autotuner (no counterpart in the reference; the analysis picks a
+ * default).  The forward-rows and backward-columns kernels give each row or
+ * column of a front 2^lg lanes, the smallest power of two with
+ * epl * 2^lg >= its length.  This times each tree level's launches for
+ * epl = 4, 8, 16, 32 (eager, HIP events, best of reps runs on the current
+ * factor), keeps the fastest per level and kernel, and drops the captured
+ * solve graphs.  Needs a factor.  A shape changes only the lane mapping, so
+ * solutions differ at rounding level (the summation order of a row or column);
+ * they stay bitwise reproducible for a given shape. */
+int kkt_autotune_solve(kkt_solver* h, int reps);
+/* The current per-level shapes (kkt_info.nlevels entries each; NULL skips). */
+int kkt_get_solve_shapes(kkt_solver* h, int* fwd_elems, int* bwd_elems);
 int kkt_timing_get(kkt_solver* h, double* ms, long long* launches, double* update_flops);
 
 /* Host-only symbolic analysis (no GPU needed): structure statistics and the

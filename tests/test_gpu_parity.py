@@ -449,3 +449,49 @@ def test_device_assembled_tet_kkt_solves(gpu, prec):
         assert (inf["npos"], inf["nneg"]) == (body.sizeM, body.numG)
     finally:
         s.dispose()
+
+
+@pytest.mark.parametrize("prec", [KKT_PREC_FP64, KKT_PREC_FP32_REFINE])
+def test_solve_shape_autotuner(gpu, prec):
+    # kkt_autotune_solve re-plans the solve kernels' lane shapes per tree level:
+    # every shape is a candidate, solutions move only at rounding level, a
+    # re-captured graph repeats bitwise, and refinement still reaches its tolerance
+    _say("solve-shape autotuner prec=%d" % prec)
+    beam = G.HexBeam(24, 8, 8, 1.0, 0.2, material=("neohookean", 3.4e4, 1.7e6), h=0.005)
+    u = np.random.default_rng(11).uniform(-1e-4, 1e-4, (beam.nn, 3))
+    vals, b = beam.system(u, np.zeros_like(u), beam.gravity_loads())
+    solvers = []
+    for graphs in (0, 1):
+        s = MI355XSolver(precision=prec)
+        s._L.kkt_set_graphs(s._h, graphs)
+        s.setIterativeRefinementSteps(0)
+        s.setRefinement(1e-10, 10)
+        s.analyzeArrays(vals, beam.colIdxs, beam.rowOffs, beam.n, Matrix.SYMMETRIC)
+        solvers.append(s)
+    try:
+        x0 = []
+        for s in solvers:
+            s.factor(vals)
+            f, bw = s.getSolveShapes()
+            assert len(f) == s.info()["nlevels"] and set(f) == {16} and set(bw) == {16}
+            x = np.zeros(beam.n)
+            s.solve(x, b)
+            x0.append(x)
+        shapes = [s.autotuneSolve(3) for s in solvers]
+        for f, bw in shapes:
+            assert set(f) <= {4, 8, 16, 32} and set(bw) <= {4, 8, 16, 32}
+        # per handle (eager, graphs): the first solve after tuning runs the new
+        # items (the graph handle captures again), the second repeats it bitwise
+        xs = []
+        for s, xa in zip(solvers, x0):
+            x = np.zeros(beam.n)
+            s.solve(x, b)
+            assert rel_residual(beam.rowOffs, beam.colIdxs, vals, x, b) <= 1e-10
+            assert np.abs(x - xa).max() <= 1e-9 * np.abs(xa).max()
+            x2 = np.zeros(beam.n)
+            s.solve(x2, b)
+            assert np.array_equal(x, x2)
+            xs.append(x)
+    finally:
+        for s in solvers:
+            s.dispose()
