@@ -929,8 +929,8 @@ struct VI16<double> {
 //           contribution rows (p >= k)
 //   kind 0: a whole small front (k <= FWD_SMALL_K, m <= FWD_SMALL_M): the same gather into
 //           LDS, then one thread per row: y_i = f_i + G(i,:) f (i < k),
-//           u_{i-k} = f_i - G(i,:) f (i >= k), G(i,:) read from column i of the
-//           upper copy
+//           u_{i-k} = f_i - G(i,:) f (i >= k), G(i,:) read across the columns of
+//           the lower copy (coalesced over the threads)
 static_assert(FWD_SMALL_M == 256, "one thread per row of a small front");
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_gather(const SolveItem* __restrict__ items, DevSym d,
@@ -973,8 +973,20 @@ __global__ __launch_bounds__(256) void k_fwd_gather(const SolveItem* __restrict_
   if (p < k) fs[ip] = acc;
   __syncthreads();
   if (p < m) {
-    const T* col = F + it.foff + (size_t)p * m;
-    const T acc = seg_dot<T, 8>(col, fs, min(p, k), 0, 1);  // k <= 32: one group
+    // row p of G from the lower copy (column l, row p): consecutive threads read
+    // consecutive entries, all FWD_SMALL_K loads issued before the first use (a
+    // column past the row's length re-reads column 0 and is dropped by a select)
+    const int len = min(p, k);
+    const T* rowp = F + it.foff + p;
+    T v[FWD_SMALL_K];
+#pragma unroll
+    for (int l = 0; l < FWD_SMALL_K; l++) v[l] = rowp[(size_t)(l < len ? l : 0) * m];
+    T acc = (T)0;
+#pragma unroll
+    for (int l = 0; l < FWD_SMALL_K; l++) {
+      const T t = fma(v[l], fs[l], acc);
+      acc = (l < len) ? t : acc;
+    }
     if (p < k) y[first + p] = fs[p] + acc;
     else u[it.uoff + p - k] = g[threadIdx.x] - acc;
   }

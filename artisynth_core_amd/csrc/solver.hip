@@ -913,31 +913,45 @@ struct kkt_solver {
     tend();
   }
   // ---- solve-shape autotuner ----
-  // Times every level's forward-rows and backward-columns launch (HIP events,
-  // eager launches, best of `reps`) for each candidate elements-per-lane shape,
-  // keeps the fastest shape per level and kernel, rebuilds the work items and
-  // drops the captured graphs (they are captured again on the next solve).
+  // Candidates: the nine uniform (forward, backward) elements-per-lane pairs from
+  // {8, 16, 32}, and a per-level pick from timing every level's launches eagerly
+  // (HIP events, best of `reps`) for 4, 8, 16, 32.  Each candidate is then timed
+  // the way solves run (its captured graph, or eager launches when graphs are
+  // off) over 4 reps triangular solves; the fastest whole solve wins, so the
+  // default (16, 16) is kept unless another candidate beats it.
   bool lev_timing = false;
   std::vector<hipEvent_t> lev_ev;
+  int time_solves(int nrep, double* b, double* x, float* ms) {
+    tri_solve(b, x);  // captures the graph of the current items
+    hipEvent_t e0 = lev_ev[0], e1 = lev_ev[1];
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < nrep; r++) tri_solve(b, x);
+    (void)hipEventRecord(e1, st);
+    if (hipStreamSynchronize(st) != hipSuccess) return fail(KKT_ERR_DEVICE, "autotune solve failed");
+    if (hipEventElapsedTime(ms, e0, e1) != hipSuccess) return fail(KKT_ERR_DEVICE, "autotune timing failed");
+    *ms /= (float)nrep;
+    return KKT_OK;
+  }
   int autotune_solve(int reps) {
     if (state < FACTORED) return fail(KKT_ERR_STATE, "factor() not previously called");
     reps = std::max(1, std::min(reps, 20));
     const int nl = (int)sv_lv.size();
-    if ((int)lev_ev.size() < 4 * nl) {
+    if ((int)lev_ev.size() < std::max(4 * nl, 2)) {
       DevOpLock lk(g_capture_mu);
-      while ((int)lev_ev.size() < 4 * nl) {
+      while ((int)lev_ev.size() < std::max(4 * nl, 2)) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return fail(KKT_ERR_DEVICE, "hipEventCreate failed");
         lev_ev.push_back(e);
       }
     }
-    static const int cand[] = {4, 8, 16, 32};
-    std::vector<float> bfr(nl, 1e30f), bbc(nl, 1e30f);
-    std::vector<int> cfr(nl, 16), cbc(nl, 16);
     double* b = d_vec + 12 * (size_t)n;
     double* x = d_vec + 13 * (size_t)n;
     launch_fill(n, 1.0, b, st);
     int rc;
+    // per-level pick from eager per-launch timing
+    static const int cand[] = {4, 8, 16, 32};
+    std::vector<float> bfr(nl, 1e30f), bbc(nl, 1e30f);
+    std::vector<int> cfr(nl, 16), cbc(nl, 16);
     for (int c : cand) {
       fr_epl.assign(nl, c);
       bc_epl.assign(nl, c);
@@ -962,8 +976,26 @@ struct kkt_solver {
         if (tbc[L] < bbc[L]) { bbc[L] = tbc[L]; cbc[L] = c; }
       }
     }
-    fr_epl = cfr;
-    bc_epl = cbc;
+    // whole-solve timing of every candidate
+    std::vector<std::pair<std::vector<int>, std::vector<int>>> cands;
+    cands.push_back({std::vector<int>(nl, 16), std::vector<int>(nl, 16)});
+    for (int f : {8, 16, 32})
+      for (int g : {8, 16, 32})
+        if (f != 16 || g != 16) cands.push_back({std::vector<int>(nl, f), std::vector<int>(nl, g)});
+    cands.push_back({cfr, cbc});
+    float best = 1e30f;
+    size_t bi = 0;
+    for (size_t q = 0; q < cands.size(); q++) {
+      fr_epl = cands[q].first;
+      bc_epl = cands[q].second;
+      if ((rc = build_solve_items())) return rc;
+      drop_graphs();
+      float ms;
+      if ((rc = time_solves(4 * reps, b, x, &ms))) return rc;
+      if (ms < best) { best = ms; bi = q; }
+    }
+    fr_epl = cands[bi].first;
+    bc_epl = cands[bi].second;
     if ((rc = build_solve_items())) return rc;
     drop_graphs();
     tflush();

@@ -242,10 +242,12 @@ void kkt_timing_reset(kkt_solver* h, int enable);
 /* Solve-shape autotuner (no counterpart in the reference; the analysis picks a
  * default).  The forward-rows and backward-columns kernels give each row or
  * column of a front 2^lg lanes, the smallest power of two with
- * epl * 2^lg >= its length.  This times each tree level's launches for
- * epl = 4, 8, 16, 32 (eager, HIP events, best of reps runs on the current
- * factor), keeps the fastest per level and kernel, and drops the captured
- * solve graphs.  Needs a factor.  A shape changes only the lane mapping, so
+ * epl * 2^lg >= its length (16 by default).  The candidates are the nine
+ * uniform (forward, backward) pairs from {8, 16, 32} and a per-level pick from
+ * timing each tree level's launches (HIP events, best of reps); every candidate
+ * is timed over 4 reps whole triangular solves as solves run (captured graph
+ * or eager), on the current factor, and the fastest is kept.  The captured
+ * solve graphs are dropped.  Needs a factor.  A shape changes only the lane mapping, so
  * solutions differ at rounding level (the summation order of a row or column);
  * they stay bitwise reproducible for a given shape. */
 int kkt_autotune_solve(kkt_solver* h, int reps);
