@@ -103,9 +103,14 @@ void launch_scatter_perm(int n, const int* perm, const TI* in, TO* out, hipStrea
 // sparse symmetric matrix (full CSR copy) in FP64
 void launch_expand_vals(long long nnzf, const int* map, const double* vals, double* valsf,
                         hipStream_t st);
-// y = A x (mode 0) or y = b - A x (mode 1)
+// y = A x (mode 0) or y = b - A x (mode 1); with part1 / part2, per-block partial
+// sums of a1 .* y and y .* y (spmv_blocks(n) of each) for launch_reduce2
 void launch_spmv(int n, const int* rowptr, const int* col, const double* val, const double* x,
-                 const double* b, double* y, int mode, hipStream_t st);
+                 const double* b, double* y, int mode, hipStream_t st, const double* a1 = nullptr,
+                 double* part1 = nullptr, double* part2 = nullptr);
+int spmv_blocks(int n);
+int vec_blocks(int n);
+void launch_reduce2(const double* p1, const double* p2, int np, double* out1, double* out2, hipStream_t st);
 void launch_absmax(long long nnz, const double* v, double* partial, double* out, hipStream_t st);
 // out[slot] = sum a.*b  (deterministic two-stage)
 void launch_dot(int n, const double* a, const double* b, double* partial, double* out,
@@ -146,9 +151,13 @@ void launch_seg_scatter(int nuniq, const int* useg, const int* uslot, const int*
                         const double* c, double scale, double* vals, hipStream_t st);
 
 void launch_bicg_p(int n, const double* sc, const double* r, const double* v, double* p, hipStream_t st);
-void launch_bicg_s(int n, const double* sc, const double* r, const double* v, double* s_, hipStream_t st);
+// s = r - alpha v, with per-block partials of |s|^2 (vec_blocks(n))
+void launch_bicg_s(int n, const double* sc, const double* r, const double* v, double* s_, double* part,
+                   hipStream_t st);
+// x, r updates, with per-block partials of |r|^2 and (r_hat, r)
 void launch_bicg_xr(int n, const double* sc, const double* y, const double* z, const double* s_,
-                    const double* t, double* x, double* r, hipStream_t st);
+                    const double* t, double* x, double* r, const double* rh, double* part1, double* part2,
+                    hipStream_t st);
 void launch_bicg_xa(int n, const double* sc, const double* y, double* x, hipStream_t st);
 void launch_bicg_scalars(double* sc, int init, hipStream_t st);
 void launch_fill(long long n, double v, double* x, hipStream_t st);

@@ -1352,19 +1352,60 @@ __global__ void k_expand_vals(long long nnzf, const int* __restrict__ map,
 }
 
 // one wave per row
+// y = A x (mode 0) or b - A x (mode 1): 8 lanes per row, 32 rows per workgroup,
+// four entries of a lane in flight.  Optional fused dot products of the result
+// (BiCGStab): part1[block] = sum of a1_i y_i, part2[block] = sum of y_i^2 over the
+// block's rows, in a fixed order.
+constexpr int SPMV_ROWS = 32;
 __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ rowptr,
                                               const int* __restrict__ col,
                                               const double* __restrict__ val,
                                               const double* __restrict__ x,
                                               const double* __restrict__ b,
-                                              double* __restrict__ y, int mode) {
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (row >= n) return;
+                                              double* __restrict__ y, int mode,
+                                              const double* __restrict__ a1,
+                                              double* __restrict__ part1,
+                                              double* __restrict__ part2) {
+  __shared__ double sm[2][4];
+  const int row = blockIdx.x * SPMV_ROWS + (threadIdx.x >> 3);
+  const int sl = threadIdx.x & 7;
   double acc = 0.0;
-  for (int e = rowptr[row] + lane; e < rowptr[row + 1]; e += 64) acc += val[e] * x[col[e]];
-  acc = wave_sum(acc);
-  if (lane == 0) y[row] = mode ? (b[row] - acc) : acc;
+  if (row < n) {
+    const int e1 = rowptr[row + 1];
+    int e = rowptr[row] + sl;
+    double a0 = 0.0, b0 = 0.0;
+    for (; e + 24 < e1; e += 32) {
+      const double v0 = val[e], v1 = val[e + 8], v2 = val[e + 16], v3 = val[e + 24];
+      const int c0 = col[e], c1 = col[e + 8], c2 = col[e + 16], c3 = col[e + 24];
+      a0 = fma(v0, x[c0], a0);
+      b0 = fma(v1, x[c1], b0);
+      a0 = fma(v2, x[c2], a0);
+      b0 = fma(v3, x[c3], b0);
+    }
+    for (; e < e1; e += 8) a0 = fma(val[e], x[col[e]], a0);
+    acc = a0 + b0;
+  }
+  for (int off = 4; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  double d1 = 0.0, d2 = 0.0;
+  if (row < n && sl == 0) {
+    const double yv = mode ? (b[row] - acc) : acc;
+    y[row] = yv;
+    if (a1) d1 = a1[row] * yv;
+    d2 = yv * yv;
+  }
+  if (part1 || part2) {
+    d1 = wave_sum(d1);
+    d2 = wave_sum(d2);
+    if ((threadIdx.x & 63) == 0) {
+      sm[0][threadIdx.x >> 6] = d1;
+      sm[1][threadIdx.x >> 6] = d2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (part1) part1[blockIdx.x] = (sm[0][0] + sm[0][1]) + (sm[0][2] + sm[0][3]);
+      if (part2) part2[blockIdx.x] = (sm[1][0] + sm[1][1]) + (sm[1][2] + sm[1][3]);
+    }
+  }
 }
 
 constexpr int RED_BLOCKS = 512;
@@ -1391,6 +1432,37 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double* __restrict__
   if (threadIdx.x == 0)
     *out = is_max ? fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3])) : (sm[0] + sm[1]) + (sm[2] + sm[3]);
 }
+// out1 = sum of p1[0..np), out2 = sum of p2[0..np) (when given), fixed order
+__global__ __launch_bounds__(256) void k_reduce2(const double* __restrict__ p1,
+                                                 const double* __restrict__ p2, int np,
+                                                 double* __restrict__ out1,
+                                                 double* __restrict__ out2) {
+  __shared__ double sm[2][4];
+  double a = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) {
+    a += p1[i];
+    if (p2) c += p2[i];
+  }
+  a = wave_sum(a);
+  c = wave_sum(c);
+  if ((threadIdx.x & 63) == 0) {
+    sm[0][threadIdx.x >> 6] = a;
+    sm[1][threadIdx.x >> 6] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    *out1 = (sm[0][0] + sm[0][1]) + (sm[0][2] + sm[0][3]);
+    if (out2) *out2 = (sm[1][0] + sm[1][1]) + (sm[1][2] + sm[1][3]);
+  }
+}
+// one block's sum of v over its 256 threads into part[blockIdx.x] (fixed order)
+__device__ inline void block_sum_to(double v, double* part) {
+  __shared__ double sm[4];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+}
 __global__ __launch_bounds__(256) void k_absmax_partial(long long n, const double* __restrict__ v,
                                                         double* __restrict__ partial) {
   __shared__ double sm[4];
@@ -1416,7 +1488,8 @@ __global__ void k_axpbypcz(int n, double a, const double* __restrict__ x, double
 
 // ------------------------------------------------------------------------
 // BiCGStab vector updates with device-resident scalars (no host round trips):
-// slots: 0 rho, 1 rho_prev, 2 alpha, 3 omega, 4 (r_hat,v), 5 |s|^2, 6 (t,s), 7 (t,t)
+// slots: 0 rho, 1 rho_prev, 2 alpha, 3 omega, 4 (r_hat,v), 5 |s|^2, 6 (t,s), 7 (t,t),
+// 8 |r|^2, 10 the next rho (r_hat, r)
 // ------------------------------------------------------------------------
 // p = r + beta (p - omega v), beta = (rho/rho_prev)(alpha/omega)
 __global__ void k_bicg_p(int n, const double* __restrict__ sc, const double* __restrict__ r,
@@ -1426,24 +1499,43 @@ __global__ void k_bicg_p(int n, const double* __restrict__ sc, const double* __r
   const double beta = (sc[0] / sc[1]) * (sc[2] / sc[3]);
   p[i] = r[i] + beta * (p[i] - sc[3] * v[i]);
 }
-// s = r - alpha v with alpha = rho / (r_hat, v)
-__global__ void k_bicg_s(int n, const double* __restrict__ sc, const double* __restrict__ r,
-                         const double* __restrict__ v, double* __restrict__ s) {
+// s = r - alpha v with alpha = rho / (r_hat, v); part[block] = sum of s_i^2
+__global__ __launch_bounds__(256) void k_bicg_s(int n, const double* __restrict__ sc,
+                                                const double* __restrict__ r,
+                                                const double* __restrict__ v, double* __restrict__ s,
+                                                double* __restrict__ part) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const double alpha = sc[0] / sc[4];
-  s[i] = r[i] - alpha * v[i];
+  double d = 0.0;
+  if (i < n) {
+    const double alpha = sc[0] / sc[4];
+    const double si = r[i] - alpha * v[i];
+    s[i] = si;
+    d = si * si;
+  }
+  block_sum_to(d, part);
 }
-// x += alpha y + omega z ; r = s - omega t, omega = (t,s)/(t,t)
-__global__ void k_bicg_xr(int n, const double* __restrict__ sc, const double* __restrict__ y,
-                          const double* __restrict__ z, const double* __restrict__ s,
-                          const double* __restrict__ t, double* __restrict__ x,
-                          double* __restrict__ r) {
+// x += alpha y + omega z ; r = s - omega t, omega = (t,s)/(t,t);
+// part1[block] = sum of r_i^2, part2[block] = sum of r_hat_i r_i (the next rho)
+__global__ __launch_bounds__(256) void k_bicg_xr(int n, const double* __restrict__ sc,
+                                                 const double* __restrict__ y,
+                                                 const double* __restrict__ z,
+                                                 const double* __restrict__ s,
+                                                 const double* __restrict__ t, double* __restrict__ x,
+                                                 double* __restrict__ r, const double* __restrict__ rh,
+                                                 double* __restrict__ part1, double* __restrict__ part2) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const double alpha = sc[0] / sc[4], omega = sc[6] / sc[7];
-  x[i] = x[i] + alpha * y[i] + omega * z[i];
-  r[i] = s[i] - omega * t[i];
+  double d1 = 0.0, d2 = 0.0;
+  if (i < n) {
+    const double alpha = sc[0] / sc[4], omega = sc[6] / sc[7];
+    x[i] = x[i] + alpha * y[i] + omega * z[i];
+    const double ri = s[i] - omega * t[i];
+    r[i] = ri;
+    d1 = ri * ri;
+    d2 = rh[i] * ri;
+  }
+  block_sum_to(d1, part1);
+  __syncthreads();  // block_sum_to's LDS is reused
+  block_sum_to(d2, part2);
 }
 // x += alpha y (early exit on small |s|)
 __global__ void k_bicg_xa(int n, const double* __restrict__ sc, const double* __restrict__ y,
@@ -1452,13 +1544,15 @@ __global__ void k_bicg_xa(int n, const double* __restrict__ sc, const double* __
   if (i >= n) return;
   x[i] += (sc[0] / sc[4]) * y[i];
 }
-// end of an iteration: alpha, omega, rho_prev for the next one; or initial values
+// start of an iteration: alpha, omega and rho_prev of the last one, rho from the
+// last one's fused (r_hat, r); or the initial values (rho = |r0|^2, r_hat = r0)
 __global__ void k_bicg_scalars(double* __restrict__ sc, int init) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (init) { sc[1] = 1.0; sc[2] = 1.0; sc[3] = 1.0; return; }
+  if (init) { sc[0] = sc[8]; sc[1] = 1.0; sc[2] = 1.0; sc[3] = 1.0; return; }
   sc[2] = sc[0] / sc[4];
   sc[3] = sc[6] / sc[7];
   sc[1] = sc[0];
+  sc[0] = sc[10];
 }
 __global__ void k_fill(long long n, double v, double* __restrict__ x) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -1550,8 +1644,14 @@ void launch_expand_vals(long long nnzf, const int* map, const double* vals, doub
     hipLaunchKernelGGL(k_expand_vals, dim3(nblk(nnzf, 256)), dim3(256), 0, st, nnzf, map, vals, valsf);
 }
 void launch_spmv(int n, const int* rowptr, const int* col, const double* val, const double* x,
-                 const double* b, double* y, int mode, hipStream_t st) {
-  hipLaunchKernelGGL(k_spmv, dim3(nblk(n, 4)), dim3(256), 0, st, n, rowptr, col, val, x, b, y, mode);
+                 const double* b, double* y, int mode, hipStream_t st, const double* a1, double* part1,
+                 double* part2) {
+  hipLaunchKernelGGL(k_spmv, dim3(nblk(n, SPMV_ROWS)), dim3(256), 0, st, n, rowptr, col, val, x, b, y, mode,
+                     a1, part1, part2);
+}
+int spmv_blocks(int n) { return (int)nblk(n, SPMV_ROWS); }
+void launch_reduce2(const double* p1, const double* p2, int np, double* out1, double* out2, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, st, p1, p2, np, out1, out2);
 }
 void launch_absmax(long long nnz, const double* v, double* partial, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_absmax_partial, dim3(RED_BLOCKS), dim3(256), 0, st, nnz, v, partial);
@@ -1626,13 +1726,17 @@ void launch_seg_scatter(int nuniq, const int* useg, const int* uslot, const int*
 void launch_bicg_p(int n, const double* sc, const double* r, const double* v, double* p, hipStream_t st) {
   hipLaunchKernelGGL(k_bicg_p, dim3(nblk(n, 256)), dim3(256), 0, st, n, sc, r, v, p);
 }
-void launch_bicg_s(int n, const double* sc, const double* r, const double* v, double* s_, hipStream_t st) {
-  hipLaunchKernelGGL(k_bicg_s, dim3(nblk(n, 256)), dim3(256), 0, st, n, sc, r, v, s_);
+void launch_bicg_s(int n, const double* sc, const double* r, const double* v, double* s_, double* part,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(k_bicg_s, dim3(nblk(n, 256)), dim3(256), 0, st, n, sc, r, v, s_, part);
 }
 void launch_bicg_xr(int n, const double* sc, const double* y, const double* z, const double* s_,
-                    const double* t, double* x, double* r, hipStream_t st) {
-  hipLaunchKernelGGL(k_bicg_xr, dim3(nblk(n, 256)), dim3(256), 0, st, n, sc, y, z, s_, t, x, r);
+                    const double* t, double* x, double* r, const double* rh, double* part1, double* part2,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(k_bicg_xr, dim3(nblk(n, 256)), dim3(256), 0, st, n, sc, y, z, s_, t, x, r, rh, part1,
+                     part2);
 }
+int vec_blocks(int n) { return (int)nblk(n, 256); }
 void launch_bicg_xa(int n, const double* sc, const double* y, double* x, hipStream_t st) {
   hipLaunchKernelGGL(k_bicg_xa, dim3(nblk(n, 256)), dim3(256), 0, st, n, sc, y, x);
 }

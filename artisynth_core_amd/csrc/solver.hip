@@ -122,6 +122,7 @@ struct kkt_solver {
   // FP64 vectors for refinement / BiCGStab / host copies
   double* d_vec = nullptr;  // 14 vectors of n (0-8 BiCGStab, 10-11 host copies, 12-13 graph I/O)
   double* d_partial = nullptr;
+  double* d_part2 = nullptr;  // fused BiCGStab partials: 2 x spmv_blocks(n)
   double* d_scal = nullptr;  // 16 scalars
   double* h_pin = nullptr;   // pinned host copy of the 16 scalars
   // device-side value assembly: scatter plans keyed by the caller's slot list
@@ -240,6 +241,7 @@ struct kkt_solver {
       if (d_valsf) (void)hipFree(d_valsf);
       if (d_vec) (void)hipFree(d_vec);
       if (d_partial) (void)hipFree(d_partial);
+      if (d_part2) (void)hipFree(d_part2);
       if (d_scal) (void)hipFree(d_scal);
       if (d_ctr) (void)hipFree(d_ctr);
       if (h_pin) (void)hipHostFree(h_pin);
@@ -251,7 +253,7 @@ struct kkt_solver {
       if (st) (void)hipStreamDestroy(st);
     }
     d_rowptr_f = d_col_f = d_map_f = nullptr;
-    d_vals = d_valsf = d_vec = d_partial = d_scal = nullptr;
+    d_vals = d_valsf = d_vec = d_partial = d_scal = d_part2 = nullptr;
     d_ctr = nullptr;
     st = nullptr;
     dev_ok = false;
@@ -344,8 +346,9 @@ struct kkt_solver {
       if (d_vals) (void)hipFree(d_vals);
       if (d_valsf) (void)hipFree(d_valsf);
       if (d_vec) (void)hipFree(d_vec);
+      if (d_part2) (void)hipFree(d_part2);
       d_rowptr_f = d_col_f = d_map_f = nullptr;
-      d_vals = d_valsf = d_vec = nullptr;
+      d_vals = d_valsf = d_vec = d_part2 = nullptr;
     }
     n = n_;
     nnz = nnz_;
@@ -481,11 +484,11 @@ struct kkt_solver {
     {
       DevOpLock lk(g_capture_mu);
       for (void* q : {(void*)d_rowptr_f, (void*)d_col_f, (void*)d_map_f, (void*)d_vals, (void*)d_valsf,
-                      (void*)d_vec})
+                      (void*)d_vec, (void*)d_part2})
         if (q) (void)hipFree(q);
     }
     d_rowptr_f = d_col_f = d_map_f = nullptr;
-    d_vals = d_valsf = d_vec = nullptr;
+    d_vals = d_valsf = d_vec = d_part2 = nullptr;
     const int ns = S.nsuper;
     std::vector<int> first(ns), kk(ns), mm(ns);
     std::vector<long long> foff(ns + 1), roff(ns + 1), woff(ns + 1), uoff(ns + 1);
@@ -736,7 +739,8 @@ struct kkt_solver {
         hipMalloc(&d_map_f, map_f.size() * sizeof(int)) != hipSuccess ||
         hipMalloc(&d_vals, std::max(nnz, 1) * sizeof(double)) != hipSuccess ||
         hipMalloc(&d_valsf, col_f.size() * sizeof(double)) != hipSuccess ||
-        hipMalloc(&d_vec, (size_t)14 * n * sizeof(double)) != hipSuccess)
+        hipMalloc(&d_vec, (size_t)14 * n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&d_part2, (size_t)2 * std::max(spmv_blocks(n), 1) * sizeof(double)) != hipSuccess)
       return fail(KKT_ERR_ALLOC, "hipMalloc failed (pattern)");
     if (hipMemcpy(d_rowptr_f, rp_f.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d_col_f, col_f.data(), col_f.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
@@ -1254,30 +1258,37 @@ struct kkt_solver {
       // Each half-iteration ends with its scalars copied to pinned host memory;
       // both halves are captured once into HIP graphs (fixed buffers), so an
       // iteration costs two graph launches and two host synchronizations.
+      // Dot products are fused into the kernels that produce their operands
+      // (per-block partials, then one fixed-order reduction of one or two sums);
+      // rho of the next iteration comes with the x / r update.
+      double *pa = d_part2, *pb = d_part2 + std::max(spmv_blocks(n), 1);
+      const int nbs = spmv_blocks(n), nbv = vec_blocks(n);
       auto head = [&](bool first) {
-        if (!first) launch_bicg_scalars(sc, 0, st);  // alpha, omega, rho_prev of the last iteration
-        dot_to(rh, r, 0);                             // rho
+        if (!first) launch_bicg_scalars(sc, 0, st);  // alpha, omega, rho_prev; rho = next rho
         tbeg(TC_VEC);
         launch_bicg_p(n, sc, r, v, p, st);            // p = r + beta (p - omega v)
         tend();
         tri_solve_eager(p, y);                        // y = M^-1 p
-        spmv(y, nullptr, v, 0);                       // v = A y
-        dot_to(rh, v, 4);                             // (r_hat, v)
-        tbeg(TC_VEC);
-        launch_bicg_s(n, sc, r, v, s, st);            // s = r - alpha v
+        tbeg(TC_SPMV);
+        launch_spmv(n, d_rowptr_f, d_col_f, d_valsf, y, nullptr, v, 0, st, rh, pa, nullptr);  // v = A y
         tend();
-        dot_to(s, s, 5);
+        tbeg(TC_VEC);
+        launch_reduce2(pa, nullptr, nbs, sc + 4, nullptr, st);  // (r_hat, v)
+        launch_bicg_s(n, sc, r, v, s, pa, st);                   // s = r - alpha v
+        launch_reduce2(pa, nullptr, nbv, sc + 5, nullptr, st);  // |s|^2
+        tend();
         (void)hipMemcpyAsync(h_pin, sc, 6 * sizeof(double), hipMemcpyDeviceToHost, st);
       };
       auto tail = [&]() {
         tri_solve_eager(s, z);                        // z = M^-1 s
-        spmv(z, nullptr, t, 0);                       // t = A z
-        dot_to(t, s, 6);
-        dot_to(t, t, 7);
-        tbeg(TC_VEC);
-        launch_bicg_xr(n, sc, y, z, s, t, x, r, st);  // x += alpha y + omega z; r = s - omega t
+        tbeg(TC_SPMV);
+        launch_spmv(n, d_rowptr_f, d_col_f, d_valsf, z, nullptr, t, 0, st, s, pa, pb);  // t = A z
         tend();
-        dot_to(r, r, 8);
+        tbeg(TC_VEC);
+        launch_reduce2(pa, pb, nbs, sc + 6, sc + 7, st);  // (s, t), (t, t)
+        launch_bicg_xr(n, sc, y, z, s, t, x, r, rh, pa, pb, st);  // x += alpha y + omega z; r = s - omega t
+        launch_reduce2(pa, pb, nbv, sc + 8, sc + 10, st);  // |r|^2, (r_hat, r)
+        tend();
         (void)hipMemcpyAsync(h_pin + 6, sc + 6, 3 * sizeof(double), hipMemcpyDeviceToHost, st);
       };
       const int q = (fprec == KKT_PREC_FP64) ? 0 : 1;
