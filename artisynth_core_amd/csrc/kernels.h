@@ -7,6 +7,7 @@ namespace kkt {
 
 constexpr int NB = 64;    // panel width (pivot columns per panel step)
 constexpr int TILE = 64;  // assembly / Schur-update tile edge
+constexpr int UPD_GROUP = 4;  // panels whose Schur updates are applied together (K = 256)
 // fronts with k <= FWD_SMALL_K pivot columns and m <= FWD_SMALL_M rows run their
 // whole forward step (gather + rows) in one workgroup
 constexpr int FWD_SMALL_K = 32;
@@ -63,11 +64,13 @@ void launch_scatter(const int* aidx, const long long* apos, int cnt, const doubl
 template <typename T>
 void launch_panel_diag(const int* items, int nitems, int p, const DevSym& d, T* F, T* E0, T* E1,
                        double pert, const double* amax, FactorCounters* ctr, hipStream_t st);
+// W of panel p goes to the supernode's W-buffer slot `slot` (m * NB each)
 template <typename T>
-void launch_panel_trsm(const int2* items, int nitems, int p, const DevSym& d, T* F, T* W,
+void launch_panel_trsm(const int2* items, int nitems, int p, int slot, const DevSym& d, T* F, T* W,
                        hipStream_t st);
+// C -= sum of W_p L_p^T over the panels pfirst.. of the items (see kernels.hip)
 template <typename T>
-void launch_panel_update(const int4* items, int nitems, int p, const DevSym& d, T* F,
+void launch_panel_update(const int4* items, int nitems, int pfirst, const DevSym& d, T* F,
                          const T* W, hipStream_t st);
 
 template <typename T>

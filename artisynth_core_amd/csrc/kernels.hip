@@ -509,7 +509,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // ------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_trsm(const int2* __restrict__ items, int p,
-                                                    DevSym d, T* __restrict__ F,
+                                                    int slot, DevSym d, T* __restrict__ F,
                                                     T* __restrict__ Wb) {
   constexpr int LD = 65;
   __shared__ T As[64 * LD];  // As[r*LD + l]: row r, permuted panel column l
@@ -535,13 +535,25 @@ __global__ __launch_bounds__(256) void k_panel_trsm(const int2* __restrict__ ite
     }
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-    const int c = e >> 6, r = e & 63;  // column c of the source, row r
-    // Linv[c][r] for r < c lives at front (row j0+r, col j0+c)
-    T v = (T)0;
-    if (c < w && r < w) v = (r < c) ? Fs[(size_t)(j0 + c) * m + j0 + r] : (r == c ? (T)1 : (T)0);
-    Li[c * LD + r] = v;
-    As[r * LD + c] = (r < nr && c < w) ? Fs[(size_t)(j0 + sig[c]) * m + rbase + r] : (T)0;
+  {
+    // this thread's 16 elements (row r, columns c00 + 4q): all loads (addresses
+    // clamped into the block) issue before the LDS stores
+    const int r = threadIdx.x & 63, c00 = threadIdx.x >> 6;
+    const int rl = min(r, w - 1), rn = min(r, nr - 1);
+    T lvv[16], avv[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int cl = min(c00 + 4 * q, w - 1);
+      // Linv[c][r] for r < c lives at front (row j0+r, col j0+c)
+      lvv[q] = Fs[(size_t)(j0 + cl) * m + j0 + rl];
+      avv[q] = Fs[(size_t)(j0 + sig[cl]) * m + rbase + rn];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int c = c00 + 4 * q;
+      Li[c * LD + r] = (c < w && r < w) ? ((r < c) ? lvv[q] : (r == c ? (T)1 : (T)0)) : (T)0;
+      As[r * LD + c] = (r < nr && c < w) ? avv[q] : (T)0;
+    }
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -572,7 +584,7 @@ __global__ __launch_bounds__(256) void k_panel_trsm(const int2* __restrict__ ite
       for (int v = 0; v < 4; v++)
         As[(wr + bi * 16 + Mfma<T>::row(lk, v)) * LD + wc + bj * 16 + lr] = acc[bi][bj][v];
   __syncthreads();
-  T* Ws = Wb + d.sn_woff[s];
+  T* Ws = Wb + d.sn_woff[s] + (size_t)slot * m * NB;
   for (int e = threadIdx.x; e < 64 * 64; e += 256) {
     const int q = e >> 6, r = e & 63;
     if (r >= nr || q >= w) continue;
@@ -594,10 +606,17 @@ __global__ __launch_bounds__(256) void k_panel_trsm(const int2* __restrict__ ite
 }
 
 // ------------------------------------------------------------------------
-// Schur update on 64x64 tiles with MFMA: C -= W L^T
+// Schur update with MFMA, C -= sum over up to UPD_GROUP panels of W_p L_p^T:
+// the panel W = L21 D of each panel (its W-buffer slot) times that panel's
+// columns of L at the tile's column rows.  C is read and written once per group
+// of panels (K = 64 per panel, accumulated in registers).  item =
+// (supernode, row tile I, column tile J, np | rs << 3): the region starts at
+// front row / column rs; J = -1 is the look-ahead form, whose columns are the
+// next panel's pivot columns [rs, rs + min(64, k - rs)).  Only the lower part
+// (row >= column) is stored.
 // ------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel_update(const int4* __restrict__ items, int p,
+__global__ __launch_bounds__(256) void k_panel_update(const int4* __restrict__ items, int pfirst,
                                                       DevSym d, T* __restrict__ F,
                                                       const T* __restrict__ Wb) {
   constexpr int LD = 65;
@@ -605,19 +624,13 @@ __global__ __launch_bounds__(256) void k_panel_update(const int4* __restrict__ i
   __shared__ T Ls[64 * LD];
   typedef typename Mfma<T>::v4 v4;
   const int4 it = items[blockIdx.x];
-  const int s = it.x, I = it.y, J = it.z;
+  const int s = it.x, I = it.y, J = it.z, np = it.w & 7, rs = it.w >> 3;
   const int k = d.sn_k[s], m = d.sn_m[s];
-  const int j0 = p * NB, w = min(NB, k - j0), r0 = j0 + w;
-  const int rI = r0 + I * TILE, rJ = r0 + J * TILE;
-  const int nI = min(TILE, m - rI), nJ = min(TILE, m - rJ);
+  const bool look = J < 0;
+  const int rI = rs + I * TILE, rJ = look ? rs : rs + J * TILE;
+  const int nI = min(TILE, m - rI), nJ = look ? min(TILE, k - rs) : min(TILE, m - rJ);
   const T* Wsrc = Wb + d.sn_woff[s];
   T* Fs = F + d.sn_foff[s];
-  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-    const int t = e >> 6, r = e & 63;
-    Ws[r * LD + t] = (r < nI && t < w) ? Wsrc[(size_t)t * m + rI + r] : (T)0;
-    Ls[r * LD + t] = (r < nJ && t < w) ? Fs[(size_t)(j0 + t) * m + rJ + r] : (T)0;
-  }
-  __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
@@ -631,16 +644,40 @@ __global__ __launch_bounds__(256) void k_panel_update(const int4* __restrict__ i
         const int row = wr + bi * 16 + Mfma<T>::row(lk, v), col = wc + bj * 16 + lr;
         acc[bi][bj][v] = (row < nI && col < nJ) ? Fs[(size_t)(rJ + col) * m + rI + row] : (T)0;
       }
-  const int kmax = (w + 3) & ~3;
-  for (int kk = 0; kk < kmax; kk += 4) {
-    const T a0 = -Ws[(wr + lr) * LD + kk + lk];
-    const T a1 = -Ws[(wr + 16 + lr) * LD + kk + lk];
-    const T b0 = Ls[(wc + lr) * LD + kk + lk];
-    const T b1 = Ls[(wc + 16 + lr) * LD + kk + lk];
-    acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
-    acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
-    acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
-    acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+  for (int pp = 0; pp < np; pp++) {
+    const int j0 = (pfirst + pp) * NB, w = min(NB, k - j0);
+    const T* Wp = Wsrc + (size_t)pp * m * NB;
+    if (pp > 0) __syncthreads();  // the last panel's tiles are consumed
+    {
+      // row r, panel columns t00 + 4q: loads first (clamped addresses), then LDS
+      const int r = threadIdx.x & 63, t00 = threadIdx.x >> 6;
+      const int rw = min(r, nI - 1), rl = min(r, nJ - 1);
+      T wv[16], lv[16];
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int tc = min(t00 + 4 * q, w - 1);
+        wv[q] = Wp[(size_t)tc * m + rI + rw];
+        lv[q] = Fs[(size_t)(j0 + tc) * m + rJ + rl];
+      }
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int t = t00 + 4 * q;
+        Ws[r * LD + t] = (r < nI && t < w) ? wv[q] : (T)0;
+        Ls[r * LD + t] = (r < nJ && t < w) ? lv[q] : (T)0;
+      }
+    }
+    __syncthreads();
+    const int kmax = (w + 3) & ~3;
+    for (int kk = 0; kk < kmax; kk += 4) {
+      const T a0 = -Ws[(wr + lr) * LD + kk + lk];
+      const T a1 = -Ws[(wr + 16 + lr) * LD + kk + lk];
+      const T b0 = Ls[(wc + lr) * LD + kk + lk];
+      const T b1 = Ls[(wc + 16 + lr) * LD + kk + lk];
+      acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
+      acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
+      acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+      acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+    }
   }
 #pragma unroll
   for (int bi = 0; bi < 2; bi++)
@@ -683,15 +720,23 @@ __global__ __launch_bounds__(256) void k_panel_xrow(const int2* __restrict__ ite
     for (int bj = 0; bj < 2; bj++) acc[bi][bj] = (v4)((T)0);
   for (int q = cb; q < p; q++) {
     const int q0 = q * 64;
-    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-      const int cc = e >> 6, rr = e & 63;
-      // L(j0+rr, q0+cc): column q0+cc, rows contiguous
-      As[rr * LD + cc] = (rr < w) ? Fs[(size_t)(q0 + cc) * m + j0 + rr] : (T)0;
-      // X(q0+cc, c0+rr) as Bn[n=rr][kk=cc]
-      T xv;
-      if (q == cb) xv = (cc > rr) ? Fs[(size_t)(q0 + cc) * m + c0 + rr] : (cc == rr ? (T)1 : (T)0);
-      else xv = Fs[(size_t)(q0 + cc) * m + c0 + rr];
-      Bn[rr * LD + cc] = xv;
+    {
+      // row rr, columns cc00 + 4i: loads first (clamped), then LDS
+      const int rr = threadIdx.x & 63, cc00 = threadIdx.x >> 6, rw = min(rr, w - 1);
+      T av[16], xw[16];
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const int cc = cc00 + 4 * i;
+        av[i] = Fs[(size_t)(q0 + cc) * m + j0 + rw];  // L(j0+rr, q0+cc): column q0+cc
+        xw[i] = Fs[(size_t)(q0 + cc) * m + c0 + rr];  // X(q0+cc, c0+rr)
+      }
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const int cc = cc00 + 4 * i;
+        As[rr * LD + cc] = (rr < w) ? av[i] : (T)0;
+        // X(q0+cc, c0+rr) as Bn[n=rr][kk=cc]
+        Bn[rr * LD + cc] = (q == cb) ? ((cc > rr) ? xw[i] : (cc == rr ? (T)1 : (T)0)) : xw[i];
+      }
     }
     __syncthreads();
 #pragma unroll 4
@@ -713,11 +758,18 @@ __global__ __launch_bounds__(256) void k_panel_xrow(const int2* __restrict__ ite
 #pragma unroll
       for (int v = 0; v < 4; v++)
         Bn[(wc + bj * 16 + lr) * LD + wr + bi * 16 + Mfma<T>::row(lk, v)] = acc[bi][bj][v];
-  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-    const int rr = e >> 6, cc = e & 63;  // Linv_P(rr, cc) at front (row j0+cc, col j0+rr)
-    T v = (T)0;
-    if (rr < w && cc < w) v = (cc < rr) ? Fs[(size_t)(j0 + rr) * m + j0 + cc] : (cc == rr ? (T)1 : (T)0);
-    As[rr * LD + cc] = -v;
+  {
+    // Linv_P(rr, cc) at front (row j0+cc, col j0+rr); column cc fixed per thread
+    const int cc = threadIdx.x & 63, rr00 = threadIdx.x >> 6, cw = min(cc, w - 1);
+    T lv[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) lv[i] = Fs[(size_t)(j0 + min(rr00 + 4 * i, w - 1)) * m + j0 + cw];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int rr = rr00 + 4 * i;
+      const T v = (rr < w && cc < w) ? ((cc < rr) ? lv[i] : (cc == rr ? (T)1 : (T)0)) : (T)0;
+      As[rr * LD + cc] = -v;
+    }
   }
   __syncthreads();
 #pragma unroll
@@ -786,16 +838,28 @@ __global__ __launch_bounds__(256) void k_zt(const int4* __restrict__ items, DevS
     for (int bj = 0; bj < 2; bj++) acc[bi][bj] = (v4)((T)0);
   for (int i0 = c0; i0 < k; i0 += 64) {
     const int ni = min(64, k - i0);
-    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-      const int ii = e >> 6, cc = e & 63;
-      T xv = (T)0, lv = (T)0;
-      if (ii < ni) {
-        const int gi = i0 + ii, gc = c0 + cc;
-        if (cc < nc) xv = (gi > gc) ? Fs[(size_t)gi * m + gc] : (gi == gc ? (T)1 : (T)0);
-        if (cc < nj) lv = Fs[(size_t)gi * m + k + jr0 + cc];
+    {
+      // column offset cc fixed per thread, rows ii00 + 4q: loads first (clamped)
+      const int cc = threadIdx.x & 63, ii00 = threadIdx.x >> 6;
+      const int cx = min(cc, nc - 1), cl = min(cc, nj - 1);
+      T xw[16], lw[16];
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int gi = i0 + min(ii00 + 4 * q, ni - 1);
+        xw[q] = Fs[(size_t)gi * m + c0 + cx];
+        lw[q] = Fs[(size_t)gi * m + k + jr0 + cl];
       }
-      As[cc * LD + ii] = xv;
-      Bs[cc * LD + ii] = lv;
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int ii = ii00 + 4 * q, gi = i0 + ii, gc = c0 + cc;
+        T xv = (T)0, lv = (T)0;
+        if (ii < ni) {
+          if (cc < nc) xv = (gi > gc) ? xw[q] : (gi == gc ? (T)1 : (T)0);
+          if (cc < nj) lv = lw[q];
+        }
+        As[cc * LD + ii] = xv;
+        Bs[cc * LD + ii] = lv;
+      }
     }
     __syncthreads();
 #pragma unroll 4
@@ -843,9 +907,17 @@ __global__ __launch_bounds__(256) void k_gform(const int4* __restrict__ items, D
     srow0 = it.y * 64; scol0 = k + it.z * 64;
     nrow = min(64, k - srow0); ncol = min(64, m - scol0);
   }
-  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-    const int x = e >> 6, y = e & 63;
-    if (x < ncol && y < nrow) tile[x * 65 + y] = Fs[(size_t)(scol0 + x) * m + srow0 + y];
+  {
+    // source row y fixed per thread, columns x00 + 4q: loads first (clamped)
+    const int y = threadIdx.x & 63, x00 = threadIdx.x >> 6, yc = min(y, nrow - 1);
+    T v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = Fs[(size_t)(scol0 + min(x00 + 4 * q, ncol - 1)) * m + srow0 + yc];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int x = x00 + 4 * q;
+      if (x < ncol && y < nrow) tile[x * 65 + y] = v[q];
+    }
   }
   __syncthreads();
   // destination: row scol0 + x, column srow0 + y
@@ -1585,16 +1657,16 @@ void launch_panel_diag(const int* items, int nitems, int p, const DevSym& d, T* 
                        amax, ctr);
 }
 template <typename T>
-void launch_panel_trsm(const int2* items, int nitems, int p, const DevSym& d, T* F, T* W,
+void launch_panel_trsm(const int2* items, int nitems, int p, int slot, const DevSym& d, T* F, T* W,
                        hipStream_t st) {
   if (nitems > 0)
-    hipLaunchKernelGGL(k_panel_trsm<T>, dim3(nitems), dim3(256), 0, st, items, p, d, F, W);
+    hipLaunchKernelGGL(k_panel_trsm<T>, dim3(nitems), dim3(256), 0, st, items, p, slot, d, F, W);
 }
 template <typename T>
-void launch_panel_update(const int4* items, int nitems, int p, const DevSym& d, T* F,
+void launch_panel_update(const int4* items, int nitems, int pfirst, const DevSym& d, T* F,
                          const T* W, hipStream_t st) {
   if (nitems > 0)
-    hipLaunchKernelGGL(k_panel_update<T>, dim3(nitems), dim3(256), 0, st, items, p, d, F, W);
+    hipLaunchKernelGGL(k_panel_update<T>, dim3(nitems), dim3(256), 0, st, items, pfirst, d, F, W);
 }
 template <typename T>
 void launch_panel_xrow(const int2* items, int nitems, int p, const DevSym& d, T* F, hipStream_t st) {
@@ -1756,7 +1828,7 @@ void launch_zero_counters(FactorCounters* c, hipStream_t st) {
                                   hipStream_t);                                                  \
   template void launch_panel_diag<T>(const int*, int, int, const DevSym&, T*, T*, T*, double,   \
                                      const double*, FactorCounters*, hipStream_t);               \
-  template void launch_panel_trsm<T>(const int2*, int, int, const DevSym&, T*, T*, hipStream_t); \
+  template void launch_panel_trsm<T>(const int2*, int, int, int, const DevSym&, T*, T*, hipStream_t); \
   template void launch_panel_update<T>(const int4*, int, int, const DevSym&, T*, const T*,       \
                                        hipStream_t);                                             \
   template void launch_panel_xrow<T>(const int2*, int, int, const DevSym&, T*, hipStream_t);     \

@@ -51,8 +51,12 @@ struct Launch {
   long long off = 0;
   int cnt = 0;
 };
+// one panel step of a tree level: look-ahead update of the panel's pivot
+// columns by the earlier panels of its group, diagonal block, TRSM into W-buffer
+// slot `slot`, and (after a group's last panel) the group's trailing update
 struct Step {
-  Launch diag, xrow, trsm, upd;
+  Launch look, diag, xrow, trsm, upd;
+  int slot = 0, pfirst = 0;
   double upd_flops = 0;
 };
 
@@ -500,7 +504,7 @@ struct kkt_solver {
       mm[s] = kk[s] + (int)r;
       foff[s + 1] = foff[s] + (long long)mm[s] * mm[s];
       roff[s + 1] = roff[s] + r;
-      woff[s + 1] = woff[s] + (long long)mm[s] * NB;
+      woff[s + 1] = woff[s] + (long long)UPD_GROUP * mm[s] * NB;
       uoff[s + 1] = uoff[s] + r;
     }
     for (int s = 0; s < ns; s++)
@@ -615,26 +619,51 @@ struct kkt_solver {
       int P = 0;
       for (int s : lv[L]) P = std::max(P, (kk[s] + NB - 1) / NB);
       steps[L].resize(P);
+      // Panels go in groups of UPD_GROUP: the trailing Schur update of a group is
+      // one launch with K = the group's columns; inside a group, each panel's
+      // pivot columns first receive the updates of the group's earlier panels
+      // (look-ahead), so the diagonal block and TRSM see the same values as with
+      // an update after every panel.
       for (int p = 0; p < P; p++) {
         Step& sp = steps[L][p];
+        const int pf = (p / UPD_GROUP) * UPD_GROUP;
+        sp.slot = p - pf;
+        sp.pfirst = pf;
+        sp.look.off = (long long)updv.size();
+        if (p > pf)
+          for (int s : lv[L]) {
+            const int j0 = p * NB;
+            if (kk[s] <= j0) continue;
+            const int w = std::min(NB, kk[s] - j0), t = (mm[s] - j0 + TILE - 1) / TILE;
+            for (int I = 0; I < t; I++) updv.push_back(make_int4(s, I, -1, (p - pf) | (j0 << 3)));
+            sp.upd_flops += 2.0 * (double)((p - pf) * NB) * (double)(mm[s] - j0) * (double)w;
+          }
+        sp.look.cnt = (int)(updv.size() - sp.look.off);
         sp.diag.off = (long long)diagv.size();
         sp.xrow.off = (long long)xrowv.size();
         sp.trsm.off = (long long)trsmv.size();
-        sp.upd.off = (long long)updv.size();
         for (int s : lv[L]) {
           int j0 = p * NB;
           if (kk[s] <= j0) continue;
           int w = std::min(NB, kk[s] - j0), r0 = j0 + w, mr = mm[s] - r0;
           diagv.push_back(s);
           for (int rb = 0; rb < (mr + 63) / 64; rb++) trsmv.push_back(make_int2(s, rb));
-          int t = (mr + TILE - 1) / TILE;
-          for (int I = 0; I < t; I++)
-            for (int J = 0; J <= I; J++) updv.push_back(make_int4(s, I, J, 0));
-          sp.upd_flops += (double)w * (double)mr * (double)(mr + 1);
         }
         sp.diag.cnt = (int)(diagv.size() - sp.diag.off);
         sp.xrow.cnt = (int)(xrowv.size() - sp.xrow.off);
         sp.trsm.cnt = (int)(trsmv.size() - sp.trsm.off);
+        sp.upd.off = (long long)updv.size();
+        if (p % UPD_GROUP == UPD_GROUP - 1 || p == P - 1)
+          for (int s : lv[L]) {
+            const int ps = (kk[s] + NB - 1) / NB;
+            if (ps <= pf) continue;
+            const int np = std::min(ps, pf + UPD_GROUP) - pf;
+            const int rs = std::min(kk[s], (pf + np) * NB), mr = mm[s] - rs;
+            const int t = (mr + TILE - 1) / TILE;
+            for (int I = 0; I < t; I++)
+              for (int J = 0; J <= I; J++) updv.push_back(make_int4(s, I, J, np | (rs << 3)));
+            sp.upd_flops += (double)(rs - pf * NB) * (double)mr * (double)(mr + 1);
+          }
         sp.upd.cnt = (int)(updv.size() - sp.upd.off);
       }
     }
@@ -791,16 +820,23 @@ struct kkt_solver {
       tend();
       for (size_t p = 0; p < steps[L].size(); p++) {
         const Step& sp = steps[L][p];
+        if (sp.look.cnt) {
+          tbeg(TC_UPD);
+          launch_panel_update<T>(d_upd + sp.look.off, sp.look.cnt, sp.pfirst, ds, F, W, st);
+          tend();
+        }
         tbeg(TC_DIAG);
         launch_panel_diag<T>(d_diag + sp.diag.off, sp.diag.cnt, (int)p, ds, F, F + front_elems,
                              F + front_elems + n, pert, d_scal + 15, d_ctr, st);
         tend();
         tbeg(TC_TRSM);
-        launch_panel_trsm<T>(d_trsm + sp.trsm.off, sp.trsm.cnt, (int)p, ds, F, W, st);
+        launch_panel_trsm<T>(d_trsm + sp.trsm.off, sp.trsm.cnt, (int)p, sp.slot, ds, F, W, st);
         tend();
-        tbeg(TC_UPD);
-        launch_panel_update<T>(d_upd + sp.upd.off, sp.upd.cnt, (int)p, ds, F, W, st);
-        tend();
+        if (sp.upd.cnt) {
+          tbeg(TC_UPD);
+          launch_panel_update<T>(d_upd + sp.upd.off, sp.upd.cnt, sp.pfirst, ds, F, W, st);
+          tend();
+        }
         if (tm_on) tm_upd_flops += sp.upd_flops;
       }
     }
