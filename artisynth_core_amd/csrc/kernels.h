@@ -35,6 +35,13 @@ struct DevSym {
   const long long* sn_goff;
   const int* gptr;
   const int* gsrc;
+  // backward vector: supernode s owns m entries at sn voff, [z of its pivots (pivot
+  // order); -x of its below-diagonal rows].  Solve position g is a below-diagonal
+  // row of the supernodes whose entries are push_pos[push_ptr[g] .. push_ptr[g+1])
+  // (ascending); wpos[q] is the entry of pivot position q
+  const int* push_ptr;
+  const int* push_pos;
+  const int* wpos;
 };
 
 // Work item of the solve kernels: one workgroup's share of a supernode, with the
@@ -49,6 +56,8 @@ struct alignas(16) SolveItem {
   long long uoff;  // update-vector offset
   long long roff;  // offset of the below-diagonal rows in rows[]
   long long goff;  // offset of the front positions in the gather map
+  long long voff;  // offset of the supernode's backward vector entries
+  long long pad2;
 };
 
 struct FactorCounters {  // device ints
@@ -90,10 +99,10 @@ template <typename T>
 void launch_fwd_rows(const SolveItem* items, int nitems, const DevSym& d, const T* F, const T* fp,
                      T* y, T* u, hipStream_t st);
 template <typename T>
-void launch_dsolve(int n, const T* E0, const T* E1, const signed char* piv, const T* y, T* z,
-                   hipStream_t st);
+void launch_dsolve(int n, const T* E0, const T* E1, const signed char* piv, const int* wpos,
+                   const T* y, T* wv, hipStream_t st);
 template <typename T>
-void launch_bwd_cols(const SolveItem* items, int nitems, const DevSym& d, const T* F, const T* z,
+void launch_bwd_cols(const SolveItem* items, int nitems, const DevSym& d, const T* F, T* wv,
                      T* xp, hipStream_t st);
 
 // permutation / precision conversion: out[j] = (TO) in[perm[j]]  (gather)

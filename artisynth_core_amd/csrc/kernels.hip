@@ -1100,90 +1100,64 @@ __global__ __launch_bounds__(256) void k_fwd_rows(const SolveItem* __restrict__ 
   }
 }
 
-// z = D^-1 y over all pivots: z_g = E0_g y_g + E1_g y_partner
+// z = D^-1 y over all pivots: z_g = E0_g y_g + E1_g y_partner, stored at the
+// pivot's entry of the backward vector
 template <typename T>
 __global__ __launch_bounds__(256) void k_dsolve(int n, const T* __restrict__ E0,
                                                 const T* __restrict__ E1,
                                                 const signed char* __restrict__ piv,
-                                                const T* __restrict__ y, T* __restrict__ z) {
+                                                const int* __restrict__ wpos,
+                                                const T* __restrict__ y, T* __restrict__ wv) {
   const int g = blockIdx.x * 256 + threadIdx.x;
   if (g >= n) return;
   const int pc = piv[g];
   const int pg = (pc == 2) ? g + 1 : (pc == 0 ? g - 1 : g);
-  z[g] = E0[g] * y[g] + E1[g] * y[pg];
+  wv[wpos[g]] = E0[g] * y[g] + E1[g] * y[pg];
 }
 
-// backward from the lower copy (column c, rows below the diagonal,
-// contiguous): x_c = z_c + sum_{c<i<k} X(i,c) z_i - sum_j Z(j,c) x[rows_j].
+// backward from the lower copy (column c, rows below the diagonal, contiguous):
+// x_c = z_c + sum_{c<i<k} X(i,c) z_i - sum_j Z(j,c) x[rows_j] = z_c + G(:,c) . w[c+1..m)
+// with w the supernode's backward vector [z; -x[rows]].  The -x entries were put
+// there by the ancestors' backward launches: every computed x_g is also written,
+// negated, to all entries that name g (the push map), so a column is one
+// contiguous dot product, like a forward row.
 // A segment of 2^lg lanes per column (lg <= 8: a long column takes two or four
 // waves); item = (s, block of 256 >> lg columns, lg)
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_cols(const SolveItem* __restrict__ items, DevSym d,
-                                                  const T* __restrict__ F,
-                                                  const T* __restrict__ z, T* __restrict__ xp) {
-  constexpr int W = 16 / sizeof(T);
-  typedef typename V16<T>::v vt;
-  typedef typename VI16<T>::v iv;
+                                                  const T* __restrict__ F, T* wv,
+                                                  T* __restrict__ xp) {
   __shared__ T part[4];
   const SolveItem it = items[blockIdx.x];
   const int lg = it.lg;  // 2^lg lanes per column, up to the whole workgroup (lg = 8)
   const int seg = 1 << lg, sl = threadIdx.x & (seg - 1);
   const int c = it.blk * (256 >> lg) + (threadIdx.x >> lg);
-  const int k = it.k, m = it.m, first = it.first, r = m - k;
+  const int k = it.k, m = it.m, first = it.first;
   const bool ok = c < k;
-  const bool own = sl == 0 && ok;
-  int dst = 0;
-  T zc = (T)0;
-  if (own) {
-    dst = d.lperm[first + c];
-    zc = z[first + c];
-  }
-  // pivot-block part X(i,c) z_i (i = c+1..k-1) and contribution part Z(j,c) x[rows_j]
-  // in one pass: every chunk load of a group of four issues at once, then the x
-  // gathers, then the products (the same chunk / select scheme as seg_dot)
-  const int lx = ok ? k - c - 1 : 0, lz = ok ? r : 0;
-  const T* colx = F + it.foff + (size_t)(ok ? c : 0) * m + c + 1;
-  const T* colz = colx + (k - c - 1);
-  const T* zx = z + first + c + 1;
-  const int* rows = d.rows + it.roff;
-  const int ncx = (lx + W - 1) / W, ncz = (lz + W - 1) / W, nc = max(ncx, ncz);
-  T ax = (T)0, az = (T)0;
-  for (int q0 = sl; q0 < nc; q0 += 4 * seg) {
-    vt vx[4], bx[4], vz[4];
-    iv iz[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int q = q0 + u * seg, qx = q < ncx ? q : 0, qz = q < ncz ? q : 0;
-      vx[u] = ldv(colx + qx * W);
-      bx[u] = ldv(zx + qx * W);
-      vz[u] = ldv(colz + qz * W);
-      __builtin_memcpy(&iz[u], rows + qz * W, sizeof(iv));
-    }
-    T gz[4][W];
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-#pragma unroll
-      for (int w = 0; w < W; w++) gz[u][w] = xp[iz[u][w]];
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-#pragma unroll
-      for (int w = 0; w < W; w++) {
-        const int e = (q0 + u * seg) * W + w;
-        const T tx = fma(vx[u][w], bx[u][w], ax), tz = fma(vz[u][w], gz[u][w], az);
-        ax = (e < lx) ? tx : ax;
-        az = (e < lz) ? tz : az;
-      }
-  }
-  T acc = seg_sum(ax - az, lg < 6 ? lg : 6);
-  if (lg > 6) {  // columns spanning 2 or 4 waves: combine the wave sums in order
+  // every lane of the segment loads these (the push below uses g and x); they are
+  // used only after the product, so their loads stay in flight meanwhile
+  const int cq = ok ? c : 0;
+  const int lp = d.lperm[first + cq];
+  const T* w = wv + it.voff;
+  const T zc = w[cq];
+  const T* col = F + it.foff + (size_t)cq * m + cq + 1;
+  // every lane of a segment ends with the same sum (xor butterfly; the wave sums
+  // of a column spanning 2 or 4 waves are combined in order)
+  T acc = seg_sum(seg_dot<T>(col, w + cq + 1, ok ? m - c - 1 : 0, sl, seg), lg < 6 ? lg : 6);
+  if (lg > 6) {
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (own) {
-      acc = part[threadIdx.x >> 6];
-      for (int q = 1; q < (seg >> 6); q++) acc += part[(threadIdx.x >> 6) + q];
-    }
+    const int w0 = (threadIdx.x >> lg) << (lg - 6);  // first wave of the column
+    acc = part[w0];
+    for (int q = 1; q < (seg >> 6); q++) acc += part[w0 + q];
   }
-  if (own) xp[first + dst] = zc + acc;
+  if (ok) {
+    const int g = first + lp;
+    const T x = zc + acc;
+    if (sl == 0) xp[g] = x;
+    const int e1 = d.push_ptr[g + 1];
+    for (int e = d.push_ptr[g] + sl; e < e1; e += seg) wv[d.push_pos[e]] = -x;
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -1693,14 +1667,15 @@ void launch_fwd_rows(const SolveItem* items, int nitems, const DevSym& d, const 
     hipLaunchKernelGGL(k_fwd_rows<T>, dim3(nitems), dim3(256), 0, st, items, d, F, fp, y, u);
 }
 template <typename T>
-void launch_dsolve(int n, const T* E0, const T* E1, const signed char* piv, const T* y, T* z,
-                   hipStream_t st) {
-  if (n > 0) hipLaunchKernelGGL(k_dsolve<T>, dim3(nblk(n, 256)), dim3(256), 0, st, n, E0, E1, piv, y, z);
+void launch_dsolve(int n, const T* E0, const T* E1, const signed char* piv, const int* wpos,
+                   const T* y, T* wv, hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_dsolve<T>, dim3(nblk(n, 256)), dim3(256), 0, st, n, E0, E1, piv, wpos, y, wv);
 }
 template <typename T>
-void launch_bwd_cols(const SolveItem* items, int nitems, const DevSym& d, const T* F, const T* z, T* xp,
+void launch_bwd_cols(const SolveItem* items, int nitems, const DevSym& d, const T* F, T* wv, T* xp,
                      hipStream_t st) {
-  if (nitems > 0) hipLaunchKernelGGL(k_bwd_cols<T>, dim3(nitems), dim3(256), 0, st, items, d, F, z, xp);
+  if (nitems > 0) hipLaunchKernelGGL(k_bwd_cols<T>, dim3(nitems), dim3(256), 0, st, items, d, F, wv, xp);
 }
 template <typename TI, typename TO>
 void launch_gather_perm(int n, const int* perm, const TI* in, TO* out, hipStream_t st) {
@@ -1838,9 +1813,9 @@ void launch_zero_counters(FactorCounters* c, hipStream_t st) {
                                      T*, hipStream_t);                                           \
   template void launch_fwd_rows<T>(const SolveItem*, int, const DevSym&, const T*, const T*, T*, T*,  \
                                    hipStream_t);                                                 \
-  template void launch_dsolve<T>(int, const T*, const T*, const signed char*, const T*, T*,      \
-                                 hipStream_t);                                                   \
-  template void launch_bwd_cols<T>(const SolveItem*, int, const DevSym&, const T*, const T*, T*,      \
+  template void launch_dsolve<T>(int, const T*, const T*, const signed char*, const int*,        \
+                                 const T*, T*, hipStream_t);                                     \
+  template void launch_bwd_cols<T>(const SolveItem*, int, const DevSym&, const T*, T*, T*,          \
                                    hipStream_t);
 KKT_INST(double)
 KKT_INST(float)

@@ -115,13 +115,14 @@ struct kkt_solver {
   // per level: assembly, scatter; forward gather / rows, backward columns
   std::vector<Launch> asm_l, scat_l, fg_l, fr_l, bc_l;
   std::vector<std::vector<Step>> steps;
-  long long front_elems = 0, w_elems = 0, u_elems = 0;
+  long long front_elems = 0, w_elems = 0, u_elems = 0, v_elems = 0;
   // numeric buffers (allocated lazily per precision)
   double* d_F64 = nullptr;
   float* d_F32 = nullptr;
   double* d_W64 = nullptr;
   float* d_W32 = nullptr;
-  void* d_svec = nullptr;  // solve scratch: bp, fp, y, z, xp (n each), u (u_elems), as T
+  // solve scratch, as T: bp, fp, y, xp (n each), u (u_elems + 16), w (v_elems + 16)
+  void* d_svec = nullptr;
   size_t svec_bytes = 0;
   // FP64 vectors for refinement / BiCGStab / host copies
   double* d_vec = nullptr;  // 14 vectors of n (0-8 BiCGStab, 10-11 host copies, 12-13 graph I/O)
@@ -418,7 +419,7 @@ struct kkt_solver {
   // per lane) is the shape the autotuner picks per level; the gather items do
   // not depend on it.
   std::vector<int> sv_k, sv_m, sv_first;
-  std::vector<long long> sv_foff, sv_uoff, sv_roff, sv_goff;
+  std::vector<long long> sv_foff, sv_uoff, sv_roff, sv_goff, sv_voff;
   std::vector<std::vector<int>> sv_lv;
   std::vector<int> fr_epl, bc_epl;
   int replace_upload(SolveItem*& dptr, const std::vector<SolveItem>& v) {
@@ -442,6 +443,7 @@ struct kkt_solver {
       it.s = s; it.blk = blk; it.lg = lg; it.kind = kind;
       it.k = sv_k[s]; it.m = sv_m[s]; it.first = sv_first[s]; it.pad = 0;
       it.foff = sv_foff[s]; it.uoff = sv_uoff[s]; it.roff = sv_roff[s]; it.goff = sv_goff[s];
+      it.voff = sv_voff[s]; it.pad2 = 0;
       return it;
     };
     auto seg_lg = [](int len, int epl, int cap) {
@@ -495,8 +497,8 @@ struct kkt_solver {
     d_vals = d_valsf = d_vec = d_part2 = nullptr;
     const int ns = S.nsuper;
     std::vector<int> first(ns), kk(ns), mm(ns);
-    std::vector<long long> foff(ns + 1), roff(ns + 1), woff(ns + 1), uoff(ns + 1);
-    foff[0] = roff[0] = woff[0] = uoff[0] = 0;
+    std::vector<long long> foff(ns + 1), roff(ns + 1), woff(ns + 1), uoff(ns + 1), voff(ns + 1);
+    foff[0] = roff[0] = woff[0] = uoff[0] = voff[0] = 0;
     for (int s = 0; s < ns; s++) {
       first[s] = S.sup_ptr[s];
       kk[s] = S.sup_ptr[s + 1] - S.sup_ptr[s];
@@ -506,12 +508,15 @@ struct kkt_solver {
       roff[s + 1] = roff[s] + r;
       woff[s + 1] = woff[s] + (long long)UPD_GROUP * mm[s] * NB;
       uoff[s + 1] = uoff[s] + r;
+      voff[s + 1] = voff[s] + mm[s];
     }
     for (int s = 0; s < ns; s++)
       if (kk[s] > 1024) return fail(KKT_ERR_ANALYSIS, "internal: supernode wider than 1024 columns");
     front_elems = foff[ns];
     w_elems = woff[ns];
     u_elems = uoff[ns];
+    v_elems = voff[ns];
+    if (v_elems >= (1LL << 31) - 16) return fail(KKT_ERR_ANALYSIS, "internal: solve vector too large");
     // children (ascending)
     std::vector<int> chp(ns + 1, 0), chl;
     for (int s = 0; s < ns; s++)
@@ -729,8 +734,22 @@ struct kkt_solver {
     // a supernode's rows)
     std::vector<int> rows_pad(S.rows);
     rows_pad.resize(S.rows.size() + 4, 0);
+    // backward vector: supernode s owns entries [voff_s, voff_s + m_s): its pivots,
+    // then its below-diagonal rows.  Push map: for each solve position g, the
+    // entries of the rows naming it (ascending); wpos: the entry of each pivot
+    std::vector<int> pptr(n + 1, 0), ppos(std::max<size_t>(S.rows.size(), 1), 0), wpos(std::max(n, 1), 0);
+    for (size_t e = 0; e < S.rows.size(); e++) pptr[S.rows[e] + 1]++;
+    for (int g = 0; g < n; g++) pptr[g + 1] += pptr[g];
+    {
+      std::vector<int> fill(pptr.begin(), pptr.end() - 1);
+      for (int s = 0; s < ns; s++) {
+        for (int t = 0; t < kk[s]; t++) wpos[first[s] + t] = (int)(voff[s] + t);
+        for (long long j = 0; j < roff[s + 1] - roff[s]; j++)
+          ppos[fill[S.rows[roff[s] + j]]++] = (int)(voff[s] + kk[s] + j);
+      }
+    }
     int rc = 0;
-    int *d_first, *d_k, *d_m, *d_chp, *d_chl, *d_rel, *d_rows;
+    int *d_first, *d_k, *d_m, *d_chp, *d_chl, *d_rel, *d_rows, *d_pptr, *d_ppos, *d_wpos;
     long long *d_foff, *d_roff, *d_woff, *d_uoff;
     if ((rc = upload(d_first, first)) || (rc = upload(d_k, kk)) || (rc = upload(d_m, mm)) ||
         (rc = upload(d_foff, foff)) || (rc = upload(d_roff, roff)) || (rc = upload(d_woff, woff)) ||
@@ -738,11 +757,12 @@ struct kkt_solver {
         (rc = upload(d_rel, rel)) || (rc = upload(d_rows, rows_pad)) || (rc = upload(d_perm, S.perm)) ||
         (rc = upload(d_asm, asmv)) || (rc = upload(d_upd, updv)) || (rc = upload(d_diag, diagv)) ||
         (rc = upload(d_aidx, aidx)) || (rc = upload(d_apos, apos)) || (rc = upload(d_trsm, trsmv)) ||
-        (rc = upload(d_xrow, xrowv)) || (rc = upload(d_zt, ztv)) || (rc = upload(d_gf, gfv)))
+        (rc = upload(d_xrow, xrowv)) || (rc = upload(d_zt, ztv)) || (rc = upload(d_gf, gfv)) ||
+        (rc = upload(d_pptr, pptr)) || (rc = upload(d_ppos, ppos)) || (rc = upload(d_wpos, wpos)))
       return rc;
     // solve work items, default shapes (sixteen elements per lane)
     sv_k = kk; sv_m = mm; sv_first = first; sv_foff = foff; sv_uoff = uoff; sv_roff = roff;
-    sv_goff = goff; sv_lv = lv;
+    sv_goff = goff; sv_voff = voff; sv_lv = lv;
     fr_epl.assign(nl, 16);
     bc_epl.assign(nl, 16);
     if ((rc = build_solve_items())) return rc;
@@ -756,6 +776,7 @@ struct kkt_solver {
     ds.sn_first = d_first; ds.sn_k = d_k; ds.sn_m = d_m;
     ds.sn_foff = d_foff; ds.sn_roff = d_roff; ds.sn_woff = d_woff; ds.sn_uoff = d_uoff;
     ds.ch_ptr = d_chp; ds.ch_list = d_chl; ds.rel = d_rel; ds.rows = d_rows;
+    ds.push_ptr = d_pptr; ds.push_pos = d_ppos; ds.wpos = d_wpos;
     ds.lperm = d_lperm; ds.ilperm = d_ilperm; ds.piv = d_piv;
     ds.sn_goff = d_goff; ds.gptr = d_gptr; ds.gsrc = d_gsrc;
     // host copies of what was uploaded, for the structure-parity check
@@ -795,7 +816,7 @@ struct kkt_solver {
           hipMalloc(&d_W32, std::max<long long>(w_elems, 1) * sizeof(float)) != hipSuccess)
         return fail(KKT_ERR_ALLOC, "hipMalloc failed (FP32 fronts)");
     }
-    size_t need = ((size_t)5 * n + (size_t)u_elems + 16) * std::max(esz, sizeof(double));
+    size_t need = ((size_t)4 * n + (size_t)u_elems + 16 + (size_t)v_elems + 16) * std::max(esz, sizeof(double));
     if (svec_bytes < need) {
       drop_graphs();
       if (d_svec) (void)hipFree(d_svec);
@@ -914,8 +935,8 @@ struct kkt_solver {
   template <typename T>
   void run_tri_solve(const T* F, const double* d_b, double* d_x) {
     T* sv = (T*)d_svec;
-    T *bp = sv, *fp = sv + n, *y = sv + 2 * (size_t)n, *z = sv + 3 * (size_t)n,
-      *xp = sv + 4 * (size_t)n, *u = sv + 5 * (size_t)n;
+    T *bp = sv, *fp = sv + n, *y = sv + 2 * (size_t)n, *xp = sv + 3 * (size_t)n,
+      *u = sv + 4 * (size_t)n, *wv = u + u_elems + 16;
     const T* E0 = F + front_elems;
     const T* E1 = E0 + n;
     tbeg(TC_VEC);
@@ -938,13 +959,13 @@ struct kkt_solver {
       }
     }
     tbeg(TC_SLEVEL);
-    launch_dsolve<T>(n, E0, E1, d_piv, y, z, st);
+    launch_dsolve<T>(n, E0, E1, d_piv, ds.wpos, y, wv, st);
     tend();
     for (int L = nl - 1; L >= 0; L--) {
       if (!bc_l[L].cnt) continue;
       tbeg(TC_SGEMV);
       if (lev_timing) (void)hipEventRecord(lev_ev[4 * L + 2], st);
-      launch_bwd_cols<T>(d_bc + bc_l[L].off, bc_l[L].cnt, ds, F, z, xp, st);
+      launch_bwd_cols<T>(d_bc + bc_l[L].off, bc_l[L].cnt, ds, F, wv, xp, st);
       if (lev_timing) (void)hipEventRecord(lev_ev[4 * L + 3], st);
       tend();
     }
