@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                     FactorCounters* __restrict__ ctr) {
   __shared__ double A[64 * 65];     // A[row*65 + col]: lower part L/D, upper part X
   __shared__ double dv[64][3];      // 1x1: {1/d, d}; 2x2 (first column): D^-1 {i11, i21, i22}
-  __shared__ double Sb[64 * 17];    // fast path: Sb[q*64 + t] = unscaled a(t, c0 + q) of the slab
+  __shared__ double Sb[64 * 16];    // fast path: Sb[q*64 + t] = unscaled a(t, c0 + q) of the slab
   __shared__ int sfail;
   __shared__ int lp[64];
   __shared__ signed char pt[64];
@@ -171,13 +171,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   int npos = 0, nneg = 0, npert = 0, n2 = 0;
   // Fast path: Bunch-Kaufman takes the 1x1 pivot without interchange whenever
   // |a_kk| >= alpha * colmax (or perturbs a pivot below tau).  Wave 0 factors the
-  // block in 16-column slabs held in registers (lane t: row t of the slab; each
-  // pivot column goes to LDS and its entries come back as broadcasts while the
-  // pivot test and reciprocal run), then all four waves apply
-  // the slab's rank-16 update to the rest of the block.  The first column that
-  // needs anything else sends the whole block back to the full pivoting loop
-  // below, from the original data.  On success the lower part holds L with D on
-  // the diagonal (what the pivoting loop and its scaling pass produce).
+  // block in 16-column slabs held in registers (lane t: row t of the slab).  Of a
+  // pivot column's entries at the slab's later rows, the one the next column
+  // needs comes as a scalar (readlane); the others go through LDS, read back
+  // while the pivot test (two ballots) and the reciprocal run.  A failed test
+  // only clears ok (no branch in the step).  Then all four waves apply the
+  // slab's rank-16 update to the rest of the block.  A slab with a failed column
+  // sends the whole block back to the full pivoting loop below, from the
+  // original data (the columns before the first failure are the same as that
+  // loop's, so the failure is found on the values it would test).  On success
+  // the lower part holds L with D on the diagonal (what the pivoting loop and its
+  // scaling pass produce).
   bool fast = true;
   for (int c0 = 0; c0 < w; c0 += 16) {
     const int c1 = min(c0 + 16, w);
@@ -189,38 +193,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
       for (int q = 0; q < 16; q++) {
         const int c = c0 + q;
-        if (ok && c < c1) {
-          const double akk = readlane_d(r[q], c);
-          Sb[q * 64 + t] = r[q];
+        if (c < c1) {
+          const double rq = r[q];  // column c, not yet scaled
+          const double akk = readlane_d(rq, c);
+          const double bn = (q < 15) ? readlane_d(rq, c0 + q + 1) : 0.0;  // a(c + 1, c)
+          Sb[q * 64 + t] = rq;
           __builtin_amdgcn_wave_barrier();
           const double absakk = fabs(akk);
           const bool below = t > c && t < w;
-          const double av = fabs(r[q]);
+          const double av = fabs(rq);
           // colmax < tau and |a_kk| < tau: perturbed 1x1 pivot; otherwise the
           // test |a_kk| >= alpha colmax must hold in every row below
           const unsigned long long big = __builtin_amdgcn_ballot_w64(below && av >= tau);
           const unsigned long long bad = __builtin_amdgcn_ballot_w64(below && absakk < alpha * av);
           const bool pz = (absakk < tau) & (big == 0ULL);
-          if (!pz & (bad != 0ULL)) {
-            ok = false;
-          } else {
-            double dd = akk;
-            if (pz || absakk < tau) {
-              dd = (dd >= 0.0) ? tau : -tau;
-              npert++;
-            }
-            if (dd > 0) npos++; else nneg++;
-            const double dinv = rcp_newton(dd);
-            // rows on or above c take lt = 0 and keep their values (a(c0 + j, c)
-            // is finite), so the update needs no select
-            const double lt = below ? r[q] * dinv : 0.0;
+          ok = ok & (pz | (bad == 0ULL));
+          const bool pert = pz | (absakk < tau);
+          const double dd = pert ? ((akk >= 0.0) ? tau : -tau) : akk;
+          npert += pert ? 1 : 0;
+          npos += (dd > 0) ? 1 : 0;
+          nneg += (dd > 0) ? 0 : 1;
+          const double dinv = rcp_newton(dd);
+          // rows on or above c take lt = 0 and keep their values (a(c0 + j, c)
+          // is finite), so the update needs no select
+          const double lt = below ? rq * dinv : 0.0;
+          if (q < 15) r[q + 1] = fma(-lt, bn, r[q + 1]);
 #pragma unroll
-            for (int j = q + 1; j < 16; j++) r[j] = fma(-lt, Sb[q * 64 + c0 + j], r[j]);
-            r[q] = below ? lt : (t == c ? dd : r[q]);
-            if (t == 0) {
-              dv[c][0] = dinv;
-              dv[c][1] = dd;
-            }
+          for (int j = q + 2; j < 16; j++) r[j] = fma(-lt, Sb[q * 64 + c0 + j], r[j]);
+          r[q] = below ? lt : (t == c ? dd : rq);
+          if (t == 0) {
+            dv[c][0] = dinv;
+            dv[c][1] = dd;
           }
         }
       }
@@ -429,19 +432,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   // as X(l,c) = 0 (l < c), 1 (l = c), so every lane runs the same loop
   if (t < 16) {
     const int b0 = wv * 16, c = t;
-    double* xc = Sb + (b0 + c) * 17;
+    double x[16];  // column c of X(b, b), in registers
 #pragma unroll
-    for (int l = 0; l < 16; l++) xc[l] = (l == c) ? 1.0 : 0.0;
-#pragma unroll 1
+    for (int l = 0; l < 16; l++) x[l] = (l == c) ? 1.0 : 0.0;
+#pragma unroll
     for (int r = 1; r < 16; r++) {
-      const double* lrow = A + (b0 + r) * 65 + b0;
-      double acc = 0.0;
-      for (int l = 0; l < r; l++) acc = fma(lrow[l], xc[l], acc);
-      if (r > c) xc[r] = -acc;
+      // row r of L, the same in every lane.  Its offset is tied to x[r - 2], so
+      // its loads issue while row r - 1 is computed and no earlier (issued all
+      // at once, the 120 loads would not fit in registers)
+      int off = (b0 + r) * 65 + b0;
+      if (r >= 2) __asm__ __volatile__("" : "+v"(off) : "v"(x[r - 2]));
+      const double* lrow = A + off;
+      double a0 = 0.0, a1 = 0.0;  // two chains halve the dependent FMAs
+#pragma unroll
+      for (int l = 0; l < r; l++) {
+        if (l & 1) a1 = fma(lrow[l], x[l], a1);
+        else a0 = fma(lrow[l], x[l], a0);
+      }
+      x[r] = (r > c) ? -(a0 + a1) : x[r];
     }
 #pragma unroll
     for (int r = 1; r < 16; r++)
-      if (r > c) A[(b0 + c) * 65 + b0 + r] = xc[r];
+      if (r > c) A[(b0 + c) * 65 + b0 + r] = x[r];
   }
   __syncthreads();
   // off-diagonal 16x16 blocks with f64 MFMA, block row bi after block row bi-1;

@@ -1054,6 +1054,9 @@ struct kkt_solver {
       float ms;
       if ((rc = time_solves(4 * reps, b, x, &ms))) return rc;
       if (ms < best) { best = ms; bi = q; }
+      if (g_timing)
+        fprintf(stderr, "[kkt-timing] autotune(%s) candidate %zu: %.3f ms per triangular solve\n",
+                fprec == KKT_PREC_FP64 ? "fp64" : "fp32", q, ms);
     }
     fr_epl = cands[bi].first;
     bc_epl = cands[bi].second;
