@@ -239,17 +239,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       fast = false;
       break;
     }
-    // A(t, j) -= sum_{q in slab} L(t, q) a(j, q) for the columns after the slab
-    // (a full slab whenever rows remain below it)
-    if (t >= c1 && t < w) {
-      double lq[16];
+    // A(t, j) -= sum_{q in slab} L(t, q) a(j, q) on the trailing block (rows and
+    // columns from c1; a full slab whenever rows remain below it): its 16x16
+    // tiles on and below the diagonal, one per wave at a time, K = 16 as four
+    // f64 MFMA steps.  Rows and columns past w hold zeros and are never stored.
+    {
+      typedef Mfma<double>::v4 v4;
+      const int nt = (w - c1 + 15) >> 4;  // tiles per side, at most 3
+      const int lr = t & 15, lk = t >> 4;
+      for (int tile = wv; tile < nt * (nt + 1) / 2; tile += 4) {
+        int bi = 0, bj = tile;  // tile -> (bi, bj), bj <= bi, row by row
+        while (bj > bi) {
+          bj -= bi + 1;
+          bi++;
+        }
+        const int R0 = c1 + 16 * bi, C0 = c1 + 16 * bj;
+        v4 acc;
 #pragma unroll
-      for (int q = 0; q < 16; q++) lq[q] = A[t * 65 + c0 + q];
-      for (int j = c1 + wv; j < w; j += 4) {
-        double acc = A[t * 65 + j];
+        for (int v = 0; v < 4; v++) acc[v] = A[(R0 + Mfma<double>::row(lk, v)) * 65 + C0 + lr];
 #pragma unroll
-        for (int q = 0; q < 16; q++) acc = fma(-lq[q], Sb[q * 64 + j], acc);
-        A[t * 65 + j] = acc;
+        for (int kk = 0; kk < 4; kk++)
+          acc = Mfma<double>::op(-A[(R0 + lr) * 65 + c0 + 4 * kk + lk],
+                                 Sb[(4 * kk + lk) * 64 + C0 + lr], acc);
+#pragma unroll
+        for (int v = 0; v < 4; v++) A[(R0 + Mfma<double>::row(lk, v)) * 65 + C0 + lr] = acc[v];
       }
     }
     __syncthreads();

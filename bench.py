@@ -38,8 +38,11 @@ def log(msg):
 def bwd_sweep_bytes(L, h, n, w):
     """Algorithmic HBM bytes of one backward sweep (all k_bwd_cols launches of a
     triangular solve): per supernode the lower copy of G once (k(k-1)/2 + k r
-    entries), z and x for its k pivots, the gathered x of its r rows, and the row /
-    permutation indices; w = bytes per entry of the front precision."""
+    entries), its backward vector once (k + r entries: z of the pivots, the
+    pushed -x of the rows), x written for its k pivots and pushed to the r row
+    entries it fills in the descendants (counted where they are filled: r per
+    supernode), and the indices (k pivot permutation and k push ranges, r push
+    positions); w = bytes per entry of the front precision."""
     from artisynth_core_amd._lib import KktInfo
     from artisynth_core_amd._lib import ptr as nptr
     inf = KktInfo()
@@ -50,7 +53,7 @@ def bwd_sweep_bytes(L, h, n, w):
     L.kkt_get_symbolic(h, None, nptr(sup, ctypes.c_int), None, nptr(rp, ctypes.c_longlong), None)
     k = np.diff(sup).astype(np.float64)
     r = np.diff(rp).astype(np.float64)
-    return float(np.sum((k * (k - 1) / 2 + k * r + 2 * k + r) * w + (r + k) * 4)), inf.nlevels
+    return float(np.sum((k * (k - 1) / 2 + k * r + 2 * k + 2 * r) * w + (r + 2 * k) * 4)), inf.nlevels
 
 
 def time_bwd_sweeps(L, h, b_ptr, x_ptr, nsolves=6):
