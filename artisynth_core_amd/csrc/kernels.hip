@@ -53,6 +53,15 @@ struct Mfma<float> {
   static __device__ inline int row(int lk, int v) { return 4 * lk + v; }
 };
 
+// Work item of workgroup b of n: the hardware deals workgroups to the eight
+// XCDs round robin (b % 8), so XCD x gets the contiguous item range [x q +
+// min(x, n % 8), ...) with q = n / 8, and neighbouring tiles, which share operand
+// panels, share that XCD's L2.  A bijection of [0, n) whatever the placement.
+__device__ inline int xcd_item(int b, int n) {
+  const int x = b & 7, i = b >> 3, q = n >> 3, rmd = n & 7;
+  return x * q + min(x, rmd) + i;
+}
+
 // ------------------------------------------------------------------------
 // assembly: one work item = one 64x64 lower tile of a parent front
 // ------------------------------------------------------------------------
@@ -648,7 +657,7 @@ __global__ __launch_bounds__(256) void k_panel_update(const int4* __restrict__ i
   __shared__ T Ws[64 * LD];  // Ws[r*LD + t]
   __shared__ T Ls[64 * LD];
   typedef typename Mfma<T>::v4 v4;
-  const int4 it = items[blockIdx.x];
+  const int4 it = items[xcd_item(blockIdx.x, gridDim.x)];
   const int s = it.x, I = it.y, J = it.z, np = it.w & 7, rs = it.w >> 3;
   const int k = d.sn_k[s], m = d.sn_m[s];
   const bool look = J < 0;
