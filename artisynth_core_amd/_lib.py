@@ -71,6 +71,7 @@ SIGNATURES = [
     ("kkt_solve", _I, [_P, _DP, _DP]),
     ("kkt_solve_device", _I, [_P, _P, _P]),
     ("kkt_solve_multi", _I, [_P, _I, _DP, _DP]),
+    ("kkt_solve_multi_device", _I, [_P, _I, _P, _P]),
     ("kkt_multiply", _I, [_P, _DP, _DP]),
     ("kkt_iterative_solve", _I, [_P, _DP, _DP, _DP, _D, _I]),
     ("kkt_iterative_solve_device", _I, [_P, _P, _P, _P, _D, _I]),

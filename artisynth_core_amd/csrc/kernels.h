@@ -12,6 +12,9 @@ constexpr int UPD_GROUP = 4;  // panels whose Schur updates are applied together
 // whole forward step (gather + rows) in one workgroup
 constexpr int FWD_SMALL_K = 32;
 constexpr int FWD_SMALL_M = 256;
+// right-hand sides one multi-RHS solve pass carries (vectors stored one after
+// another, each with its own leading dimension)
+constexpr int SOLVE_NRHS = 4;
 
 // Device copy of the supernodal structure (all arrays device pointers).
 struct DevSym {
@@ -104,6 +107,27 @@ void launch_dsolve(int n, const T* E0, const T* E1, const signed char* piv, cons
 template <typename T>
 void launch_bwd_cols(const SolveItem* items, int nitems, const DevSym& d, const T* F, T* wv,
                      T* xp, hipStream_t st);
+
+// the same for nr <= SOLVE_NRHS right-hand sides at once (vector j of a set at
+// j * ld; ldn, ldu, ldw: n-, update- and backward-vector sets)
+template <typename T>
+void launch_fwd_gather_m(const SolveItem* items, int nitems, const DevSym& d, const T* F, int nr, const T* bp,
+                         T* fp, T* y, T* u, long long ldn, long long ldu, hipStream_t st);
+template <typename T>
+void launch_fwd_rows_m(const SolveItem* items, int nitems, const DevSym& d, const T* F, int nr, const T* fp,
+                       T* y, T* u, long long ldn, long long ldu, hipStream_t st);
+template <typename T>
+void launch_dsolve_m(int n, int nr, const T* E0, const T* E1, const signed char* piv, const int* wpos,
+                     const T* y, T* wv, long long ldn, long long ldw, hipStream_t st);
+template <typename T>
+void launch_bwd_cols_m(const SolveItem* items, int nitems, const DevSym& d, const T* F, int nr, T* wv, T* xp,
+                       long long ldn, long long ldw, hipStream_t st);
+template <typename TI, typename TO>
+void launch_gather_perm_m(int n, int nr, const int* perm, const TI* in, long long ldi, TO* out, long long ldo,
+                          hipStream_t st);
+template <typename TI, typename TO>
+void launch_scatter_perm_m(int n, int nr, const int* perm, const TI* in, long long ldi, TO* out, long long ldo,
+                           hipStream_t st);
 
 // permutation / precision conversion: out[j] = (TO) in[perm[j]]  (gather)
 template <typename TI, typename TO>

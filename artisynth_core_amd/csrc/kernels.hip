@@ -1195,6 +1195,224 @@ __global__ __launch_bounds__(256) void k_bwd_cols(const SolveItem* __restrict__ 
 }
 
 // ------------------------------------------------------------------------
+// Multi-RHS triangular solves (LCP block solves, KKTSolver.java:1537-1588):
+// the same work items, launches and per-lane accumulation order as the
+// single-RHS kernels above, for nr <= SOLVE_NRHS right-hand sides at once, so
+// a front's factor data is read once per pass and each result equals the
+// single-RHS one.  Vector j of a set starts at j * ld (ldn: n-vectors, ldu:
+// update vectors, ldw: backward vectors; each with the spare entries the chunk
+// loads may run into).
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_gather_m(const SolveItem* __restrict__ items, DevSym d,
+                                                      const T* __restrict__ F, int nr,
+                                                      const T* __restrict__ bp, T* __restrict__ fp,
+                                                      T* __restrict__ y, T* __restrict__ u, long long ldn,
+                                                      long long ldu) {
+  __shared__ T g[SOLVE_NRHS][256];
+  __shared__ __align__(16) T fs[SOLVE_NRHS][FWD_SMALL_K + 4];
+  const SolveItem it = items[blockIdx.x];
+  const int k = it.k, m = it.m, first = it.first;
+  const int p = it.blk * 256 + threadIdx.x;
+  T acc[SOLVE_NRHS];
+#pragma unroll
+  for (int j = 0; j < SOLVE_NRHS; j++) acc[j] = (T)0;
+  int ip = 0;
+  if (p < m) {
+    const int* gp = d.gptr + it.goff + p;
+    const int e0 = gp[0], e1 = gp[1];
+    if (p < k) {
+#pragma unroll
+      for (int j = 0; j < SOLVE_NRHS; j++)
+        if (j < nr) acc[j] = bp[j * ldn + first + p];
+      ip = d.ilperm[first + p];
+    }
+    // children in ascending order (the single-RHS order), the index shared by
+    // every right-hand side
+    for (int e = e0; e < e1; e++) {
+      const int src = d.gsrc[e];
+#pragma unroll
+      for (int j = 0; j < SOLVE_NRHS; j++)
+        if (j < nr) acc[j] += u[j * ldu + src];
+    }
+  }
+  if (it.kind == 1) {
+#pragma unroll
+    for (int j = 0; j < SOLVE_NRHS; j++)
+      if (j < nr) {
+        if (p < k) fp[j * ldn + first + ip] = acc[j];
+        else if (p < m) u[j * ldu + it.uoff + p - k] = acc[j];
+      }
+    return;
+  }
+#pragma unroll
+  for (int j = 0; j < SOLVE_NRHS; j++) g[j][threadIdx.x] = acc[j];
+  __syncthreads();
+  if (p < k) {
+#pragma unroll
+    for (int j = 0; j < SOLVE_NRHS; j++) fs[j][ip] = acc[j];
+  }
+  __syncthreads();
+  if (p < m) {
+    const int len = min(p, k);
+    const T* rowp = F + it.foff + p;
+    T v[FWD_SMALL_K];
+#pragma unroll
+    for (int l = 0; l < FWD_SMALL_K; l++) v[l] = rowp[(size_t)(l < len ? l : 0) * m];
+#pragma unroll
+    for (int j = 0; j < SOLVE_NRHS; j++) {
+      if (j >= nr) break;
+      T a = (T)0;
+#pragma unroll
+      for (int l = 0; l < FWD_SMALL_K; l++) {
+        const T t = fma(v[l], fs[j][l], a);
+        a = (l < len) ? t : a;
+      }
+      if (p < k) y[j * ldn + first + p] = fs[j][p] + a;
+      else u[j * ldu + it.uoff + p - k] = g[j][threadIdx.x] - a;
+    }
+  }
+}
+
+// Lane sl's partial sums sum_{e < len} a[e] b_j[e] for nr vectors b_j = b + j ld,
+// over the lane's chunks in the order of seg_dot (groups of two chunks; the
+// chunk loads of a group issue before its products)
+template <typename T>
+__device__ inline void seg_dot_m(const T* __restrict__ a, const T* __restrict__ b, long long ld, int nr,
+                                 int len, int sl, int seg, T* acc) {
+  constexpr int W = 16 / sizeof(T), G = 2;
+  typedef typename V16<T>::v vt;
+  const int nc = (len + W - 1) / W;
+#pragma unroll
+  for (int j = 0; j < SOLVE_NRHS; j++) acc[j] = (T)0;
+  for (int q0 = sl; q0 < nc; q0 += G * seg) {
+    vt x[G], y[G][SOLVE_NRHS];
+#pragma unroll
+    for (int u = 0; u < G; u++) {
+      const int q = q0 + u * seg, qc = q < nc ? q : 0;
+      x[u] = ldv(a + qc * W);
+#pragma unroll
+      for (int j = 0; j < SOLVE_NRHS; j++) y[u][j] = ldv(b + (j < nr ? j : 0) * ld + qc * W);
+    }
+#pragma unroll
+    for (int u = 0; u < G; u++)
+#pragma unroll
+      for (int j = 0; j < SOLVE_NRHS; j++)
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+          const T t = fma(x[u][w], y[u][j][w], acc[j]);
+          acc[j] = ((q0 + u * seg) * W + w < len) ? t : acc[j];
+        }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_rows_m(const SolveItem* __restrict__ items, DevSym d,
+                                                    const T* __restrict__ F, int nr,
+                                                    const T* __restrict__ fp, T* __restrict__ y,
+                                                    T* __restrict__ u, long long ldn, long long ldu) {
+  const SolveItem it = items[blockIdx.x];
+  const int lg = it.lg;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int seg = 1 << lg, sl = lane & (seg - 1);
+  const int i = (it.blk * 4 + wv) * (64 >> lg) + (lane >> lg);
+  const int k = it.k, m = it.m, first = it.first;
+  const T* col = F + it.foff + (size_t)(i < m ? i : 0) * m;
+  const bool own = sl == 0 && i < m;
+  T base[SOLVE_NRHS];
+#pragma unroll
+  for (int j = 0; j < SOLVE_NRHS; j++) {
+    base[j] = (T)0;
+    if (own && j < nr) base[j] = (i < k) ? fp[j * ldn + first + i] : u[j * ldu + it.uoff + i - k];
+  }
+  const int len = (i < m) ? min(i, k) : 0;
+  T acc[SOLVE_NRHS];
+  seg_dot_m<T>(col, fp + first, ldn, nr, len, sl, seg, acc);
+#pragma unroll
+  for (int j = 0; j < SOLVE_NRHS; j++) {
+    const T a = seg_sum(acc[j], lg);
+    if (own && j < nr) {
+      if (i < k) y[j * ldn + first + i] = base[j] + a;
+      else u[j * ldu + it.uoff + i - k] = base[j] - a;
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_dsolve_m(int n, int nr, const T* __restrict__ E0,
+                                                  const T* __restrict__ E1,
+                                                  const signed char* __restrict__ piv,
+                                                  const int* __restrict__ wpos, const T* __restrict__ y,
+                                                  T* __restrict__ wv, long long ldn, long long ldw) {
+  const int g = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+  if (g >= n || j >= nr) return;
+  const int pc = piv[g];
+  const int pg = (pc == 2) ? g + 1 : (pc == 0 ? g - 1 : g);
+  wv[j * ldw + wpos[g]] = E0[g] * y[j * ldn + g] + E1[g] * y[j * ldn + pg];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_cols_m(const SolveItem* __restrict__ items, DevSym d,
+                                                    const T* __restrict__ F, int nr, T* wv,
+                                                    T* __restrict__ xp, long long ldn, long long ldw) {
+  __shared__ T part[SOLVE_NRHS][4];
+  const SolveItem it = items[blockIdx.x];
+  const int lg = it.lg;
+  const int seg = 1 << lg, sl = threadIdx.x & (seg - 1);
+  const int c = it.blk * (256 >> lg) + (threadIdx.x >> lg);
+  const int k = it.k, m = it.m, first = it.first;
+  const bool ok = c < k;
+  const int cq = ok ? c : 0;
+  const int lp = d.lperm[first + cq];
+  const T* w = wv + it.voff;
+  T zc[SOLVE_NRHS];
+#pragma unroll
+  for (int j = 0; j < SOLVE_NRHS; j++) zc[j] = w[(j < nr ? j : 0) * ldw + cq];
+  const T* col = F + it.foff + (size_t)cq * m + cq + 1;
+  T acc[SOLVE_NRHS];
+  seg_dot_m<T>(col, w + cq + 1, ldw, nr, ok ? m - c - 1 : 0, sl, seg, acc);
+#pragma unroll
+  for (int j = 0; j < SOLVE_NRHS; j++) acc[j] = seg_sum(acc[j], lg < 6 ? lg : 6);
+  if (lg > 6) {
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+      for (int j = 0; j < SOLVE_NRHS; j++) part[j][threadIdx.x >> 6] = acc[j];
+    }
+    __syncthreads();
+    const int w0 = (threadIdx.x >> lg) << (lg - 6);
+#pragma unroll
+    for (int j = 0; j < SOLVE_NRHS; j++) {
+      acc[j] = part[j][w0];
+      for (int q = 1; q < (seg >> 6); q++) acc[j] += part[j][w0 + q];
+    }
+  }
+  if (ok) {
+    const int g = first + lp;
+    const int e0 = d.push_ptr[g], e1 = d.push_ptr[g + 1];
+#pragma unroll
+    for (int j = 0; j < SOLVE_NRHS; j++) {
+      if (j >= nr) break;
+      const T x = zc[j] + acc[j];
+      if (sl == 0) xp[j * ldn + g] = x;
+      for (int e = e0 + sl; e < e1; e += seg) wv[j * ldw + d.push_pos[e]] = -x;
+    }
+  }
+}
+
+template <typename TI, typename TO>
+__global__ void k_gather_perm_m(int n, int nr, const int* __restrict__ perm, const TI* __restrict__ in,
+                                long long ldi, TO* __restrict__ out, long long ldo) {
+  const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+  if (i < n && j < nr) out[j * ldo + i] = (TO)in[j * ldi + perm[i]];
+}
+template <typename TI, typename TO>
+__global__ void k_scatter_perm_m(int n, int nr, const int* __restrict__ perm, const TI* __restrict__ in,
+                                 long long ldi, TO* __restrict__ out, long long ldo) {
+  const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+  if (i < n && j < nr) out[j * ldo + perm[i]] = (TO)in[j * ldi + i];
+}
+
+// ------------------------------------------------------------------------
 // Device-side value assembly (CuDssSolver.*DeviceValues, cudssBridgeKernels.cu:57-380).
 // Each call computes its contributions into a scratch array, then one thread
 // per distinct CSR slot adds them in a fixed order (the plan lists the
@@ -1711,6 +1929,46 @@ void launch_bwd_cols(const SolveItem* items, int nitems, const DevSym& d, const 
                      hipStream_t st) {
   if (nitems > 0) hipLaunchKernelGGL(k_bwd_cols<T>, dim3(nitems), dim3(256), 0, st, items, d, F, wv, xp);
 }
+template <typename T>
+void launch_fwd_gather_m(const SolveItem* items, int nitems, const DevSym& d, const T* F, int nr, const T* bp,
+                         T* fp, T* y, T* u, long long ldn, long long ldu, hipStream_t st) {
+  if (nitems > 0)
+    hipLaunchKernelGGL(k_fwd_gather_m<T>, dim3(nitems), dim3(256), 0, st, items, d, F, nr, bp, fp, y, u, ldn,
+                       ldu);
+}
+template <typename T>
+void launch_fwd_rows_m(const SolveItem* items, int nitems, const DevSym& d, const T* F, int nr, const T* fp,
+                       T* y, T* u, long long ldn, long long ldu, hipStream_t st) {
+  if (nitems > 0)
+    hipLaunchKernelGGL(k_fwd_rows_m<T>, dim3(nitems), dim3(256), 0, st, items, d, F, nr, fp, y, u, ldn, ldu);
+}
+template <typename T>
+void launch_dsolve_m(int n, int nr, const T* E0, const T* E1, const signed char* piv, const int* wpos,
+                     const T* y, T* wv, long long ldn, long long ldw, hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_dsolve_m<T>, dim3(nblk(n, 256), nr), dim3(256), 0, st, n, nr, E0, E1, piv, wpos, y,
+                       wv, ldn, ldw);
+}
+template <typename T>
+void launch_bwd_cols_m(const SolveItem* items, int nitems, const DevSym& d, const T* F, int nr, T* wv, T* xp,
+                       long long ldn, long long ldw, hipStream_t st) {
+  if (nitems > 0)
+    hipLaunchKernelGGL(k_bwd_cols_m<T>, dim3(nitems), dim3(256), 0, st, items, d, F, nr, wv, xp, ldn, ldw);
+}
+template <typename TI, typename TO>
+void launch_gather_perm_m(int n, int nr, const int* perm, const TI* in, long long ldi, TO* out, long long ldo,
+                          hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL((k_gather_perm_m<TI, TO>), dim3(nblk(n, 256), nr), dim3(256), 0, st, n, nr, perm, in,
+                       ldi, out, ldo);
+}
+template <typename TI, typename TO>
+void launch_scatter_perm_m(int n, int nr, const int* perm, const TI* in, long long ldi, TO* out, long long ldo,
+                           hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL((k_scatter_perm_m<TI, TO>), dim3(nblk(n, 256), nr), dim3(256), 0, st, n, nr, perm, in,
+                       ldi, out, ldo);
+}
 template <typename TI, typename TO>
 void launch_gather_perm(int n, const int* perm, const TI* in, TO* out, hipStream_t st) {
   hipLaunchKernelGGL((k_gather_perm<TI, TO>), dim3(nblk(n, 256)), dim3(256), 0, st, n, perm, in, out);
@@ -1857,5 +2115,20 @@ template void launch_gather_perm<double, double>(int, const int*, const double*,
 template void launch_gather_perm<double, float>(int, const int*, const double*, float*, hipStream_t);
 template void launch_scatter_perm<double, double>(int, const int*, const double*, double*, hipStream_t);
 template void launch_scatter_perm<float, double>(int, const int*, const float*, double*, hipStream_t);
+#define KKT_INST_M(T)                                                                                        \
+  template void launch_fwd_gather_m<T>(const SolveItem*, int, const DevSym&, const T*, int, const T*, T*, T*, \
+                                       T*, long long, long long, hipStream_t);                               \
+  template void launch_fwd_rows_m<T>(const SolveItem*, int, const DevSym&, const T*, int, const T*, T*, T*,   \
+                                     long long, long long, hipStream_t);                                     \
+  template void launch_dsolve_m<T>(int, int, const T*, const T*, const signed char*, const int*, const T*,   \
+                                   T*, long long, long long, hipStream_t);                                   \
+  template void launch_bwd_cols_m<T>(const SolveItem*, int, const DevSym&, const T*, int, T*, T*, long long,  \
+                                     long long, hipStream_t);                                                \
+  template void launch_gather_perm_m<double, T>(int, int, const int*, const double*, long long, T*,          \
+                                                long long, hipStream_t);                                     \
+  template void launch_scatter_perm_m<T, double>(int, int, const int*, const T*, long long, double*,         \
+                                                 long long, hipStream_t);
+KKT_INST_M(float)
+KKT_INST_M(double)
 
 }  // namespace kkt

@@ -124,6 +124,13 @@ struct kkt_solver {
   // solve scratch, as T: bp, fp, y, xp (n each), u (u_elems + 16), w (v_elems + 16)
   void* d_svec = nullptr;
   size_t svec_bytes = 0;
+  // multi-RHS solves: the solve vectors of SOLVE_NRHS right-hand sides (as T) and
+  // six FP64 sets of SOLVE_NRHS n-vectors (graph input, graph output, residual,
+  // correction, host-path right-hand sides, host-path solutions)
+  void* d_msvec = nullptr;
+  size_t msvec_bytes = 0;
+  double* d_mvec = nullptr;
+  size_t mvec_bytes = 0;
   // FP64 vectors for refinement / BiCGStab / host copies
   double* d_vec = nullptr;  // 14 vectors of n (0-8 BiCGStab, 10-11 host copies, 12-13 graph I/O)
   double* d_partial = nullptr;
@@ -150,6 +157,8 @@ struct kkt_solver {
   hipGraphExec_t fexec[2] = {nullptr, nullptr};  // captured factorizations
   // captured BiCGStab half-iterations per precision: 0 first head, 1 head, 2 tail
   hipGraphExec_t bexec[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  // captured multi-RHS triangular solves per precision and count of right-hand sides
+  hipGraphExec_t mexec[2][SOLVE_NRHS] = {};
   bool graphs_on = true;
   void drop_graphs() {
     for (int q = 0; q < 2; q++) {
@@ -157,6 +166,8 @@ struct kkt_solver {
       if (fexec[q]) { (void)hipGraphExecDestroy(fexec[q]); fexec[q] = nullptr; }
       for (int w = 0; w < 3; w++)
         if (bexec[q][w]) { (void)hipGraphExecDestroy(bexec[q][w]); bexec[q][w] = nullptr; }
+      for (int w = 0; w < SOLVE_NRHS; w++)
+        if (mexec[q][w]) { (void)hipGraphExecDestroy(mexec[q][w]); mexec[q][w] = nullptr; }
     }
   }
   // Captures `fn` (launches on st only) into exec, or returns false.
@@ -212,8 +223,11 @@ struct kkt_solver {
     if (d_W64) (void)hipFree(d_W64);
     if (d_W32) (void)hipFree(d_W32);
     if (d_svec) (void)hipFree(d_svec);
+    if (d_msvec) (void)hipFree(d_msvec);
+    if (d_mvec) (void)hipFree(d_mvec);
     d_F64 = nullptr; d_F32 = nullptr; d_W64 = nullptr; d_W32 = nullptr; d_svec = nullptr;
-    svec_bytes = 0;
+    d_msvec = nullptr; d_mvec = nullptr;
+    svec_bytes = msvec_bytes = mvec_bytes = 0;
   }
   void free_plan(AddPlan& p) {
     DevOpLock lk(g_capture_mu);
@@ -972,6 +986,162 @@ struct kkt_solver {
     tbeg(TC_VEC);
     launch_scatter_perm<T, double>(n, d_perm, xp, d_x, st);
     tend();
+  }
+  // ---- multi-RHS solves (CuDssSolver.java:814-826, cudssBridge.cc:1486-1511;
+  // the LCP block solves of KKTSolver.java:1537-1588) ----
+  long long m_ldn() const { return (long long)n + 16; }
+  int ensure_multi() {
+    const size_t esz = std::max(sizeof(double), sizeof(float));
+    const size_t need = (size_t)SOLVE_NRHS * (4 * (size_t)m_ldn() + (size_t)u_elems + 16 + (size_t)v_elems + 16) * esz;
+    const size_t needv = (size_t)6 * SOLVE_NRHS * (size_t)n * sizeof(double);
+    if (msvec_bytes >= need && mvec_bytes >= needv) return KKT_OK;
+    DevOpLock lk(g_capture_mu);
+    drop_graphs();
+    if (d_msvec) (void)hipFree(d_msvec);
+    if (d_mvec) (void)hipFree(d_mvec);
+    d_msvec = nullptr;
+    d_mvec = nullptr;
+    msvec_bytes = mvec_bytes = 0;
+    if (hipMalloc(&d_msvec, need) != hipSuccess || hipMalloc((void**)&d_mvec, needv) != hipSuccess)
+      return fail(KKT_ERR_ALLOC, "hipMalloc failed (multi-RHS solve)");
+    msvec_bytes = need;
+    mvec_bytes = needv;
+    return KKT_OK;
+  }
+  double* mset(int i) { return d_mvec + (size_t)i * SOLVE_NRHS * n; }
+  // nr right-hand sides B (column j at B + j ldb) to X, one launch sequence
+  template <typename T>
+  void run_tri_solve_multi(const T* F, const double* B, long long ldb, double* X, long long ldx, int nr) {
+    const long long ldn = m_ldn(), ldu = u_elems + 16, ldw = v_elems + 16;
+    T* sv = (T*)d_msvec;
+    T *bp = sv, *fp = bp + SOLVE_NRHS * ldn, *y = fp + SOLVE_NRHS * ldn, *xp = y + SOLVE_NRHS * ldn,
+      *u = xp + SOLVE_NRHS * ldn, *wv = u + SOLVE_NRHS * ldu;
+    const T* E0 = F + front_elems;
+    const T* E1 = E0 + n;
+    const int nl = S.nlevels;
+    tbeg(TC_VEC);
+    launch_gather_perm_m<double, T>(n, nr, d_perm, B, ldb, bp, ldn, st);
+    tend();
+    for (int L = 0; L < nl; L++) {
+      tbeg(TC_SLEVEL);
+      launch_fwd_gather_m<T>(d_fg + fg_l[L].off, fg_l[L].cnt, ds, F, nr, bp, fp, y, u, ldn, ldu, st);
+      launch_fwd_rows_m<T>(d_fr + fr_l[L].off, fr_l[L].cnt, ds, F, nr, fp, y, u, ldn, ldu, st);
+      tend();
+    }
+    tbeg(TC_SLEVEL);
+    launch_dsolve_m<T>(n, nr, E0, E1, d_piv, ds.wpos, y, wv, ldn, ldw, st);
+    tend();
+    for (int L = nl - 1; L >= 0; L--) {
+      tbeg(TC_SGEMV);
+      launch_bwd_cols_m<T>(d_bc + bc_l[L].off, bc_l[L].cnt, ds, F, nr, wv, xp, ldn, ldw, st);
+      tend();
+    }
+    tbeg(TC_VEC);
+    launch_scatter_perm_m<T, double>(n, nr, d_perm, xp, ldn, X, ldx, st);
+    tend();
+  }
+  void tri_solve_multi_eager(const double* B, long long ldb, double* X, long long ldx, int nr) {
+    if (fprec == KKT_PREC_FP64) run_tri_solve_multi<double>(d_F64, B, ldb, X, ldx, nr);
+    else run_tri_solve_multi<float>(d_F32, B, ldb, X, ldx, nr);
+  }
+  // captured once per precision and count, reading set 0 and writing set 1
+  // (ld = n); other buffers are copied in and out
+  void tri_solve_multi(const double* B, double* X, int nr) {
+    const int solve_cls = (1 << TC_SLEVEL) | (1 << TC_SGEMV) | (1 << TC_VEC);
+    if (!graphs_on || (tm_on && (tm_mask & solve_cls))) { tri_solve_multi_eager(B, n, X, n, nr); return; }
+    const int q = (fprec == KKT_PREC_FP64) ? 0 : 1;
+    double* gin = mset(0);
+    double* gout = mset(1);
+    if (!mexec[q][nr - 1] && !capture(mexec[q][nr - 1], [&]() { tri_solve_multi_eager(gin, n, gout, n, nr); })) {
+      graphs_on = false;
+      tri_solve_multi_eager(B, n, X, n, nr);
+      return;
+    }
+    const size_t bytes = (size_t)nr * n * sizeof(double);
+    if (B != gin) (void)hipMemcpyAsync(gin, B, bytes, hipMemcpyDeviceToDevice, st);
+    (void)hipGraphLaunch(mexec[q][nr - 1], st);
+    if (X != gout) (void)hipMemcpyAsync(X, gout, bytes, hipMemcpyDeviceToDevice, st);
+  }
+  // dot products of nr column pairs into pinned host memory (one synchronization)
+  int dots_multi(const double* A, const double* Bv, int nr, double* out) {
+    tbeg(TC_VEC);
+    for (int j = 0; j < nr; j++) launch_dot(n, A + (size_t)j * n, Bv + (size_t)j * n, d_partial, d_scal + j, st);
+    tend();
+    if (hipMemcpyAsync(h_pin, d_scal, nr * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return fail(KKT_ERR_REDUCTION, "reduction failed");
+    for (int j = 0; j < nr; j++) out[j] = h_pin[j];
+    return KKT_OK;
+  }
+  // refined solves of nrhs columns (ld = n) on device arrays, SOLVE_NRHS at a
+  // time.  Per column the same steps as solve_device: FP64 factors take ir_steps
+  // refinement steps, FP32 factors refine to refine_tol; a column's correction is
+  // applied only while it is still refining, so each result equals its
+  // single-RHS solve.  A stalled column falls back as solve_device does (FP64
+  // refactor; then the chunk's columns are solved one by one).
+  int solve_multi_device(int nrhs, const double* d_B, double* d_X) {
+    if (state != FACTORED) return fail(KKT_ERR_STATE, "solve called before factor");
+    int rc = ensure_multi();
+    if (rc) return rc;
+    double* R = mset(2);
+    double* D = mset(3);
+    for (int c0 = 0; c0 < nrhs; c0 += SOLVE_NRHS) {
+      const int nr = std::min(SOLVE_NRHS, nrhs - c0);
+      const double* B = d_B + (size_t)c0 * n;
+      double* X = d_X + (size_t)c0 * n;
+      tri_solve_multi(B, X, nr);
+      const int want = (fprec == KKT_PREC_FP32_REFINE) ? refine_max : ir_steps;
+      last_refine_steps = 0;
+      if (want <= 0) continue;
+      double bn[SOLVE_NRHS], prev[SOLVE_NRHS], rn2[SOLVE_NRHS];
+      int stall[SOLVE_NRHS];
+      bool active[SOLVE_NRHS];
+      if ((rc = dots_multi(B, B, nr, bn))) return rc;
+      for (int j = 0; j < nr; j++) {
+        bn[j] = std::sqrt(bn[j]);
+        if (bn[j] == 0.0) bn[j] = 1.0;
+        prev[j] = 0.0;
+        stall[j] = 0;
+        active[j] = true;
+      }
+      bool fallback = false;
+      for (int step = 0;; step++) {
+        for (int j = 0; j < nr; j++)
+          if (active[j]) spmv(X + (size_t)j * n, B + (size_t)j * n, R + (size_t)j * n, 1);
+        if ((rc = dots_multi(R, R, nr, rn2))) return rc;
+        int nact = 0;
+        for (int j = 0; j < nr; j++) {
+          if (!active[j]) continue;
+          const double rel = std::sqrt(rn2[j]) / bn[j];
+          last_relres = rel;
+          if (fprec == KKT_PREC_FP32_REFINE) {
+            if (rel <= refine_tol) { active[j] = false; continue; }
+            if (step > 0 && rel > 0.5 * prev[j]) stall[j]++; else stall[j] = 0;
+            if (step >= refine_max || stall[j] >= 2 || !std::isfinite(rel)) fallback = true;
+          } else if (step >= ir_steps) {
+            active[j] = false;
+            continue;
+          }
+          prev[j] = rel;
+          nact++;
+        }
+        if (fallback || nact == 0) break;
+        tri_solve_multi(R, D, nr);
+        tbeg(TC_VEC);
+        for (int j = 0; j < nr; j++)
+          if (active[j]) launch_axpby(n, 1.0, D + (size_t)j * n, 1.0, X + (size_t)j * n, st);
+        tend();
+        last_refine_steps++;
+      }
+      if (fallback) {
+        // as solve_device: FP64 refactor with the same values, then each column
+        for (int j = 0; j < nr; j++)
+          if ((rc = solve_device(B + (size_t)j * n, X + (size_t)j * n))) return rc;
+      }
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) return fail(KKT_ERR_SOLVE, "solve kernels failed");
+    tflush();
+    return KKT_OK;
   }
   // ---- solve-shape autotuner ----
   // Candidates: the nine uniform (forward, backward) elements-per-lane pairs from
@@ -1796,11 +1966,34 @@ int kkt_solve_device(kkt_solver* h, const double* d_b, double* d_x) {
 int kkt_solve_multi(kkt_solver* h, int nrhs, const double* B, double* X) {
   CHECK_H(h);
   if (nrhs <= 0 || !B || !X) return h->fail(KKT_ERR_ARGUMENT, "nrhs must be positive");
-  for (int c = 0; c < nrhs; c++) {
-    int rc = kkt_solve(h, B + (size_t)c * h->n, X + (size_t)c * h->n);
-    if (rc) return rc;
+  if (h->state != kkt_solver::FACTORED) return h->fail(KKT_ERR_STATE, "solve called before factor");
+  int rc = h->ensure_multi();
+  if (rc) return rc;
+  double* dB = h->mset(4);
+  double* dX = h->mset(5);
+  for (int c0 = 0; c0 < nrhs; c0 += SOLVE_NRHS) {
+    const int nr = std::min(SOLVE_NRHS, nrhs - c0);
+    const size_t bytes = (size_t)nr * h->n * sizeof(double);
+    if (hipMemcpyAsync(dB, B + (size_t)c0 * h->n, bytes, hipMemcpyHostToDevice, h->st) != hipSuccess)
+      return h->fail(KKT_ERR_COPY, "hipMemcpyAsync (B) failed");
+    if ((rc = h->solve_multi_device(nr, dB, dX))) return rc;
+    if (hipMemcpyAsync(X + (size_t)c0 * h->n, dX, bytes, hipMemcpyDeviceToHost, h->st) != hipSuccess ||
+        hipStreamSynchronize(h->st) != hipSuccess)
+      return h->fail(KKT_ERR_COPY, "hipMemcpyAsync (X) failed");
   }
+  h->ok();
   return KKT_OK;
+}
+
+int kkt_solve_multi_device(kkt_solver* h, int nrhs, const double* d_B, double* d_X) {
+  CHECK_H(h);
+  if (nrhs <= 0 || !d_B || !d_X) return h->fail(KKT_ERR_ARGUMENT, "nrhs must be positive");
+  const size_t bytes = (size_t)nrhs * h->n * sizeof(double);
+  const char *b0 = (const char*)d_B, *x0 = (const char*)d_X;
+  if (b0 < x0 + bytes && x0 < b0 + bytes) return h->fail(KKT_ERR_ARGUMENT, "aliased right-hand sides and solutions");
+  int rc = h->solve_multi_device(nrhs, d_B, d_X);
+  if (!rc) h->ok();
+  return rc;
 }
 
 int kkt_multiply(kkt_solver* h, const double* x, double* y) {

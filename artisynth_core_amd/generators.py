@@ -513,7 +513,7 @@ def rigid_contact_system(ndof, seed, h=0.01):
 
 
 def batch_mix(nsys=256, seed=0xBA7C, dof_lo=300, dof_hi=4000):
-    """nsys independent KKT systems with DOF ~ U[dof_lo, dof_hi]."""
+    """The batching workload: nsys independent KKT systems with DOF ~ U[dof_lo, dof_hi]."""
     rng = np.random.default_rng(seed)
     systems = []
     for s in range(nsys):
@@ -586,7 +586,7 @@ def serving_instance(kind, ndof, seed):
 
 
 def serving_mix(ninst=64, seed=0x5E7E, dof_lo=1000, dof_hi=90000):
-    """ninst instances, DOF log-uniform in [dof_lo, dof_hi], kinds cycling
+    """The serving workload: ninst instances, DOF log-uniform in [dof_lo, dof_hi], kinds cycling
     beam / tet / rigid."""
     rng = np.random.default_rng(seed)
     out = []

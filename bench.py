@@ -112,11 +112,13 @@ def main():
     ap.add_argument("--nsys", type=int, default=20, help="systems in the step sequence")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--grid", default="100x16x16")
-    ap.add_argument("--mode", default="bench", choices=["bench", "pmc", "batch", "serve", "precision"],
+    ap.add_argument("--mode", default="bench", choices=["bench", "pmc", "batch", "serve", "precision", "multi"],
                     help="pmc: one FP32 factor + solve and a 256 MiB copy, for counter runs; "
                          "batch: the 256-system mix, one batched launch vs per-system launches; "
                          "precision: the ~210k-DOF Mooney-Rivlin tet box, direct FP32-MFMA+refine "
-                         "vs FP64 solves")
+                         "vs FP64 solves; multi: block solves of --nrhs right-hand sides on the "
+                         "beam's first KKT, batched vs one at a time")
+    ap.add_argument("--nrhs", type=int, default=64, help="multi: right-hand sides per block solve")
     ap.add_argument("--tet", default="121x24x24", help="precision: Delaunay lattice points")
     ap.add_argument("--oracle", action="store_true",
                     help="precision: also run the CPU oracle on step 0 (minutes at full size)")
@@ -150,6 +152,8 @@ def main():
     if args.mode == "precision":
         return precision_run(tuple(int(v) for v in args.tet.split("x")), min(args.nsys, 4),
                              args.warmup, args.oracle)
+    if args.mode == "multi":
+        return multi_run(nx, ny, nz, args.nrhs, max(args.steps // 4, 2), args.warmup)
     t0 = time.time()
     beam, V, B = G.large_beam_sequence(nsteps=args.nsys, nx=nx, ny=ny, nz=nz,
                                        width=0.16 * ny / 16)
@@ -314,8 +318,89 @@ def pmc_run(nx, ny, nz):
     s.dispose()
 
 
+def multi_run(nx, ny, nz, nrhs, reps, warmup):
+    """Block solves with many right-hand sides (the LCP callers' solveMG of
+    KKTSolver.java:1537-1588) on the hex beam's first KKT system: kkt_solve_multi_device
+    (four right-hand sides per launch sequence, refinement per column) against the
+    same right-hand sides solved by one kkt_solve_device call each, for both factor
+    precisions.  Every column is checked equal to its single solve, and residuals
+    are recomputed on the host."""
+    import torch
+    import scipy.sparse as sp
+    from artisynth_core_amd import KKT_PREC_FP32_REFINE, KKT_PREC_FP64
+    from artisynth_core_amd import generators as G
+    from artisynth_core_amd._lib import lib
+    from artisynth_core_amd._lib import ptr as nptr
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    beam, V, Bs = G.large_beam_sequence(nsteps=1, nx=nx, ny=ny, nz=nz, width=0.16 * ny / 16)
+    n, nnz = beam.n, beam.nnz
+    log("generated n=%d nnz=%d in %.1fs" % (n, nnz, time.time() - t0))
+    ro = np.ascontiguousarray(beam.rowOffs, np.int32)
+    ci = np.ascontiguousarray(beam.colIdxs, np.int32)
+    v0 = np.ascontiguousarray(V[0])
+    U = sp.csr_matrix((v0, ci, ro), shape=(n, n))
+    A = U + sp.triu(U, 1).T
+    rng = np.random.default_rng(7)
+    Bh = rng.uniform(-1.0, 1.0, (nrhs, n))
+    Bh[0] = Bs[0]
+    dB = torch.from_numpy(np.ascontiguousarray(Bh.ravel())).to(dev)
+    dXm = torch.zeros(n * nrhs, dtype=torch.float64, device=dev)
+    dXs = torch.zeros(n * nrhs, dtype=torch.float64, device=dev)
+    dv = torch.from_numpy(v0).to(dev)
+    L = lib()
+    out = {}
+    for prec, name in ((KKT_PREC_FP32_REFINE, "fp32_mfma_refine"), (KKT_PREC_FP64, "fp64")):
+        h = L.kkt_create()
+        L.kkt_set_precision(h, prec)
+        L.kkt_set_refine_tol(h, 1e-10, 10)
+        if prec == KKT_PREC_FP64:
+            L.kkt_set_ir_steps(h, 1)
+        assert L.kkt_set_pattern(h, n, nnz, nptr(ro, ctypes.c_int), nptr(ci, ctypes.c_int), 1) == 0
+        assert L.kkt_analyze_values(h, nptr(v0, ctypes.c_double)) == 0, L.kkt_last_error(h)
+        assert L.kkt_factor_device(h, dv.data_ptr()) == 0, L.kkt_last_error(h)
+
+        def multi():
+            assert L.kkt_solve_multi_device(h, nrhs, dB.data_ptr(), dXm.data_ptr()) == 0, L.kkt_last_error(h)
+
+        def single():
+            for j in range(nrhs):
+                assert L.kkt_solve_device(h, dB.data_ptr() + 8 * n * j, dXs.data_ptr() + 8 * n * j) == 0
+
+        for _ in range(max(warmup, 1)):
+            multi()
+            single()
+        torch.cuda.synchronize()
+        t = {}
+        for key, fn in (("multi", multi), ("single", single)):
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            torch.cuda.synchronize()
+            t[key] = (time.perf_counter() - t0) / reps
+        Xm = dXm.cpu().numpy().reshape(nrhs, n)
+        Xs = dXs.cpu().numpy().reshape(nrhs, n)
+        equal = bool(np.array_equal(Xm, Xs))
+        res = max(float(np.linalg.norm(A @ Xm[j] - Bh[j]) / np.linalg.norm(Bh[j])) for j in range(nrhs))
+        out[name] = {"rhs_per_s": nrhs / t["multi"], "single_rhs_per_s": nrhs / t["single"],
+                     "speedup": t["single"] / t["multi"], "ms_per_block": t["multi"] * 1e3,
+                     "equal_to_single": equal, "max_rel_residual": res}
+        log("%s: %d right-hand sides in %.2f ms batched (%.0f RHS/s) vs %.2f ms one by one "
+            "(%.0f RHS/s): %.2fx, equal to single %s, max resid %.2e"
+            % (name, nrhs, t["multi"] * 1e3, nrhs / t["multi"], t["single"] * 1e3, nrhs / t["single"],
+               t["single"] / t["multi"], equal, res))
+        L.kkt_destroy(h)
+    print(json.dumps({"metric": "multi-RHS block solves/s (right-hand sides per second)",
+                      "value": out["fp32_mfma_refine"]["rhs_per_s"], "unit": "RHS/s",
+                      "higher_is_better": True, "n_gpus": 1, "reps": reps, "warmup": warmup,
+                      "dtype": "fp32+fp64", "data": "synthetic (seeded neo-Hookean hex beam, random RHS)",
+                      "config": {"workload": "block solve of %d RHS on the constrained hex beam %dx%dx%d, n=%d"
+                                 % (nrhs, nx, ny, nz, n)},
+                      "results": out}), flush=True)
+
+
 def batch_run(nsys, reps, warmup):
-    """nsys independent KKT systems (DOF ~ U[300,4000], rigid bodies + tet
+    """The batching workload: nsys independent KKT systems (DOF ~ U[300,4000], rigid bodies + tet
     blocks + contact equalities, FP64).  One batched factor + solve (one captured graph
     each) against the same systems factored and solved one handle at a time."""
     import torch
@@ -404,7 +489,7 @@ def batch_run(nsys, reps, warmup):
 
 
 def serve_run(ninst, nreq, rate, nworkers, budget_gb):
-    """ninst instances (1k-90k DOF, beam/tet/rigid), a Poisson stream of
+    """The serving workload: ninst instances (1k-90k DOF, beam/tet/rigid), a Poisson stream of
     step requests at `rate` req/s, HBM-budget admission, FP32-MFMA + FP64 refinement
     (1e-10 relative residual), p50/p99 latency from submission to completion."""
     import scipy.sparse as sp
@@ -471,7 +556,7 @@ def serve_run(ninst, nreq, rate, nworkers, budget_gb):
 
 
 def precision_run(dims, nsteps, warmup, with_oracle):
-    """seeded Delaunay tet box (0.1 x 0.02 x 0.02 m, ~210k DOF at the
+    """The precision workload: seeded Delaunay tet box (0.1 x 0.02 x 0.02 m, ~210k DOF at the
     default lattice), Mooney-Rivlin c10 = 1e3, c01 = 0, kappa = 1e5, attached end.
     Every step is a direct solve: FP32 fronts on MFMA with FP64 refinement to the
     metric's 1e-8 relative residual, against FP64 fronts with one refinement step;

@@ -187,8 +187,14 @@ int kkt_factor_device_values(kkt_solver* h);
 int kkt_solve(kkt_solver* h, const double* b, double* x);
 /* Same on device arrays. */
 int kkt_solve_device(kkt_solver* h, const double* d_b, double* d_x);
-/* CuDssSolver.doSolveMulti (cudssBridge.cc:1454-1523): column-major n x nrhs. */
+/* CuDssSolver.doSolveMulti (cudssBridge.cc:1454-1523; CuDssSolver.java:814-826):
+ * column-major n x nrhs.  The right-hand sides are solved four at a time in one
+ * launch sequence (each front's factor data read once per four), with the
+ * configured refinement per column; each column's result equals its kkt_solve
+ * result. */
 int kkt_solve_multi(kkt_solver* h, int nrhs, const double* B, double* X);
+/* Same on device arrays (B and X must not overlap). */
+int kkt_solve_multi_device(kkt_solver* h, int nrhs, const double* d_B, double* d_X);
 /* CuDssSolver.doMultiply (CuDssJNI.cc:654-675; cudssBridge.cc:1419-1452;
  * CuDssSolver.java:443-465): y = A x with the matrix in the device value buffer
  * (the last factor's or iterative solve's values, or device assembly).  Needs a

@@ -280,6 +280,95 @@ def test_serving_alone_equals_mix(gpu):
 
 
 @pytest.mark.parametrize("prec", [KKT_PREC_FP64, KKT_PREC_FP32_REFINE])
+def test_multi_rhs_solve_equals_single_solves(gpu, prec):
+    # KKTSolver block solves (KKTSolver.java:1537-1588) through kkt_solve_multi:
+    # the right-hand sides go through the batched triangular solve four at a time
+    # (partial last group included), with per-column refinement; every column must
+    # equal its single-RHS solve bitwise (same launches, same accumulation order)
+    _say("multi-RHS vs single prec=%d" % prec)
+    import ctypes
+    import torch
+    from artisynth_core_amd._lib import ptr
+    beam = G.HexBeam(24, 8, 8, 1.0, 0.2, material=("neohookean", 3.4e4, 1.7e6), h=0.005)
+    u = np.random.default_rng(31).uniform(-1e-4, 1e-4, (beam.nn, 3))
+    vals, b = beam.system(u, np.zeros_like(u), beam.gravity_loads())
+    rng = np.random.default_rng(32)
+    n = beam.n
+    s = MI355XSolver(precision=prec)
+    try:
+        if prec == KKT_PREC_FP64:
+            s.setIterativeRefinementSteps(1)
+        s.analyzeArrays(vals, beam.colIdxs, beam.rowOffs, n, Matrix.SYMMETRIC)
+        s.factor(vals)
+        for nrhs in (1, 3, 4, 5, 9):
+            B = rng.uniform(-1, 1, n * nrhs)
+            B[:n] = b
+            X = np.zeros(n * nrhs)
+            s.solve(X, B, nrhs)
+            for j in range(nrhs):
+                xs = np.zeros(n)
+                s.solve(xs, B[j * n:(j + 1) * n])
+                assert np.array_equal(X[j * n:(j + 1) * n], xs), (nrhs, j)
+                assert rel_residual(beam.rowOffs, beam.colIdxs, vals, xs, B[j * n:(j + 1) * n]) <= 1e-8
+        # device arrays
+        nrhs = 6
+        B = rng.uniform(-1, 1, n * nrhs)
+        dB = torch.tensor(B, device="cuda")
+        dX = torch.zeros(n * nrhs, dtype=torch.float64, device="cuda")
+        rc = s._L.kkt_solve_multi_device(s._h, nrhs, ctypes.c_void_p(dB.data_ptr()), ctypes.c_void_p(dX.data_ptr()))
+        assert rc == 0
+        X = dX.cpu().numpy()
+        for j in range(nrhs):
+            xs = np.zeros(n)
+            s.solve(xs, B[j * n:(j + 1) * n])
+            assert np.array_equal(X[j * n:(j + 1) * n], xs), j
+        # overlapping input and output are rejected
+        assert s._L.kkt_solve_multi_device(s._h, nrhs, ctypes.c_void_p(dB.data_ptr()),
+                                           ctypes.c_void_p(dB.data_ptr() + 8 * n)) < 0
+    finally:
+        s.dispose()
+
+
+@pytest.mark.parametrize("graphs", [1, 0])
+def test_iterative_solve_iteration_limit(gpu, graphs):
+    # kkt_iterative_solve with maxIter at, and one below, the iterations the solve
+    # needs: converged with that count, then not converged (-13); the queued
+    # half-iterations past the limit or past convergence must not change x
+    _say("iterative iteration limit graphs=%d" % graphs)
+    import ctypes
+    from artisynth_core_amd._lib import ptr
+    beam, vals, b = G.small_beam(26)
+    s = MI355XSolver()
+    try:
+        s._L.kkt_set_graphs(s._h, graphs)
+        s.analyzeArrays(vals, beam.colIdxs, beam.rowOffs, beam.n, Matrix.SYMMETRIC)
+        s.factor(vals)
+        # entry-wise perturbation (a uniform scale would leave the factor exact)
+        v2 = vals * np.random.default_rng(21).uniform(0.9, 1.1, vals.shape)
+        vv, bb = np.ascontiguousarray(v2), np.ascontiguousarray(b)
+
+        def run(max_iter):
+            x = np.zeros(beam.n)
+            rc = s._L.kkt_iterative_solve(s._h, ptr(vv, ctypes.c_double), ptr(bb, ctypes.c_double),
+                                          ptr(x, ctypes.c_double), 1e-10, max_iter)
+            return rc, x
+
+        need, x_need = run(32)
+        assert need >= 2
+        assert rel_residual(beam.rowOffs, beam.colIdxs, v2, x_need, b) <= 1e-10
+        rc, x = run(need)
+        assert rc == need
+        assert np.array_equal(x, x_need)
+        rc, _ = run(need - 1)
+        assert rc == -13
+        # the handle still solves afterwards
+        rc, x = run(32)
+        assert rc == need and np.array_equal(x, x_need)
+    finally:
+        s.dispose()
+
+
+@pytest.mark.parametrize("prec", [KKT_PREC_FP64, KKT_PREC_FP32_REFINE])
 def test_graph_replays_match_eager_launches(gpu, prec):
     # the factor, solve and BiCGStab launch sequences are captured once into HIP
     # graphs and replayed; every replay (with eager work of another handle in
