@@ -14,7 +14,7 @@ constexpr int FWD_SMALL_K = 32;
 constexpr int FWD_SMALL_M = 256;
 // right-hand sides one multi-RHS solve pass carries (vectors stored one after
 // another, each with its own leading dimension)
-constexpr int SOLVE_NRHS = 4;
+constexpr int SOLVE_NRHS = 8;
 
 // Device copy of the supernodal structure (all arrays device pointers).
 struct DevSym {
@@ -147,6 +147,14 @@ void launch_spmv(int n, const int* rowptr, const int* col, const double* val, co
 int spmv_blocks(int n);
 int vec_blocks(int n);
 void launch_reduce2(const double* p1, const double* p2, int np, double* out1, double* out2, hipStream_t st);
+// Y_j = B_j - A X_j and out[j] = (A_j, B_j) for nr <= SOLVE_NRHS columns (column j at
+// j * ld), each column's value equal to launch_spmv (mode 1) / launch_dot's;
+// partial holds dot_m_partials() doubles
+void launch_spmv_resid_m(int n, int nr, const int* rowptr, const int* col, const double* val, const double* X,
+                         const double* B, double* Y, long long ld, hipStream_t st);
+int dot_m_partials();
+void launch_dot_m(int n, int nr, const double* A, const double* B, long long ld, double* partial, double* out,
+                  hipStream_t st);
 void launch_absmax(long long nnz, const double* v, double* partial, double* out, hipStream_t st);
 // out[slot] = sum a.*b  (deterministic two-stage)
 void launch_dot(int n, const double* a, const double* b, double* partial, double* out,

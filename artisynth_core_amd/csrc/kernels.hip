@@ -1275,12 +1275,13 @@ __global__ __launch_bounds__(256) void k_fwd_gather_m(const SolveItem* __restric
 }
 
 // Lane sl's partial sums sum_{e < len} a[e] b_j[e] for nr vectors b_j = b + j ld,
-// over the lane's chunks in the order of seg_dot (groups of two chunks; the
-// chunk loads of a group issue before its products)
+// over the lane's chunks in the order of seg_dot (groups of G chunks, G * the
+// number of vectors <= 16 chunk registers; the loads of a group issue before its
+// products)
 template <typename T>
 __device__ inline void seg_dot_m(const T* __restrict__ a, const T* __restrict__ b, long long ld, int nr,
                                  int len, int sl, int seg, T* acc) {
-  constexpr int W = 16 / sizeof(T), G = 2;
+  constexpr int W = 16 / sizeof(T), G = (SOLVE_NRHS >= 8) ? 1 : 16 / (2 * SOLVE_NRHS);
   typedef typename V16<T>::v vt;
   const int nc = (len + W - 1) / W;
 #pragma unroll
@@ -1730,6 +1731,82 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double* __restrict__
   if (threadIdx.x == 0)
     *out = is_max ? fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3])) : (sm[0] + sm[1]) + (sm[2] + sm[3]);
 }
+// Residuals and dot products of nr <= SOLVE_NRHS columns (column j at j * ld) in
+// one launch each, per column in the order of k_spmv (mode 1) and of
+// k_dot_partial + k_reduce_final, so each column's value equals the single one
+__global__ __launch_bounds__(256) void k_spmv_resid_m(int n, int nr, const int* __restrict__ rowptr,
+                                                      const int* __restrict__ col,
+                                                      const double* __restrict__ val,
+                                                      const double* __restrict__ X,
+                                                      const double* __restrict__ Bv, double* __restrict__ Y,
+                                                      long long ld) {
+  const int row = blockIdx.x * SPMV_ROWS + (threadIdx.x >> 3);
+  const int sl = threadIdx.x & 7;
+  double acc[SOLVE_NRHS];
+#pragma unroll
+  for (int j = 0; j < SOLVE_NRHS; j++) acc[j] = 0.0;
+  if (row < n) {
+    const int e1 = rowptr[row + 1];
+    int e = rowptr[row] + sl;
+    double a0[SOLVE_NRHS], b0[SOLVE_NRHS];
+#pragma unroll
+    for (int j = 0; j < SOLVE_NRHS; j++) a0[j] = b0[j] = 0.0;
+    for (; e + 24 < e1; e += 32) {
+      const double v0 = val[e], v1 = val[e + 8], v2 = val[e + 16], v3 = val[e + 24];
+      const int c0 = col[e], c1 = col[e + 8], c2 = col[e + 16], c3 = col[e + 24];
+#pragma unroll
+      for (int j = 0; j < SOLVE_NRHS; j++) {
+        if (j >= nr) break;
+        const double* x = X + j * ld;
+        a0[j] = fma(v0, x[c0], a0[j]);
+        b0[j] = fma(v1, x[c1], b0[j]);
+        a0[j] = fma(v2, x[c2], a0[j]);
+        b0[j] = fma(v3, x[c3], b0[j]);
+      }
+    }
+    for (; e < e1; e += 8) {
+      const double v = val[e];
+      const int c = col[e];
+#pragma unroll
+      for (int j = 0; j < SOLVE_NRHS; j++)
+        if (j < nr) a0[j] = fma(v, X[j * ld + c], a0[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < SOLVE_NRHS; j++) acc[j] = a0[j] + b0[j];
+  }
+#pragma unroll
+  for (int j = 0; j < SOLVE_NRHS; j++)
+    for (int off = 4; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, 64);
+  if (row < n && sl == 0) {
+#pragma unroll
+    for (int j = 0; j < SOLVE_NRHS; j++)
+      if (j < nr) Y[j * ld + row] = Bv[j * ld + row] - acc[j];
+  }
+}
+__global__ __launch_bounds__(256) void k_dot_partial_m(int n, const double* __restrict__ A,
+                                                       const double* __restrict__ Bv, long long ld,
+                                                       double* __restrict__ partial) {
+  __shared__ double sm[4];
+  const double* a = A + blockIdx.y * ld;
+  const double* b = Bv + blockIdx.y * ld;
+  double acc = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += RED_BLOCKS * 256) acc += a[i] * b[i];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.y * RED_BLOCKS + blockIdx.x] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+}
+__global__ __launch_bounds__(256) void k_reduce_final_m(const double* __restrict__ partial,
+                                                        double* __restrict__ out) {
+  __shared__ double sm[4];
+  const double* p = partial + blockIdx.x * RED_BLOCKS;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < RED_BLOCKS; i += 256) acc = acc + p[i];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+}
 // out1 = sum of p1[0..np), out2 = sum of p2[0..np) (when given), fixed order
 __global__ __launch_bounds__(256) void k_reduce2(const double* __restrict__ p1,
                                                  const double* __restrict__ p2, int np,
@@ -2000,6 +2077,19 @@ void launch_dot(int n, const double* a, const double* b, double* partial, double
                 hipStream_t st) {
   hipLaunchKernelGGL(k_dot_partial, dim3(RED_BLOCKS), dim3(256), 0, st, n, a, b, partial);
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, (const double*)partial, RED_BLOCKS, out, 0);
+}
+void launch_spmv_resid_m(int n, int nr, const int* rowptr, const int* col, const double* val, const double* X,
+                         const double* B, double* Y, long long ld, hipStream_t st) {
+  if (n > 0 && nr > 0)
+    hipLaunchKernelGGL(k_spmv_resid_m, dim3(nblk(n, SPMV_ROWS)), dim3(256), 0, st, n, nr, rowptr, col, val, X, B,
+                       Y, ld);
+}
+int dot_m_partials() { return RED_BLOCKS * SOLVE_NRHS; }
+void launch_dot_m(int n, int nr, const double* A, const double* B, long long ld, double* partial, double* out,
+                  hipStream_t st) {
+  if (nr <= 0) return;
+  hipLaunchKernelGGL(k_dot_partial_m, dim3(RED_BLOCKS, nr), dim3(256), 0, st, n, A, B, ld, partial);
+  hipLaunchKernelGGL(k_reduce_final_m, dim3(nr), dim3(256), 0, st, (const double*)partial, out);
 }
 void launch_axpby(int n, double alpha, const double* x, double beta, double* y, hipStream_t st) {
   hipLaunchKernelGGL(k_axpby, dim3(nblk(n, 256)), dim3(256), 0, st, n, alpha, x, beta, y);

@@ -124,13 +124,18 @@ struct kkt_solver {
   // solve scratch, as T: bp, fp, y, xp (n each), u (u_elems + 16), w (v_elems + 16)
   void* d_svec = nullptr;
   size_t svec_bytes = 0;
-  // multi-RHS solves: the solve vectors of SOLVE_NRHS right-hand sides (as T) and
-  // six FP64 sets of SOLVE_NRHS n-vectors (graph input, graph output, residual,
-  // correction, host-path right-hand sides, host-path solutions)
+  // multi-RHS solves: the solve vectors of SOLVE_NRHS right-hand sides (as T);
+  // FP64 n-vectors: graph input and output (SOLVE_NRHS each), then residuals,
+  // corrections, host-path right-hand sides and solutions (MULTI_BLOCK each);
+  // MULTI_BLOCK dot products and their pinned host copy
+  static constexpr int MULTI_BLOCK = 64;
   void* d_msvec = nullptr;
   size_t msvec_bytes = 0;
   double* d_mvec = nullptr;
   size_t mvec_bytes = 0;
+  double* d_mdot = nullptr;
+  double* h_mdot = nullptr;
+  double* d_mpart = nullptr;
   // FP64 vectors for refinement / BiCGStab / host copies
   double* d_vec = nullptr;  // 14 vectors of n (0-8 BiCGStab, 10-11 host copies, 12-13 graph I/O)
   double* d_partial = nullptr;
@@ -225,8 +230,12 @@ struct kkt_solver {
     if (d_svec) (void)hipFree(d_svec);
     if (d_msvec) (void)hipFree(d_msvec);
     if (d_mvec) (void)hipFree(d_mvec);
+    if (d_mdot) (void)hipFree(d_mdot);
+    if (h_mdot) (void)hipHostFree(h_mdot);
+    if (d_mpart) (void)hipFree(d_mpart);
+    d_mpart = nullptr;
     d_F64 = nullptr; d_F32 = nullptr; d_W64 = nullptr; d_W32 = nullptr; d_svec = nullptr;
-    d_msvec = nullptr; d_mvec = nullptr;
+    d_msvec = nullptr; d_mvec = nullptr; d_mdot = nullptr; h_mdot = nullptr;
     svec_bytes = msvec_bytes = mvec_bytes = 0;
   }
   void free_plan(AddPlan& p) {
@@ -993,22 +1002,32 @@ struct kkt_solver {
   int ensure_multi() {
     const size_t esz = std::max(sizeof(double), sizeof(float));
     const size_t need = (size_t)SOLVE_NRHS * (4 * (size_t)m_ldn() + (size_t)u_elems + 16 + (size_t)v_elems + 16) * esz;
-    const size_t needv = (size_t)6 * SOLVE_NRHS * (size_t)n * sizeof(double);
-    if (msvec_bytes >= need && mvec_bytes >= needv) return KKT_OK;
+    const size_t needv = ((size_t)2 * SOLVE_NRHS + 4 * (size_t)MULTI_BLOCK) * (size_t)n * sizeof(double);
+    if (msvec_bytes >= need && mvec_bytes >= needv && d_mdot) return KKT_OK;
     DevOpLock lk(g_capture_mu);
     drop_graphs();
     if (d_msvec) (void)hipFree(d_msvec);
     if (d_mvec) (void)hipFree(d_mvec);
+    if (d_mdot) (void)hipFree(d_mdot);
+    if (h_mdot) (void)hipHostFree(h_mdot);
+    if (d_mpart) (void)hipFree(d_mpart);
     d_msvec = nullptr;
     d_mvec = nullptr;
+    d_mdot = h_mdot = d_mpart = nullptr;
     msvec_bytes = mvec_bytes = 0;
-    if (hipMalloc(&d_msvec, need) != hipSuccess || hipMalloc((void**)&d_mvec, needv) != hipSuccess)
+    if (hipMalloc(&d_msvec, need) != hipSuccess || hipMalloc((void**)&d_mvec, needv) != hipSuccess ||
+        hipMalloc((void**)&d_mdot, MULTI_BLOCK * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&d_mpart, dot_m_partials() * sizeof(double)) != hipSuccess ||
+        hipHostMalloc((void**)&h_mdot, MULTI_BLOCK * sizeof(double), hipHostMallocDefault) != hipSuccess)
       return fail(KKT_ERR_ALLOC, "hipMalloc failed (multi-RHS solve)");
     msvec_bytes = need;
     mvec_bytes = needv;
     return KKT_OK;
   }
-  double* mset(int i) { return d_mvec + (size_t)i * SOLVE_NRHS * n; }
+  double* m_gin() { return d_mvec; }
+  double* m_gout() { return d_mvec + (size_t)SOLVE_NRHS * n; }
+  // i = 0 residuals, 1 corrections, 2 host-path right-hand sides, 3 host-path solutions
+  double* m_blk(int i) { return d_mvec + ((size_t)2 * SOLVE_NRHS + (size_t)i * MULTI_BLOCK) * n; }
   // nr right-hand sides B (column j at B + j ldb) to X, one launch sequence
   template <typename T>
   void run_tri_solve_multi(const T* F, const double* B, long long ldb, double* X, long long ldx, int nr) {
@@ -1050,8 +1069,8 @@ struct kkt_solver {
     const int solve_cls = (1 << TC_SLEVEL) | (1 << TC_SGEMV) | (1 << TC_VEC);
     if (!graphs_on || (tm_on && (tm_mask & solve_cls))) { tri_solve_multi_eager(B, n, X, n, nr); return; }
     const int q = (fprec == KKT_PREC_FP64) ? 0 : 1;
-    double* gin = mset(0);
-    double* gout = mset(1);
+    double* gin = m_gin();
+    double* gout = m_gout();
     if (!mexec[q][nr - 1] && !capture(mexec[q][nr - 1], [&]() { tri_solve_multi_eager(gin, n, gout, n, nr); })) {
       graphs_on = false;
       tri_solve_multi_eager(B, n, X, n, nr);
@@ -1062,42 +1081,50 @@ struct kkt_solver {
     (void)hipGraphLaunch(mexec[q][nr - 1], st);
     if (X != gout) (void)hipMemcpyAsync(X, gout, bytes, hipMemcpyDeviceToDevice, st);
   }
-  // dot products of nr column pairs into pinned host memory (one synchronization)
-  int dots_multi(const double* A, const double* Bv, int nr, double* out) {
+  // squared norms of nb columns (ld = n) into out (one synchronization), each as
+  // launch_dot computes it
+  int norms_multi(const double* A, int nb, double* out) {
     tbeg(TC_VEC);
-    for (int j = 0; j < nr; j++) launch_dot(n, A + (size_t)j * n, Bv + (size_t)j * n, d_partial, d_scal + j, st);
+    for (int j0 = 0; j0 < nb; j0 += SOLVE_NRHS) {
+      const double* a = A + (size_t)j0 * n;
+      launch_dot_m(n, std::min(SOLVE_NRHS, nb - j0), a, a, n, d_mpart, d_mdot + j0, st);
+    }
     tend();
-    if (hipMemcpyAsync(h_pin, d_scal, nr * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+    const int nr = nb;
+    if (hipMemcpyAsync(h_mdot, d_mdot, nr * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
       return fail(KKT_ERR_REDUCTION, "reduction failed");
-    for (int j = 0; j < nr; j++) out[j] = h_pin[j];
+    for (int j = 0; j < nr; j++) out[j] = h_mdot[j];
     return KKT_OK;
   }
-  // refined solves of nrhs columns (ld = n) on device arrays, SOLVE_NRHS at a
-  // time.  Per column the same steps as solve_device: FP64 factors take ir_steps
-  // refinement steps, FP32 factors refine to refine_tol; a column's correction is
-  // applied only while it is still refining, so each result equals its
-  // single-RHS solve.  A stalled column falls back as solve_device does (FP64
-  // refactor; then the chunk's columns are solved one by one).
+  // refined solves of nrhs columns (ld = n) on device arrays, MULTI_BLOCK columns
+  // at a time, whose triangular solves go SOLVE_NRHS at a time.  Per column the
+  // same steps as solve_device: FP64 factors take ir_steps refinement steps, FP32
+  // factors refine to refine_tol; a column's correction is applied only while it
+  // is still refining, so each result equals its single-RHS solve.  The columns
+  // of a block refine in lockstep (one host synchronization per step).  A
+  // stalled column falls back as solve_device does (FP64 refactor; then the
+  // block's columns are solved one by one).
   int solve_multi_device(int nrhs, const double* d_B, double* d_X) {
     if (state != FACTORED) return fail(KKT_ERR_STATE, "solve called before factor");
     int rc = ensure_multi();
     if (rc) return rc;
-    double* R = mset(2);
-    double* D = mset(3);
-    for (int c0 = 0; c0 < nrhs; c0 += SOLVE_NRHS) {
-      const int nr = std::min(SOLVE_NRHS, nrhs - c0);
+    double* R = m_blk(0);
+    double* D = m_blk(1);
+    for (int c0 = 0; c0 < nrhs; c0 += MULTI_BLOCK) {
+      const int nb = std::min(MULTI_BLOCK, nrhs - c0);
       const double* B = d_B + (size_t)c0 * n;
       double* X = d_X + (size_t)c0 * n;
-      tri_solve_multi(B, X, nr);
+      for (int j0 = 0; j0 < nb; j0 += SOLVE_NRHS)
+        tri_solve_multi(B + (size_t)j0 * n, X + (size_t)j0 * n, std::min(SOLVE_NRHS, nb - j0));
       const int want = (fprec == KKT_PREC_FP32_REFINE) ? refine_max : ir_steps;
       last_refine_steps = 0;
       if (want <= 0) continue;
-      double bn[SOLVE_NRHS], prev[SOLVE_NRHS], rn2[SOLVE_NRHS];
-      int stall[SOLVE_NRHS];
-      bool active[SOLVE_NRHS];
-      if ((rc = dots_multi(B, B, nr, bn))) return rc;
-      for (int j = 0; j < nr; j++) {
+      double bn[MULTI_BLOCK], prev[MULTI_BLOCK], rn2[MULTI_BLOCK];
+      int stall[MULTI_BLOCK];
+      bool active[MULTI_BLOCK];
+      if ((rc = norms_multi(B, nb, bn))) return rc;
+      for (int j = 0; j < nb; j++) {
         bn[j] = std::sqrt(bn[j]);
         if (bn[j] == 0.0) bn[j] = 1.0;
         prev[j] = 0.0;
@@ -1106,11 +1133,15 @@ struct kkt_solver {
       }
       bool fallback = false;
       for (int step = 0;; step++) {
-        for (int j = 0; j < nr; j++)
-          if (active[j]) spmv(X + (size_t)j * n, B + (size_t)j * n, R + (size_t)j * n, 1);
-        if ((rc = dots_multi(R, R, nr, rn2))) return rc;
+        // residuals of every column (those already done are not used)
+        tbeg(TC_SPMV);
+        for (int j0 = 0; j0 < nb; j0 += SOLVE_NRHS)
+          launch_spmv_resid_m(n, std::min(SOLVE_NRHS, nb - j0), d_rowptr_f, d_col_f, d_valsf, X + (size_t)j0 * n,
+                              B + (size_t)j0 * n, R + (size_t)j0 * n, n, st);
+        tend();
+        if ((rc = norms_multi(R, nb, rn2))) return rc;
         int nact = 0;
-        for (int j = 0; j < nr; j++) {
+        for (int j = 0; j < nb; j++) {
           if (!active[j]) continue;
           const double rel = std::sqrt(rn2[j]) / bn[j];
           last_relres = rel;
@@ -1126,16 +1157,22 @@ struct kkt_solver {
           nact++;
         }
         if (fallback || nact == 0) break;
-        tri_solve_multi(R, D, nr);
+        // corrections for the groups that hold a refining column
+        for (int j0 = 0; j0 < nb; j0 += SOLVE_NRHS) {
+          const int nr = std::min(SOLVE_NRHS, nb - j0);
+          bool any = false;
+          for (int j = j0; j < j0 + nr; j++) any = any || active[j];
+          if (any) tri_solve_multi(R + (size_t)j0 * n, D + (size_t)j0 * n, nr);
+        }
         tbeg(TC_VEC);
-        for (int j = 0; j < nr; j++)
+        for (int j = 0; j < nb; j++)
           if (active[j]) launch_axpby(n, 1.0, D + (size_t)j * n, 1.0, X + (size_t)j * n, st);
         tend();
         last_refine_steps++;
       }
       if (fallback) {
         // as solve_device: FP64 refactor with the same values, then each column
-        for (int j = 0; j < nr; j++)
+        for (int j = 0; j < nb; j++)
           if ((rc = solve_device(B + (size_t)j * n, X + (size_t)j * n))) return rc;
       }
     }
@@ -1969,10 +2006,10 @@ int kkt_solve_multi(kkt_solver* h, int nrhs, const double* B, double* X) {
   if (h->state != kkt_solver::FACTORED) return h->fail(KKT_ERR_STATE, "solve called before factor");
   int rc = h->ensure_multi();
   if (rc) return rc;
-  double* dB = h->mset(4);
-  double* dX = h->mset(5);
-  for (int c0 = 0; c0 < nrhs; c0 += SOLVE_NRHS) {
-    const int nr = std::min(SOLVE_NRHS, nrhs - c0);
+  double* dB = h->m_blk(2);
+  double* dX = h->m_blk(3);
+  for (int c0 = 0; c0 < nrhs; c0 += kkt_solver::MULTI_BLOCK) {
+    const int nr = std::min(kkt_solver::MULTI_BLOCK, nrhs - c0);
     const size_t bytes = (size_t)nr * h->n * sizeof(double);
     if (hipMemcpyAsync(dB, B + (size_t)c0 * h->n, bytes, hipMemcpyHostToDevice, h->st) != hipSuccess)
       return h->fail(KKT_ERR_COPY, "hipMemcpyAsync (B) failed");

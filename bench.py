@@ -321,7 +321,7 @@ def pmc_run(nx, ny, nz):
 def multi_run(nx, ny, nz, nrhs, reps, warmup):
     """Block solves with many right-hand sides (the LCP callers' solveMG of
     KKTSolver.java:1537-1588) on the hex beam's first KKT system: kkt_solve_multi_device
-    (four right-hand sides per launch sequence, refinement per column) against the
+    (eight right-hand sides per launch sequence, refinement per column) against the
     same right-hand sides solved by one kkt_solve_device call each, for both factor
     precisions.  Every column is checked equal to its single solve, and residuals
     are recomputed on the host."""

@@ -188,8 +188,8 @@ int kkt_solve(kkt_solver* h, const double* b, double* x);
 /* Same on device arrays. */
 int kkt_solve_device(kkt_solver* h, const double* d_b, double* d_x);
 /* CuDssSolver.doSolveMulti (cudssBridge.cc:1454-1523; CuDssSolver.java:814-826):
- * column-major n x nrhs.  The right-hand sides are solved four at a time in one
- * launch sequence (each front's factor data read once per four), with the
+ * column-major n x nrhs.  The right-hand sides are solved eight at a time in one
+ * launch sequence (each front's factor data read once per eight), with the
  * configured refinement per column; each column's result equals its kkt_solve
  * result. */
 int kkt_solve_multi(kkt_solver* h, int nrhs, const double* B, double* X);
