@@ -69,7 +69,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_assemble(const int4* __restrict__ items, DevSym d,
                                                   T* __restrict__ F) {
   __shared__ T tile[TILE * (TILE + 1)];  // tile[c*65 + r]
-  const int4 it = items[blockIdx.x];
+  const int4 it = items[xcd_item(blockIdx.x, gridDim.x)];
   const int s = it.x, I = it.y, J = it.z;
   const int m = d.sn_m[s];
   const int r0 = I * TILE, c0 = J * TILE;
@@ -552,7 +552,7 @@ __global__ __launch_bounds__(256) void k_panel_trsm(const int2* __restrict__ ite
   __shared__ int sig[64];
   __shared__ signed char pv[64];
   typedef typename Mfma<T>::v4 v4;
-  const int2 it = items[blockIdx.x];
+  const int2 it = items[xcd_item(blockIdx.x, gridDim.x)];
   const int s = it.x, rb = it.y;
   const int k = d.sn_k[s], m = d.sn_m[s], first = d.sn_first[s];
   const int j0 = p * NB, w = min(NB, k - j0), r0 = j0 + w;
@@ -739,7 +739,7 @@ __global__ __launch_bounds__(256) void k_panel_xrow(const int2* __restrict__ ite
   __shared__ T As[64 * LD];  // A operand, As[r*LD + kk]
   __shared__ T Bn[64 * LD];  // B operand stored n-major, Bn[n*LD + kk]
   typedef typename Mfma<T>::v4 v4;
-  const int2 it = items[blockIdx.x];
+  const int2 it = items[xcd_item(blockIdx.x, gridDim.x)];
   const int s = it.x, cb = it.y;
   const int k = d.sn_k[s], m = d.sn_m[s];
   const int j0 = p * NB, w = min(NB, k - j0), c0 = cb * 64;
@@ -856,7 +856,7 @@ __global__ __launch_bounds__(256) void k_zt(const int4* __restrict__ items, DevS
   __shared__ T As[64 * LD];  // As[c*LD + i] = X(i0 + i, c0 + c)
   __shared__ T Bs[64 * LD];  // Bs[j*LD + i] = L21(jr0 + j, i0 + i)
   typedef typename Mfma<T>::v4 v4;
-  const int4 it = items[blockIdx.x];
+  const int4 it = items[xcd_item(blockIdx.x, gridDim.x)];
   const int s = it.x, cb = it.y, jb = it.z;
   const int k = d.sn_k[s], m = d.sn_m[s], r = m - k;
   const int c0 = cb * 64, jr0 = jb * 64;
@@ -929,7 +929,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_gform(const int4* __restrict__ items, DevSym d,
                                                T* __restrict__ F) {
   __shared__ T tile[64 * 65];  // tile[x*65 + y]: source column offset x, source row offset y
-  const int4 it = items[blockIdx.x];
+  const int4 it = items[xcd_item(blockIdx.x, gridDim.x)];
   const int s = it.x;
   const int k = d.sn_k[s], m = d.sn_m[s];
   T* Fs = F + d.sn_foff[s];
