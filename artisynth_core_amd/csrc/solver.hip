@@ -1749,7 +1749,9 @@ int kkt_add_material_stiffness3_device_values(kkt_solver* h, const int* blockSlo
 // ---- element-batched forms (CuDssJNI.cc / cudssBridgeKernels.cu:336-739) ----
 // Shared checks of the compact element descriptors: offsets start at 0 and do not
 // decrease, pair node indices lie inside their element, node target dimensions
-// are 1..6, and each element's integration-point / gradient ranges fit.
+// are 1..6, and each element's integration-point / gradient ranges fit.  (C++
+// helpers: outside the C-linkage block.)
+}  // extern "C"
 namespace {
 bool offsets_ok(const int* off, int nelems) {
   if (off[0] != 0) return false;
@@ -1785,6 +1787,7 @@ std::vector<int> masked_slots36(const int* nodeOff, const int* pairOff, const in
   return m;
 }
 }  // namespace
+extern "C" {
 
 int kkt_add_material_stiffness3_element_device_values(
     kkt_solver* h, const int* elemNodeCounts, const int* elemNodeOffsets, const int* elemPairOffsets,
