@@ -96,11 +96,8 @@ template <typename T>
 void launch_gform(const int4* items, int nitems, const DevSym& d, T* F, hipStream_t st);
 // level-scheduled solves (vectors in the front precision T)
 template <typename T>
-void launch_fwd_gather(const SolveItem* items, int nitems, const DevSym& d, const T* F,
-                       const T* bp, T* fp, T* y, T* u, hipStream_t st);
-template <typename T>
-void launch_fwd_rows(const SolveItem* items, int nitems, const DevSym& d, const T* F, const T* fp,
-                     T* y, T* u, hipStream_t st);
+void launch_fwd_level(const SolveItem* items, int nitems, const DevSym& d, const T* F, const T* bp,
+                      T* fp, T* y, T* u, hipStream_t st);
 template <typename T>
 void launch_dsolve(int n, const T* E0, const T* E1, const signed char* piv, const int* wpos,
                    const T* y, T* wv, hipStream_t st);

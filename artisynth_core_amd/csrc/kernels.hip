@@ -842,8 +842,8 @@ __global__ __launch_bounds__(256) void k_panel_xrow(const int2* __restrict__ ite
 //   z = D^-1 y for all pivots at once
 //   backward: x = z + G^T [z; -x_rows]
 //             (one wave per pivot column, reading column c of the lower copy)
-// so every product streams contiguous columns, and a level costs two launches
-// forward and one backward.
+// so every product streams contiguous columns, and a level costs one launch
+// forward (two where a separate gather launch is faster) and one backward.
 // ------------------------------------------------------------------------
 constexpr int KMAX = 1024;  // widest supernode (the symbolic analysis splits wider ones)
 
@@ -1028,26 +1028,27 @@ template <>
 struct VI16<double> {
   typedef int v __attribute__((ext_vector_type(2)));
 };
-// forward, first launch of a level.  item = (s, block, kind) with the structure:
-//   kind 1: gather block of a large front, one thread per position p (children
-//           summed in ascending order): f_p = b_p + the children's update entries
-//           at p; fp = f in pivot order (p < k), u_s = the sums on the
-//           contribution rows (p >= k)
-//   kind 0: a whole small front (k <= FWD_SMALL_K, m <= FWD_SMALL_M): the same gather into
-//           LDS, then one thread per row: y_i = f_i + G(i,:) f (i < k),
-//           u_{i-k} = f_i - G(i,:) f (i >= k), G(i,:) read across the columns of
-//           the lower copy (coalesced over the threads)
+// forward, a whole small front (k <= FWD_SMALL_K, m <= FWD_SMALL_M) in one
+// workgroup: one thread per position p gathers f_p = b_p + the children's update
+// entries at p (summed in ascending order) into LDS in pivot order, then one
+// thread per row: y_i = f_i + G(i,:) f (i < k), u_{i-k} = f_i - G(i,:) f (i >= k),
+// G(i,:) read across the columns of the lower copy (coalesced over the threads)
 static_assert(FWD_SMALL_M == 256, "one thread per row of a small front");
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_gather(const SolveItem* __restrict__ items, DevSym d,
-                                                    const T* __restrict__ F,
-                                                    const T* __restrict__ bp, T* __restrict__ fp,
-                                                    T* __restrict__ y, T* __restrict__ u) {
+__device__ inline void fwd_small_front(const SolveItem& it, const DevSym& d, const T* __restrict__ F,
+                                       const T* __restrict__ bp, T* __restrict__ y, T* __restrict__ u) {
   __shared__ T g[256];
   __shared__ __align__(16) T fs[FWD_SMALL_K + 4];  // chunk loads may run 3 past k
-  const SolveItem it = items[blockIdx.x];
   const int k = it.k, m = it.m, first = it.first;
-  const int p = it.blk * 256 + threadIdx.x;
+  const int p = threadIdx.x;
+  // row p of G from the lower copy (column l, row p): consecutive threads read
+  // consecutive entries; all FWD_SMALL_K loads issue before the gather (a column
+  // past the row's length re-reads column 0 and is dropped by a select)
+  const int len = (p < m) ? min(p, k) : 0;
+  const T* rowp = F + it.foff + (p < m ? p : 0);
+  T v[FWD_SMALL_K];
+#pragma unroll
+  for (int l = 0; l < FWD_SMALL_K; l++) v[l] = rowp[(size_t)(l < len ? l : 0) * m];
   T acc = (T)0;
   int ip = 0;
   if (p < m) {
@@ -1069,24 +1070,11 @@ __global__ __launch_bounds__(256) void k_fwd_gather(const SolveItem* __restrict_
     }
     for (; e < e1; e++) acc += u[d.gsrc[e]];
   }
-  if (it.kind == 1) {
-    if (p < k) fp[first + ip] = acc;
-    else if (p < m) u[it.uoff + p - k] = acc;
-    return;
-  }
   g[threadIdx.x] = acc;
   __syncthreads();
   if (p < k) fs[ip] = acc;
   __syncthreads();
   if (p < m) {
-    // row p of G from the lower copy (column l, row p): consecutive threads read
-    // consecutive entries, all FWD_SMALL_K loads issued before the first use (a
-    // column past the row's length re-reads column 0 and is dropped by a select)
-    const int len = min(p, k);
-    const T* rowp = F + it.foff + p;
-    T v[FWD_SMALL_K];
-#pragma unroll
-    for (int l = 0; l < FWD_SMALL_K; l++) v[l] = rowp[(size_t)(l < len ? l : 0) * m];
     T acc = (T)0;
 #pragma unroll
     for (int l = 0; l < FWD_SMALL_K; l++) {
@@ -1105,16 +1093,44 @@ __device__ inline T seg_sum(T v, int lg) {
   return v;
 }
 
-// forward products from the upper copy (column i, rows l < min(i, k),
-// contiguous): y_i = f_i + G(i,:) f (i < k) or u_{i-k} -= G(i,:) f (i >= k).
-// A segment of 2^lg lanes per row, 64 >> lg rows per wave;
-// item = (s, block of 4 * (64 >> lg) rows, lg)
+// forward gather block of a large front (two-launch form, first launch): one
+// thread per position p, f_p = b_p + the children's update entries at p (summed
+// in ascending order); fp = f in pivot order (p < k), u_s = the sums on the
+// contribution rows (p >= k)
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_rows(const SolveItem* __restrict__ items, DevSym d,
-                                                  const T* __restrict__ F,
-                                                  const T* __restrict__ fp, T* __restrict__ y,
-                                                  T* __restrict__ u) {
-  const SolveItem it = items[blockIdx.x];
+__device__ inline void fwd_gather_block(const SolveItem& it, const DevSym& d, const T* __restrict__ bp,
+                                        T* __restrict__ fp, T* __restrict__ u) {
+  const int k = it.k, m = it.m, first = it.first;
+  const int p = it.blk * 256 + threadIdx.x;
+  if (p >= m) return;
+  T acc = (T)0;
+  int ip = 0;
+  const int* gp = d.gptr + it.goff + p;
+  const int e0 = gp[0], e1 = gp[1];
+  if (p < k) {
+    acc = bp[first + p];
+    ip = d.ilperm[first + p];
+  }
+  int e = e0;
+  for (; e + 3 < e1; e += 4) {
+    const int i0 = d.gsrc[e], i1 = d.gsrc[e + 1], i2 = d.gsrc[e + 2], i3 = d.gsrc[e + 3];
+    const T v0 = u[i0], v1 = u[i1], v2 = u[i2], v3 = u[i3];
+    acc += v0;
+    acc += v1;
+    acc += v2;
+    acc += v3;
+  }
+  for (; e < e1; e++) acc += u[d.gsrc[e]];
+  if (p < k) fp[first + ip] = acc;
+  else u[it.uoff + p - k] = acc;
+}
+
+// forward rows of a large front (two-launch form, second launch): products from
+// the upper copy (column i, rows l < min(i, k), contiguous), y_i = f_i + G(i,:) f
+// (i < k) or u_{i-k} -= G(i,:) f (i >= k), f = fp from the gather launch
+template <typename T>
+__device__ inline void fwd_rows_block(const SolveItem& it, const T* __restrict__ F, const T* __restrict__ fp,
+                                      T* __restrict__ y, T* __restrict__ u) {
   const int lg = it.lg;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int seg = 1 << lg, sl = lane & (seg - 1);
@@ -1123,11 +1139,142 @@ __global__ __launch_bounds__(256) void k_fwd_rows(const SolveItem* __restrict__ 
   const T* col = F + it.foff + (size_t)(i < m ? i : 0) * m;
   const T* f = fp + first;
   const bool own = sl == 0 && i < m;
-  // the value this row's result combines with, loaded before the dot product
   T base = (T)0;
   if (own) base = (i < k) ? f[i] : u[it.uoff + i - k];
   const int len = (i < m) ? min(i, k) : 0;
   const T acc = seg_sum(seg_dot<T>(col, f, len, sl, seg), lg);
+  if (own) {
+    if (i < k) y[first + i] = base + acc;
+    else u[it.uoff + i - k] = base - acc;
+  }
+}
+
+// forward, one or two launches per tree level, by item kind:
+//   0  a whole small front (fwd_small_front);
+//   1  a gather block of a large front (fwd_gather_block), then in the next
+//   3  launch its row blocks (fwd_rows_block);
+//   2  a row block of a large front that gathers for itself (one launch per
+//      level): products from the upper copy as in kind 3,
+//        y_i = f_i + G(i,:) f (i < k),   u_{i-k} = g_i - G(i,:) f (i >= k),
+//      with every such item gathering the front's pivot part f = b + the
+//      children's update entries into LDS in pivot order, and g_i (the children's
+//      entries on its own contribution rows) for its rows.
+// Row blocks: a segment of 2^lg lanes per row, 64 >> lg rows per wave (item =
+// (s, block of 4 * (64 >> lg) rows, lg)).  Every sum adds the children's entries
+// in ascending order and every row product runs in the seg_dot order, so both
+// forms give bitwise the same y and u.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__ items, DevSym d,
+                                                   const T* __restrict__ F,
+                                                   const T* __restrict__ bp, T* __restrict__ fp,
+                                                   T* __restrict__ y, T* __restrict__ u) {
+  __shared__ __align__(16) T fs[KMAX + 4];  // chunk loads may run 3 past k
+  const SolveItem it = items[blockIdx.x];
+  if (it.kind == 0) {
+    fwd_small_front<T>(it, d, F, bp, y, u);
+    return;
+  }
+  if (it.kind == 1) {
+    fwd_gather_block<T>(it, d, bp, fp, u);
+    return;
+  }
+  if (it.kind == 3) {
+    fwd_rows_block<T>(it, F, fp, y, u);
+    return;
+  }
+  const int lg = it.lg;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int seg = 1 << lg, sl = lane & (seg - 1);
+  const int i = (it.blk * 4 + wv) * (64 >> lg) + (lane >> lg);
+  const int k = it.k, m = it.m, first = it.first;
+  const bool own = sl == 0 && i < m;
+  // the first chunk of this lane's part of row i (the seg_dot order: chunks
+  // sl, sl + seg, ..., 16 bytes each), loaded before the gather
+  constexpr int W = 16 / sizeof(T), GC = 4;
+  typedef typename V16<T>::v vt;
+  const T* col = F + it.foff + (size_t)(i < m ? i : 0) * m;
+  const int len = (i < m) ? min(i, k) : 0;
+  const int nc = (len + W - 1) / W;
+  vt x0[GC];
+#pragma unroll
+  for (int q = 0; q < GC; q++) {
+    const int c = sl + q * seg;
+    x0[q] = ldv(col + (c < nc ? c : 0) * W);
+  }
+  // this row's children entries (contribution rows)
+  T base = (T)0;
+  if (own && i >= k) {
+    const int* gp = d.gptr + it.goff + i;
+    const int e1 = gp[1];
+    for (int e = gp[0]; e < e1; e++) base += u[d.gsrc[e]];
+  }
+  // f in pivot order: the index loads of all (up to four) positions of a
+  // thread issue before the first sum
+  {
+    int e0[KMAX / 256], e1[KMAX / 256], ip[KMAX / 256];
+    T acc[KMAX / 256];
+#pragma unroll
+    for (int j = 0; j < KMAX / 256; j++) {
+      const int p = threadIdx.x + 256 * j;
+      e0[j] = e1[j] = ip[j] = 0;
+      acc[j] = (T)0;
+      if (p < k) {
+        const int* gp = d.gptr + it.goff + p;
+        e0[j] = gp[0];
+        e1[j] = gp[1];
+        acc[j] = bp[first + p];
+        ip[j] = d.ilperm[first + p];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < KMAX / 256; j++) {
+      if (threadIdx.x + 256 * j >= k) break;
+      T a = acc[j];
+      int e = e0[j];
+      for (; e + 3 < e1[j]; e += 4) {
+        const int i0 = d.gsrc[e], i1 = d.gsrc[e + 1], i2 = d.gsrc[e + 2], i3 = d.gsrc[e + 3];
+        const T v0 = u[i0], v1 = u[i1], v2 = u[i2], v3 = u[i3];
+        a += v0;
+        a += v1;
+        a += v2;
+        a += v3;
+      }
+      for (; e < e1[j]; e++) a += u[d.gsrc[e]];
+      fs[ip[j]] = a;
+    }
+  }
+  __syncthreads();
+  if (own && i < k) base = fs[i];
+  // seg_dot(col, fs, len) with its first chunk from x0 (the same products in
+  // the same order; a chunk past the row adds nothing)
+  T acc = (T)0;
+#pragma unroll
+  for (int q = 0; q < GC; q++) {
+    const int c = sl + q * seg;
+    const vt yv = ldv(fs + (c < nc ? c : 0) * W);
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+      const T t = fma(x0[q][w], yv[w], acc);
+      acc = (c * W + w < len) ? t : acc;
+    }
+  }
+  for (int q0 = sl + GC * seg; q0 < nc; q0 += GC * seg) {
+    vt x[GC], yv[GC];
+#pragma unroll
+    for (int q = 0; q < GC; q++) {
+      const int c = q0 + q * seg, cc = c < nc ? c : 0;
+      x[q] = ldv(col + cc * W);
+      yv[q] = ldv(fs + cc * W);
+    }
+#pragma unroll
+    for (int q = 0; q < GC; q++)
+#pragma unroll
+      for (int w = 0; w < W; w++) {
+        const T t = fma(x[q][w], yv[q][w], acc);
+        acc = ((q0 + q * seg) * W + w < len) ? t : acc;
+      }
+  }
+  acc = seg_sum(acc, lg);
   if (own) {
     if (i < k) y[first + i] = base + acc;
     else u[it.uoff + i - k] = base - acc;
@@ -1984,16 +2131,10 @@ void launch_gform(const int4* items, int nitems, const DevSym& d, T* F, hipStrea
   if (nitems > 0) hipLaunchKernelGGL(k_gform<T>, dim3(nitems), dim3(256), 0, st, items, d, F);
 }
 template <typename T>
-void launch_fwd_gather(const SolveItem* items, int nitems, const DevSym& d, const T* F, const T* bp,
-                       T* fp, T* y, T* u, hipStream_t st) {
+void launch_fwd_level(const SolveItem* items, int nitems, const DevSym& d, const T* F, const T* bp, T* fp,
+                      T* y, T* u, hipStream_t st) {
   if (nitems > 0)
-    hipLaunchKernelGGL(k_fwd_gather<T>, dim3(nitems), dim3(256), 0, st, items, d, F, bp, fp, y, u);
-}
-template <typename T>
-void launch_fwd_rows(const SolveItem* items, int nitems, const DevSym& d, const T* F, const T* fp, T* y,
-                     T* u, hipStream_t st) {
-  if (nitems > 0)
-    hipLaunchKernelGGL(k_fwd_rows<T>, dim3(nitems), dim3(256), 0, st, items, d, F, fp, y, u);
+    hipLaunchKernelGGL(k_fwd_level<T>, dim3(nitems), dim3(256), 0, st, items, d, F, bp, fp, y, u);
 }
 template <typename T>
 void launch_dsolve(int n, const T* E0, const T* E1, const signed char* piv, const int* wpos,
@@ -2191,10 +2332,8 @@ void launch_zero_counters(FactorCounters* c, hipStream_t st) {
   template void launch_panel_xrow<T>(const int2*, int, int, const DevSym&, T*, hipStream_t);     \
   template void launch_zt<T>(const int4*, int, const DevSym&, T*, hipStream_t);                \
   template void launch_gform<T>(const int4*, int, const DevSym&, T*, hipStream_t);             \
-  template void launch_fwd_gather<T>(const SolveItem*, int, const DevSym&, const T*, const T*, T*, T*, \
-                                     T*, hipStream_t);                                           \
-  template void launch_fwd_rows<T>(const SolveItem*, int, const DevSym&, const T*, const T*, T*, T*,  \
-                                   hipStream_t);                                                 \
+  template void launch_fwd_level<T>(const SolveItem*, int, const DevSym&, const T*, const T*, T*, T*, \
+                                    T*, hipStream_t);                                            \
   template void launch_dsolve<T>(int, const T*, const T*, const signed char*, const int*,        \
                                  const T*, T*, hipStream_t);                                     \
   template void launch_bwd_cols<T>(const SolveItem*, int, const DevSym&, const T*, T*, T*,          \

@@ -105,7 +105,7 @@ struct kkt_solver {
   int4 *d_asm = nullptr, *d_upd = nullptr;
   int *d_diag = nullptr, *d_aidx = nullptr;
   int2 *d_trsm = nullptr, *d_xrow = nullptr;
-  SolveItem *d_fg = nullptr, *d_fr = nullptr, *d_bc = nullptr;
+  SolveItem *d_fg = nullptr, *d_fr = nullptr, *d_bc = nullptr, *d_ff = nullptr;
   int *d_ilperm = nullptr, *d_gptr = nullptr, *d_gsrc = nullptr;
   long long* d_goff = nullptr;
   int4 *d_zt = nullptr, *d_gf = nullptr;
@@ -113,7 +113,7 @@ struct kkt_solver {
   Launch zt_all, gf_all;       // Z^T and G-form post-passes, one launch each
   long long* d_apos = nullptr;
   // per level: assembly, scatter; forward gather / rows, backward columns
-  std::vector<Launch> asm_l, scat_l, fg_l, fr_l, bc_l;
+  std::vector<Launch> asm_l, scat_l, fg_l, fr_l, bc_l, ff_l, fs_l;
   std::vector<std::vector<Step>> steps;
   long long front_elems = 0, w_elems = 0, u_elems = 0, v_elems = 0;
   // numeric buffers (allocated lazily per precision)
@@ -445,6 +445,7 @@ struct kkt_solver {
   std::vector<long long> sv_foff, sv_uoff, sv_roff, sv_goff, sv_voff;
   std::vector<std::vector<int>> sv_lv;
   std::vector<int> fr_epl, bc_epl;
+  std::vector<int> fr_fuse;  // per level: 1 = one forward launch (kind 2 items), 0 = gather + rows
   int replace_upload(SolveItem*& dptr, const std::vector<SolveItem>& v) {
     // a pointer no longer in allocs was already freed with them (new analysis)
     if (dptr) {
@@ -460,7 +461,13 @@ struct kkt_solver {
   }
   int build_solve_items() {
     const int nl = (int)sv_lv.size();
-    std::vector<SolveItem> fgv, frv, bcv;
+    // fg / fr: the two forward launches per level of the multi-RHS solve (gather,
+    // then rows).  ff: the forward launches of the single solve, per level either
+    // one (small fronts whole, row blocks of large fronts with their own gather,
+    // kind 2) or two (small fronts and gather blocks, kinds 0 and 1, then the
+    // row blocks, kind 3: the fs range)
+    std::vector<SolveItem> fgv, frv, bcv, ffv, fsv;
+    if ((int)fr_fuse.size() != nl) fr_fuse.assign(nl, 1);
     auto item = [&](int s, int blk, int lg, int kind) {
       SolveItem it;
       it.s = s; it.blk = blk; it.lg = lg; it.kind = kind;
@@ -477,19 +484,31 @@ struct kkt_solver {
     fg_l.assign(nl, Launch());
     fr_l.assign(nl, Launch());
     bc_l.assign(nl, Launch());
+    ff_l.assign(nl, Launch());
+    fs_l.assign(nl, Launch());
     for (int L = 0; L < nl; L++) {
+      fsv.clear();
       fg_l[L].off = (long long)fgv.size();
       fr_l[L].off = (long long)frv.size();
       bc_l[L].off = (long long)bcv.size();
+      ff_l[L].off = (long long)ffv.size();
       for (int s : sv_lv[L]) {
         const int k = sv_k[s], m = sv_m[s];
         if (k <= FWD_SMALL_K && m <= FWD_SMALL_M) {
           // small front: gather and rows in one workgroup (the first forward launch)
           fgv.push_back(item(s, 0, 0, 0));
+          ffv.push_back(item(s, 0, 0, 0));
         } else {
-          for (int gb = 0; gb < (m + 255) / 256; gb++) fgv.push_back(item(s, gb, 0, 1));
+          for (int gb = 0; gb < (m + 255) / 256; gb++) {
+            fgv.push_back(item(s, gb, 0, 1));
+            if (!fr_fuse[L]) ffv.push_back(item(s, gb, 0, 1));
+          }
           const int lf = seg_lg(k, fr_epl[L], 6), rf = 4 * (64 >> lf);
-          for (int rb = 0; rb < (m + rf - 1) / rf; rb++) frv.push_back(item(s, rb, lf, 0));
+          for (int rb = 0; rb < (m + rf - 1) / rf; rb++) {
+            frv.push_back(item(s, rb, lf, 0));
+            if (fr_fuse[L]) ffv.push_back(item(s, rb, lf, 2));
+            else fsv.push_back(item(s, rb, lf, 3));
+          }
         }
         const int lb = seg_lg(m, bc_epl[L], 8), rbw = 256 >> lb;
         for (int cb = 0; cb < (k + rbw - 1) / rbw; cb++) bcv.push_back(item(s, cb, lb, 0));
@@ -497,9 +516,14 @@ struct kkt_solver {
       fg_l[L].cnt = (int)(fgv.size() - fg_l[L].off);
       fr_l[L].cnt = (int)(frv.size() - fr_l[L].off);
       bc_l[L].cnt = (int)(bcv.size() - bc_l[L].off);
+      ff_l[L].cnt = (int)(ffv.size() - ff_l[L].off);
+      fs_l[L].off = (long long)ffv.size();
+      fs_l[L].cnt = (int)fsv.size();
+      ffv.insert(ffv.end(), fsv.begin(), fsv.end());
     }
     int rc;
-    if ((rc = replace_upload(d_fg, fgv)) || (rc = replace_upload(d_fr, frv)) || (rc = replace_upload(d_bc, bcv)))
+    if ((rc = replace_upload(d_fg, fgv)) || (rc = replace_upload(d_fr, frv)) || (rc = replace_upload(d_bc, bcv)) ||
+        (rc = replace_upload(d_ff, ffv)))
       return rc;
     return KKT_OK;
   }
@@ -783,11 +807,13 @@ struct kkt_solver {
         (rc = upload(d_xrow, xrowv)) || (rc = upload(d_zt, ztv)) || (rc = upload(d_gf, gfv)) ||
         (rc = upload(d_pptr, pptr)) || (rc = upload(d_ppos, ppos)) || (rc = upload(d_wpos, wpos)))
       return rc;
-    // solve work items, default shapes (sixteen elements per lane)
+    // solve work items, default shapes (sixteen elements per lane, one forward
+    // launch per level)
     sv_k = kk; sv_m = mm; sv_first = first; sv_foff = foff; sv_uoff = uoff; sv_roff = roff;
     sv_goff = goff; sv_voff = voff; sv_lv = lv;
     fr_epl.assign(nl, 16);
     bc_epl.assign(nl, 16);
+    fr_fuse.assign(nl, 1);
     if ((rc = build_solve_items())) return rc;
     std::vector<int> lp0(n);
     for (int s = 0; s < ns; s++)
@@ -968,18 +994,13 @@ struct kkt_solver {
     const int nl = S.nlevels;
     // timed per launch: class TC_SLEVEL = forward launches, TC_SGEMV = backward
     for (int L = 0; L < nl; L++) {
-      if (fg_l[L].cnt) {
-        tbeg(TC_SLEVEL);
-        launch_fwd_gather<T>(d_fg + fg_l[L].off, fg_l[L].cnt, ds, F, bp, fp, y, u, st);
-        tend();
-      }
-      if (fr_l[L].cnt) {
-        tbeg(TC_SLEVEL);
-        if (lev_timing) (void)hipEventRecord(lev_ev[4 * L], st);
-        launch_fwd_rows<T>(d_fr + fr_l[L].off, fr_l[L].cnt, ds, F, fp, y, u, st);
-        if (lev_timing) (void)hipEventRecord(lev_ev[4 * L + 1], st);
-        tend();
-      }
+      if (!ff_l[L].cnt) continue;
+      tbeg(TC_SLEVEL);
+      if (lev_timing) (void)hipEventRecord(lev_ev[4 * L], st);
+      launch_fwd_level<T>(d_ff + ff_l[L].off, ff_l[L].cnt, ds, F, bp, fp, y, u, st);
+      if (fs_l[L].cnt) launch_fwd_level<T>(d_ff + fs_l[L].off, fs_l[L].cnt, ds, F, bp, fp, y, u, st);
+      if (lev_timing) (void)hipEventRecord(lev_ev[4 * L + 1], st);
+      tend();
     }
     tbeg(TC_SLEVEL);
     launch_dsolve<T>(n, E0, E1, d_piv, ds.wpos, y, wv, st);
@@ -1216,13 +1237,16 @@ struct kkt_solver {
     double* x = d_vec + 13 * (size_t)n;
     launch_fill(n, 1.0, b, st);
     int rc;
-    // per-level pick from eager per-launch timing
+    // per-level pick from eager per-launch timing: the lane shape of both sweeps
+    // and the one- or two-launch forward form
     static const int cand[] = {4, 8, 16, 32};
     std::vector<float> bfr(nl, 1e30f), bbc(nl, 1e30f);
-    std::vector<int> cfr(nl, 16), cbc(nl, 16);
-    for (int c : cand) {
+    std::vector<int> cfr(nl, 16), cbc(nl, 16), cfu(nl, 1);
+    for (int pass = 0; pass < 8; pass++) {
+      const int c = cand[pass >> 1], fu = (pass & 1) ? 0 : 1;
       fr_epl.assign(nl, c);
       bc_epl.assign(nl, c);
+      fr_fuse.assign(nl, fu);
       if ((rc = build_solve_items())) return rc;
       tri_solve_eager(b, x);
       std::vector<float> tfr(nl, 1e30f), tbc(nl, 1e30f);
@@ -1233,29 +1257,33 @@ struct kkt_solver {
         if (hipStreamSynchronize(st) != hipSuccess) return fail(KKT_ERR_DEVICE, "autotune solve failed");
         for (int L = 0; L < nl; L++) {
           float ms;
-          if (fr_l[L].cnt && hipEventElapsedTime(&ms, lev_ev[4 * L], lev_ev[4 * L + 1]) == hipSuccess)
+          if (ff_l[L].cnt && hipEventElapsedTime(&ms, lev_ev[4 * L], lev_ev[4 * L + 1]) == hipSuccess)
             tfr[L] = std::min(tfr[L], ms);
           if (bc_l[L].cnt && hipEventElapsedTime(&ms, lev_ev[4 * L + 2], lev_ev[4 * L + 3]) == hipSuccess)
             tbc[L] = std::min(tbc[L], ms);
         }
       }
       for (int L = 0; L < nl; L++) {
-        if (tfr[L] < bfr[L]) { bfr[L] = tfr[L]; cfr[L] = c; }
-        if (tbc[L] < bbc[L]) { bbc[L] = tbc[L]; cbc[L] = c; }
+        if (tfr[L] < bfr[L]) { bfr[L] = tfr[L]; cfr[L] = c; cfu[L] = fu; }
+        if (fu && tbc[L] < bbc[L]) { bbc[L] = tbc[L]; cbc[L] = c; }
       }
     }
-    // whole-solve timing of every candidate
-    std::vector<std::pair<std::vector<int>, std::vector<int>>> cands;
-    cands.push_back({std::vector<int>(nl, 16), std::vector<int>(nl, 16)});
+    // whole-solve timing of every candidate: the nine uniform shape pairs with
+    // one forward launch per level, and the per-level picks
+    struct Cand { std::vector<int> f, b, fu; };
+    std::vector<Cand> cands;
+    const std::vector<int> all1(nl, 1);
+    cands.push_back({std::vector<int>(nl, 16), std::vector<int>(nl, 16), all1});
     for (int f : {8, 16, 32})
       for (int g : {8, 16, 32})
-        if (f != 16 || g != 16) cands.push_back({std::vector<int>(nl, f), std::vector<int>(nl, g)});
-    cands.push_back({cfr, cbc});
+        if (f != 16 || g != 16) cands.push_back({std::vector<int>(nl, f), std::vector<int>(nl, g), all1});
+    cands.push_back({cfr, cbc, cfu});
     float best = 1e30f;
     size_t bi = 0;
     for (size_t q = 0; q < cands.size(); q++) {
-      fr_epl = cands[q].first;
-      bc_epl = cands[q].second;
+      fr_epl = cands[q].f;
+      bc_epl = cands[q].b;
+      fr_fuse = cands[q].fu;
       if ((rc = build_solve_items())) return rc;
       drop_graphs();
       float ms;
@@ -1265,8 +1293,16 @@ struct kkt_solver {
         fprintf(stderr, "[kkt-timing] autotune(%s) candidate %zu: %.3f ms per triangular solve\n",
                 fprec == KKT_PREC_FP64 ? "fp64" : "fp32", q, ms);
     }
-    fr_epl = cands[bi].first;
-    bc_epl = cands[bi].second;
+    fr_epl = cands[bi].f;
+    bc_epl = cands[bi].b;
+    fr_fuse = cands[bi].fu;
+    if (g_timing) {
+      fprintf(stderr, "[kkt-timing] autotune(%s): candidate %zu, two-launch forward levels:",
+              fprec == KKT_PREC_FP64 ? "fp64" : "fp32", bi);
+      for (int L = 0; L < nl; L++)
+        if (!fr_fuse[L]) fprintf(stderr, " %d", L);
+      fprintf(stderr, "\n");
+    }
     if ((rc = build_solve_items())) return rc;
     drop_graphs();
     tflush();
@@ -2225,6 +2261,15 @@ int kkt_get_solve_shapes(kkt_solver* h, int* fwd_elems, int* bwd_elems) {
     if (fwd_elems) fwd_elems[L] = h->fr_epl[L];
     if (bwd_elems) bwd_elems[L] = h->bc_epl[L];
   }
+  h->ok();
+  return KKT_OK;
+}
+
+int kkt_get_forward_forms(kkt_solver* h, int* launches) {
+  CHECK_H(h);
+  if (h->state < kkt_solver::ANALYZED) return h->fail(KKT_ERR_STATE, "no analysis");
+  for (size_t L = 0; L < h->fr_fuse.size(); L++)
+    if (launches) launches[L] = h->fr_fuse[L] ? 1 : 2;
   h->ok();
   return KKT_OK;
 }

@@ -192,9 +192,12 @@ def main():
         t_tune = time.time() - t_tune
         shp_f = np.zeros(inf0.nlevels, np.int32)
         shp_b = np.zeros(inf0.nlevels, np.int32)
+        shp_n = np.zeros(inf0.nlevels, np.int32)
         L.kkt_get_solve_shapes(h, nptr(shp_f, ctypes.c_int), nptr(shp_b, ctypes.c_int))
-        log("%s: autotuned solve shapes (elements per lane, levels 0..%d) fwd %s bwd %s in %.2f s"
-            % (name, inf0.nlevels - 1, shp_f.tolist(), shp_b.tolist(), t_tune))
+        L.kkt_get_forward_forms(h, nptr(shp_n, ctypes.c_int))
+        log("%s: autotuned solve shapes (elements per lane, levels 0..%d) fwd %s bwd %s, forward launches "
+            "per level %s, in %.2f s" % (name, inf0.nlevels - 1, shp_f.tolist(), shp_b.tolist(), shp_n.tolist(),
+                                         t_tune))
         L.kkt_timing_reset(h, 0x0F)  # factor kernel classes; solves stay on their HIP graph
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -227,7 +230,7 @@ def main():
             "kernel_launches": {TIMING_CLASSES[c]: cnt[c] for c in range(8)},
             "update_flops": uflops.value, "info": inf.as_dict(),
             "bwd_ms": bwd_ms, "bwd_launches": bwd_launches, "bwd_sweep_bytes": sweep_bytes,
-            "nlevels": nlev, "solve_shapes": {"fwd": shp_f.tolist(), "bwd": shp_b.tolist(),
+            "nlevels": nlev, "solve_shapes": {"fwd": shp_f.tolist(), "bwd": shp_b.tolist(), "fwd_launches": shp_n.tolist(),
                                              "tune_s": t_tune},
             "x": [x_d[i].cpu().numpy() for i in range(min(args.steps, len(V)))],
         }

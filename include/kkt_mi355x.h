@@ -250,7 +250,10 @@ void kkt_timing_reset(kkt_solver* h, int enable);
  * column of a front 2^lg lanes, the smallest power of two with
  * epl * 2^lg >= its length (16 by default).  The candidates are the nine
  * uniform (forward, backward) pairs from {8, 16, 32} and a per-level pick from
- * timing each tree level's launches (HIP events, best of reps); every candidate
+ * timing each tree level's launches (HIP events, best of reps), which also
+ * chooses per level between one forward launch (each row block of a large front
+ * gathers the front's right-hand side itself) and two (a gather launch, then the
+ * rows; the same sums in the same order, so bitwise the same); every candidate
  * is timed over 4 reps whole triangular solves as solves run (captured graph
  * or eager), on the current factor, and the fastest is kept.  The captured
  * solve graphs are dropped.  Needs a factor.  A shape changes only the lane mapping, so
@@ -259,6 +262,9 @@ void kkt_timing_reset(kkt_solver* h, int enable);
 int kkt_autotune_solve(kkt_solver* h, int reps);
 /* The current per-level shapes (kkt_info.nlevels entries each; NULL skips). */
 int kkt_get_solve_shapes(kkt_solver* h, int* fwd_elems, int* bwd_elems);
+/* The forward form per tree level (kkt_info.nlevels entries): 1 = one launch
+ * (the default), 2 = a gather launch, then the rows (autotuner's pick). */
+int kkt_get_forward_forms(kkt_solver* h, int* launches);
 int kkt_timing_get(kkt_solver* h, double* ms, long long* launches, double* update_flops);
 
 /* Host-only symbolic analysis (no GPU needed): structure statistics and the
