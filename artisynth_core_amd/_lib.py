@@ -69,6 +69,7 @@ SIGNATURES = [
     ("kkt_autotune_solve", _I, [_P, _I]),
     ("kkt_get_solve_shapes", _I, [_P, _IP, _IP]),
     ("kkt_get_forward_forms", _I, [_P, _IP]),
+    ("kkt_set_forward_forms", _I, [_P, _IP]),
     ("kkt_solve", _I, [_P, _DP, _DP]),
     ("kkt_solve_device", _I, [_P, _P, _P]),
     ("kkt_solve_multi", _I, [_P, _I, _DP, _DP]),

@@ -2274,6 +2274,21 @@ int kkt_get_forward_forms(kkt_solver* h, int* launches) {
   return KKT_OK;
 }
 
+int kkt_set_forward_forms(kkt_solver* h, const int* launches) {
+  CHECK_H(h);
+  if (h->state < kkt_solver::ANALYZED) return h->fail(KKT_ERR_STATE, "no analysis");
+  if (!launches) return h->fail(KKT_ERR_ARGUMENT, "null forms");
+  const int nl = (int)h->fr_fuse.size();
+  for (int L = 0; L < nl; L++)
+    if (launches[L] != 1 && launches[L] != 2) return h->fail(KKT_ERR_ARGUMENT, "forward form must be 1 or 2");
+  for (int L = 0; L < nl; L++) h->fr_fuse[L] = launches[L] == 1 ? 1 : 0;
+  int rc;
+  if ((rc = h->build_solve_items())) return rc;
+  h->drop_graphs();
+  h->ok();
+  return KKT_OK;
+}
+
 int kkt_timing_get(kkt_solver* h, double* ms, long long* launches, double* update_flops) {
   CHECK_H(h);
   for (int c = 0; c < TC_N; c++) {

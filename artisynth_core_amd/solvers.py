@@ -446,6 +446,22 @@ class MI355XSolver:
                     "getSolveShapes")
         return f[:nl], b[:nl]
 
+    def getForwardForms(self):
+        """Forward launches of every tree level: 1 (row blocks gather for
+        themselves) or 2 (a gather launch, then the rows)."""
+        nl = self.info()["nlevels"]
+        f = np.zeros(max(nl, 1), np.int32)
+        self._check(self._L.kkt_get_forward_forms(self._h, ptr(f, ctypes.c_int)), "getForwardForms")
+        return f[:nl]
+
+    def setForwardForms(self, forms):
+        """Sets the forward form of every tree level (1 or 2 each); the solution
+        is bitwise the same either way."""
+        f = np.ascontiguousarray(forms, np.int32)
+        if len(f) != self.info()["nlevels"]:
+            raise ValueError("one form per tree level expected")
+        self._check(self._L.kkt_set_forward_forms(self._h, ptr(f, ctypes.c_int)), "setForwardForms")
+
     def factorDeviceValues(self):
         self._need_analyze()
         self._check(self._L.kkt_factor_device_values(self._h), "factorDeviceValues")

@@ -265,6 +265,9 @@ int kkt_get_solve_shapes(kkt_solver* h, int* fwd_elems, int* bwd_elems);
 /* The forward form per tree level (kkt_info.nlevels entries): 1 = one launch
  * (the default), 2 = a gather launch, then the rows (autotuner's pick). */
 int kkt_get_forward_forms(kkt_solver* h, int* launches);
+/* Sets the forward form per tree level (1 or 2, as above), e.g. to pin a tuned
+ * plan; the solution is bitwise the same either way. */
+int kkt_set_forward_forms(kkt_solver* h, const int* launches);
 int kkt_timing_get(kkt_solver* h, double* ms, long long* launches, double* update_flops);
 
 /* Host-only symbolic analysis (no GPU needed): structure statistics and the

@@ -141,6 +141,45 @@ def test_delaunay_tet_kkt_vs_oracle(gpu, prec):
         s.dispose()
 
 
+@pytest.mark.parametrize("prec", [KKT_PREC_FP64, KKT_PREC_FP32_REFINE])
+def test_forward_forms_are_bitwise_equal(gpu, prec):
+    # the forward sweep of a level runs as one launch (every row block of a
+    # large front gathers the front's right-hand side itself) or as two (a gather
+    # launch, then the rows); both add the same terms in the same order, so the
+    # solve (triangular solves and refinement) is bitwise the same for any mix of
+    # forms, on graph replay and eager launches alike, and matches the oracle
+    _say("forward forms prec=%d" % prec)
+    body, V, B = G.tet_precision_sequence(nsteps=2, dims=(21, 6, 6))
+    vals, b = V[1], B[1]
+    s = MI355XSolver(precision=prec)
+    try:
+        s.analyzeArrays(vals, body.colIdxs, body.rowOffs, body.n, Matrix.SYMMETRIC)
+        s.factor(vals)
+        nl = s.info()["nlevels"]
+        assert list(s.getForwardForms()) == [1] * nl  # the default
+        xs = []
+        rng = np.random.default_rng(5)
+        plans = [[1] * nl, [2] * nl, list(rng.integers(1, 3, nl)), [1] * nl]
+        for graphs in (1, 0):
+            s._L.kkt_set_graphs(s._h, graphs)
+            for forms in plans:
+                s.setForwardForms(forms)
+                assert list(s.getForwardForms()) == [int(f) for f in forms]
+                x = np.zeros(body.n)
+                s.solve(x, b)
+                xs.append(x)
+        for x in xs[1:]:
+            assert np.array_equal(x, xs[0])
+        sym = s.symbolic()
+        ref = O.Factor(body.rowOffs, body.colIdxs, vals, perm=sym["perm"], sup_ptr=sym["sup_ptr"])
+        assert rel_residual(body.rowOffs, body.colIdxs, vals, xs[0], b) <= 1e-8
+        assert rel_err(xs[0], ref.solve(b)) <= 1e-8
+        with pytest.raises(Exception):
+            s.setForwardForms([3] * nl)
+    finally:
+        s.dispose()
+
+
 def test_iterative_solve_stale_factor(gpu):
     # CuDssSolverTest.java:993-1094 / PardisoSolverTest.java:668-688: values perturbed
     # after the factor; BiCGStab with the stale factor reaches 1e-9

@@ -121,6 +121,9 @@ def test_abi_argument_and_state_errors_without_gpu():
         assert L.kkt_set_precision(h, 7) == -14
         assert L.kkt_set_refine_tol(h, -1.0, 3) == -14
         assert L.kkt_set_ir_steps(h, -1) == -14
+        forms = np.ones(2, np.int32)
+        assert L.kkt_set_forward_forms(h, _lib.ptr(forms, _lib.ctypes.c_int)) == -8  # STATE: no analysis
+        assert L.kkt_get_forward_forms(h, _lib.ptr(forms, _lib.ctypes.c_int)) == -8
     finally:
         L.kkt_destroy(h)
     L.kkt_destroy(None)
