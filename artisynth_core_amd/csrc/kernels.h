@@ -12,6 +12,9 @@ constexpr int UPD_GROUP = 4;  // panels whose Schur updates are applied together
 // whole forward step (gather + rows) in one workgroup
 constexpr int FWD_SMALL_K = 32;
 constexpr int FWD_SMALL_M = 256;
+// widest supernode the solve kernels take (LDS vectors of KMAX entries); the
+// symbolic analysis splits wider ones, and the solve plan checks it
+constexpr int KMAX = 1024;
 // right-hand sides one multi-RHS solve pass carries (vectors stored one after
 // another, each with its own leading dimension)
 constexpr int SOLVE_NRHS = 8;

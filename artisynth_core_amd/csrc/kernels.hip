@@ -845,7 +845,6 @@ __global__ __launch_bounds__(256) void k_panel_xrow(const int2* __restrict__ ite
 // so every product streams contiguous columns, and a level costs one launch
 // forward (two where a separate gather launch is faster) and one backward.
 // ------------------------------------------------------------------------
-constexpr int KMAX = 1024;  // widest supernode (the symbolic analysis splits wider ones)
 
 // Z^T(c, j) = sum_{i >= c} X(i, c) L21(j, i) into front (row c, column k + j);
 // item = (supernode, 64-column block of Z, 64-row block of L21)

@@ -494,6 +494,7 @@ struct kkt_solver {
       ff_l[L].off = (long long)ffv.size();
       for (int s : sv_lv[L]) {
         const int k = sv_k[s], m = sv_m[s];
+        if (k > KMAX) return fail(KKT_ERR_ANALYSIS, "supernode wider than the solve kernels take");
         if (k <= FWD_SMALL_K && m <= FWD_SMALL_M) {
           // small front: gather and rows in one workgroup (the first forward launch)
           fgv.push_back(item(s, 0, 0, 0));
