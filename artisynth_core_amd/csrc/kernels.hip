@@ -1041,13 +1041,15 @@ __device__ inline void fwd_small_front(const SolveItem& it, const DevSym& d, con
   const int k = it.k, m = it.m, first = it.first;
   const int p = threadIdx.x;
   // row p of G from the lower copy (column l, row p): consecutive threads read
-  // consecutive entries; all FWD_SMALL_K loads issue before the gather (a column
-  // past the row's length re-reads column 0 and is dropped by a select)
+  // consecutive entries (a column past the row's length re-reads column 0 and is
+  // dropped by a select).  The first HOIST loads issue before the gather, the
+  // rest after it (FP64: half, so the kernel keeps more waves resident).
+  constexpr int HOIST = sizeof(T) == 8 ? FWD_SMALL_K / 2 : FWD_SMALL_K;
   const int len = (p < m) ? min(p, k) : 0;
   const T* rowp = F + it.foff + (p < m ? p : 0);
   T v[FWD_SMALL_K];
 #pragma unroll
-  for (int l = 0; l < FWD_SMALL_K; l++) v[l] = rowp[(size_t)(l < len ? l : 0) * m];
+  for (int l = 0; l < HOIST; l++) v[l] = rowp[(size_t)(l < len ? l : 0) * m];
   T acc = (T)0;
   int ip = 0;
   if (p < m) {
@@ -1074,6 +1076,8 @@ __device__ inline void fwd_small_front(const SolveItem& it, const DevSym& d, con
   if (p < k) fs[ip] = acc;
   __syncthreads();
   if (p < m) {
+#pragma unroll
+    for (int l = HOIST; l < FWD_SMALL_K; l++) v[l] = rowp[(size_t)(l < len ? l : 0) * m];
     T acc = (T)0;
 #pragma unroll
     for (int l = 0; l < FWD_SMALL_K; l++) {
