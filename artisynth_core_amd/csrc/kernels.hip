@@ -1193,14 +1193,15 @@ __global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__
   const bool own = sl == 0 && i < m;
   // the first chunk of this lane's part of row i (the seg_dot order: chunks
   // sl, sl + seg, ..., 16 bytes each), loaded before the gather
-  constexpr int W = 16 / sizeof(T), GC = 4;
+  // (PF chunks; the chunks are added in ascending order whatever the grouping)
+  constexpr int W = 16 / sizeof(T), GC = 4, PF = 3;
   typedef typename V16<T>::v vt;
   const T* col = F + it.foff + (size_t)(i < m ? i : 0) * m;
   const int len = (i < m) ? min(i, k) : 0;
   const int nc = (len + W - 1) / W;
-  vt x0[GC];
+  vt x0[PF];
 #pragma unroll
-  for (int q = 0; q < GC; q++) {
+  for (int q = 0; q < PF; q++) {
     const int c = sl + q * seg;
     x0[q] = ldv(col + (c < nc ? c : 0) * W);
   }
@@ -1252,7 +1253,7 @@ __global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__
   // the same order; a chunk past the row adds nothing)
   T acc = (T)0;
 #pragma unroll
-  for (int q = 0; q < GC; q++) {
+  for (int q = 0; q < PF; q++) {
     const int c = sl + q * seg;
     const vt yv = ldv(fs + (c < nc ? c : 0) * W);
 #pragma unroll
@@ -1261,7 +1262,7 @@ __global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__
       acc = (c * W + w < len) ? t : acc;
     }
   }
-  for (int q0 = sl + GC * seg; q0 < nc; q0 += GC * seg) {
+  for (int q0 = sl + PF * seg; q0 < nc; q0 += GC * seg) {
     vt x[GC], yv[GC];
 #pragma unroll
     for (int q = 0; q < GC; q++) {
