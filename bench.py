@@ -599,6 +599,9 @@ def precision_run(dims, nsteps, warmup, with_oracle):
         for _ in range(max(warmup, 1)):
             assert L.kkt_factor_device(h, vals_d[0].data_ptr()) == 0, L.kkt_last_error(h)
             assert L.kkt_solve_device(h, b_d[0].data_ptr(), x_d.data_ptr()) == 0
+        # the solve-shape autotuner on the warm-up factor (untimed setup, as in the
+        # sequence bench)
+        assert L.kkt_autotune_solve(h, 3) == 0, L.kkt_last_error(h)
         torch.cuda.synchronize()
         tf, ts, xs, info = [], [], [], []
         for k in range(len(V)):
