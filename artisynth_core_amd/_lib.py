@@ -35,6 +35,17 @@ class KktInfo(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class KktBatchSysInfo(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int), ("npos", ctypes.c_int), ("nneg", ctypes.c_int),
+        ("nperturbed", ctypes.c_int), ("n2x2", ctypes.c_int), ("refine_steps", ctypes.c_int),
+        ("last_rel_residual", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 # (name, restype, argtypes) for every symbol of include/kkt_mi355x.h
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -90,6 +101,20 @@ SIGNATURES = [
     ("kkt_verify_device_structure", _I, [_P]),
     ("kkt_timing_reset", None, [_P, _I]),
     ("kkt_timing_get", _I, [_P, _DP, _LLP, _DP]),
+    ("kkt_batch_create", _P, []),
+    ("kkt_batch_destroy", None, [_P]),
+    ("kkt_batch_set_systems", _I, [_P, _I, _IP, _IP, _IP, _I]),
+    ("kkt_batch_analyze", _I, [_P, _DP]),
+    ("kkt_batch_factor", _I, [_P, _DP]),
+    ("kkt_batch_factor_device", _I, [_P, _P]),
+    ("kkt_batch_solve", _I, [_P, _DP, _DP]),
+    ("kkt_batch_solve_device", _I, [_P, _P, _P]),
+    ("kkt_batch_set_precision", _I, [_P, _I]),
+    ("kkt_batch_set_refine_tol", _I, [_P, _D, _I]),
+    ("kkt_batch_set_ir_steps", _I, [_P, _I]),
+    ("kkt_batch_get_info", _I, [_P, ctypes.POINTER(KktInfo)]),
+    ("kkt_batch_system_info", _I, [_P, _I, ctypes.POINTER(KktBatchSysInfo)]),
+    ("kkt_batch_last_error", ctypes.c_char_p, [_P]),
     ("kkt_symbolic_create", _P, [_I, _I, _IP, _IP, _UCP, _I]),
     ("kkt_symbolic_info", _I, [_P, ctypes.POINTER(KktInfo)]),
     ("kkt_symbolic_get", _I, [_P, _IP, _IP, _IP, _IP, _IP, _LLP, _IP, _IP]),

@@ -1,65 +1,79 @@
 """Batched KKT solves: many independent systems of uneven size factored and solved
-together.
+together, through the library's kkt_batch_* entry points (include/kkt_mi355x.h).
 
-The systems are concatenated into one block-diagonal upper CSR.  Its assembly
-forest (one tree per system) goes through the same level-scheduled kernels as a
-single large system: every launch covers the fronts of all systems at the same
-tree level, and the whole factor (and the whole triangular solve) is one
-captured HIP graph launch.  Per-system results are slices of the batch vectors.
+The library concatenates the systems into one block-diagonal upper CSR whose
+assembly forest holds one tree per system, so every factor and triangular-solve
+launch covers the fronts of all systems at one tree level, and the whole factor
+(and the whole triangular solve) is one captured HIP graph launch.  Pivot
+perturbation, inertia and the refinement stopping test are per system.  This
+module only packs the per-system arrays and reads results back.
 """
 import ctypes
 
 import numpy as np
 
-from . import generators as G
-from ._lib import KKT_PREC_FP64, lib
+from ._lib import KKT_PREC_FP64, KktBatchSysInfo, KktInfo, lib
 from ._lib import ptr as nptr
 
 
 class BatchedKKT:
     """Factor/solve a list of (rowOffs, colIdxs, vals, b) upper-CSR systems as one batch."""
 
-    def __init__(self, systems, precision=KKT_PREC_FP64, ir_steps=1):
+    def __init__(self, systems, precision=KKT_PREC_FP64, ir_steps=1, refine_tol=None):
         self.L = lib()
-        ro, ci, va, b, offs = G.block_diag_csr(systems)
-        self.rowOffs, self.colIdxs, self.vals, self.b, self.offs = ro, ci, va, b, offs
-        self.n = len(ro) - 1
-        self.nnz = int(ro[-1])
         self.nsys = len(systems)
-        self.val_offs = np.zeros(self.nsys + 1, np.int64)
-        for s, sysd in enumerate(systems):
-            self.val_offs[s + 1] = self.val_offs[s] + int(sysd[0][-1])
-        h = self.L.kkt_create()
+        self.sys_n = np.array([len(s[0]) - 1 for s in systems], np.int32)
+        self.rowOffs = np.ascontiguousarray(np.concatenate([np.asarray(s[0], np.int32) for s in systems]))
+        self.colIdxs = np.ascontiguousarray(np.concatenate([np.asarray(s[1], np.int32) for s in systems]))
+        self.vals = np.ascontiguousarray(np.concatenate([np.asarray(s[2], np.float64) for s in systems]))
+        self.b = np.ascontiguousarray(np.concatenate([np.asarray(s[3], np.float64) for s in systems]))
+        self.offs = np.concatenate([[0], np.cumsum(self.sys_n, dtype=np.int64)])
+        self.n = int(self.offs[-1])
+        self.nnz = len(self.vals)
+        h = self.L.kkt_batch_create()
         if not h:
-            raise RuntimeError("kkt_create failed")
+            raise RuntimeError("kkt_batch_create failed")
         self.h = h
-        self._check(self.L.kkt_set_precision(h, precision), "setPrecision")
-        self._check(self.L.kkt_set_ir_steps(h, ir_steps), "setIrSteps")
-        self._check(self.L.kkt_set_pattern(h, self.n, self.nnz, nptr(ro, ctypes.c_int),
-                                           nptr(ci, ctypes.c_int), 1), "setPattern")
-        self._check(self.L.kkt_analyze_values(h, nptr(va, ctypes.c_double)), "analyze")
+        self._check(self.L.kkt_batch_set_precision(h, precision), "setPrecision")
+        self._check(self.L.kkt_batch_set_ir_steps(h, ir_steps), "setIrSteps")
+        if refine_tol is not None:
+            self._check(self.L.kkt_batch_set_refine_tol(h, refine_tol[0], refine_tol[1]), "setRefineTol")
+        self._check(self.L.kkt_batch_set_systems(h, self.nsys, nptr(self.sys_n, ctypes.c_int),
+                                                 nptr(self.rowOffs, ctypes.c_int),
+                                                 nptr(self.colIdxs, ctypes.c_int), 1), "setSystems")
+        self._check(self.L.kkt_batch_analyze(h, nptr(self.vals, ctypes.c_double)), "analyze")
 
     def _check(self, rc, op):
         if rc != 0:
-            m = self.L.kkt_last_error(self.h)
+            m = self.L.kkt_batch_last_error(self.h)
             raise RuntimeError("batch %s failed (%d): %s" % (op, rc, m.decode() if m else ""))
 
     def factor_device(self, d_vals):
-        self._check(self.L.kkt_factor_device(self.h, d_vals), "factor")
+        self._check(self.L.kkt_batch_factor_device(self.h, d_vals), "factor")
 
     def solve_device(self, d_b, d_x):
-        self._check(self.L.kkt_solve_device(self.h, d_b, d_x), "solve")
+        self._check(self.L.kkt_batch_solve_device(self.h, d_b, d_x), "solve")
 
     def factor(self, vals=None):
         v = np.ascontiguousarray(self.vals if vals is None else vals, np.float64)
-        self._check(self.L.kkt_factor(self.h, nptr(v, ctypes.c_double)), "factor")
+        self._check(self.L.kkt_batch_factor(self.h, nptr(v, ctypes.c_double)), "factor")
 
     def solve(self, b=None):
         bb = np.ascontiguousarray(self.b if b is None else b, np.float64)
         x = np.zeros(self.n)
-        self._check(self.L.kkt_solve(self.h, nptr(bb, ctypes.c_double), nptr(x, ctypes.c_double)),
+        self._check(self.L.kkt_batch_solve(self.h, nptr(bb, ctypes.c_double), nptr(x, ctypes.c_double)),
                     "solve")
         return x
+
+    def info(self):
+        inf = KktInfo()
+        self._check(self.L.kkt_batch_get_info(self.h, ctypes.byref(inf)), "getInfo")
+        return inf.as_dict()
+
+    def system_info(self, q):
+        si = KktBatchSysInfo()
+        self._check(self.L.kkt_batch_system_info(self.h, int(q), ctypes.byref(si)), "systemInfo")
+        return si.as_dict()
 
     def split(self, x):
         return [x[self.offs[s]:self.offs[s + 1]] for s in range(self.nsys)]
@@ -77,7 +91,7 @@ class BatchedKKT:
 
     def dispose(self):
         if getattr(self, "h", None):
-            self.L.kkt_destroy(self.h)
+            self.L.kkt_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

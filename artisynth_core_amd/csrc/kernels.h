@@ -48,6 +48,8 @@ struct DevSym {
   const int* push_ptr;
   const int* push_pos;
   const int* wpos;
+  // connected component (assembly tree) of each supernode, 0..nroots-1
+  const int* sn_root;
 };
 
 // Work item of the solve kernels: one workgroup's share of a supernode, with the
@@ -78,7 +80,7 @@ void launch_scatter(const int* aidx, const long long* apos, int cnt, const doubl
                     hipStream_t st);
 template <typename T>
 void launch_panel_diag(const int* items, int nitems, int p, const DevSym& d, T* F, T* E0, T* E1,
-                       double pert, const double* amax, FactorCounters* ctr, hipStream_t st);
+                       double pert, const double* amax, FactorCounters* ctr, int* rctr, hipStream_t st);
 // W of panel p goes to the supernode's W-buffer slot `slot` (m * NB each)
 template <typename T>
 void launch_panel_trsm(const int2* items, int nitems, int p, int slot, const DevSym& d, T* F, T* W,
@@ -205,6 +207,19 @@ void launch_bicg_xr(int n, const double* sc, const double* y, const double* z, c
 void launch_bicg_xa(int n, const double* sc, const double* y, double* x, hipStream_t st);
 void launch_bicg_scalars(double* sc, int init, hipStream_t st);
 void launch_fill(long long n, double v, double* x, hipStream_t st);
-void launch_zero_counters(FactorCounters* c, hipStream_t st);
+// zeroes the totals and the nr per-component counters (four per component:
+// positive, negative, perturbed pivots, 2x2 pivots)
+void launch_zero_counters(FactorCounters* c, int* rctr, int nr, hipStream_t st);
+// out[g] = sum of v_i^2 over the ranges of segment g: chunks [ca[q], cb[q]) for
+// q in [cptr[g], cptr[g+1]), summed in a fixed order; partial holds nchunks doubles
+void launch_seg_sumsq(int nchunks, const long long* ca, const long long* cb, const double* v, double* partial,
+                      int nseg, const int* cptr, double* out, hipStream_t st);
+// x_i += d_i where active[row_seg[i]] != 0
+void launch_axpy_masked(long long n, const int* row_seg, const unsigned char* active, const double* d, double* x,
+                        hipStream_t st);
+// amax[r] = max |v| over the value ranges of component r: chunks [ca[q], cb[q])
+// for q in [cptr[r], cptr[r+1]); partial holds nchunks doubles
+void launch_absmax_roots(int nchunks, const long long* ca, const long long* cb, const double* v,
+                         double* partial, int nroots, const int* cptr, double* amax, hipStream_t st);
 
 }  // namespace kkt

@@ -14,6 +14,7 @@
 // per level and sweep (forward bottom-up, backward top-down) with the update
 // vectors of the multifrontal scheme.  No floating-point atomics are used
 // anywhere: results are bitwise reproducible run to run.
+#include <algorithm>
 #include "kernels.h"
 #include "wave.h"
 
@@ -141,7 +142,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                     T* __restrict__ F, T* __restrict__ E0,
                                                     T* __restrict__ E1, double pert,
                                                     const double* __restrict__ amax,
-                                                    FactorCounters* __restrict__ ctr) {
+                                                    FactorCounters* __restrict__ ctr,
+                                                    int* __restrict__ rctr) {
   __shared__ double A[64 * 65];     // A[row*65 + col]: lower part L/D, upper part X
   __shared__ double dv[64][3];      // 1x1: {1/d, d}; 2x2 (first column): D^-1 {i11, i21, i22}
   __shared__ double Sb[64 * 16];    // fast path: Sb[q*64 + t] = unscaled a(t, c0 + q) of the slab
@@ -176,7 +178,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   if (tid == 0) sfail = 0;
   __syncthreads();
   const double alpha = (1.0 + sqrt(17.0)) / 8.0;
-  const double tau = pert * amax[0];
+  // perturbation threshold: pert * max|a_ij| of the front's connected component
+  // (its assembly tree), so a system's pivots do not depend on other systems
+  // factored in the same batch
+  const int root = d.sn_root[s];
+  const double tau = pert * amax[root];
   int npos = 0, nneg = 0, npert = 0, n2 = 0;
   // Fast path: Bunch-Kaufman takes the 1x1 pivot without interchange whenever
   // |a_kk| >= alpha * colmax (or perturbs a pivot below tau).  Wave 0 factors the
@@ -534,6 +540,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     atomicAdd(&ctr->npert, npert);
     atomicAdd(&ctr->n2x2, n2);
     if (!fast) atomicAdd(&ctr->npivoted, 1);
+    int* rc = rctr + 4 * root;
+    atomicAdd(rc, npos);
+    atomicAdd(rc + 1, nneg);
+    atomicAdd(rc + 2, npert);
+    atomicAdd(rc + 3, n2);
   }
 }
 
@@ -2084,8 +2095,64 @@ __global__ void k_fill(long long n, double v, double* __restrict__ x) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   if (i < n) x[i] = v;
 }
-__global__ void k_zero_counters(FactorCounters* c) {
-  if (threadIdx.x == 0) { c->npos = 0; c->nneg = 0; c->npert = 0; c->n2x2 = 0; c->npivoted = 0; }
+__global__ void k_zero_counters(FactorCounters* c, int* __restrict__ rctr, int nr) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i == 0) { c->npos = 0; c->nneg = 0; c->npert = 0; c->n2x2 = 0; c->npivoted = 0; }
+  if (i < nr) rctr[i] = 0;
+}
+// max |v| over each chunk [ca[q], cb[q]) of the value array (one workgroup each)
+__global__ __launch_bounds__(256) void k_absmax_chunks(const long long* __restrict__ ca,
+                                                       const long long* __restrict__ cb,
+                                                       const double* __restrict__ v,
+                                                       double* __restrict__ partial) {
+  __shared__ double sm[4];
+  const long long a = ca[blockIdx.x], b = cb[blockIdx.x];
+  double acc = 0.0;
+  for (long long i = a + threadIdx.x; i < b; i += 256) acc = fmax(acc, fabs(v[i]));
+  acc = wave_max(acc);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+}
+// sum of squares of v over each chunk [ca[q], cb[q]) (one workgroup each, fixed
+// order: thread-strided partial sums, wave sums, then the four waves in order)
+__global__ __launch_bounds__(256) void k_sumsq_chunks(const long long* __restrict__ ca,
+                                                      const long long* __restrict__ cb,
+                                                      const double* __restrict__ v,
+                                                      double* __restrict__ partial) {
+  __shared__ double sm[4];
+  const long long a = ca[blockIdx.x], b = cb[blockIdx.x];
+  double acc = 0.0;
+  for (long long i = a + threadIdx.x; i < b; i += 256) acc += v[i] * v[i];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+}
+// per segment g: the sum of its chunks' partial sums in chunk order
+__global__ void k_seg_sum(int nseg, const int* __restrict__ cptr, const double* __restrict__ partial,
+                          double* __restrict__ out) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= nseg) return;
+  double acc = 0.0;
+  for (int q = cptr[g]; q < cptr[g + 1]; q++) acc += partial[q];
+  out[g] = acc;
+}
+// x_i += d_i for the rows whose segment is still active
+__global__ void k_axpy_masked(long long n, const int* __restrict__ row_seg,
+                              const unsigned char* __restrict__ active, const double* __restrict__ d,
+                              double* __restrict__ x) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n && active[row_seg[i]]) x[i] += d[i];
+}
+// per component r: max of its chunks' maxima, chunks [cptr[r], cptr[r+1])
+__global__ void k_absmax_roots(int nroots, const int* __restrict__ cptr, const double* __restrict__ partial,
+                               double* __restrict__ amax) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= nroots) return;
+  double acc = 0.0;
+  for (int q = cptr[r]; q < cptr[r + 1]; q++) acc = fmax(acc, partial[q]);
+  amax[r] = acc;
 }
 
 // ------------------------------------------------------------------------
@@ -2105,10 +2172,10 @@ void launch_scatter(const int* aidx, const long long* apos, int cnt, const doubl
 }
 template <typename T>
 void launch_panel_diag(const int* items, int nitems, int p, const DevSym& d, T* F, T* E0, T* E1,
-                       double pert, const double* amax, FactorCounters* ctr, hipStream_t st) {
+                       double pert, const double* amax, FactorCounters* ctr, int* rctr, hipStream_t st) {
   if (nitems > 0)
     hipLaunchKernelGGL(k_panel_diag<T>, dim3(nitems), dim3(256), 0, st, items, p, d, F, E0, E1, pert,
-                       amax, ctr);
+                       amax, ctr, rctr);
 }
 template <typename T>
 void launch_panel_trsm(const int2* items, int nitems, int p, int slot, const DevSym& d, T* F, T* W,
@@ -2320,8 +2387,26 @@ void launch_bicg_scalars(double* sc, int init, hipStream_t st) {
 void launch_fill(long long n, double v, double* x, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, st, n, v, x);
 }
-void launch_zero_counters(FactorCounters* c, hipStream_t st) {
-  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, c);
+void launch_zero_counters(FactorCounters* c, int* rctr, int nr, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_counters, dim3(nblk(std::max(nr, 1), 256)), dim3(256), 0, st, c, rctr, nr);
+}
+void launch_seg_sumsq(int nchunks, const long long* ca, const long long* cb, const double* v, double* partial,
+                      int nseg, const int* cptr, double* out, hipStream_t st) {
+  if (nchunks > 0) hipLaunchKernelGGL(k_sumsq_chunks, dim3(nchunks), dim3(256), 0, st, ca, cb, v, partial);
+  if (nseg > 0)
+    hipLaunchKernelGGL(k_seg_sum, dim3(nblk(nseg, 256)), dim3(256), 0, st, nseg, cptr, (const double*)partial,
+                       out);
+}
+void launch_axpy_masked(long long n, const int* row_seg, const unsigned char* active, const double* d, double* x,
+                        hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_axpy_masked, dim3(nblk(n, 256)), dim3(256), 0, st, n, row_seg, active, d, x);
+}
+void launch_absmax_roots(int nchunks, const long long* ca, const long long* cb, const double* v,
+                         double* partial, int nroots, const int* cptr, double* amax, hipStream_t st) {
+  if (nchunks > 0) hipLaunchKernelGGL(k_absmax_chunks, dim3(nchunks), dim3(256), 0, st, ca, cb, v, partial);
+  if (nroots > 0)
+    hipLaunchKernelGGL(k_absmax_roots, dim3(nblk(nroots, 256)), dim3(256), 0, st, nroots, cptr,
+                       (const double*)partial, amax);
 }
 
 #define KKT_INST(T)                                                                              \
@@ -2329,7 +2414,7 @@ void launch_zero_counters(FactorCounters* c, hipStream_t st) {
   template void launch_scatter<T>(const int*, const long long*, int, const double*, T*,          \
                                   hipStream_t);                                                  \
   template void launch_panel_diag<T>(const int*, int, int, const DevSym&, T*, T*, T*, double,   \
-                                     const double*, FactorCounters*, hipStream_t);               \
+                                     const double*, FactorCounters*, int*, hipStream_t);         \
   template void launch_panel_trsm<T>(const int2*, int, int, int, const DevSym&, T*, T*, hipStream_t); \
   template void launch_panel_update<T>(const int4*, int, int, const DevSym&, T*, const T*,       \
                                        hipStream_t);                                             \

@@ -82,6 +82,7 @@ struct kkt_solver {
   int ir_steps = 0;
   int last_refine_steps = 0;
   int fp64_fallbacks = 0;
+  bool fallback_active = false;  // the current FP64 factor replaced a stalled FP32 one
   double last_relres = 0.0;
   std::string err;
   bool has_err = false;
@@ -156,6 +157,15 @@ struct kkt_solver {
   char* d_stage = nullptr;      // per-call uploaded inputs
   size_t stage_cap = 0;         // bytes
   FactorCounters* d_ctr = nullptr;
+  // connected components (assembly trees): per-component max|a| for the pivot
+  // perturbation threshold and per-component inertia counters
+  int nroots = 0, nachunks = 0;
+  long long *d_achunk_a = nullptr, *d_achunk_b = nullptr;
+  int* d_acptr = nullptr;
+  double *d_amax = nullptr, *d_apart = nullptr;
+  int* d_rctr = nullptr;
+  std::vector<int> h_rctr;         // after a factor: 4 counters per component
+  std::vector<int> root_of_row;    // component of each (original) row
   std::vector<double> multiB;
   // captured triangular-solve graphs (per factor precision), replayed per solve
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -796,8 +806,46 @@ struct kkt_solver {
           ppos[fill[S.rows[roff[s] + j]]++] = (int)(voff[s] + kk[s] + j);
       }
     }
+    // connected components: the assembly trees, numbered in ascending order of
+    // their root supernode; each CSR row belongs to the tree of its pivot column.
+    // Value ranges of consecutive rows of one component are cut into chunks of at
+    // most 8192 entries for the per-component max|a| pass.
+    std::vector<int> sroot(ns, -1);
+    nroots = 0;
+    {
+      std::vector<int> rid(ns, -1);
+      for (int s = 0; s < ns; s++)
+        if (S.sparent[s] < 0) rid[s] = nroots++;
+      for (int s = ns - 1; s >= 0; s--) sroot[s] = S.sparent[s] < 0 ? rid[s] : sroot[S.sparent[s]];
+    }
+    root_of_row.assign(n, 0);
+    for (int i = 0; i < n; i++) root_of_row[i] = sroot[S.col2sup[S.iperm[i]]];
+    std::vector<std::vector<std::pair<long long, long long>>> rranges(std::max(nroots, 1));
+    for (int i = 0; i < n;) {
+      int j = i;
+      while (j + 1 < n && root_of_row[j + 1] == root_of_row[i]) j++;
+      rranges[root_of_row[i]].push_back({(long long)rowOffs[i], (long long)rowOffs[j + 1]});
+      i = j + 1;
+    }
+    std::vector<long long> ca, cb;
+    std::vector<int> cptr(std::max(nroots, 1) + 1, 0);
+    for (int r = 0; r < nroots; r++) {
+      for (auto& q : rranges[r])
+        for (long long a = q.first; a < q.second; a += 8192) {
+          ca.push_back(a);
+          cb.push_back(std::min(q.second, a + 8192));
+        }
+      cptr[r + 1] = (int)ca.size();
+    }
+    nachunks = (int)ca.size();
     int rc = 0;
-    int *d_first, *d_k, *d_m, *d_chp, *d_chl, *d_rel, *d_rows, *d_pptr, *d_ppos, *d_wpos;
+    int *d_first, *d_k, *d_m, *d_chp, *d_chl, *d_rel, *d_rows, *d_pptr, *d_ppos, *d_wpos, *d_sroot;
+    std::vector<double> zd(std::max(std::max(nachunks, nroots), 1), 0.0);
+    std::vector<int> zi(4 * std::max(nroots, 1), 0);
+    if ((rc = upload(d_achunk_a, ca)) || (rc = upload(d_achunk_b, cb)) || (rc = upload(d_acptr, cptr)) ||
+        (rc = upload(d_sroot, sroot)) || (rc = upload(d_amax, zd)) || (rc = upload(d_apart, zd)) ||
+        (rc = upload(d_rctr, zi)))
+      return rc;
     long long *d_foff, *d_roff, *d_woff, *d_uoff;
     if ((rc = upload(d_first, first)) || (rc = upload(d_k, kk)) || (rc = upload(d_m, mm)) ||
         (rc = upload(d_foff, foff)) || (rc = upload(d_roff, roff)) || (rc = upload(d_woff, woff)) ||
@@ -829,6 +877,7 @@ struct kkt_solver {
     ds.push_ptr = d_pptr; ds.push_pos = d_ppos; ds.wpos = d_wpos;
     ds.lperm = d_lperm; ds.ilperm = d_ilperm; ds.piv = d_piv;
     ds.sn_goff = d_goff; ds.gptr = d_gptr; ds.gsrc = d_gsrc;
+    ds.sn_root = d_sroot;
     // host copies of what was uploaded, for the structure-parity check
     h_first = first; h_k = kk; h_m = mm; h_foff = foff; h_roff = roff; h_chp = chp; h_chl = chl;
     h_rel = rel;
@@ -880,9 +929,9 @@ struct kkt_solver {
   template <typename T>
   void run_factor(T* F, T* W, double pert) {
     tbeg(TC_VEC);
-    launch_absmax(nnz, d_vals, d_partial, d_scal + 15, st);
+    launch_absmax_roots(nachunks, d_achunk_a, d_achunk_b, d_vals, d_apart, nroots, d_acptr, d_amax, st);
     tend();
-    launch_zero_counters(d_ctr, st);
+    launch_zero_counters(d_ctr, d_rctr, 4 * nroots, st);
     const int nl = S.nlevels;
     for (int L = 0; L < nl; L++) {
       tbeg(TC_ASM);
@@ -898,7 +947,7 @@ struct kkt_solver {
         }
         tbeg(TC_DIAG);
         launch_panel_diag<T>(d_diag + sp.diag.off, sp.diag.cnt, (int)p, ds, F, F + front_elems,
-                             F + front_elems + n, pert, d_scal + 15, d_ctr, st);
+                             F + front_elems + n, pert, d_amax, d_ctr, d_rctr, st);
         tend();
         tbeg(TC_TRSM);
         launch_panel_trsm<T>(d_trsm + sp.trsm.off, sp.trsm.cnt, (int)p, sp.slot, ds, F, W, st);
@@ -926,6 +975,7 @@ struct kkt_solver {
 
   int factor_device(const double* dv, bool from_host_copy) {
     if (state < ANALYZED) return fail(KKT_ERR_STATE, "factor called before analyze");
+    fallback_active = false;
     if (!from_host_copy && dv != d_vals) {
       if (hipMemcpyAsync(d_vals, dv, (size_t)nnz * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess)
         return fail(KKT_ERR_COPY, "hipMemcpyAsync (vals D2D) failed");
@@ -969,6 +1019,11 @@ struct kkt_solver {
     else body();
     hipError_t e = hipMemcpyAsync(&ctr, d_ctr, sizeof(FactorCounters), hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return fail(KKT_ERR_COPY, hip_msg("hipMemcpyAsync (counters) failed", e));
+    h_rctr.assign(4 * (size_t)nroots, 0);
+    if (nroots > 0 &&
+        (e = hipMemcpyAsync(h_rctr.data(), d_rctr, 4 * (size_t)nroots * sizeof(int), hipMemcpyDeviceToHost, st)) !=
+            hipSuccess)
+      return fail(KKT_ERR_COPY, hip_msg("hipMemcpyAsync (component counters) failed", e));
     if ((e = hipStreamSynchronize(st)) != hipSuccess)
       return fail(KKT_ERR_FACTORIZATION, hip_msg("factor kernels failed", e));
     tflush();
@@ -1139,7 +1194,8 @@ struct kkt_solver {
       double* X = d_X + (size_t)c0 * n;
       for (int j0 = 0; j0 < nb; j0 += SOLVE_NRHS)
         tri_solve_multi(B + (size_t)j0 * n, X + (size_t)j0 * n, std::min(SOLVE_NRHS, nb - j0));
-      const int want = (fprec == KKT_PREC_FP32_REFINE) ? refine_max : ir_steps;
+      const bool to_tol = fprec == KKT_PREC_FP32_REFINE || fallback_active;
+      const int want = to_tol ? refine_max : ir_steps;
       last_refine_steps = 0;
       if (want <= 0) continue;
       double bn[MULTI_BLOCK], prev[MULTI_BLOCK], rn2[MULTI_BLOCK];
@@ -1167,7 +1223,7 @@ struct kkt_solver {
           if (!active[j]) continue;
           const double rel = std::sqrt(rn2[j]) / bn[j];
           last_relres = rel;
-          if (fprec == KKT_PREC_FP32_REFINE) {
+          if (to_tol) {
             if (rel <= refine_tol) { active[j] = false; continue; }
             if (step > 0 && rel > 0.5 * prev[j]) stall[j]++; else stall[j] = 0;
             if (step >= refine_max || stall[j] >= 2 || !std::isfinite(rel)) fallback = true;
@@ -1193,7 +1249,10 @@ struct kkt_solver {
         last_refine_steps++;
       }
       if (fallback) {
-        // as solve_device: FP64 refactor with the same values, then each column
+        // as solve_device: the FP64 refactor with the same values (once), then
+        // each column of the block solved and refined to refine_tol on it; a
+        // column that still stalls returns KKT_ERR_REFINE_STALL
+        if (fprec == KKT_PREC_FP32_REFINE && (rc = fallback_to_fp64())) return rc;
         for (int j = 0; j < nb; j++)
           if ((rc = solve_device(B + (size_t)j * n, X + (size_t)j * n))) return rc;
       }
@@ -1348,13 +1407,31 @@ struct kkt_solver {
     tend();
   }
 
-  // refined solve on device vectors; d_b and d_x must not alias
+  // FP64 refactor with the current values after FP32 refinement stalled; the
+  // handle keeps the FP64 factor (and refines to refine_tol) until the next
+  // factor call
+  int fallback_to_fp64() {
+    fp64_fallbacks++;
+    int rc = do_factor(KKT_PREC_FP64);
+    if (rc) return rc;
+    fallback_active = true;
+    return KKT_OK;
+  }
+
+  // Refined solve on device vectors; d_b and d_x must not alias.  FP32 factors
+  // (and the FP64 factor of a fallback) refine to refine_tol: a stall (no 2x
+  // contraction for two steps, more than refine_max steps, or a non-finite
+  // residual) on an FP32 factor triggers the FP64 refactor and the refinement
+  // starts again from the FP64 solve; a stall on the FP64 factor returns
+  // KKT_ERR_REFINE_STALL with the best solution left in d_x.  Plain FP64
+  // factors take ir_steps refinement steps, as the bridge's IR setting.
   int solve_device(const double* d_b, double* d_x) {
     if (state != FACTORED) return fail(KKT_ERR_STATE, "solve called before factor");
     double t0 = g_timing ? now_ms() : 0;
     tri_solve(d_b, d_x);
     last_refine_steps = 0;
-    int want = (fprec == KKT_PREC_FP32_REFINE) ? refine_max : ir_steps;
+    const bool to_tol = fprec == KKT_PREC_FP32_REFINE || fallback_active;
+    const int want = to_tol ? refine_max : ir_steps;
     if (want > 0) {
       double* r = vec(0);
       double* dx = vec(1);
@@ -1371,23 +1448,24 @@ struct kkt_solver {
         if ((rc = dotv(r, r, rn2))) return rc;
         double rel = std::sqrt(rn2) / bn;
         last_relres = rel;
-        if (fprec == KKT_PREC_FP32_REFINE) {
+        if (to_tol) {
           if (rel <= refine_tol) break;
           if (step > 0 && rel > 0.5 * prev) stall++; else stall = 0;
           if (step >= refine_max || stall >= 2 || !std::isfinite(rel)) {
-            // refinement stalled: FP64 refactor fallback with the same values
-            fp64_fallbacks++;
-            if ((rc = do_factor(KKT_PREC_FP64))) return rc;
-            tri_solve(d_b, d_x);
-            for (int s2 = 0; s2 < 4; s2++) {
-              spmv(d_x, d_b, r, 1);
-              if ((rc = dotv(r, r, rn2))) return rc;
-              last_relres = std::sqrt(rn2) / bn;
-              if (last_relres <= refine_tol) break;
-              tri_solve(r, dx);
-              launch_axpby(n, 1.0, dx, 1.0, d_x, st);
+            if (fprec == KKT_PREC_FP32_REFINE) {
+              if ((rc = fallback_to_fp64())) return rc;
+              tri_solve(d_b, d_x);
+              step = -1;
+              stall = 0;
+              prev = 0.0;
+              continue;
             }
-            break;
+            (void)hipStreamSynchronize(st);
+            tflush();
+            char msg[160];
+            snprintf(msg, sizeof msg, "refinement stalled at relative residual %.3g > %.3g after the FP64 refactor",
+                     rel, refine_tol);
+            return fail(KKT_ERR_REFINE_STALL, msg);
           }
         } else if (step >= ir_steps) {
           break;
@@ -1629,6 +1707,238 @@ struct kkt_solver {
     if (g_timing)
       fprintf(stderr, "[kkt-timing] bicgstab n=%d iters=%d: total=%.2fms (%.2fms/iter)\n", n, it,
               now_ms() - t0, (now_ms() - t0) / std::max(1, it));
+    return KKT_OK;
+  }
+};
+
+// =====================================================================
+// Batched systems: many independent KKT systems of uneven size in one handle.
+// The systems are concatenated into one block-diagonal upper CSR whose
+// assembly forest holds one tree per system (one per connected component),
+// so every factor and triangular-solve launch covers all systems at one tree
+// level and each is one captured graph.  Pivot perturbation thresholds and
+// inertia are per component; refinement and its stopping test are per system
+// (segmented residual norms, corrections applied only to systems still
+// refining), so a system's result does not depend on the rest of the batch.
+// =====================================================================
+struct kkt_batch {
+  kkt_solver* h = nullptr;
+  int nsys = 0;
+  std::vector<int> sys_n;
+  std::vector<long long> row_off, val_off;  // nsys + 1 each
+  // per-system row chunks (at most 8192 rows) for the segmented norms
+  long long *d_rca = nullptr, *d_rcb = nullptr;
+  int *d_rcptr = nullptr, *d_row_sys = nullptr;
+  double *d_rpart = nullptr, *d_rnorm = nullptr;
+  unsigned char* d_active = nullptr;
+  int nrchunks = 0;
+  std::vector<int> refine_steps;
+  std::vector<double> relres;
+  std::string err;
+  bool has_err = false;
+
+  int fail(int code, const std::string& m) {
+    err = m;
+    has_err = true;
+    return code;
+  }
+  int from_h(int rc) {
+    if (rc) {
+      const char* m = kkt_last_error(h);
+      return fail(rc, m ? m : "batch operation failed");
+    }
+    has_err = false;
+    return KKT_OK;
+  }
+  void free_dev() {
+    DevOpLock lk(g_capture_mu);
+    for (void* q : {(void*)d_rca, (void*)d_rcb, (void*)d_rcptr, (void*)d_row_sys, (void*)d_rpart,
+                    (void*)d_rnorm, (void*)d_active})
+      if (q) (void)hipFree(q);
+    d_rca = d_rcb = nullptr;
+    d_rcptr = d_row_sys = nullptr;
+    d_rpart = d_rnorm = nullptr;
+    d_active = nullptr;
+  }
+  ~kkt_batch() {
+    if (h && h->dev_ok) (void)hipStreamSynchronize(h->st);
+    free_dev();
+    delete h;
+  }
+
+  int set_systems(int ns, const int* sysN, const int* ro, const int* ci, int mtype) {
+    if (ns <= 0 || !sysN || !ro || !ci) return fail(KKT_ERR_ARGUMENT, "bad batch arrays");
+    std::vector<int> gro(1, 0), gci;
+    std::vector<int> sn(ns);
+    std::vector<long long> roff(ns + 1, 0), voff(ns + 1, 0);
+    long long rpos = 0, cpos = 0;
+    for (int q = 0; q < ns; q++) {
+      const int n = sysN[q];
+      if (n <= 0) return fail(KKT_ERR_ARGUMENT, "system " + std::to_string(q) + " has no rows");
+      const int* r = ro + rpos;
+      const int nnz = r[n] - r[0];
+      if (r[0] != 0) return fail(KKT_ERR_PATTERN, "system " + std::to_string(q) + ": rowOffs must start at 0");
+      std::string e;
+      if (validate_pattern(n, nnz, r, ci + cpos, e) != 0)
+        return fail(KKT_ERR_PATTERN, "system " + std::to_string(q) + ": " + e);
+      if (roff[q] + n > INT32_MAX || (long long)gro.back() + nnz > INT32_MAX)
+        return fail(KKT_ERR_ARGUMENT, "batch larger than 2^31 rows or entries");
+      const int base = (int)roff[q], vbase = gro.back();
+      for (int i = 1; i <= n; i++) gro.push_back(vbase + r[i]);
+      for (int p = 0; p < nnz; p++) gci.push_back(base + ci[cpos + p]);
+      sn[q] = n;
+      roff[q + 1] = roff[q] + n;
+      voff[q + 1] = voff[q] + nnz;
+      rpos += n + 1;
+      cpos += nnz;
+    }
+    if (!h && !(h = kkt_create())) return fail(KKT_ERR_INIT, "handle creation failed");
+    free_dev();
+    int rc = from_h(kkt_set_pattern(h, (int)roff[ns], (int)gci.size(), gro.data(), gci.data(), mtype));
+    if (rc) return rc;
+    nsys = ns;
+    sys_n = sn;
+    row_off = roff;
+    val_off = voff;
+    refine_steps.assign(ns, 0);
+    relres.assign(ns, 0.0);
+    return KKT_OK;
+  }
+
+  int ensure_dev() {
+    if (d_rca) return KKT_OK;
+    std::vector<long long> ca, cb;
+    std::vector<int> cptr(nsys + 1, 0), rsys(row_off[nsys]);
+    for (int q = 0; q < nsys; q++) {
+      for (long long a = row_off[q]; a < row_off[q + 1]; a += 8192) {
+        ca.push_back(a);
+        cb.push_back(std::min(row_off[q + 1], a + 8192));
+      }
+      cptr[q + 1] = (int)ca.size();
+      for (long long i = row_off[q]; i < row_off[q + 1]; i++) rsys[i] = q;
+    }
+    nrchunks = (int)ca.size();
+    DevOpLock lk(g_capture_mu);
+    bool ok = hipMalloc(&d_rca, ca.size() * sizeof(long long)) == hipSuccess &&
+              hipMalloc(&d_rcb, cb.size() * sizeof(long long)) == hipSuccess &&
+              hipMalloc(&d_rcptr, cptr.size() * sizeof(int)) == hipSuccess &&
+              hipMalloc(&d_row_sys, rsys.size() * sizeof(int)) == hipSuccess &&
+              hipMalloc(&d_rpart, ca.size() * sizeof(double)) == hipSuccess &&
+              hipMalloc(&d_rnorm, nsys * sizeof(double)) == hipSuccess &&
+              hipMalloc(&d_active, nsys) == hipSuccess &&
+              hipMemcpy(d_rca, ca.data(), ca.size() * sizeof(long long), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d_rcb, cb.data(), cb.size() * sizeof(long long), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d_rcptr, cptr.data(), cptr.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d_row_sys, rsys.data(), rsys.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) return fail(KKT_ERR_ALLOC, "device allocation failed (batch norms)");
+    return KKT_OK;
+  }
+
+  // squared 2-norms of each system's slice of v (one synchronization)
+  int seg_norms(const double* v, std::vector<double>& out) {
+    launch_seg_sumsq(nrchunks, d_rca, d_rcb, v, d_rpart, nsys, d_rcptr, d_rnorm, h->st);
+    out.resize(nsys);
+    if (hipMemcpyAsync(out.data(), d_rnorm, nsys * sizeof(double), hipMemcpyDeviceToHost, h->st) != hipSuccess ||
+        hipStreamSynchronize(h->st) != hipSuccess)
+      return fail(KKT_ERR_REDUCTION, "segmented reduction failed");
+    return KKT_OK;
+  }
+
+  // Refined solve of every system, with the single-system rules per system:
+  // FP32 factors (and the FP64 factor of a fallback) refine each system to
+  // refine_tol; a stall in any system refactors the batch in FP64 (once) and
+  // the refinement restarts from the FP64 solve; a stall on the FP64 factor
+  // returns KKT_ERR_REFINE_STALL.  Plain FP64 factors take ir_steps steps.
+  int solve_device(const double* d_b, double* d_x) {
+    kkt_solver& S = *h;
+    if (S.state != kkt_solver::FACTORED) return fail(KKT_ERR_STATE, "solve called before factor");
+    int rc = ensure_dev();
+    if (rc) return rc;
+    S.tri_solve(d_b, d_x);
+    std::fill(refine_steps.begin(), refine_steps.end(), 0);
+    std::fill(relres.begin(), relres.end(), 0.0);
+    const bool to_tol = S.fprec == KKT_PREC_FP32_REFINE || S.fallback_active;
+    const int want = to_tol ? S.refine_max : S.ir_steps;
+    if (want > 0) {
+      double* r = S.vec(0);
+      double* dx = S.vec(1);
+      std::vector<double> bn, rn, prev(nsys, 0.0);
+      std::vector<int> stall(nsys, 0);
+      std::vector<unsigned char> active(nsys, 1);
+      if ((rc = seg_norms(d_b, bn))) return rc;
+      for (double& v : bn) v = (v == 0.0) ? 1.0 : std::sqrt(v);
+      for (int step = 0;; step++) {
+        S.spmv(d_x, d_b, r, 1);
+        if ((rc = seg_norms(r, rn))) return rc;
+        int nact = 0, stalled = -1;
+        for (int q = 0; q < nsys; q++) {
+          if (!active[q]) continue;
+          const double rel = std::sqrt(rn[q]) / bn[q];
+          relres[q] = rel;
+          if (to_tol) {
+            if (rel <= S.refine_tol) { active[q] = 0; continue; }
+            if (step > 0 && rel > 0.5 * prev[q]) stall[q]++; else stall[q] = 0;
+            if (step >= S.refine_max || stall[q] >= 2 || !std::isfinite(rel)) {
+              if (stalled < 0) stalled = q;
+            }
+          } else if (step >= S.ir_steps) {
+            active[q] = 0;
+            continue;
+          }
+          prev[q] = rel;
+          nact++;
+        }
+        if (stalled >= 0) {
+          if (S.fprec == KKT_PREC_FP32_REFINE) {
+            if ((rc = from_h(S.fallback_to_fp64() ? S.fail(KKT_ERR_FACTORIZATION, S.err) : KKT_OK))) return rc;
+            S.tri_solve(d_b, d_x);
+            std::fill(active.begin(), active.end(), 1);
+            std::fill(stall.begin(), stall.end(), 0);
+            std::fill(prev.begin(), prev.end(), 0.0);
+            std::fill(refine_steps.begin(), refine_steps.end(), 0);
+            step = -1;
+            continue;
+          }
+          (void)hipStreamSynchronize(S.st);
+          char msg[200];
+          snprintf(msg, sizeof msg, "refinement of system %d stalled at relative residual %.3g > %.3g after the "
+                   "FP64 refactor", stalled, relres[stalled], S.refine_tol);
+          return fail(KKT_ERR_REFINE_STALL, msg);
+        }
+        if (nact == 0) break;
+        if (hipMemcpyAsync(d_active, active.data(), nsys, hipMemcpyHostToDevice, S.st) != hipSuccess)
+          return fail(KKT_ERR_COPY, "hipMemcpyAsync (active systems) failed");
+        S.tri_solve(r, dx);
+        launch_axpy_masked(row_off[nsys], d_row_sys, d_active, dx, d_x, S.st);
+        for (int q = 0; q < nsys; q++) refine_steps[q] += active[q];
+      }
+    }
+    if (hipStreamSynchronize(S.st) != hipSuccess) return fail(KKT_ERR_SOLVE, "solve kernels failed");
+    has_err = false;
+    return KKT_OK;
+  }
+
+  int system_info(int q, kkt_batch_sysinfo* o) {
+    if (q < 0 || q >= nsys || !o) return fail(KKT_ERR_ARGUMENT, "system index out of range");
+    memset(o, 0, sizeof(*o));
+    o->n = sys_n[q];
+    o->refine_steps = refine_steps[q];
+    o->last_rel_residual = relres[q];
+    const kkt_solver& S = *h;
+    if (S.state < kkt_solver::ANALYZED) return fail(KKT_ERR_STATE, "no analysis");
+    if (S.state == kkt_solver::FACTORED && (int)S.h_rctr.size() == 4 * S.nroots) {
+      std::vector<char> seen(S.nroots, 0);
+      for (long long i = row_off[q]; i < row_off[q + 1]; i++) {
+        const int r = S.root_of_row[i];
+        if (seen[r]) continue;
+        seen[r] = 1;
+        o->npos += S.h_rctr[4 * r];
+        o->nneg += S.h_rctr[4 * r + 1];
+        o->nperturbed += S.h_rctr[4 * r + 2];
+        o->n2x2 += S.h_rctr[4 * r + 3];
+      }
+    }
+    has_err = false;
     return KKT_OK;
   }
 };
@@ -2343,5 +2653,106 @@ int kkt_symbolic_get(kkt_symbolic* s, int* perm, int* parent, int* colcount, int
 }
 
 void kkt_symbolic_destroy(kkt_symbolic* s) { delete s; }
+
+// ---- batched systems ----
+#define CHECK_B(b) \
+  if (!(b)) return KKT_ERR_ARGUMENT
+kkt_batch* kkt_batch_create(void) {
+  try {
+    return new kkt_batch();
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+void kkt_batch_destroy(kkt_batch* b) { delete b; }
+
+int kkt_batch_set_systems(kkt_batch* b, int nsys, const int* sysN, const int* rowOffs, const int* colIdxs,
+                          int mtype) {
+  CHECK_B(b);
+  int rc = b->set_systems(nsys, sysN, rowOffs, colIdxs, mtype);
+  if (!rc) b->has_err = false;
+  return rc;
+}
+
+int kkt_batch_analyze(kkt_batch* b, const double* vals) {
+  CHECK_B(b);
+  if (!b->h) return b->fail(KKT_ERR_STATE, "analyze called before setSystems");
+  b->free_dev();
+  return b->from_h(vals ? kkt_analyze_values(b->h, vals) : kkt_analyze(b->h));
+}
+
+int kkt_batch_factor(kkt_batch* b, const double* vals) {
+  CHECK_B(b);
+  if (!b->h) return b->fail(KKT_ERR_STATE, "factor called before setSystems");
+  return b->from_h(kkt_factor(b->h, vals));
+}
+
+int kkt_batch_factor_device(kkt_batch* b, const double* d_vals) {
+  CHECK_B(b);
+  if (!b->h) return b->fail(KKT_ERR_STATE, "factor called before setSystems");
+  return b->from_h(kkt_factor_device(b->h, d_vals));
+}
+
+int kkt_batch_solve_device(kkt_batch* b, const double* d_B, double* d_X) {
+  CHECK_B(b);
+  if (!b->h) return b->fail(KKT_ERR_STATE, "solve called before setSystems");
+  if (!d_B || !d_X || d_B == d_X) return b->fail(KKT_ERR_ARGUMENT, "null or aliased vectors");
+  return b->solve_device(d_B, d_X);
+}
+
+int kkt_batch_solve(kkt_batch* b, const double* B, double* X) {
+  CHECK_B(b);
+  if (!b->h) return b->fail(KKT_ERR_STATE, "solve called before setSystems");
+  if (!B || !X) return b->fail(KKT_ERR_ARGUMENT, "null vector");
+  kkt_solver* h = b->h;
+  if (h->state != kkt_solver::FACTORED) return b->fail(KKT_ERR_STATE, "solve called before factor");
+  double* db = h->vec(10);
+  double* dx = h->vec(11);
+  const size_t bytes = (size_t)h->n * sizeof(double);
+  if (hipMemcpyAsync(db, B, bytes, hipMemcpyHostToDevice, h->st) != hipSuccess)
+    return b->fail(KKT_ERR_COPY, "hipMemcpyAsync (B) failed");
+  int rc = b->solve_device(db, dx);
+  if (rc) return rc;
+  if (hipMemcpyAsync(X, dx, bytes, hipMemcpyDeviceToHost, h->st) != hipSuccess ||
+      hipStreamSynchronize(h->st) != hipSuccess)
+    return b->fail(KKT_ERR_COPY, "hipMemcpyAsync (X) failed");
+  return KKT_OK;
+}
+
+int kkt_batch_set_precision(kkt_batch* b, int prec) {
+  CHECK_B(b);
+  if (!b->h && !(b->h = kkt_create())) return b->fail(KKT_ERR_INIT, "handle creation failed");
+  return b->from_h(kkt_set_precision(b->h, prec));
+}
+
+int kkt_batch_set_refine_tol(kkt_batch* b, double tol, int max_steps) {
+  CHECK_B(b);
+  if (!b->h && !(b->h = kkt_create())) return b->fail(KKT_ERR_INIT, "handle creation failed");
+  return b->from_h(kkt_set_refine_tol(b->h, tol, max_steps));
+}
+
+int kkt_batch_set_ir_steps(kkt_batch* b, int nsteps) {
+  CHECK_B(b);
+  if (!b->h && !(b->h = kkt_create())) return b->fail(KKT_ERR_INIT, "handle creation failed");
+  return b->from_h(kkt_set_ir_steps(b->h, nsteps));
+}
+
+int kkt_batch_get_info(kkt_batch* b, kkt_info* out) {
+  CHECK_B(b);
+  if (!b->h) return b->fail(KKT_ERR_STATE, "no systems");
+  return b->from_h(kkt_get_info(b->h, out));
+}
+
+int kkt_batch_system_info(kkt_batch* b, int sys, kkt_batch_sysinfo* out) {
+  CHECK_B(b);
+  if (!b->h) return b->fail(KKT_ERR_STATE, "no systems");
+  return b->system_info(sys, out);
+}
+
+const char* kkt_batch_last_error(kkt_batch* b) {
+  if (!b || !b->has_err) return nullptr;
+  return b->err.c_str();
+}
 
 }  // extern "C"

@@ -496,11 +496,14 @@ class MI355XSolver:
         return rc
 
     def autoFactorAndSolve(self, x, b, tolExp):
-        """CuDssSolver.autoFactorAndSolve (:866-913)."""
+        """CuDssSolver.autoFactorAndSolve (:866-913).  lastAutoIterations records the
+        branch taken: the BiCGStab iterations with the stale factor, or 0 after a
+        factor + solve."""
         self._alive()
         if self._state == self.UNSET or self._matrix is None:
             raise ImproperStateException(
                 "analyze(Matrix) or analyzeAndFactor(Matrix) not previously called")
+        self.lastAutoIterations = 0
         if tolExp <= 0 or self._state != self.FACTORED:
             self.factor()
             self.solve(x, b)
@@ -510,6 +513,7 @@ class MI355XSolver:
         rc = self.iterativeSolve(vals, xx, _f64(b)[:self._size], tolExp)
         if rc > 0:
             x[:self._size] = xx
+            self.lastAutoIterations = rc
             return
         self.factor()
         self.solve(x, b)

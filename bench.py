@@ -473,9 +473,7 @@ def batch_run(nsys, reps, warmup):
         diff = max(diff, float(np.linalg.norm(xa - xs) / max(np.linalg.norm(xa), 1e-300)))
     for h in hs:
         L.kkt_destroy(h)
-    from artisynth_core_amd._lib import KktInfo
-    inf = KktInfo()
-    L.kkt_get_info(B.h, ctypes.byref(inf))
+    inf = B.info()
     t_b, t_1 = float(np.median(lat)), float(np.median(lat1))
     line = {"mode": "batch", "nsys": nsys, "n_total": B.n, "nnz_total": B.nnz,
             "batched_ms": t_b, "batched_p99_ms": float(np.percentile(lat, 99)),
@@ -483,7 +481,7 @@ def batch_run(nsys, reps, warmup):
             "batched_systems_per_s": nsys / (t_b * 1e-3),
             "per_system_systems_per_s": nsys / (t_1 * 1e-3),
             "max_rel_residual": max(res), "max_diff_alone_vs_batch": diff,
-            "factor_gflop": inf.factor_flops / 1e9, "nlevels": inf.nlevels,
+            "factor_gflop": inf["factor_flops"] / 1e9, "nlevels": inf["nlevels"],
             "analyze_s": t_an, "dtype": "f64"}
     log("batch: %.2f ms per batch of %d vs %.2f ms one by one (%.1fx), max resid %.2e" % (
         t_b, nsys, t_1, t_1 / t_b, max(res)))

@@ -270,6 +270,51 @@ int kkt_get_forward_forms(kkt_solver* h, int* launches);
 int kkt_set_forward_forms(kkt_solver* h, const int* launches);
 int kkt_timing_get(kkt_solver* h, double* ms, long long* launches, double* update_flops);
 
+/* ---- batched systems ----
+ * Many independent KKT systems of uneven size (the contact / rigid-body
+ * mixes of a scene) factored and solved together.  No counterpart in the
+ * reference, whose cuDSS bridge keeps one handle and stream per system
+ * (cudssBridge.cc:157-218, CuDssJNI.cc:11-23); each system's arrays have the
+ * DirectSolver array layout of kkt_set_pattern (KKTSolver.java:171-454).
+ * The systems become one block-diagonal matrix with one assembly tree per
+ * system, so each factor and each triangular solve is one captured launch
+ * sequence covering every system.  Pivot perturbation thresholds and inertia
+ * are per system, and refinement stops per system on that system's own
+ * residual, so each result is what the system gets alone (within the
+ * refinement tolerance; bitwise when the systems' refinement decisions match). */
+typedef struct kkt_batch kkt_batch;
+typedef struct kkt_batch_sysinfo {
+  int n;                    /* rows of the system */
+  int npos, nneg;           /* inertia of D */
+  int nperturbed;           /* perturbed pivots */
+  int n2x2;                 /* 2x2 Bunch-Kaufman pivots */
+  int refine_steps;         /* refinement steps of the last solve */
+  double last_rel_residual; /* ||b - A x|| / ||b|| of the last refined solve (0 without refinement) */
+} kkt_batch_sysinfo;
+kkt_batch* kkt_batch_create(void);
+void kkt_batch_destroy(kkt_batch* b);
+/* nsys systems; system q has sysN[q] rows.  rowOffs holds each system's own
+ * 0-based upper-CSR row offsets one system after another (sysN[q] + 1 entries
+ * each, starting at 0), colIdxs each system's local column indices one system
+ * after another.  Values, right-hand sides and solutions are passed the same
+ * way: concatenated in system order. */
+int kkt_batch_set_systems(kkt_batch* b, int nsys, const int* sysN, const int* rowOffs, const int* colIdxs,
+                          int mtype);
+/* Analysis of every system (vals: the concatenated values, so zero-diagonal
+ * constraint rows are ordered after their neighbours; NULL: pattern only). */
+int kkt_batch_analyze(kkt_batch* b, const double* vals);
+int kkt_batch_factor(kkt_batch* b, const double* vals);
+int kkt_batch_factor_device(kkt_batch* b, const double* d_vals);
+int kkt_batch_solve(kkt_batch* b, const double* B, double* X);
+int kkt_batch_solve_device(kkt_batch* b, const double* d_B, double* d_X);
+int kkt_batch_set_precision(kkt_batch* b, int prec);
+int kkt_batch_set_refine_tol(kkt_batch* b, double tol, int max_steps);
+int kkt_batch_set_ir_steps(kkt_batch* b, int nsteps);
+/* statistics of the whole batch (as kkt_get_info) and of one system */
+int kkt_batch_get_info(kkt_batch* b, kkt_info* out);
+int kkt_batch_system_info(kkt_batch* b, int sys, kkt_batch_sysinfo* out);
+const char* kkt_batch_last_error(kkt_batch* b);
+
 /* Host-only symbolic analysis (no GPU needed): structure statistics and the
  * same arrays kkt_get_symbolic returns, for pattern checks on the CPU. */
 typedef struct kkt_symbolic kkt_symbolic;

@@ -381,12 +381,21 @@ orc_factor* orc_factor_create(int n, const int* rowOffs, const int* colIdxs, con
   f->lperm = (int*)malloc(sizeof(int) * n);
   f->piv = (int*)malloc(sizeof(int) * n);
   if (!f->F) { free(iperm); free(c2s); orc_factor_free(f); return NULL; }
-  /* max |a_ij| for the perturbation threshold */
-  double amax = 0.0;
+  /* perturbation threshold pert * max|a_ij| per connected component: the
+   * assembly tree of the supernode (PARDISO's iparm[9] threshold, pardisoMkl.cc:
+   * 89-110, taken per independent block so that a block-diagonal batch factors
+   * each system as it would alone; one component = the whole-matrix maximum) */
+  int* sroot = (int*)malloc(sizeof(int) * (nsuper > 0 ? nsuper : 1));
+  for (int s = nsuper - 1; s >= 0; s--) sroot[s] = f->sparent[s] < 0 ? s : sroot[f->sparent[s]];
+  double* amax = (double*)calloc(nsuper > 0 ? nsuper : 1, sizeof(double));
   for (int i = 0; i < n; i++)
-    for (int p = rowOffs[i]; p < rowOffs[i + 1]; p++)
-      if (colIdxs[p] >= i && fabs(vals[p]) > amax) amax = fabs(vals[p]);
-  double tau = pert * amax;
+    for (int p = rowOffs[i]; p < rowOffs[i + 1]; p++) {
+      int j = colIdxs[p];
+      if (j < i) continue;
+      int c = iperm[i] < iperm[j] ? iperm[i] : iperm[j];
+      int r = sroot[c2s[c]];
+      if (fabs(vals[p]) > amax[r]) amax[r] = fabs(vals[p]);
+    }
   /* scatter A into fronts */
   for (int i = 0; i < n; i++)
     for (int p = rowOffs[i]; p < rowOffs[i + 1]; p++) {
@@ -407,7 +416,7 @@ orc_factor* orc_factor_create(int n, const int* rowOffs, const int* colIdxs, con
     int m = k + (int)(f->rows_ptr[s + 1] - f->rows_ptr[s]);
     double* Fs = f->F + f->foff[s];
     for (int t = 0; t < k; t++) f->lperm[fs + t] = t;
-    front_factor(Fs, m, k, f->nb, tau, f->lperm + fs, f->piv + fs, &f->npos, &f->nneg,
+    front_factor(Fs, m, k, f->nb, pert * amax[sroot[s]], f->lperm + fs, f->piv + fs, &f->npos, &f->nneg,
                  &f->npert, &f->n2x2, W);
     /* extend-add CB into parent */
     int p = f->sparent[s];
@@ -424,7 +433,7 @@ orc_factor* orc_factor_create(int n, const int* rowOffs, const int* colIdxs, con
           Fp[(long)rel[j] * mp + rel[i]] += Fs[(long)(k + j) * m + (k + i)];
     }
   }
-  free(W); free(rel); free(iperm); free(c2s);
+  free(W); free(rel); free(iperm); free(c2s); free(sroot); free(amax);
   return f;
 }
 

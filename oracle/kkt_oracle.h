@@ -41,7 +41,8 @@ long orc_supernode_rows(int n, const int* rowOffs, const int* colIdxs, const int
 
 /* Multifrontal LDL^T (FP64) on the given ordering and supernode partition, with
  * Bunch-Kaufman pivoting restricted to panels of `nb` columns inside each
- * supernode and pivots smaller than pert*max|a_ij| perturbed.  NULL on failure. */
+ * supernode and pivots smaller than pert*max|a_ij| perturbed, the maximum taken
+ * over the pivot's connected component (its assembly tree).  NULL on failure. */
 orc_factor* orc_factor_create(int n, const int* rowOffs, const int* colIdxs, const double* vals,
                               const int* perm, int nsuper, const int* sup_ptr, int nb,
                               double pert);

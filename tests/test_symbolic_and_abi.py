@@ -129,6 +129,31 @@ def test_abi_argument_and_state_errors_without_gpu():
     L.kkt_destroy(None)
 
 
+def test_batch_abi_argument_errors_without_gpu():
+    L = _lib.lib()
+    b = L.kkt_batch_create()
+    assert b
+    I = _lib.ctypes.c_int
+    try:
+        assert L.kkt_batch_analyze(b, None) == -8  # STATE: no systems
+        sn = np.array([2, 1], np.int32)
+        ro = np.array([0, 1, 2, 0, 1], np.int32)
+        ci = np.array([0, 1, 0], np.int32)
+        assert L.kkt_batch_set_systems(b, 0, _lib.ptr(sn, I), _lib.ptr(ro, I), _lib.ptr(ci, I), 1) == -14
+        bad = np.array([0, 1, 2, 0, 1], np.int32)
+        badc = np.array([0, 5, 0], np.int32)  # column outside system 0
+        assert L.kkt_batch_set_systems(b, 2, _lib.ptr(sn, I), _lib.ptr(bad, I), _lib.ptr(badc, I), 1) == -4
+        assert b"system 0" in L.kkt_batch_last_error(b)
+        assert L.kkt_batch_set_systems(b, 2, _lib.ptr(sn, I), _lib.ptr(ro, I), _lib.ptr(ci, I), 1) == 0
+        assert L.kkt_batch_last_error(b) is None
+        assert L.kkt_batch_set_precision(b, 9) == -14
+        si = _lib.KktBatchSysInfo()
+        assert L.kkt_batch_system_info(b, 5, _lib.ctypes.byref(si)) == -14
+    finally:
+        L.kkt_batch_destroy(b)
+    L.kkt_batch_destroy(None)
+
+
 def test_direct_solver_mirror_state_machine():
     s = MI355XSolver()
     from artisynth_core_amd.solvers import ImproperStateException
