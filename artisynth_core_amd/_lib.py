@@ -83,6 +83,7 @@ SIGNATURES = [
     ("kkt_set_forward_forms", _I, [_P, _IP]),
     ("kkt_solve", _I, [_P, _DP, _DP]),
     ("kkt_solve_device", _I, [_P, _P, _P]),
+    ("kkt_solve_device_noalloc", _I, [_P, _P, _P]),
     ("kkt_solve_multi", _I, [_P, _I, _DP, _DP]),
     ("kkt_solve_multi_device", _I, [_P, _I, _P, _P]),
     ("kkt_multiply", _I, [_P, _DP, _DP]),

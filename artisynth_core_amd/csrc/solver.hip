@@ -29,6 +29,38 @@ namespace {
 
 bool g_timing = false;
 
+// Scratch of kkt_solve_device_noalloc: one 64 MiB region per device, held by
+// every device that loads the library until the process ends.  Host code never
+// uses the symbol itself (on the host it is only a stand-in, not a device
+// address): the device address comes from hipGetSymbolAddress, once per device
+// with that device current, and is kept in g_scratch_ptr.  All calls on one
+// device share the region, so overlapping calls on one device race.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[MAX_DEVICES] = {};
+
+// The device address of `scratch` on device dev (looked up on first use).
+hipError_t scratch_on_device(int dev, unsigned char** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != dev && (e = hipSetDevice(dev)) != hipSuccess) return e;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[dev] = (unsigned char*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
 // Handles may be driven from several host threads (the serving loop does).
 // Allocation, free and synchronous copies issued by one thread while another
 // thread has a stream capture open are rejected by HIP and can invalidate the
@@ -297,11 +329,13 @@ struct kkt_solver {
     dev_ok = false;
   }
 
+  int dev_id = 0;  // the device current when the handle's stream was created
   int ensure_device() {
     if (dev_ok) return KKT_OK;
     int cnt = 0;
     if (hipGetDeviceCount(&cnt) != hipSuccess || cnt <= 0)
       return fail(KKT_ERR_DEVICE, "no HIP device available");
+    if (hipGetDevice(&dev_id) != hipSuccess) return fail(KKT_ERR_DEVICE, "hipGetDevice failed");
     DevOpLock lk(g_capture_mu);
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
       return fail(KKT_ERR_INIT, "hipStreamCreate failed");
@@ -1038,8 +1072,8 @@ struct kkt_solver {
 
   // ------------------------------------------------------------------
   template <typename T>
-  void run_tri_solve(const T* F, const double* d_b, double* d_x) {
-    T* sv = (T*)d_svec;
+  void run_tri_solve(const T* F, const double* d_b, double* d_x, void* svec = nullptr) {
+    T* sv = (T*)(svec ? svec : d_svec);
     T *bp = sv, *fp = sv + n, *y = sv + 2 * (size_t)n, *xp = sv + 3 * (size_t)n,
       *u = sv + 4 * (size_t)n, *wv = u + u_elems + 16;
     const T* E0 = F + front_elems;
@@ -1484,6 +1518,96 @@ struct kkt_solver {
       fprintf(stderr, "[kkt-timing] solve n=%d: %.3f ms (refine steps %d, rel resid %.3g)\n", n,
               now_ms() - t0, last_refine_steps, last_relres);
     return KKT_OK;
+  }
+
+  // ---- refined solve without allocation (kkt_solve_device_noalloc) ----
+  // The per-call working set -- the triangular solves' vectors (permuted
+  // right-hand side, forward and D^-1 vectors, update and backward vectors, in
+  // the factor precision), the refinement residual and correction, the dot
+  // products' partial sums and result -- lives in the caller-supplied scratch
+  // region `scr` of `cap` bytes instead of the handle's buffers.  Launches are
+  // eager (no graph capture or instantiation), no timing events are created,
+  // and a stalled FP32 refinement returns KKT_ERR_REFINE_STALL instead of
+  // refactoring in FP64 (which would allocate the FP64 fronts).
+  static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+  size_t noalloc_bytes() const {
+    const size_t esz = (fprec == KKT_PREC_FP64) ? sizeof(double) : sizeof(float);
+    const size_t sv = align256(((size_t)4 * n + (size_t)u_elems + 16 + (size_t)v_elems + 16) * esz);
+    return sv + 2 * align256((size_t)n * sizeof(double)) + align256(1024 * sizeof(double)) + 256;
+  }
+  int solve_device_noalloc(const double* d_b, double* d_x, unsigned char* scr, size_t cap) {
+    if (state != FACTORED) return fail(KKT_ERR_STATE, "solve called before factor");
+    const size_t need = noalloc_bytes();
+    if (need > cap) {
+      char msg[256];
+      snprintf(msg, sizeof msg, "this system's solve needs %zu bytes of scratch, more than the %zu bytes the "
+               "no-allocation solve holds (use kkt_solve_device)", need, cap);
+      return fail(KKT_ERR_ARGUMENT, msg);
+    }
+    const size_t esz = (fprec == KKT_PREC_FP64) ? sizeof(double) : sizeof(float);
+    unsigned char* p = scr;
+    void* svec = p;
+    p += align256(((size_t)4 * n + (size_t)u_elems + 16 + (size_t)v_elems + 16) * esz);
+    double* r = (double*)p;
+    p += align256((size_t)n * sizeof(double));
+    double* dx = (double*)p;
+    p += align256((size_t)n * sizeof(double));
+    double* part = (double*)p;
+    p += align256(1024 * sizeof(double));
+    double* scal = (double*)p;
+    auto tri = [&](const double* b, double* x) {
+      if (fprec == KKT_PREC_FP64) run_tri_solve<double>(d_F64, b, x, svec);
+      else run_tri_solve<float>(d_F32, b, x, svec);
+    };
+    auto dot = [&](const double* a, const double* b, double& out) -> int {
+      launch_dot(n, a, b, part, scal, st);
+      if (hipMemcpyAsync(&out, scal, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        return fail(KKT_ERR_REDUCTION, "reduction failed");
+      return KKT_OK;
+    };
+    const bool tm = tm_on;
+    tm_on = false;  // no timing events (creating them would allocate)
+    tri(d_b, d_x);
+    last_refine_steps = 0;
+    int rc = KKT_OK;
+    const bool to_tol = fprec == KKT_PREC_FP32_REFINE || fallback_active;
+    const int want = to_tol ? refine_max : ir_steps;
+    if (want > 0) {
+      double bn2;
+      if ((rc = dot(d_b, d_b, bn2))) { tm_on = tm; return rc; }
+      double bn = std::sqrt(bn2);
+      if (bn == 0.0) bn = 1.0;
+      double prev = 0.0;
+      int stall = 0;
+      for (int step = 0;; step++) {
+        launch_spmv(n, d_rowptr_f, d_col_f, d_valsf, d_x, d_b, r, 1, st);
+        double rn2;
+        if ((rc = dot(r, r, rn2))) break;
+        const double rel = std::sqrt(rn2) / bn;
+        last_relres = rel;
+        if (to_tol) {
+          if (rel <= refine_tol) break;
+          if (step > 0 && rel > 0.5 * prev) stall++; else stall = 0;
+          if (step >= refine_max || stall >= 2 || !std::isfinite(rel)) {
+            char msg[200];
+            snprintf(msg, sizeof msg, "refinement stalled at relative residual %.3g > %.3g (the no-allocation "
+                     "solve does not refactor in FP64)", rel, refine_tol);
+            rc = fail(KKT_ERR_REFINE_STALL, msg);
+            break;
+          }
+        } else if (step >= ir_steps) {
+          break;
+        }
+        prev = rel;
+        tri(r, dx);
+        launch_axpby(n, 1.0, dx, 1.0, d_x, st);
+        last_refine_steps++;
+      }
+    }
+    tm_on = tm;
+    if (hipStreamSynchronize(st) != hipSuccess) return fail(KKT_ERR_SOLVE, "solve kernels failed");
+    return rc;
   }
 
   int set_values_device(const double* dv) {
@@ -2346,6 +2470,18 @@ int kkt_solve_device(kkt_solver* h, const double* d_b, double* d_x) {
   CHECK_H(h);
   if (!d_b || !d_x || d_b == d_x) return h->fail(KKT_ERR_ARGUMENT, "null or aliased vectors");
   int rc = h->solve_device(d_b, d_x);
+  if (!rc) h->ok();
+  return rc;
+}
+
+int kkt_solve_device_noalloc(kkt_solver* h, const double* d_b, double* d_x) {
+  CHECK_H(h);
+  if (!d_b || !d_x || d_b == d_x) return h->fail(KKT_ERR_ARGUMENT, "null or aliased vectors");
+  if (h->state != kkt_solver::FACTORED) return h->fail(KKT_ERR_STATE, "solve called before factor");
+  unsigned char* scr = nullptr;
+  hipError_t e = scratch_on_device(h->dev_id, &scr);
+  if (e != hipSuccess) return h->fail(KKT_ERR_DEVICE, hip_msg("hipGetSymbolAddress (scratch) failed", e));
+  int rc = h->solve_device_noalloc(d_b, d_x, scr, NOALLOC_SCRATCH_BYTES);
   if (!rc) h->ok();
   return rc;
 }

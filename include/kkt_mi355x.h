@@ -187,6 +187,18 @@ int kkt_factor_device_values(kkt_solver* h);
 int kkt_solve(kkt_solver* h, const double* b, double* x);
 /* Same on device arrays. */
 int kkt_solve_device(kkt_solver* h, const double* d_b, double* d_x);
+/* No counterpart in the reference: kkt_solve_device for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Allocates nothing: the call's working set
+ * (triangular-solve vectors, refinement residual and correction, reduction
+ * partials) lives in a static 64 MiB device region per GPU, held from the
+ * library's load until the process ends; a system whose working set is larger
+ * (about 16 (4n + sum r + sum m) bytes with FP64 fronts) is rejected with
+ * KKT_ERR_ARGUMENT.  Launches are eager, and a stalled FP32 refinement returns
+ * KKT_ERR_REFINE_STALL rather than refactoring in FP64.  Every call on a GPU
+ * shares that GPU's region and leaves its contents behind: calls that overlap on
+ * one GPU (from several threads or handles) race. */
+int kkt_solve_device_noalloc(kkt_solver* h, const double* d_b, double* d_x);
 /* CuDssSolver.doSolveMulti (cudssBridge.cc:1454-1523; CuDssSolver.java:814-826):
  * column-major n x nrhs.  The right-hand sides are solved eight at a time in one
  * launch sequence (each front's factor data read once per eight), with the
