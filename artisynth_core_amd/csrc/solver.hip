@@ -1083,25 +1083,30 @@ struct kkt_solver {
     tend();
     const int nl = S.nlevels;
     // timed per launch: class TC_SLEVEL = forward launches, TC_SGEMV = backward
+    // class TC_SLEVEL events bracket each k_fwd_level launch on its own
     for (int L = 0; L < nl; L++) {
       if (!ff_l[L].cnt) continue;
-      tbeg(TC_SLEVEL);
       if (lev_timing) (void)hipEventRecord(lev_ev[4 * L], st);
+      tbeg(TC_SLEVEL);
       launch_fwd_level<T>(d_ff + ff_l[L].off, ff_l[L].cnt, ds, F, bp, fp, y, u, st);
-      if (fs_l[L].cnt) launch_fwd_level<T>(d_ff + fs_l[L].off, fs_l[L].cnt, ds, F, bp, fp, y, u, st);
-      if (lev_timing) (void)hipEventRecord(lev_ev[4 * L + 1], st);
       tend();
+      if (fs_l[L].cnt) {
+        tbeg(TC_SLEVEL);
+        launch_fwd_level<T>(d_ff + fs_l[L].off, fs_l[L].cnt, ds, F, bp, fp, y, u, st);
+        tend();
+      }
+      if (lev_timing) (void)hipEventRecord(lev_ev[4 * L + 1], st);
     }
-    tbeg(TC_SLEVEL);
+    tbeg(TC_VEC);
     launch_dsolve<T>(n, E0, E1, d_piv, ds.wpos, y, wv, st);
     tend();
     for (int L = nl - 1; L >= 0; L--) {
       if (!bc_l[L].cnt) continue;
-      tbeg(TC_SGEMV);
       if (lev_timing) (void)hipEventRecord(lev_ev[4 * L + 2], st);
+      tbeg(TC_SGEMV);
       launch_bwd_cols<T>(d_bc + bc_l[L].off, bc_l[L].cnt, ds, F, wv, xp, st);
-      if (lev_timing) (void)hipEventRecord(lev_ev[4 * L + 3], st);
       tend();
+      if (lev_timing) (void)hipEventRecord(lev_ev[4 * L + 3], st);
     }
     tbeg(TC_VEC);
     launch_scatter_perm<T, double>(n, d_perm, xp, d_x, st);

@@ -204,11 +204,14 @@ class HexBeam:
         vals[self._kpos] = sv
         vals[self._gpos] = 1.0
         vals[self._lampos] = 0.0
-        # f = fext - fint - D v, D = beta K ; b = M v + h f
+        # f = fext - fint - D v, D = beta K; b = M v + h f (constrainedBackwardEuler,
+        # MechSystemSolver.java:4851-4862) plus the right-hand-side Jacobian term
+        # (a2 df/dv + a3 df/dx) v with a2 = -h, a3 = 0, i.e. + h D v
+        # (KKTFactorAndSolve, MechSystemSolver.java:2686-2692, 2900-2950)
         vv = v.reshape(-1)
         Kv = self._kmul(kv, vv)
         f = fext.reshape(-1) - fint - beta * Kv
-        bm = self.mass * vv + h * f
+        bm = self.mass * vv + h * f + h * beta * Kv
         bg = np.zeros(self.numG)
         return vals, np.concatenate([bm, bg])
 
@@ -269,6 +272,38 @@ def large_beam_sequence(nsteps=20, seed=0x5EED, nx=100, ny=16, nz=16, length=1.0
         out_b.append(b)
         u_prev = u
     return beam, out_v, out_b
+
+
+def large_beam_dynamics(solve, nsteps=20, seed=0x5EED, nx=100, ny=16, nz=16, length=1.0, width=0.16,
+                        tip_scale=20.0):
+    """Config L as an integrated ConstrainedBackwardEuler run: the hex beam of
+    large_beam_sequence released from rest under gravity, its free end pushed by a
+    seeded random-walk load (steps of up to tip_scale N), h = 0.005.  Step k's KKT
+    system is assembled at the state (u_k, v_k) the previous steps produced and
+    handed to solve(beam, vals, b) -- the caller's FP64 direct solve -- whose first sizeM
+    entries are v_{k+1}; then u_{k+1} = u_k + h v_{k+1} (MechSystemSolver.java:
+    4804-4939: velocity solve, then the position update).  Returns (beam,
+    [vals_k], [b_k], tip deflection (m) after each step)."""
+    beam = HexBeam(nx, ny, nz, length, width, density=1000.0,
+                   material=("neohookean", 3.4e4, 1.7e6), h=0.005)
+    rng = np.random.default_rng(seed)
+    grav = beam.gravity_loads()
+    u = np.zeros((beam.nn, 3))
+    v = np.zeros((beam.nn, 3))
+    tipload = np.zeros(3)
+    out_v, out_b, tip = [], [], []
+    for k in range(nsteps):
+        tipload = tipload + tip_scale * rng.uniform(-1.0, 1.0, size=3) * np.array([0.2, 1.0, 1.0])
+        fext = grav.copy()
+        fext[beam.tip] += tipload[None, :] / len(beam.tip)
+        vals, b = beam.system(u, v, fext)
+        out_v.append(vals)
+        out_b.append(b)
+        x = np.asarray(solve(beam, vals, b))
+        v = x[:beam.sizeM].reshape(-1, 3).copy()
+        u = u + beam.h * v
+        tip.append(float(np.linalg.norm(u[beam.tip], axis=1).mean()))
+    return beam, out_v, out_b, tip
 
 
 # ---------------------------------------------------------------------------

@@ -35,14 +35,8 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def bwd_sweep_bytes(L, h, n, w):
-    """Algorithmic HBM bytes of one backward sweep (all k_bwd_cols launches of a
-    triangular solve): per supernode the lower copy of G once (k(k-1)/2 + k r
-    entries), its backward vector once (k + r entries: z of the pivots, the
-    pushed -x of the rows), x written for its k pivots and pushed to the r row
-    entries it fills in the descendants (counted where they are filled: r per
-    supernode), and the indices (k pivot permutation and k push ranges, r push
-    positions); w = bytes per entry of the front precision."""
+def _structure(L, h):
+    """Supernode sizes k, r and tree levels of the handle's analysis."""
     from artisynth_core_amd._lib import KktInfo
     from artisynth_core_amd._lib import ptr as nptr
     inf = KktInfo()
@@ -50,27 +44,73 @@ def bwd_sweep_bytes(L, h, n, w):
     ns = inf.nsuper
     sup = np.zeros(ns + 1, np.int32)
     rp = np.zeros(ns + 1, np.int64)
-    L.kkt_get_symbolic(h, None, nptr(sup, ctypes.c_int), None, nptr(rp, ctypes.c_longlong), None)
-    k = np.diff(sup).astype(np.float64)
-    r = np.diff(rp).astype(np.float64)
-    return float(np.sum((k * (k - 1) / 2 + k * r + 2 * k + 2 * r) * w + (r + 2 * k) * 4)), inf.nlevels
+    par = np.zeros(ns, np.int32)
+    L.kkt_get_symbolic(h, None, nptr(sup, ctypes.c_int), nptr(par, ctypes.c_int), nptr(rp, ctypes.c_longlong), None)
+    k = np.diff(sup).astype(np.int64)
+    r = np.diff(rp).astype(np.int64)
+    lev = np.zeros(ns, np.int64)
+    for s_ in range(ns):  # children precede parents; level = longest path from a leaf
+        if par[s_] >= 0:
+            lev[par[s_]] = max(lev[par[s_]], lev[s_] + 1)
+    assert lev.max() + 1 == inf.nlevels
+    return k, r, lev, inf
 
 
-def time_bwd_sweeps(L, h, b_ptr, x_ptr, nsolves=6):
-    """Per-launch HIP-event time of the backward-sweep kernel (timing class 5, eager
-    launches on the solver's stream) over nsolves refined solves."""
-    L.kkt_timing_reset(h, 1 << 5)
-    for _ in range(nsolves):
-        assert L.kkt_solve_device(h, b_ptr, x_ptr) == 0
-    ms = (ctypes.c_double * 8)()
-    cnt = (ctypes.c_longlong * 8)()
-    uf = ctypes.c_double()
-    L.kkt_timing_get(h, ms, cnt, ctypes.byref(uf))
-    L.kkt_timing_reset(h, 0)
-    return ms[5], int(cnt[5])
+def bwd_sweep_bytes(L, h, w):
+    """Algorithmic HBM bytes of one backward sweep (all k_bwd_cols launches of a
+    triangular solve): per supernode the lower copy of G once (k(k-1)/2 + k r
+    entries), its backward vector once (k + r entries), x written for its k pivots
+    and pushed to the r row entries it fills in the descendants, and the indices
+    (pivot permutation and push ranges, k each; push positions, r); w = bytes per
+    entry of the front precision."""
+    k, r, lev, inf = _structure(L, h)
+    k, r = k.astype(np.float64), r.astype(np.float64)
+    return float(np.sum((k * (k - 1) / 2 + k * r + 2 * k + 2 * r) * w + (r + 2 * k) * 4))
 
 
-def run_sequence(L, h, vals_d, b_d, x_d, steps, nsys, tol_exp=10, max_iter=30, record=None):
+def fwd_sweep_bytes(L, h, w, fwd_epl, forms):
+    """Algorithmic HBM bytes of one forward sweep (all k_fwd_level launches of a
+    triangular solve), per supernode (k pivots, r rows below): the upper copy of
+    G once (k(k-1)/2 entries of X, unit diagonal implicit, and k r of Z), the
+    right-hand side at the pivots (k) and the children's update entries (counted
+    once, at the parent: r per supernode), y written for the pivots (k) and the
+    update vector for the rows (r), and the gather map (m + 1 offsets and r
+    source indices, 4 bytes each); w = bytes per vector/factor entry.
+    Returned separately: the redundant bytes of the one-launch form, where every
+    row block of a large front gathers the front's right-hand side itself (m
+    entries and the map once more per extra row block)."""
+    k, r, lev, inf = _structure(L, h)
+    m = k + r
+    kf, rf = k.astype(np.float64), r.astype(np.float64)
+    alg = float(np.sum((kf * (kf - 1) / 2 + kf * rf + 2 * kf + 2 * rf) * w + (m + 1 + r) * 4.0))
+    red = 0.0
+    for s_ in range(len(k)):
+        if k[s_] <= 32 and m[s_] <= 256:  # small fronts: gather and rows in one workgroup
+            continue
+        L_ = lev[s_]
+        if forms[L_] != 1:
+            continue
+        lg = 0
+        while lg < 6 and fwd_epl[L_] * (1 << lg) < k[s_]:
+            lg += 1
+        rows_per_block = 4 * (64 >> lg)
+        nblk = (m[s_] + rows_per_block - 1) // rows_per_block
+        red += (nblk - 1) * (m[s_] * w + (m[s_] + 1 + r[s_]) * 4.0)
+    return alg, red
+
+
+def spmv_bytes(n, nnz_upper, mode_resid=True):
+    """SpMV of refinement / BiCGStab on the full symmetric CSR the library keeps:
+    values and column indices of every stored entry (nnz_full = 2 nnz_upper - n
+    with the diagonal once), row offsets, x read, y written, b read in the
+    residual form; and the minimum of upper storage for comparison."""
+    nnz_full = 2 * nnz_upper - n
+    full = nnz_full * 12 + (n + 1) * 4 + (3 if mode_resid else 2) * n * 8
+    upper = nnz_upper * 12 + (n + 1) * 4 + 2 * n * 8
+    return float(full), float(upper), nnz_full
+
+
+def run_sequence(L, h, vals_d, b_d, x_d, steps, nsys, tol_exp=10, max_iter=30):
     """Runs `steps` KKT solves over the systems 0..nsys-1 (cyclic) with the hybrid
     policy of KKTSolver.factorAndSolve; returns per-step latencies (ms) and kinds."""
     import torch
@@ -102,6 +142,46 @@ def run_sequence(L, h, vals_d, b_d, x_d, steps, nsys, tol_exp=10, max_iter=30, r
         lat.append((time.perf_counter() - t0) * 1e3)
     torch.cuda.synchronize()
     return lat, kinds, iters
+
+
+def _timing(L, h):
+    ms = (ctypes.c_double * 8)()
+    cnt = (ctypes.c_longlong * 8)()
+    uf = ctypes.c_double()
+    L.kkt_timing_get(h, ms, cnt, ctypes.byref(uf))
+    return [ms[c] for c in range(8)], [int(cnt[c]) for c in range(8)], uf.value
+
+
+def kernel_pass(L, h, vals_ptr, b_ptr, x_ptr, nsolves=6):
+    """After the timed region: per-launch HIP-event times (events on the solver's
+    stream, eager launches) of one factor's kernel classes and of the triangular
+    sweeps and SpMVs of nsolves refined solves, plus the factor's wall time as it
+    runs in the sequence (graph replay, no events)."""
+    import torch
+    from artisynth_core_amd._lib import KktInfo
+    L.kkt_timing_reset(h, 0x8F)  # factor classes + vector ops
+    assert L.kkt_factor_device(h, vals_ptr) == 0, L.kkt_last_error(h)
+    fms, fcnt, uflops = _timing(L, h)
+    L.kkt_timing_reset(h, (1 << 4) | (1 << 5) | (1 << 6))
+    for _ in range(nsolves):
+        assert L.kkt_solve_device(h, b_ptr, x_ptr) == 0, L.kkt_last_error(h)
+    sms, scnt, _ = _timing(L, h)
+    inf = KktInfo()
+    L.kkt_get_info(h, ctypes.byref(inf))
+    L.kkt_timing_reset(h, 0)
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        assert L.kkt_factor_device(h, vals_ptr) == 0
+    torch.cuda.synchronize()
+    factor_ms = (time.perf_counter() - t0) * 1e3 / reps
+    return {"factor_kernel_ms": {TIMING_CLASSES[c]: fms[c] for c in (0, 1, 2, 3, 7)},
+            "factor_kernel_launches": {TIMING_CLASSES[c]: fcnt[c] for c in (0, 1, 2, 3, 7)},
+            "update_flops": uflops, "factor_ms": factor_ms,
+            "fwd_ms": sms[4], "fwd_launches": scnt[4], "bwd_ms": sms[5], "bwd_launches": scnt[5],
+            "spmv_ms": sms[6], "spmv_launches": scnt[6],
+            "sweeps": nsolves * (1 + inf.refine_steps), "refine_steps": inf.refine_steps}
 
 
 def main():
@@ -154,10 +234,37 @@ def main():
                              args.warmup, args.oracle)
     if args.mode == "multi":
         return multi_run(nx, ny, nz, args.nrhs, max(args.steps // 4, 2), args.warmup)
+    L = lib()
+    # setup (untimed): integrate the ConstrainedBackwardEuler sequence with the
+    # FP64 GPU path, so each step's system is assembled at the state the previous
+    # steps produced
     t0 = time.time()
-    beam, V, B = G.large_beam_sequence(nsteps=args.nsys, nx=nx, ny=ny, nz=nz,
-                                       width=0.16 * ny / 16)
-    log("generated %d systems n=%d nnz=%d in %.1fs" % (len(V), beam.n, beam.nnz, time.time() - t0))
+    integ = {}
+
+    def fp64_solve(beam, vals, b):
+        if "h" not in integ:
+            hh = L.kkt_create()
+            L.kkt_set_precision(hh, KKT_PREC_FP64)
+            L.kkt_set_ir_steps(hh, 1)
+            ro_ = np.ascontiguousarray(beam.rowOffs, np.int32)
+            ci_ = np.ascontiguousarray(beam.colIdxs, np.int32)
+            assert L.kkt_set_pattern(hh, beam.n, beam.nnz, nptr(ro_, ctypes.c_int), nptr(ci_, ctypes.c_int), 1) == 0
+            v_ = np.ascontiguousarray(vals)
+            assert L.kkt_analyze_values(hh, nptr(v_, ctypes.c_double)) == 0, L.kkt_last_error(hh)
+            integ["h"] = hh
+        hh = integ["h"]
+        v_ = np.ascontiguousarray(vals)
+        b_ = np.ascontiguousarray(b)
+        x_ = np.zeros(beam.n)
+        assert L.kkt_factor(hh, nptr(v_, ctypes.c_double)) == 0, L.kkt_last_error(hh)
+        assert L.kkt_solve(hh, nptr(b_, ctypes.c_double), nptr(x_, ctypes.c_double)) == 0, L.kkt_last_error(hh)
+        return x_
+
+    beam, V, B, tipdefl = G.large_beam_dynamics(fp64_solve, nsteps=args.nsys, nx=nx, ny=ny, nz=nz,
+                                                width=0.16 * ny / 16)
+    L.kkt_destroy(integ.pop("h"))
+    log("integrated %d ConstrainedBackwardEuler steps n=%d nnz=%d in %.1fs (tip displacement %.1f mm)" % (
+        len(V), beam.n, beam.nnz, time.time() - t0, 1e3 * tipdefl[-1]))
     n, nnz = beam.n, beam.nnz
     ro = np.ascontiguousarray(beam.rowOffs, np.int32)
     ci = np.ascontiguousarray(beam.colIdxs, np.int32)
@@ -165,7 +272,6 @@ def main():
     b_d = [torch.from_numpy(np.ascontiguousarray(b)).to(dev) for b in B]
     x_d = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(args.steps)]
     torch.cuda.synchronize()
-    L = lib()
 
     results = {}
     for prec, name in ((KKT_PREC_FP32_REFINE, "fp32_mfma_refine"), (KKT_PREC_FP64, "fp64")):
@@ -198,77 +304,90 @@ def main():
         log("%s: autotuned solve shapes (elements per lane, levels 0..%d) fwd %s bwd %s, forward launches "
             "per level %s, in %.2f s" % (name, inf0.nlevels - 1, shp_f.tolist(), shp_b.tolist(), shp_n.tolist(),
                                          t_tune))
-        L.kkt_timing_reset(h, 0x0F)  # factor kernel classes; solves stay on their HIP graph
+        # the timed region: no kernel timing (factors and solves replay their graphs)
+        L.kkt_timing_reset(h, 0)
+        run_sequence(L, h, vals_d, b_d, x_d, 1, len(V))  # graphs captured for the re-planned solve
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         lat, kinds, iters = run_sequence(L, h, vals_d, b_d, x_d, args.steps, len(V))
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
-        ms = (ctypes.c_double * 8)()
-        cnt = (ctypes.c_longlong * 8)()
-        uflops = ctypes.c_double()
-        L.kkt_timing_get(h, ms, cnt, ctypes.byref(uflops))
         inf = KktInfo()
         L.kkt_get_info(h, ctypes.byref(inf))
-        # after the timed region: live per-launch timing of the backward sweep kernel
+        # after the timed region: per-launch kernel times on the solver's stream
         x_t = torch.zeros(n, dtype=torch.float64, device=dev)  # not one of the verified solutions
-        bwd_ms, bwd_launches = time_bwd_sweeps(L, h, b_d[0].data_ptr(), x_t.data_ptr())
-        sweep_bytes, nlev = bwd_sweep_bytes(L, h, n, 4 if prec == KKT_PREC_FP32_REFINE else 8)
+        kp = kernel_pass(L, h, vals_d[0].data_ptr(), b_d[0].data_ptr(), x_t.data_ptr())
+        w = 4 if prec == KKT_PREC_FP32_REFINE else 8
+        fwd_bytes, fwd_red = fwd_sweep_bytes(L, h, w, shp_f, shp_n)
+        bwd_bytes = bwd_sweep_bytes(L, h, w)
         # verify every step's solution (host, after timing)
+        import scipy.sparse as sp
         res = []
         for i in range(args.steps):
             k = i % len(V)
             x = x_d[i].cpu().numpy()
-            import scipy.sparse as sp
             U = sp.csr_matrix((V[k], ci, ro), shape=(n, n))
             A = U + sp.triu(U, 1).T
             res.append(float(np.linalg.norm(A @ x - B[k]) / np.linalg.norm(B[k])))
         results[name] = {
             "solves_per_s": args.steps / elapsed, "elapsed_s": elapsed, "lat_ms": lat,
             "kinds": kinds, "iters": iters, "rel_residual": res, "analyze_s": t_an,
-            "kernel_ms": {TIMING_CLASSES[c]: ms[c] for c in range(8)},
-            "kernel_launches": {TIMING_CLASSES[c]: cnt[c] for c in range(8)},
-            "update_flops": uflops.value, "info": inf.as_dict(),
-            "bwd_ms": bwd_ms, "bwd_launches": bwd_launches, "bwd_sweep_bytes": sweep_bytes,
-            "nlevels": nlev, "solve_shapes": {"fwd": shp_f.tolist(), "bwd": shp_b.tolist(), "fwd_launches": shp_n.tolist(),
-                                             "tune_s": t_tune},
+            "kp": kp, "info": inf.as_dict(), "fwd_sweep_bytes": fwd_bytes, "fwd_redundant_bytes": fwd_red,
+            "bwd_sweep_bytes": bwd_bytes, "nlevels": inf.nlevels,
+            "solve_shapes": {"fwd": shp_f.tolist(), "bwd": shp_b.tolist(), "fwd_launches": shp_n.tolist(),
+                             "tune_s": t_tune},
             "x": [x_d[i].cpu().numpy() for i in range(min(args.steps, len(V)))],
         }
-        log("%s: %.2f solves/s, p50 %.2f ms, p99 %.2f ms, kinds %s, max rel resid %.2e" % (
+        log("%s: %.2f solves/s, p50 %.2f ms, p99 %.2f ms, kinds %s, iters %s, max rel resid %.2e" % (
             name, args.steps / elapsed, np.percentile(lat, 50), np.percentile(lat, 99),
-            "".join("D" if k == "direct" else "i" for k in kinds), max(res)))
+            "".join("D" if k == "direct" else "i" for k in kinds), iters, max(res)))
         L.kkt_destroy(h)
 
     r32, r64 = results["fp32_mfma_refine"], results["fp64"]
-    # forward error of each FP32-path solution against the FP64 path (same step)
     fwd = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(r32["x"], r64["x"])]
-    # dominant kernel over the sequence: the backward triangular sweep of the FP32
-    # path (k_bwd_cols<float>, HBM bound by design); algorithmic bytes per launch =
-    # the sweep's bytes / its launches (one per tree level)
-    nb = max(r32["bwd_launches"], 1)  # one timed launch per tree level and sweep
-    sweeps = nb / r32["nlevels"]
-    bpl = r32["bwd_sweep_bytes"] / r32["nlevels"]
-    achieved = r32["bwd_sweep_bytes"] * sweeps / (r32["bwd_ms"] * 1e-3) / 1e9 if r32["bwd_ms"] > 0 else 0.0
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_bwd_cols_fp32_hexbeam.json")
-    if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("bytes_per_launch")
-    roofline = {"bound": "hbm", "kernel": "k_bwd_cols<float>", "achieved": achieved,
-                "peak": PEAKS["hbm_GBs"], "unit": "GB/s", "frac": achieved / PEAKS["hbm_GBs"],
-                "traffic": traffic, "avg_launch_ms": r32["bwd_ms"] / nb, "launches": nb,
-                "bytes_per_launch": bpl}
-    # the factor's MFMA Schur update, for reference
-    upd_ms = r32["kernel_ms"]["panel_update"]
-    upd_n = max(r32["kernel_launches"]["panel_update"], 1)
-    upd_tf = r32["update_flops"] / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
-    roofline_update = {"bound": "mfma", "kernel": "k_panel_update<float>", "achieved": upd_tf,
-                       "peak": PEAKS["fp32_TFLOPs"], "unit": "TFLOP/s",
-                       "frac": upd_tf / PEAKS["fp32_TFLOPs"], "avg_launch_ms": upd_ms / upd_n,
-                       "launches": upd_n, "flops_per_launch": r32["update_flops"] / upd_n}
+    kp = r32["kp"]
+    # dominant kernel of the FP32 sequence: the forward triangular sweep
+    # (k_fwd_level<float>, HBM/latency bound); algorithmic bytes per launch = the
+    # sweep's bytes / its launches, average launch time from the HIP events
+    def hbm_roof(kernel, sweep_bytes, ms, launches, sweeps, traffic_key):
+        if launches <= 0 or ms <= 0:
+            return None
+        bpl = sweep_bytes * sweeps / launches
+        avg = ms / launches
+        ach = bpl / (avg * 1e-3) / 1e9
+        return {"bound": "hbm", "kernel": kernel, "achieved": ach, "peak": PEAKS["hbm_GBs"], "unit": "GB/s",
+                "frac": ach / PEAKS["hbm_GBs"], "traffic": profile_traffic(traffic_key),
+                "avg_launch_ms": avg, "launches": launches, "bytes_per_launch": bpl}
+    roofline = hbm_roof("k_fwd_level<float>", r32["fwd_sweep_bytes"], kp["fwd_ms"], kp["fwd_launches"],
+                        kp["sweeps"], "k_fwd_level<float>")
+    roofline["redundant_gather_bytes_per_launch"] = r32["fwd_redundant_bytes"] * kp["sweeps"] / kp["fwd_launches"]
+    roof_bwd = hbm_roof("k_bwd_cols<float>", r32["bwd_sweep_bytes"], kp["bwd_ms"], kp["bwd_launches"],
+                        kp["sweeps"], "k_bwd_cols<float>")
+    sp_full, sp_upper, nnz_full = spmv_bytes(n, nnz)
+    roof_spmv = None
+    if kp["spmv_launches"] > 0:
+        avg = kp["spmv_ms"] / kp["spmv_launches"]
+        ach = sp_full / (avg * 1e-3) / 1e9
+        roof_spmv = {"bound": "hbm", "kernel": "k_spmv", "achieved": ach, "peak": PEAKS["hbm_GBs"], "unit": "GB/s",
+                     "frac": ach / PEAKS["hbm_GBs"], "avg_launch_ms": avg, "launches": kp["spmv_launches"],
+                     "bytes_per_launch": sp_full, "upper_storage_min_bytes": sp_upper,
+                     "frac_of_upper_min_rate": sp_upper / (avg * 1e-3) / 1e9 / PEAKS["hbm_GBs"]}
+    # the factor: whole-factor rate against the FP32 MFMA peak, and the Schur update
+    fl = r32["info"]["factor_flops"]
+    fac_tf = fl / (kp["factor_ms"] * 1e-3) / 1e12
+    roof_factor = {"bound": "mfma", "kernel": "whole FP32 factor (graph replay)", "achieved": fac_tf,
+                   "peak": PEAKS["fp32_TFLOPs"], "unit": "TFLOP/s", "frac": fac_tf / PEAKS["fp32_TFLOPs"],
+                   "factor_ms": kp["factor_ms"], "flops": fl}
+    upd_ms = kp["factor_kernel_ms"]["panel_update"]
+    upd_n = max(kp["factor_kernel_launches"]["panel_update"], 1)
+    upd_tf = kp["update_flops"] / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
+    roof_update = {"bound": "mfma", "kernel": "k_panel_update<float>", "achieved": upd_tf,
+                   "peak": PEAKS["fp32_TFLOPs"], "unit": "TFLOP/s", "frac": upd_tf / PEAKS["fp32_TFLOPs"],
+                   "avg_launch_ms": upd_ms / upd_n, "launches": upd_n, "flops_per_launch": kp["update_flops"] / upd_n}
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(beam, V[0], B[0], r64["x"][0], r32["x"][0])
+        cpu = cpu_baseline(beam, V, B, r64["x"], r32["x"])
 
     lat = np.array(r32["lat_ms"])
     line = {
@@ -276,10 +395,11 @@ def main():
         "value": r32["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * r32["elapsed_s"] / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64",
-        "data": "synthetic (seeded neo-Hookean hex-beam KKT sequence, generated on the host)",
-        "config": {"workload": "constrained hex beam %s, n=%d (87,567 DOF + 867 constraints), "
-                               "%d-step ConstrainedBackwardEuler sequence, hybrid stale-factor "
-                               "BiCGStab (<=30 it)" % (args.grid, n, len(V)),
+        "data": "synthetic (seeded neo-Hookean hex beam; the %d-step ConstrainedBackwardEuler run integrated "
+                "during setup with the FP64 GPU solve)" % len(V),
+        "config": {"workload": "constrained hex beam %s, n=%d (87,567 DOF + 867 constraints), %d integrated "
+                               "ConstrainedBackwardEuler steps (h=0.005, gravity + seeded tip loads), hybrid "
+                               "stale-factor BiCGStab (<=30 it)" % (args.grid, n, len(V)),
                    "n": n, "nnz_upper": nnz, "precision": "fp32 fronts on MFMA + fp64 refinement"},
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "max_rel_residual": max(r32["rel_residual"]),
@@ -289,14 +409,29 @@ def main():
         "fp64_max_rel_residual": max(r64["rel_residual"]),
         "max_fwd_err_vs_fp64": max(fwd),
         "step_kinds": "".join("D" if k == "direct" else "i" for k in r32["kinds"]),
-        "bicgstab_iters": r32["iters"],
-        "factor_gflop": r32["info"]["factor_flops"] / 1e9, "nnzL": r32["info"]["nnzL"],
+        "bicgstab_iters": r32["iters"], "fp64_bicgstab_iters": r64["iters"],
+        "tip_displacement_mm": [round(1e3 * t, 3) for t in tipdefl],
+        "factor_gflop": fl / 1e9, "nnzL": r32["info"]["nnzL"],
         "n2x2": r32["info"]["n2x2"], "pivoted_blocks": r32["info"]["pivoted_blocks"],
         "solve_shapes_fp32": r32["solve_shapes"],
-        "kernel_ms_fp32": r32["kernel_ms"], "kernel_ms_fp64": r64["kernel_ms"],
-        "roofline": roofline, "roofline_factor_update": roofline_update, "cpu_baseline": cpu,
+        "factor_ms_fp32": kp["factor_ms"], "factor_ms_fp64": r64["kp"]["factor_ms"],
+        "kernel_ms_fp32": r32["kp"]["factor_kernel_ms"], "kernel_ms_fp64": r64["kp"]["factor_kernel_ms"],
+        "roofline": roofline, "roofline_bwd": roof_bwd, "roofline_spmv": roof_spmv,
+        "roofline_factor": roof_factor, "roofline_factor_update": roof_update, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def profile_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter profile
+    (profiles/traffic_hexbeam_100x16x16.json: FETCH_SIZE + WRITE_SIZE per dispatch
+    from two rocprofv3 --pmc runs of `bench.py --mode pmc`, each scaled by the
+    factor the same counter needs on a 256 MiB device copy); None if absent."""
+    tf = os.path.join(ROOT, "profiles", "traffic_hexbeam_100x16x16.json")
+    if not os.path.exists(tf):
+        return None
+    d = json.load(open(tf)).get("kernels", {})
+    return d.get(kernel, {}).get("bytes_per_launch")
 
 
 def pmc_run(nx, ny, nz):
@@ -308,6 +443,7 @@ def pmc_run(nx, ny, nz):
     from artisynth_core_amd.solvers import Matrix, MI355XSolver
     beam, V, B = G.large_beam_sequence(nsteps=1, nx=nx, ny=ny, nz=nz, width=0.16 * ny / 16)
     s = MI355XSolver(precision=KKT_PREC_FP32_REFINE)
+    s._L.kkt_set_graphs(s._h, 0)  # kernels dispatched one by one for the counters
     s.analyzeArrays(V[0], beam.colIdxs, beam.rowOffs, beam.n, Matrix.SYMMETRIC)
     s.factor(V[0])
     x = np.zeros(beam.n)
@@ -674,7 +810,45 @@ def precision_run(dims, nsteps, warmup, with_oracle):
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline(beam, vals, b, x64, x32):
+def cpu_baseline(beam, V, B, x64, x32):
+    """CPU baseline on the GPU box's host: MKL PARDISO with the reference's
+    settings, dlopened from the host's MKL install and driven through the same
+    step sequence with KKTSolver.factorAndSolve's policy (oracle/pardiso_run.py,
+    a child process: PARDISO fails in a process that has loaded the GPU
+    libraries), on all the threads this process may use and on one thread.
+    When MKL is absent, the oracle (one thread, one direct solve of step 0) is
+    reported instead, and the sample says so."""
+    import subprocess
+    import tempfile
+    tmp = tempfile.mkdtemp(prefix="kkt_cpu_")
+    inp, outp = os.path.join(tmp, "in.npz"), os.path.join(tmp, "out.json")
+    try:
+        np.savez(inp, rowOffs=np.asarray(beam.rowOffs, np.int32), colIdxs=np.asarray(beam.colIdxs, np.int32),
+                 V=np.stack(V), B=np.stack(B), X64=np.stack(x64), X32=np.stack(x32))
+        r = subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "pardiso_run.py"), inp, outp],
+                           capture_output=True, text=True, timeout=600)
+        if r.returncode != 0 or not os.path.exists(outp):
+            return {"value": None, "unit": "solves/s", "cores": None, "kind": "reference",
+                    "sample": "PARDISO run failed (rc %d): %s" % (r.returncode, r.stderr[-500:])}
+        d = json.load(open(outp))
+    finally:
+        for f in (inp, outp):
+            if os.path.exists(f):
+                os.remove(f)
+        os.rmdir(tmp)
+    if not d.get("available"):
+        return cpu_baseline_oracle(beam, V[0], B[0], x64[0], x32[0], d.get("why", "MKL PARDISO unavailable"))
+    out = {"value": d.pop("solves_per_s"), "unit": "solves/s", "cores": d["threads"], "kind": "reference",
+           "sample": "MKL PARDISO (reference settings: mtype=-2, METIS, Bunch-Kaufman, perturbation 1e-8, "
+                     "scaling + matching, 1 refinement step) on the same %d-step sequence with the factorAndSolve "
+                     "policy (phase 22+33, then phase-33 CGS with the stale factor), %d repetitions on %d threads; "
+                     "10 direct factor+solve repetitions; one thread: 2 direct + 5 hybrid steps"
+                     % (d["steps"], d["reps"], d["threads"])}
+    out.update(d)
+    return out
+
+
+def cpu_baseline_oracle(beam, vals, b, x64, x32, why):
     """The CPU oracle (FP64 multifrontal LDL^T, one thread) on the first step of the
     same sequence: analysis excluded, factor + solve + one refinement step timed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -688,13 +862,13 @@ def cpu_baseline(beam, vals, b, x64, x32):
         steps, xo = f.refine(b, tol=1e-10, maxSteps=1)
         dt = time.perf_counter() - t0
         return {"value": 1.0 / dt, "unit": "solves/s", "cores": 1, "kind": "port",
-                "sample": "one direct solve (factor + solve + 1 refinement step) of step 0 "
-                          "of the same n=%d sequence, %.1f s" % (beam.n, dt),
+                "sample": "%s; the oracle instead: one direct solve (factor + solve + 1 refinement step) of "
+                          "step 0 of the same n=%d sequence, %.1f s" % (why, beam.n, dt),
                 "gpu_fp64_err_vs_oracle": float(np.linalg.norm(x64 - xo) / np.linalg.norm(xo)),
                 "gpu_fp32_err_vs_oracle": float(np.linalg.norm(x32 - xo) / np.linalg.norm(xo))}
     except Exception as e:  # report, never fall back to anything
         return {"value": None, "unit": "solves/s", "cores": 1, "kind": "port",
-                "sample": "failed: %s" % e}
+                "sample": "%s; oracle failed: %s" % (why, e)}
 
 
 if __name__ == "__main__":

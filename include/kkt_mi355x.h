@@ -253,8 +253,11 @@ int kkt_verify_device_structure(kkt_solver* h);
 /* Kernel timing (HIP events on the solver's stream) accumulated since the
  * last reset, per kernel class: 0 assemble, 1 panel-diag (+ inverse rows),
  * 2 panel-trsm (+ the solve-form post-passes), 3 panel-update (MFMA),
- * 4 forward solve, 5 backward solve, 6 spmv, 7 vector ops.  enable: 0 off, -1 all classes, else a bit mask (1 << class).
- * Solves replay a captured HIP graph unless a solve class is timed.
+ * 4 forward solve (one count per k_fwd_level launch), 5 backward solve (one per
+ * k_bwd_cols launch), 6 spmv, 7 vector ops (including the D^-1 pass).
+ * enable: 0 off, -1 all classes, else a bit mask (1 << class).  Factors and
+ * solves replay their captured HIP graphs unless one of their classes is timed
+ * (then they launch eagerly), so time outside a measured region.
  * ms[c], launches[c]; `update_flops` = algorithmic flops of class-3 launches. */
 void kkt_timing_reset(kkt_solver* h, int enable);
 /* Solve-shape autotuner (no counterpart in the reference; the analysis picks a

@@ -218,3 +218,29 @@ def test_mooney_rivlin_tangent_is_the_stress_derivative():
     lin = (mu * (np.einsum("ik,JL->iJkL", I3, I3) + np.einsum("iL,Jk->iJkL", I3, I3)) +
            lam * np.einsum("iJ,kL->iJkL", I3, I3))
     assert np.abs(P0).max() < 1e-9 and np.allclose(A0[0], lin, rtol=0, atol=1e-9 * kappa)
+
+
+def test_beam_dynamics_integrates_the_constrained_step():
+    # the integrated ConstrainedBackwardEuler sequence of the benchmark, on a small
+    # beam with the oracle as the FP64 solve: the attached end stays fixed (G v = 0),
+    # the beam falls under gravity, each step's right-hand side is M v + h (fext -
+    # fint) (b = M v + h f plus the h D v Jacobian term) at the state the previous
+    # steps produced
+    import oracle as Orc
+    states = []
+
+    def solve(beam, vals, b):
+        S = product_symbolic(beam.rowOffs, beam.colIdxs, vals)
+        x = Orc.Factor(beam.rowOffs, beam.colIdxs, vals, perm=S["perm"], sup_ptr=S["sup_ptr"]).solve(b)
+        states.append(x)
+        return x
+
+    beam, V, B, tip = G.large_beam_dynamics(solve, nsteps=4, nx=8, ny=2, nz=2, width=0.2, tip_scale=0.0)
+    assert len(V) == len(B) == len(tip) == 4
+    for x in states:
+        v = x[:beam.sizeM].reshape(-1, 3)
+        assert np.abs(v[beam.attached]).max() <= 1e-12
+    assert tip[0] < tip[1] < tip[2] < tip[3]
+    # step 0 from rest: b = h * fext (gravity only), the lambda rows zero
+    assert np.allclose(B[0][:beam.sizeM], beam.h * beam.gravity_loads().reshape(-1), rtol=0, atol=1e-15)
+    assert not B[0][beam.sizeM:].any()
