@@ -29,6 +29,8 @@ class KktInfo(ctypes.Structure):
         ("nperturbed", ctypes.c_int), ("n2x2", ctypes.c_int), ("precision", ctypes.c_int),
         ("refine_steps", ctypes.c_int), ("fp64_fallbacks", ctypes.c_int),
         ("last_rel_residual", ctypes.c_double), ("pivoted_blocks", ctypes.c_int),
+        ("device_bytes", ctypes.c_longlong), ("factored_bytes", ctypes.c_longlong),
+        ("fp64_fallback_bytes", ctypes.c_longlong),
     ]
 
     def as_dict(self):
@@ -44,6 +46,27 @@ class KktBatchSysInfo(ctypes.Structure):
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class KktServeConfig(ctypes.Structure):
+    _fields_ = [("budget_bytes", ctypes.c_longlong), ("nworkers", ctypes.c_int), ("precision", ctypes.c_int),
+                ("tol_exp", ctypes.c_int), ("max_iter", ctypes.c_int), ("refine_tol", ctypes.c_double)]
+
+
+class KktServeResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("kind", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("rel_residual", ctypes.c_double), ("submit_ms", ctypes.c_double), ("start_ms", ctypes.c_double),
+                ("done_ms", ctypes.c_double), ("message", ctypes.c_char * 160)]
+
+
+class KktServeStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_longlong), ("completed", ctypes.c_longlong), ("errors", ctypes.c_longlong),
+                ("direct", ctypes.c_longlong), ("iterative", ctypes.c_longlong),
+                ("admissions", ctypes.c_longlong), ("evictions", ctypes.c_longlong),
+                ("queue_depth", ctypes.c_longlong), ("queue_max", ctypes.c_longlong),
+                ("budget_bytes", ctypes.c_longlong), ("bytes_resident", ctypes.c_longlong),
+                ("bytes_peak", ctypes.c_longlong), ("instances_resident", ctypes.c_int),
+                ("service_ms", ctypes.c_double), ("elapsed_ms", ctypes.c_double)]
 
 
 # (name, restype, argtypes) for every symbol of include/kkt_mi355x.h
@@ -102,6 +125,7 @@ SIGNATURES = [
     ("kkt_verify_device_structure", _I, [_P]),
     ("kkt_timing_reset", None, [_P, _I]),
     ("kkt_timing_get", _I, [_P, _DP, _LLP, _DP]),
+    ("kkt_time_kernels", _I, [_P, _I, _I, _DP, _IP]),
     ("kkt_batch_create", _P, []),
     ("kkt_batch_destroy", None, [_P]),
     ("kkt_batch_set_systems", _I, [_P, _I, _IP, _IP, _IP, _I]),
@@ -116,6 +140,15 @@ SIGNATURES = [
     ("kkt_batch_get_info", _I, [_P, ctypes.POINTER(KktInfo)]),
     ("kkt_batch_system_info", _I, [_P, _I, ctypes.POINTER(KktBatchSysInfo)]),
     ("kkt_batch_last_error", ctypes.c_char_p, [_P]),
+    ("kkt_serve_default_config", None, [ctypes.POINTER(KktServeConfig)]),
+    ("kkt_serve_create", _P, [ctypes.POINTER(KktServeConfig)]),
+    ("kkt_serve_destroy", None, [_P]),
+    ("kkt_serve_register", _I, [_P, _I, _I, _I, _IP, _IP, _DP, _I]),
+    ("kkt_serve_submit", ctypes.c_longlong, [_P, _I, _DP, _DP, _DP]),
+    ("kkt_serve_submit_device", ctypes.c_longlong, [_P, _I, _P, _P, _P]),
+    ("kkt_serve_wait", _I, [_P, ctypes.c_longlong, ctypes.POINTER(KktServeResult)]),
+    ("kkt_serve_get_stats", _I, [_P, ctypes.POINTER(KktServeStats)]),
+    ("kkt_serve_last_error", ctypes.c_char_p, [_P]),
     ("kkt_symbolic_create", _P, [_I, _I, _IP, _IP, _UCP, _I]),
     ("kkt_symbolic_info", _I, [_P, ctypes.POINTER(KktInfo)]),
     ("kkt_symbolic_get", _I, [_P, _IP, _IP, _IP, _IP, _IP, _LLP, _IP, _IP]),

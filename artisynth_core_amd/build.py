@@ -13,8 +13,10 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkkt_mi355x.so")
+# the HBM streaming probe of bench.py --mode stream (measurement only)
+PROBE_LIB = os.path.join(HERE, "libkkt_probe.so")
 
-SOURCES = ["kernels.hip", "solver.hip", "symbolic.cpp"]
+SOURCES = ["kernels.hip", "solver.hip", "symbolic.cpp", "serve.cpp"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -50,6 +52,10 @@ def build(verbose=False, jobs=3):
     if verbose:
         print(" ".join(cmd), flush=True)
     _wait(("link", subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+    cmd = [hipcc] + FLAGS + ["-x", "hip", "-shared", os.path.join(CSRC, "probe.hip"), "-o", PROBE_LIB]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    _wait(("probe.hip", subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
     return LIB
 
 

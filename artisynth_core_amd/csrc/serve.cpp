@@ -1,0 +1,420 @@
+// Serving loop: step-solve requests from many concurrent simulation instances
+// (the kkt_serve_* entry points of include/kkt_mi355x.h), built on the
+// solver's own C ABI.
+//
+// - An instance is one simulation's KKT pattern.  It gets its own solver handle
+//   (own HIP stream) on its first request; the analysis and the factor stay
+//   resident in HBM between its requests.
+// - Admission by an HBM budget: after the analysis (which holds only the
+//   structure) the handle reports what it will hold once factored; that much is
+//   reserved before the factor allocates its fronts, evicting idle resident
+//   instances least recently used first.  An instance larger than the whole
+//   budget fails its requests; one that fits only after busy instances finish
+//   waits for them.  An FP64 refactor after stalled FP32 refinement is charged
+//   when it happens.
+// - Continuous batching: requests enter one FIFO queue; `nworkers` threads take
+//   the oldest request whose instance is idle, so arrivals start as soon as a
+//   worker and memory are free, and different instances' device work overlaps
+//   on their streams.  An instance's requests run in submission order, one at a
+//   time, on its own handle, so its results do not depend on the mix (an
+//   evicted instance refactors on its next request).
+// - A request is one KKT step with KKTSolver.factorAndSolve's policy
+//   (KKTSolver.java:930-1014): BiCGStab with the resident (stale) factor while it
+//   converges within max_iter, else a factor and a refined direct solve.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "kkt_mi355x.h"
+
+namespace {
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Instance {
+  int iid = 0;
+  int n = 0, nnz = 0, mtype = KKT_MT_SYMMETRIC;
+  std::vector<int> ro, ci;
+  std::vector<double> vals0;
+  kkt_solver* h = nullptr;
+  long long charged = 0;   // bytes reserved against the budget
+  bool busy = false;
+  bool factored = false;
+  bool too_large = false;
+  double last_used = 0;
+  long long requests = 0;
+};
+
+struct Request {
+  long long id = 0;
+  int iid = 0;
+  bool device = false;
+  std::vector<double> vals, b;  // host submissions: copies
+  const double* d_vals = nullptr;
+  const double* d_b = nullptr;
+  double* x = nullptr;          // caller's output (host or device)
+  kkt_serve_result res{};
+  bool done = false;
+};
+
+}  // namespace
+
+struct kkt_server {
+  kkt_serve_config cfg{};
+  std::mutex mu;
+  std::condition_variable cv;       // work available / instance became idle
+  std::condition_variable done_cv;  // a request completed
+  std::map<int, std::unique_ptr<Instance>> inst;
+  std::deque<Request*> queue;
+  std::map<long long, std::unique_ptr<Request>> reqs;
+  std::vector<std::thread> workers;
+  long long next_id = 1;
+  long long used = 0, peak = 0;
+  bool stop = false;
+  double t0 = now_ms();
+  kkt_serve_stats st{};
+  double service_ms = 0;
+  std::string err;
+  bool has_err = false;
+
+  int fail(int code, const std::string& m) {
+    std::lock_guard<std::mutex> lk(mu);
+    err = m;
+    has_err = true;
+    return code;
+  }
+
+  // oldest queued request whose instance is idle (keeps per-instance order)
+  Request* next_locked() {
+    std::vector<int> blocked;
+    for (auto it = queue.begin(); it != queue.end(); ++it) {
+      Request* r = *it;
+      if (std::find(blocked.begin(), blocked.end(), r->iid) != blocked.end()) continue;
+      Instance* in = inst[r->iid].get();
+      if (in->busy) {
+        blocked.push_back(r->iid);
+        continue;
+      }
+      queue.erase(it);
+      return r;
+    }
+    return nullptr;
+  }
+
+  bool evict_one_locked(const Instance* self) {
+    Instance* victim = nullptr;
+    for (auto& kv : inst) {
+      Instance* in = kv.second.get();
+      if (in == self || !in->h || in->busy) continue;
+      if (!victim || in->last_used < victim->last_used) victim = in;
+    }
+    if (!victim) return false;
+    kkt_destroy(victim->h);
+    victim->h = nullptr;
+    victim->factored = false;
+    used -= victim->charged;
+    victim->charged = 0;
+    st.evictions++;
+    return true;
+  }
+
+  // reserve `need` more bytes for instance `in` (evicting idle instances, or
+  // waiting for busy ones); false if it can never fit
+  bool reserve(Instance* in, long long need, std::unique_lock<std::mutex>& lk) {
+    if (in->charged + need > cfg.budget_bytes) return false;
+    while (used + need > cfg.budget_bytes) {
+      if (!evict_one_locked(in)) cv.wait_for(lk, std::chrono::milliseconds(20));
+    }
+    used += need;
+    in->charged += need;
+    peak = std::max(peak, used);
+    return true;
+  }
+
+  // analysis of a non-resident instance (structure only, no fronts yet)
+  int admit(Instance* in, std::unique_lock<std::mutex>& lk, std::string& why) {
+    lk.unlock();
+    kkt_solver* h = kkt_create();
+    int rc = h ? KKT_OK : KKT_ERR_INIT;
+    if (!rc) rc = kkt_set_precision(h, cfg.precision);
+    if (!rc) rc = kkt_set_refine_tol(h, cfg.refine_tol, 10);
+    if (!rc) rc = kkt_set_pattern(h, in->n, in->nnz, in->ro.data(), in->ci.data(), in->mtype);
+    if (!rc) rc = kkt_analyze_values(h, in->vals0.data());
+    kkt_info info{};
+    if (!rc) rc = kkt_get_info(h, &info);
+    if (rc) {
+      const char* m = h ? kkt_last_error(h) : nullptr;
+      why = std::string("admission of instance ") + std::to_string(in->iid) + " failed: " + (m ? m : "");
+      if (h) kkt_destroy(h);
+      lk.lock();
+      return rc;
+    }
+    lk.lock();
+    const long long need = info.factored_bytes;
+    if (!reserve(in, need, lk)) {
+      in->too_large = true;
+      why = "instance " + std::to_string(in->iid) + " needs " + std::to_string(need) +
+            " bytes of device memory, more than the whole budget of " + std::to_string(cfg.budget_bytes);
+      lk.unlock();
+      kkt_destroy(h);
+      lk.lock();
+      return KKT_ERR_ALLOC;
+    }
+    in->h = h;
+    in->factored = false;
+    st.admissions++;
+    return KKT_OK;
+  }
+
+  // one request on its (resident, busy) instance, lock released
+  void run(Request* r, Instance* in) {
+    kkt_solver* h = in->h;
+    const double tol = std::pow(10.0, -cfg.tol_exp);
+    int rc = -1;
+    const double* vals = r->device ? r->d_vals : r->vals.data();
+    const double* b = r->device ? r->d_b : r->b.data();
+    if (in->factored) {
+      rc = r->device ? kkt_iterative_solve_device(h, vals, b, r->x, tol, cfg.max_iter)
+                     : kkt_iterative_solve(h, vals, b, r->x, tol, cfg.max_iter);
+    }
+    if (rc > 0) {
+      r->res.kind = 1;
+      r->res.iterations = rc;
+      r->res.status = KKT_OK;
+    } else {
+      int st_ = r->device ? kkt_factor_device(h, vals) : kkt_factor(h, vals);
+      if (!st_) st_ = r->device ? kkt_solve_device(h, b, r->x) : kkt_solve(h, b, r->x);
+      r->res.kind = 2;
+      r->res.iterations = 0;
+      r->res.status = st_;
+      in->factored = st_ == KKT_OK;
+    }
+    kkt_info info{};
+    if (kkt_get_info(h, &info) == KKT_OK) r->res.rel_residual = info.last_rel_residual;
+    if (r->res.status) {
+      const char* m = kkt_last_error(h);
+      snprintf(r->res.message, sizeof r->res.message, "%s", m ? m : "solve failed");
+    }
+  }
+
+  void worker() {
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      Request* r = nullptr;
+      cv.wait(lk, [&] { return stop || (r = next_locked()) != nullptr; });
+      if (!r) return;  // stopping with nothing runnable
+      Instance* in = inst[r->iid].get();
+      in->busy = true;
+      r->res.start_ms = now_ms() - t0;
+      int rc = KKT_OK;
+      std::string why;
+      if (in->too_large) {
+        rc = KKT_ERR_ALLOC;
+        why = "instance " + std::to_string(in->iid) + " does not fit in the budget";
+      } else if (!in->h) {
+        rc = admit(in, lk, why);
+      }
+      if (!rc) {
+        lk.unlock();
+        const double ts = now_ms();
+        run(r, in);
+        const double te = now_ms();
+        kkt_info info{};
+        const bool fell_back = kkt_get_info(in->h, &info) == KKT_OK && info.fp64_fallbacks > 0;
+        lk.lock();
+        service_ms += te - ts;
+        // an FP64 refactor after a stall now holds FP64 fronts as well: charged
+        // once it happened (the memory is already held), idle instances are
+        // evicted until the budget holds again
+        if (fell_back) {
+          const long long extra = info.device_bytes - in->charged;
+          if (extra > 0) {
+            used += extra;
+            in->charged += extra;
+            peak = std::max(peak, used);
+            while (used > cfg.budget_bytes && evict_one_locked(in)) {
+            }
+          }
+        }
+      } else {
+        r->res.status = rc;
+        snprintf(r->res.message, sizeof r->res.message, "%s", why.c_str());
+      }
+      r->res.done_ms = now_ms() - t0;
+      in->busy = false;
+      in->last_used = now_ms();
+      in->requests++;
+      st.completed++;
+      if (r->res.status) st.errors++;
+      else if (r->res.kind == 1) st.iterative++;
+      else st.direct++;
+      r->done = true;
+      done_cv.notify_all();
+      cv.notify_all();
+    }
+  }
+};
+
+extern "C" {
+
+void kkt_serve_default_config(kkt_serve_config* c) {
+  if (!c) return;
+  c->budget_bytes = 200LL << 30;
+  c->nworkers = 8;
+  c->precision = KKT_PREC_FP32_REFINE;
+  c->tol_exp = 10;
+  c->max_iter = 30;
+  c->refine_tol = 1e-10;
+}
+
+kkt_server* kkt_serve_create(const kkt_serve_config* cfg) {
+  if (!cfg || cfg->budget_bytes <= 0 || cfg->nworkers < 1 || cfg->nworkers > 64 || cfg->max_iter < 1 ||
+      cfg->tol_exp < 1 || !(cfg->refine_tol > 0) ||
+      (cfg->precision != KKT_PREC_FP64 && cfg->precision != KKT_PREC_FP32_REFINE))
+    return nullptr;
+  kkt_server* s = nullptr;
+  try {
+    s = new kkt_server();
+    s->cfg = *cfg;
+    s->st.budget_bytes = cfg->budget_bytes;
+    for (int w = 0; w < cfg->nworkers; w++) s->workers.emplace_back([s] { s->worker(); });
+  } catch (...) {
+    delete s;
+    return nullptr;
+  }
+  return s;
+}
+
+void kkt_serve_destroy(kkt_server* s) {
+  if (!s) return;
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    // finish what was submitted, then stop the workers
+    s->done_cv.wait(lk, [&] { return s->queue.empty(); });
+    s->stop = true;
+    s->cv.notify_all();
+  }
+  for (auto& t : s->workers) t.join();
+  for (auto& kv : s->inst)
+    if (kv.second->h) kkt_destroy(kv.second->h);
+  delete s;
+}
+
+int kkt_serve_register(kkt_server* s, int iid, int n, int nnz, const int* rowOffs, const int* colIdxs,
+                       const double* vals0, int mtype) {
+  if (!s) return KKT_ERR_ARGUMENT;
+  if (n <= 0 || nnz <= 0 || !rowOffs || !colIdxs || !vals0)
+    return s->fail(KKT_ERR_ARGUMENT, "bad instance arrays");
+  if (mtype != KKT_MT_SYMMETRIC && mtype != KKT_MT_SPD) return s->fail(KKT_ERR_UNKNOWN_MTYPE, "symmetric types only");
+  if (rowOffs[0] != 0 || rowOffs[n] != nnz) return s->fail(KKT_ERR_PATTERN, "rowOffs must run from 0 to nnz");
+  auto in = std::make_unique<Instance>();
+  in->iid = iid;
+  in->n = n;
+  in->nnz = nnz;
+  in->mtype = mtype;
+  in->ro.assign(rowOffs, rowOffs + n + 1);
+  in->ci.assign(colIdxs, colIdxs + nnz);
+  in->vals0.assign(vals0, vals0 + nnz);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->inst.count(iid)) {
+    s->err = "instance " + std::to_string(iid) + " is already registered";
+    s->has_err = true;
+    return KKT_ERR_ARGUMENT;
+  }
+  s->inst[iid] = std::move(in);
+  s->has_err = false;
+  return KKT_OK;
+}
+
+static long long submit(kkt_server* s, int iid, bool device, const double* vals, const double* b, double* x) {
+  if (!s) return KKT_ERR_ARGUMENT;
+  if (!vals || !b || !x) return s->fail(KKT_ERR_ARGUMENT, "null array");
+  auto r = std::make_unique<Request>();
+  r->iid = iid;
+  r->device = device;
+  r->x = x;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->inst.find(iid);
+  if (it == s->inst.end()) {
+    s->err = "instance " + std::to_string(iid) + " is not registered";
+    s->has_err = true;
+    return KKT_ERR_ARGUMENT;
+  }
+  const Instance* in = it->second.get();
+  if (device) {
+    r->d_vals = vals;
+    r->d_b = b;
+  } else {
+    r->vals.assign(vals, vals + in->nnz);
+    r->b.assign(b, b + in->n);
+  }
+  r->id = s->next_id++;
+  r->res.submit_ms = now_ms() - s->t0;
+  Request* rp = r.get();
+  s->reqs[r->id] = std::move(r);
+  s->queue.push_back(rp);
+  s->st.submitted++;
+  s->st.queue_max = std::max<long long>(s->st.queue_max, (long long)s->queue.size());
+  s->cv.notify_one();
+  s->has_err = false;
+  return rp->id;
+}
+
+long long kkt_serve_submit(kkt_server* s, int iid, const double* vals, const double* b, double* x) {
+  return submit(s, iid, false, vals, b, x);
+}
+
+long long kkt_serve_submit_device(kkt_server* s, int iid, const double* d_vals, const double* d_b, double* d_x) {
+  return submit(s, iid, true, d_vals, d_b, d_x);
+}
+
+int kkt_serve_wait(kkt_server* s, long long req, kkt_serve_result* out) {
+  if (!s) return KKT_ERR_ARGUMENT;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(req);
+  if (it == s->reqs.end()) {
+    s->err = "unknown request " + std::to_string(req);
+    s->has_err = true;
+    return KKT_ERR_ARGUMENT;
+  }
+  Request* r = it->second.get();
+  s->done_cv.wait(lk, [&] { return r->done; });
+  if (out) *out = r->res;
+  const int rc = r->res.status;
+  s->reqs.erase(it);
+  return rc;
+}
+
+int kkt_serve_get_stats(kkt_server* s, kkt_serve_stats* out) {
+  if (!s || !out) return KKT_ERR_ARGUMENT;
+  std::lock_guard<std::mutex> lk(s->mu);
+  *out = s->st;
+  out->bytes_resident = s->used;
+  out->bytes_peak = s->peak;
+  out->queue_depth = (long long)s->queue.size();
+  int res = 0;
+  for (auto& kv : s->inst) res += kv.second->h != nullptr;
+  out->instances_resident = res;
+  out->service_ms = s->service_ms;
+  out->elapsed_ms = now_ms() - s->t0;
+  return KKT_OK;
+}
+
+const char* kkt_serve_last_error(kkt_server* s) {
+  if (!s || !s->has_err) return nullptr;
+  return s->err.c_str();
+}
+
+}  // extern "C"

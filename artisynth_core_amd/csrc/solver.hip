@@ -17,6 +17,7 @@
 #include <mutex>
 #include <shared_mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "kernels.h"
@@ -261,6 +262,34 @@ struct kkt_solver {
     for (void* p : v)
       if (p) (void)hipFree(p);
     v.clear();
+    if (&v == &allocs) alloc_sizes.clear();
+  }
+  // ---- device memory accounting (bytes) ----
+  std::vector<size_t> alloc_sizes;  // parallel to allocs
+  size_t plan_bytes() const {
+    size_t t = 0;
+    for (size_t b : alloc_sizes) t += b;
+    return t;
+  }
+  size_t pattern_bytes() const {
+    return (size_t)(n + 1) * 4 + (size_t)std::max<long long>(nnzf, 1) * (4 + 4 + 8) + (size_t)std::max(nnz, 1) * 8 +
+           (size_t)14 * n * 8 + (size_t)2 * std::max(spmv_blocks(n), 1) * 8;
+  }
+  // fronts, D^-1, panel workspace and solve vectors of a factor in precision p
+  size_t numeric_bytes(int p) const {
+    const size_t esz = (p == KKT_PREC_FP64) ? 8 : 4;
+    return ((size_t)front_elems + 2 * (size_t)n + 4) * esz + (size_t)std::max<long long>(w_elems, 1) * esz +
+           ((size_t)4 * n + (size_t)u_elems + 16 + (size_t)v_elems + 16) * 8;
+  }
+  size_t held_bytes() const {
+    if (!dev_ok || state < ANALYZED) return 0;
+    size_t t = plan_bytes() + pattern_bytes();
+    const size_t f64 = ((size_t)front_elems + 2 * (size_t)n + 4 + (size_t)std::max<long long>(w_elems, 1)) * 8;
+    const size_t f32 = ((size_t)front_elems + 2 * (size_t)n + 4 + (size_t)std::max<long long>(w_elems, 1)) * 4;
+    if (d_F64) t += f64;
+    if (d_F32) t += f32;
+    t += svec_bytes + msvec_bytes + mvec_bytes;
+    return t;
   }
   void release_numeric() {
     drop_graphs();
@@ -355,6 +384,7 @@ struct kkt_solver {
     hipError_t e = hipMalloc(&dptr, bytes);
     if (e != hipSuccess) return fail(KKT_ERR_ALLOC, hip_msg("hipMalloc failed (plan)", e));
     allocs.push_back(dptr);
+    alloc_sizes.push_back(bytes);
     if (!h.empty() &&
         (e = hipMemcpy(dptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice)) != hipSuccess)
       return fail(KKT_ERR_COPY, hip_msg("hipMemcpy failed (plan)", e));
@@ -496,6 +526,7 @@ struct kkt_solver {
       DevOpLock lk(g_capture_mu);
       auto it = std::find(allocs.begin(), allocs.end(), (void*)dptr);
       if (it != allocs.end()) {
+        alloc_sizes.erase(alloc_sizes.begin() + (it - allocs.begin()));
         allocs.erase(it);
         (void)hipFree(dptr);
       }
@@ -1615,6 +1646,79 @@ struct kkt_solver {
     return rc;
   }
 
+  // ---- per-launch kernel times without per-launch events ----
+  // One forward sweep (what 0), one backward sweep (1) or one residual SpMV (2)
+  // of the current factor, captured into a graph and replayed `reps` times
+  // between two HIP events on the handle's stream: graph-replayed kernels run
+  // back to back, so elapsed / (reps x launches) is the kernels' average
+  // duration, as a kernel trace reports it.  Replays are idempotent (each
+  // recomputes the same vectors from the same inputs).
+  int time_kernels(int what, int reps, double* ms_per_launch, int* launches) {
+    if (state != FACTORED) return fail(KKT_ERR_STATE, "factor() not previously called");
+    if (what < 0 || what > 2 || reps < 1) return fail(KKT_ERR_ARGUMENT, "bad kernel group or repetitions");
+    const bool f64 = fprec == KKT_PREC_FP64;
+    void* sv = d_svec;
+    const int nl = S.nlevels;
+    int cnt = 0;
+    for (int L = 0; L < nl; L++) {
+      if (what == 0) cnt += (ff_l[L].cnt > 0) + (fs_l[L].cnt > 0);
+      if (what == 1) cnt += bc_l[L].cnt > 0;
+    }
+    if (what == 2) cnt = 1;
+    double* b = vec(12);
+    double* x = vec(13);
+    // inputs of the sweeps: one eager triangular solve fills every vector
+    launch_fill(n, 1.0, b, st);
+    tri_solve_eager(b, x);
+    auto body = [&]() {
+      if (what == 2) {
+        launch_spmv(n, d_rowptr_f, d_col_f, d_valsf, x, b, vec(0), 1, st);
+        return;
+      }
+      auto sweep = [&](auto* F) {
+        typedef typename std::remove_pointer<decltype(F)>::type T;
+        T* svT = (T*)sv;
+        T *bp = svT, *fp = svT + n, *y = svT + 2 * (size_t)n, *xp = svT + 3 * (size_t)n,
+          *u = svT + 4 * (size_t)n, *wv = u + u_elems + 16;
+        for (int L = 0; L < nl; L++) {
+          if (what == 0) {
+            if (ff_l[L].cnt) launch_fwd_level<T>(d_ff + ff_l[L].off, ff_l[L].cnt, ds, F, bp, fp, y, u, st);
+            if (fs_l[L].cnt) launch_fwd_level<T>(d_ff + fs_l[L].off, fs_l[L].cnt, ds, F, bp, fp, y, u, st);
+          } else {
+            const int Lb = nl - 1 - L;
+            if (bc_l[Lb].cnt) launch_bwd_cols<T>(d_bc + bc_l[Lb].off, bc_l[Lb].cnt, ds, F, wv, xp, st);
+          }
+        }
+      };
+      if (f64) sweep(d_F64);
+      else sweep(d_F32);
+    };
+    hipGraphExec_t ex = nullptr;
+    if (!capture(ex, body)) return fail(KKT_ERR_DEVICE, "graph capture failed (kernel timing)");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    {
+      DevOpLock lk(g_capture_mu);
+      if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        (void)hipGraphExecDestroy(ex);
+        return fail(KKT_ERR_DEVICE, "hipEventCreate failed");
+      }
+    }
+    (void)hipGraphLaunch(ex, st);  // warm
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < reps; r++) (void)hipGraphLaunch(ex, st);
+    (void)hipEventRecord(e1, st);
+    float ms = 0;
+    hipError_t err = hipStreamSynchronize(st);
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipGraphExecDestroy(ex);
+    if (err != hipSuccess) return fail(KKT_ERR_DEVICE, hip_msg("kernel timing failed", err));
+    *ms_per_launch = cnt ? (double)ms / ((double)reps * cnt) : 0.0;
+    *launches = cnt;
+    return KKT_OK;
+  }
+
   int set_values_device(const double* dv) {
     if (dv != d_vals &&
         hipMemcpyAsync(d_vals, dv, (size_t)nnz * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess)
@@ -2656,6 +2760,12 @@ int kkt_get_info(kkt_solver* h, kkt_info* out) {
   out->refine_steps = h->last_refine_steps;
   out->fp64_fallbacks = h->fp64_fallbacks;
   out->last_rel_residual = h->last_relres;
+  out->device_bytes = (long long)h->held_bytes();
+  out->factored_bytes = (long long)(h->plan_bytes() + h->pattern_bytes() + h->numeric_bytes(h->prec));
+  out->fp64_fallback_bytes =
+      (h->prec == KKT_PREC_FP32_REFINE && !h->d_F64)
+          ? (long long)(((size_t)h->front_elems + 2 * (size_t)h->n + 4 + (size_t)std::max<long long>(h->w_elems, 1)) * 8)
+          : 0;
   return KKT_OK;
 }
 
@@ -2739,6 +2849,14 @@ int kkt_set_forward_forms(kkt_solver* h, const int* launches) {
   h->drop_graphs();
   h->ok();
   return KKT_OK;
+}
+
+int kkt_time_kernels(kkt_solver* h, int what, int reps, double* ms_per_launch, int* launches_per_rep) {
+  CHECK_H(h);
+  if (!ms_per_launch || !launches_per_rep) return h->fail(KKT_ERR_ARGUMENT, "null output");
+  int rc = h->time_kernels(what, reps, ms_per_launch, launches_per_rep);
+  if (!rc) h->ok();
+  return rc;
 }
 
 int kkt_timing_get(kkt_solver* h, double* ms, long long* launches, double* update_flops) {

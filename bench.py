@@ -152,23 +152,28 @@ def _timing(L, h):
     return [ms[c] for c in range(8)], [int(cnt[c]) for c in range(8)], uf.value
 
 
-def kernel_pass(L, h, vals_ptr, b_ptr, x_ptr, nsolves=6):
+def kernel_pass(L, h, vals_ptr, b_ptr, x_ptr):
     """After the timed region: per-launch HIP-event times (events on the solver's
-    stream, eager launches) of one factor's kernel classes and of the triangular
-    sweeps and SpMVs of nsolves refined solves, plus the factor's wall time as it
-    runs in the sequence (graph replay, no events)."""
+    stream, eager launches) of one factor's kernel classes; the average kernel
+    duration of the forward sweep, the backward sweep and the residual SpMV from
+    20 graph replays each (one sweep / one SpMV per replay); and the factor's wall
+    time as it runs in the sequence (graph replay, no events)."""
     import torch
     from artisynth_core_amd._lib import KktInfo
     L.kkt_timing_reset(h, 0x8F)  # factor classes + vector ops
     assert L.kkt_factor_device(h, vals_ptr) == 0, L.kkt_last_error(h)
     fms, fcnt, uflops = _timing(L, h)
-    L.kkt_timing_reset(h, (1 << 4) | (1 << 5) | (1 << 6))
-    for _ in range(nsolves):
-        assert L.kkt_solve_device(h, b_ptr, x_ptr) == 0, L.kkt_last_error(h)
-    sms, scnt, _ = _timing(L, h)
+    L.kkt_timing_reset(h, 0)
+    # sweep and SpMV kernel durations: graph replays between two HIP events on
+    # the solver's stream (no per-launch events, so they match a kernel trace)
+    kt = {}
+    for what, key in ((0, "fwd"), (1, "bwd"), (2, "spmv")):
+        ms = ctypes.c_double()
+        cnt = ctypes.c_int()
+        assert L.kkt_time_kernels(h, what, 20, ctypes.byref(ms), ctypes.byref(cnt)) == 0, L.kkt_last_error(h)
+        kt[key] = (ms.value, cnt.value)
     inf = KktInfo()
     L.kkt_get_info(h, ctypes.byref(inf))
-    L.kkt_timing_reset(h, 0)
     torch.cuda.synchronize()
     reps = 5
     t0 = time.perf_counter()
@@ -179,9 +184,9 @@ def kernel_pass(L, h, vals_ptr, b_ptr, x_ptr, nsolves=6):
     return {"factor_kernel_ms": {TIMING_CLASSES[c]: fms[c] for c in (0, 1, 2, 3, 7)},
             "factor_kernel_launches": {TIMING_CLASSES[c]: fcnt[c] for c in (0, 1, 2, 3, 7)},
             "update_flops": uflops, "factor_ms": factor_ms,
-            "fwd_ms": sms[4], "fwd_launches": scnt[4], "bwd_ms": sms[5], "bwd_launches": scnt[5],
-            "spmv_ms": sms[6], "spmv_launches": scnt[6],
-            "sweeps": nsolves * (1 + inf.refine_steps), "refine_steps": inf.refine_steps}
+            "fwd_ms": kt["fwd"][0] * kt["fwd"][1], "fwd_launches": kt["fwd"][1],
+            "bwd_ms": kt["bwd"][0] * kt["bwd"][1], "bwd_launches": kt["bwd"][1],
+            "spmv_ms": kt["spmv"][0], "spmv_launches": 1, "sweeps": 1}
 
 
 def main():
@@ -192,12 +197,14 @@ def main():
     ap.add_argument("--nsys", type=int, default=20, help="systems in the step sequence")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--grid", default="100x16x16")
-    ap.add_argument("--mode", default="bench", choices=["bench", "pmc", "batch", "serve", "precision", "multi"],
+    ap.add_argument("--mode", default="bench",
+                    choices=["bench", "pmc", "batch", "serve", "precision", "multi", "stream"],
                     help="pmc: one FP32 factor + solve and a 256 MiB copy, for counter runs; "
                          "batch: the 256-system mix, one batched launch vs per-system launches; "
                          "precision: the ~210k-DOF Mooney-Rivlin tet box, direct FP32-MFMA+refine "
                          "vs FP64 solves; multi: block solves of --nrhs right-hand sides on the "
-                         "beam's first KKT, batched vs one at a time")
+                         "beam's first KKT, batched vs one at a time; stream: streaming read and copy "
+                         "rates of one launch over 4 MB - 1 GB with cold caches (the achievable HBM ceiling)")
     ap.add_argument("--nrhs", type=int, default=64, help="multi: right-hand sides per block solve")
     ap.add_argument("--tet", default="121x24x24", help="precision: Delaunay lattice points")
     ap.add_argument("--oracle", action="store_true",
@@ -234,6 +241,8 @@ def main():
                              args.warmup, args.oracle)
     if args.mode == "multi":
         return multi_run(nx, ny, nz, args.nrhs, max(args.steps // 4, 2), args.warmup)
+    if args.mode == "stream":
+        return stream_run()
     L = lib()
     # setup (untimed): integrate the ConstrainedBackwardEuler sequence with the
     # FP64 GPU path, so each step's system is assembled at the state the previous
@@ -434,6 +443,47 @@ def profile_traffic(kernel):
     return d.get(kernel, {}).get("bytes_per_launch")
 
 
+def stream_run(reps=5):
+    """Achievable HBM read rate of one launch at the sizes of the solve's tree
+    levels and beyond: the streaming probe (artisynth_core_amd/csrc/probe.hip,
+    16-byte loads, 1-16 in flight per lane, one pass over the buffer) launched
+    alone with cold caches -- before every timed launch the probe reads a
+    separate 1 GiB buffer (a read, not a fill, so no dirty lines are written back
+    during the timed launch).  Times from HIP events around each launch on
+    torch's current stream (where the probe runs); a rocprofv3 kernel trace of
+    this mode gives the kernel durations without launch overhead."""
+    import torch
+    P = ctypes.CDLL(os.path.join(ROOT, "artisynth_core_amd", "libkkt_probe.so"))
+    P.kkt_probe_stream_read.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.c_void_p]
+    P.kkt_probe_stream_read.restype = ctypes.c_int
+    st = torch.cuda.current_stream().cuda_stream
+    flush = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+    sink = torch.zeros((1 << 30) // 4096 + 16, dtype=torch.float32, device="cuda")
+    out = []
+    for mb in (4, 8, 16, 32, 64, 128, 256, 512, 1024):
+        a = torch.ones(mb << 18, dtype=torch.float32, device="cuda")
+        r = {"MB": mb}
+        for g in (1, 4, 8, 16):
+            ts = []
+            for _ in range(reps):
+                assert P.kkt_probe_stream_read(flush.data_ptr(), flush.numel() * 4, 8, sink.data_ptr(), st) == 0
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                assert P.kkt_probe_stream_read(a.data_ptr(), mb << 20, g, sink.data_ptr(), st) == 0
+                e1.record()
+                torch.cuda.synchronize()
+                ts.append(e0.elapsed_time(e1))
+            t = float(np.median(ts))
+            r["read_g%d_ms" % g] = t
+            r["read_g%d_GBs" % g] = (mb << 20) / (t * 1e-3) / 1e9
+        out.append(r)
+        log("stream %5d MB: read GB/s with 1/4/8/16 loads in flight per lane: %s" % (
+            mb, " / ".join("%.0f" % r["read_g%d_GBs" % g] for g in (1, 4, 8, 16))))
+        del a
+    print(json.dumps({"mode": "stream", "peak_GBs": PEAKS["hbm_GBs"], "results": out}), flush=True)
+
+
 def pmc_run(nx, ny, nz):
     """Counter-collection workload: one FP32 factor + solve of step 0, then a plain
     256 MiB device copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters."""
@@ -625,10 +675,18 @@ def batch_run(nsys, reps, warmup):
     B.dispose()
 
 
-def serve_run(ninst, nreq, rate, nworkers, budget_gb):
-    """The serving workload: ninst instances (1k-90k DOF, beam/tet/rigid), a Poisson stream of
-    step requests at `rate` req/s, HBM-budget admission, FP32-MFMA + FP64 refinement
-    (1e-10 relative residual), p50/p99 latency from submission to completion."""
+def serve_run(ninst, nreq, rate, nworkers, budget_gb, eps=3e-3):
+    """The serving workload: ninst instances (1k-90k DOF, beam/tet/rigid), a Poisson
+    stream of step requests at `rate` req/s, HBM-budget admission, FP32-MFMA + FP64
+    refinement (1e-10 relative residual), through the C++ serving loop
+    (kkt_serve_*).  Each instance's values drift as a seeded random walk (every
+    entry moves by up to eps relatively per step), so the resident factor goes
+    stale and some steps refactor.  All request inputs are generated in HBM before
+    the clock starts and submitted as device arrays (kkt_serve_submit_device); the
+    submission loop only sleeps and submits.  Latency = submission to completion
+    (server clock); p50/p99, achieved offered rate, queue depth, admissions and
+    evictions are reported."""
+    import torch
     import scipy.sparse as sp
     from artisynth_core_amd import generators as G
     from artisynth_core_amd.serving import SolveServer, latency_stats
@@ -637,22 +695,42 @@ def serve_run(ninst, nreq, rate, nworkers, budget_gb):
     log("generated %d instances (n %d..%d, total %d) in %.1fs" % (
         ninst, min(len(m[2]) - 1 for m in mix), max(len(m[2]) - 1 for m in mix),
         sum(len(m[2]) - 1 for m in mix), time.time() - t0))
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(2000)
+    gaps = rng.exponential(1.0 / rate, nreq)
+    who = rng.integers(0, ninst, nreq)
+    # inputs in HBM: per instance its base values and right-hand side; per request
+    # the step's values (random walk) and a solution buffer
+    t0 = time.time()
+    base = [torch.from_numpy(np.ascontiguousarray(m[4])).to(dev) for m in mix]
+    rhs = [torch.from_numpy(np.ascontiguousarray(m[5])).to(dev) for m in mix]
+    drift = [torch.zeros_like(v) for v in base]
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(2000)
+    vals, xs, steps = [], [], np.zeros(ninst, np.int64)
+    for q in range(nreq):
+        i = int(who[q])
+        drift[i] += eps * (2.0 * torch.rand(base[i].shape, generator=gen, device=dev, dtype=torch.float64) - 1.0)
+        vals.append(base[i] * (1.0 + drift[i]))
+        xs.append(torch.zeros(len(mix[i][5]), dtype=torch.float64, device=dev))
+        steps[i] += 1
+    torch.cuda.synchronize()
+    log("request inputs in HBM: %.1f GB in %.1fs" % (sum(v.numel() for v in vals) * 8 / 1e9, time.time() - t0))
     srv = SolveServer(budget_bytes=int(budget_gb * (1 << 30)), nworkers=nworkers)
     for i, (kind, nd, ro, ci, va, b) in enumerate(mix):
         srv.register(i, ro, ci, va)
     t0 = time.time()
-    warm = [srv.submit(i, mix[i][4], mix[i][5]) for i in range(ninst)]
+    warm_x = [torch.zeros(len(m[5]), dtype=torch.float64, device=dev) for m in mix]
+    warm = [srv.submit_device(i, base[i].data_ptr(), rhs[i].data_ptr(), warm_x[i].data_ptr()) for i in range(ninst)]
     for r in warm:
-        r.done.wait()
+        r.wait()
     errs = [r.error for r in warm if r.error]
     assert not errs, errs[:3]
+    st0 = srv.stats()
     log("admitted + first factor of %d instances in %.1fs (%.1f GB resident)" % (
-        ninst, time.time() - t0, srv.used / 2 ** 30))
-    rng = np.random.default_rng(2000)
-    gaps = rng.exponential(1.0 / rate, nreq)
-    who = rng.integers(0, ninst, nreq)
-    step = np.zeros(ninst, np.int64)
+        ninst, time.time() - t0, st0["bytes_resident"] / 2 ** 30))
     reqs = []
+    st_start = srv.stats()
     t_start = time.perf_counter()
     t_next = t_start
     for q in range(nreq):
@@ -661,33 +739,39 @@ def serve_run(ninst, nreq, rate, nworkers, budget_gb):
         if t_next > now:
             time.sleep(t_next - now)
         i = int(who[q])
-        step[i] += 1
-        kind, nd, ro, ci, va, b = mix[i]
-        reqs.append(srv.submit(i, G.perturbed_values(va, int(step[i]), i), b))
+        reqs.append(srv.submit_device(i, vals[q].data_ptr(), rhs[i].data_ptr(), xs[q].data_ptr()))
+    t_sub = time.perf_counter()
     for r in reqs:
-        r.done.wait()
-    t_end = max(r.t_done for r in reqs)
+        r.wait()
+    t_end = time.perf_counter()
+    st = srv.stats()
     errs = [r.error for r in reqs if r.error]
-    st = latency_stats(reqs)
+    lat = latency_stats(reqs)
     # residual check on a sample of the results (host)
     res = []
-    for r in reqs[:: max(1, nreq // 100)]:
-        kind, nd, ro, ci, va, b = mix[r.iid]
+    for q in range(0, nreq, max(1, nreq // 100)):
+        i = int(who[q])
+        kind, nd, ro, ci, va, b = mix[i]
         n = len(ro) - 1
-        U = sp.csr_matrix((r.vals, ci, ro), shape=(n, n))
+        U = sp.csr_matrix((vals[q].cpu().numpy(), ci, ro), shape=(n, n))
         A = U + sp.triu(U, 1).T
-        res.append(float(np.linalg.norm(A @ r.x - r.b) / np.linalg.norm(r.b)))
+        x = xs[q].cpu().numpy()
+        res.append(float(np.linalg.norm(A @ x - b) / np.linalg.norm(b)))
     kinds = [r.kind for r in reqs]
-    line = {"mode": "serve", "ninst": ninst, "nreq": nreq, "offered_rate": rate,
-            "workers": nworkers, "budget_gb": budget_gb,
-            "throughput_req_s": nreq / (t_end - t_start), "p50_ms": st["p50_ms"],
-            "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"], "errors": len(errs),
-            "direct": kinds.count("direct"), "iterative": kinds.count("iterative"),
-            "max_rel_residual_sample": max(res), "resident_gb": srv.used / 2 ** 30,
-            "admissions": srv.admissions, "evictions": srv.evictions,
-            "total_dof": int(sum(len(m[2]) - 1 for m in mix))}
-    log("serve: %.0f req/s (offered %.0f), p50 %.2f ms, p99 %.2f ms, max resid %.2e" % (
-        line["throughput_req_s"], rate, st["p50_ms"], st["p99_ms"], max(res)))
+    offered = nreq / (t_sub - t_start)
+    line = {"mode": "serve", "ninst": ninst, "nreq": nreq, "rate_target": rate,
+            "offered_rate_achieved": offered, "workers": nworkers, "budget_gb": budget_gb,
+            "throughput_req_s": nreq / (t_end - t_start), "p50_ms": lat["p50_ms"], "p99_ms": lat["p99_ms"],
+            "mean_ms": lat["mean_ms"], "errors": len(errs), "direct": kinds.count("direct"),
+            "iterative": kinds.count("iterative"), "max_rel_residual_sample": max(res),
+            "resident_gb": st["bytes_resident"] / 2 ** 30, "peak_gb": st["bytes_peak"] / 2 ** 30,
+            "admissions": st["admissions"], "evictions": st["evictions"], "queue_max": st["queue_max"],
+            "concurrency": (st["service_ms"] - st_start["service_ms"]) / (1e3 * (t_end - t_start)),
+            "admissions_timed": st["admissions"] - st_start["admissions"],
+            "total_dof": int(sum(len(m[2]) - 1 for m in mix)), "inputs": "device (HBM), drift eps %g" % eps}
+    log("serve: offered %.0f req/s (target %.0f), served %.0f req/s, p50 %.2f ms, p99 %.2f ms, direct %d, "
+        "evictions %d, max resid %.2e" % (offered, rate, line["throughput_req_s"], lat["p50_ms"], lat["p99_ms"],
+                                          line["direct"], line["evictions"], max(res)))
     print(json.dumps(line), flush=True)
     srv.shutdown()
 

@@ -70,6 +70,11 @@ typedef struct kkt_info {
   int pivoted_blocks;    /* 64-column diagonal blocks that needed Bunch-Kaufman
                             interchanges or 2x2 pivots (the rest took 1x1 pivots
                             in order) */
+  long long device_bytes;        /* device memory the handle holds now */
+  long long factored_bytes;      /* ... once factored in the requested precision
+                                    (structure, pattern, fronts, solve vectors) */
+  long long fp64_fallback_bytes; /* what an FP64 refactor after stalled FP32
+                                    refinement would add (0 if already held) */
 } kkt_info;
 
 /* CuDssSolver.doInit (CuDssJNI.cc:11-23): creates a handle (NULL on failure). */
@@ -284,6 +289,13 @@ int kkt_get_forward_forms(kkt_solver* h, int* launches);
  * plan; the solution is bitwise the same either way. */
 int kkt_set_forward_forms(kkt_solver* h, const int* launches);
 int kkt_timing_get(kkt_solver* h, double* ms, long long* launches, double* update_flops);
+/* Kernel durations without per-launch events (no counterpart in the
+ * reference): one forward sweep (what 0, the k_fwd_level launches of a
+ * triangular solve), one backward sweep (1, the k_bwd_cols launches) or one
+ * residual SpMV (2) on the current factor, captured into a graph and replayed
+ * `reps` times between two HIP events on the handle's stream.  Returns the
+ * average duration of one launch and the launches per replay. */
+int kkt_time_kernels(kkt_solver* h, int what, int reps, double* ms_per_launch, int* launches_per_rep);
 
 /* ---- batched systems ----
  * Many independent KKT systems of uneven size (the contact / rigid-body
@@ -329,6 +341,73 @@ int kkt_batch_set_ir_steps(kkt_batch* b, int nsteps);
 int kkt_batch_get_info(kkt_batch* b, kkt_info* out);
 int kkt_batch_system_info(kkt_batch* b, int sys, kkt_batch_sysinfo* out);
 const char* kkt_batch_last_error(kkt_batch* b);
+
+/* ---- serving loop ----
+ * Step-solve requests from many concurrent simulation instances (no
+ * counterpart in the reference, where every simulation drives its own solver).
+ * Each registered instance gets its own handle (own HIP stream) on its first
+ * request; its analysis and factor stay resident in HBM between requests.
+ * Admission by an HBM budget: after an instance's analysis, what its handle
+ * will hold once factored (kkt_info.factored_bytes) is reserved before the
+ * factor allocates, evicting idle resident instances least recently used
+ * first; an instance larger than the whole budget fails its requests with
+ * KKT_ERR_ALLOC.  Requests enter one FIFO queue served by `nworkers` threads,
+ * each taking the oldest request whose instance is idle (continuous batching:
+ * arrivals start as soon as a worker and memory are free, instances' device
+ * work overlaps on their streams).  An instance's requests run in submission
+ * order, one at a time, each a KKT step with KKTSolver.factorAndSolve's policy
+ * (KKTSolver.java:930-1014): BiCGStab with the resident factor while it
+ * converges within max_iter iterations, else a factor and a refined solve. */
+typedef struct kkt_server kkt_server;
+typedef struct kkt_serve_config {
+  long long budget_bytes; /* device memory the resident instances may hold (default 200 GiB) */
+  int nworkers;           /* worker threads, 1..64 (default 8) */
+  int precision;          /* KKT_PREC_* of every instance (default FP32 + refinement) */
+  int tol_exp;            /* iterative solves to 10^-tol_exp relative residual (default 10) */
+  int max_iter;           /* BiCGStab iterations before a refactor (default 30) */
+  double refine_tol;      /* refinement target of FP32 factors (default 1e-10) */
+} kkt_serve_config;
+typedef struct kkt_serve_result {
+  int status;               /* KKT_OK or a negative KKT_ERR_* code */
+  int kind;                 /* 1: iterative solve with the resident factor, 2: factor + solve */
+  int iterations;           /* BiCGStab iterations (kind 1) */
+  double rel_residual;      /* the handle's last refinement residual */
+  double submit_ms, start_ms, done_ms; /* since the server's creation */
+  char message[160];        /* error text when status != 0 */
+} kkt_serve_result;
+typedef struct kkt_serve_stats {
+  long long submitted, completed, errors, direct, iterative;
+  long long admissions, evictions;
+  long long queue_depth, queue_max;      /* requests waiting now / at most */
+  long long budget_bytes, bytes_resident, bytes_peak;
+  int instances_resident;
+  double service_ms;                     /* sum of the requests' solve times */
+  double elapsed_ms;                     /* since the server's creation */
+} kkt_serve_stats;
+/* The defaults listed above. */
+void kkt_serve_default_config(kkt_serve_config* cfg);
+/* Starts the worker threads; NULL on a bad configuration. */
+kkt_server* kkt_serve_create(const kkt_serve_config* cfg);
+/* Finishes every submitted request, stops the workers and frees every handle.
+ * Requests not yet waited for are dropped with the server. */
+void kkt_serve_destroy(kkt_server* s);
+/* An instance's pattern (0-based upper CSR, as kkt_set_pattern) and the values
+ * its analysis orders by (zero-diagonal rows after their neighbours); copied. */
+int kkt_serve_register(kkt_server* s, int iid, int n, int nnz, const int* rowOffs, const int* colIdxs,
+                       const double* vals0, int mtype);
+/* Queues one step of instance iid: the step's values (nnz) and right-hand side
+ * (n) are copied at submission; x (n doubles) is written when the request
+ * completes and must stay valid until kkt_serve_wait returns.  Returns the
+ * request id (> 0) or a negative status. */
+long long kkt_serve_submit(kkt_server* s, int iid, const double* vals, const double* b, double* x);
+/* The same with device arrays (used in place: they must hold their data when
+ * submitted and stay valid until kkt_serve_wait returns). */
+long long kkt_serve_submit_device(kkt_server* s, int iid, const double* d_vals, const double* d_b, double* d_x);
+/* Blocks until the request completes; fills out (may be NULL) and returns its
+ * status.  Each request is waited for once. */
+int kkt_serve_wait(kkt_server* s, long long req, kkt_serve_result* out);
+int kkt_serve_get_stats(kkt_server* s, kkt_serve_stats* out);
+const char* kkt_serve_last_error(kkt_server* s);
 
 /* Host-only symbolic analysis (no GPU needed): structure statistics and the
  * same arrays kkt_get_symbolic returns, for pattern checks on the CPU. */

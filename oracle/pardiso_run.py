@@ -18,7 +18,9 @@ Policy (KKTSolver.java:930-1014 with PardisoSolver.iterativeSolve,
 PardisoSolver.java:1697-1737): step 0 factors (phase 22) and solves (phase 33,
 one refinement step); each later step first runs phase 33 with iparm[3] = 102 on
 its new values (CGS preconditioned by the stale factor, pardisoMkl.cc:704-750)
-and refactors + solves when that returns <= 0.
+and refactors + solves when that returns <= 0 or needed more than MAX_ITER
+iterations -- the GPU sequence's rule (the stale factor is kept until the
+iterative solve exceeds 30 iterations); the failed attempt's time counts.
 """
 import json
 import os
@@ -30,6 +32,8 @@ import scipy.sparse as sp
 
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 import pardiso as P  # noqa: E402
+
+MAX_ITER = 30
 
 
 def main(inp, outp):
@@ -57,7 +61,7 @@ def main(inp, outp):
             it = 0
             if have:
                 it, x = p.iterative_solve(V[k], B[k], 10)
-            if it > 0:
+            if 0 < it <= MAX_ITER:
                 kinds.append("i")
             else:
                 p.factor(V[k])

@@ -285,7 +285,8 @@ def test_batched_systems_vs_oracle(gpu):
 
 
 def test_serving_alone_equals_mix(gpu):
-    # each instance's results are the same alone and in a mix with eviction pressure
+    # each instance's results are the same alone and in a mix (no eviction: the
+    # same step kinds in the same order on each instance's own handle)
     _say("serving alone vs mix")
     from artisynth_core_amd.serving import SolveServer
     mix = [("beam", 900), ("tet", 1200), ("rigid", 1500), ("beam", 2500)]
@@ -299,9 +300,9 @@ def test_serving_alone_equals_mix(gpu):
         srv.register(i, inst[i][0], inst[i][1], inst[i][2])
         rs = [srv.submit(i, v, b) for v, b in seq[i]]
         for r in rs:
-            r.done.wait()
+            r.wait()
             assert r.error is None, r.error
-        alone[i] = [r.x for r in rs]
+        alone[i] = [(r.x, r.kind) for r in rs]
         srv.shutdown()
     srv = SolveServer(nworkers=3)
     for i in range(len(inst)):
@@ -311,10 +312,86 @@ def test_serving_alone_equals_mix(gpu):
         for i in range(len(inst)):
             rs.append((i, s_, srv.submit(i, *seq[i][s_])))
     for i, s_, r in rs:
-        r.done.wait()
+        r.wait()
         assert r.error is None, r.error
-        assert rel_err(r.x, alone[i][s_]) <= 1e-8
+        assert r.kind == alone[i][s_][1]
+        assert rel_err(r.x, alone[i][s_][0]) <= 1e-12
         assert rel_residual(inst[i][0], inst[i][1], seq[i][s_][0], r.x, seq[i][s_][1]) <= 1e-8
+    st = srv.stats()
+    assert st["evictions"] == 0 and st["admissions"] == len(inst) and st["completed"] == len(rs)
+    srv.shutdown()
+
+
+def test_serving_eviction_and_readmission(gpu):
+    # 18 instances under a budget that holds about four of them: requests evict
+    # idle instances (least recently used first) and re-admit evicted ones; every
+    # result meets the tolerance and matches the instance's solve alone (an
+    # evicted instance refactors on its next step, so it matches to the
+    # refinement tolerance rather than bitwise); an instance larger than the whole
+    # budget fails its requests with KKT_ERR_ALLOC while the rest go on
+    _say("serving eviction")
+    import ctypes
+    from artisynth_core_amd._lib import KktInfo, lib
+    from artisynth_core_amd._lib import ptr as nptr
+    from artisynth_core_amd.serving import SolveServer
+    kinds = ("beam", "tet", "rigid")
+    inst = [G.serving_instance(kinds[i % 3], 700 + 150 * i, 1700 + i) for i in range(18)]
+    L = lib()
+    need = []
+    for ro, ci, va, b in inst:
+        h = L.kkt_create()
+        L.kkt_set_precision(h, KKT_PREC_FP32_REFINE)
+        ro32, ci32 = np.ascontiguousarray(ro, np.int32), np.ascontiguousarray(ci, np.int32)
+        va64 = np.ascontiguousarray(va)
+        assert L.kkt_set_pattern(h, len(ro) - 1, int(ro[-1]), nptr(ro32, ctypes.c_int), nptr(ci32, ctypes.c_int), 1) == 0
+        assert L.kkt_analyze_values(h, nptr(va64, ctypes.c_double)) == 0
+        inf = KktInfo()
+        L.kkt_get_info(h, ctypes.byref(inf))
+        need.append(inf.factored_bytes)
+        L.kkt_destroy(h)
+    budget = int(sum(sorted(need)[-4:]) * 1.05)
+    steps = 3
+    seq = {i: [(G.perturbed_values(inst[i][2], s_, i, eps=2e-3), inst[i][3]) for s_ in range(steps)]
+           for i in range(len(inst))}
+    alone = {}
+    srv = SolveServer(nworkers=1)
+    for i in range(len(inst)):
+        srv.register(i, inst[i][0], inst[i][1], inst[i][2])
+    for i in range(len(inst)):
+        alone[i] = [srv.submit(i, v, b).wait() for v, b in seq[i]]
+    srv.shutdown()
+    big = G.serving_instance("beam", 30000, 77)
+    srv = SolveServer(budget_bytes=budget, nworkers=4)
+    for i in range(len(inst)):
+        srv.register(i, inst[i][0], inst[i][1], inst[i][2])
+    srv.register(99, big[0], big[1], big[2])
+    rs = []
+    for s_ in range(steps):
+        for i in range(len(inst)):
+            rs.append((i, s_, srv.submit(i, *seq[i][s_])))
+        if s_ == 1:
+            rbig = srv.submit(99, big[2], big[3])
+    worst = 0.0
+    for i, s_, r in rs:
+        r.wait()
+        assert r.error is None, r.error
+        ro, ci = inst[i][0], inst[i][1]
+        va, b = seq[i][s_]
+        assert rel_residual(ro, ci, va, r.x, b) <= 1e-9
+        a = alone[i][s_]
+        assert rel_residual(ro, ci, va, a.x, b) <= 1e-9
+        worst = max(worst, rel_err(r.x, a.x))
+        if r.kind == a.kind == "direct" and s_ == 0:
+            assert rel_err(r.x, a.x) <= 1e-12  # first step: the same factor and solve
+    assert worst <= 1e-6, worst
+    rbig.wait()
+    assert rbig.status == -2 and "more than the whole budget" in rbig.error
+    st = srv.stats()
+    print("[serve] budget %.1f MB, evictions %d, admissions %d, peak %.1f MB, worst diff %.2e" % (
+        budget / 2 ** 20, st["evictions"], st["admissions"], st["bytes_peak"] / 2 ** 20, worst), flush=True)
+    assert st["evictions"] > 0 and st["admissions"] > len(inst)
+    assert st["bytes_peak"] <= budget
+    assert st["completed"] == len(rs) + 1 and st["errors"] == 1
     srv.shutdown()
 
 

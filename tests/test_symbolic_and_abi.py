@@ -244,3 +244,47 @@ def test_beam_dynamics_integrates_the_constrained_step():
     # step 0 from rest: b = h * fext (gravity only), the lambda rows zero
     assert np.allclose(B[0][:beam.sizeM], beam.h * beam.gravity_loads().reshape(-1), rtol=0, atol=1e-15)
     assert not B[0][beam.sizeM:].any()
+
+
+def test_serving_loop_mechanics_without_gpu():
+    # the serving loop's host side: configuration checks, registration errors,
+    # per-instance order and error reporting (without a GPU every admission fails
+    # with KKT_ERR_DEVICE, which each request reports, and the loop keeps going)
+    import ctypes
+    L = _lib.lib()
+    cfg = _lib.KktServeConfig()
+    L.kkt_serve_default_config(ctypes.byref(cfg))
+    assert (cfg.budget_bytes, cfg.nworkers, cfg.max_iter, cfg.tol_exp) == (200 << 30, 8, 30, 10)
+    bad = _lib.KktServeConfig.from_buffer_copy(cfg)
+    bad.nworkers = 0
+    assert not L.kkt_serve_create(ctypes.byref(bad))
+    cfg.nworkers = 2
+    s = L.kkt_serve_create(ctypes.byref(cfg))
+    assert s
+    try:
+        ro = np.array([0, 2, 3], np.int32)
+        ci = np.array([0, 1, 1], np.int32)
+        va = np.array([2.0, 1.0, 3.0])
+        I, D = ctypes.c_int, ctypes.c_double
+        assert L.kkt_serve_register(s, 7, 2, 3, _lib.ptr(ro, I), _lib.ptr(ci, I), _lib.ptr(va, D), 1) == 0
+        assert L.kkt_serve_register(s, 7, 2, 3, _lib.ptr(ro, I), _lib.ptr(ci, I), _lib.ptr(va, D), 1) == -14
+        assert b"already registered" in L.kkt_serve_last_error(s)
+        assert L.kkt_serve_register(s, 8, 2, 3, _lib.ptr(ro, I), _lib.ptr(ci, I), _lib.ptr(va, D), 0) == -9
+        b = np.ones(2)
+        x = np.zeros(2)
+        assert L.kkt_serve_submit(s, 99, _lib.ptr(va, D), _lib.ptr(b, D), _lib.ptr(x, D)) == -14
+        ids = [L.kkt_serve_submit(s, 7, _lib.ptr(va, D), _lib.ptr(b, D), _lib.ptr(x, D)) for _ in range(3)]
+        assert all(i > 0 for i in ids) and ids == sorted(ids)
+        for i in ids:
+            res = _lib.KktServeResult()
+            rc = L.kkt_serve_wait(s, i, ctypes.byref(res))
+            assert rc == res.status and rc in (-15, -1, -2)  # no device: admission fails
+            assert b"admission of instance 7 failed" in res.message
+            assert res.submit_ms <= res.start_ms <= res.done_ms
+        assert L.kkt_serve_wait(s, ids[0], None) == -14  # each request is waited for once
+        st = _lib.KktServeStats()
+        L.kkt_serve_get_stats(s, ctypes.byref(st))
+        assert (st.submitted, st.completed, st.errors, st.admissions) == (3, 3, 3, 0)
+    finally:
+        L.kkt_serve_destroy(s)
+    L.kkt_serve_destroy(None)
