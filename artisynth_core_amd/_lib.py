@@ -104,6 +104,8 @@ SIGNATURES = [
     ("kkt_get_solve_shapes", _I, [_P, _IP, _IP]),
     ("kkt_get_forward_forms", _I, [_P, _IP]),
     ("kkt_set_forward_forms", _I, [_P, _IP]),
+    ("kkt_get_forward_row_groups", _I, [_P, _IP]),
+    ("kkt_set_forward_row_groups", _I, [_P, _IP]),
     ("kkt_solve", _I, [_P, _DP, _DP]),
     ("kkt_solve_device", _I, [_P, _P, _P]),
     ("kkt_solve_device_noalloc", _I, [_P, _P, _P]),

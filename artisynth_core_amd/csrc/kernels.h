@@ -15,6 +15,10 @@ constexpr int FWD_SMALL_M = 256;
 // widest supernode the solve kernels take (LDS vectors of KMAX entries); the
 // symbolic analysis splits wider ones, and the solve plan checks it
 constexpr int KMAX = 1024;
+// rows of one forward row block of a large front at most (its row groups share
+// one gather of the front's right-hand side; the contribution rows' entries are
+// staged in LDS)
+constexpr int FWD_GROUP_ROWS = 1024;
 // right-hand sides one multi-RHS solve pass carries (vectors stored one after
 // another, each with its own leading dimension)
 constexpr int SOLVE_NRHS = 8;
@@ -50,6 +54,8 @@ struct DevSym {
   const int* wpos;
   // connected component (assembly tree) of each supernode, 0..nroots-1
   const int* sn_root;
+  // offset of each supernode's block in the packed solve arrays
+  const long long* sn_poff;
 };
 
 // Work item of the solve kernels: one workgroup's share of a supernode, with the
@@ -59,14 +65,28 @@ struct alignas(16) SolveItem {
   int blk;   // block of positions / rows / columns of the supernode
   int lg;    // 2^lg lanes per row or column (rows / columns kernels)
   int kind;  // forward gather: 1 = gather block of a large front, 0 = whole small front
-  int k, m, first, pad;
+  int k, m, first;
+  int pad;   // forward row blocks (kind 2): row groups per block
   long long foff;  // front offset (elements)
   long long uoff;  // update-vector offset
   long long roff;  // offset of the below-diagonal rows in rows[]
   long long goff;  // offset of the front positions in the gather map
   long long voff;  // offset of the supernode's backward vector entries
-  long long pad2;
+  long long poff;  // offset of the supernode's block in the packed solve arrays
 };
+
+// Packed solve arrays (written after each factor from the solve form of the
+// fronts, supernodes in level-major order, k(k-1)/2 + k r entries each, so one
+// tree level's data is one contiguous range):
+//   forward  row i of supernode s: G(i, 0 .. min(i, k)) -- X(i, :) without the
+//            unit diagonal for i < k, Z(i - k, :) below -- at poff + prow_off(i, k);
+//   backward column c < k: G(c+1 .. m, c) at poff + pcol_off(c, m).
+__host__ __device__ inline long long prow_off(int i, int k) {
+  return i < k ? (long long)i * (i - 1) / 2 : (long long)k * (k - 1) / 2 + (long long)(i - k) * k;
+}
+__host__ __device__ inline long long pcol_off(int c, int m) {
+  return (long long)c * (m - 1) - (long long)c * (c - 1) / 2;
+}
 
 struct FactorCounters {  // device ints
   int npos, nneg, npert, n2x2;
@@ -99,15 +119,20 @@ template <typename T>
 void launch_zt(const int4* items, int nitems, const DevSym& d, T* F, hipStream_t st);
 template <typename T>
 void launch_gform(const int4* items, int nitems, const DevSym& d, T* F, hipStream_t st);
-// level-scheduled solves (vectors in the front precision T)
+// copies the solve form of every front into the packed solve arrays (items:
+// (supernode, block of 4 front columns))
 template <typename T>
-void launch_fwd_level(const SolveItem* items, int nitems, const DevSym& d, const T* F, const T* bp,
+void launch_pack_solve(const int2* items, int nitems, const DevSym& d, const T* F, T* PF, T* PB, hipStream_t st);
+// level-scheduled solves (vectors in the front precision T) on the packed
+// arrays: PF forward rows, PB backward columns
+template <typename T>
+void launch_fwd_level(const SolveItem* items, int nitems, const DevSym& d, const T* PF, const T* bp,
                       T* fp, T* y, T* u, hipStream_t st);
 template <typename T>
 void launch_dsolve(int n, const T* E0, const T* E1, const signed char* piv, const int* wpos,
                    const T* y, T* wv, hipStream_t st);
 template <typename T>
-void launch_bwd_cols(const SolveItem* items, int nitems, const DevSym& d, const T* F, T* wv,
+void launch_bwd_cols(const SolveItem* items, int nitems, const DevSym& d, const T* PB, T* wv,
                      T* xp, hipStream_t st);
 
 // the same for nr <= SOLVE_NRHS right-hand sides at once (vector j of a set at

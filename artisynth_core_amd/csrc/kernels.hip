@@ -1042,7 +1042,7 @@ struct VI16<double> {
 // workgroup: one thread per position p gathers f_p = b_p + the children's update
 // entries at p (summed in ascending order) into LDS in pivot order, then one
 // thread per row: y_i = f_i + G(i,:) f (i < k), u_{i-k} = f_i - G(i,:) f (i >= k),
-// G(i,:) read across the columns of the lower copy (coalesced over the threads)
+// G(i,:) read from the row's packed forward entries (contiguous per thread)
 static_assert(FWD_SMALL_M == 256, "one thread per row of a small front");
 template <typename T>
 __device__ inline void fwd_small_front(const SolveItem& it, const DevSym& d, const T* __restrict__ F,
@@ -1057,10 +1057,10 @@ __device__ inline void fwd_small_front(const SolveItem& it, const DevSym& d, con
   // rest after it (FP64: half, so the kernel keeps more waves resident).
   constexpr int HOIST = sizeof(T) == 8 ? FWD_SMALL_K / 2 : FWD_SMALL_K;
   const int len = (p < m) ? min(p, k) : 0;
-  const T* rowp = F + it.foff + (p < m ? p : 0);
+  const T* rowp = F + it.poff + (p < m ? prow_off(p, k) : 0);
   T v[FWD_SMALL_K];
 #pragma unroll
-  for (int l = 0; l < HOIST; l++) v[l] = rowp[(size_t)(l < len ? l : 0) * m];
+  for (int l = 0; l < HOIST; l++) v[l] = rowp[l < len ? l : 0];
   T acc = (T)0;
   int ip = 0;
   if (p < m) {
@@ -1088,7 +1088,7 @@ __device__ inline void fwd_small_front(const SolveItem& it, const DevSym& d, con
   __syncthreads();
   if (p < m) {
 #pragma unroll
-    for (int l = HOIST; l < FWD_SMALL_K; l++) v[l] = rowp[(size_t)(l < len ? l : 0) * m];
+    for (int l = HOIST; l < FWD_SMALL_K; l++) v[l] = rowp[l < len ? l : 0];
     T acc = (T)0;
 #pragma unroll
     for (int l = 0; l < FWD_SMALL_K; l++) {
@@ -1150,7 +1150,7 @@ __device__ inline void fwd_rows_block(const SolveItem& it, const T* __restrict__
   const int seg = 1 << lg, sl = lane & (seg - 1);
   const int i = (it.blk * 4 + wv) * (64 >> lg) + (lane >> lg);
   const int k = it.k, m = it.m, first = it.first;
-  const T* col = F + it.foff + (size_t)(i < m ? i : 0) * m;
+  const T* col = F + it.poff + (i < m ? prow_off(i, k) : 0);
   const T* f = fp + first;
   const bool own = sl == 0 && i < m;
   T base = (T)0;
@@ -1196,54 +1196,72 @@ __global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__
     fwd_rows_block<T>(it, F, fp, y, u);
     return;
   }
+  // kind 2: nrg = it.pad row groups of rpg = 4 * (64 >> lg) rows each (rows
+  // row0 .. row0 + nrg * rpg), one gather for all of them
   const int lg = it.lg;
+  const int nrg = it.pad > 0 ? it.pad : 1;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int seg = 1 << lg, sl = lane & (seg - 1);
-  const int i = (it.blk * 4 + wv) * (64 >> lg) + (lane >> lg);
+  const int rpg = 4 * (64 >> lg);
+  const int row0 = it.blk * nrg * rpg;
+  const int myrow = wv * (64 >> lg) + (lane >> lg);  // row within a group
   const int k = it.k, m = it.m, first = it.first;
-  const bool own = sl == 0 && i < m;
-  // the first chunk of this lane's part of row i (the seg_dot order: chunks
-  // sl, sl + seg, ..., 16 bytes each), loaded before the gather
-  // (PF chunks; the chunks are added in ascending order whatever the grouping)
+  // the first PF chunks of this lane's part of its row in group 0 (the seg_dot
+  // order: chunks sl, sl + seg, ..., 16 bytes each), loaded before the gather;
+  // the chunks are added in ascending order whatever the grouping
   constexpr int W = 16 / sizeof(T), GC = 4, PF = 3;
   typedef typename V16<T>::v vt;
-  const T* col = F + it.foff + (size_t)(i < m ? i : 0) * m;
-  const int len = (i < m) ? min(i, k) : 0;
-  const int nc = (len + W - 1) / W;
   vt x0[PF];
+  auto prefetch = [&](int rg) {
+    const int i = row0 + rg * rpg + myrow;
+    const T* col = F + it.poff + (i < m ? prow_off(i, k) : 0);
+    const int len = (i < m) ? min(i, k) : 0;
+    const int nc = (len + W - 1) / W;
 #pragma unroll
-  for (int q = 0; q < PF; q++) {
-    const int c = sl + q * seg;
-    x0[q] = ldv(col + (c < nc ? c : 0) * W);
-  }
-  // this row's children entries (contribution rows)
-  T base = (T)0;
-  if (own && i >= k) {
-    const int* gp = d.gptr + it.goff + i;
-    const int e1 = gp[1];
-    for (int e = gp[0]; e < e1; e++) base += u[d.gsrc[e]];
-  }
-  // f in pivot order: the index loads of all (up to four) positions of a
-  // thread issue before the first sum
+    for (int q = 0; q < PF; q++) {
+      const int c = sl + q * seg;
+      x0[q] = ldv(col + (c < nc ? c : 0) * W);
+    }
+  };
+  prefetch(0);
+  // gather: f in pivot order (positions p < k) into fs, and the children's
+  // entries of the item's contribution rows (k <= r < m) into gb; every
+  // thread's index loads issue before its first sum
+  __shared__ T gb[FWD_GROUP_ROWS];
   {
-    int e0[KMAX / 256], e1[KMAX / 256], ip[KMAX / 256];
-    T acc[KMAX / 256];
+    const int rlo = max(row0, k), rhi = min(row0 + nrg * rpg, m);
+    constexpr int NS = KMAX / 256 + FWD_GROUP_ROWS / 256;
+    int e0[NS], e1[NS], dst[NS];
+    T acc[NS];
 #pragma unroll
-    for (int j = 0; j < KMAX / 256; j++) {
-      const int p = threadIdx.x + 256 * j;
-      e0[j] = e1[j] = ip[j] = 0;
+    for (int j = 0; j < NS; j++) {
+      e0[j] = e1[j] = 0;
+      dst[j] = -1;
       acc[j] = (T)0;
-      if (p < k) {
+      int p = -1;
+      if (j < KMAX / 256) {
+        const int q = threadIdx.x + 256 * j;
+        if (q < k) {
+          p = q;
+          acc[j] = bp[first + q];
+          dst[j] = d.ilperm[first + q];
+        }
+      } else {
+        const int q = rlo + threadIdx.x + 256 * (j - KMAX / 256);
+        if (q < rhi) {
+          p = q;
+          dst[j] = KMAX + 4 + (q - row0);  // marks gb[q - row0]
+        }
+      }
+      if (p >= 0) {
         const int* gp = d.gptr + it.goff + p;
         e0[j] = gp[0];
         e1[j] = gp[1];
-        acc[j] = bp[first + p];
-        ip[j] = d.ilperm[first + p];
       }
     }
 #pragma unroll
-    for (int j = 0; j < KMAX / 256; j++) {
-      if (threadIdx.x + 256 * j >= k) break;
+    for (int j = 0; j < NS; j++) {
+      if (dst[j] < 0) continue;
       T a = acc[j];
       int e = e0[j];
       for (; e + 3 < e1[j]; e += 4) {
@@ -1255,44 +1273,78 @@ __global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__
         a += v3;
       }
       for (; e < e1[j]; e++) a += u[d.gsrc[e]];
-      fs[ip[j]] = a;
+      if (dst[j] < KMAX + 4) fs[dst[j]] = a;
+      else gb[dst[j] - (KMAX + 4)] = a;
     }
   }
   __syncthreads();
-  if (own && i < k) base = fs[i];
-  // seg_dot(col, fs, len) with its first chunk from x0 (the same products in
-  // the same order; a chunk past the row adds nothing)
-  T acc = (T)0;
+  for (int rg = 0; rg < nrg; rg++) {
+    const int i = row0 + rg * rpg + myrow;
+    const bool own = sl == 0 && i < m;
+    const T* col = F + it.poff + (i < m ? prow_off(i, k) : 0);
+    const int len = (i < m) ? min(i, k) : 0;
+    const int nc = (len + W - 1) / W;
+    vt xc[PF];
 #pragma unroll
-  for (int q = 0; q < PF; q++) {
-    const int c = sl + q * seg;
-    const vt yv = ldv(fs + (c < nc ? c : 0) * W);
+    for (int q = 0; q < PF; q++) xc[q] = x0[q];
+    if (rg + 1 < nrg) prefetch(rg + 1);  // the next group's first chunks in flight
+    // seg_dot(col, fs, len) with its first PF chunks from xc (the same products
+    // in the same order; a chunk past the row adds nothing)
+    T acc = (T)0;
 #pragma unroll
-    for (int w = 0; w < W; w++) {
-      const T t = fma(x0[q][w], yv[w], acc);
-      acc = (c * W + w < len) ? t : acc;
-    }
-  }
-  for (int q0 = sl + PF * seg; q0 < nc; q0 += GC * seg) {
-    vt x[GC], yv[GC];
-#pragma unroll
-    for (int q = 0; q < GC; q++) {
-      const int c = q0 + q * seg, cc = c < nc ? c : 0;
-      x[q] = ldv(col + cc * W);
-      yv[q] = ldv(fs + cc * W);
-    }
-#pragma unroll
-    for (int q = 0; q < GC; q++)
+    for (int q = 0; q < PF; q++) {
+      const int c = sl + q * seg;
+      const vt yv = ldv(fs + (c < nc ? c : 0) * W);
 #pragma unroll
       for (int w = 0; w < W; w++) {
-        const T t = fma(x[q][w], yv[q][w], acc);
-        acc = ((q0 + q * seg) * W + w < len) ? t : acc;
+        const T t = fma(xc[q][w], yv[w], acc);
+        acc = (c * W + w < len) ? t : acc;
       }
+    }
+    for (int q0 = sl + PF * seg; q0 < nc; q0 += GC * seg) {
+      vt x[GC], yv[GC];
+#pragma unroll
+      for (int q = 0; q < GC; q++) {
+        const int c = q0 + q * seg, cc = c < nc ? c : 0;
+        x[q] = ldv(col + cc * W);
+        yv[q] = ldv(fs + cc * W);
+      }
+#pragma unroll
+      for (int q = 0; q < GC; q++)
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+          const T t = fma(x[q][w], yv[q][w], acc);
+          acc = ((q0 + q * seg) * W + w < len) ? t : acc;
+        }
+    }
+    acc = seg_sum(acc, lg);
+    if (own) {
+      if (i < k) y[first + i] = fs[i] + acc;
+      else u[it.uoff + i - k] = gb[i - row0] - acc;
+    }
   }
-  acc = seg_sum(acc, lg);
-  if (own) {
-    if (i < k) y[first + i] = base + acc;
-    else u[it.uoff + i - k] = base - acc;
+}
+
+// packed solve arrays from the solve form of the fronts: front column i gives
+// forward row i (its rows 0 .. min(i, k), the upper copy) and, for i < k,
+// backward column i (its rows i+1 .. m, the lower copy G); one wave per column,
+// item = (supernode, block of 4 columns)
+template <typename T>
+__global__ __launch_bounds__(256) void k_pack_solve(const int2* __restrict__ items, DevSym d,
+                                                    const T* __restrict__ F, T* __restrict__ PF,
+                                                    T* __restrict__ PB) {
+  const int2 it = items[blockIdx.x];
+  const int s = it.x, i = it.y * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int k = d.sn_k[s], m = d.sn_m[s];
+  if (i >= m) return;
+  const T* colp = F + d.sn_foff[s] + (size_t)i * m;
+  const long long po = d.sn_poff[s];
+  const int len = min(i, k);
+  T* fr = PF + po + prow_off(i, k);
+  for (int t = lane; t < len; t += 64) fr[t] = colp[t];
+  if (i < k) {
+    T* bc = PB + po + pcol_off(i, m);
+    for (int t = lane; t < m - 1 - i; t += 64) bc[t] = colp[i + 1 + t];
   }
 }
 
@@ -1336,7 +1388,7 @@ __global__ __launch_bounds__(256) void k_bwd_cols(const SolveItem* __restrict__ 
   const int lp = d.lperm[first + cq];
   const T* w = wv + it.voff;
   const T zc = w[cq];
-  const T* col = F + it.foff + (size_t)cq * m + cq + 1;
+  const T* col = F + it.poff + pcol_off(cq, m);
   // every lane of a segment ends with the same sum (xor butterfly; the wave sums
   // of a column spanning 2 or 4 waves are combined in order)
   T acc = seg_sum(seg_dot<T>(col, w + cq + 1, ok ? m - c - 1 : 0, sl, seg), lg < 6 ? lg : 6);
@@ -2202,6 +2254,10 @@ void launch_gform(const int4* items, int nitems, const DevSym& d, T* F, hipStrea
   if (nitems > 0) hipLaunchKernelGGL(k_gform<T>, dim3(nitems), dim3(256), 0, st, items, d, F);
 }
 template <typename T>
+void launch_pack_solve(const int2* items, int nitems, const DevSym& d, const T* F, T* PF, T* PB, hipStream_t st) {
+  if (nitems > 0) hipLaunchKernelGGL(k_pack_solve<T>, dim3(nitems), dim3(256), 0, st, items, d, F, PF, PB);
+}
+template <typename T>
 void launch_fwd_level(const SolveItem* items, int nitems, const DevSym& d, const T* F, const T* bp, T* fp,
                       T* y, T* u, hipStream_t st) {
   if (nitems > 0)
@@ -2421,6 +2477,7 @@ void launch_absmax_roots(int nchunks, const long long* ca, const long long* cb, 
   template void launch_panel_xrow<T>(const int2*, int, int, const DevSym&, T*, hipStream_t);     \
   template void launch_zt<T>(const int4*, int, const DevSym&, T*, hipStream_t);                \
   template void launch_gform<T>(const int4*, int, const DevSym&, T*, hipStream_t);             \
+  template void launch_pack_solve<T>(const int2*, int, const DevSym&, const T*, T*, T*, hipStream_t); \
   template void launch_fwd_level<T>(const SolveItem*, int, const DevSym&, const T*, const T*, T*, T*, \
                                     T*, hipStream_t);                                            \
   template void launch_dsolve<T>(int, const T*, const T*, const signed char*, const int*,        \

@@ -153,6 +153,17 @@ struct kkt_solver {
   // numeric buffers (allocated lazily per precision)
   double* d_F64 = nullptr;
   float* d_F32 = nullptr;
+  // packed solve arrays per precision: forward rows, then (pack_elems + 16
+  // later) backward columns
+  double* d_PK64 = nullptr;
+  float* d_PK32 = nullptr;
+  long long pack_elems = 0;
+  int2* d_pack = nullptr;  // packing work items (supernode, block of 4 columns)
+  int pack_cnt = 0;
+  template <typename T>
+  T* pk_fwd(T* PK) const { return PK; }
+  template <typename T>
+  T* pk_bwd(T* PK) const { return PK + pack_elems + 16; }
   double* d_W64 = nullptr;
   float* d_W32 = nullptr;
   // solve scratch, as T: bp, fp, y, xp (n each), u (u_elems + 16), w (v_elems + 16)
@@ -279,15 +290,15 @@ struct kkt_solver {
   size_t numeric_bytes(int p) const {
     const size_t esz = (p == KKT_PREC_FP64) ? 8 : 4;
     return ((size_t)front_elems + 2 * (size_t)n + 4) * esz + (size_t)std::max<long long>(w_elems, 1) * esz +
-           ((size_t)4 * n + (size_t)u_elems + 16 + (size_t)v_elems + 16) * 8;
+           (size_t)2 * (pack_elems + 16) * esz + ((size_t)4 * n + (size_t)u_elems + 16 + (size_t)v_elems + 16) * 8;
   }
   size_t held_bytes() const {
     if (!dev_ok || state < ANALYZED) return 0;
     size_t t = plan_bytes() + pattern_bytes();
     const size_t f64 = ((size_t)front_elems + 2 * (size_t)n + 4 + (size_t)std::max<long long>(w_elems, 1)) * 8;
     const size_t f32 = ((size_t)front_elems + 2 * (size_t)n + 4 + (size_t)std::max<long long>(w_elems, 1)) * 4;
-    if (d_F64) t += f64;
-    if (d_F32) t += f32;
+    if (d_F64) t += f64 + (size_t)2 * (pack_elems + 16) * 8;
+    if (d_F32) t += f32 + (size_t)2 * (pack_elems + 16) * 4;
     t += svec_bytes + msvec_bytes + mvec_bytes;
     return t;
   }
@@ -296,6 +307,10 @@ struct kkt_solver {
     DevOpLock lk(g_capture_mu);
     if (d_F64) (void)hipFree(d_F64);
     if (d_F32) (void)hipFree(d_F32);
+    if (d_PK64) (void)hipFree(d_PK64);
+    if (d_PK32) (void)hipFree(d_PK32);
+    d_PK64 = nullptr;
+    d_PK32 = nullptr;
     if (d_W64) (void)hipFree(d_W64);
     if (d_W32) (void)hipFree(d_W32);
     if (d_svec) (void)hipFree(d_svec);
@@ -516,10 +531,32 @@ struct kkt_solver {
   // per lane) is the shape the autotuner picks per level; the gather items do
   // not depend on it.
   std::vector<int> sv_k, sv_m, sv_first;
-  std::vector<long long> sv_foff, sv_uoff, sv_roff, sv_goff, sv_voff;
+  std::vector<long long> sv_foff, sv_uoff, sv_roff, sv_goff, sv_voff, sv_poff;
   std::vector<std::vector<int>> sv_lv;
   std::vector<int> fr_epl, bc_epl;
   std::vector<int> fr_fuse;  // per level: 1 = one forward launch (kind 2 items), 0 = gather + rows
+  std::vector<int> fr_rg;    // per level: row groups per kind-2 item (sharing one gather)
+  // Row groups per forward row block, per level: the most (up to eight) that
+  // still leave about 1024 row blocks in the level's launch, so every CU keeps
+  // several workgroups while the blocks of a front gather its right-hand side
+  // fewer times.
+  std::vector<int> default_row_groups(int epl) const {
+    const int nl = (int)sv_lv.size();
+    std::vector<int> rg(nl, 1);
+    for (int L = 0; L < nl; L++) {
+      long long blocks = 0;
+      for (int s : sv_lv[L]) {
+        const int k = sv_k[s], m = sv_m[s];
+        if (k <= FWD_SMALL_K && m <= FWD_SMALL_M) continue;
+        int lg = 0;
+        while (lg < 6 && epl * (1 << lg) < k) lg++;
+        const int rf = 4 * (64 >> lg);
+        blocks += (m + rf - 1) / rf;
+      }
+      while (rg[L] < 8 && blocks / (2 * rg[L]) >= 1024) rg[L] *= 2;
+    }
+    return rg;
+  }
   int replace_upload(SolveItem*& dptr, const std::vector<SolveItem>& v) {
     // a pointer no longer in allocs was already freed with them (new analysis)
     if (dptr) {
@@ -543,12 +580,13 @@ struct kkt_solver {
     // row blocks, kind 3: the fs range)
     std::vector<SolveItem> fgv, frv, bcv, ffv, fsv;
     if ((int)fr_fuse.size() != nl) fr_fuse.assign(nl, 1);
+    if ((int)fr_rg.size() != nl) fr_rg.assign(nl, 1);
     auto item = [&](int s, int blk, int lg, int kind) {
       SolveItem it;
       it.s = s; it.blk = blk; it.lg = lg; it.kind = kind;
       it.k = sv_k[s]; it.m = sv_m[s]; it.first = sv_first[s]; it.pad = 0;
       it.foff = sv_foff[s]; it.uoff = sv_uoff[s]; it.roff = sv_roff[s]; it.goff = sv_goff[s];
-      it.voff = sv_voff[s]; it.pad2 = 0;
+      it.voff = sv_voff[s]; it.poff = sv_poff[s];
       return it;
     };
     auto seg_lg = [](int len, int epl, int cap) {
@@ -582,8 +620,17 @@ struct kkt_solver {
           const int lf = seg_lg(k, fr_epl[L], 6), rf = 4 * (64 >> lf);
           for (int rb = 0; rb < (m + rf - 1) / rf; rb++) {
             frv.push_back(item(s, rb, lf, 0));
-            if (fr_fuse[L]) ffv.push_back(item(s, rb, lf, 2));
-            else fsv.push_back(item(s, rb, lf, 3));
+            if (!fr_fuse[L]) fsv.push_back(item(s, rb, lf, 3));
+          }
+          if (fr_fuse[L]) {
+            // one-launch form: row blocks of nrg groups of rf rows share a gather
+            int nrg = std::max(1, fr_rg[L]);
+            while (nrg > 1 && nrg * rf > FWD_GROUP_ROWS) nrg >>= 1;
+            for (int rb = 0; rb < (m + nrg * rf - 1) / (nrg * rf); rb++) {
+              SolveItem it = item(s, rb, lf, 2);
+              it.pad = nrg;
+              ffv.push_back(it);
+            }
           }
         }
         const int lb = seg_lg(m, bc_epl[L], 8), rbw = 256 >> lb;
@@ -903,7 +950,22 @@ struct kkt_solver {
       cptr[r + 1] = (int)ca.size();
     }
     nachunks = (int)ca.size();
+    // packed solve arrays: supernodes in level-major order, k(k-1)/2 + k r
+    // entries each; packing items (supernode, block of 4 front columns)
+    std::vector<long long> poffv(std::max(ns, 1), 0);
+    std::vector<int2> packv;
+    pack_elems = 0;
+    for (int L = 0; L < nl; L++)
+      for (int s : lv[L]) {
+        poffv[s] = pack_elems;
+        const long long k = kk[s], r = mm[s] - kk[s];
+        pack_elems += k * (k - 1) / 2 + k * r;
+        for (int cb = 0; cb < (mm[s] + 3) / 4; cb++) packv.push_back(make_int2(s, cb));
+      }
+    pack_cnt = (int)packv.size();
     int rc = 0;
+    long long* d_poffp = nullptr;
+    if ((rc = upload(d_poffp, poffv)) || (rc = upload(d_pack, packv))) return rc;
     int *d_first, *d_k, *d_m, *d_chp, *d_chl, *d_rel, *d_rows, *d_pptr, *d_ppos, *d_wpos, *d_sroot;
     std::vector<double> zd(std::max(std::max(nachunks, nroots), 1), 0.0);
     std::vector<int> zi(4 * std::max(nroots, 1), 0);
@@ -924,10 +986,11 @@ struct kkt_solver {
     // solve work items, default shapes (sixteen elements per lane, one forward
     // launch per level)
     sv_k = kk; sv_m = mm; sv_first = first; sv_foff = foff; sv_uoff = uoff; sv_roff = roff;
-    sv_goff = goff; sv_voff = voff; sv_lv = lv;
+    sv_goff = goff; sv_voff = voff; sv_lv = lv; sv_poff = poffv;
     fr_epl.assign(nl, 16);
     bc_epl.assign(nl, 16);
     fr_fuse.assign(nl, 1);
+    fr_rg = default_row_groups(16);
     if ((rc = build_solve_items())) return rc;
     std::vector<int> lp0(n);
     for (int s = 0; s < ns; s++)
@@ -943,6 +1006,7 @@ struct kkt_solver {
     ds.lperm = d_lperm; ds.ilperm = d_ilperm; ds.piv = d_piv;
     ds.sn_goff = d_goff; ds.gptr = d_gptr; ds.gsrc = d_gsrc;
     ds.sn_root = d_sroot;
+    ds.sn_poff = d_poffp;
     // host copies of what was uploaded, for the structure-parity check
     h_first = first; h_k = kk; h_m = mm; h_foff = foff; h_roff = roff; h_chp = chp; h_chl = chl;
     h_rel = rel;
@@ -972,12 +1036,14 @@ struct kkt_solver {
       // fronts, then the two D^-1 vectors (E0, E1) of n entries each, then 4 spare
       // entries (the solve kernels' 16-byte chunk loads may run 3 past a column)
       if (hipMalloc(&d_F64, (front_elems + 2LL * n + 4) * sizeof(double)) != hipSuccess ||
-          hipMalloc(&d_W64, std::max<long long>(w_elems, 1) * sizeof(double)) != hipSuccess)
+          hipMalloc(&d_W64, std::max<long long>(w_elems, 1) * sizeof(double)) != hipSuccess ||
+          hipMalloc(&d_PK64, 2 * (pack_elems + 16) * sizeof(double)) != hipSuccess)
         return fail(KKT_ERR_ALLOC, "hipMalloc failed (FP64 fronts)");
     }
     if (p == KKT_PREC_FP32_REFINE && !d_F32) {
       if (hipMalloc(&d_F32, (front_elems + 2LL * n + 4) * sizeof(float)) != hipSuccess ||
-          hipMalloc(&d_W32, std::max<long long>(w_elems, 1) * sizeof(float)) != hipSuccess)
+          hipMalloc(&d_W32, std::max<long long>(w_elems, 1) * sizeof(float)) != hipSuccess ||
+          hipMalloc(&d_PK32, 2 * (pack_elems + 16) * sizeof(float)) != hipSuccess)
         return fail(KKT_ERR_ALLOC, "hipMalloc failed (FP32 fronts)");
     }
     size_t need = ((size_t)4 * n + (size_t)u_elems + 16 + (size_t)v_elems + 16) * std::max(esz, sizeof(double));
@@ -992,7 +1058,7 @@ struct kkt_solver {
 
   // ------------------------------------------------------------------
   template <typename T>
-  void run_factor(T* F, T* W, double pert) {
+  void run_factor(T* F, T* W, T* PK, double pert) {
     tbeg(TC_VEC);
     launch_absmax_roots(nachunks, d_achunk_a, d_achunk_b, d_vals, d_apart, nroots, d_acptr, d_amax, st);
     tend();
@@ -1035,6 +1101,8 @@ struct kkt_solver {
     tbeg(TC_TRSM);
     launch_zt<T>(d_zt, zt_all.cnt, ds, F, st);
     launch_gform<T>(d_gf, gf_all.cnt, ds, F, st);
+    // the triangular sweeps' operands, one contiguous range per tree level
+    launch_pack_solve<T>(d_pack, pack_cnt, ds, F, pk_fwd(PK), pk_bwd(PK), st);
     tend();
   }
 
@@ -1070,8 +1138,8 @@ struct kkt_solver {
     double t0 = g_timing ? now_ms() : 0;
     auto body = [&]() {
       launch_expand_vals(nnzf, d_map_f, d_vals, d_valsf, st);
-      if (p == KKT_PREC_FP64) run_factor<double>(d_F64, d_W64, 1e-8);
-      else run_factor<float>(d_F32, d_W32, 1e-6);
+      if (p == KKT_PREC_FP64) run_factor<double>(d_F64, d_W64, d_PK64, 1e-8);
+      else run_factor<float>(d_F32, d_W32, d_PK32, 1e-6);
     };
     // the factor's launch sequence is fixed by the analysis: one graph per precision,
     // unless factor kernels are being timed with events
@@ -1103,7 +1171,7 @@ struct kkt_solver {
 
   // ------------------------------------------------------------------
   template <typename T>
-  void run_tri_solve(const T* F, const double* d_b, double* d_x, void* svec = nullptr) {
+  void run_tri_solve(const T* F, const T* PK, const double* d_b, double* d_x, void* svec = nullptr) {
     T* sv = (T*)(svec ? svec : d_svec);
     T *bp = sv, *fp = sv + n, *y = sv + 2 * (size_t)n, *xp = sv + 3 * (size_t)n,
       *u = sv + 4 * (size_t)n, *wv = u + u_elems + 16;
@@ -1119,11 +1187,11 @@ struct kkt_solver {
       if (!ff_l[L].cnt) continue;
       if (lev_timing) (void)hipEventRecord(lev_ev[4 * L], st);
       tbeg(TC_SLEVEL);
-      launch_fwd_level<T>(d_ff + ff_l[L].off, ff_l[L].cnt, ds, F, bp, fp, y, u, st);
+      launch_fwd_level<T>(d_ff + ff_l[L].off, ff_l[L].cnt, ds, pk_fwd(PK), bp, fp, y, u, st);
       tend();
       if (fs_l[L].cnt) {
         tbeg(TC_SLEVEL);
-        launch_fwd_level<T>(d_ff + fs_l[L].off, fs_l[L].cnt, ds, F, bp, fp, y, u, st);
+        launch_fwd_level<T>(d_ff + fs_l[L].off, fs_l[L].cnt, ds, pk_fwd(PK), bp, fp, y, u, st);
         tend();
       }
       if (lev_timing) (void)hipEventRecord(lev_ev[4 * L + 1], st);
@@ -1135,7 +1203,7 @@ struct kkt_solver {
       if (!bc_l[L].cnt) continue;
       if (lev_timing) (void)hipEventRecord(lev_ev[4 * L + 2], st);
       tbeg(TC_SGEMV);
-      launch_bwd_cols<T>(d_bc + bc_l[L].off, bc_l[L].cnt, ds, F, wv, xp, st);
+      launch_bwd_cols<T>(d_bc + bc_l[L].off, bc_l[L].cnt, ds, pk_bwd(PK), wv, xp, st);
       tend();
       if (lev_timing) (void)hipEventRecord(lev_ev[4 * L + 3], st);
     }
@@ -1369,14 +1437,18 @@ struct kkt_solver {
     int rc;
     // per-level pick from eager per-launch timing: the lane shape of both sweeps
     // and the one- or two-launch forward form
+    // forms per lane shape: one launch with 1, 2, 4 or 8 row groups per block,
+    // or two launches
     static const int cand[] = {4, 8, 16, 32};
+    static const int rgs[] = {1, 2, 4, 8, 0};  // 0: the two-launch form
     std::vector<float> bfr(nl, 1e30f), bbc(nl, 1e30f);
-    std::vector<int> cfr(nl, 16), cbc(nl, 16), cfu(nl, 1);
-    for (int pass = 0; pass < 8; pass++) {
-      const int c = cand[pass >> 1], fu = (pass & 1) ? 0 : 1;
+    std::vector<int> cfr(nl, 16), cbc(nl, 16), cfu(nl, 1), crg(nl, 1);
+    for (int pass = 0; pass < 20; pass++) {
+      const int c = cand[pass / 5], rgp = rgs[pass % 5], fu = rgp ? 1 : 0;
       fr_epl.assign(nl, c);
       bc_epl.assign(nl, c);
       fr_fuse.assign(nl, fu);
+      fr_rg.assign(nl, rgp ? rgp : 1);
       if ((rc = build_solve_items())) return rc;
       tri_solve_eager(b, x);
       std::vector<float> tfr(nl, 1e30f), tbc(nl, 1e30f);
@@ -1394,26 +1466,28 @@ struct kkt_solver {
         }
       }
       for (int L = 0; L < nl; L++) {
-        if (tfr[L] < bfr[L]) { bfr[L] = tfr[L]; cfr[L] = c; cfu[L] = fu; }
-        if (fu && tbc[L] < bbc[L]) { bbc[L] = tbc[L]; cbc[L] = c; }
+        if (tfr[L] < bfr[L]) { bfr[L] = tfr[L]; cfr[L] = c; cfu[L] = fu; crg[L] = rgp ? rgp : 1; }
+        if (rgp == 1 && tbc[L] < bbc[L]) { bbc[L] = tbc[L]; cbc[L] = c; }
       }
     }
     // whole-solve timing of every candidate: the nine uniform shape pairs with
-    // one forward launch per level, and the per-level picks
-    struct Cand { std::vector<int> f, b, fu; };
+    // one forward launch per level (default row groups), and the per-level picks
+    struct Cand { std::vector<int> f, b, fu, rg; };
     std::vector<Cand> cands;
     const std::vector<int> all1(nl, 1);
-    cands.push_back({std::vector<int>(nl, 16), std::vector<int>(nl, 16), all1});
+    cands.push_back({std::vector<int>(nl, 16), std::vector<int>(nl, 16), all1, default_row_groups(16)});
     for (int f : {8, 16, 32})
       for (int g : {8, 16, 32})
-        if (f != 16 || g != 16) cands.push_back({std::vector<int>(nl, f), std::vector<int>(nl, g), all1});
-    cands.push_back({cfr, cbc, cfu});
+        if (f != 16 || g != 16)
+          cands.push_back({std::vector<int>(nl, f), std::vector<int>(nl, g), all1, default_row_groups(f)});
+    cands.push_back({cfr, cbc, cfu, crg});
     float best = 1e30f;
     size_t bi = 0;
     for (size_t q = 0; q < cands.size(); q++) {
       fr_epl = cands[q].f;
       bc_epl = cands[q].b;
       fr_fuse = cands[q].fu;
+      fr_rg = cands[q].rg;
       if ((rc = build_solve_items())) return rc;
       drop_graphs();
       float ms;
@@ -1426,6 +1500,7 @@ struct kkt_solver {
     fr_epl = cands[bi].f;
     bc_epl = cands[bi].b;
     fr_fuse = cands[bi].fu;
+    fr_rg = cands[bi].rg;
     if (g_timing) {
       fprintf(stderr, "[kkt-timing] autotune(%s): candidate %zu, two-launch forward levels:",
               fprec == KKT_PREC_FP64 ? "fp64" : "fp32", bi);
@@ -1439,8 +1514,8 @@ struct kkt_solver {
     return KKT_OK;
   }
   void tri_solve_eager(const double* d_b, double* d_x) {
-    if (fprec == KKT_PREC_FP64) run_tri_solve<double>(d_F64, d_b, d_x);
-    else run_tri_solve<float>(d_F32, d_b, d_x);
+    if (fprec == KKT_PREC_FP64) run_tri_solve<double>(d_F64, d_PK64, d_b, d_x);
+    else run_tri_solve<float>(d_F32, d_PK32, d_b, d_x);
   }
   // The launch sequence of a solve is fixed by the analysis, so it is captured
   // once into a HIP graph (reading vec(12), writing vec(13)) and replayed.
@@ -1592,8 +1667,8 @@ struct kkt_solver {
     p += align256(1024 * sizeof(double));
     double* scal = (double*)p;
     auto tri = [&](const double* b, double* x) {
-      if (fprec == KKT_PREC_FP64) run_tri_solve<double>(d_F64, b, x, svec);
-      else run_tri_solve<float>(d_F32, b, x, svec);
+      if (fprec == KKT_PREC_FP64) run_tri_solve<double>(d_F64, d_PK64, b, x, svec);
+      else run_tri_solve<float>(d_F32, d_PK32, b, x, svec);
     };
     auto dot = [&](const double* a, const double* b, double& out) -> int {
       launch_dot(n, a, b, part, scal, st);
@@ -1675,23 +1750,23 @@ struct kkt_solver {
         launch_spmv(n, d_rowptr_f, d_col_f, d_valsf, x, b, vec(0), 1, st);
         return;
       }
-      auto sweep = [&](auto* F) {
-        typedef typename std::remove_pointer<decltype(F)>::type T;
+      auto sweep = [&](auto* PK) {
+        typedef typename std::remove_pointer<decltype(PK)>::type T;
         T* svT = (T*)sv;
         T *bp = svT, *fp = svT + n, *y = svT + 2 * (size_t)n, *xp = svT + 3 * (size_t)n,
           *u = svT + 4 * (size_t)n, *wv = u + u_elems + 16;
         for (int L = 0; L < nl; L++) {
           if (what == 0) {
-            if (ff_l[L].cnt) launch_fwd_level<T>(d_ff + ff_l[L].off, ff_l[L].cnt, ds, F, bp, fp, y, u, st);
-            if (fs_l[L].cnt) launch_fwd_level<T>(d_ff + fs_l[L].off, fs_l[L].cnt, ds, F, bp, fp, y, u, st);
+            if (ff_l[L].cnt) launch_fwd_level<T>(d_ff + ff_l[L].off, ff_l[L].cnt, ds, pk_fwd(PK), bp, fp, y, u, st);
+            if (fs_l[L].cnt) launch_fwd_level<T>(d_ff + fs_l[L].off, fs_l[L].cnt, ds, pk_fwd(PK), bp, fp, y, u, st);
           } else {
             const int Lb = nl - 1 - L;
-            if (bc_l[Lb].cnt) launch_bwd_cols<T>(d_bc + bc_l[Lb].off, bc_l[Lb].cnt, ds, F, wv, xp, st);
+            if (bc_l[Lb].cnt) launch_bwd_cols<T>(d_bc + bc_l[Lb].off, bc_l[Lb].cnt, ds, pk_bwd(PK), wv, xp, st);
           }
         }
       };
-      if (f64) sweep(d_F64);
-      else sweep(d_F32);
+      if (f64) sweep(d_PK64);
+      else sweep(d_PK32);
     };
     hipGraphExec_t ex = nullptr;
     if (!capture(ex, body)) return fail(KKT_ERR_DEVICE, "graph capture failed (kernel timing)");
@@ -2857,6 +2932,31 @@ int kkt_time_kernels(kkt_solver* h, int what, int reps, double* ms_per_launch, i
   int rc = h->time_kernels(what, reps, ms_per_launch, launches_per_rep);
   if (!rc) h->ok();
   return rc;
+}
+
+int kkt_get_forward_row_groups(kkt_solver* h, int* groups) {
+  CHECK_H(h);
+  if (h->state < kkt_solver::ANALYZED) return h->fail(KKT_ERR_STATE, "no analysis");
+  if (!groups) return h->fail(KKT_ERR_ARGUMENT, "null groups");
+  for (size_t L = 0; L < h->fr_rg.size(); L++) groups[L] = h->fr_rg[L];
+  h->ok();
+  return KKT_OK;
+}
+
+int kkt_set_forward_row_groups(kkt_solver* h, const int* groups) {
+  CHECK_H(h);
+  if (h->state < kkt_solver::ANALYZED) return h->fail(KKT_ERR_STATE, "no analysis");
+  if (!groups) return h->fail(KKT_ERR_ARGUMENT, "null groups");
+  const int nl = (int)h->fr_rg.size();
+  for (int L = 0; L < nl; L++)
+    if (groups[L] != 1 && groups[L] != 2 && groups[L] != 4 && groups[L] != 8)
+      return h->fail(KKT_ERR_ARGUMENT, "row groups must be 1, 2, 4 or 8");
+  for (int L = 0; L < nl; L++) h->fr_rg[L] = groups[L];
+  int rc;
+  if ((rc = h->build_solve_items())) return rc;
+  h->drop_graphs();
+  h->ok();
+  return KKT_OK;
 }
 
 int kkt_timing_get(kkt_solver* h, double* ms, long long* launches, double* update_flops) {

@@ -462,6 +462,20 @@ class MI355XSolver:
             raise ValueError("one form per tree level expected")
         self._check(self._L.kkt_set_forward_forms(self._h, ptr(f, ctypes.c_int)), "setForwardForms")
 
+    def getForwardRowGroups(self):
+        """Row groups per forward row block of every tree level (1, 2, 4 or 8)."""
+        nl = self.info()["nlevels"]
+        g = np.zeros(max(nl, 1), np.int32)
+        self._check(self._L.kkt_get_forward_row_groups(self._h, ptr(g, ctypes.c_int)), "getForwardRowGroups")
+        return g[:nl]
+
+    def setForwardRowGroups(self, groups):
+        """Sets the row groups of every tree level; the solution is bitwise the same."""
+        g = np.ascontiguousarray(groups, np.int32)
+        if len(g) != self.info()["nlevels"]:
+            raise ValueError("one value per tree level expected")
+        self._check(self._L.kkt_set_forward_row_groups(self._h, ptr(g, ctypes.c_int)), "setForwardRowGroups")
+
     def factorDeviceValues(self):
         self._need_analyze()
         self._check(self._L.kkt_factor_device_values(self._h), "factorDeviceValues")

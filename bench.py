@@ -68,7 +68,7 @@ def bwd_sweep_bytes(L, h, w):
     return float(np.sum((k * (k - 1) / 2 + k * r + 2 * k + 2 * r) * w + (r + 2 * k) * 4))
 
 
-def fwd_sweep_bytes(L, h, w, fwd_epl, forms):
+def fwd_sweep_bytes(L, h, w, fwd_epl, forms, groups):
     """Algorithmic HBM bytes of one forward sweep (all k_fwd_level launches of a
     triangular solve), per supernode (k pivots, r rows below): the upper copy of
     G once (k(k-1)/2 entries of X, unit diagonal implicit, and k r of Z), the
@@ -78,7 +78,8 @@ def fwd_sweep_bytes(L, h, w, fwd_epl, forms):
     source indices, 4 bytes each); w = bytes per vector/factor entry.
     Returned separately: the redundant bytes of the one-launch form, where every
     row block of a large front gathers the front's right-hand side itself (m
-    entries and the map once more per extra row block)."""
+    entries and the map once more per extra row block; a block covers `groups`
+    row groups of its level)."""
     k, r, lev, inf = _structure(L, h)
     m = k + r
     kf, rf = k.astype(np.float64), r.astype(np.float64)
@@ -93,7 +94,10 @@ def fwd_sweep_bytes(L, h, w, fwd_epl, forms):
         lg = 0
         while lg < 6 and fwd_epl[L_] * (1 << lg) < k[s_]:
             lg += 1
-        rows_per_block = 4 * (64 >> lg)
+        rg = int(groups[L_])
+        while rg > 1 and rg * 4 * (64 >> lg) > 1024:
+            rg //= 2
+        rows_per_block = 4 * (64 >> lg) * rg
         nblk = (m[s_] + rows_per_block - 1) // rows_per_block
         red += (nblk - 1) * (m[s_] * w + (m[s_] + 1 + r[s_]) * 4.0)
     return alg, red
@@ -310,9 +314,11 @@ def main():
         shp_n = np.zeros(inf0.nlevels, np.int32)
         L.kkt_get_solve_shapes(h, nptr(shp_f, ctypes.c_int), nptr(shp_b, ctypes.c_int))
         L.kkt_get_forward_forms(h, nptr(shp_n, ctypes.c_int))
+        shp_g = np.zeros(inf0.nlevels, np.int32)
+        L.kkt_get_forward_row_groups(h, nptr(shp_g, ctypes.c_int))
         log("%s: autotuned solve shapes (elements per lane, levels 0..%d) fwd %s bwd %s, forward launches "
-            "per level %s, in %.2f s" % (name, inf0.nlevels - 1, shp_f.tolist(), shp_b.tolist(), shp_n.tolist(),
-                                         t_tune))
+            "per level %s, row groups %s, in %.2f s" % (name, inf0.nlevels - 1, shp_f.tolist(), shp_b.tolist(),
+                                                       shp_n.tolist(), shp_g.tolist(), t_tune))
         # the timed region: no kernel timing (factors and solves replay their graphs)
         L.kkt_timing_reset(h, 0)
         run_sequence(L, h, vals_d, b_d, x_d, 1, len(V))  # graphs captured for the re-planned solve
@@ -327,7 +333,7 @@ def main():
         x_t = torch.zeros(n, dtype=torch.float64, device=dev)  # not one of the verified solutions
         kp = kernel_pass(L, h, vals_d[0].data_ptr(), b_d[0].data_ptr(), x_t.data_ptr())
         w = 4 if prec == KKT_PREC_FP32_REFINE else 8
-        fwd_bytes, fwd_red = fwd_sweep_bytes(L, h, w, shp_f, shp_n)
+        fwd_bytes, fwd_red = fwd_sweep_bytes(L, h, w, shp_f, shp_n, shp_g)
         bwd_bytes = bwd_sweep_bytes(L, h, w)
         # verify every step's solution (host, after timing)
         import scipy.sparse as sp
@@ -344,7 +350,7 @@ def main():
             "kp": kp, "info": inf.as_dict(), "fwd_sweep_bytes": fwd_bytes, "fwd_redundant_bytes": fwd_red,
             "bwd_sweep_bytes": bwd_bytes, "nlevels": inf.nlevels,
             "solve_shapes": {"fwd": shp_f.tolist(), "bwd": shp_b.tolist(), "fwd_launches": shp_n.tolist(),
-                             "tune_s": t_tune},
+                             "fwd_row_groups": shp_g.tolist(), "tune_s": t_tune},
             "x": [x_d[i].cpu().numpy() for i in range(min(args.steps, len(V)))],
         }
         log("%s: %.2f solves/s, p50 %.2f ms, p99 %.2f ms, kinds %s, iters %s, max rel resid %.2e" % (

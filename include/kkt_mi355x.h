@@ -288,6 +288,14 @@ int kkt_get_forward_forms(kkt_solver* h, int* launches);
 /* Sets the forward form per tree level (1 or 2, as above), e.g. to pin a tuned
  * plan; the solution is bitwise the same either way. */
 int kkt_set_forward_forms(kkt_solver* h, const int* launches);
+/* Row groups per forward row block of the one-launch form, per tree level
+ * (kkt_info.nlevels entries; 1, 2, 4 or 8): the row blocks of a large front
+ * each gather the front's right-hand side once for that many groups of rows.
+ * The default leaves about 1024 blocks per level; the autotuner times 1-8 per
+ * level.  The solution is bitwise the same for any choice (no counterpart in
+ * the reference). */
+int kkt_get_forward_row_groups(kkt_solver* h, int* groups);
+int kkt_set_forward_row_groups(kkt_solver* h, const int* groups);
 int kkt_timing_get(kkt_solver* h, double* ms, long long* launches, double* update_flops);
 /* Kernel durations without per-launch events (no counterpart in the
  * reference): one forward sweep (what 0, the k_fwd_level launches of a

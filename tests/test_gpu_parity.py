@@ -160,11 +160,14 @@ def test_forward_forms_are_bitwise_equal(gpu, prec):
         xs = []
         rng = np.random.default_rng(5)
         plans = [[1] * nl, [2] * nl, list(rng.integers(1, 3, nl)), [1] * nl]
+        groups = [[1] * nl, [8] * nl, list(2 ** rng.integers(0, 4, nl)), [2] * nl]
         for graphs in (1, 0):
             s._L.kkt_set_graphs(s._h, graphs)
-            for forms in plans:
+            for forms, rg in zip(plans, groups):
                 s.setForwardForms(forms)
+                s.setForwardRowGroups(rg)
                 assert list(s.getForwardForms()) == [int(f) for f in forms]
+                assert list(s.getForwardRowGroups()) == [int(g) for g in rg]
                 x = np.zeros(body.n)
                 s.solve(x, b)
                 xs.append(x)
@@ -176,6 +179,8 @@ def test_forward_forms_are_bitwise_equal(gpu, prec):
         assert rel_err(xs[0], ref.solve(b)) <= 1e-8
         with pytest.raises(Exception):
             s.setForwardForms([3] * nl)
+        with pytest.raises(Exception):
+            s.setForwardRowGroups([3] * nl)
     finally:
         s.dispose()
 
