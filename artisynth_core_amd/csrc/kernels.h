@@ -73,6 +73,11 @@ struct alignas(16) SolveItem {
   long long goff;  // offset of the front positions in the gather map
   long long voff;  // offset of the supernode's backward vector entries
   long long poff;  // offset of the supernode's block in the packed solve arrays
+  long long soff;  // offset of the supernode's update-slot block (nch x m)
+  long long sdst;  // where its contribution rows go: its column of the parent's
+                   // slot block (the parent's soff + j m, j = its child index)
+  int nch;         // number of children
+  int pad1, pad2, pad3;
 };
 
 // Packed solve arrays (written after each factor from the solve form of the
@@ -81,6 +86,12 @@ struct alignas(16) SolveItem {
 //   forward  row i of supernode s: G(i, 0 .. min(i, k)) -- X(i, :) without the
 //            unit diagonal for i < k, Z(i - k, :) below -- at poff + prow_off(i, k);
 //   backward column c < k: G(c+1 .. m, c) at poff + pcol_off(c, m).
+// After both, the forward update slots: supernode s with nch children has an
+// nch x m block at soff, child j's update entry for parent position p at
+// soff + j m + p (the child writes it, the parent reads the column of p); an
+// entry no child row maps to stays zero (the block is zeroed when allocated and
+// those entries are never written), so the parent adds all nch entries of a
+// position in ascending child order with no index loads.
 __host__ __device__ inline long long prow_off(int i, int k) {
   return i < k ? (long long)i * (i - 1) / 2 : (long long)k * (k - 1) / 2 + (long long)(i - k) * k;
 }
@@ -124,10 +135,10 @@ void launch_gform(const int4* items, int nitems, const DevSym& d, T* F, hipStrea
 template <typename T>
 void launch_pack_solve(const int2* items, int nitems, const DevSym& d, const T* F, T* PF, T* PB, hipStream_t st);
 // level-scheduled solves (vectors in the front precision T) on the packed
-// arrays: PF forward rows, PB backward columns
+// arrays: PF forward rows, PB backward columns, SL the forward update slots
 template <typename T>
 void launch_fwd_level(const SolveItem* items, int nitems, const DevSym& d, const T* PF, const T* bp,
-                      T* fp, T* y, T* u, hipStream_t st);
+                      T* fp, T* y, T* SL, hipStream_t st);
 template <typename T>
 void launch_dsolve(int n, const T* E0, const T* E1, const signed char* piv, const int* wpos,
                    const T* y, T* wv, hipStream_t st);

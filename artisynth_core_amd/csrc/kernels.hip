@@ -1038,11 +1038,30 @@ template <>
 struct VI16<double> {
   typedef int v __attribute__((ext_vector_type(2)));
 };
+// a + the children's update entries at one front position: sl points at the
+// position's entry of child 0 in the supernode's slot block, child j's is
+// nch_m = m entries further on each; up to eight loads issue before the first
+// add (nch is uniform over the workgroup), added in ascending child order
+template <typename T>
+__device__ inline T slot_sum(const T* __restrict__ sl, int nch, int m, T a) {
+  for (int j0 = 0; j0 < nch; j0 += 8) {
+    T v[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      if (j0 + q < nch) v[q] = sl[(size_t)(j0 + q) * m];
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      if (j0 + q < nch) a += v[q];
+  }
+  return a;
+}
+
 // forward, a whole small front (k <= FWD_SMALL_K, m <= FWD_SMALL_M) in one
 // workgroup: one thread per position p gathers f_p = b_p + the children's update
 // entries at p (summed in ascending order) into LDS in pivot order, then one
-// thread per row: y_i = f_i + G(i,:) f (i < k), u_{i-k} = f_i - G(i,:) f (i >= k),
-// G(i,:) read from the row's packed forward entries (contiguous per thread)
+// thread per row: y_i = f_i + G(i,:) f (i < k), and for i >= k the update entry
+// f_i - G(i,:) f goes to the supernode's slot in its parent's block, G(i,:) read
+// from the row's packed forward entries (contiguous per thread)
 static_assert(FWD_SMALL_M == 256, "one thread per row of a small front");
 template <typename T>
 __device__ inline void fwd_small_front(const SolveItem& it, const DevSym& d, const T* __restrict__ F,
@@ -1061,26 +1080,16 @@ __device__ inline void fwd_small_front(const SolveItem& it, const DevSym& d, con
   T v[FWD_SMALL_K];
 #pragma unroll
   for (int l = 0; l < HOIST; l++) v[l] = rowp[l < len ? l : 0];
+  // destination of a contribution row's update entry in the parent's block
+  const int dr = (p >= k && p < m) ? d.rel[it.roff + p - k] : 0;
   T acc = (T)0;
   int ip = 0;
   if (p < m) {
-    const int* gp = d.gptr + it.goff + p;
-    const int e0 = gp[0], e1 = gp[1];
     if (p < k) {
       acc = bp[first + p];
       ip = d.ilperm[first + p];
     }
-    // children in ascending order; the index loads of four entries issue together
-    int e = e0;
-    for (; e + 3 < e1; e += 4) {
-      const int i0 = d.gsrc[e], i1 = d.gsrc[e + 1], i2 = d.gsrc[e + 2], i3 = d.gsrc[e + 3];
-      const T v0 = u[i0], v1 = u[i1], v2 = u[i2], v3 = u[i3];
-      acc += v0;
-      acc += v1;
-      acc += v2;
-      acc += v3;
-    }
-    for (; e < e1; e++) acc += u[d.gsrc[e]];
+    acc = slot_sum(u + it.soff + p, it.nch, m, acc);
   }
   g[threadIdx.x] = acc;
   __syncthreads();
@@ -1096,7 +1105,7 @@ __device__ inline void fwd_small_front(const SolveItem& it, const DevSym& d, con
       acc = (l < len) ? t : acc;
     }
     if (p < k) y[first + p] = fs[p] + acc;
-    else u[it.uoff + p - k] = g[threadIdx.x] - acc;
+    else u[it.sdst + dr] = g[threadIdx.x] - acc;
   }
 }
 
@@ -1109,42 +1118,33 @@ __device__ inline T seg_sum(T v, int lg) {
 
 // forward gather block of a large front (two-launch form, first launch): one
 // thread per position p, f_p = b_p + the children's update entries at p (summed
-// in ascending order); fp = f in pivot order (p < k), u_s = the sums on the
-// contribution rows (p >= k)
+// in ascending order); fp = f in pivot order (p < k); on a contribution row
+// (p >= k) the sum goes to the row's slot in the parent's block, where the rows
+// launch reads it and leaves the row's update entry
 template <typename T>
 __device__ inline void fwd_gather_block(const SolveItem& it, const DevSym& d, const T* __restrict__ bp,
                                         T* __restrict__ fp, T* __restrict__ u) {
   const int k = it.k, m = it.m, first = it.first;
   const int p = it.blk * 256 + threadIdx.x;
   if (p >= m) return;
+  const int dr = p >= k ? d.rel[it.roff + p - k] : 0;
   T acc = (T)0;
   int ip = 0;
-  const int* gp = d.gptr + it.goff + p;
-  const int e0 = gp[0], e1 = gp[1];
   if (p < k) {
     acc = bp[first + p];
     ip = d.ilperm[first + p];
   }
-  int e = e0;
-  for (; e + 3 < e1; e += 4) {
-    const int i0 = d.gsrc[e], i1 = d.gsrc[e + 1], i2 = d.gsrc[e + 2], i3 = d.gsrc[e + 3];
-    const T v0 = u[i0], v1 = u[i1], v2 = u[i2], v3 = u[i3];
-    acc += v0;
-    acc += v1;
-    acc += v2;
-    acc += v3;
-  }
-  for (; e < e1; e++) acc += u[d.gsrc[e]];
+  acc = slot_sum(u + it.soff + p, it.nch, m, acc);
   if (p < k) fp[first + ip] = acc;
-  else u[it.uoff + p - k] = acc;
+  else u[it.sdst + dr] = acc;
 }
 
 // forward rows of a large front (two-launch form, second launch): products from
 // the upper copy (column i, rows l < min(i, k), contiguous), y_i = f_i + G(i,:) f
 // (i < k) or u_{i-k} -= G(i,:) f (i >= k), f = fp from the gather launch
 template <typename T>
-__device__ inline void fwd_rows_block(const SolveItem& it, const T* __restrict__ F, const T* __restrict__ fp,
-                                      T* __restrict__ y, T* __restrict__ u) {
+__device__ inline void fwd_rows_block(const SolveItem& it, const DevSym& d, const T* __restrict__ F,
+                                      const T* __restrict__ fp, T* __restrict__ y, T* __restrict__ u) {
   const int lg = it.lg;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int seg = 1 << lg, sl = lane & (seg - 1);
@@ -1154,12 +1154,14 @@ __device__ inline void fwd_rows_block(const SolveItem& it, const T* __restrict__
   const T* f = fp + first;
   const bool own = sl == 0 && i < m;
   T base = (T)0;
-  if (own) base = (i < k) ? f[i] : u[it.uoff + i - k];
+  T* dst = nullptr;  // the contribution row's slot in the parent's block
+  if (own && i >= k) dst = u + it.sdst + d.rel[it.roff + i - k];
+  if (own) base = (i < k) ? f[i] : *dst;
   const int len = (i < m) ? min(i, k) : 0;
   const T acc = seg_sum(seg_dot<T>(col, f, len, sl, seg), lg);
   if (own) {
     if (i < k) y[first + i] = base + acc;
-    else u[it.uoff + i - k] = base - acc;
+    else *dst = base - acc;
   }
 }
 
@@ -1169,14 +1171,16 @@ __device__ inline void fwd_rows_block(const SolveItem& it, const T* __restrict__
 //   3  launch its row blocks (fwd_rows_block);
 //   2  a row block of a large front that gathers for itself (one launch per
 //      level): products from the upper copy as in kind 3,
-//        y_i = f_i + G(i,:) f (i < k),   u_{i-k} = g_i - G(i,:) f (i >= k),
+//        y_i = f_i + G(i,:) f (i < k),   u_i = g_i - G(i,:) f (i >= k),
 //      with every such item gathering the front's pivot part f = b + the
 //      children's update entries into LDS in pivot order, and g_i (the children's
 //      entries on its own contribution rows) for its rows.
-// Row blocks: a segment of 2^lg lanes per row, 64 >> lg rows per wave (item =
-// (s, block of 4 * (64 >> lg) rows, lg)).  Every sum adds the children's entries
-// in ascending order and every row product runs in the seg_dot order, so both
-// forms give bitwise the same y and u.
+// The update entry u_i of a contribution row goes to the supernode's slot for
+// that row in its parent's slot block.  Row blocks: a segment of 2^lg lanes per
+// row, 64 >> lg rows per wave (item = (s, block of 4 * (64 >> lg) rows, lg)).
+// Every sum adds the children's entries in ascending order and every row
+// product runs in the seg_dot order, so both forms give bitwise the same y and
+// update entries.
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__ items, DevSym d,
                                                    const T* __restrict__ F,
@@ -1193,7 +1197,7 @@ __global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__
     return;
   }
   if (it.kind == 3) {
-    fwd_rows_block<T>(it, F, fp, y, u);
+    fwd_rows_block<T>(it, d, F, fp, y, u);
     return;
   }
   // kind 2: nrg = it.pad row groups of rpg = 4 * (64 >> lg) rows each (rows
@@ -1225,54 +1229,39 @@ __global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__
   };
   prefetch(0);
   // gather: f in pivot order (positions p < k) into fs, and the children's
-  // entries of the item's contribution rows (k <= r < m) into gb; every
-  // thread's index loads issue before its first sum
+  // entries of the item's contribution rows (k <= r < m) into gb, from the
+  // supernode's slot block
   __shared__ T gb[FWD_GROUP_ROWS];
   {
     const int rlo = max(row0, k), rhi = min(row0 + nrg * rpg, m);
     constexpr int NS = KMAX / 256 + FWD_GROUP_ROWS / 256;
-    int e0[NS], e1[NS], dst[NS];
+    int pos[NS], dst[NS];
     T acc[NS];
 #pragma unroll
     for (int j = 0; j < NS; j++) {
-      e0[j] = e1[j] = 0;
       dst[j] = -1;
+      pos[j] = 0;
       acc[j] = (T)0;
-      int p = -1;
       if (j < KMAX / 256) {
         const int q = threadIdx.x + 256 * j;
         if (q < k) {
-          p = q;
+          pos[j] = q;
           acc[j] = bp[first + q];
           dst[j] = d.ilperm[first + q];
         }
       } else {
         const int q = rlo + threadIdx.x + 256 * (j - KMAX / 256);
         if (q < rhi) {
-          p = q;
+          pos[j] = q;
           dst[j] = KMAX + 4 + (q - row0);  // marks gb[q - row0]
         }
       }
-      if (p >= 0) {
-        const int* gp = d.gptr + it.goff + p;
-        e0[j] = gp[0];
-        e1[j] = gp[1];
-      }
     }
+    const T* sb = u + it.soff;
 #pragma unroll
     for (int j = 0; j < NS; j++) {
       if (dst[j] < 0) continue;
-      T a = acc[j];
-      int e = e0[j];
-      for (; e + 3 < e1[j]; e += 4) {
-        const int i0 = d.gsrc[e], i1 = d.gsrc[e + 1], i2 = d.gsrc[e + 2], i3 = d.gsrc[e + 3];
-        const T v0 = u[i0], v1 = u[i1], v2 = u[i2], v3 = u[i3];
-        a += v0;
-        a += v1;
-        a += v2;
-        a += v3;
-      }
-      for (; e < e1[j]; e++) a += u[d.gsrc[e]];
+      const T a = slot_sum(sb + pos[j], it.nch, m, acc[j]);
       if (dst[j] < KMAX + 4) fs[dst[j]] = a;
       else gb[dst[j] - (KMAX + 4)] = a;
     }
@@ -1284,6 +1273,8 @@ __global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__
     const T* col = F + it.poff + (i < m ? prow_off(i, k) : 0);
     const int len = (i < m) ? min(i, k) : 0;
     const int nc = (len + W - 1) / W;
+    // a contribution row's slot in the parent's block (loaded before the product)
+    const int dr = (own && i >= k) ? d.rel[it.roff + i - k] : 0;
     vt xc[PF];
 #pragma unroll
     for (int q = 0; q < PF; q++) xc[q] = x0[q];
@@ -1320,7 +1311,7 @@ __global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__
     acc = seg_sum(acc, lg);
     if (own) {
       if (i < k) y[first + i] = fs[i] + acc;
-      else u[it.uoff + i - k] = gb[i - row0] - acc;
+      else u[it.sdst + dr] = gb[i - row0] - acc;
     }
   }
 }

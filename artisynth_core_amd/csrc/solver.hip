@@ -154,19 +154,25 @@ struct kkt_solver {
   double* d_F64 = nullptr;
   float* d_F32 = nullptr;
   // packed solve arrays per precision: forward rows, then (pack_elems + 16
-  // later) backward columns
+  // later) backward columns, then the forward update slots (sl_elems; zeroed
+  // when allocated)
   double* d_PK64 = nullptr;
   float* d_PK32 = nullptr;
-  long long pack_elems = 0;
+  long long pack_elems = 0, sl_elems = 0;
   int2* d_pack = nullptr;  // packing work items (supernode, block of 4 columns)
   int pack_cnt = 0;
   template <typename T>
   T* pk_fwd(T* PK) const { return PK; }
   template <typename T>
   T* pk_bwd(T* PK) const { return PK + pack_elems + 16; }
+  template <typename T>
+  T* pk_slot(T* PK) const { return PK + 2 * (pack_elems + 16); }
+  size_t pk_elems() const { return (size_t)2 * (pack_elems + 16) + (size_t)sl_elems + 16; }
+  // single-solve vectors, as T: bp, fp, y, xp (n each), w (v_elems + 16)
+  size_t svec_elems() const { return (size_t)4 * n + (size_t)v_elems + 16; }
   double* d_W64 = nullptr;
   float* d_W32 = nullptr;
-  // solve scratch, as T: bp, fp, y, xp (n each), u (u_elems + 16), w (v_elems + 16)
+  // solve scratch (svec_elems() entries of the front precision)
   void* d_svec = nullptr;
   size_t svec_bytes = 0;
   // multi-RHS solves: the solve vectors of SOLVE_NRHS right-hand sides (as T);
@@ -290,15 +296,15 @@ struct kkt_solver {
   size_t numeric_bytes(int p) const {
     const size_t esz = (p == KKT_PREC_FP64) ? 8 : 4;
     return ((size_t)front_elems + 2 * (size_t)n + 4) * esz + (size_t)std::max<long long>(w_elems, 1) * esz +
-           (size_t)2 * (pack_elems + 16) * esz + ((size_t)4 * n + (size_t)u_elems + 16 + (size_t)v_elems + 16) * 8;
+           pk_elems() * esz + svec_elems() * 8;
   }
   size_t held_bytes() const {
     if (!dev_ok || state < ANALYZED) return 0;
     size_t t = plan_bytes() + pattern_bytes();
     const size_t f64 = ((size_t)front_elems + 2 * (size_t)n + 4 + (size_t)std::max<long long>(w_elems, 1)) * 8;
     const size_t f32 = ((size_t)front_elems + 2 * (size_t)n + 4 + (size_t)std::max<long long>(w_elems, 1)) * 4;
-    if (d_F64) t += f64 + (size_t)2 * (pack_elems + 16) * 8;
-    if (d_F32) t += f32 + (size_t)2 * (pack_elems + 16) * 4;
+    if (d_F64) t += f64 + pk_elems() * 8;
+    if (d_F32) t += f32 + pk_elems() * 4;
     t += svec_bytes + msvec_bytes + mvec_bytes;
     return t;
   }
@@ -531,7 +537,8 @@ struct kkt_solver {
   // per lane) is the shape the autotuner picks per level; the gather items do
   // not depend on it.
   std::vector<int> sv_k, sv_m, sv_first;
-  std::vector<long long> sv_foff, sv_uoff, sv_roff, sv_goff, sv_voff, sv_poff;
+  std::vector<long long> sv_foff, sv_uoff, sv_roff, sv_goff, sv_voff, sv_poff, sv_soff, sv_sdst;
+  std::vector<int> sv_nch;
   std::vector<std::vector<int>> sv_lv;
   std::vector<int> fr_epl, bc_epl;
   std::vector<int> fr_fuse;  // per level: 1 = one forward launch (kind 2 items), 0 = gather + rows
@@ -587,6 +594,8 @@ struct kkt_solver {
       it.k = sv_k[s]; it.m = sv_m[s]; it.first = sv_first[s]; it.pad = 0;
       it.foff = sv_foff[s]; it.uoff = sv_uoff[s]; it.roff = sv_roff[s]; it.goff = sv_goff[s];
       it.voff = sv_voff[s]; it.poff = sv_poff[s];
+      it.soff = sv_soff[s]; it.sdst = sv_sdst[s]; it.nch = sv_nch[s];
+      it.pad1 = it.pad2 = it.pad3 = 0;
       return it;
     };
     auto seg_lg = [](int len, int epl, int cap) {
@@ -743,6 +752,16 @@ struct kkt_solver {
             gsrc[fill[goff[s] + rel[i]]++] = (int)(uoff[c] + (i - roff[c]));
         }
     }
+    // forward update slots: an nch x m block per supernode (children in the
+    // ascending order above), and where each child's contribution rows go
+    std::vector<long long> soff(ns + 1, 0), sdst(std::max(ns, 1), 0);
+    std::vector<int> nchv(std::max(ns, 1), 0);
+    for (int s = 0; s < ns; s++) {
+      nchv[s] = chp[s + 1] - chp[s];
+      soff[s + 1] = soff[s] + (long long)nchv[s] * mm[s];
+      for (int q = chp[s]; q < chp[s + 1]; q++) sdst[chl[q]] = soff[s] + (long long)(q - chp[s]) * mm[s];
+    }
+    sl_elems = soff[ns];
     // levels
     int nl = S.nlevels;
     std::vector<std::vector<int>> lv(nl);
@@ -987,6 +1006,7 @@ struct kkt_solver {
     // launch per level)
     sv_k = kk; sv_m = mm; sv_first = first; sv_foff = foff; sv_uoff = uoff; sv_roff = roff;
     sv_goff = goff; sv_voff = voff; sv_lv = lv; sv_poff = poffv;
+    sv_soff.assign(soff.begin(), soff.end() - 1); sv_sdst = sdst; sv_nch = nchv;
     fr_epl.assign(nl, 16);
     bc_epl.assign(nl, 16);
     fr_fuse.assign(nl, 1);
@@ -1037,16 +1057,20 @@ struct kkt_solver {
       // entries (the solve kernels' 16-byte chunk loads may run 3 past a column)
       if (hipMalloc(&d_F64, (front_elems + 2LL * n + 4) * sizeof(double)) != hipSuccess ||
           hipMalloc(&d_W64, std::max<long long>(w_elems, 1) * sizeof(double)) != hipSuccess ||
-          hipMalloc(&d_PK64, 2 * (pack_elems + 16) * sizeof(double)) != hipSuccess)
+          hipMalloc(&d_PK64, pk_elems() * sizeof(double)) != hipSuccess)
         return fail(KKT_ERR_ALLOC, "hipMalloc failed (FP64 fronts)");
+      if (hipMemsetAsync(d_PK64, 0, pk_elems() * sizeof(double), st) != hipSuccess)
+        return fail(KKT_ERR_DEVICE, "clearing the update slots failed");
     }
     if (p == KKT_PREC_FP32_REFINE && !d_F32) {
       if (hipMalloc(&d_F32, (front_elems + 2LL * n + 4) * sizeof(float)) != hipSuccess ||
           hipMalloc(&d_W32, std::max<long long>(w_elems, 1) * sizeof(float)) != hipSuccess ||
-          hipMalloc(&d_PK32, 2 * (pack_elems + 16) * sizeof(float)) != hipSuccess)
+          hipMalloc(&d_PK32, pk_elems() * sizeof(float)) != hipSuccess)
         return fail(KKT_ERR_ALLOC, "hipMalloc failed (FP32 fronts)");
+      if (hipMemsetAsync(d_PK32, 0, pk_elems() * sizeof(float), st) != hipSuccess)
+        return fail(KKT_ERR_DEVICE, "clearing the update slots failed");
     }
-    size_t need = ((size_t)4 * n + (size_t)u_elems + 16 + (size_t)v_elems + 16) * std::max(esz, sizeof(double));
+    size_t need = svec_elems() * std::max(esz, sizeof(double));
     if (svec_bytes < need) {
       drop_graphs();
       if (d_svec) (void)hipFree(d_svec);
@@ -1171,10 +1195,10 @@ struct kkt_solver {
 
   // ------------------------------------------------------------------
   template <typename T>
-  void run_tri_solve(const T* F, const T* PK, const double* d_b, double* d_x, void* svec = nullptr) {
+  void run_tri_solve(const T* F, T* PK, const double* d_b, double* d_x, void* svec = nullptr) {
     T* sv = (T*)(svec ? svec : d_svec);
-    T *bp = sv, *fp = sv + n, *y = sv + 2 * (size_t)n, *xp = sv + 3 * (size_t)n,
-      *u = sv + 4 * (size_t)n, *wv = u + u_elems + 16;
+    T *bp = sv, *fp = sv + n, *y = sv + 2 * (size_t)n, *xp = sv + 3 * (size_t)n, *wv = sv + 4 * (size_t)n;
+    T* u = pk_slot(PK);
     const T* E0 = F + front_elems;
     const T* E1 = E0 + n;
     tbeg(TC_VEC);
@@ -1643,7 +1667,7 @@ struct kkt_solver {
   static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
   size_t noalloc_bytes() const {
     const size_t esz = (fprec == KKT_PREC_FP64) ? sizeof(double) : sizeof(float);
-    const size_t sv = align256(((size_t)4 * n + (size_t)u_elems + 16 + (size_t)v_elems + 16) * esz);
+    const size_t sv = align256(svec_elems() * esz);
     return sv + 2 * align256((size_t)n * sizeof(double)) + align256(1024 * sizeof(double)) + 256;
   }
   int solve_device_noalloc(const double* d_b, double* d_x, unsigned char* scr, size_t cap) {
@@ -1658,7 +1682,7 @@ struct kkt_solver {
     const size_t esz = (fprec == KKT_PREC_FP64) ? sizeof(double) : sizeof(float);
     unsigned char* p = scr;
     void* svec = p;
-    p += align256(((size_t)4 * n + (size_t)u_elems + 16 + (size_t)v_elems + 16) * esz);
+    p += align256(svec_elems() * esz);
     double* r = (double*)p;
     p += align256((size_t)n * sizeof(double));
     double* dx = (double*)p;
@@ -1754,7 +1778,8 @@ struct kkt_solver {
         typedef typename std::remove_pointer<decltype(PK)>::type T;
         T* svT = (T*)sv;
         T *bp = svT, *fp = svT + n, *y = svT + 2 * (size_t)n, *xp = svT + 3 * (size_t)n,
-          *u = svT + 4 * (size_t)n, *wv = u + u_elems + 16;
+          *wv = svT + 4 * (size_t)n;
+        T* u = pk_slot(PK);
         for (int L = 0; L < nl; L++) {
           if (what == 0) {
             if (ff_l[L].cnt) launch_fwd_level<T>(d_ff + ff_l[L].off, ff_l[L].cnt, ds, pk_fwd(PK), bp, fp, y, u, st);
