@@ -139,6 +139,7 @@ SIGNATURES = [
     ("kkt_batch_set_precision", _I, [_P, _I]),
     ("kkt_batch_set_refine_tol", _I, [_P, _D, _I]),
     ("kkt_batch_set_ir_steps", _I, [_P, _I]),
+    ("kkt_batch_autotune_solve", _I, [_P, _I]),
     ("kkt_batch_get_info", _I, [_P, ctypes.POINTER(KktInfo)]),
     ("kkt_batch_system_info", _I, [_P, _I, ctypes.POINTER(KktBatchSysInfo)]),
     ("kkt_batch_last_error", ctypes.c_char_p, [_P]),

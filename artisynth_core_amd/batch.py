@@ -54,6 +54,11 @@ class BatchedKKT:
     def solve_device(self, d_b, d_x):
         self._check(self.L.kkt_batch_solve_device(self.h, d_b, d_x), "solve")
 
+    def autotune_solve(self, reps=3):
+        """Times the solve shapes per tree level on the factored batch and keeps
+        the fastest (kkt_batch_autotune_solve)."""
+        self._check(self.L.kkt_batch_autotune_solve(self.h, int(reps)), "autotune")
+
     def factor(self, vals=None):
         v = np.ascontiguousarray(self.vals if vals is None else vals, np.float64)
         self._check(self.L.kkt_batch_factor(self.h, nptr(v, ctypes.c_double)), "factor")

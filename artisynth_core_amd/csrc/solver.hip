@@ -3122,6 +3122,12 @@ int kkt_batch_set_ir_steps(kkt_batch* b, int nsteps) {
   return b->from_h(kkt_set_ir_steps(b->h, nsteps));
 }
 
+int kkt_batch_autotune_solve(kkt_batch* b, int reps) {
+  CHECK_B(b);
+  if (!b->h) return b->fail(KKT_ERR_STATE, "no systems");
+  return b->from_h(kkt_autotune_solve(b->h, reps));
+}
+
 int kkt_batch_get_info(kkt_batch* b, kkt_info* out) {
   CHECK_B(b);
   if (!b->h) return b->fail(KKT_ERR_STATE, "no systems");

@@ -345,6 +345,9 @@ int kkt_batch_solve_device(kkt_batch* b, const double* d_B, double* d_X);
 int kkt_batch_set_precision(kkt_batch* b, int prec);
 int kkt_batch_set_refine_tol(kkt_batch* b, double tol, int max_steps);
 int kkt_batch_set_ir_steps(kkt_batch* b, int nsteps);
+/* picks the batch's solve shapes per tree level by timing, as kkt_autotune_solve
+ * (after a factor; the batch's levels hold the fronts of every system) */
+int kkt_batch_autotune_solve(kkt_batch* b, int reps);
 /* statistics of the whole batch (as kkt_get_info) and of one system */
 int kkt_batch_get_info(kkt_batch* b, kkt_info* out);
 int kkt_batch_system_info(kkt_batch* b, int sys, kkt_batch_sysinfo* out);

@@ -289,6 +289,27 @@ def test_batched_systems_vs_oracle(gpu):
         B.dispose()
 
 
+def test_batched_autotuned_shapes(gpu):
+    # the autotuner on a batch: the tuned solve still matches the oracle per
+    # system, and the tuned shapes are within the kernels' limits
+    _say("batched mix 12 systems, autotuned solve shapes")
+    from artisynth_core_amd.batch import BatchedKKT
+    systems = G.batch_mix(12, seed=78, dof_lo=300, dof_hi=1500)
+    B = BatchedKKT(systems)
+    try:
+        B.factor()
+        x0 = B.solve()
+        B.autotune_solve(2)
+        x = B.solve()
+        assert max(B.per_system_residuals(x, systems)) <= 1e-8
+        assert rel_err(x, x0) <= 1e-9
+        for q, (ro, ci, va, b) in enumerate(systems):
+            xo = O.Factor(ro, ci, va).solve(b)
+            assert rel_err(B.split(x)[q], xo) <= 1e-8
+    finally:
+        B.dispose()
+
+
 def test_serving_alone_equals_mix(gpu):
     # each instance's results are the same alone and in a mix (no eviction: the
     # same step kinds in the same order on each instance's own handle)
