@@ -14,7 +14,7 @@ constexpr int FWD_SMALL_K = 32;
 constexpr int FWD_SMALL_M = 256;
 // widest supernode the solve kernels take (LDS vectors of KMAX entries); the
 // symbolic analysis splits wider ones, and the solve plan checks it
-constexpr int KMAX = 1024;
+constexpr int KMAX = 2048;
 // rows of one forward row block of a large front at most (its row groups share
 // one gather of the front's right-hand side; the contribution rows' entries are
 // staged in LDS)

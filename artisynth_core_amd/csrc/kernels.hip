@@ -1233,38 +1233,11 @@ __global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__
   // supernode's slot block
   __shared__ T gb[FWD_GROUP_ROWS];
   {
-    const int rlo = max(row0, k), rhi = min(row0 + nrg * rpg, m);
-    constexpr int NS = KMAX / 256 + FWD_GROUP_ROWS / 256;
-    int pos[NS], dst[NS];
-    T acc[NS];
-#pragma unroll
-    for (int j = 0; j < NS; j++) {
-      dst[j] = -1;
-      pos[j] = 0;
-      acc[j] = (T)0;
-      if (j < KMAX / 256) {
-        const int q = threadIdx.x + 256 * j;
-        if (q < k) {
-          pos[j] = q;
-          acc[j] = bp[first + q];
-          dst[j] = d.ilperm[first + q];
-        }
-      } else {
-        const int q = rlo + threadIdx.x + 256 * (j - KMAX / 256);
-        if (q < rhi) {
-          pos[j] = q;
-          dst[j] = KMAX + 4 + (q - row0);  // marks gb[q - row0]
-        }
-      }
-    }
     const T* sb = u + it.soff;
-#pragma unroll
-    for (int j = 0; j < NS; j++) {
-      if (dst[j] < 0) continue;
-      const T a = slot_sum(sb + pos[j], it.nch, m, acc[j]);
-      if (dst[j] < KMAX + 4) fs[dst[j]] = a;
-      else gb[dst[j] - (KMAX + 4)] = a;
-    }
+    for (int q = threadIdx.x; q < k; q += 256)
+      fs[d.ilperm[first + q]] = slot_sum(sb + q, it.nch, m, bp[first + q]);
+    const int rlo = max(row0, k), rhi = min(row0 + nrg * rpg, m);
+    for (int q = rlo + threadIdx.x; q < rhi; q += 256) gb[q - row0] = slot_sum(sb + q, it.nch, m, (T)0);
   }
   __syncthreads();
   for (int rg = 0; rg < nrg; rg++) {

@@ -690,7 +690,7 @@ struct kkt_solver {
       voff[s + 1] = voff[s] + mm[s];
     }
     for (int s = 0; s < ns; s++)
-      if (kk[s] > 1024) return fail(KKT_ERR_ANALYSIS, "internal: supernode wider than 1024 columns");
+      if (kk[s] > KMAX) return fail(KKT_ERR_ANALYSIS, "internal: supernode wider than the solve kernels take");
     front_elems = foff[ns];
     w_elems = woff[ns];
     u_elems = uoff[ns];

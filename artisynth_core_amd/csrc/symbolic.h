@@ -26,7 +26,8 @@ struct SymOptions {
   // for a few percent more fill
   int nrelax0 = 8, nrelax1 = 96, nrelax2 = 256;
   double zrelax0 = 0.8, zrelax1 = 0.3, zrelax2 = 0.1;
-  int max_super = 1024;    // widest supernode; wider ones are split into a chain
+  int max_super = 2048;    // widest supernode (the solve kernels' limit); wider
+                           // ones are split into a chain
 };
 
 struct Symbolic {

@@ -53,7 +53,7 @@ def test_symbolic_matches_oracle(case, ordering):
     assert np.array_equal(rows, S["rows"])
     sup = S["sup_ptr"]
     ns = len(sup) - 1
-    assert sup[0] == 0 and sup[-1] == n and np.all(np.diff(sup) > 0) and np.all(np.diff(sup) <= 1024)
+    assert sup[0] == 0 and sup[-1] == n and np.all(np.diff(sup) > 0) and np.all(np.diff(sup) <= 2048)
     # nested structure: child rows outside the parent's columns lie in the parent's rows
     for s in range(ns):
         p = S["sparent"][s]
