@@ -35,8 +35,9 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def _structure(L, h):
-    """Supernode sizes k, r and tree levels of the handle's analysis."""
+def _structure(L, h, with_parent=False):
+    """Supernode sizes k, r and tree levels of the handle's analysis (and the
+    assembly-tree parents)."""
     from artisynth_core_amd._lib import KktInfo
     from artisynth_core_amd._lib import ptr as nptr
     inf = KktInfo()
@@ -53,6 +54,8 @@ def _structure(L, h):
         if par[s_] >= 0:
             lev[par[s_]] = max(lev[par[s_]], lev[s_] + 1)
     assert lev.max() + 1 == inf.nlevels
+    if with_parent:
+        return k, r, lev, inf, par
     return k, r, lev, inf
 
 
@@ -72,34 +75,45 @@ def fwd_sweep_bytes(L, h, w, fwd_epl, forms, groups):
     """Algorithmic HBM bytes of one forward sweep (all k_fwd_level launches of a
     triangular solve), per supernode (k pivots, r rows below): the upper copy of
     G once (k(k-1)/2 entries of X, unit diagonal implicit, and k r of Z), the
-    right-hand side at the pivots (k) and the children's update entries (counted
-    once, at the parent: r per supernode), y written for the pivots (k) and the
-    update vector for the rows (r), and the gather map (m + 1 offsets and r
-    source indices, 4 bytes each); w = bytes per vector/factor entry.
-    Returned separately: the redundant bytes of the one-launch form, where every
-    row block of a large front gathers the front's right-hand side itself (m
-    entries and the map once more per extra row block; a block covers `groups`
-    row groups of its level)."""
-    k, r, lev, inf = _structure(L, h)
+    right-hand side at the pivots (k) and y written for them (k), its r update
+    entries written once and read once by the parent, the pivot permutation (k)
+    and the rows' slot positions in the parent (r), 4 bytes each; w = bytes per
+    vector/factor entry.
+    Returned separately, the layout's extra reads: the zero entries of each dense
+    update-slot block (nch x m entries read, of which only the children's rows
+    were written) and, in the one-launch form, every further row block of a large
+    front gathering the front's pivot part again (k positions: right-hand side,
+    nch slots and the permutation; a block covers `groups` row groups of its
+    level); in the two-launch form, the gathered pivot part written once and read
+    by every row block."""
+    k, r, lev, inf, par = _structure(L, h, with_parent=True)
     m = k + r
-    kf, rf = k.astype(np.float64), r.astype(np.float64)
-    alg = float(np.sum((kf * (kf - 1) / 2 + kf * rf + 2 * kf + 2 * rf) * w + (m + 1 + r) * 4.0))
-    red = 0.0
+    kf, rf, mf = k.astype(np.float64), r.astype(np.float64), m.astype(np.float64)
+    nch = np.zeros(len(k), np.float64)
+    rch = np.zeros(len(k), np.float64)  # update entries the children write into the block
+    for s_ in range(len(k)):
+        if par[s_] >= 0:
+            nch[par[s_]] += 1
+            rch[par[s_]] += rf[s_]
+    alg = float(np.sum((kf * (kf - 1) / 2 + kf * rf + 2 * kf + 2 * rf) * w + (kf + rf) * 4.0))
+    red = float(np.sum((nch * mf - rch) * w))
     for s_ in range(len(k)):
         if k[s_] <= 32 and m[s_] <= 256:  # small fronts: gather and rows in one workgroup
             continue
         L_ = lev[s_]
-        if forms[L_] != 1:
-            continue
         lg = 0
         while lg < 6 and fwd_epl[L_] * (1 << lg) < k[s_]:
             lg += 1
+        if forms[L_] != 1:
+            nblk = (m[s_] + 4 * (64 >> lg) - 1) // (4 * (64 >> lg))
+            red += (nblk + 1) * kf[s_] * w
+            continue
         rg = int(groups[L_])
         while rg > 1 and rg * 4 * (64 >> lg) > 1024:
             rg //= 2
         rows_per_block = 4 * (64 >> lg) * rg
         nblk = (m[s_] + rows_per_block - 1) // rows_per_block
-        red += (nblk - 1) * (m[s_] * w + (m[s_] + 1 + r[s_]) * 4.0)
+        red += (nblk - 1) * kf[s_] * ((1 + nch[s_]) * w + 4.0)
     return alg, red
 
 
