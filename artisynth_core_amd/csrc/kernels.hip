@@ -1829,11 +1829,18 @@ __global__ void k_expand_vals(long long nnzf, const int* __restrict__ map,
 }
 
 // one wave per row
-// y = A x (mode 0) or b - A x (mode 1): 8 lanes per row, 32 rows per workgroup,
-// four entries of a lane in flight.  Optional fused dot products of the result
-// (BiCGStab): part1[block] = sum of a1_i y_i, part2[block] = sum of y_i^2 over the
-// block's rows, in a fixed order.
-constexpr int SPMV_ROWS = 32;
+// y = A x (mode 0) or b - A x (mode 1): SPMV_LANES lanes per row, SPMV_ROWS rows
+// per workgroup.  Lane sl takes the row's entries sl, sl + 16, sl + 32, ...
+// (its q-th at rowptr[row] + sl + 16 q), eight per round: every column index and
+// value load of a round issues first, then the round's eight x loads, so a row of
+// up to 128 entries costs two dependent round trips.  Entries with q even are
+// added into one accumulator and q odd into another, then the two, then the
+// lanes of the row (xor butterfly 8, 4, 2, 1); k_spmv_resid_m adds in this same
+// order.  Optional fused dot products of the result (BiCGStab): part1[block] =
+// sum of a1_i y_i, part2[block] = sum of y_i^2 over the block's rows, in a fixed
+// order.
+constexpr int SPMV_LANES = 16;
+constexpr int SPMV_ROWS = 256 / SPMV_LANES;
 __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ rowptr,
                                               const int* __restrict__ col,
                                               const double* __restrict__ val,
@@ -1844,25 +1851,37 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ row
                                               double* __restrict__ part1,
                                               double* __restrict__ part2) {
   __shared__ double sm[2][4];
-  const int row = blockIdx.x * SPMV_ROWS + (threadIdx.x >> 3);
-  const int sl = threadIdx.x & 7;
+  const int row = blockIdx.x * SPMV_ROWS + threadIdx.x / SPMV_LANES;
+  const int sl = threadIdx.x & (SPMV_LANES - 1);
   double acc = 0.0;
   if (row < n) {
     const int e1 = rowptr[row + 1];
-    int e = rowptr[row] + sl;
     double a0 = 0.0, b0 = 0.0;
-    for (; e + 24 < e1; e += 32) {
-      const double v0 = val[e], v1 = val[e + 8], v2 = val[e + 16], v3 = val[e + 24];
-      const int c0 = col[e], c1 = col[e + 8], c2 = col[e + 16], c3 = col[e + 24];
-      a0 = fma(v0, x[c0], a0);
-      b0 = fma(v1, x[c1], b0);
-      a0 = fma(v2, x[c2], a0);
-      b0 = fma(v3, x[c3], b0);
+    for (int e = rowptr[row] + sl; e < e1; e += 8 * SPMV_LANES) {
+      int c[8];
+      double v[8], xv[8];
+      bool ok[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        ok[q] = e + q * SPMV_LANES < e1;
+        c[q] = 0;
+        v[q] = 0.0;
+        if (ok[q]) {
+          c[q] = col[e + q * SPMV_LANES];
+          v[q] = val[e + q * SPMV_LANES];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 8; q++) xv[q] = ok[q] ? x[c[q]] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 8; q += 2) {
+        a0 = ok[q] ? fma(v[q], xv[q], a0) : a0;
+        b0 = ok[q + 1] ? fma(v[q + 1], xv[q + 1], b0) : b0;
+      }
     }
-    for (; e < e1; e += 8) a0 = fma(val[e], x[col[e]], a0);
     acc = a0 + b0;
   }
-  for (int off = 4; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  for (int off = SPMV_LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
   double d1 = 0.0, d2 = 0.0;
   if (row < n && sl == 0) {
     const double yv = mode ? (b[row] - acc) : acc;
@@ -1918,43 +1937,36 @@ __global__ __launch_bounds__(256) void k_spmv_resid_m(int n, int nr, const int* 
                                                       const double* __restrict__ X,
                                                       const double* __restrict__ Bv, double* __restrict__ Y,
                                                       long long ld) {
-  const int row = blockIdx.x * SPMV_ROWS + (threadIdx.x >> 3);
-  const int sl = threadIdx.x & 7;
+  const int row = blockIdx.x * SPMV_ROWS + threadIdx.x / SPMV_LANES;
+  const int sl = threadIdx.x & (SPMV_LANES - 1);
   double acc[SOLVE_NRHS];
 #pragma unroll
   for (int j = 0; j < SOLVE_NRHS; j++) acc[j] = 0.0;
   if (row < n) {
     const int e1 = rowptr[row + 1];
-    int e = rowptr[row] + sl;
     double a0[SOLVE_NRHS], b0[SOLVE_NRHS];
 #pragma unroll
     for (int j = 0; j < SOLVE_NRHS; j++) a0[j] = b0[j] = 0.0;
-    for (; e + 24 < e1; e += 32) {
-      const double v0 = val[e], v1 = val[e + 8], v2 = val[e + 16], v3 = val[e + 24];
-      const int c0 = col[e], c1 = col[e + 8], c2 = col[e + 16], c3 = col[e + 24];
+    // k_spmv's order: the lane's q-th entry into a0 (q even) or b0 (q odd)
+    for (int e = rowptr[row] + sl; e < e1; e += 2 * SPMV_LANES) {
+      const bool ok1 = e + SPMV_LANES < e1;
+      const double v0 = val[e], v1 = ok1 ? val[e + SPMV_LANES] : 0.0;
+      const int c0 = col[e], c1 = ok1 ? col[e + SPMV_LANES] : 0;
 #pragma unroll
       for (int j = 0; j < SOLVE_NRHS; j++) {
         if (j >= nr) break;
         const double* x = X + j * ld;
-        a0[j] = fma(v0, x[c0], a0[j]);
-        b0[j] = fma(v1, x[c1], b0[j]);
-        a0[j] = fma(v2, x[c2], a0[j]);
-        b0[j] = fma(v3, x[c3], b0[j]);
+        const double x0 = x[c0], x1 = ok1 ? x[c1] : 0.0;
+        a0[j] = fma(v0, x0, a0[j]);
+        b0[j] = ok1 ? fma(v1, x1, b0[j]) : b0[j];
       }
-    }
-    for (; e < e1; e += 8) {
-      const double v = val[e];
-      const int c = col[e];
-#pragma unroll
-      for (int j = 0; j < SOLVE_NRHS; j++)
-        if (j < nr) a0[j] = fma(v, X[j * ld + c], a0[j]);
     }
 #pragma unroll
     for (int j = 0; j < SOLVE_NRHS; j++) acc[j] = a0[j] + b0[j];
   }
 #pragma unroll
   for (int j = 0; j < SOLVE_NRHS; j++)
-    for (int off = 4; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, 64);
+    for (int off = SPMV_LANES / 2; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, 64);
   if (row < n && sl == 0) {
 #pragma unroll
     for (int j = 0; j < SOLVE_NRHS; j++)

@@ -178,18 +178,25 @@ def kernel_pass(L, h, vals_ptr, b_ptr, x_ptr):
     time as it runs in the sequence (graph replay, no events)."""
     import torch
     from artisynth_core_amd._lib import KktInfo
-    L.kkt_timing_reset(h, 0x8F)  # factor classes + vector ops
-    assert L.kkt_factor_device(h, vals_ptr) == 0, L.kkt_last_error(h)
-    fms, fcnt, uflops = _timing(L, h)
-    L.kkt_timing_reset(h, 0)
     # sweep and SpMV kernel durations: graph replays between two HIP events on
-    # the solver's stream (no per-launch events, so they match a kernel trace)
+    # the solver's stream (no per-launch events, so they match a kernel trace).
+    # The SpMV first, on the sequence's last values and factor: after the
+    # refactor below (step 0's values) its replays measured 26-27 us instead of
+    # 20 us, a state the timed sequence never runs in
     kt = {}
-    for what, key in ((0, "fwd"), (1, "bwd"), (2, "spmv")):
+
+    def time_kernels(what, key):
         ms = ctypes.c_double()
         cnt = ctypes.c_int()
         assert L.kkt_time_kernels(h, what, 20, ctypes.byref(ms), ctypes.byref(cnt)) == 0, L.kkt_last_error(h)
         kt[key] = (ms.value, cnt.value)
+    time_kernels(2, "spmv")
+    L.kkt_timing_reset(h, 0x8F)  # factor classes + vector ops
+    assert L.kkt_factor_device(h, vals_ptr) == 0, L.kkt_last_error(h)
+    fms, fcnt, uflops = _timing(L, h)
+    L.kkt_timing_reset(h, 0)
+    time_kernels(0, "fwd")
+    time_kernels(1, "bwd")
     inf = KktInfo()
     L.kkt_get_info(h, ctypes.byref(inf))
     torch.cuda.synchronize()
