@@ -180,9 +180,11 @@ def kernel_pass(L, h, vals_ptr, b_ptr, x_ptr):
     from artisynth_core_amd._lib import KktInfo
     # sweep and SpMV kernel durations: graph replays between two HIP events on
     # the solver's stream (no per-launch events, so they match a kernel trace).
-    # The SpMV first, on the sequence's last values and factor: after the
-    # refactor below (step 0's values) its replays measured 26-27 us instead of
-    # 20 us, a state the timed sequence never runs in
+    # The SpMV first, on the sequence's last values and factor (20 us per launch
+    # at config L); replayed right after the refactor below, on step 0's values,
+    # the same launch measured 26-27 us, for a reason not identified (the values
+    # hold no subnormals); the iterative steps that dominate the sequence run on
+    # the later values
     kt = {}
 
     def time_kernels(what, key):
