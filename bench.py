@@ -396,7 +396,8 @@ def main():
         avg = ms / launches
         ach = bpl / (avg * 1e-3) / 1e9
         return {"bound": "hbm", "kernel": kernel, "achieved": ach, "peak": PEAKS["hbm_GBs"], "unit": "GB/s",
-                "frac": ach / PEAKS["hbm_GBs"], "traffic": profile_traffic(traffic_key),
+                "frac": ach / PEAKS["hbm_GBs"],
+                "traffic": profile_traffic(traffic_key, r32["info"]["nlevels"], launches / sweeps),
                 "avg_launch_ms": avg, "launches": launches, "bytes_per_launch": bpl}
     roofline = hbm_roof("k_fwd_level<float>", r32["fwd_sweep_bytes"], kp["fwd_ms"], kp["fwd_launches"],
                         kp["sweeps"], "k_fwd_level<float>")
@@ -462,16 +463,24 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def profile_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed counter profile
-    (profiles/traffic_hexbeam_100x16x16.json: FETCH_SIZE + WRITE_SIZE per dispatch
-    from two rocprofv3 --pmc runs of `bench.py --mode pmc`, each scaled by the
-    factor the same counter needs on a 256 MiB device copy); None if absent."""
+def profile_traffic(kernel, nlevels, launches_per_sweep):
+    """HBM bytes of `kernel` per launch of this run's plan, from the committed
+    counter profile (profiles/traffic_hexbeam_100x16x16.json: FETCH_SIZE +
+    WRITE_SIZE per dispatch from two rocprofv3 --pmc runs of `bench.py --mode
+    pmc`, each scaled by the factor the same counter needs on a 256 MiB device
+    copy).  The profile runs the default solve plan (one launch per tree level),
+    so its bytes per sweep (per-dispatch bytes x dispatches / sweeps, one
+    k_bwd_cols dispatch per level and sweep) are spread over this run's
+    launches per sweep.  None if absent."""
     tf = os.path.join(ROOT, "profiles", "traffic_hexbeam_100x16x16.json")
     if not os.path.exists(tf):
         return None
     d = json.load(open(tf)).get("kernels", {})
-    return d.get(kernel, {}).get("bytes_per_launch")
+    k, b = d.get(kernel), d.get("k_bwd_cols<float>")
+    if not k or not b or nlevels <= 0 or launches_per_sweep <= 0:
+        return None
+    sweeps = b["launches"] / float(nlevels)
+    return k["bytes_per_launch"] * k["launches"] / sweeps / launches_per_sweep
 
 
 def stream_run(reps=5):
