@@ -122,6 +122,7 @@ SIGNATURES = [
     ("kkt_set_refine_tol", _I, [_P, _D, _I]),
     ("kkt_set_graphs", _I, [_P, _I]),
     ("kkt_set_ordering", _I, [_P, _I]),
+    ("kkt_set_max_supernode", _I, [_P, _I]),
     ("kkt_get_info", _I, [_P, ctypes.POINTER(KktInfo)]),
     ("kkt_get_symbolic", _I, [_P, _IP, _IP, _IP, _LLP, _IP]),
     ("kkt_verify_device_structure", _I, [_P]),

@@ -107,6 +107,7 @@ struct kkt_solver {
   std::vector<int> rowOffs, colIdxs;
   std::vector<int> diagPos;  // position of the diagonal entry of each row, -1 if absent
   int ordering = 1;
+  int max_super = KMAX;  // widest supernode of the next analysis
   int prec = KKT_PREC_FP64;   // requested precision of the next factor
   int fprec = KKT_PREC_FP64;  // precision of the current factor
   bool delay_known = false;
@@ -490,6 +491,7 @@ struct kkt_solver {
   int analyze_with_mask(const unsigned char* delay) {
     SymOptions opt;
     opt.ordering = ordering;
+    opt.max_super = max_super;
     std::string e;
     double t0 = now_ms();
     int rc = symbolic_analyze(n, rowOffs.data(), colIdxs.data(), delay, opt, S, e);
@@ -2829,6 +2831,14 @@ int kkt_set_ordering(kkt_solver* h, int ordering) {
   CHECK_H(h);
   if (ordering != 0 && ordering != 1) return h->fail(KKT_ERR_ARGUMENT, "unknown ordering");
   h->ordering = ordering;
+  h->ok();
+  return KKT_OK;
+}
+
+int kkt_set_max_supernode(kkt_solver* h, int cols) {
+  CHECK_H(h);
+  if (cols < 64 || cols > KMAX) return h->fail(KKT_ERR_ARGUMENT, "supernode width outside 64..2048");
+  h->max_super = cols;
   h->ok();
   return KKT_OK;
 }

@@ -469,6 +469,11 @@ class MI355XSolver:
         self._check(self._L.kkt_get_forward_row_groups(self._h, ptr(g, ctypes.c_int)), "getForwardRowGroups")
         return g[:nl]
 
+    def setMaxSupernode(self, cols):
+        """Widest supernode (64..2048 columns) of the next analysis: wider means
+        fewer tree levels (faster triangular solves) for more factor work."""
+        self._check(self._L.kkt_set_max_supernode(self._h, int(cols)), "setMaxSupernode")
+
     def setForwardRowGroups(self, groups):
         """Sets the row groups of every tree level; the solution is bitwise the same."""
         g = np.ascontiguousarray(groups, np.int32)

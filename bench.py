@@ -234,6 +234,8 @@ def main():
                          "rates of one launch over 4 MB - 1 GB with cold caches (the achievable HBM ceiling)")
     ap.add_argument("--nrhs", type=int, default=64, help="multi: right-hand sides per block solve")
     ap.add_argument("--tet", default="121x24x24", help="precision: Delaunay lattice points")
+    ap.add_argument("--max-super", type=int, default=2048,
+                    help="widest supernode of the analysis (64..2048; kkt_set_max_supernode)")
     ap.add_argument("--oracle", action="store_true",
                     help="precision: also run the CPU oracle on step 0 (minutes at full size)")
     ap.add_argument("--nbatch", type=int, default=256)
@@ -267,7 +269,7 @@ def main():
         return serve_run(args.ninst, args.nreq, args.rate, args.workers, args.budget_gb, args.eps)
     if args.mode == "precision":
         return precision_run(tuple(int(v) for v in args.tet.split("x")), min(args.nsys, 4),
-                             args.warmup, args.oracle)
+                             args.warmup, args.oracle, args.max_super)
     if args.mode == "multi":
         return multi_run(nx, ny, nz, args.nrhs, max(args.steps // 4, 2), args.warmup)
     if args.mode == "stream":
@@ -843,7 +845,7 @@ def serve_run(ninst, nreq, rate, nworkers, budget_gb, eps=3e-3):
     srv.shutdown()
 
 
-def precision_run(dims, nsteps, warmup, with_oracle):
+def precision_run(dims, nsteps, warmup, with_oracle, max_super=2048):
     """The precision workload: seeded Delaunay tet box (0.1 x 0.02 x 0.02 m, ~210k DOF at the
     default lattice), Mooney-Rivlin c10 = 1e3, c01 = 0, kappa = 1e5, attached end.
     Every step is a direct solve: FP32 fronts on MFMA with FP64 refinement to the
@@ -874,6 +876,7 @@ def precision_run(dims, nsteps, warmup, with_oracle):
         h = L.kkt_create()
         L.kkt_set_precision(h, prec)
         L.kkt_set_refine_tol(h, 1e-8, 10)
+        assert L.kkt_set_max_supernode(h, max_super) == 0, L.kkt_last_error(h)
         if prec == KKT_PREC_FP64:
             L.kkt_set_ir_steps(h, 1)
         ta = time.time()
@@ -937,7 +940,8 @@ def precision_run(dims, nsteps, warmup, with_oracle):
     line = {"mode": "precision", "workload": "Delaunay tet box %s lattice, n=%d, nnz_upper=%d, "
                                             "%d tets, Mooney-Rivlin c10=1e3 kappa=1e5" % (
                                                 "x".join(map(str, dims)), n, nnz, len(body.elems)),
-            "n": n, "nnz_upper": nnz, "steps": len(V),
+            "n": n, "nnz_upper": nnz, "steps": len(V), "max_supernode": max_super,
+            "nlevels": info[0]["nlevels"],
             "factor_gflop": info[0]["factor_flops"] / 1e9, "nnzL": info[0]["nnzL"],
             "fp32": {k: v for k, v in r32.items() if k != "x"},
             "fp64": {k: v for k, v in r64.items() if k != "x"},

@@ -246,6 +246,14 @@ int kkt_set_refine_tol(kkt_solver* h, double tol, int max_steps);
 int kkt_set_graphs(kkt_solver* h, int on);
 /* ordering: 0 natural, 1 nested dissection (default) */
 int kkt_set_ordering(kkt_solver* h, int ordering);
+/* Widest supernode the analysis builds (64..2048 columns, default 2048; wider
+ * separators become chains), applied from the next analysis.  No counterpart in
+ * the reference.  Wider supernodes mean fewer tree levels, so fewer launches per
+ * triangular solve, for more fill and factor work from the relaxed amalgamation:
+ * on a 210k-DOF tet box 2048 gives 19 levels and 793 GFLOP per factor, 1024
+ * gives 27 levels and 705 GFLOP.  Solve-heavy callers (stale-factor iterations)
+ * gain from the default, direct-solve-heavy ones may prefer 1024. */
+int kkt_set_max_supernode(kkt_solver* h, int cols);
 int kkt_get_info(kkt_solver* h, kkt_info* out);
 /* Copies of the host symbolic structure (for structure-parity checks):
  * perm (n), sup_ptr (nsuper+1), sparent (nsuper), rows_ptr (nsuper+1, int64),

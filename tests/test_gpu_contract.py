@@ -219,3 +219,42 @@ def test_batch_of_mixed_scales_matches_alone(gpu, prec):
         assert B.info()["fp64_fallbacks"] == 0
     finally:
         B.dispose()
+
+
+@pytest.mark.parametrize("prec", [KKT_PREC_FP64, KKT_PREC_FP32_REFINE])
+def test_max_supernode_setting(gpu, prec):
+    # the analysis honours the supernode width limit (more, narrower supernodes
+    # and more tree levels at 64 columns than at the default) and the solve on
+    # that partition matches the oracle factored on the same partition; widths
+    # outside 64..2048 are rejected
+    _say("max supernode 64 vs default, prec %d" % prec)
+    beam, vals, b = G.small_beam(26)
+    ro, ci = beam.rowOffs, beam.colIdxs
+    shapes = {}
+    for width in (None, 64):
+        s = MI355XSolver(precision=prec)
+        try:
+            if width:
+                s.setMaxSupernode(width)
+            s.analyzeArrays(vals, ci, ro, beam.n, Matrix.SYMMETRIC)
+            s.factor(vals)
+            x = np.zeros(beam.n)
+            s.solve(x, b)
+            sym = s.symbolic()
+            k = np.diff(sym["sup_ptr"])
+            shapes[width] = (len(k), int(k.max()), s.info()["nlevels"])
+            if width:
+                assert k.max() <= width
+            xo = O.Factor(ro, ci, vals, perm=sym["perm"], sup_ptr=sym["sup_ptr"]).solve(b)
+            assert rel_err(x, xo) <= (1e-8 if prec == KKT_PREC_FP64 else 1e-7)
+            assert rel_residual(ro, ci, vals, x, b) <= 1e-8
+        finally:
+            s.dispose()
+    assert shapes[64][0] > shapes[None][0] and shapes[64][2] >= shapes[None][2], shapes
+    s = MI355XSolver()
+    try:
+        for bad in (32, 4096):
+            with pytest.raises(NumericalException):
+                s.setMaxSupernode(bad)
+    finally:
+        s.dispose()
