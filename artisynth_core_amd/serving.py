@@ -53,7 +53,7 @@ class Request:
 
 
 class SolveServer:
-    def __init__(self, budget_bytes=200 << 30, nworkers=8, precision=KKT_PREC_FP32_REFINE,
+    def __init__(self, budget_bytes=200 << 30, nworkers=12, precision=KKT_PREC_FP32_REFINE,
                  tol_exp=10, max_iter=30, refine_tol=1e-10):
         self.L = lib()
         cfg = KktServeConfig()

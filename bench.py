@@ -242,7 +242,7 @@ def main():
     ap.add_argument("--ninst", type=int, default=64)
     ap.add_argument("--nreq", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=2000.0, help="serving: offered requests/s")
-    ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--workers", type=int, default=12)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--eps", type=float, default=3e-3,
                     help="serving: per-step relative drift of every matrix entry (random walk)")

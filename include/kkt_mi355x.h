@@ -380,7 +380,7 @@ const char* kkt_batch_last_error(kkt_batch* b);
 typedef struct kkt_server kkt_server;
 typedef struct kkt_serve_config {
   long long budget_bytes; /* device memory the resident instances may hold (default 200 GiB) */
-  int nworkers;           /* worker threads, 1..64 (default 8) */
+  int nworkers;           /* worker threads, 1..64 (default 12) */
   int precision;          /* KKT_PREC_* of every instance (default FP32 + refinement) */
   int tol_exp;            /* iterative solves to 10^-tol_exp relative residual (default 10) */
   int max_iter;           /* BiCGStab iterations before a refactor (default 30) */
