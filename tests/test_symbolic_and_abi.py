@@ -254,7 +254,7 @@ def test_serving_loop_mechanics_without_gpu():
     L = _lib.lib()
     cfg = _lib.KktServeConfig()
     L.kkt_serve_default_config(ctypes.byref(cfg))
-    assert (cfg.budget_bytes, cfg.nworkers, cfg.max_iter, cfg.tol_exp) == (200 << 30, 8, 30, 10)
+    assert (cfg.budget_bytes, cfg.nworkers, cfg.max_iter, cfg.tol_exp) == (200 << 30, 12, 30, 10)
     bad = _lib.KktServeConfig.from_buffer_copy(cfg)
     bad.nworkers = 0
     assert not L.kkt_serve_create(ctypes.byref(bad))
