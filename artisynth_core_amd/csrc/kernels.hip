@@ -1056,6 +1056,28 @@ __device__ inline T slot_sum(const T* __restrict__ sl, int nch, int m, T a) {
   return a;
 }
 
+// the same for the four positions q0 + 256 j (j < 4) below qend: a[j] += the
+// children's entries at position q0 + 256 j, in ascending child order; every
+// load of a pass (four positions x four children) issues before the first add
+template <typename T>
+__device__ inline void slot_sum4(const T* __restrict__ sb, int nch, int m, int q0, int qend, T a[4]) {
+  for (int c0 = 0; c0 < nch; c0 += 4) {
+    T v[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const int q = q0 + 256 * j;
+        v[j][c] = (q < qend && c0 + c < nch) ? sb[q + (size_t)(c0 + c) * m] : (T)0;
+      }
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        if (c0 + c < nch) a[j] += v[j][c];
+  }
+}
+
 // forward, a whole small front (k <= FWD_SMALL_K, m <= FWD_SMALL_M) in one
 // workgroup: one thread per position p gathers f_p = b_p + the children's update
 // entries at p (summed in ascending order) into LDS in pivot order, then one
@@ -1234,10 +1256,30 @@ __global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__
   __shared__ T gb[FWD_GROUP_ROWS];
   {
     const T* sb = u + it.soff;
-    for (int q = threadIdx.x; q < k; q += 256)
-      fs[d.ilperm[first + q]] = slot_sum(sb + q, it.nch, m, bp[first + q]);
+    // four positions per thread and pass, their right-hand side, permutation and
+    // slot loads in flight together
+    for (int q0 = threadIdx.x; q0 < k; q0 += 1024) {
+      T a[4];
+      int ip[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int q = q0 + 256 * j;
+        a[j] = q < k ? bp[first + q] : (T)0;
+        ip[j] = q < k ? d.ilperm[first + q] : 0;
+      }
+      slot_sum4(sb, it.nch, m, q0, k, a);
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (q0 + 256 * j < k) fs[ip[j]] = a[j];
+    }
     const int rlo = max(row0, k), rhi = min(row0 + nrg * rpg, m);
-    for (int q = rlo + threadIdx.x; q < rhi; q += 256) gb[q - row0] = slot_sum(sb + q, it.nch, m, (T)0);
+    for (int q0 = rlo + threadIdx.x; q0 < rhi; q0 += 1024) {
+      T a[4] = {(T)0, (T)0, (T)0, (T)0};
+      slot_sum4(sb, it.nch, m, q0, rhi, a);
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (q0 + 256 * j < rhi) gb[q0 + 256 * j - row0] = a[j];
+    }
   }
   __syncthreads();
   for (int rg = 0; rg < nrg; rg++) {
