@@ -348,6 +348,59 @@ def test_serving_alone_equals_mix(gpu):
     srv.shutdown()
 
 
+@pytest.mark.parametrize("prec", [KKT_PREC_FP32_REFINE, KKT_PREC_FP64])
+def test_serving_device_inputs_equal_host_inputs(gpu, prec):
+    # kkt_serve_submit_device (values, right-hand side and solution in HBM, read
+    # in place) gives bitwise the results of kkt_serve_submit (host arrays copied
+    # at submission) for the same instance and step sequence, in both precisions
+    _say("serving device vs host inputs prec=%d" % prec)
+    import torch
+    from artisynth_core_amd.serving import SolveServer
+    dev = torch.device("cuda:0")
+    inst = [G.serving_instance("tet", 1500, 71), G.serving_instance("rigid", 1200, 72)]
+    steps = 3
+    seq = {i: [(G.perturbed_values(inst[i][2], s_, 50 + i, eps=3e-2), inst[i][3]) for s_ in range(steps)]
+           for i in range(len(inst))}
+    host, device = {}, {}
+    srv = SolveServer(nworkers=2, precision=prec)
+    try:
+        for i in range(len(inst)):
+            srv.register(i, inst[i][0], inst[i][1], inst[i][2])
+        for i in range(len(inst)):
+            rs = [srv.submit(i, v, b) for v, b in seq[i]]
+            host[i] = [(r.wait().x, r.kind, r.error) for r in rs]
+    finally:
+        srv.shutdown()
+    srv = SolveServer(nworkers=2, precision=prec)
+    try:
+        for i in range(len(inst)):
+            srv.register(i, inst[i][0], inst[i][1], inst[i][2])
+        for i in range(len(inst)):
+            keep = []
+            rs = []
+            for v, b in seq[i]:
+                dv = torch.from_numpy(np.ascontiguousarray(v)).to(dev)
+                db = torch.from_numpy(np.ascontiguousarray(b)).to(dev)
+                dx = torch.zeros(len(b), dtype=torch.float64, device=dev)
+                keep.append((dv, db, dx))
+                rs.append(srv.submit_device(i, dv.data_ptr(), db.data_ptr(), dx.data_ptr(), keep=(dv, db, dx)))
+            device[i] = []
+            for r, (dv, db, dx) in zip(rs, keep):
+                r.wait()
+                device[i].append((dx.cpu().numpy(), r.kind, r.error))
+    finally:
+        srv.shutdown()
+    for i in range(len(inst)):
+        kinds = [k for _, k, _ in host[i]]
+        assert "direct" in kinds  # the first step of an instance factors
+        for (xh, kh, eh), (xd, kd, ed) in zip(host[i], device[i]):
+            assert eh is None and ed is None, (eh, ed)
+            assert kh == kd
+            assert np.array_equal(xh, xd)
+        for s_, (xh, _, _) in enumerate(host[i]):
+            assert rel_residual(inst[i][0], inst[i][1], seq[i][s_][0], xh, seq[i][s_][1]) <= 1e-8
+
+
 def test_serving_eviction_and_readmission(gpu):
     # 18 instances under a budget that holds about four of them: requests evict
     # idle instances (least recently used first) and re-admit evicted ones; every
