@@ -997,7 +997,7 @@ def cpu_baseline(beam, V, B, x64, x32):
            "sample": "MKL PARDISO (reference settings: mtype=-2, METIS, Bunch-Kaufman, perturbation 1e-8, "
                      "scaling + matching, 1 refinement step) on the same %d-step sequence with the factorAndSolve "
                      "policy (phase 22+33, then phase-33 CGS with the stale factor), %d repetitions on %d threads; "
-                     "10 direct factor+solve repetitions; one thread: 2 direct + 5 hybrid steps"
+                     "20 direct factor+solve repetitions; one thread: 2 direct + 5 hybrid steps"
                      % (d["steps"], d["reps"], d["threads"])}
     out.update(d)
     return out

@@ -86,7 +86,7 @@ def main(inp, outp):
         seq_lat += lat
         seq_kinds += kinds
     direct = []
-    for r in range(10):
+    for r in range(20):
         t0 = time.perf_counter()
         p.factor(V[0])
         p.solve(B[0])
