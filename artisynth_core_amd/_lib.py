@@ -30,7 +30,7 @@ class KktInfo(ctypes.Structure):
         ("refine_steps", ctypes.c_int), ("fp64_fallbacks", ctypes.c_int),
         ("last_rel_residual", ctypes.c_double), ("pivoted_blocks", ctypes.c_int),
         ("device_bytes", ctypes.c_longlong), ("factored_bytes", ctypes.c_longlong),
-        ("fp64_fallback_bytes", ctypes.c_longlong),
+        ("fp64_fallback_bytes", ctypes.c_longlong), ("precond_half", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -123,6 +123,7 @@ SIGNATURES = [
     ("kkt_set_graphs", _I, [_P, _I]),
     ("kkt_set_ordering", _I, [_P, _I]),
     ("kkt_set_max_supernode", _I, [_P, _I]),
+    ("kkt_set_iterative_half", _I, [_P, _I]),
     ("kkt_get_info", _I, [_P, ctypes.POINTER(KktInfo)]),
     ("kkt_get_symbolic", _I, [_P, _IP, _IP, _IP, _LLP, _IP]),
     ("kkt_verify_device_structure", _I, [_P]),

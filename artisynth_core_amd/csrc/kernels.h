@@ -102,6 +102,7 @@ __host__ __device__ inline long long pcol_off(int c, int m) {
 struct FactorCounters {  // device ints
   int npos, nneg, npert, n2x2;
   int npivoted;  // diagonal blocks that needed the full pivoting loop
+  int half_overflow;  // an FP32 solve entry too large for the half-precision copy
 };
 
 template <typename T>
@@ -134,16 +135,20 @@ void launch_gform(const int4* items, int nitems, const DevSym& d, T* F, hipStrea
 // (supernode, block of 4 front columns))
 template <typename T>
 void launch_pack_solve(const int2* items, int nitems, const DevSym& d, const T* F, T* PF, T* PB, hipStream_t st);
+// half-precision copy of n FP32 packed entries (ctr->half_overflow set when one
+// does not fit)
+void launch_pack_half(long long n, const float* in, _Float16* out, FactorCounters* ctr, hipStream_t st);
 // level-scheduled solves (vectors in the front precision T) on the packed
-// arrays: PF forward rows, PB backward columns, SL the forward update slots
-template <typename T>
-void launch_fwd_level(const SolveItem* items, int nitems, const DevSym& d, const T* PF, const T* bp,
+// arrays: PF forward rows, PB backward columns, SL the forward update slots;
+// the packed entries in S (T, or _Float16 for T = float)
+template <typename T, typename S>
+void launch_fwd_level(const SolveItem* items, int nitems, const DevSym& d, const S* PF, const T* bp,
                       T* fp, T* y, T* SL, hipStream_t st);
 template <typename T>
 void launch_dsolve(int n, const T* E0, const T* E1, const signed char* piv, const int* wpos,
                    const T* y, T* wv, hipStream_t st);
-template <typename T>
-void launch_bwd_cols(const SolveItem* items, int nitems, const DevSym& d, const T* PB, T* wv,
+template <typename T, typename S>
+void launch_bwd_cols(const SolveItem* items, int nitems, const DevSym& d, const S* PB, T* wv,
                      T* xp, hipStream_t st);
 
 // the same for nr <= SOLVE_NRHS right-hand sides at once (vector j of a set at
@@ -173,6 +178,10 @@ void launch_gather_perm(int n, const int* perm, const TI* in, TO* out, hipStream
 // out[perm[j]] = (TO) in[j]  (scatter)
 template <typename TI, typename TO>
 void launch_scatter_perm(int n, const int* perm, const TI* in, TO* out, hipStream_t st);
+// gather in pivot order with BiCGStab's p = r + beta (p - omega v) fused in (p updated in place)
+template <typename TO>
+void launch_gather_bicg_p(int n, const int* perm, const double* sc, const double* r, const double* v, double* p,
+                          TO* out, hipStream_t st);
 
 // sparse symmetric matrix (full CSR copy) in FP64
 void launch_expand_vals(long long nnzf, const int* map, const double* vals, double* valsf,
@@ -184,7 +193,10 @@ void launch_spmv(int n, const int* rowptr, const int* col, const double* val, co
                  double* part1 = nullptr, double* part2 = nullptr);
 int spmv_blocks(int n);
 int vec_blocks(int n);
-void launch_reduce2(const double* p1, const double* p2, int np, double* out1, double* out2, hipStream_t st);
+// out1 / out2 = the sums of np partials (fixed order); with sc, the next BiCGStab
+// iteration's scalars are formed from the BiCGStab slots, out2 being the next rho
+void launch_reduce2(const double* p1, const double* p2, int np, double* out1, double* out2, hipStream_t st,
+                    double* sc = nullptr);
 // Y_j = B_j - A X_j and out[j] = (A_j, B_j) for nr <= SOLVE_NRHS columns (column j at
 // j * ld), each column's value equal to launch_spmv (mode 1) / launch_dot's;
 // partial holds dot_m_partials() doubles
@@ -232,7 +244,6 @@ void launch_contrib_dilational3_elem(int nelems, const int* nodeOff, const int* 
 void launch_seg_scatter(int nuniq, const int* useg, const int* uslot, const int* order,
                         const double* c, double scale, double* vals, hipStream_t st);
 
-void launch_bicg_p(int n, const double* sc, const double* r, const double* v, double* p, hipStream_t st);
 // s = r - alpha v, with per-block partials of |s|^2 (vec_blocks(n))
 void launch_bicg_s(int n, const double* sc, const double* r, const double* v, double* s_, double* part,
                    hipStream_t st);
@@ -241,7 +252,7 @@ void launch_bicg_xr(int n, const double* sc, const double* y, const double* z, c
                     const double* t, double* x, double* r, const double* rh, double* part1, double* part2,
                     hipStream_t st);
 void launch_bicg_xa(int n, const double* sc, const double* y, double* x, hipStream_t st);
-void launch_bicg_scalars(double* sc, int init, hipStream_t st);
+void launch_bicg_init(double* sc, hipStream_t st);
 void launch_fill(long long n, double v, double* x, hipStream_t st);
 // zeroes the totals and the nr per-component counters (four per component:
 // positive, negative, perturbed pivots, 2x2 pivots)

@@ -985,10 +985,27 @@ template <>
 struct V16<double> {
   typedef double v __attribute__((ext_vector_type(2)));
 };
+template <>
+struct V16<_Float16> {
+  typedef _Float16 v __attribute__((ext_vector_type(8)));
+};
 template <typename T>
 __device__ inline typename V16<T>::v ldv(const T* p) {
   typename V16<T>::v r;
   __builtin_memcpy(&r, p, 16);
+  return r;
+}
+// W consecutive entries of a vector of T: the operand that pairs with one
+// 16-byte chunk of the factor's storage type S (W = 16 / sizeof(S); one 16-byte
+// load when S is T, two for half-precision storage of an FP32 solve)
+template <typename T, int W>
+struct VecW {
+  T e[W];
+};
+template <typename T, int W>
+__device__ inline VecW<T, W> ldw(const T* p) {
+  VecW<T, W> r;
+  __builtin_memcpy(&r, p, W * sizeof(T));
   return r;
 }
 template <typename T>
@@ -997,31 +1014,34 @@ __device__ inline T vdot(typename V16<T>::v a, typename V16<T>::v b, T acc) {
   for (int w = 0; w < (int)(16 / sizeof(T)); w++) acc = fma(a[w], b[w], acc);
   return acc;
 }
-// Partial sum_{e < len} a[e] b[e] of lane sl of a segment of seg lanes.  The lane
-// takes the 16-byte chunks sl, sl + seg, ... in groups of G whose loads all issue
-// before the first use; the last chunk may run up to W-1 elements past len (the
-// callers' arrays are followed by mapped memory), and every element at or past
-// len is dropped by a select, so whatever it holds never reaches the sum.
-template <typename T, int G = 4>
-__device__ inline T seg_dot(const T* __restrict__ a, const T* __restrict__ b, int len, int sl,
+// Partial sum_{e < len} a[e] b[e] of lane sl of a segment of seg lanes, a in the
+// storage type S of the factor (T, or half precision for an FP32 solve; products
+// and sums in T).  The lane takes the 16-byte chunks of a (W = 16 / sizeof(S)
+// entries) sl, sl + seg, ... in groups of G whose loads all issue before the
+// first use; the last chunk may run up to W-1 elements past len (the callers'
+// arrays are followed by mapped memory), and every element at or past len is
+// dropped by a select, so whatever it holds never reaches the sum.
+template <typename T, typename S = T, int G = 4>
+__device__ inline T seg_dot(const S* __restrict__ a, const T* __restrict__ b, int len, int sl,
                             int seg) {
-  constexpr int W = 16 / sizeof(T);
-  typedef typename V16<T>::v vt;
+  constexpr int W = 16 / sizeof(S);
+  typedef typename V16<S>::v vs;
   const int nc = (len + W - 1) / W;
   T acc = (T)0;
   for (int q0 = sl; q0 < nc; q0 += G * seg) {
-    vt x[G], y[G];
+    vs x[G];
+    VecW<T, W> y[G];
 #pragma unroll
     for (int u = 0; u < G; u++) {
       const int q = q0 + u * seg, qc = q < nc ? q : 0;
       x[u] = ldv(a + qc * W);
-      y[u] = ldv(b + qc * W);
+      y[u] = ldw<T, W>(b + qc * W);
     }
 #pragma unroll
     for (int u = 0; u < G; u++)
 #pragma unroll
       for (int w = 0; w < W; w++) {
-        const T t = fma(x[u][w], y[u][w], acc);
+        const T t = fma((T)x[u][w], y[u].e[w], acc);
         acc = ((q0 + u * seg) * W + w < len) ? t : acc;
       }
   }
@@ -1085,8 +1105,8 @@ __device__ inline void slot_sum4(const T* __restrict__ sb, int nch, int m, int q
 // f_i - G(i,:) f goes to the supernode's slot in its parent's block, G(i,:) read
 // from the row's packed forward entries (contiguous per thread)
 static_assert(FWD_SMALL_M == 256, "one thread per row of a small front");
-template <typename T>
-__device__ inline void fwd_small_front(const SolveItem& it, const DevSym& d, const T* __restrict__ F,
+template <typename T, typename S>
+__device__ inline void fwd_small_front(const SolveItem& it, const DevSym& d, const S* __restrict__ F,
                                        const T* __restrict__ bp, T* __restrict__ y, T* __restrict__ u) {
   __shared__ T g[256];
   __shared__ __align__(16) T fs[FWD_SMALL_K + 4];  // chunk loads may run 3 past k
@@ -1098,10 +1118,10 @@ __device__ inline void fwd_small_front(const SolveItem& it, const DevSym& d, con
   // rest after it (FP64: half, so the kernel keeps more waves resident).
   constexpr int HOIST = sizeof(T) == 8 ? FWD_SMALL_K / 2 : FWD_SMALL_K;
   const int len = (p < m) ? min(p, k) : 0;
-  const T* rowp = F + it.poff + (p < m ? prow_off(p, k) : 0);
+  const S* rowp = F + it.poff + (p < m ? prow_off(p, k) : 0);
   T v[FWD_SMALL_K];
 #pragma unroll
-  for (int l = 0; l < HOIST; l++) v[l] = rowp[l < len ? l : 0];
+  for (int l = 0; l < HOIST; l++) v[l] = (T)rowp[l < len ? l : 0];
   // destination of a contribution row's update entry in the parent's block
   const int dr = (p >= k && p < m) ? d.rel[it.roff + p - k] : 0;
   T acc = (T)0;
@@ -1119,7 +1139,7 @@ __device__ inline void fwd_small_front(const SolveItem& it, const DevSym& d, con
   __syncthreads();
   if (p < m) {
 #pragma unroll
-    for (int l = HOIST; l < FWD_SMALL_K; l++) v[l] = rowp[l < len ? l : 0];
+    for (int l = HOIST; l < FWD_SMALL_K; l++) v[l] = (T)rowp[l < len ? l : 0];
     T acc = (T)0;
 #pragma unroll
     for (int l = 0; l < FWD_SMALL_K; l++) {
@@ -1164,15 +1184,15 @@ __device__ inline void fwd_gather_block(const SolveItem& it, const DevSym& d, co
 // forward rows of a large front (two-launch form, second launch): products from
 // the upper copy (column i, rows l < min(i, k), contiguous), y_i = f_i + G(i,:) f
 // (i < k) or u_{i-k} -= G(i,:) f (i >= k), f = fp from the gather launch
-template <typename T>
-__device__ inline void fwd_rows_block(const SolveItem& it, const DevSym& d, const T* __restrict__ F,
+template <typename T, typename S>
+__device__ inline void fwd_rows_block(const SolveItem& it, const DevSym& d, const S* __restrict__ F,
                                       const T* __restrict__ fp, T* __restrict__ y, T* __restrict__ u) {
   const int lg = it.lg;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int seg = 1 << lg, sl = lane & (seg - 1);
   const int i = (it.blk * 4 + wv) * (64 >> lg) + (lane >> lg);
   const int k = it.k, m = it.m, first = it.first;
-  const T* col = F + it.poff + (i < m ? prow_off(i, k) : 0);
+  const S* col = F + it.poff + (i < m ? prow_off(i, k) : 0);
   const T* f = fp + first;
   const bool own = sl == 0 && i < m;
   T base = (T)0;
@@ -1180,7 +1200,7 @@ __device__ inline void fwd_rows_block(const SolveItem& it, const DevSym& d, cons
   if (own && i >= k) dst = u + it.sdst + d.rel[it.roff + i - k];
   if (own) base = (i < k) ? f[i] : *dst;
   const int len = (i < m) ? min(i, k) : 0;
-  const T acc = seg_sum(seg_dot<T>(col, f, len, sl, seg), lg);
+  const T acc = seg_sum(seg_dot<T, S>(col, f, len, sl, seg), lg);
   if (own) {
     if (i < k) y[first + i] = base + acc;
     else *dst = base - acc;
@@ -1203,15 +1223,15 @@ __device__ inline void fwd_rows_block(const SolveItem& it, const DevSym& d, cons
 // Every sum adds the children's entries in ascending order and every row
 // product runs in the seg_dot order, so both forms give bitwise the same y and
 // update entries.
-template <typename T>
+template <typename T, typename S>
 __global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__ items, DevSym d,
-                                                   const T* __restrict__ F,
+                                                   const S* __restrict__ F,
                                                    const T* __restrict__ bp, T* __restrict__ fp,
                                                    T* __restrict__ y, T* __restrict__ u) {
-  __shared__ __align__(16) T fs[KMAX + 4];  // chunk loads may run 3 past k
+  __shared__ __align__(16) T fs[KMAX + 8];  // chunk loads may run 7 past k
   const SolveItem it = items[blockIdx.x];
   if (it.kind == 0) {
-    fwd_small_front<T>(it, d, F, bp, y, u);
+    fwd_small_front<T, S>(it, d, F, bp, y, u);
     return;
   }
   if (it.kind == 1) {
@@ -1219,7 +1239,7 @@ __global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__
     return;
   }
   if (it.kind == 3) {
-    fwd_rows_block<T>(it, d, F, fp, y, u);
+    fwd_rows_block<T, S>(it, d, F, fp, y, u);
     return;
   }
   // kind 2: nrg = it.pad row groups of rpg = 4 * (64 >> lg) rows each (rows
@@ -1233,14 +1253,15 @@ __global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__
   const int myrow = wv * (64 >> lg) + (lane >> lg);  // row within a group
   const int k = it.k, m = it.m, first = it.first;
   // the first PF chunks of this lane's part of its row in group 0 (the seg_dot
-  // order: chunks sl, sl + seg, ..., 16 bytes each), loaded before the gather;
-  // the chunks are added in ascending order whatever the grouping
-  constexpr int W = 16 / sizeof(T), GC = 4, PF = 3;
-  typedef typename V16<T>::v vt;
+  // order: chunks sl, sl + seg, ..., 16 bytes of S each), loaded before the
+  // gather; the chunks are added in ascending order whatever the grouping
+  constexpr int W = 16 / sizeof(S), GC = 4, PF = 3;
+  typedef typename V16<S>::v vt;
+  typedef VecW<T, W> vw;
   vt x0[PF];
   auto prefetch = [&](int rg) {
     const int i = row0 + rg * rpg + myrow;
-    const T* col = F + it.poff + (i < m ? prow_off(i, k) : 0);
+    const S* col = F + it.poff + (i < m ? prow_off(i, k) : 0);
     const int len = (i < m) ? min(i, k) : 0;
     const int nc = (len + W - 1) / W;
 #pragma unroll
@@ -1285,7 +1306,7 @@ __global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__
   for (int rg = 0; rg < nrg; rg++) {
     const int i = row0 + rg * rpg + myrow;
     const bool own = sl == 0 && i < m;
-    const T* col = F + it.poff + (i < m ? prow_off(i, k) : 0);
+    const S* col = F + it.poff + (i < m ? prow_off(i, k) : 0);
     const int len = (i < m) ? min(i, k) : 0;
     const int nc = (len + W - 1) / W;
     // a contribution row's slot in the parent's block (loaded before the product)
@@ -1300,26 +1321,27 @@ __global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__
 #pragma unroll
     for (int q = 0; q < PF; q++) {
       const int c = sl + q * seg;
-      const vt yv = ldv(fs + (c < nc ? c : 0) * W);
+      const vw yv = ldw<T, W>(fs + (c < nc ? c : 0) * W);
 #pragma unroll
       for (int w = 0; w < W; w++) {
-        const T t = fma(xc[q][w], yv[w], acc);
+        const T t = fma((T)xc[q][w], yv.e[w], acc);
         acc = (c * W + w < len) ? t : acc;
       }
     }
     for (int q0 = sl + PF * seg; q0 < nc; q0 += GC * seg) {
-      vt x[GC], yv[GC];
+      vt x[GC];
+      vw yv[GC];
 #pragma unroll
       for (int q = 0; q < GC; q++) {
         const int c = q0 + q * seg, cc = c < nc ? c : 0;
         x[q] = ldv(col + cc * W);
-        yv[q] = ldv(fs + cc * W);
+        yv[q] = ldw<T, W>(fs + cc * W);
       }
 #pragma unroll
       for (int q = 0; q < GC; q++)
 #pragma unroll
         for (int w = 0; w < W; w++) {
-          const T t = fma(x[q][w], yv[q][w], acc);
+          const T t = fma((T)x[q][w], yv[q].e[w], acc);
           acc = ((q0 + q * seg) * W + w < len) ? t : acc;
         }
     }
@@ -1354,6 +1376,18 @@ __global__ __launch_bounds__(256) void k_pack_solve(const int2* __restrict__ ite
   }
 }
 
+// half-precision copy of the FP32 packed solve arrays (the iterative solve's
+// preconditioner reads it); ctr->half_overflow is set when an entry is too large
+// for half precision, so the copy is not used for that factor
+__global__ void k_pack_half(long long n, const float* __restrict__ in, _Float16* __restrict__ out,
+                            FactorCounters* __restrict__ ctr) {
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= n) return;
+  const float v = in[e];
+  out[e] = (_Float16)v;
+  if (!(fabsf(v) < 65504.0f)) ctr->half_overflow = 1;
+}
+
 // z = D^-1 y over all pivots: z_g = E0_g y_g + E1_g y_partner, stored at the
 // pivot's entry of the backward vector
 template <typename T>
@@ -1377,9 +1411,9 @@ __global__ __launch_bounds__(256) void k_dsolve(int n, const T* __restrict__ E0,
 // contiguous dot product, like a forward row.
 // A segment of 2^lg lanes per column (lg <= 8: a long column takes two or four
 // waves); item = (s, block of 256 >> lg columns, lg)
-template <typename T>
+template <typename T, typename S>
 __global__ __launch_bounds__(256) void k_bwd_cols(const SolveItem* __restrict__ items, DevSym d,
-                                                  const T* __restrict__ F, T* wv,
+                                                  const S* __restrict__ F, T* wv,
                                                   T* __restrict__ xp) {
   __shared__ T part[4];
   const SolveItem it = items[blockIdx.x];
@@ -1394,10 +1428,10 @@ __global__ __launch_bounds__(256) void k_bwd_cols(const SolveItem* __restrict__ 
   const int lp = d.lperm[first + cq];
   const T* w = wv + it.voff;
   const T zc = w[cq];
-  const T* col = F + it.poff + pcol_off(cq, m);
+  const S* col = F + it.poff + pcol_off(cq, m);
   // every lane of a segment ends with the same sum (xor butterfly; the wave sums
   // of a column spanning 2 or 4 waves are combined in order)
-  T acc = seg_sum(seg_dot<T>(col, w + cq + 1, ok ? m - c - 1 : 0, sl, seg), lg < 6 ? lg : 6);
+  T acc = seg_sum(seg_dot<T, S>(col, w + cq + 1, ok ? m - c - 1 : 0, sl, seg), lg < 6 ? lg : 6);
   if (lg > 6) {
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
     __syncthreads();
@@ -1863,6 +1897,21 @@ __global__ void k_scatter_perm(int n, const int* __restrict__ perm, const TI* __
   const int j = blockIdx.x * 256 + threadIdx.x;
   if (j < n) out[perm[j]] = (TO)in[j];
 }
+// the gather of a BiCGStab preconditioner application with the search-direction
+// update of k_bicg_p fused in: for i = perm[j], p_i = r_i + beta (p_i - omega v_i),
+// written back to p and, in the front precision, to out[j]
+template <typename TO>
+__global__ void k_gather_bicg_p(int n, const int* __restrict__ perm, const double* __restrict__ sc,
+                                const double* __restrict__ r, const double* __restrict__ v,
+                                double* __restrict__ p, TO* __restrict__ out) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const int i = perm[j];
+  const double beta = (sc[0] / sc[1]) * (sc[2] / sc[3]);
+  const double pi = r[i] + beta * (p[i] - sc[3] * v[i]);
+  p[i] = pi;
+  out[j] = (TO)pi;
+}
 
 __global__ void k_expand_vals(long long nnzf, const int* __restrict__ map,
                               const double* __restrict__ vals, double* __restrict__ valsf) {
@@ -1963,7 +2012,15 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double* __restrict__
                                                       double* __restrict__ out, int is_max) {
   __shared__ double sm[4];
   double acc = 0.0;
-  for (int i = threadIdx.x; i < np; i += 256) acc = is_max ? fmax(acc, partial[i]) : acc + partial[i];
+  // thread t takes entries t, t + 256, ... in turn; loads issue 8 at a time
+  for (int i0 = threadIdx.x; i0 < np; i0 += 256 * 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = (i0 + 256 * u < np) ? partial[i0 + 256 * u] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (i0 + 256 * u < np) acc = is_max ? fmax(acc, v[u]) : acc + v[u];
+  }
   if (is_max) acc = wave_max(acc); else acc = wave_sum(acc);
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
   __syncthreads();
@@ -2039,16 +2096,35 @@ __global__ __launch_bounds__(256) void k_reduce_final_m(const double* __restrict
   __syncthreads();
   if (threadIdx.x == 0) out[blockIdx.x] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
 }
-// out1 = sum of p1[0..np), out2 = sum of p2[0..np) (when given), fixed order
+// One workgroup: out1 = sum of p1[0, np), out2 = sum of p2[0, np) in a fixed
+// order (thread t adds entries t, t + 256, ... in turn, then the wave sums, then
+// the four waves in order).  A thread's loads issue RED_AHEAD at a time before its
+// adds, so the partials cost a few memory round trips instead of one per entry.
+// With sc, thread 0 then starts the next BiCGStab iteration's scalars (slots of
+// the BiCGStab kernels below): alpha = rho / (r_hat, v), omega = (t, s) / (t, t),
+// rho_prev = rho, rho = out2's value (the caller passes out2 = sc + 10).
+constexpr int RED_AHEAD = 8;
 __global__ __launch_bounds__(256) void k_reduce2(const double* __restrict__ p1,
                                                  const double* __restrict__ p2, int np,
                                                  double* __restrict__ out1,
-                                                 double* __restrict__ out2) {
+                                                 double* __restrict__ out2, double* __restrict__ sc) {
   __shared__ double sm[2][4];
   double a = 0.0, c = 0.0;
-  for (int i = threadIdx.x; i < np; i += 256) {
-    a += p1[i];
-    if (p2) c += p2[i];
+  for (int i0 = threadIdx.x; i0 < np; i0 += 256 * RED_AHEAD) {
+    double va[RED_AHEAD], vc[RED_AHEAD];
+#pragma unroll
+    for (int u = 0; u < RED_AHEAD; u++) {
+      const int i = i0 + 256 * u;
+      va[u] = (i < np) ? p1[i] : 0.0;
+      vc[u] = (p2 && i < np) ? p2[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < RED_AHEAD; u++) {
+      if (i0 + 256 * u < np) {
+        a += va[u];
+        c += vc[u];
+      }
+    }
   }
   a = wave_sum(a);
   c = wave_sum(c);
@@ -2058,8 +2134,17 @@ __global__ __launch_bounds__(256) void k_reduce2(const double* __restrict__ p1,
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    *out1 = (sm[0][0] + sm[0][1]) + (sm[0][2] + sm[0][3]);
-    if (out2) *out2 = (sm[1][0] + sm[1][1]) + (sm[1][2] + sm[1][3]);
+    const double s1 = (sm[0][0] + sm[0][1]) + (sm[0][2] + sm[0][3]);
+    const double s2 = (sm[1][0] + sm[1][1]) + (sm[1][2] + sm[1][3]);
+    *out1 = s1;
+    if (out2) *out2 = s2;
+    if (sc) {
+      const double rho = sc[0];
+      sc[2] = rho / sc[4];
+      sc[3] = sc[6] / sc[7];
+      sc[1] = rho;
+      sc[0] = s2;
+    }
   }
 }
 // one block's sum of v over its 256 threads into part[blockIdx.x] (fixed order)
@@ -2098,14 +2183,6 @@ __global__ void k_axpbypcz(int n, double a, const double* __restrict__ x, double
 // slots: 0 rho, 1 rho_prev, 2 alpha, 3 omega, 4 (r_hat,v), 5 |s|^2, 6 (t,s), 7 (t,t),
 // 8 |r|^2, 10 the next rho (r_hat, r)
 // ------------------------------------------------------------------------
-// p = r + beta (p - omega v), beta = (rho/rho_prev)(alpha/omega)
-__global__ void k_bicg_p(int n, const double* __restrict__ sc, const double* __restrict__ r,
-                         const double* __restrict__ v, double* __restrict__ p) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const double beta = (sc[0] / sc[1]) * (sc[2] / sc[3]);
-  p[i] = r[i] + beta * (p[i] - sc[3] * v[i]);
-}
 // s = r - alpha v with alpha = rho / (r_hat, v); part[block] = sum of s_i^2
 __global__ __launch_bounds__(256) void k_bicg_s(int n, const double* __restrict__ sc,
                                                 const double* __restrict__ r,
@@ -2151,15 +2228,14 @@ __global__ void k_bicg_xa(int n, const double* __restrict__ sc, const double* __
   if (i >= n) return;
   x[i] += (sc[0] / sc[4]) * y[i];
 }
-// start of an iteration: alpha, omega and rho_prev of the last one, rho from the
-// last one's fused (r_hat, r); or the initial values (rho = |r0|^2, r_hat = r0)
-__global__ void k_bicg_scalars(double* __restrict__ sc, int init) {
+// the first iteration's scalars: rho = |r0|^2 (r_hat = r0), rho_prev = alpha =
+// omega = 1 (later iterations' come from k_reduce2 with sc)
+__global__ void k_bicg_init(double* __restrict__ sc) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (init) { sc[0] = sc[8]; sc[1] = 1.0; sc[2] = 1.0; sc[3] = 1.0; return; }
-  sc[2] = sc[0] / sc[4];
-  sc[3] = sc[6] / sc[7];
-  sc[1] = sc[0];
-  sc[0] = sc[10];
+  sc[0] = sc[8];
+  sc[1] = 1.0;
+  sc[2] = 1.0;
+  sc[3] = 1.0;
 }
 __global__ void k_fill(long long n, double v, double* __restrict__ x) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -2167,7 +2243,7 @@ __global__ void k_fill(long long n, double v, double* __restrict__ x) {
 }
 __global__ void k_zero_counters(FactorCounters* c, int* __restrict__ rctr, int nr) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i == 0) { c->npos = 0; c->nneg = 0; c->npert = 0; c->n2x2 = 0; c->npivoted = 0; }
+  if (i == 0) { c->npos = 0; c->nneg = 0; c->npert = 0; c->n2x2 = 0; c->npivoted = 0; c->half_overflow = 0; }
   if (i < nr) rctr[i] = 0;
 }
 // max |v| over each chunk [ca[q], cb[q]) of the value array (one workgroup each)
@@ -2275,11 +2351,11 @@ template <typename T>
 void launch_pack_solve(const int2* items, int nitems, const DevSym& d, const T* F, T* PF, T* PB, hipStream_t st) {
   if (nitems > 0) hipLaunchKernelGGL(k_pack_solve<T>, dim3(nitems), dim3(256), 0, st, items, d, F, PF, PB);
 }
-template <typename T>
-void launch_fwd_level(const SolveItem* items, int nitems, const DevSym& d, const T* F, const T* bp, T* fp,
+template <typename T, typename S>
+void launch_fwd_level(const SolveItem* items, int nitems, const DevSym& d, const S* F, const T* bp, T* fp,
                       T* y, T* u, hipStream_t st) {
   if (nitems > 0)
-    hipLaunchKernelGGL(k_fwd_level<T>, dim3(nitems), dim3(256), 0, st, items, d, F, bp, fp, y, u);
+    hipLaunchKernelGGL((k_fwd_level<T, S>), dim3(nitems), dim3(256), 0, st, items, d, F, bp, fp, y, u);
 }
 template <typename T>
 void launch_dsolve(int n, const T* E0, const T* E1, const signed char* piv, const int* wpos,
@@ -2287,10 +2363,10 @@ void launch_dsolve(int n, const T* E0, const T* E1, const signed char* piv, cons
   if (n > 0)
     hipLaunchKernelGGL(k_dsolve<T>, dim3(nblk(n, 256)), dim3(256), 0, st, n, E0, E1, piv, wpos, y, wv);
 }
-template <typename T>
-void launch_bwd_cols(const SolveItem* items, int nitems, const DevSym& d, const T* F, T* wv, T* xp,
+template <typename T, typename S>
+void launch_bwd_cols(const SolveItem* items, int nitems, const DevSym& d, const S* F, T* wv, T* xp,
                      hipStream_t st) {
-  if (nitems > 0) hipLaunchKernelGGL(k_bwd_cols<T>, dim3(nitems), dim3(256), 0, st, items, d, F, wv, xp);
+  if (nitems > 0) hipLaunchKernelGGL((k_bwd_cols<T, S>), dim3(nitems), dim3(256), 0, st, items, d, F, wv, xp);
 }
 template <typename T>
 void launch_fwd_gather_m(const SolveItem* items, int nitems, const DevSym& d, const T* F, int nr, const T* bp,
@@ -2340,6 +2416,11 @@ template <typename TI, typename TO>
 void launch_scatter_perm(int n, const int* perm, const TI* in, TO* out, hipStream_t st) {
   hipLaunchKernelGGL((k_scatter_perm<TI, TO>), dim3(nblk(n, 256)), dim3(256), 0, st, n, perm, in, out);
 }
+template <typename TO>
+void launch_gather_bicg_p(int n, const int* perm, const double* sc, const double* r, const double* v, double* p,
+                          TO* out, hipStream_t st) {
+  hipLaunchKernelGGL((k_gather_bicg_p<TO>), dim3(nblk(n, 256)), dim3(256), 0, st, n, perm, sc, r, v, p, out);
+}
 void launch_expand_vals(long long nnzf, const int* map, const double* vals, double* valsf,
                         hipStream_t st) {
   if (nnzf > 0)
@@ -2352,8 +2433,9 @@ void launch_spmv(int n, const int* rowptr, const int* col, const double* val, co
                      a1, part1, part2);
 }
 int spmv_blocks(int n) { return (int)nblk(n, SPMV_ROWS); }
-void launch_reduce2(const double* p1, const double* p2, int np, double* out1, double* out2, hipStream_t st) {
-  hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, st, p1, p2, np, out1, out2);
+void launch_reduce2(const double* p1, const double* p2, int np, double* out1, double* out2, hipStream_t st,
+                    double* sc) {
+  hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, st, p1, p2, np, out1, out2, sc);
 }
 void launch_absmax(long long nnz, const double* v, double* partial, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_absmax_partial, dim3(RED_BLOCKS), dim3(256), 0, st, nnz, v, partial);
@@ -2438,9 +2520,6 @@ void launch_seg_scatter(int nuniq, const int* useg, const int* uslot, const int*
     hipLaunchKernelGGL(k_seg_scatter, dim3(nblk(nuniq, 256)), dim3(256), 0, st, nuniq, useg, uslot, order,
                        c, scale, vals);
 }
-void launch_bicg_p(int n, const double* sc, const double* r, const double* v, double* p, hipStream_t st) {
-  hipLaunchKernelGGL(k_bicg_p, dim3(nblk(n, 256)), dim3(256), 0, st, n, sc, r, v, p);
-}
 void launch_bicg_s(int n, const double* sc, const double* r, const double* v, double* s_, double* part,
                    hipStream_t st) {
   hipLaunchKernelGGL(k_bicg_s, dim3(nblk(n, 256)), dim3(256), 0, st, n, sc, r, v, s_, part);
@@ -2455,11 +2534,14 @@ int vec_blocks(int n) { return (int)nblk(n, 256); }
 void launch_bicg_xa(int n, const double* sc, const double* y, double* x, hipStream_t st) {
   hipLaunchKernelGGL(k_bicg_xa, dim3(nblk(n, 256)), dim3(256), 0, st, n, sc, y, x);
 }
-void launch_bicg_scalars(double* sc, int init, hipStream_t st) {
-  hipLaunchKernelGGL(k_bicg_scalars, dim3(1), dim3(64), 0, st, sc, init);
+void launch_bicg_init(double* sc, hipStream_t st) {
+  hipLaunchKernelGGL(k_bicg_init, dim3(1), dim3(64), 0, st, sc);
 }
 void launch_fill(long long n, double v, double* x, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, st, n, v, x);
+}
+void launch_pack_half(long long n, const float* in, _Float16* out, FactorCounters* ctr, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_pack_half, dim3(nblk(n, 256)), dim3(256), 0, st, n, in, out, ctr);
 }
 void launch_zero_counters(FactorCounters* c, int* rctr, int nr, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_counters, dim3(nblk(std::max(nr, 1), 256)), dim3(256), 0, st, c, rctr, nr);
@@ -2496,16 +2578,24 @@ void launch_absmax_roots(int nchunks, const long long* ca, const long long* cb, 
   template void launch_zt<T>(const int4*, int, const DevSym&, T*, hipStream_t);                \
   template void launch_gform<T>(const int4*, int, const DevSym&, T*, hipStream_t);             \
   template void launch_pack_solve<T>(const int2*, int, const DevSym&, const T*, T*, T*, hipStream_t); \
-  template void launch_fwd_level<T>(const SolveItem*, int, const DevSym&, const T*, const T*, T*, T*, \
-                                    T*, hipStream_t);                                            \
+  template void launch_fwd_level<T, T>(const SolveItem*, int, const DevSym&, const T*, const T*, T*, T*, \
+                                       T*, hipStream_t);                                         \
   template void launch_dsolve<T>(int, const T*, const T*, const signed char*, const int*,        \
                                  const T*, T*, hipStream_t);                                     \
-  template void launch_bwd_cols<T>(const SolveItem*, int, const DevSym&, const T*, T*, T*,          \
-                                   hipStream_t);
+  template void launch_bwd_cols<T, T>(const SolveItem*, int, const DevSym&, const T*, T*, T*,       \
+                                      hipStream_t);
 KKT_INST(double)
 KKT_INST(float)
+template void launch_fwd_level<float, _Float16>(const SolveItem*, int, const DevSym&, const _Float16*, const float*,
+                                                float*, float*, float*, hipStream_t);
+template void launch_bwd_cols<float, _Float16>(const SolveItem*, int, const DevSym&, const _Float16*, float*,
+                                               float*, hipStream_t);
 template void launch_gather_perm<double, double>(int, const int*, const double*, double*, hipStream_t);
 template void launch_gather_perm<double, float>(int, const int*, const double*, float*, hipStream_t);
+template void launch_gather_bicg_p<double>(int, const int*, const double*, const double*, const double*, double*,
+                                           double*, hipStream_t);
+template void launch_gather_bicg_p<float>(int, const int*, const double*, const double*, const double*, double*,
+                                          float*, hipStream_t);
 template void launch_scatter_perm<double, double>(int, const int*, const double*, double*, hipStream_t);
 template void launch_scatter_perm<float, double>(int, const int*, const float*, double*, hipStream_t);
 #define KKT_INST_M(T)                                                                                        \

@@ -159,6 +159,11 @@ struct kkt_solver {
   // when allocated)
   double* d_PK64 = nullptr;
   float* d_PK32 = nullptr;
+  // half-precision copy of the FP32 forward rows and backward columns (same
+  // offsets; the FP32 array's slots serve both), read by the iterative solve's
+  // preconditioner: half_on = the user setting, have16 = valid for the factor
+  _Float16* d_PK16 = nullptr;
+  bool half_on = true, have16 = false;
   long long pack_elems = 0, sl_elems = 0;
   int2* d_pack = nullptr;  // packing work items (supernode, block of 4 columns)
   int pack_cnt = 0;
@@ -169,6 +174,8 @@ struct kkt_solver {
   template <typename T>
   T* pk_slot(T* PK) const { return PK + 2 * (pack_elems + 16); }
   size_t pk_elems() const { return (size_t)2 * (pack_elems + 16) + (size_t)sl_elems + 16; }
+  size_t pk16_elems() const { return (size_t)2 * (pack_elems + 16); }
+  bool use16() const { return have16 && fprec == KKT_PREC_FP32_REFINE; }
   // single-solve vectors, as T: bp, fp, y, xp (n each), w (v_elems + 16)
   size_t svec_elems() const { return (size_t)4 * n + (size_t)v_elems + 16; }
   double* d_W64 = nullptr;
@@ -218,20 +225,23 @@ struct kkt_solver {
   std::vector<int> h_rctr;         // after a factor: 4 counters per component
   std::vector<int> root_of_row;    // component of each (original) row
   std::vector<double> multiB;
-  // captured triangular-solve graphs (per factor precision), replayed per solve
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  // captured triangular-solve graphs, replayed per solve, by solve_q(): 0 FP64,
+  // 1 FP32, 2 FP32 fronts with the half-precision copy (the preconditioner)
+  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
   hipGraphExec_t fexec[2] = {nullptr, nullptr};  // captured factorizations
-  // captured BiCGStab half-iterations per precision: 0 first head, 1 head, 2 tail
-  hipGraphExec_t bexec[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  // captured BiCGStab half-iterations by solve_q(true): 0 head, 1 tail
+  hipGraphExec_t bexec[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
   // captured multi-RHS triangular solves per precision and count of right-hand sides
   hipGraphExec_t mexec[2][SOLVE_NRHS] = {};
   bool graphs_on = true;
   void drop_graphs() {
-    for (int q = 0; q < 2; q++) {
+    for (int q = 0; q < 3; q++) {
       if (gexec[q]) { (void)hipGraphExecDestroy(gexec[q]); gexec[q] = nullptr; }
-      if (fexec[q]) { (void)hipGraphExecDestroy(fexec[q]); fexec[q] = nullptr; }
-      for (int w = 0; w < 3; w++)
+      for (int w = 0; w < 2; w++)
         if (bexec[q][w]) { (void)hipGraphExecDestroy(bexec[q][w]); bexec[q][w] = nullptr; }
+    }
+    for (int q = 0; q < 2; q++) {
+      if (fexec[q]) { (void)hipGraphExecDestroy(fexec[q]); fexec[q] = nullptr; }
       for (int w = 0; w < SOLVE_NRHS; w++)
         if (mexec[q][w]) { (void)hipGraphExecDestroy(mexec[q][w]); mexec[q][w] = nullptr; }
     }
@@ -297,7 +307,7 @@ struct kkt_solver {
   size_t numeric_bytes(int p) const {
     const size_t esz = (p == KKT_PREC_FP64) ? 8 : 4;
     return ((size_t)front_elems + 2 * (size_t)n + 4) * esz + (size_t)std::max<long long>(w_elems, 1) * esz +
-           pk_elems() * esz + svec_elems() * 8;
+           pk_elems() * esz + svec_elems() * 8 + ((p == KKT_PREC_FP32_REFINE && half_on) ? pk16_elems() * 2 : 0);
   }
   size_t held_bytes() const {
     if (!dev_ok || state < ANALYZED) return 0;
@@ -306,6 +316,7 @@ struct kkt_solver {
     const size_t f32 = ((size_t)front_elems + 2 * (size_t)n + 4 + (size_t)std::max<long long>(w_elems, 1)) * 4;
     if (d_F64) t += f64 + pk_elems() * 8;
     if (d_F32) t += f32 + pk_elems() * 4;
+    if (d_PK16) t += pk16_elems() * 2;
     t += svec_bytes + msvec_bytes + mvec_bytes;
     return t;
   }
@@ -316,8 +327,11 @@ struct kkt_solver {
     if (d_F32) (void)hipFree(d_F32);
     if (d_PK64) (void)hipFree(d_PK64);
     if (d_PK32) (void)hipFree(d_PK32);
+    if (d_PK16) (void)hipFree(d_PK16);
     d_PK64 = nullptr;
     d_PK32 = nullptr;
+    d_PK16 = nullptr;
+    have16 = false;
     if (d_W64) (void)hipFree(d_W64);
     if (d_W32) (void)hipFree(d_W32);
     if (d_svec) (void)hipFree(d_svec);
@@ -1072,6 +1086,11 @@ struct kkt_solver {
       if (hipMemsetAsync(d_PK32, 0, pk_elems() * sizeof(float), st) != hipSuccess)
         return fail(KKT_ERR_DEVICE, "clearing the update slots failed");
     }
+    if (p == KKT_PREC_FP32_REFINE && half_on && !d_PK16) {
+      drop_graphs();  // the factor graph gains the conversion
+      if (hipMalloc(&d_PK16, pk16_elems() * sizeof(_Float16)) != hipSuccess)
+        return fail(KKT_ERR_ALLOC, "hipMalloc failed (half-precision solve copy)");
+    }
     size_t need = svec_elems() * std::max(esz, sizeof(double));
     if (svec_bytes < need) {
       drop_graphs();
@@ -1164,8 +1183,12 @@ struct kkt_solver {
     double t0 = g_timing ? now_ms() : 0;
     auto body = [&]() {
       launch_expand_vals(nnzf, d_map_f, d_vals, d_valsf, st);
-      if (p == KKT_PREC_FP64) run_factor<double>(d_F64, d_W64, d_PK64, 1e-8);
-      else run_factor<float>(d_F32, d_W32, d_PK32, 1e-6);
+      if (p == KKT_PREC_FP64) {
+        run_factor<double>(d_F64, d_W64, d_PK64, 1e-8);
+      } else {
+        run_factor<float>(d_F32, d_W32, d_PK32, 1e-6);
+        if (half_on && d_PK16) launch_pack_half((long long)pk16_elems(), d_PK32, d_PK16, d_ctr, st);
+      }
     };
     // the factor's launch sequence is fixed by the analysis: one graph per precision,
     // unless factor kernels are being timed with events
@@ -1188,6 +1211,7 @@ struct kkt_solver {
     tflush();
     if (hipGetLastError() != hipSuccess) return fail(KKT_ERR_FACTORIZATION, "kernel launch failed");
     fprec = p;
+    have16 = (p == KKT_PREC_FP32_REFINE) && half_on && d_PK16 && !ctr.half_overflow;
     state = FACTORED;
     if (g_timing)
       fprintf(stderr, "[kkt-timing] factor(%s) n=%d: %.3f ms (inertia +%d/-%d, perturbed %d)\n",
@@ -1196,15 +1220,27 @@ struct kkt_solver {
   }
 
   // ------------------------------------------------------------------
-  template <typename T>
-  void run_tri_solve(const T* F, T* PK, const double* d_b, double* d_x, void* svec = nullptr) {
+  // BiCGStab's search-direction update p = r + beta (p - omega v), run by a
+  // solve's gather when p is the right-hand side (p updated in place)
+  struct BicgPre {
+    const double* sc;
+    const double* r;
+    const double* v;
+    double* p;
+  };
+  // F: the fronts (for D^-1), PK: the packed arrays (their update slots), PG:
+  // the packed rows and columns the sweeps read (PK itself, or its half copy)
+  template <typename T, typename TS>
+  void run_tri_solve(const T* F, T* PK, const TS* PG, const double* d_b, double* d_x, void* svec = nullptr,
+                     const BicgPre* pre = nullptr) {
     T* sv = (T*)(svec ? svec : d_svec);
     T *bp = sv, *fp = sv + n, *y = sv + 2 * (size_t)n, *xp = sv + 3 * (size_t)n, *wv = sv + 4 * (size_t)n;
     T* u = pk_slot(PK);
     const T* E0 = F + front_elems;
     const T* E1 = E0 + n;
     tbeg(TC_VEC);
-    launch_gather_perm<double, T>(n, d_perm, d_b, bp, st);
+    if (pre) launch_gather_bicg_p<T>(n, d_perm, pre->sc, pre->r, pre->v, pre->p, bp, st);
+    else launch_gather_perm<double, T>(n, d_perm, d_b, bp, st);
     tend();
     const int nl = S.nlevels;
     // timed per launch: class TC_SLEVEL = forward launches, TC_SGEMV = backward
@@ -1213,11 +1249,11 @@ struct kkt_solver {
       if (!ff_l[L].cnt) continue;
       if (lev_timing) (void)hipEventRecord(lev_ev[4 * L], st);
       tbeg(TC_SLEVEL);
-      launch_fwd_level<T>(d_ff + ff_l[L].off, ff_l[L].cnt, ds, pk_fwd(PK), bp, fp, y, u, st);
+      launch_fwd_level<T>(d_ff + ff_l[L].off, ff_l[L].cnt, ds, pk_fwd(PG), bp, fp, y, u, st);
       tend();
       if (fs_l[L].cnt) {
         tbeg(TC_SLEVEL);
-        launch_fwd_level<T>(d_ff + fs_l[L].off, fs_l[L].cnt, ds, pk_fwd(PK), bp, fp, y, u, st);
+        launch_fwd_level<T>(d_ff + fs_l[L].off, fs_l[L].cnt, ds, pk_fwd(PG), bp, fp, y, u, st);
         tend();
       }
       if (lev_timing) (void)hipEventRecord(lev_ev[4 * L + 1], st);
@@ -1229,7 +1265,7 @@ struct kkt_solver {
       if (!bc_l[L].cnt) continue;
       if (lev_timing) (void)hipEventRecord(lev_ev[4 * L + 2], st);
       tbeg(TC_SGEMV);
-      launch_bwd_cols<T>(d_bc + bc_l[L].off, bc_l[L].cnt, ds, pk_bwd(PK), wv, xp, st);
+      launch_bwd_cols<T>(d_bc + bc_l[L].off, bc_l[L].cnt, ds, pk_bwd(PG), wv, xp, st);
       tend();
       if (lev_timing) (void)hipEventRecord(lev_ev[4 * L + 3], st);
     }
@@ -1431,14 +1467,16 @@ struct kkt_solver {
   // (HIP events, best of `reps`) for 4, 8, 16, 32.  Each candidate is then timed
   // the way solves run (its captured graph, or eager launches when graphs are
   // off) over 4 reps triangular solves; the fastest whole solve wins, so the
-  // default (16, 16) is kept unless another candidate beats it.
+  // default (16, 16) is kept unless another candidate beats it.  An FP32 factor
+  // with a half-precision copy is tuned on the copy: the iterative solve's
+  // preconditioner applications are most of its solves.
   bool lev_timing = false;
   std::vector<hipEvent_t> lev_ev;
-  int time_solves(int nrep, double* b, double* x, float* ms) {
-    tri_solve(b, x);  // captures the graph of the current items
+  int time_solves(int nrep, double* b, double* x, float* ms, bool half) {
+    tri_solve(b, x, half);  // captures the graph of the current items
     hipEvent_t e0 = lev_ev[0], e1 = lev_ev[1];
     (void)hipEventRecord(e0, st);
-    for (int r = 0; r < nrep; r++) tri_solve(b, x);
+    for (int r = 0; r < nrep; r++) tri_solve(b, x, half);
     (void)hipEventRecord(e1, st);
     if (hipStreamSynchronize(st) != hipSuccess) return fail(KKT_ERR_DEVICE, "autotune solve failed");
     if (hipEventElapsedTime(ms, e0, e1) != hipSuccess) return fail(KKT_ERR_DEVICE, "autotune timing failed");
@@ -1460,6 +1498,7 @@ struct kkt_solver {
     double* b = d_vec + 12 * (size_t)n;
     double* x = d_vec + 13 * (size_t)n;
     launch_fill(n, 1.0, b, st);
+    const bool half = use16();
     int rc;
     // per-level pick from eager per-launch timing: the lane shape of both sweeps
     // and the one- or two-launch forward form
@@ -1476,11 +1515,11 @@ struct kkt_solver {
       fr_fuse.assign(nl, fu);
       fr_rg.assign(nl, rgp ? rgp : 1);
       if ((rc = build_solve_items())) return rc;
-      tri_solve_eager(b, x);
+      tri_solve_eager(b, x, nullptr, half);
       std::vector<float> tfr(nl, 1e30f), tbc(nl, 1e30f);
       for (int r = 0; r < reps; r++) {
         lev_timing = true;
-        tri_solve_eager(b, x);
+        tri_solve_eager(b, x, nullptr, half);
         lev_timing = false;
         if (hipStreamSynchronize(st) != hipSuccess) return fail(KKT_ERR_DEVICE, "autotune solve failed");
         for (int L = 0; L < nl; L++) {
@@ -1517,7 +1556,7 @@ struct kkt_solver {
       if ((rc = build_solve_items())) return rc;
       drop_graphs();
       float ms;
-      if ((rc = time_solves(4 * reps, b, x, &ms))) return rc;
+      if ((rc = time_solves(4 * reps, b, x, &ms, half))) return rc;
       if (ms < best) { best = ms; bi = q; }
       if (g_timing)
         fprintf(stderr, "[kkt-timing] autotune(%s) candidate %zu: %.3f ms per triangular solve\n",
@@ -1539,21 +1578,25 @@ struct kkt_solver {
     tflush();
     return KKT_OK;
   }
-  void tri_solve_eager(const double* d_b, double* d_x) {
-    if (fprec == KKT_PREC_FP64) run_tri_solve<double>(d_F64, d_PK64, d_b, d_x);
-    else run_tri_solve<float>(d_F32, d_PK32, d_b, d_x);
+  // half: the iterative solve's preconditioner (the half-precision copy, when
+  // the FP32 factor has one)
+  void tri_solve_eager(const double* d_b, double* d_x, const BicgPre* pre = nullptr, bool half = false) {
+    if (fprec == KKT_PREC_FP64) run_tri_solve<double>(d_F64, d_PK64, (const double*)d_PK64, d_b, d_x, nullptr, pre);
+    else if (half && use16()) run_tri_solve<float>(d_F32, d_PK32, (const _Float16*)d_PK16, d_b, d_x, nullptr, pre);
+    else run_tri_solve<float>(d_F32, d_PK32, (const float*)d_PK32, d_b, d_x, nullptr, pre);
   }
   // The launch sequence of a solve is fixed by the analysis, so it is captured
   // once into a HIP graph (reading vec(12), writing vec(13)) and replayed.
-  void tri_solve(const double* d_b, double* d_x) {
+  int solve_q(bool half) const { return fprec == KKT_PREC_FP64 ? 0 : (half && use16() ? 2 : 1); }
+  void tri_solve(const double* d_b, double* d_x, bool half = false) {
     const int solve_cls = (1 << TC_SLEVEL) | (1 << TC_SGEMV) | (1 << TC_VEC);
-    if (!graphs_on || (tm_on && (tm_mask & solve_cls))) { tri_solve_eager(d_b, d_x); return; }
-    const int q = (fprec == KKT_PREC_FP64) ? 0 : 1;
+    if (!graphs_on || (tm_on && (tm_mask & solve_cls))) { tri_solve_eager(d_b, d_x, nullptr, half); return; }
+    const int q = solve_q(half);
     double* gin = vec(12);
     double* gout = vec(13);
-    if (!gexec[q] && !capture(gexec[q], [&]() { tri_solve_eager(gin, gout); })) {
+    if (!gexec[q] && !capture(gexec[q], [&]() { tri_solve_eager(gin, gout, nullptr, half); })) {
       graphs_on = false;
-      tri_solve_eager(d_b, d_x);
+      tri_solve_eager(d_b, d_x, nullptr, half);
       return;
     }
     if (d_b != gin) (void)hipMemcpyAsync(gin, d_b, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st);
@@ -1693,8 +1736,8 @@ struct kkt_solver {
     p += align256(1024 * sizeof(double));
     double* scal = (double*)p;
     auto tri = [&](const double* b, double* x) {
-      if (fprec == KKT_PREC_FP64) run_tri_solve<double>(d_F64, d_PK64, b, x, svec);
-      else run_tri_solve<float>(d_F32, d_PK32, b, x, svec);
+      if (fprec == KKT_PREC_FP64) run_tri_solve<double>(d_F64, d_PK64, (const double*)d_PK64, b, x, svec);
+      else run_tri_solve<float>(d_F32, d_PK32, (const float*)d_PK32, b, x, svec);
     };
     auto dot = [&](const double* a, const double* b, double& out) -> int {
       launch_dot(n, a, b, part, scal, st);
@@ -1749,15 +1792,18 @@ struct kkt_solver {
 
   // ---- per-launch kernel times without per-launch events ----
   // One forward sweep (what 0), one backward sweep (1) or one residual SpMV (2)
-  // of the current factor, captured into a graph and replayed `reps` times
+  // of the current factor, or a forward (3) or backward (4) sweep reading an FP32
+  // factor's half-precision copy, captured into a graph and replayed `reps` times
   // between two HIP events on the handle's stream: graph-replayed kernels run
   // back to back, so elapsed / (reps x launches) is the kernels' average
   // duration, as a kernel trace reports it.  Replays are idempotent (each
   // recomputes the same vectors from the same inputs).
   int time_kernels(int what, int reps, double* ms_per_launch, int* launches) {
     if (state != FACTORED) return fail(KKT_ERR_STATE, "factor() not previously called");
-    if (what < 0 || what > 2 || reps < 1) return fail(KKT_ERR_ARGUMENT, "bad kernel group or repetitions");
-    const bool f64 = fprec == KKT_PREC_FP64;
+    if (what < 0 || what > 4 || reps < 1) return fail(KKT_ERR_ARGUMENT, "bad kernel group or repetitions");
+    if (what >= 3 && !use16()) return fail(KKT_ERR_STATE, "the factor has no half-precision solve copy");
+    const bool f64 = fprec == KKT_PREC_FP64, half = what >= 3;
+    if (half) what -= 3;
     void* sv = d_svec;
     const int nl = S.nlevels;
     int cnt = 0;
@@ -1776,7 +1822,7 @@ struct kkt_solver {
         launch_spmv(n, d_rowptr_f, d_col_f, d_valsf, x, b, vec(0), 1, st);
         return;
       }
-      auto sweep = [&](auto* PK) {
+      auto sweep = [&](auto* PK, const auto* PG) {
         typedef typename std::remove_pointer<decltype(PK)>::type T;
         T* svT = (T*)sv;
         T *bp = svT, *fp = svT + n, *y = svT + 2 * (size_t)n, *xp = svT + 3 * (size_t)n,
@@ -1784,16 +1830,17 @@ struct kkt_solver {
         T* u = pk_slot(PK);
         for (int L = 0; L < nl; L++) {
           if (what == 0) {
-            if (ff_l[L].cnt) launch_fwd_level<T>(d_ff + ff_l[L].off, ff_l[L].cnt, ds, pk_fwd(PK), bp, fp, y, u, st);
-            if (fs_l[L].cnt) launch_fwd_level<T>(d_ff + fs_l[L].off, fs_l[L].cnt, ds, pk_fwd(PK), bp, fp, y, u, st);
+            if (ff_l[L].cnt) launch_fwd_level<T>(d_ff + ff_l[L].off, ff_l[L].cnt, ds, pk_fwd(PG), bp, fp, y, u, st);
+            if (fs_l[L].cnt) launch_fwd_level<T>(d_ff + fs_l[L].off, fs_l[L].cnt, ds, pk_fwd(PG), bp, fp, y, u, st);
           } else {
             const int Lb = nl - 1 - L;
-            if (bc_l[Lb].cnt) launch_bwd_cols<T>(d_bc + bc_l[Lb].off, bc_l[Lb].cnt, ds, pk_bwd(PK), wv, xp, st);
+            if (bc_l[Lb].cnt) launch_bwd_cols<T>(d_bc + bc_l[Lb].off, bc_l[Lb].cnt, ds, pk_bwd(PG), wv, xp, st);
           }
         }
       };
-      if (f64) sweep(d_PK64);
-      else sweep(d_PK32);
+      if (f64) sweep(d_PK64, (const double*)d_PK64);
+      else if (half) sweep(d_PK32, (const _Float16*)d_PK16);
+      else sweep(d_PK32, (const float*)d_PK32);
     };
     hipGraphExec_t ex = nullptr;
     if (!capture(ex, body)) return fail(KKT_ERR_DEVICE, "graph capture failed (kernel timing)");
@@ -1956,7 +2003,9 @@ struct kkt_solver {
     double *r = vec(0), *rh = vec(1), *p = vec(2), *v = vec(3), *s = vec(4), *t = vec(5),
            *y = vec(6), *z = vec(7), *x = vec(8);
     double* sc = d_scal;
-    tri_solve(d_b, x);
+    // the preconditioner M^-1 reads the half-precision copy of an FP32 factor's
+    // solve arrays when there is one (products and sums stay FP32)
+    tri_solve(d_b, x, true);
     spmv(x, d_b, r, 1);
     dot_to(d_b, d_b, 9);
     dot_to(r, r, 8);
@@ -1970,7 +2019,7 @@ struct kkt_solver {
       (void)hipMemcpyAsync(rh, r, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st);
       launch_fill(n, 0.0, p, st);
       launch_fill(n, 0.0, v, st);
-      launch_bicg_scalars(sc, 1, st);
+      launch_bicg_init(sc, st);
       const double eps = 1e-30;
       // Each half-iteration ends with its scalars copied to pinned host memory;
       // both halves are captured once into HIP graphs (fixed buffers), so an
@@ -1980,12 +2029,12 @@ struct kkt_solver {
       // rho of the next iteration comes with the x / r update.
       double *pa = d_part2, *pb = d_part2 + std::max(spmv_blocks(n), 1);
       const int nbs = spmv_blocks(n), nbv = vec_blocks(n);
-      auto head = [&](bool first) {
-        if (!first) launch_bicg_scalars(sc, 0, st);  // alpha, omega, rho_prev; rho = next rho
-        tbeg(TC_VEC);
-        launch_bicg_p(n, sc, r, v, p, st);            // p = r + beta (p - omega v)
-        tend();
-        tri_solve_eager(p, y);                        // y = M^-1 p
+      // The scalars of an iteration (alpha, omega, rho_prev; rho = next rho) are
+      // formed by the previous iteration's last reduction; p = r + beta (p - omega v)
+      // runs in the gather of y = M^-1 p.
+      const BicgPre pre{sc, r, v, p};
+      auto head = [&]() {
+        tri_solve_eager(p, y, &pre, true);            // p updated, y = M^-1 p
         tbeg(TC_SPMV);
         launch_spmv(n, d_rowptr_f, d_col_f, d_valsf, y, nullptr, v, 0, st, rh, pa, nullptr);  // v = A y
         tend();
@@ -1997,18 +2046,18 @@ struct kkt_solver {
         (void)hipMemcpyAsync(h_pin, sc, 6 * sizeof(double), hipMemcpyDeviceToHost, st);
       };
       auto tail = [&]() {
-        tri_solve_eager(s, z);                        // z = M^-1 s
+        tri_solve_eager(s, z, nullptr, true);         // z = M^-1 s
         tbeg(TC_SPMV);
         launch_spmv(n, d_rowptr_f, d_col_f, d_valsf, z, nullptr, t, 0, st, s, pa, pb);  // t = A z
         tend();
         tbeg(TC_VEC);
         launch_reduce2(pa, pb, nbs, sc + 6, sc + 7, st);  // (s, t), (t, t)
         launch_bicg_xr(n, sc, y, z, s, t, x, r, rh, pa, pb, st);  // x += alpha y + omega z; r = s - omega t
-        launch_reduce2(pa, pb, nbv, sc + 8, sc + 10, st);  // |r|^2, (r_hat, r)
+        launch_reduce2(pa, pb, nbv, sc + 8, sc + 10, st, sc);  // |r|^2, (r_hat, r); next scalars
         tend();
         (void)hipMemcpyAsync(h_pin + 6, sc + 6, 3 * sizeof(double), hipMemcpyDeviceToHost, st);
       };
-      const int q = (fprec == KKT_PREC_FP64) ? 0 : 1;
+      const int q = solve_q(true);
       const int cls = (1 << TC_SLEVEL) | (1 << TC_SGEMV) | (1 << TC_VEC) | (1 << TC_SPMV);
       auto run = [&](int w, auto fn) -> int {
         const bool use = graphs_on && !(tm_on && (tm_mask & cls));
@@ -2018,7 +2067,7 @@ struct kkt_solver {
         return KKT_OK;
       };
       for (it = 1; it <= maxIter; it++) {
-        if ((rc = (it == 1) ? run(0, [&]() { head(true); }) : run(1, [&]() { head(false); }))) return rc;
+        if ((rc = run(0, head))) return rc;
         const double* h5 = h_pin;
         if (std::fabs(h5[0]) < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: rho ~ 0");
         if (std::fabs(h5[4]) < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: (r_hat,v) ~ 0");
@@ -2026,7 +2075,7 @@ struct kkt_solver {
           launch_bicg_xa(n, sc, y, x, st);            // x += alpha y
           break;
         }
-        if ((rc = run(2, tail))) return rc;
+        if ((rc = run(1, tail))) return rc;
         const double* h3 = h_pin + 6;
         if (h3[1] < eps) return fail(KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: (t,t) ~ 0");
         if (std::sqrt(h3[2]) / bn <= tolRel) break;
@@ -2827,6 +2876,18 @@ int kkt_set_graphs(kkt_solver* h, int on) {
   return KKT_OK;
 }
 
+int kkt_set_iterative_half(kkt_solver* h, int on) {
+  CHECK_H(h);
+  const bool v = on != 0;
+  if (v != h->half_on) {
+    h->half_on = v;
+    h->have16 = false;  // until the next factor
+    if (h->dev_ok) h->drop_graphs();
+  }
+  h->ok();
+  return KKT_OK;
+}
+
 int kkt_set_ordering(kkt_solver* h, int ordering) {
   CHECK_H(h);
   if (ordering != 0 && ordering != 1) return h->fail(KKT_ERR_ARGUMENT, "unknown ordering");
@@ -2876,6 +2937,7 @@ int kkt_get_info(kkt_solver* h, kkt_info* out) {
       (h->prec == KKT_PREC_FP32_REFINE && !h->d_F64)
           ? (long long)(((size_t)h->front_elems + 2 * (size_t)h->n + 4 + (size_t)std::max<long long>(h->w_elems, 1)) * 8)
           : 0;
+  out->precond_half = h->use16() ? 1 : 0;
   return KKT_OK;
 }
 

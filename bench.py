@@ -59,26 +59,29 @@ def _structure(L, h, with_parent=False):
     return k, r, lev, inf
 
 
-def bwd_sweep_bytes(L, h, w):
+def bwd_sweep_bytes(L, h, w, wg=None):
     """Algorithmic HBM bytes of one backward sweep (all k_bwd_cols launches of a
     triangular solve): per supernode the lower copy of G once (k(k-1)/2 + k r
-    entries), its backward vector once (k + r entries), x written for its k pivots
-    and pushed to the r row entries it fills in the descendants, and the indices
-    (pivot permutation and push ranges, k each; push positions, r); w = bytes per
-    entry of the front precision."""
+    entries, wg bytes each: the front precision, or 2 for the half-precision
+    copy), its backward vector once (k + r entries), x written for its k pivots
+    and pushed to the r row entries it fills in the descendants (w bytes per
+    vector entry), and the indices (pivot permutation and push ranges, k each;
+    push positions, r)."""
+    wg = w if wg is None else wg
     k, r, lev, inf = _structure(L, h)
     k, r = k.astype(np.float64), r.astype(np.float64)
-    return float(np.sum((k * (k - 1) / 2 + k * r + 2 * k + 2 * r) * w + (r + 2 * k) * 4))
+    return float(np.sum((k * (k - 1) / 2 + k * r) * wg + (2 * k + 2 * r) * w + (r + 2 * k) * 4))
 
 
-def fwd_sweep_bytes(L, h, w, fwd_epl, forms, groups):
+def fwd_sweep_bytes(L, h, w, fwd_epl, forms, groups, wg=None):
     """Algorithmic HBM bytes of one forward sweep (all k_fwd_level launches of a
     triangular solve), per supernode (k pivots, r rows below): the upper copy of
     G once (k(k-1)/2 entries of X, unit diagonal implicit, and k r of Z), the
     right-hand side at the pivots (k) and y written for them (k), its r update
     entries written once and read once by the parent, the pivot permutation (k)
     and the rows' slot positions in the parent (r), 4 bytes each; w = bytes per
-    vector/factor entry.
+    vector entry, wg per entry of G (the front precision, or 2 for the
+    half-precision copy).
     Returned separately, the layout's extra reads: the zero entries of each dense
     update-slot block (nch x m entries read, of which only the children's rows
     were written) and, in the one-launch form, every further row block of a large
@@ -86,6 +89,7 @@ def fwd_sweep_bytes(L, h, w, fwd_epl, forms, groups):
     nch slots and the permutation; a block covers `groups` row groups of its
     level); in the two-launch form, the gathered pivot part written once and read
     by every row block."""
+    wg = w if wg is None else wg
     k, r, lev, inf, par = _structure(L, h, with_parent=True)
     m = k + r
     kf, rf, mf = k.astype(np.float64), r.astype(np.float64), m.astype(np.float64)
@@ -95,7 +99,7 @@ def fwd_sweep_bytes(L, h, w, fwd_epl, forms, groups):
         if par[s_] >= 0:
             nch[par[s_]] += 1
             rch[par[s_]] += rf[s_]
-    alg = float(np.sum((kf * (kf - 1) / 2 + kf * rf + 2 * kf + 2 * rf) * w + (kf + rf) * 4.0))
+    alg = float(np.sum((kf * (kf - 1) / 2 + kf * rf) * wg + (2 * kf + 2 * rf) * w + (kf + rf) * 4.0))
     red = float(np.sum((nch * mf - rch) * w))
     for s_ in range(len(k)):
         if k[s_] <= 32 and m[s_] <= 256:  # small fronts: gather and rows in one workgroup
@@ -201,6 +205,9 @@ def kernel_pass(L, h, vals_ptr, b_ptr, x_ptr):
     time_kernels(1, "bwd")
     inf = KktInfo()
     L.kkt_get_info(h, ctypes.byref(inf))
+    if inf.precond_half:  # the iterative solve's sweeps on the half-precision copy
+        time_kernels(3, "fwd16")
+        time_kernels(4, "bwd16")
     torch.cuda.synchronize()
     reps = 5
     t0 = time.perf_counter()
@@ -213,7 +220,11 @@ def kernel_pass(L, h, vals_ptr, b_ptr, x_ptr):
             "update_flops": uflops, "factor_ms": factor_ms,
             "fwd_ms": kt["fwd"][0] * kt["fwd"][1], "fwd_launches": kt["fwd"][1],
             "bwd_ms": kt["bwd"][0] * kt["bwd"][1], "bwd_launches": kt["bwd"][1],
-            "spmv_ms": kt["spmv"][0], "spmv_launches": 1, "sweeps": 1}
+            "spmv_ms": kt["spmv"][0], "spmv_launches": 1, "sweeps": 1,
+            "fwd16_ms": kt["fwd16"][0] * kt["fwd16"][1] if "fwd16" in kt else 0.0,
+            "fwd16_launches": kt["fwd16"][1] if "fwd16" in kt else 0,
+            "bwd16_ms": kt["bwd16"][0] * kt["bwd16"][1] if "bwd16" in kt else 0.0,
+            "bwd16_launches": kt["bwd16"][1] if "bwd16" in kt else 0}
 
 
 def main():
@@ -234,6 +245,9 @@ def main():
                          "rates of one launch over 4 MB - 1 GB with cold caches (the achievable HBM ceiling)")
     ap.add_argument("--nrhs", type=int, default=64, help="multi: right-hand sides per block solve")
     ap.add_argument("--tet", default="121x24x24", help="precision: Delaunay lattice points")
+    ap.add_argument("--iterative-half", type=int, default=1, choices=[0, 1],
+                    help="1: the FP32 path's iterative preconditioner reads the half-precision copy of the "
+                         "solve arrays (kkt_set_iterative_half); 0: the FP32 arrays")
     ap.add_argument("--max-super", type=int, default=2048,
                     help="widest supernode of the analysis (64..2048; kkt_set_max_supernode)")
     ap.add_argument("--oracle", action="store_true",
@@ -318,6 +332,7 @@ def main():
         h = L.kkt_create()
         L.kkt_set_precision(h, prec)
         L.kkt_set_refine_tol(h, 1e-10, 10)
+        L.kkt_set_iterative_half(h, args.iterative_half)
         if prec == KKT_PREC_FP64:
             L.kkt_set_ir_steps(h, 1)  # one refinement step (PARDISO iparm[7]=1)
         t0 = time.time()
@@ -362,6 +377,8 @@ def main():
         w = 4 if prec == KKT_PREC_FP32_REFINE else 8
         fwd_bytes, fwd_red = fwd_sweep_bytes(L, h, w, shp_f, shp_n, shp_g)
         bwd_bytes = bwd_sweep_bytes(L, h, w)
+        fwd16_bytes, fwd16_red = fwd_sweep_bytes(L, h, w, shp_f, shp_n, shp_g, wg=2)
+        bwd16_bytes = bwd_sweep_bytes(L, h, w, wg=2)
         # verify every step's solution (host, after timing)
         import scipy.sparse as sp
         res = []
@@ -376,6 +393,7 @@ def main():
             "kinds": kinds, "iters": iters, "rel_residual": res, "analyze_s": t_an,
             "kp": kp, "info": inf.as_dict(), "fwd_sweep_bytes": fwd_bytes, "fwd_redundant_bytes": fwd_red,
             "bwd_sweep_bytes": bwd_bytes, "nlevels": inf.nlevels,
+            "fwd16_sweep_bytes": fwd16_bytes, "fwd16_redundant_bytes": fwd16_red, "bwd16_sweep_bytes": bwd16_bytes,
             "solve_shapes": {"fwd": shp_f.tolist(), "bwd": shp_b.tolist(), "fwd_launches": shp_n.tolist(),
                              "fwd_row_groups": shp_g.tolist(), "tune_s": t_tune},
             "x": [x_d[i].cpu().numpy() for i in range(min(args.steps, len(V)))],
@@ -401,11 +419,24 @@ def main():
                 "frac": ach / PEAKS["hbm_GBs"],
                 "traffic": profile_traffic(traffic_key, r32["info"]["nlevels"], launches / sweeps),
                 "avg_launch_ms": avg, "launches": launches, "bytes_per_launch": bpl}
-    roofline = hbm_roof("k_fwd_level<float>", r32["fwd_sweep_bytes"], kp["fwd_ms"], kp["fwd_launches"],
-                        kp["sweeps"], "k_fwd_level<float>")
-    roofline["redundant_gather_bytes_per_launch"] = r32["fwd_redundant_bytes"] * kp["sweeps"] / kp["fwd_launches"]
-    roof_bwd = hbm_roof("k_bwd_cols<float>", r32["bwd_sweep_bytes"], kp["bwd_ms"], kp["bwd_launches"],
-                        kp["sweeps"], "k_bwd_cols<float>")
+    # with the half-precision copy, the iterative steps' preconditioner sweeps
+    # (most of the sequence) read it: the headline entry is that forward sweep
+    # and the FP32 arrays' sweeps (direct solves, refinement) are reported beside
+    roof_fwd32 = hbm_roof("k_fwd_level<float>", r32["fwd_sweep_bytes"], kp["fwd_ms"], kp["fwd_launches"],
+                          kp["sweeps"], "k_fwd_level<float>")
+    roof_fwd32["redundant_gather_bytes_per_launch"] = r32["fwd_redundant_bytes"] * kp["sweeps"] / kp["fwd_launches"]
+    roof_bwd32 = hbm_roof("k_bwd_cols<float>", r32["bwd_sweep_bytes"], kp["bwd_ms"], kp["bwd_launches"],
+                          kp["sweeps"], "k_bwd_cols<float>")
+    roof_fwd16 = roof_bwd16 = None
+    if kp["fwd16_launches"]:
+        roof_fwd16 = hbm_roof("k_fwd_level<float, _Float16>", r32["fwd16_sweep_bytes"], kp["fwd16_ms"],
+                              kp["fwd16_launches"], kp["sweeps"], "k_fwd_level<float, _Float16>")
+        roof_fwd16["redundant_gather_bytes_per_launch"] = (r32["fwd16_redundant_bytes"] * kp["sweeps"] /
+                                                           kp["fwd16_launches"])
+        roof_bwd16 = hbm_roof("k_bwd_cols<float, _Float16>", r32["bwd16_sweep_bytes"], kp["bwd16_ms"],
+                              kp["bwd16_launches"], kp["sweeps"], "k_bwd_cols<float, _Float16>")
+    roofline = roof_fwd16 or roof_fwd32
+    roof_bwd = roof_bwd16 or roof_bwd32
     sp_full, sp_upper, nnz_full = spmv_bytes(n, nnz)
     roof_spmv = None
     if kp["spmv_launches"] > 0:
@@ -433,6 +464,14 @@ def main():
         cpu = cpu_baseline(beam, V, B, r64["x"], r32["x"])
 
     lat = np.array(r32["lat_ms"])
+
+    def iter_ms(r):
+        # wall time of the steps BiCGStab finished, per iteration (each step's
+        # start-up solve and residual included): the cost of an iteration apart
+        # from how many iterations the rounding of a run leads to
+        it = [(t, c) for t, k, c in zip(r["lat_ms"], r["kinds"], r["iters"]) if k == "iterative"]
+        return sum(t for t, c in it) / max(1, sum(c for t, c in it))
+
     line = {
         "metric": "KKT solves/s at <=1e-8 rel resid, FP32-MFMA+refine vs FP64 LDLT; p50/p99 lat",
         "value": r32["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -453,6 +492,8 @@ def main():
         "max_fwd_err_vs_fp64": max(fwd),
         "step_kinds": "".join("D" if k == "direct" else "i" for k in r32["kinds"]),
         "bicgstab_iters": r32["iters"], "fp64_bicgstab_iters": r64["iters"],
+        "ms_per_bicgstab_iter": iter_ms(r32), "fp64_ms_per_bicgstab_iter": iter_ms(r64),
+        "step_ms": [round(t, 3) for t in r32["lat_ms"]], "fp64_step_ms": [round(t, 3) for t in r64["lat_ms"]],
         "tip_displacement_mm": [round(1e3 * t, 3) for t in tipdefl],
         "factor_gflop": fl / 1e9, "nnzL": r32["info"]["nnzL"],
         "n2x2": r32["info"]["n2x2"], "pivoted_blocks": r32["info"]["pivoted_blocks"],
@@ -460,6 +501,8 @@ def main():
         "factor_ms_fp32": kp["factor_ms"], "factor_ms_fp64": r64["kp"]["factor_ms"],
         "kernel_ms_fp32": r32["kp"]["factor_kernel_ms"], "kernel_ms_fp64": r64["kp"]["factor_kernel_ms"],
         "roofline": roofline, "roofline_bwd": roof_bwd, "roofline_spmv": roof_spmv,
+        "roofline_fwd_fp32": roof_fwd32 if roof_fwd16 else None, "roofline_bwd_fp32": roof_bwd32 if roof_fwd16 else None,
+        "precond_half": bool(r32["info"]["precond_half"]),
         "roofline_factor": roof_factor, "roofline_factor_update": roof_update, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -479,6 +522,8 @@ def profile_traffic(kernel, nlevels, launches_per_sweep):
         return None
     d = json.load(open(tf)).get("kernels", {})
     k, b = d.get(kernel), d.get("k_bwd_cols<float>")
+    if k and kernel.endswith("_Float16>"):  # the iterative solve's dispatches: per its own backward ones
+        b = d.get("k_bwd_cols<float, _Float16>")
     if not k or not b or nlevels <= 0 or launches_per_sweep <= 0:
         return None
     sweeps = b["launches"] / float(nlevels)
@@ -540,6 +585,10 @@ def pmc_run(nx, ny, nz):
     s.factor(V[0])
     x = np.zeros(beam.n)
     s.solve(x, B[0])
+    # a few stale-factor iterations: the preconditioner's sweeps on the
+    # half-precision copy
+    x2 = np.zeros(beam.n)
+    s.iterativeSolve(V[0] * np.random.default_rng(5).uniform(0.999, 1.001, V[0].shape), x2, B[0], 10)
     a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
     b = torch.ones(256 << 20, dtype=torch.uint8, device="cuda")
     a.copy_(b)
