@@ -163,7 +163,7 @@ struct kkt_solver {
   // offsets; the FP32 array's slots serve both), read by the iterative solve's
   // preconditioner: half_on = the user setting, have16 = valid for the factor
   _Float16* d_PK16 = nullptr;
-  bool half_on = true, have16 = false;
+  bool half_on = false, have16 = false;
   long long pack_elems = 0, sl_elems = 0;
   int2* d_pack = nullptr;  // packing work items (supernode, block of 4 columns)
   int pack_cnt = 0;

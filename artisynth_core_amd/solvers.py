@@ -475,8 +475,8 @@ class MI355XSolver:
         self._check(self._L.kkt_set_max_supernode(self._h, int(cols)), "setMaxSupernode")
 
     def setIterativeHalf(self, on):
-        """1 (default): iterativeSolve's preconditioner reads a half-precision copy
-        of an FP32 factor's solve arrays (FP32 arithmetic); 0: the FP32 arrays.
+        """1: iterativeSolve's preconditioner reads a half-precision copy of an
+        FP32 factor's solve arrays (FP32 arithmetic); 0 (default): the FP32 arrays.
         Applies from the next factor; info()["precond_half"] says which is used."""
         self._alive()
         self._check(self._L.kkt_set_iterative_half(self._h, 1 if on else 0), "setIterativeHalf")

@@ -12,6 +12,12 @@ directly with refinement.  The FP32-MFMA + FP64-refinement path is the headline
 Inputs (all 20 value arrays and right-hand sides) are resident in HBM before the
 timed region; the library is driven through its device-pointer entry points.
 
+After the headline sequence the default run also measures the two other
+workloads in their own timed regions and adds them to its line: "batch" (256
+uneven FP64 systems, one batched factor + solve against one handle per system)
+and "serve" (64 instances, 1k-90k DOF, Poisson requests at 2,000/s through the
+serving loop, 200 GB budget); --no-mix skips both.
+
   python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 """
 import argparse
@@ -234,18 +240,24 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nsys", type=int, default=20, help="systems in the step sequence")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mix", action="store_true",
+                    help="bench: skip the 256-system batch and the 64-instance serving runs that the "
+                         "default run adds to its line (batch / serve objects)")
     ap.add_argument("--grid", default="100x16x16")
     ap.add_argument("--mode", default="bench",
-                    choices=["bench", "pmc", "batch", "serve", "precision", "multi", "stream"],
+                    choices=["bench", "pmc", "batch", "serve", "precision", "multi", "stream", "levels"],
                     help="pmc: one FP32 factor + solve and a 256 MiB copy, for counter runs; "
                          "batch: the 256-system mix, one batched launch vs per-system launches; "
                          "precision: the ~210k-DOF Mooney-Rivlin tet box, direct FP32-MFMA+refine "
                          "vs FP64 solves; multi: block solves of --nrhs right-hand sides on the "
                          "beam's first KKT, batched vs one at a time; stream: streaming read and copy "
-                         "rates of one launch over 4 MB - 1 GB with cold caches (the achievable HBM ceiling)")
+                         "rates of one launch over 4 MB - 1 GB with cold caches (the achievable HBM ceiling); "
+                         "levels: per precision one factor of the beam's first KKT, the solve-shape autotuner and "
+                         "four stale-factor iterative solves on the second (graphs on), for a kernel trace of "
+                         "the sweeps level by level")
     ap.add_argument("--nrhs", type=int, default=64, help="multi: right-hand sides per block solve")
     ap.add_argument("--tet", default="121x24x24", help="precision: Delaunay lattice points")
-    ap.add_argument("--iterative-half", type=int, default=1, choices=[0, 1],
+    ap.add_argument("--iterative-half", type=int, default=0, choices=[0, 1],
                     help="1: the FP32 path's iterative preconditioner reads the half-precision copy of the "
                          "solve arrays (kkt_set_iterative_half); 0: the FP32 arrays")
     ap.add_argument("--max-super", type=int, default=2048,
@@ -258,7 +270,7 @@ def main():
     ap.add_argument("--rate", type=float, default=2000.0, help="serving: offered requests/s")
     ap.add_argument("--workers", type=int, default=12)
     ap.add_argument("--budget-gb", type=float, default=200.0)
-    ap.add_argument("--eps", type=float, default=3e-3,
+    ap.add_argument("--eps", type=float, default=5e-3,
                     help="serving: per-step relative drift of every matrix entry (random walk)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -288,6 +300,8 @@ def main():
         return multi_run(nx, ny, nz, args.nrhs, max(args.steps // 4, 2), args.warmup)
     if args.mode == "stream":
         return stream_run()
+    if args.mode == "levels":
+        return levels_run(nx, ny, nz, args.iterative_half)
     L = lib()
     # setup (untimed): integrate the ConstrainedBackwardEuler sequence with the
     # FP64 GPU path, so each step's system is assembled at the state the previous
@@ -505,6 +519,18 @@ def main():
         "precond_half": bool(r32["info"]["precond_half"]),
         "roofline_factor": roof_factor, "roofline_factor_update": roof_update, "cpu_baseline": cpu,
     }
+    if not args.no_mix:
+        # the other two north-star workloads, measured in the same run (after the
+        # headline sequence; their own timed regions)
+        bl = batch_run(args.nbatch, args.steps, args.warmup, emit=False)
+        line["batch"] = {k: bl[k] for k in ("nsys", "n_total", "batched_ms", "per_system_ms", "speedup",
+                                            "batched_systems_per_s", "max_rel_residual",
+                                            "max_diff_alone_vs_batch", "dtype")}
+        sl = serve_run(args.ninst, args.nreq, args.rate, args.workers, args.budget_gb, args.eps, emit=False)
+        line["serve"] = {k: sl[k] for k in ("ninst", "nreq", "rate_target", "offered_rate_achieved", "workers",
+                                            "budget_gb", "throughput_req_s", "p50_ms", "p99_ms", "errors",
+                                            "direct", "iterative", "max_rel_residual_sample", "resident_gb",
+                                            "evictions", "queue_max", "concurrency", "total_dof", "inputs")}
     print(json.dumps(line), flush=True)
 
 
@@ -581,6 +607,7 @@ def pmc_run(nx, ny, nz):
     beam, V, B = G.large_beam_sequence(nsteps=1, nx=nx, ny=ny, nz=nz, width=0.16 * ny / 16)
     s = MI355XSolver(precision=KKT_PREC_FP32_REFINE)
     s._L.kkt_set_graphs(s._h, 0)  # kernels dispatched one by one for the counters
+    s.setIterativeHalf(True)  # counters for both the FP32 arrays' and the half copy's sweeps
     s.analyzeArrays(V[0], beam.colIdxs, beam.rowOffs, beam.n, Matrix.SYMMETRIC)
     s.factor(V[0])
     x = np.zeros(beam.n)
@@ -596,6 +623,32 @@ def pmc_run(nx, ny, nz):
     print(json.dumps({"mode": "pmc", "copy_bytes": 256 << 20, "n": beam.n,
                       "info": s.info()}), flush=True)
     s.dispose()
+
+
+def levels_run(nx, ny, nz, half):
+    """Kernel-trace workload for the per-level sweep times: for each precision one
+    factor of the beam's first KKT system, the solve-shape autotuner on it, then
+    four stale-factor iterative solves of the second system (BiCGStab, tolExp 10)
+    replaying the tuned graphs, as in the bench sequence."""
+    import torch
+    from artisynth_core_amd import KKT_PREC_FP32_REFINE, KKT_PREC_FP64
+    from artisynth_core_amd import generators as G
+    from artisynth_core_amd.solvers import Matrix, MI355XSolver
+    beam, V, B = G.large_beam_sequence(nsteps=2, nx=nx, ny=ny, nz=nz, width=0.16 * ny / 16)
+    out = {"mode": "levels", "n": beam.n}
+    for prec, name in ((KKT_PREC_FP32_REFINE, "fp32_mfma_refine"), (KKT_PREC_FP64, "fp64")):
+        s = MI355XSolver(precision=prec)
+        s.setIterativeHalf(bool(half))
+        s.analyzeArrays(V[0], beam.colIdxs, beam.rowOffs, beam.n, Matrix.SYMMETRIC)
+        s.factor(V[0])
+        s.autotuneSolve(3)
+        x = np.zeros(beam.n)
+        its = [int(s.iterativeSolve(V[1], x, B[1], 10)) for _ in range(4)]
+        torch.cuda.synchronize()
+        out[name] = {"nlevels": s.info()["nlevels"], "iterations": its, "shapes": s.getSolveShapes()}
+        log("%s: %d levels, iterations %s" % (name, out[name]["nlevels"], its))
+        s.dispose()
+    print(json.dumps(out, default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o)), flush=True)
 
 
 def multi_run(nx, ny, nz, nrhs, reps, warmup):
@@ -679,7 +732,7 @@ def multi_run(nx, ny, nz, nrhs, reps, warmup):
                       "results": out}), flush=True)
 
 
-def batch_run(nsys, reps, warmup):
+def batch_run(nsys, reps, warmup, emit=True):
     """The batching workload: nsys independent KKT systems (DOF ~ U[300,4000], rigid bodies + tet
     blocks + contact equalities, FP64).  One batched factor + solve (one captured graph
     each) against the same systems factored and solved one handle at a time."""
@@ -789,11 +842,13 @@ def batch_run(nsys, reps, warmup):
     log("batch: %.2f ms per batch of %d vs %.2f ms one by one (%.1fx; autotuned shapes; default %.2f vs %.2f), "
         "max resid %.2e" % (t_b, nsys, t_1, t_1 / t_b, float(np.median(lat)), float(np.median(lat1)),
                             line["max_rel_residual"]))
-    print(json.dumps(line), flush=True)
+    if emit:
+        print(json.dumps(line), flush=True)
     B.dispose()
+    return line
 
 
-def serve_run(ninst, nreq, rate, nworkers, budget_gb, eps=3e-3):
+def serve_run(ninst, nreq, rate, nworkers, budget_gb, eps=5e-3, emit=True):
     """The serving workload: ninst instances (1k-90k DOF, beam/tet/rigid), a Poisson
     stream of step requests at `rate` req/s, HBM-budget admission, FP32-MFMA + FP64
     refinement (1e-10 relative residual), through the C++ serving loop
@@ -890,8 +945,12 @@ def serve_run(ninst, nreq, rate, nworkers, budget_gb, eps=3e-3):
     log("serve: offered %.0f req/s (target %.0f), served %.0f req/s, p50 %.2f ms, p99 %.2f ms, direct %d, "
         "evictions %d, max resid %.2e" % (offered, rate, line["throughput_req_s"], lat["p50_ms"], lat["p99_ms"],
                                           line["direct"], line["evictions"], max(res)))
-    print(json.dumps(line), flush=True)
+    if emit:
+        print(json.dumps(line), flush=True)
     srv.shutdown()
+    del vals, xs, base, rhs, drift
+    torch.cuda.empty_cache()
+    return line
 
 
 def precision_run(dims, nsteps, warmup, with_oracle, max_super=2048):

@@ -257,14 +257,15 @@ int kkt_set_ordering(kkt_solver* h, int ordering);
  * gives 27 levels and 705 GFLOP.  Solve-heavy callers (stale-factor iterations)
  * gain from the default, direct-solve-heavy ones may prefer 1024. */
 int kkt_set_max_supernode(kkt_solver* h, int cols);
-/* 1 (default): an FP32 factor also keeps a half-precision copy of its packed
+/* 1: an FP32 factor also keeps a half-precision copy of its packed
  * triangular-solve arrays (2 bytes an entry, half the FP32 arrays' size), and the
  * stale-factor iterative solve (kkt_iterative_solve*, the BiCGStab
  * preconditioner M^-1) reads that copy, with products and sums in FP32; direct
  * solves and their refinement keep the FP32 arrays.  A factor with an entry
  * beyond the half range (|v| >= 65504) keeps to the FP32 arrays (kkt_info
- * precond_half = 0).  0: the preconditioner reads the FP32 arrays.  Applies from
- * the next factor.  No counterpart in the reference: the preconditioner only
+ * precond_half = 0).  0 (default): the preconditioner reads the FP32 arrays (on
+ * the 88k-DOF beam sequence the copy makes each BiCGStab iteration 3% faster
+ * but needs 15% more iterations).  Applies from the next factor.  No counterpart in the reference: the preconditioner only
  * steers the iteration, whose residual test is FP64, so the solution meets the
  * same tolerance. */
 int kkt_set_iterative_half(kkt_solver* h, int on);

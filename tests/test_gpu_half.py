@@ -116,6 +116,7 @@ def test_out_of_range_factor_keeps_fp32_arrays(gpu):
     n = len(ro) - 1
     b = np.cos(np.arange(n) * 0.37)
     s = MI355XSolver(precision=KKT_PREC_FP32_REFINE, ordering=0)
+    s.setIterativeHalf(True)
     try:
         s.analyzeArrays(va, ci, ro, n, Matrix.SYMMETRIC)
         s.factor(va)
