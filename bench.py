@@ -16,7 +16,9 @@ After the headline sequence the default run also measures the two other
 workloads in their own timed regions and adds them to its line: "batch" (256
 uneven FP64 systems, one batched factor + solve against one handle per system)
 and "serve" (64 instances, 1k-90k DOF, Poisson requests at 2,000/s through the
-serving loop, 200 GB budget); --no-mix skips both.
+serving loop, 200 GB budget), plus "serve_budget_12gb", the same stream under a
+12 GB budget (below the instances' resident total, so admission evicts and
+re-admits); --no-mix skips them.
 
   python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 """
@@ -526,11 +528,16 @@ def main():
         line["batch"] = {k: bl[k] for k in ("nsys", "n_total", "batched_ms", "per_system_ms", "speedup",
                                             "batched_systems_per_s", "max_rel_residual",
                                             "max_diff_alone_vs_batch", "dtype")}
+        keys = ("ninst", "nreq", "rate_target", "offered_rate_achieved", "workers", "budget_gb",
+                "throughput_req_s", "p50_ms", "p99_ms", "errors", "direct", "iterative",
+                "max_rel_residual_sample", "resident_gb", "evictions", "admissions_timed", "queue_max",
+                "concurrency", "total_dof", "inputs")
         sl = serve_run(args.ninst, args.nreq, args.rate, args.workers, args.budget_gb, args.eps, emit=False)
-        line["serve"] = {k: sl[k] for k in ("ninst", "nreq", "rate_target", "offered_rate_achieved", "workers",
-                                            "budget_gb", "throughput_req_s", "p50_ms", "p99_ms", "errors",
-                                            "direct", "iterative", "max_rel_residual_sample", "resident_gb",
-                                            "evictions", "queue_max", "concurrency", "total_dof", "inputs")}
+        line["serve"] = {k: sl[k] for k in keys}
+        # the same stream with a budget below the instances' resident total, so
+        # admission evicts idle instances and re-admits them (refactor) under load
+        sl = serve_run(args.ninst, args.nreq, args.rate, args.workers, 12.0, args.eps, emit=False)
+        line["serve_budget_12gb"] = {k: sl[k] for k in keys}
     print(json.dumps(line), flush=True)
 
 
