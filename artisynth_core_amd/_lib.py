@@ -77,6 +77,7 @@ _IP = ctypes.POINTER(ctypes.c_int)
 _DP = ctypes.POINTER(ctypes.c_double)
 _LLP = ctypes.POINTER(ctypes.c_longlong)
 _UCP = ctypes.POINTER(ctypes.c_ubyte)
+_PP = ctypes.POINTER(ctypes.c_void_p)  # arrays of handles / device pointers
 SIGNATURES = [
     ("kkt_create", _P, []),
     ("kkt_destroy", None, [_P]),
@@ -114,6 +115,10 @@ SIGNATURES = [
     ("kkt_multiply", _I, [_P, _DP, _DP]),
     ("kkt_iterative_solve", _I, [_P, _DP, _DP, _DP, _D, _I]),
     ("kkt_iterative_solve_device", _I, [_P, _P, _P, _P, _D, _I]),
+    ("kkt_group_create", _P, []),
+    ("kkt_group_destroy", None, [_P]),
+    ("kkt_group_iterative_solve_device", _I, [_P, _I, _PP, _PP, _PP, _PP, _D, _I, _IP]),
+    ("kkt_group_last_error", ctypes.c_char_p, [_P]),
     ("kkt_set_ir_steps", _I, [_P, _I]),
     ("kkt_last_error", ctypes.c_char_p, [_P]),
     ("kkt_version", ctypes.c_char_p, []),

@@ -104,3 +104,42 @@ class BatchedKKT:
             self.dispose()
         except Exception:
             pass
+
+
+class IterativeGroup:
+    """Stale-factor iterative solves of many factored handles in lockstep
+    (kkt_group_iterative_solve_device): each BiCGStab step is one launch for every
+    handle still iterating, and each handle's result is bitwise the one its own
+    kkt_iterative_solve_device call gives."""
+
+    def __init__(self):
+        self.L = lib()
+        self.g = self.L.kkt_group_create()
+        if not self.g:
+            raise RuntimeError("kkt_group_create failed")
+
+    def solve(self, handles, d_vals, d_b, d_x, tol_rel, max_iter):
+        """handles: raw kkt_solver pointers; d_vals, d_b, d_x: device pointers (ints).
+        Returns (rc, statuses): per handle the iterations or a negative status."""
+        k = len(handles)
+        PA = ctypes.c_void_p * k
+        st = np.zeros(k, np.int32)
+        rc = self.L.kkt_group_iterative_solve_device(
+            self.g, k, PA(*handles), PA(*d_vals), PA(*d_b), PA(*d_x), float(tol_rel), int(max_iter),
+            nptr(st, ctypes.c_int))
+        return rc, st
+
+    def last_error(self):
+        m = self.L.kkt_group_last_error(self.g)
+        return m.decode() if m else None
+
+    def dispose(self):
+        if self.g:
+            self.L.kkt_group_destroy(self.g)
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.dispose()
+        except Exception:
+            pass

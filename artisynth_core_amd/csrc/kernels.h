@@ -105,6 +105,78 @@ struct FactorCounters {  // device ints
   int half_overflow;  // an FP32 solve entry too large for the half-precision copy
 };
 
+// ---- group launches: one launch covers the same step of many handles ----
+// The workgroups of a group launch are dealt to its segments (one per handle)
+// by the prefix counts pre[0..nseg] (pre[0] = 0, pre[nseg] = the grid); a
+// segment's blocks run the same kernel body, with the same block indices and
+// arguments, as that handle's own launch of the step.
+constexpr int SEG_MAX = 256;  // segments (handles) per group launch
+enum SegVecOp : int {
+  SEG_GATHER = 1,  // tout[j] = a[idx[j]] in the front precision             n idx a tout
+  SEG_GATHER_P,    // BiCGStab p = r + beta (p - omega v), gathered to tout    n idx sc a=r b=v d=p tout
+  SEG_SCATTER,     // d[idx[j]] = tin[j]                                      n idx tin d
+  SEG_DSOLVE,      // tout[wpos[g]] = E0 y + E1 y_partner                     n tin=E0 tin2=E1 piv wpos tin3=y tout
+  SEG_EXPAND,      // d[e] = a[idx[e]] (full symmetric copy of new values)    n idx a d
+  SEG_FILL,        // d[i] = v                                                n v d
+  SEG_COPY,        // d[i] = a[i]                                             n a d
+  SEG_BICG_S,      // s = r - alpha v, |s|^2 partials                         n sc a=r b=v d=s part1
+  SEG_BICG_XR,     // x, r updates, |r|^2 and (r_hat, r) partials             n sc a=y b=z c=s e=t d=x f=r g=rh part1 part2
+  SEG_BICG_XA,     // x += alpha y                                            n sc a=y d=x
+  SEG_BICG_INIT,   // the first iteration's scalars                           d=sc
+  SEG_COLLECT,     // d[0..15] = a[0..15] (one block)                         a d
+};
+struct SegVec {
+  int op, pad;
+  long long n;
+  const int* idx;
+  const signed char* piv;
+  const int* wpos;
+  const double* sc;
+  const double *a, *b, *c, *e, *g;
+  double *d, *f, *part1, *part2;
+  const void *tin, *tin2, *tin3;
+  void* tout;
+  double v;
+};
+enum SegRedOp : int {
+  SEG_RED2 = 1,  // k_reduce2: p1 p2 np out1 out2 sc
+  SEG_DOTP,      // k_dot_partial over RED_BLOCKS blocks: n a=p1 b=p2 partial=out1
+  SEG_REDF,      // k_reduce_final (sum): partial=p1 np out1
+};
+struct SegRed {
+  int op, np, n, pad;
+  const double *p1, *p2;
+  double *out1, *out2, *sc;
+};
+struct SegSpmv {
+  int n, mode;
+  const int *rowptr, *col;
+  const double *val, *x, *b, *a1;
+  double *y, *part1, *part2;
+};
+struct SegFwd {
+  DevSym d;
+  const SolveItem* items;
+  const void* F;
+  const void* bp;
+  void *fp, *y, *u;
+};
+struct SegBwd {
+  DevSym d;
+  const SolveItem* items;
+  const void* F;
+  void *wv, *xp;
+};
+int red_blocks();
+template <typename T>
+void launch_seg_vec(const int* pre, int nseg, const SegVec* a, int nblk, hipStream_t st);
+void launch_seg_red(const int* pre, int nseg, const SegRed* a, int nblk, hipStream_t st);
+void launch_seg_spmv(const int* pre, int nseg, const SegSpmv* a, int nblk, hipStream_t st);
+template <typename T, typename S>
+void launch_seg_fwd(const int* pre, int nseg, const SegFwd* a, int nblk, hipStream_t st);
+template <typename T, typename S>
+void launch_seg_bwd(const int* pre, int nseg, const SegBwd* a, int nblk, hipStream_t st);
+
 template <typename T>
 void launch_assemble(const int4* items, int nitems, const DevSym& d, T* F, hipStream_t st);
 template <typename T>

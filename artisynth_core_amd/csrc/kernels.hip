@@ -1224,12 +1224,11 @@ __device__ inline void fwd_rows_block(const SolveItem& it, const DevSym& d, cons
 // product runs in the seg_dot order, so both forms give bitwise the same y and
 // update entries.
 template <typename T, typename S>
-__global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__ items, DevSym d,
-                                                   const S* __restrict__ F,
-                                                   const T* __restrict__ bp, T* __restrict__ fp,
-                                                   T* __restrict__ y, T* __restrict__ u) {
+__device__ inline void fwd_level_body(int bx, const SolveItem* __restrict__ items, const DevSym& d,
+                                      const S* __restrict__ F, const T* __restrict__ bp, T* __restrict__ fp,
+                                      T* __restrict__ y, T* __restrict__ u) {
   __shared__ __align__(16) T fs[KMAX + 8];  // chunk loads may run 7 past k
-  const SolveItem it = items[blockIdx.x];
+  const SolveItem it = items[bx];
   if (it.kind == 0) {
     fwd_small_front<T, S>(it, d, F, bp, y, u);
     return;
@@ -1352,6 +1351,13 @@ __global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__
     }
   }
 }
+template <typename T, typename S>
+__global__ __launch_bounds__(256) void k_fwd_level(const SolveItem* __restrict__ items, DevSym d,
+                                                   const S* __restrict__ F,
+                                                   const T* __restrict__ bp, T* __restrict__ fp,
+                                                   T* __restrict__ y, T* __restrict__ u) {
+  fwd_level_body<T, S>(blockIdx.x, items, d, F, bp, fp, y, u);
+}
 
 // packed solve arrays from the solve form of the fronts: front column i gives
 // forward row i (its rows 0 .. min(i, k), the upper copy) and, for i < k,
@@ -1391,16 +1397,22 @@ __global__ void k_pack_half(long long n, const float* __restrict__ in, _Float16*
 // z = D^-1 y over all pivots: z_g = E0_g y_g + E1_g y_partner, stored at the
 // pivot's entry of the backward vector
 template <typename T>
+__device__ inline void dsolve_body(int bx, int n, const T* __restrict__ E0, const T* __restrict__ E1,
+                                   const signed char* __restrict__ piv, const int* __restrict__ wpos,
+                                   const T* __restrict__ y, T* __restrict__ wv) {
+  const int g = bx * 256 + threadIdx.x;
+  if (g >= n) return;
+  const int pc = piv[g];
+  const int pg = (pc == 2) ? g + 1 : (pc == 0 ? g - 1 : g);
+  wv[wpos[g]] = E0[g] * y[g] + E1[g] * y[pg];
+}
+template <typename T>
 __global__ __launch_bounds__(256) void k_dsolve(int n, const T* __restrict__ E0,
                                                 const T* __restrict__ E1,
                                                 const signed char* __restrict__ piv,
                                                 const int* __restrict__ wpos,
                                                 const T* __restrict__ y, T* __restrict__ wv) {
-  const int g = blockIdx.x * 256 + threadIdx.x;
-  if (g >= n) return;
-  const int pc = piv[g];
-  const int pg = (pc == 2) ? g + 1 : (pc == 0 ? g - 1 : g);
-  wv[wpos[g]] = E0[g] * y[g] + E1[g] * y[pg];
+  dsolve_body<T>(blockIdx.x, n, E0, E1, piv, wpos, y, wv);
 }
 
 // backward from the lower copy (column c, rows below the diagonal, contiguous):
@@ -1412,11 +1424,10 @@ __global__ __launch_bounds__(256) void k_dsolve(int n, const T* __restrict__ E0,
 // A segment of 2^lg lanes per column (lg <= 8: a long column takes two or four
 // waves); item = (s, block of 256 >> lg columns, lg)
 template <typename T, typename S>
-__global__ __launch_bounds__(256) void k_bwd_cols(const SolveItem* __restrict__ items, DevSym d,
-                                                  const S* __restrict__ F, T* wv,
-                                                  T* __restrict__ xp) {
+__device__ inline void bwd_cols_body(int bx, const SolveItem* __restrict__ items, const DevSym& d,
+                                     const S* __restrict__ F, T* wv, T* __restrict__ xp) {
   __shared__ T part[4];
-  const SolveItem it = items[blockIdx.x];
+  const SolveItem it = items[bx];
   const int lg = it.lg;  // 2^lg lanes per column, up to the whole workgroup (lg = 8)
   const int seg = 1 << lg, sl = threadIdx.x & (seg - 1);
   const int c = it.blk * (256 >> lg) + (threadIdx.x >> lg);
@@ -1446,6 +1457,12 @@ __global__ __launch_bounds__(256) void k_bwd_cols(const SolveItem* __restrict__ 
     const int e1 = d.push_ptr[g + 1];
     for (int e = d.push_ptr[g] + sl; e < e1; e += seg) wv[d.push_pos[e]] = -x;
   }
+}
+template <typename T, typename S>
+__global__ __launch_bounds__(256) void k_bwd_cols(const SolveItem* __restrict__ items, DevSym d,
+                                                  const S* __restrict__ F, T* wv,
+                                                  T* __restrict__ xp) {
+  bwd_cols_body<T, S>(blockIdx.x, items, d, F, wv, xp);
 }
 
 // ------------------------------------------------------------------------
@@ -1885,26 +1902,41 @@ __global__ void k_seg_scatter(int nuniq, const int* __restrict__ useg, const int
 // ------------------------------------------------------------------------
 // vectors and SpMV
 // ------------------------------------------------------------------------
+// Kernel bodies take their block index bx as an argument: the kernels below
+// pass blockIdx.x, and the group kernels (many handles' launches of one step in
+// one launch, further down) pass the block's index within its handle's share, so
+// a handle's blocks compute the same values either way.
+template <typename TI, typename TO>
+__device__ inline void gather_perm_body(int bx, int n, const int* __restrict__ perm, const TI* __restrict__ in,
+                                        TO* __restrict__ out) {
+  const int j = bx * 256 + threadIdx.x;
+  if (j < n) out[j] = (TO)in[perm[j]];
+}
 template <typename TI, typename TO>
 __global__ void k_gather_perm(int n, const int* __restrict__ perm, const TI* __restrict__ in,
                               TO* __restrict__ out) {
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j < n) out[j] = (TO)in[perm[j]];
+  gather_perm_body<TI, TO>(blockIdx.x, n, perm, in, out);
+}
+template <typename TI, typename TO>
+__device__ inline void scatter_perm_body(int bx, int n, const int* __restrict__ perm, const TI* __restrict__ in,
+                                         TO* __restrict__ out) {
+  const int j = bx * 256 + threadIdx.x;
+  if (j < n) out[perm[j]] = (TO)in[j];
 }
 template <typename TI, typename TO>
 __global__ void k_scatter_perm(int n, const int* __restrict__ perm, const TI* __restrict__ in,
                                TO* __restrict__ out) {
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j < n) out[perm[j]] = (TO)in[j];
+  scatter_perm_body<TI, TO>(blockIdx.x, n, perm, in, out);
 }
 // the gather of a BiCGStab preconditioner application with the search-direction
 // update of k_bicg_p fused in: for i = perm[j], p_i = r_i + beta (p_i - omega v_i),
 // written back to p and, in the front precision, to out[j]
 template <typename TO>
-__global__ void k_gather_bicg_p(int n, const int* __restrict__ perm, const double* __restrict__ sc,
-                                const double* __restrict__ r, const double* __restrict__ v,
-                                double* __restrict__ p, TO* __restrict__ out) {
-  const int j = blockIdx.x * 256 + threadIdx.x;
+__device__ inline void gather_bicg_p_body(int bx, int n, const int* __restrict__ perm,
+                                          const double* __restrict__ sc, const double* __restrict__ r,
+                                          const double* __restrict__ v, double* __restrict__ p,
+                                          TO* __restrict__ out) {
+  const int j = bx * 256 + threadIdx.x;
   if (j >= n) return;
   const int i = perm[j];
   const double beta = (sc[0] / sc[1]) * (sc[2] / sc[3]);
@@ -1912,11 +1944,21 @@ __global__ void k_gather_bicg_p(int n, const int* __restrict__ perm, const doubl
   p[i] = pi;
   out[j] = (TO)pi;
 }
+template <typename TO>
+__global__ void k_gather_bicg_p(int n, const int* __restrict__ perm, const double* __restrict__ sc,
+                                const double* __restrict__ r, const double* __restrict__ v,
+                                double* __restrict__ p, TO* __restrict__ out) {
+  gather_bicg_p_body<TO>(blockIdx.x, n, perm, sc, r, v, p, out);
+}
 
+__device__ inline void expand_vals_body(int bx, long long nnzf, const int* __restrict__ map,
+                                        const double* __restrict__ vals, double* __restrict__ valsf) {
+  const long long e = (long long)bx * 256 + threadIdx.x;
+  if (e < nnzf) valsf[e] = vals[map[e]];
+}
 __global__ void k_expand_vals(long long nnzf, const int* __restrict__ map,
                               const double* __restrict__ vals, double* __restrict__ valsf) {
-  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (e < nnzf) valsf[e] = vals[map[e]];
+  expand_vals_body(blockIdx.x, nnzf, map, vals, valsf);
 }
 
 // one wave per row
@@ -1932,17 +1974,13 @@ __global__ void k_expand_vals(long long nnzf, const int* __restrict__ map,
 // order.
 constexpr int SPMV_LANES = 16;
 constexpr int SPMV_ROWS = 256 / SPMV_LANES;
-__global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ rowptr,
-                                              const int* __restrict__ col,
-                                              const double* __restrict__ val,
-                                              const double* __restrict__ x,
-                                              const double* __restrict__ b,
-                                              double* __restrict__ y, int mode,
-                                              const double* __restrict__ a1,
-                                              double* __restrict__ part1,
-                                              double* __restrict__ part2) {
+__device__ inline void spmv_body(int bx, int n, const int* __restrict__ rowptr, const int* __restrict__ col,
+                                 const double* __restrict__ val, const double* __restrict__ x,
+                                 const double* __restrict__ b, double* __restrict__ y, int mode,
+                                 const double* __restrict__ a1, double* __restrict__ part1,
+                                 double* __restrict__ part2) {
   __shared__ double sm[2][4];
-  const int row = blockIdx.x * SPMV_ROWS + threadIdx.x / SPMV_LANES;
+  const int row = bx * SPMV_ROWS + threadIdx.x / SPMV_LANES;
   const int sl = threadIdx.x & (SPMV_LANES - 1);
   double acc = 0.0;
   if (row < n) {
@@ -1989,27 +2027,48 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ row
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      if (part1) part1[blockIdx.x] = (sm[0][0] + sm[0][1]) + (sm[0][2] + sm[0][3]);
-      if (part2) part2[blockIdx.x] = (sm[1][0] + sm[1][1]) + (sm[1][2] + sm[1][3]);
+      if (part1) part1[bx] = (sm[0][0] + sm[0][1]) + (sm[0][2] + sm[0][3]);
+      if (part2) part2[bx] = (sm[1][0] + sm[1][1]) + (sm[1][2] + sm[1][3]);
     }
   }
+}
+__global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ rowptr,
+                                              const int* __restrict__ col,
+                                              const double* __restrict__ val,
+                                              const double* __restrict__ x,
+                                              const double* __restrict__ b,
+                                              double* __restrict__ y, int mode,
+                                              const double* __restrict__ a1,
+                                              double* __restrict__ part1,
+                                              double* __restrict__ part2) {
+  spmv_body(blockIdx.x, n, rowptr, col, val, x, b, y, mode, a1, part1, part2);
 }
 
 constexpr int RED_BLOCKS = 512;
 
-__global__ __launch_bounds__(256) void k_dot_partial(int n, const double* __restrict__ a,
-                                                     const double* __restrict__ b,
-                                                     double* __restrict__ partial) {
+__device__ inline void dot_partial_body(int bx, int n, const double* __restrict__ a,
+                                        const double* __restrict__ b, double* __restrict__ partial) {
   __shared__ double sm[4];
   double acc = 0.0;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += RED_BLOCKS * 256) acc += a[i] * b[i];
+  for (int i = bx * 256 + threadIdx.x; i < n; i += RED_BLOCKS * 256) acc += a[i] * b[i];
   acc = wave_sum(acc);
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
   __syncthreads();
-  if (threadIdx.x == 0) partial[blockIdx.x] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+  if (threadIdx.x == 0) partial[bx] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
 }
+__global__ __launch_bounds__(256) void k_dot_partial(int n, const double* __restrict__ a,
+                                                     const double* __restrict__ b,
+                                                     double* __restrict__ partial) {
+  dot_partial_body(blockIdx.x, n, a, b, partial);
+}
+__device__ inline void reduce_final_body(const double* __restrict__ partial, int np, double* __restrict__ out,
+                                         int is_max);
 __global__ __launch_bounds__(256) void k_reduce_final(const double* __restrict__ partial, int np,
                                                       double* __restrict__ out, int is_max) {
+  reduce_final_body(partial, np, out, is_max);
+}
+__device__ inline void reduce_final_body(const double* __restrict__ partial, int np, double* __restrict__ out,
+                                         int is_max) {
   __shared__ double sm[4];
   double acc = 0.0;
   // thread t takes entries t, t + 256, ... in turn; loads issue 8 at a time
@@ -2104,10 +2163,18 @@ __global__ __launch_bounds__(256) void k_reduce_final_m(const double* __restrict
 // the BiCGStab kernels below): alpha = rho / (r_hat, v), omega = (t, s) / (t, t),
 // rho_prev = rho, rho = out2's value (the caller passes out2 = sc + 10).
 constexpr int RED_AHEAD = 8;
+__device__ inline void reduce2_body(const double* __restrict__ p1, const double* __restrict__ p2, int np,
+                                    double* __restrict__ out1, double* __restrict__ out2,
+                                    double* __restrict__ sc);
 __global__ __launch_bounds__(256) void k_reduce2(const double* __restrict__ p1,
                                                  const double* __restrict__ p2, int np,
                                                  double* __restrict__ out1,
                                                  double* __restrict__ out2, double* __restrict__ sc) {
+  reduce2_body(p1, p2, np, out1, out2, sc);
+}
+__device__ inline void reduce2_body(const double* __restrict__ p1, const double* __restrict__ p2, int np,
+                                    double* __restrict__ out1, double* __restrict__ out2,
+                                    double* __restrict__ sc) {
   __shared__ double sm[2][4];
   double a = 0.0, c = 0.0;
   for (int i0 = threadIdx.x; i0 < np; i0 += 256 * RED_AHEAD) {
@@ -2147,13 +2214,13 @@ __global__ __launch_bounds__(256) void k_reduce2(const double* __restrict__ p1,
     }
   }
 }
-// one block's sum of v over its 256 threads into part[blockIdx.x] (fixed order)
-__device__ inline void block_sum_to(double v, double* part) {
+// one block's sum of v over its 256 threads into part[bx] (fixed order)
+__device__ inline void block_sum_to(double v, double* part, int bx) {
   __shared__ double sm[4];
   v = wave_sum(v);
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
   __syncthreads();
-  if (threadIdx.x == 0) part[blockIdx.x] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+  if (threadIdx.x == 0) part[bx] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
 }
 __global__ __launch_bounds__(256) void k_absmax_partial(long long n, const double* __restrict__ v,
                                                         double* __restrict__ partial) {
@@ -2184,11 +2251,10 @@ __global__ void k_axpbypcz(int n, double a, const double* __restrict__ x, double
 // 8 |r|^2, 10 the next rho (r_hat, r)
 // ------------------------------------------------------------------------
 // s = r - alpha v with alpha = rho / (r_hat, v); part[block] = sum of s_i^2
-__global__ __launch_bounds__(256) void k_bicg_s(int n, const double* __restrict__ sc,
-                                                const double* __restrict__ r,
-                                                const double* __restrict__ v, double* __restrict__ s,
-                                                double* __restrict__ part) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
+__device__ inline void bicg_s_body(int bx, int n, const double* __restrict__ sc, const double* __restrict__ r,
+                                   const double* __restrict__ v, double* __restrict__ s,
+                                   double* __restrict__ part) {
+  const int i = bx * 256 + threadIdx.x;
   double d = 0.0;
   if (i < n) {
     const double alpha = sc[0] / sc[4];
@@ -2196,18 +2262,22 @@ __global__ __launch_bounds__(256) void k_bicg_s(int n, const double* __restrict_
     s[i] = si;
     d = si * si;
   }
-  block_sum_to(d, part);
+  block_sum_to(d, part, bx);
+}
+__global__ __launch_bounds__(256) void k_bicg_s(int n, const double* __restrict__ sc,
+                                                const double* __restrict__ r,
+                                                const double* __restrict__ v, double* __restrict__ s,
+                                                double* __restrict__ part) {
+  bicg_s_body(blockIdx.x, n, sc, r, v, s, part);
 }
 // x += alpha y + omega z ; r = s - omega t, omega = (t,s)/(t,t);
 // part1[block] = sum of r_i^2, part2[block] = sum of r_hat_i r_i (the next rho)
-__global__ __launch_bounds__(256) void k_bicg_xr(int n, const double* __restrict__ sc,
-                                                 const double* __restrict__ y,
-                                                 const double* __restrict__ z,
-                                                 const double* __restrict__ s,
-                                                 const double* __restrict__ t, double* __restrict__ x,
-                                                 double* __restrict__ r, const double* __restrict__ rh,
-                                                 double* __restrict__ part1, double* __restrict__ part2) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
+__device__ inline void bicg_xr_body(int bx, int n, const double* __restrict__ sc, const double* __restrict__ y,
+                                    const double* __restrict__ z, const double* __restrict__ s,
+                                    const double* __restrict__ t, double* __restrict__ x, double* __restrict__ r,
+                                    const double* __restrict__ rh, double* __restrict__ part1,
+                                    double* __restrict__ part2) {
+  const int i = bx * 256 + threadIdx.x;
   double d1 = 0.0, d2 = 0.0;
   if (i < n) {
     const double alpha = sc[0] / sc[4], omega = sc[6] / sc[7];
@@ -2217,30 +2287,45 @@ __global__ __launch_bounds__(256) void k_bicg_xr(int n, const double* __restrict
     d1 = ri * ri;
     d2 = rh[i] * ri;
   }
-  block_sum_to(d1, part1);
+  block_sum_to(d1, part1, bx);
   __syncthreads();  // block_sum_to's LDS is reused
-  block_sum_to(d2, part2);
+  block_sum_to(d2, part2, bx);
+}
+__global__ __launch_bounds__(256) void k_bicg_xr(int n, const double* __restrict__ sc,
+                                                 const double* __restrict__ y,
+                                                 const double* __restrict__ z,
+                                                 const double* __restrict__ s,
+                                                 const double* __restrict__ t, double* __restrict__ x,
+                                                 double* __restrict__ r, const double* __restrict__ rh,
+                                                 double* __restrict__ part1, double* __restrict__ part2) {
+  bicg_xr_body(blockIdx.x, n, sc, y, z, s, t, x, r, rh, part1, part2);
 }
 // x += alpha y (early exit on small |s|)
-__global__ void k_bicg_xa(int n, const double* __restrict__ sc, const double* __restrict__ y,
-                          double* __restrict__ x) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
+__device__ inline void bicg_xa_body(int bx, int n, const double* __restrict__ sc, const double* __restrict__ y,
+                                    double* __restrict__ x) {
+  const int i = bx * 256 + threadIdx.x;
   if (i >= n) return;
   x[i] += (sc[0] / sc[4]) * y[i];
 }
+__global__ void k_bicg_xa(int n, const double* __restrict__ sc, const double* __restrict__ y,
+                          double* __restrict__ x) {
+  bicg_xa_body(blockIdx.x, n, sc, y, x);
+}
 // the first iteration's scalars: rho = |r0|^2 (r_hat = r0), rho_prev = alpha =
 // omega = 1 (later iterations' come from k_reduce2 with sc)
-__global__ void k_bicg_init(double* __restrict__ sc) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ inline void bicg_init_body(int bx, double* __restrict__ sc) {
+  if (threadIdx.x != 0 || bx != 0) return;
   sc[0] = sc[8];
   sc[1] = 1.0;
   sc[2] = 1.0;
   sc[3] = 1.0;
 }
-__global__ void k_fill(long long n, double v, double* __restrict__ x) {
-  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+__global__ void k_bicg_init(double* __restrict__ sc) { bicg_init_body(blockIdx.x, sc); }
+__device__ inline void fill_body(int bx, long long n, double v, double* __restrict__ x) {
+  const long long i = (long long)bx * 256 + threadIdx.x;
   if (i < n) x[i] = v;
 }
+__global__ void k_fill(long long n, double v, double* __restrict__ x) { fill_body(blockIdx.x, n, v, x); }
 __global__ void k_zero_counters(FactorCounters* c, int* __restrict__ rctr, int nr) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i == 0) { c->npos = 0; c->nneg = 0; c->npert = 0; c->n2x2 = 0; c->npivoted = 0; c->half_overflow = 0; }
@@ -2613,5 +2698,128 @@ template void launch_scatter_perm<float, double>(int, const int*, const float*, 
                                                  long long, hipStream_t);
 KKT_INST_M(float)
 KKT_INST_M(double)
+
+// ------------------------------------------------------------------------
+// Group launches: the same step of many handles' solves in one launch.  Block
+// b belongs to segment h with pre[h] <= b < pre[h + 1] (the prefix counts go
+// through LDS, then a binary search) and runs the step's body with block index
+// b - pre[h] and the segment's arguments, i.e. exactly what that handle's own
+// launch of the step would run in that block.
+// ------------------------------------------------------------------------
+__device__ inline int seg_find(const int* __restrict__ pre, int nseg, int& bx) {
+  __shared__ int sp[SEG_MAX + 1];
+  for (int i = threadIdx.x; i <= nseg; i += 256) sp[i] = pre[i];
+  __syncthreads();
+  const int b = blockIdx.x;
+  int lo = 0, hi = nseg;  // sp[lo] <= b < sp[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (sp[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  // every lane found the same segment: say so, so that the segment's arguments
+  // load into scalar registers
+  const int h = __builtin_amdgcn_readfirstlane(lo);
+  bx = __builtin_amdgcn_readfirstlane(b - sp[lo]);
+  return h;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_seg_vec(const int* __restrict__ pre, int nseg,
+                                                 const SegVec* __restrict__ args) {
+  int bx;
+  const int h = seg_find(pre, nseg, bx);
+  const SegVec& a = args[h];
+  const int n = (int)a.n;
+  switch (a.op) {
+    case SEG_GATHER: gather_perm_body<double, T>(bx, n, a.idx, a.a, (T*)a.tout); break;
+    case SEG_GATHER_P: gather_bicg_p_body<T>(bx, n, a.idx, a.sc, a.a, a.b, a.d, (T*)a.tout); break;
+    case SEG_SCATTER: scatter_perm_body<T, double>(bx, n, a.idx, (const T*)a.tin, a.d); break;
+    case SEG_DSOLVE:
+      dsolve_body<T>(bx, n, (const T*)a.tin, (const T*)a.tin2, a.piv, a.wpos, (const T*)a.tin3, (T*)a.tout);
+      break;
+    case SEG_EXPAND: expand_vals_body(bx, a.n, a.idx, a.a, a.d); break;
+    case SEG_FILL: fill_body(bx, a.n, a.v, a.d); break;
+    case SEG_COPY: {
+      const long long i = (long long)bx * 256 + threadIdx.x;
+      if (i < a.n) a.d[i] = a.a[i];
+    } break;
+    case SEG_BICG_S: bicg_s_body(bx, n, a.sc, a.a, a.b, a.d, a.part1); break;
+    case SEG_BICG_XR: bicg_xr_body(bx, n, a.sc, a.a, a.b, a.c, a.e, a.d, a.f, a.g, a.part1, a.part2); break;
+    case SEG_BICG_XA: bicg_xa_body(bx, n, a.sc, a.a, a.d); break;
+    case SEG_BICG_INIT: bicg_init_body(bx, a.d); break;
+    case SEG_COLLECT:
+      if (bx == 0 && threadIdx.x < 16) a.d[threadIdx.x] = a.a[threadIdx.x];
+      break;
+    default: break;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_seg_red(const int* __restrict__ pre, int nseg,
+                                                 const SegRed* __restrict__ args) {
+  int bx;
+  const int h = seg_find(pre, nseg, bx);
+  const SegRed& a = args[h];
+  switch (a.op) {
+    case SEG_RED2: reduce2_body(a.p1, a.p2, a.np, a.out1, a.out2, a.sc); break;
+    case SEG_DOTP: dot_partial_body(bx, a.n, a.p1, a.p2, a.out1); break;
+    case SEG_REDF: reduce_final_body(a.p1, a.np, a.out1, 0); break;
+    default: break;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_seg_spmv(const int* __restrict__ pre, int nseg,
+                                                  const SegSpmv* __restrict__ args) {
+  int bx;
+  const int h = seg_find(pre, nseg, bx);
+  const SegSpmv& a = args[h];
+  spmv_body(bx, a.n, a.rowptr, a.col, a.val, a.x, a.b, a.y, a.mode, a.a1, a.part1, a.part2);
+}
+
+template <typename T, typename S>
+__global__ __launch_bounds__(256) void k_seg_fwd(const int* __restrict__ pre, int nseg,
+                                                 const SegFwd* __restrict__ args) {
+  int bx;
+  const int h = seg_find(pre, nseg, bx);
+  const SegFwd& a = args[h];
+  fwd_level_body<T, S>(bx, a.items, a.d, (const S*)a.F, (const T*)a.bp, (T*)a.fp, (T*)a.y, (T*)a.u);
+}
+
+template <typename T, typename S>
+__global__ __launch_bounds__(256) void k_seg_bwd(const int* __restrict__ pre, int nseg,
+                                                 const SegBwd* __restrict__ args) {
+  int bx;
+  const int h = seg_find(pre, nseg, bx);
+  const SegBwd& a = args[h];
+  bwd_cols_body<T, S>(bx, a.items, a.d, (const S*)a.F, (T*)a.wv, (T*)a.xp);
+}
+
+int red_blocks() { return RED_BLOCKS; }
+template <typename T>
+void launch_seg_vec(const int* pre, int nseg, const SegVec* a, int nblk, hipStream_t st) {
+  if (nblk > 0) hipLaunchKernelGGL(k_seg_vec<T>, dim3(nblk), dim3(256), 0, st, pre, nseg, a);
+}
+void launch_seg_red(const int* pre, int nseg, const SegRed* a, int nblk, hipStream_t st) {
+  if (nblk > 0) hipLaunchKernelGGL(k_seg_red, dim3(nblk), dim3(256), 0, st, pre, nseg, a);
+}
+void launch_seg_spmv(const int* pre, int nseg, const SegSpmv* a, int nblk, hipStream_t st) {
+  if (nblk > 0) hipLaunchKernelGGL(k_seg_spmv, dim3(nblk), dim3(256), 0, st, pre, nseg, a);
+}
+template <typename T, typename S>
+void launch_seg_fwd(const int* pre, int nseg, const SegFwd* a, int nblk, hipStream_t st) {
+  if (nblk > 0) hipLaunchKernelGGL((k_seg_fwd<T, S>), dim3(nblk), dim3(256), 0, st, pre, nseg, a);
+}
+template <typename T, typename S>
+void launch_seg_bwd(const int* pre, int nseg, const SegBwd* a, int nblk, hipStream_t st) {
+  if (nblk > 0) hipLaunchKernelGGL((k_seg_bwd<T, S>), dim3(nblk), dim3(256), 0, st, pre, nseg, a);
+}
+template void launch_seg_vec<float>(const int*, int, const SegVec*, int, hipStream_t);
+template void launch_seg_vec<double>(const int*, int, const SegVec*, int, hipStream_t);
+template void launch_seg_fwd<double, double>(const int*, int, const SegFwd*, int, hipStream_t);
+template void launch_seg_fwd<float, float>(const int*, int, const SegFwd*, int, hipStream_t);
+template void launch_seg_fwd<float, _Float16>(const int*, int, const SegFwd*, int, hipStream_t);
+template void launch_seg_bwd<double, double>(const int*, int, const SegBwd*, int, hipStream_t);
+template void launch_seg_bwd<float, float>(const int*, int, const SegBwd*, int, hipStream_t);
+template void launch_seg_bwd<float, _Float16>(const int*, int, const SegBwd*, int, hipStream_t);
 
 }  // namespace kkt

@@ -2096,6 +2096,549 @@ struct kkt_solver {
 };
 
 // =====================================================================
+// Group iterative solves: the stale-factor BiCGStab of many handles (one per
+// serving instance) in lockstep.  Every step of an iteration -- the gather, each
+// forward tree level, D^-1, each backward level, the scatter, the SpMV, the
+// reductions and the vector updates -- is one launch for all the group's active
+// handles: the launch's workgroups are dealt to per-handle segments, and each
+// segment runs the step's kernel body with the handle's own arguments and block
+// indices (kernels.hip, group launches).  A handle's result is therefore
+// bitwise the one kkt_iterative_solve_device gives it, whatever else is in the
+// group, while the group pays each step's launch and latency once.  Handles
+// leave the group as they converge, break down or reach maxIter.
+// =====================================================================
+struct kkt_group {
+  hipStream_t st = nullptr;
+  // device blob: per-launch segment prefixes and arguments; pinned staging
+  char* d_blob = nullptr;
+  char* h_blob = nullptr;
+  size_t blob_bytes = 0;
+  int gen = 0;             // bumped when the blob is reallocated
+  double* d_sc = nullptr;  // gathered scalars: 16 per handle
+  double* h_sc = nullptr;
+  int sc_cap = 0;
+  std::string err;
+  bool has_err = false;
+
+  int fail(int code, const std::string& m) {
+    err = m;
+    has_err = true;
+    return code;
+  }
+  ~kkt_group() {
+    DevOpLock lk(g_capture_mu);
+    if (d_blob) (void)hipFree(d_blob);
+    if (h_blob) (void)hipHostFree(h_blob);
+    if (d_sc) (void)hipFree(d_sc);
+    if (h_sc) (void)hipHostFree(h_sc);
+    if (st) (void)hipStreamDestroy(st);
+  }
+  int ensure(size_t bytes, int nh) {
+    DevOpLock lk(g_capture_mu);
+    if (bytes > blob_bytes) {
+      if (d_blob) (void)hipFree(d_blob);
+      if (h_blob) (void)hipHostFree(h_blob);
+      d_blob = h_blob = nullptr;
+      blob_bytes = 0;
+      const size_t cap = std::max(bytes + bytes / 2, (size_t)1 << 20);
+      if (hipMalloc((void**)&d_blob, cap) != hipSuccess ||
+          hipHostMalloc((void**)&h_blob, cap, hipHostMallocDefault) != hipSuccess)
+        return fail(KKT_ERR_ALLOC, "hipMalloc failed (group tables)");
+      blob_bytes = cap;
+      gen++;
+    }
+    if (nh > sc_cap) {
+      if (d_sc) (void)hipFree(d_sc);
+      if (h_sc) (void)hipHostFree(h_sc);
+      d_sc = h_sc = nullptr;
+      sc_cap = 0;
+      if (hipMalloc((void**)&d_sc, (size_t)16 * nh * sizeof(double)) != hipSuccess ||
+          hipHostMalloc((void**)&h_sc, (size_t)16 * nh * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return fail(KKT_ERR_ALLOC, "hipMalloc failed (group scalars)");
+      sc_cap = nh;
+    }
+    return KKT_OK;
+  }
+};
+
+namespace {
+
+// One group launch under construction: its family, segment prefixes and
+// arguments (packed into the blob at `off`).
+enum GroupFam { GF_VEC, GF_RED, GF_SPMV, GF_FWD, GF_BWD };
+struct GLaunch {
+  int fam = GF_VEC;
+  std::vector<int> pre{0};
+  std::vector<char> args;
+  size_t off = 0;  // blob offset of pre, args at off + args_off
+  size_t args_off = 0;
+  template <typename A>
+  void add(const A& a, int nblk) {
+    if (nblk <= 0) return;
+    const size_t o = args.size();
+    args.resize(o + sizeof(A));
+    std::memcpy(args.data() + o, &a, sizeof(A));
+    pre.push_back(pre.back() + nblk);
+  }
+  int nseg() const { return (int)pre.size() - 1; }
+  int nblk() const { return pre.back(); }
+};
+
+// Lays the launches out in the staging buffer (16-byte aligned sections) from
+// byte `base`; returns the end.
+bool segs_fit(const std::vector<GLaunch>& L) {
+  for (const GLaunch& g : L)
+    if (g.nseg() > SEG_MAX) return false;
+  return true;
+}
+
+size_t pack_launches(std::vector<GLaunch>& L, char* h, size_t base) {
+  size_t o = base;
+  auto al = [](size_t v) { return (v + 15) & ~(size_t)15; };
+  for (GLaunch& g : L) {
+    o = al(o);
+    g.off = o;
+    if (h) std::memcpy(h + o, g.pre.data(), g.pre.size() * sizeof(int));
+    o = al(o + g.pre.size() * sizeof(int));
+    g.args_off = o;
+    if (h) std::memcpy(h + o, g.args.data(), g.args.size());
+    o += g.args.size();
+  }
+  return o;
+}
+
+template <typename T, typename S>
+void run_launches(const std::vector<GLaunch>& L, const char* d, hipStream_t st) {
+  for (const GLaunch& g : L) {
+    // the group kernels' segment table holds SEG_MAX + 1 prefixes (checked by
+    // the driver before anything is uploaded)
+    if (!g.nblk() || g.nseg() > SEG_MAX) continue;
+    const int* pre = (const int*)(d + g.off);
+    const void* a = d + g.args_off;
+    switch (g.fam) {
+      case GF_VEC: launch_seg_vec<T>(pre, g.nseg(), (const SegVec*)a, g.nblk(), st); break;
+      case GF_RED: launch_seg_red(pre, g.nseg(), (const SegRed*)a, g.nblk(), st); break;
+      case GF_SPMV: launch_seg_spmv(pre, g.nseg(), (const SegSpmv*)a, g.nblk(), st); break;
+      case GF_FWD: launch_seg_fwd<T, S>(pre, g.nseg(), (const SegFwd*)a, g.nblk(), st); break;
+      case GF_BWD: launch_seg_bwd<T, S>(pre, g.nseg(), (const SegBwd*)a, g.nblk(), st); break;
+    }
+  }
+}
+
+SegVec vop(int op, long long n) {
+  SegVec v;
+  std::memset(&v, 0, sizeof v);
+  v.op = op;
+  v.n = n;
+  return v;
+}
+int vblk(long long n) { return (int)((n + 255) / 256); }
+
+}  // namespace
+
+// the group driver, for handles of one solve precision class q (0: FP64
+// arrays, 1: FP32, 2: FP32 with the half-precision preconditioner copy)
+template <typename T, typename S>
+static int group_bicgstab(kkt_group* g, int K, kkt_solver* const* hs, const double* const* vin,
+                          const double* const* bin, double* const* xout, double tolRel, int maxIter, int* status) {
+  const double eps = 1e-30;
+  hipStream_t st = g->st;
+  // the gathered scalars' buffer holds every handle's 16 from the start: the
+  // collect launches below take its address when they are built
+  {
+    const int rc0 = g->ensure(0, K);
+    if (rc0) return rc0;
+  }
+  // per handle: the arrays of its single-handle path (kkt_solver::bicgstab,
+  // run_tri_solve)
+  struct HV {
+    kkt_solver* h;
+    double *r, *rh, *p, *v, *s, *t, *y, *z, *x, *sc, *pa, *pb;
+    int nbs, nbv;
+    T *bp, *fp, *yy, *xp, *wv, *u;
+    const T *E0, *E1;
+    const S* PG;
+    double bn;
+    int it;
+  };
+  std::vector<HV> H(K);
+  for (int q = 0; q < K; q++) {
+    kkt_solver* h = hs[q];
+    HV& a = H[q];
+    a.h = h;
+    a.r = h->vec(0); a.rh = h->vec(1); a.p = h->vec(2); a.v = h->vec(3); a.s = h->vec(4);
+    a.t = h->vec(5); a.y = h->vec(6); a.z = h->vec(7); a.x = h->vec(8);
+    a.sc = h->d_scal;
+    a.nbs = spmv_blocks(h->n);
+    a.nbv = vec_blocks(h->n);
+    a.pa = h->d_part2;
+    a.pb = h->d_part2 + std::max(a.nbs, 1);
+    T* sv = (T*)h->d_svec;
+    const size_t n = (size_t)h->n;
+    a.bp = sv; a.fp = sv + n; a.yy = sv + 2 * n; a.xp = sv + 3 * n; a.wv = sv + 4 * n;
+    T* F = (T*)(h->fprec == KKT_PREC_FP64 ? (void*)h->d_F64 : (void*)h->d_F32);
+    T* PK = (T*)(h->fprec == KKT_PREC_FP64 ? (void*)h->d_PK64 : (void*)h->d_PK32);
+    a.u = h->pk_slot(PK);
+    a.E0 = F + h->front_elems;
+    a.E1 = a.E0 + h->n;
+    a.PG = (h->solve_q(true) == 2) ? (const S*)h->d_PK16 : (const S*)PK;
+    a.bn = 1.0;
+    a.it = 0;
+    status[q] = 0;
+  }
+  // the triangular solve's core (forward levels, D^-1, backward levels) for a
+  // set of handles: bitwise the launches of run_tri_solve, level by level
+  auto core = [&](const std::vector<int>& act, std::vector<GLaunch>& L) {
+    int maxl = 0;
+    for (int q : act) maxl = std::max(maxl, H[q].h->S.nlevels);
+    for (int Lv = 0; Lv < maxl; Lv++) {
+      GLaunch g1, g2;
+      g1.fam = g2.fam = GF_FWD;
+      for (int q : act) {
+        kkt_solver* h = H[q].h;
+        if (Lv >= h->S.nlevels || !h->ff_l[Lv].cnt) continue;
+        SegFwd f;
+        f.d = h->ds; f.F = h->pk_fwd(H[q].PG); f.bp = H[q].bp; f.fp = H[q].fp; f.y = H[q].yy; f.u = H[q].u;
+        f.items = h->d_ff + h->ff_l[Lv].off;
+        g1.add(f, h->ff_l[Lv].cnt);
+        if (h->fs_l[Lv].cnt) {
+          f.items = h->d_ff + h->fs_l[Lv].off;
+          g2.add(f, h->fs_l[Lv].cnt);
+        }
+      }
+      L.push_back(std::move(g1));
+      L.push_back(std::move(g2));
+    }
+    GLaunch gd;
+    for (int q : act) {
+      SegVec v = vop(SEG_DSOLVE, H[q].h->n);
+      v.tin = H[q].E0; v.tin2 = H[q].E1; v.piv = H[q].h->d_piv; v.wpos = H[q].h->ds.wpos;
+      v.tin3 = H[q].yy; v.tout = H[q].wv;
+      gd.add(v, vblk(H[q].h->n));
+    }
+    L.push_back(std::move(gd));
+    for (int j = 0; j < maxl; j++) {
+      GLaunch gb;
+      gb.fam = GF_BWD;
+      for (int q : act) {
+        kkt_solver* h = H[q].h;
+        const int Lv = h->S.nlevels - 1 - j;
+        if (Lv < 0 || !h->bc_l[Lv].cnt) continue;
+        SegBwd b;
+        b.d = h->ds; b.F = h->pk_bwd(H[q].PG); b.wv = H[q].wv; b.xp = H[q].xp;
+        b.items = h->d_bc + h->bc_l[Lv].off;
+        gb.add(b, h->bc_l[Lv].cnt);
+      }
+      L.push_back(std::move(gb));
+    }
+  };
+  // uploads the core tables (kept while the active set is unchanged) and a
+  // phase's own launches, runs pre + core + post, gathers the scalars of `act`
+  // and waits for them
+  std::vector<GLaunch> coreL;
+  std::vector<int> core_set;
+  size_t core_end = 0;  // 0: the core tables are not (or no longer) in the device blob
+  int core_gen = -1;
+  auto run_phase = [&](const std::vector<int>& act, std::vector<GLaunch>& pre, bool with_core,
+                       std::vector<GLaunch>& post) -> int {
+    if (with_core && act != core_set) {
+      coreL.clear();
+      core(act, coreL);
+      core_set = act;
+      core_end = 0;
+    }
+    if ((int)act.size() > g->sc_cap || !g->d_sc)
+      return g->fail(KKT_ERR_ARGUMENT, "group scalar buffer smaller than the group");
+    GLaunch col;
+    for (size_t i = 0; i < act.size(); i++) {
+      SegVec v = vop(SEG_COLLECT, 16);
+      v.a = H[act[i]].sc;
+      v.d = g->d_sc + 16 * i;
+      col.add(v, 1);
+    }
+    post.push_back(std::move(col));
+    if (!segs_fit(coreL) || !segs_fit(pre) || !segs_fit(post))
+      return g->fail(KKT_ERR_ARGUMENT, "group launch exceeds SEG_MAX segments");
+    const size_t need_core = with_core ? pack_launches(coreL, nullptr, 0) : 0;
+    size_t need = pack_launches(pre, nullptr, need_core);
+    need = pack_launches(post, nullptr, need);
+    int rc = g->ensure(need, (int)act.size());
+    if (rc) return rc;
+    if (core_gen != g->gen) core_end = 0;  // a reallocated blob lost the core tables
+    if (with_core && core_end == 0) {
+      core_end = pack_launches(coreL, g->h_blob, 0);
+      core_gen = g->gen;
+      if (hipMemcpyAsync(g->d_blob, g->h_blob, core_end, hipMemcpyHostToDevice, st) != hipSuccess)
+        return g->fail(KKT_ERR_COPY, "group table upload failed");
+    }
+    const size_t b0 = (need_core + 15) & ~(size_t)15;
+    size_t e = pack_launches(pre, g->h_blob, b0);
+    e = pack_launches(post, g->h_blob, e);
+    if (e > b0 && hipMemcpyAsync(g->d_blob + b0, g->h_blob + b0, e - b0, hipMemcpyHostToDevice, st) != hipSuccess)
+      return g->fail(KKT_ERR_COPY, "group table upload failed");
+    run_launches<T, S>(pre, g->d_blob, st);
+    if (with_core) run_launches<T, S>(coreL, g->d_blob, st);
+    run_launches<T, S>(post, g->d_blob, st);
+    if (hipMemcpyAsync(g->h_sc, g->d_sc, (size_t)16 * act.size() * sizeof(double), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return g->fail(KKT_ERR_REDUCTION, "group step failed");
+    return KKT_OK;
+  };
+  auto finish = [&](int q, int code, const char* msg) {
+    status[q] = code;
+    if (code <= 0) H[q].h->fail(code, msg);
+    else H[q].h->ok();
+  };
+
+  // ---- start: new values, x0 = M^-1 b, r = b - A x0, |b|, |r| ----
+  std::vector<int> act(K);
+  for (int q = 0; q < K; q++) act[q] = q;
+  {
+    std::vector<GLaunch> pre, post;
+    GLaunch cp, ex, ga;
+    for (int q : act) {
+      kkt_solver* h = H[q].h;
+      if (vin[q] != h->d_vals) {
+        SegVec v = vop(SEG_COPY, h->nnz);
+        v.a = vin[q]; v.d = h->d_vals;
+        cp.add(v, vblk(h->nnz));
+      }
+      SegVec e = vop(SEG_EXPAND, h->nnzf);
+      e.idx = h->d_map_f; e.a = h->d_vals; e.d = h->d_valsf;
+      ex.add(e, vblk(h->nnzf));
+      SegVec gg = vop(SEG_GATHER, h->n);
+      gg.idx = h->d_perm; gg.a = bin[q]; gg.tout = H[q].bp;
+      ga.add(gg, vblk(h->n));
+    }
+    pre.push_back(std::move(cp));
+    pre.push_back(std::move(ex));
+    pre.push_back(std::move(ga));
+    GLaunch sc, sp, dp, rf;
+    sp.fam = GF_SPMV;
+    dp.fam = rf.fam = GF_RED;
+    for (int q : act) {
+      kkt_solver* h = H[q].h;
+      SegVec v = vop(SEG_SCATTER, h->n);
+      v.idx = h->d_perm; v.tin = H[q].xp; v.d = H[q].x;
+      sc.add(v, vblk(h->n));
+      SegSpmv m{};
+      m.n = h->n; m.mode = 1; m.rowptr = h->d_rowptr_f; m.col = h->d_col_f; m.val = h->d_valsf;
+      m.x = H[q].x; m.b = bin[q]; m.y = H[q].r;
+      sp.add(m, spmv_blocks(h->n));
+      // |b|^2 into slot 9 and |r|^2 into slot 8, each through its own half of
+      // the handle's partials
+      SegRed d1{}, d2{};
+      d1.op = d2.op = SEG_DOTP;
+      d1.n = d2.n = h->n;
+      d1.p1 = d1.p2 = bin[q]; d1.out1 = h->d_partial;
+      d2.p1 = d2.p2 = H[q].r; d2.out1 = h->d_partial + red_blocks();
+      dp.add(d1, red_blocks());
+      dp.add(d2, red_blocks());
+      SegRed f1{}, f2{};
+      f1.op = f2.op = SEG_REDF;
+      f1.np = f2.np = red_blocks();
+      f1.p1 = h->d_partial; f1.out1 = H[q].sc + 9;
+      f2.p1 = h->d_partial + red_blocks(); f2.out1 = H[q].sc + 8;
+      rf.add(f1, 1);
+      rf.add(f2, 1);
+    }
+    post.push_back(std::move(sc));
+    post.push_back(std::move(sp));
+    post.push_back(std::move(dp));
+    post.push_back(std::move(rf));
+    int rc = run_phase(act, pre, true, post);
+    if (rc) return rc;
+    std::vector<int> next;
+    GLaunch ini_rh, ini_p, ini_v, ini_sc;
+    for (size_t i = 0; i < act.size(); i++) {
+      const int q = act[i];
+      const double* s = g->h_sc + 16 * i;
+      double bn = std::sqrt(s[9]);
+      const double rn = std::sqrt(s[8]);
+      if (bn == 0.0) bn = 1.0;
+      H[q].bn = bn;
+      H[q].it = 1;
+      if (rn / bn <= tolRel) {
+        finish(q, 1, "");
+        continue;
+      }
+      next.push_back(q);
+      kkt_solver* h = H[q].h;
+      SegVec c = vop(SEG_COPY, h->n);
+      c.a = H[q].r; c.d = H[q].rh;
+      ini_rh.add(c, vblk(h->n));
+      SegVec f = vop(SEG_FILL, h->n);
+      f.d = H[q].p;
+      ini_p.add(f, vblk(h->n));
+      f.d = H[q].v;
+      ini_v.add(f, vblk(h->n));
+      SegVec z = vop(SEG_BICG_INIT, 1);
+      z.d = H[q].sc;
+      ini_sc.add(z, 1);
+    }
+    act.swap(next);
+    if (!act.empty()) {
+      std::vector<GLaunch> L;
+      L.push_back(std::move(ini_rh));
+      L.push_back(std::move(ini_p));
+      L.push_back(std::move(ini_v));
+      L.push_back(std::move(ini_sc));
+      const size_t b0 = (core_end + 15) & ~(size_t)15;
+      size_t need = pack_launches(L, nullptr, b0);
+      if ((rc = g->ensure(need, K))) return rc;
+      const size_t e = pack_launches(L, g->h_blob, b0);
+      if (hipMemcpyAsync(g->d_blob + b0, g->h_blob + b0, e - b0, hipMemcpyHostToDevice, st) != hipSuccess)
+        return g->fail(KKT_ERR_COPY, "group table upload failed");
+      run_launches<T, S>(L, g->d_blob, st);
+      if (hipStreamSynchronize(st) != hipSuccess) return g->fail(KKT_ERR_REDUCTION, "group step failed");
+    }
+  }
+  // ---- iterations ----
+  for (int it = 1; it <= maxIter && !act.empty(); it++) {
+    // head: p update + y = M^-1 p, v = A y, alpha, s = r - alpha v, |s|
+    std::vector<GLaunch> pre, post;
+    GLaunch gp, scat, sp, r1, bs, r2;
+    sp.fam = GF_SPMV;
+    r1.fam = r2.fam = GF_RED;
+    for (int q : act) {
+      kkt_solver* h = H[q].h;
+      SegVec v = vop(SEG_GATHER_P, h->n);
+      v.idx = h->d_perm; v.sc = H[q].sc; v.a = H[q].r; v.b = H[q].v; v.d = H[q].p; v.tout = H[q].bp;
+      gp.add(v, vblk(h->n));
+      SegVec s = vop(SEG_SCATTER, h->n);
+      s.idx = h->d_perm; s.tin = H[q].xp; s.d = H[q].y;
+      scat.add(s, vblk(h->n));
+      SegSpmv m{};
+      m.n = h->n; m.mode = 0; m.rowptr = h->d_rowptr_f; m.col = h->d_col_f; m.val = h->d_valsf;
+      m.x = H[q].y; m.y = H[q].v; m.a1 = H[q].rh; m.part1 = H[q].pa;
+      sp.add(m, H[q].nbs);
+      SegRed a{};
+      a.op = SEG_RED2; a.p1 = H[q].pa; a.np = H[q].nbs; a.out1 = H[q].sc + 4;
+      r1.add(a, 1);
+      SegVec b = vop(SEG_BICG_S, h->n);
+      b.sc = H[q].sc; b.a = H[q].r; b.b = H[q].v; b.d = H[q].s; b.part1 = H[q].pa;
+      bs.add(b, vblk(h->n));
+      SegRed c{};
+      c.op = SEG_RED2; c.p1 = H[q].pa; c.np = H[q].nbv; c.out1 = H[q].sc + 5;
+      r2.add(c, 1);
+    }
+    pre.push_back(std::move(gp));
+    post.push_back(std::move(scat));
+    post.push_back(std::move(sp));
+    post.push_back(std::move(r1));
+    post.push_back(std::move(bs));
+    post.push_back(std::move(r2));
+    int rc = run_phase(act, pre, true, post);
+    if (rc) return rc;
+    std::vector<int> tail;
+    GLaunch xa;
+    for (size_t i = 0; i < act.size(); i++) {
+      const int q = act[i];
+      const double* s = g->h_sc + 16 * i;
+      H[q].it = it;
+      if (std::fabs(s[0]) < eps) { finish(q, KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: rho ~ 0"); continue; }
+      if (std::fabs(s[4]) < eps) { finish(q, KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: (r_hat,v) ~ 0"); continue; }
+      if (std::sqrt(s[5]) / H[q].bn <= tolRel) {
+        SegVec v = vop(SEG_BICG_XA, H[q].h->n);
+        v.sc = H[q].sc; v.a = H[q].y; v.d = H[q].x;
+        xa.add(v, vblk(H[q].h->n));
+        finish(q, it, "");
+        continue;
+      }
+      tail.push_back(q);
+    }
+    if (xa.nblk()) {
+      std::vector<GLaunch> L;
+      L.push_back(std::move(xa));
+      const size_t b0 = (core_end + 15) & ~(size_t)15;
+      if ((rc = g->ensure(pack_launches(L, nullptr, b0), K))) return rc;
+      const size_t e = pack_launches(L, g->h_blob, b0);
+      if (hipMemcpyAsync(g->d_blob + b0, g->h_blob + b0, e - b0, hipMemcpyHostToDevice, st) != hipSuccess)
+        return g->fail(KKT_ERR_COPY, "group table upload failed");
+      run_launches<T, S>(L, g->d_blob, st);
+      // the staging buffer is rewritten by the next phase only after this
+      // upload has been read
+      if (hipStreamSynchronize(st) != hipSuccess) return g->fail(KKT_ERR_REDUCTION, "group step failed");
+    }
+    act.swap(tail);
+    if (act.empty()) break;
+    // tail: z = M^-1 s, t = A z, omega, x and r updates, |r|, next rho
+    pre.clear();
+    post.clear();
+    GLaunch ga, sc2, sp2, r3, xr, r4;
+    sp2.fam = GF_SPMV;
+    r3.fam = r4.fam = GF_RED;
+    for (int q : act) {
+      kkt_solver* h = H[q].h;
+      SegVec v = vop(SEG_GATHER, h->n);
+      v.idx = h->d_perm; v.a = H[q].s; v.tout = H[q].bp;
+      ga.add(v, vblk(h->n));
+      SegVec s = vop(SEG_SCATTER, h->n);
+      s.idx = h->d_perm; s.tin = H[q].xp; s.d = H[q].z;
+      sc2.add(s, vblk(h->n));
+      SegSpmv m{};
+      m.n = h->n; m.mode = 0; m.rowptr = h->d_rowptr_f; m.col = h->d_col_f; m.val = h->d_valsf;
+      m.x = H[q].z; m.y = H[q].t; m.a1 = H[q].s; m.part1 = H[q].pa; m.part2 = H[q].pb;
+      sp2.add(m, H[q].nbs);
+      SegRed a{};
+      a.op = SEG_RED2; a.p1 = H[q].pa; a.p2 = H[q].pb; a.np = H[q].nbs; a.out1 = H[q].sc + 6; a.out2 = H[q].sc + 7;
+      r3.add(a, 1);
+      SegVec b = vop(SEG_BICG_XR, h->n);
+      b.sc = H[q].sc; b.a = H[q].y; b.b = H[q].z; b.c = H[q].s; b.e = H[q].t; b.d = H[q].x; b.f = H[q].r;
+      b.g = H[q].rh; b.part1 = H[q].pa; b.part2 = H[q].pb;
+      xr.add(b, vblk(h->n));
+      SegRed c{};
+      c.op = SEG_RED2; c.p1 = H[q].pa; c.p2 = H[q].pb; c.np = H[q].nbv; c.out1 = H[q].sc + 8; c.out2 = H[q].sc + 10;
+      c.sc = H[q].sc;
+      r4.add(c, 1);
+    }
+    pre.push_back(std::move(ga));
+    post.push_back(std::move(sc2));
+    post.push_back(std::move(sp2));
+    post.push_back(std::move(r3));
+    post.push_back(std::move(xr));
+    post.push_back(std::move(r4));
+    if ((rc = run_phase(act, pre, true, post))) return rc;
+    std::vector<int> next;
+    for (size_t i = 0; i < act.size(); i++) {
+      const int q = act[i];
+      const double* s = g->h_sc + 16 * i + 6;
+      if (s[1] < eps) { finish(q, KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: (t,t) ~ 0"); continue; }
+      if (std::sqrt(s[2]) / H[q].bn <= tolRel) { finish(q, it, ""); continue; }
+      if (std::fabs(s[0] / s[1]) < eps) { finish(q, KKT_ERR_ITER_BREAKDOWN, "BiCGStab breakdown: omega ~ 0"); continue; }
+      if (it == maxIter) {
+        finish(q, KKT_ERR_ITER_NO_CONVERGE, "BiCGStab did not converge within iteration limit");
+        continue;
+      }
+      next.push_back(q);
+    }
+    act.swap(next);
+  }
+  for (int q : act) finish(q, KKT_ERR_ITER_NO_CONVERGE, "BiCGStab did not converge within iteration limit");
+  // converged handles: x to the caller's vector
+  GLaunch out;
+  for (int q = 0; q < K; q++) {
+    if (status[q] <= 0) continue;
+    SegVec v = vop(SEG_COPY, H[q].h->n);
+    v.a = H[q].x; v.d = xout[q];
+    out.add(v, vblk(H[q].h->n));
+  }
+  if (out.nblk()) {
+    std::vector<GLaunch> L;
+    L.push_back(std::move(out));
+    int rc;
+    if ((rc = g->ensure(pack_launches(L, nullptr, 0), K))) return rc;
+    const size_t e = pack_launches(L, g->h_blob, 0);
+    if (hipMemcpyAsync(g->d_blob, g->h_blob, e, hipMemcpyHostToDevice, st) != hipSuccess)
+      return g->fail(KKT_ERR_COPY, "group table upload failed");
+    run_launches<T, S>(L, g->d_blob, st);
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) return g->fail(KKT_ERR_SOLVE, "group solve failed");
+  return KKT_OK;
+}
+
+// =====================================================================
 // Batched systems: many independent KKT systems of uneven size in one handle.
 // The systems are concatenated into one block-diagonal upper CSR whose
 // assembly forest holds one tree per system (one per connected component),
@@ -2812,6 +3355,63 @@ int kkt_iterative_solve_device(kkt_solver* h, const double* d_vals, const double
   if (rc) return rc;
   h->ok();
   return it;
+}
+
+kkt_group* kkt_group_create(void) {
+  kkt_group* g = new (std::nothrow) kkt_group();
+  if (!g) return nullptr;
+  if (hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking) != hipSuccess) {
+    g->st = nullptr;
+    delete g;
+    return nullptr;
+  }
+  return g;
+}
+
+void kkt_group_destroy(kkt_group* g) { delete g; }
+
+const char* kkt_group_last_error(kkt_group* g) {
+  if (!g || !g->has_err) return nullptr;
+  return g->err.c_str();
+}
+
+int kkt_group_iterative_solve_device(kkt_group* g, int nh, kkt_solver* const* hs, const double* const* d_vals,
+                                     const double* const* d_b, double* const* d_x, double tolRel, int maxIter,
+                                     int* status) {
+  if (!g) return KKT_ERR_ARGUMENT;
+  g->has_err = false;
+  if (nh < 1 || nh > KKT_GROUP_MAX) return g->fail(KKT_ERR_ARGUMENT, "group size must be 1..KKT_GROUP_MAX");
+  if (!hs || !d_vals || !d_b || !d_x || !status) return g->fail(KKT_ERR_ARGUMENT, "null argument");
+  if (maxIter <= 0) return g->fail(KKT_ERR_ARGUMENT, "maxIter must be positive");
+  int q0 = -1;
+  for (int i = 0; i < nh; i++) {
+    kkt_solver* h = hs[i];
+    if (!h || !d_vals[i] || !d_b[i] || !d_x[i]) return g->fail(KKT_ERR_ARGUMENT, "null handle or array");
+    for (int j = 0; j < i; j++)
+      if (hs[j] == h) return g->fail(KKT_ERR_ARGUMENT, "a handle appears twice in the group");
+    if (h->state != kkt_solver::FACTORED)
+      return g->fail(KKT_ERR_STATE, "iterativeSolve called before factor (handle " + std::to_string(i) + ")");
+    const int q = h->solve_q(true);
+    if (q0 < 0) q0 = q;
+    if (q != q0)
+      return g->fail(KKT_ERR_ARGUMENT, "group handles must share the factor precision and preconditioner copy");
+  }
+  // the handles' own streams are idle before the group writes their buffers
+  for (int i = 0; i < nh; i++)
+    if (hipStreamSynchronize(hs[i]->st) != hipSuccess) return g->fail(KKT_ERR_DEVICE, "handle stream failed");
+  int rc;
+  if (q0 == 0) rc = group_bicgstab<double, double>(g, nh, hs, d_vals, d_b, d_x, tolRel, maxIter, status);
+  else if (q0 == 1) rc = group_bicgstab<float, float>(g, nh, hs, d_vals, d_b, d_x, tolRel, maxIter, status);
+  else rc = group_bicgstab<float, _Float16>(g, nh, hs, d_vals, d_b, d_x, tolRel, maxIter, status);
+  if (rc) {
+    (void)hipStreamSynchronize(g->st);
+    for (int i = 0; i < nh; i++)
+      if (status[i] == 0) {
+        status[i] = rc;
+        hs[i]->fail(rc, g->err);
+      }
+  }
+  return rc;
 }
 
 int kkt_iterative_solve(kkt_solver* h, const double* vals, const double* b, double* x, double tolRel,

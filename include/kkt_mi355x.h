@@ -228,6 +228,27 @@ int kkt_iterative_solve(kkt_solver* h, const double* vals, const double* b, doub
 /* Same on device arrays. */
 int kkt_iterative_solve_device(kkt_solver* h, const double* d_vals, const double* d_b,
                                double* d_x, double tolRel, int maxIter);
+/* Group iterative solves (no counterpart in the reference, whose bridges each
+ * drive one handle on its own stream, cudssBridge.cc:162-179): the
+ * kkt_iterative_solve_device of nh distinct factored handles in lockstep, each
+ * step of a BiCGStab iteration one launch for all handles still iterating, so
+ * many small systems share every launch.  Handle i solves with d_vals[i], d_b[i]
+ * into d_x[i]; status[i] receives what kkt_iterative_solve_device would return
+ * for it (iterations >= 1, or a negative status; the handle's last error is set
+ * as by that call), and its solution is bitwise the one that call gives.  All
+ * handles must share the factor precision and the kkt_set_iterative_half
+ * setting; none may be in use elsewhere during the call.  The group object owns
+ * a stream and staging buffers; one group object is driven by one thread at a
+ * time.  Returns KKT_OK, or an error code (then every handle not yet finished
+ * has that status). */
+#define KKT_GROUP_MAX 128
+typedef struct kkt_group kkt_group;
+kkt_group* kkt_group_create(void);
+void kkt_group_destroy(kkt_group* g);
+int kkt_group_iterative_solve_device(kkt_group* g, int nh, kkt_solver* const* hs,
+                                     const double* const* d_vals, const double* const* d_b,
+                                     double* const* d_x, double tolRel, int maxIter, int* status);
+const char* kkt_group_last_error(kkt_group* g);
 /* CuDssSolver.doSetIrSteps (cudssBridge.cc setIterativeRefinementSteps):
  * refinement steps after each FP64 solve (default 0; FP32 factors always
  * refine to the refinement tolerance). */
