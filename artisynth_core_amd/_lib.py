@@ -50,7 +50,8 @@ class KktBatchSysInfo(ctypes.Structure):
 
 class KktServeConfig(ctypes.Structure):
     _fields_ = [("budget_bytes", ctypes.c_longlong), ("nworkers", ctypes.c_int), ("precision", ctypes.c_int),
-                ("tol_exp", ctypes.c_int), ("max_iter", ctypes.c_int), ("refine_tol", ctypes.c_double)]
+                ("tol_exp", ctypes.c_int), ("max_iter", ctypes.c_int), ("refine_tol", ctypes.c_double),
+                ("group_runners", ctypes.c_int), ("group_max", ctypes.c_int)]
 
 
 class KktServeResult(ctypes.Structure):
@@ -66,7 +67,8 @@ class KktServeStats(ctypes.Structure):
                 ("queue_depth", ctypes.c_longlong), ("queue_max", ctypes.c_longlong),
                 ("budget_bytes", ctypes.c_longlong), ("bytes_resident", ctypes.c_longlong),
                 ("bytes_peak", ctypes.c_longlong), ("instances_resident", ctypes.c_int),
-                ("service_ms", ctypes.c_double), ("elapsed_ms", ctypes.c_double)]
+                ("service_ms", ctypes.c_double), ("elapsed_ms", ctypes.c_double),
+                ("group_calls", ctypes.c_longlong), ("group_requests", ctypes.c_longlong)]
 
 
 # (name, restype, argtypes) for every symbol of include/kkt_mi355x.h

@@ -21,6 +21,13 @@
 // - A request is one KKT step with KKTSolver.factorAndSolve's policy
 //   (KKTSolver.java:930-1014): BiCGStab with the resident (stale) factor while it
 //   converges within max_iter, else a factor and a refined direct solve.
+// - Group solves (group_runners > 0): device-array requests of resident,
+//   factored, idle instances are taken together, up to group_max at a time, by
+//   a group runner thread and solved in one kkt_group_iterative_solve_device
+//   call (every BiCGStab step one launch for all of them); a request whose
+//   group solve does not converge goes to a worker for the factor and direct
+//   solve, its instance still held.  Each instance's result is bitwise the one
+//   its own iterative solve gives, so the mix still does not change it.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -59,6 +66,7 @@ struct Instance {
 
 struct Request {
   long long id = 0;
+  bool group_failed = false;    // its group solve did not converge: direct next
   int iid = 0;
   bool device = false;
   std::vector<double> vals, b;  // host submissions: copies
@@ -78,6 +86,9 @@ struct kkt_server {
   std::condition_variable done_cv;  // a request completed
   std::map<int, std::unique_ptr<Instance>> inst;
   std::deque<Request*> queue;
+  std::deque<Request*> fallback;    // group solves that did not converge (instance held)
+  long long inflight = 0;           // taken, not yet done
+  int group_live = 0;               // group runners holding a group object
   std::map<long long, std::unique_ptr<Request>> reqs;
   std::vector<std::thread> workers;
   long long next_id = 1;
@@ -96,21 +107,59 @@ struct kkt_server {
     return code;
   }
 
-  // oldest queued request whose instance is idle (keeps per-instance order)
+  // a request the group runners take: device arrays, instance resident and factored
+  bool group_eligible(const Request* r, const Instance* in) const {
+    return group_live > 0 && r->device && in->h && in->factored && !in->too_large;
+  }
+
+  // for a worker: a failed group solve's direct step first, else the oldest
+  // queued request whose instance is idle and that no group runner takes
+  // (keeps per-instance order: an instance's later requests wait behind its
+  // first queued one)
   Request* next_locked() {
+    if (!fallback.empty()) {
+      Request* r = fallback.front();
+      fallback.pop_front();
+      return r;
+    }
     std::vector<int> blocked;
     for (auto it = queue.begin(); it != queue.end(); ++it) {
       Request* r = *it;
       if (std::find(blocked.begin(), blocked.end(), r->iid) != blocked.end()) continue;
       Instance* in = inst[r->iid].get();
-      if (in->busy) {
+      if (in->busy || group_eligible(r, in)) {
         blocked.push_back(r->iid);
         continue;
       }
       queue.erase(it);
+      inflight++;
       return r;
     }
     return nullptr;
+  }
+
+  // for a group runner: the first queued request of every idle instance, in
+  // queue order, while group-eligible, at most group_max
+  void next_group_locked(std::vector<Request*>& out) {
+    out.clear();
+    std::vector<int> blocked;
+    for (auto it = queue.begin(); it != queue.end() && (int)out.size() < cfg.group_max;) {
+      Request* r = *it;
+      if (std::find(blocked.begin(), blocked.end(), r->iid) != blocked.end()) {
+        ++it;
+        continue;
+      }
+      blocked.push_back(r->iid);
+      Instance* in = inst[r->iid].get();
+      if (in->busy || !group_eligible(r, in)) {
+        ++it;
+        continue;
+      }
+      in->busy = true;
+      out.push_back(r);
+      it = queue.erase(it);
+      inflight++;
+    }
   }
 
   bool evict_one_locked(const Instance* self) {
@@ -185,7 +234,7 @@ struct kkt_server {
     int rc = -1;
     const double* vals = r->device ? r->d_vals : r->vals.data();
     const double* b = r->device ? r->d_b : r->b.data();
-    if (in->factored) {
+    if (in->factored && !r->group_failed) {
       rc = r->device ? kkt_iterative_solve_device(h, vals, b, r->x, tol, cfg.max_iter)
                      : kkt_iterative_solve(h, vals, b, r->x, tol, cfg.max_iter);
     }
@@ -213,11 +262,11 @@ struct kkt_server {
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
       Request* r = nullptr;
-      cv.wait(lk, [&] { return stop || (r = next_locked()) != nullptr; });
+      cv.wait(lk, [&] { return (r = next_locked()) != nullptr || stop; });
       if (!r) return;  // stopping with nothing runnable
       Instance* in = inst[r->iid].get();
       in->busy = true;
-      r->res.start_ms = now_ms() - t0;
+      if (!r->group_failed) r->res.start_ms = now_ms() - t0;
       int rc = KKT_OK;
       std::string why;
       if (in->too_large) {
@@ -252,18 +301,95 @@ struct kkt_server {
         r->res.status = rc;
         snprintf(r->res.message, sizeof r->res.message, "%s", why.c_str());
       }
-      r->res.done_ms = now_ms() - t0;
-      in->busy = false;
-      in->last_used = now_ms();
-      in->requests++;
-      st.completed++;
-      if (r->res.status) st.errors++;
-      else if (r->res.kind == 1) st.iterative++;
-      else st.direct++;
-      r->done = true;
-      done_cv.notify_all();
+      finish_locked(r, in);
+    }
+  }
+
+  void finish_locked(Request* r, Instance* in) {
+    r->res.done_ms = now_ms() - t0;
+    in->busy = false;
+    in->last_used = now_ms();
+    in->requests++;
+    st.completed++;
+    if (r->res.status) st.errors++;
+    else if (r->res.kind == 1) st.iterative++;
+    else st.direct++;
+    r->done = true;
+    inflight--;
+    done_cv.notify_all();
+    cv.notify_all();
+  }
+
+  // group runner: solves what next_group_locked takes in one group call.  Its
+  // group object (own stream) is made here; without one (no device) the runner
+  // ends and the workers take every request.
+  void group_runner() {
+    kkt_group* g = kkt_group_create();
+    if (!g) return;
+    const double tol = std::pow(10.0, -cfg.tol_exp);
+    std::vector<Request*> batch;
+    std::vector<kkt_solver*> hs;
+    std::vector<const double*> vs, bs;
+    std::vector<double*> xs;
+    std::vector<int> status;
+    std::vector<double> relres;
+    std::unique_lock<std::mutex> lk(mu);
+    group_live++;
+    for (;;) {
+      batch.clear();
+      cv.wait(lk, [&] {
+        if (stop) return true;
+        next_group_locked(batch);
+        return !batch.empty();
+      });
+      if (batch.empty()) break;  // stopping
+      const int K = (int)batch.size();
+      hs.resize(K);
+      vs.resize(K);
+      bs.resize(K);
+      xs.resize(K);
+      status.assign(K, 0);
+      relres.assign(K, 0.0);
+      const double ts = now_ms();
+      for (int k = 0; k < K; k++) {
+        Request* r = batch[k];
+        r->res.start_ms = ts - t0;
+        hs[k] = inst[r->iid]->h;
+        vs[k] = r->d_vals;
+        bs[k] = r->d_b;
+        xs[k] = r->x;
+      }
+      st.group_calls++;
+      st.group_requests += K;
+      lk.unlock();
+      (void)kkt_group_iterative_solve_device(g, K, hs.data(), vs.data(), bs.data(), xs.data(), tol, cfg.max_iter,
+                                             status.data());
+      for (int k = 0; k < K; k++) {
+        kkt_info info{};
+        if (kkt_get_info(hs[k], &info) == KKT_OK) relres[k] = info.last_rel_residual;
+      }
+      const double te = now_ms();
+      lk.lock();
+      service_ms += K * (te - ts);
+      for (int k = 0; k < K; k++) {
+        Request* r = batch[k];
+        Instance* in = inst[r->iid].get();
+        if (status[k] > 0) {
+          r->res.kind = 1;
+          r->res.iterations = status[k];
+          r->res.status = KKT_OK;
+          r->res.rel_residual = relres[k];
+          finish_locked(r, in);
+        } else {
+          r->group_failed = true;  // refactor + direct solve on a worker, instance still held
+          fallback.push_back(r);
+        }
+      }
       cv.notify_all();
     }
+    group_live--;
+    lk.unlock();
+    kkt_group_destroy(g);
   }
 };
 
@@ -277,12 +403,15 @@ void kkt_serve_default_config(kkt_serve_config* c) {
   c->tol_exp = 10;
   c->max_iter = 30;
   c->refine_tol = 1e-10;
+  c->group_runners = 2;
+  c->group_max = 64;
 }
 
 kkt_server* kkt_serve_create(const kkt_serve_config* cfg) {
   if (!cfg || cfg->budget_bytes <= 0 || cfg->nworkers < 1 || cfg->nworkers > 64 || cfg->max_iter < 1 ||
       cfg->tol_exp < 1 || !(cfg->refine_tol > 0) ||
-      (cfg->precision != KKT_PREC_FP64 && cfg->precision != KKT_PREC_FP32_REFINE))
+      (cfg->precision != KKT_PREC_FP64 && cfg->precision != KKT_PREC_FP32_REFINE) || cfg->group_runners < 0 ||
+      cfg->group_runners > 8 || (cfg->group_runners > 0 && (cfg->group_max < 1 || cfg->group_max > KKT_GROUP_MAX)))
     return nullptr;
   kkt_server* s = nullptr;
   try {
@@ -290,6 +419,7 @@ kkt_server* kkt_serve_create(const kkt_serve_config* cfg) {
     s->cfg = *cfg;
     s->st.budget_bytes = cfg->budget_bytes;
     for (int w = 0; w < cfg->nworkers; w++) s->workers.emplace_back([s] { s->worker(); });
+    for (int q = 0; q < cfg->group_runners; q++) s->workers.emplace_back([s] { s->group_runner(); });
   } catch (...) {
     delete s;
     return nullptr;
@@ -302,7 +432,7 @@ void kkt_serve_destroy(kkt_server* s) {
   {
     std::unique_lock<std::mutex> lk(s->mu);
     // finish what was submitted, then stop the workers
-    s->done_cv.wait(lk, [&] { return s->queue.empty(); });
+    s->done_cv.wait(lk, [&] { return s->queue.empty() && s->fallback.empty() && s->inflight == 0; });
     s->stop = true;
     s->cv.notify_all();
   }

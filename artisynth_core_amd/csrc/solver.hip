@@ -3383,7 +3383,6 @@ int kkt_group_iterative_solve_device(kkt_group* g, int nh, kkt_solver* const* hs
   if (nh < 1 || nh > KKT_GROUP_MAX) return g->fail(KKT_ERR_ARGUMENT, "group size must be 1..KKT_GROUP_MAX");
   if (!hs || !d_vals || !d_b || !d_x || !status) return g->fail(KKT_ERR_ARGUMENT, "null argument");
   if (maxIter <= 0) return g->fail(KKT_ERR_ARGUMENT, "maxIter must be positive");
-  int q0 = -1;
   for (int i = 0; i < nh; i++) {
     kkt_solver* h = hs[i];
     if (!h || !d_vals[i] || !d_b[i] || !d_x[i]) return g->fail(KKT_ERR_ARGUMENT, "null handle or array");
@@ -3391,27 +3390,49 @@ int kkt_group_iterative_solve_device(kkt_group* g, int nh, kkt_solver* const* hs
       if (hs[j] == h) return g->fail(KKT_ERR_ARGUMENT, "a handle appears twice in the group");
     if (h->state != kkt_solver::FACTORED)
       return g->fail(KKT_ERR_STATE, "iterativeSolve called before factor (handle " + std::to_string(i) + ")");
-    const int q = h->solve_q(true);
-    if (q0 < 0) q0 = q;
-    if (q != q0)
-      return g->fail(KKT_ERR_ARGUMENT, "group handles must share the factor precision and preconditioner copy");
   }
   // the handles' own streams are idle before the group writes their buffers
   for (int i = 0; i < nh; i++)
     if (hipStreamSynchronize(hs[i]->st) != hipSuccess) return g->fail(KKT_ERR_DEVICE, "handle stream failed");
-  int rc;
-  if (q0 == 0) rc = group_bicgstab<double, double>(g, nh, hs, d_vals, d_b, d_x, tolRel, maxIter, status);
-  else if (q0 == 1) rc = group_bicgstab<float, float>(g, nh, hs, d_vals, d_b, d_x, tolRel, maxIter, status);
-  else rc = group_bicgstab<float, _Float16>(g, nh, hs, d_vals, d_b, d_x, tolRel, maxIter, status);
-  if (rc) {
-    (void)hipStreamSynchronize(g->st);
+  for (int i = 0; i < nh; i++) status[i] = 0;
+  // one lockstep group per solve precision (FP64 factors, FP32 factors, FP32
+  // factors whose preconditioner reads the half-precision copy), in that order
+  int rc_all = KKT_OK;
+  for (int q = 0; q < 3; q++) {
+    std::vector<int> idx;
     for (int i = 0; i < nh; i++)
-      if (status[i] == 0) {
-        status[i] = rc;
-        hs[i]->fail(rc, g->err);
-      }
+      if (hs[i]->solve_q(true) == q) idx.push_back(i);
+    if (idx.empty()) continue;
+    const int K = (int)idx.size();
+    std::vector<kkt_solver*> sh(K);
+    std::vector<const double*> sv(K), sb(K);
+    std::vector<double*> sx(K);
+    std::vector<int> sst(K, 0);
+    for (int k = 0; k < K; k++) {
+      sh[k] = hs[idx[k]];
+      sv[k] = d_vals[idx[k]];
+      sb[k] = d_b[idx[k]];
+      sx[k] = d_x[idx[k]];
+    }
+    int rc;
+    if (q == 0) rc = group_bicgstab<double, double>(g, K, sh.data(), sv.data(), sb.data(), sx.data(), tolRel, maxIter,
+                                                     sst.data());
+    else if (q == 1) rc = group_bicgstab<float, float>(g, K, sh.data(), sv.data(), sb.data(), sx.data(), tolRel,
+                                                       maxIter, sst.data());
+    else rc = group_bicgstab<float, _Float16>(g, K, sh.data(), sv.data(), sb.data(), sx.data(), tolRel, maxIter,
+                                              sst.data());
+    if (rc) {
+      (void)hipStreamSynchronize(g->st);
+      for (int k = 0; k < K; k++)
+        if (sst[k] == 0) {
+          sst[k] = rc;
+          sh[k]->fail(rc, g->err);
+        }
+      if (!rc_all) rc_all = rc;
+    }
+    for (int k = 0; k < K; k++) status[idx[k]] = sst[k];
   }
-  return rc;
+  return rc_all;
 }
 
 int kkt_iterative_solve(kkt_solver* h, const double* vals, const double* b, double* x, double tolRel,

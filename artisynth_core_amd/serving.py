@@ -15,6 +15,11 @@ its worker threads, admission and eviction are C++ in csrc/serve.cpp).
 - A request is one KKT step with KKTSolver.factorAndSolve's policy
   (KKTSolver.java:930-1014): BiCGStab with the resident factor while it
   converges within `max_iter`, else a refactor and a refined direct solve.
+- With `group_runners` > 0, device-array requests of resident, factored, idle
+  instances are solved together (up to `group_max` per call) by group runner
+  threads, each BiCGStab step one launch for all of them; a request that does
+  not converge there is refactored and solved directly by a worker.  Results
+  are bitwise those of the one-instance-at-a-time path.
 This module only marshals arrays and results.
 """
 import ctypes
@@ -54,7 +59,7 @@ class Request:
 
 class SolveServer:
     def __init__(self, budget_bytes=200 << 30, nworkers=12, precision=KKT_PREC_FP32_REFINE,
-                 tol_exp=10, max_iter=30, refine_tol=1e-10):
+                 tol_exp=10, max_iter=30, refine_tol=1e-10, group_runners=None, group_max=None):
         self.L = lib()
         cfg = KktServeConfig()
         self.L.kkt_serve_default_config(ctypes.byref(cfg))
@@ -64,6 +69,10 @@ class SolveServer:
         cfg.tol_exp = int(tol_exp)
         cfg.max_iter = int(max_iter)
         cfg.refine_tol = float(refine_tol)
+        if group_runners is not None:
+            cfg.group_runners = int(group_runners)
+        if group_max is not None:
+            cfg.group_max = int(group_max)
         self.h = self.L.kkt_serve_create(ctypes.byref(cfg))
         if not self.h:
             raise RuntimeError("kkt_serve_create failed (bad configuration)")

@@ -247,7 +247,7 @@ def main():
                          "default run adds to its line (batch / serve objects)")
     ap.add_argument("--grid", default="100x16x16")
     ap.add_argument("--mode", default="bench",
-                    choices=["bench", "pmc", "batch", "serve", "precision", "multi", "stream", "levels"],
+                    choices=["bench", "pmc", "batch", "serve", "precision", "multi", "stream", "levels", "group"],
                     help="pmc: one FP32 factor + solve and a 256 MiB copy, for counter runs; "
                          "batch: the 256-system mix, one batched launch vs per-system launches; "
                          "precision: the ~210k-DOF Mooney-Rivlin tet box, direct FP32-MFMA+refine "
@@ -256,7 +256,8 @@ def main():
                          "rates of one launch over 4 MB - 1 GB with cold caches (the achievable HBM ceiling); "
                          "levels: per precision one factor of the beam's first KKT, the solve-shape autotuner and "
                          "four stale-factor iterative solves on the second (graphs on), for a kernel trace of "
-                         "the sweeps level by level")
+                         "the sweeps level by level; group: one stale-factor iterative step of every "
+                         "serving instance, as one group solve vs one handle at a time vs --workers threads")
     ap.add_argument("--nrhs", type=int, default=64, help="multi: right-hand sides per block solve")
     ap.add_argument("--tet", default="121x24x24", help="precision: Delaunay lattice points")
     ap.add_argument("--iterative-half", type=int, default=0, choices=[0, 1],
@@ -271,6 +272,10 @@ def main():
     ap.add_argument("--nreq", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=2000.0, help="serving: offered requests/s")
     ap.add_argument("--workers", type=int, default=12)
+    ap.add_argument("--group-runners", type=int, default=2,
+                    help="serve: group runner threads (iterative steps of idle factored instances solved "
+                         "together); 0: every request on a worker")
+    ap.add_argument("--group-max", type=int, default=64, help="serve: requests per group solve")
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--eps", type=float, default=5e-3,
                     help="serving: per-step relative drift of every matrix entry (random walk)")
@@ -294,7 +299,8 @@ def main():
     if args.mode == "batch":
         return batch_run(args.nbatch, args.steps, args.warmup)
     if args.mode == "serve":
-        return serve_run(args.ninst, args.nreq, args.rate, args.workers, args.budget_gb, args.eps)
+        return serve_run(args.ninst, args.nreq, args.rate, args.workers, args.budget_gb, args.eps,
+                         group_runners=args.group_runners, group_max=args.group_max)
     if args.mode == "precision":
         return precision_run(tuple(int(v) for v in args.tet.split("x")), min(args.nsys, 4),
                              args.warmup, args.oracle, args.max_super)
@@ -302,6 +308,8 @@ def main():
         return multi_run(nx, ny, nz, args.nrhs, max(args.steps // 4, 2), args.warmup)
     if args.mode == "stream":
         return stream_run()
+    if args.mode == "group":
+        return group_run(args.ninst, args.eps, args.steps, args.workers)
     if args.mode == "levels":
         return levels_run(nx, ny, nz, args.iterative_half)
     L = lib()
@@ -532,11 +540,13 @@ def main():
                 "throughput_req_s", "p50_ms", "p99_ms", "errors", "direct", "iterative",
                 "max_rel_residual_sample", "resident_gb", "evictions", "admissions_timed", "queue_max",
                 "concurrency", "total_dof", "inputs")
-        sl = serve_run(args.ninst, args.nreq, args.rate, args.workers, args.budget_gb, args.eps, emit=False)
+        sl = serve_run(args.ninst, args.nreq, args.rate, args.workers, args.budget_gb, args.eps, emit=False,
+                       group_runners=args.group_runners, group_max=args.group_max)
         line["serve"] = {k: sl[k] for k in keys}
         # the same stream with a budget below the instances' resident total, so
         # admission evicts idle instances and re-admits them (refactor) under load
-        sl = serve_run(args.ninst, args.nreq, args.rate, args.workers, 12.0, args.eps, emit=False)
+        sl = serve_run(args.ninst, args.nreq, args.rate, args.workers, 12.0, args.eps, emit=False,
+                       group_runners=args.group_runners, group_max=args.group_max)
         line["serve_budget_12gb"] = {k: sl[k] for k in keys}
     print(json.dumps(line), flush=True)
 
@@ -855,7 +865,100 @@ def batch_run(nsys, reps, warmup, emit=True):
     return line
 
 
-def serve_run(ninst, nreq, rate, nworkers, budget_gb, eps=5e-3, emit=True):
+def group_run(ninst, eps, reps, nworkers, emit=True):
+    """One stale-factor iterative step of each of ninst serving instances (FP32
+    factor + FP64 BiCGStab to 1e-10, each instance's values drifted by up to eps
+    relatively), three ways: all of them in one group solve
+    (kkt_group_iterative_solve_device), one handle at a time, and nworkers host
+    threads each running its share one handle at a time (the serving loop's
+    concurrency).  Same results on every path (bitwise; checked)."""
+    import threading
+    import torch
+    from artisynth_core_amd import KKT_PREC_FP32_REFINE
+    from artisynth_core_amd import generators as G
+    from artisynth_core_amd._lib import lib
+    from artisynth_core_amd._lib import ptr as nptr
+    from artisynth_core_amd.batch import IterativeGroup
+    t0 = time.time()
+    mix = G.serving_mix(ninst)
+    log("generated %d instances (n %d..%d) in %.1fs" % (
+        ninst, min(len(m[2]) - 1 for m in mix), max(len(m[2]) - 1 for m in mix), time.time() - t0))
+    L = lib()
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(77)
+    hs, dv, db, dx = [], [], [], []
+    t0 = time.time()
+    for kind, nd, ro, ci, va, b in mix:
+        h = L.kkt_create()
+        ro32 = np.ascontiguousarray(ro, np.int32)
+        ci32 = np.ascontiguousarray(ci, np.int32)
+        va64 = np.ascontiguousarray(va, np.float64)
+        assert L.kkt_set_precision(h, KKT_PREC_FP32_REFINE) == 0
+        assert L.kkt_set_pattern(h, len(ro) - 1, int(ro[-1]), nptr(ro32, ctypes.c_int),
+                                 nptr(ci32, ctypes.c_int), 1) == 0
+        assert L.kkt_analyze_values(h, nptr(va64, ctypes.c_double)) == 0, L.kkt_last_error(h)
+        assert L.kkt_factor(h, nptr(va64, ctypes.c_double)) == 0, L.kkt_last_error(h)
+        hs.append(h)
+        dv.append(torch.from_numpy(va64 * (1.0 + eps * rng.uniform(-1, 1, len(va64)))).to(dev))
+        db.append(torch.from_numpy(np.ascontiguousarray(b, np.float64)).to(dev))
+        dx.append(torch.zeros(len(b), dtype=torch.float64, device=dev))
+    torch.cuda.synchronize()
+    log("analysed + factored %d handles in %.1fs" % (ninst, time.time() - t0))
+    tol, mi = 1e-10, 30
+
+    def single(qs, out):
+        for q in qs:
+            out[q] = L.kkt_iterative_solve_device(hs[q], dv[q].data_ptr(), db[q].data_ptr(), dx[q].data_ptr(),
+                                                  tol, mi)
+
+    its = [0] * ninst
+    single(range(ninst), its)  # warm (graphs captured)
+    x1 = [t.cpu().numpy() for t in dx]
+    grp = IterativeGroup()
+    ptrs = ([t.data_ptr() for t in dv], [t.data_ptr() for t in db], [t.data_ptr() for t in dx])
+    rc, st = grp.solve(hs, *ptrs, tol, mi)
+    assert rc == 0, grp.last_error()
+    same = list(st) == its and all(np.array_equal(x1[q], dx[q].cpu().numpy()) for q in range(ninst))
+
+    def timed(fn):
+        out = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            out.append((time.perf_counter() - t0) * 1e3)
+        return float(np.median(out))
+
+    t_grp = timed(lambda: grp.solve(hs, *ptrs, tol, mi))
+    t_one = timed(lambda: single(range(ninst), [0] * ninst))
+
+    def threaded():
+        out = [0] * ninst
+        th = [threading.Thread(target=single, args=(range(w, ninst, nworkers), out)) for w in range(nworkers)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+
+    t_thr = timed(threaded)
+    grp.dispose()
+    for h in hs:
+        L.kkt_destroy(h)
+    line = {"mode": "group", "ninst": ninst, "eps": eps, "total_dof": int(sum(len(m[2]) - 1 for m in mix)),
+            "iterations_min": int(min(its)), "iterations_max": int(max(its)),
+            "iterations_mean": float(np.mean(its)), "unconverged": int(sum(1 for i in its if i <= 0)),
+            "group_ms": t_grp, "one_by_one_ms": t_one, "threads_ms": t_thr, "threads": nworkers,
+            "group_solves_per_s": ninst / (t_grp * 1e-3), "one_by_one_solves_per_s": ninst / (t_one * 1e-3),
+            "threads_solves_per_s": ninst / (t_thr * 1e-3), "bitwise_same": bool(same)}
+    log("group: %d stale-factor solves (iterations %d..%d) in %.2f ms as one group, %.2f ms one by one, "
+        "%.2f ms on %d threads; bitwise same %s" % (ninst, min(its), max(its), t_grp, t_one, t_thr, nworkers, same))
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
+
+
+def serve_run(ninst, nreq, rate, nworkers, budget_gb, eps=5e-3, emit=True, group_runners=2, group_max=64):
     """The serving workload: ninst instances (1k-90k DOF, beam/tet/rigid), a Poisson
     stream of step requests at `rate` req/s, HBM-budget admission, FP32-MFMA + FP64
     refinement (1e-10 relative residual), through the C++ serving loop
@@ -865,7 +968,9 @@ def serve_run(ninst, nreq, rate, nworkers, budget_gb, eps=5e-3, emit=True):
     the clock starts and submitted as device arrays (kkt_serve_submit_device); the
     submission loop only sleeps and submits.  Latency = submission to completion
     (server clock); p50/p99, achieved offered rate, queue depth, admissions and
-    evictions are reported."""
+    evictions are reported.  With group_runners > 0 the iterative steps of idle,
+    factored instances are solved together in group solves (reported as
+    group_calls / group_requests)."""
     import torch
     import scipy.sparse as sp
     from artisynth_core_amd import generators as G
@@ -896,7 +1001,8 @@ def serve_run(ninst, nreq, rate, nworkers, budget_gb, eps=5e-3, emit=True):
         steps[i] += 1
     torch.cuda.synchronize()
     log("request inputs in HBM: %.1f GB in %.1fs" % (sum(v.numel() for v in vals) * 8 / 1e9, time.time() - t0))
-    srv = SolveServer(budget_bytes=int(budget_gb * (1 << 30)), nworkers=nworkers)
+    srv = SolveServer(budget_bytes=int(budget_gb * (1 << 30)), nworkers=nworkers, group_runners=group_runners,
+                      group_max=group_max)
     for i, (kind, nd, ro, ci, va, b) in enumerate(mix):
         srv.register(i, ro, ci, va)
     t0 = time.time()
@@ -948,10 +1054,14 @@ def serve_run(ninst, nreq, rate, nworkers, budget_gb, eps=5e-3, emit=True):
             "admissions": st["admissions"], "evictions": st["evictions"], "queue_max": st["queue_max"],
             "concurrency": (st["service_ms"] - st_start["service_ms"]) / (1e3 * (t_end - t_start)),
             "admissions_timed": st["admissions"] - st_start["admissions"],
+            "group_runners": group_runners, "group_max": group_max,
+            "group_calls": st["group_calls"] - st_start["group_calls"],
+            "group_requests": st["group_requests"] - st_start["group_requests"],
             "total_dof": int(sum(len(m[2]) - 1 for m in mix)), "inputs": "device (HBM), drift eps %g" % eps}
     log("serve: offered %.0f req/s (target %.0f), served %.0f req/s, p50 %.2f ms, p99 %.2f ms, direct %d, "
-        "evictions %d, max resid %.2e" % (offered, rate, line["throughput_req_s"], lat["p50_ms"], lat["p99_ms"],
-                                          line["direct"], line["evictions"], max(res)))
+        "evictions %d, group solves %d of %.1f requests on average, max resid %.2e" % (
+            offered, rate, line["throughput_req_s"], lat["p50_ms"], lat["p99_ms"], line["direct"], line["evictions"],
+            line["group_calls"], line["group_requests"] / max(1, line["group_calls"]), max(res)))
     if emit:
         print(json.dumps(line), flush=True)
     srv.shutdown()

@@ -235,9 +235,10 @@ int kkt_iterative_solve_device(kkt_solver* h, const double* d_vals, const double
  * many small systems share every launch.  Handle i solves with d_vals[i], d_b[i]
  * into d_x[i]; status[i] receives what kkt_iterative_solve_device would return
  * for it (iterations >= 1, or a negative status; the handle's last error is set
- * as by that call), and its solution is bitwise the one that call gives.  All
- * handles must share the factor precision and the kkt_set_iterative_half
- * setting; none may be in use elsewhere during the call.  The group object owns
+ * as by that call), and its solution is bitwise the one that call gives.
+ * Handles whose current factors differ in precision (FP64, FP32, FP32 with the
+ * half-precision preconditioner copy) iterate as separate lockstep groups, one
+ * after the other.  No handle may be in use elsewhere during the call.  The group object owns
  * a stream and staging buffers; one group object is driven by one thread at a
  * time.  Returns KKT_OK, or an error code (then every handle not yet finished
  * has that status). */
@@ -421,6 +422,8 @@ typedef struct kkt_serve_config {
   int tol_exp;            /* iterative solves to 10^-tol_exp relative residual (default 10) */
   int max_iter;           /* BiCGStab iterations before a refactor (default 30) */
   double refine_tol;      /* refinement target of FP32 factors (default 1e-10) */
+  int group_runners;      /* 0..8 threads solving iterative steps in groups (default 2); 0: on the workers */
+  int group_max;          /* requests per group solve, 1..KKT_GROUP_MAX (default 64) */
 } kkt_serve_config;
 typedef struct kkt_serve_result {
   int status;               /* KKT_OK or a negative KKT_ERR_* code */
@@ -438,6 +441,7 @@ typedef struct kkt_serve_stats {
   int instances_resident;
   double service_ms;                     /* sum of the requests' solve times */
   double elapsed_ms;                     /* since the server's creation */
+  long long group_calls, group_requests; /* group solves run / requests they took */
 } kkt_serve_stats;
 /* The defaults listed above. */
 void kkt_serve_default_config(kkt_serve_config* cfg);

@@ -138,8 +138,17 @@ def test_group_argument_errors(gpu):
         dx = [torch.zeros(len(s[4]), dtype=torch.float64, device=dev) for s in syss]
         torch.cuda.synchronize()
         args = ([t.data_ptr() for t in dv], [t.data_ptr() for t in db], [t.data_ptr() for t in dx])
-        rc, _ = grp.solve(hs, *args, 1e-10, 30)
-        assert rc == -14 and "precision" in grp.last_error()
+        # an FP64 and an FP32 handle: two lockstep groups, each as its own solve
+        ref = [L.kkt_iterative_solve_device(hs[q], args[0][q], args[1][q], args[2][q], 1e-10, 30) for q in range(2)]
+        torch.cuda.synchronize()
+        x_ref = [t.cpu().numpy() for t in dx]
+        for t in dx:
+            t.zero_()
+        torch.cuda.synchronize()
+        rc, st = grp.solve(hs, *args, 1e-10, 30)
+        assert rc == 0 and list(st) == ref and min(ref) > 0, (rc, list(st), ref)
+        for q in range(2):
+            assert np.array_equal(dx[q].cpu().numpy(), x_ref[q]), q
         rc, _ = grp.solve([hs[0], hs[0]], [args[0][0]] * 2, [args[1][0]] * 2, [args[2][0]] * 2, 1e-10, 30)
         assert rc == -14 and "twice" in grp.last_error()
         rc, _ = grp.solve([unf], args[0][:1], args[1][:1], args[2][:1], 1e-10, 30)
@@ -151,3 +160,61 @@ def test_group_argument_errors(gpu):
         L.kkt_destroy(unf)
         for h in hs:
             L.kkt_destroy(h)
+
+
+def test_serving_group_runners_equal_workers(gpu):
+    # the serving loop with group runners (device requests of factored instances
+    # solved together) gives bitwise the results of the loop without them: the
+    # same step kinds, iteration counts and solutions per instance and step; with
+    # max_iter 3 some group solves do not converge and their requests are
+    # refactored and solved directly by a worker
+    _say("serving: group runners vs workers only")
+    from artisynth_core_amd.serving import SolveServer
+    dev = torch.device("cuda:0")
+    kinds = ("beam", "tet", "rigid")
+    inst = [G.serving_instance(kinds[i % 3], 700 + 400 * i, 300 + i) for i in range(9)]
+    steps = 4
+    seq = {i: [(G.perturbed_values(inst[i][2], s_, 40 + i, eps=5e-3), inst[i][3]) for s_ in range(steps)]
+           for i in range(len(inst))}
+    dv = {(i, s_): torch.from_numpy(np.ascontiguousarray(seq[i][s_][0])).to(dev)
+          for i in range(len(inst)) for s_ in range(steps)}
+    db = {i: torch.from_numpy(np.ascontiguousarray(inst[i][3], np.float64)).to(dev) for i in range(len(inst))}
+    torch.cuda.synchronize()
+    out = {}
+    for runners in (0, 2):
+        dx = {k: torch.zeros(len(inst[k[0]][3]), dtype=torch.float64, device=dev) for k in dv}
+        torch.cuda.synchronize()
+        srv = SolveServer(nworkers=3, max_iter=3, group_runners=runners, group_max=8)
+        try:
+            for i in range(len(inst)):
+                srv.register(i, inst[i][0], inst[i][1], inst[i][2])
+            rs = []
+            for s_ in range(steps):
+                for i in range(len(inst)):
+                    rs.append(((i, s_), srv.submit_device(i, dv[(i, s_)].data_ptr(), db[i].data_ptr(),
+                                                          dx[(i, s_)].data_ptr())))
+            res = {}
+            for k, r in rs:
+                r.wait()
+                assert r.error is None, (k, r.error)
+                res[k] = (r.kind, r.iters)
+            st = srv.stats()
+        finally:
+            srv.shutdown()
+        torch.cuda.synchronize()
+        out[runners] = (res, {k: t.cpu().numpy() for k, t in dx.items()}, st)
+    (r0, x0, st0), (r2, x2, st2) = out[0], out[2]
+    assert st0["group_calls"] == 0
+    assert st2["group_calls"] > 0 and st2["group_requests"] >= sum(1 for k in r2 if r2[k][0] == "iterative")
+    later = [r2[k][0] for k in r2 if k[1] > 0]
+    assert "iterative" in later and "direct" in later, later  # both outcomes of a group solve occur
+    for k in r0:
+        assert r0[k] == r2[k], (k, r0[k], r2[k])
+        assert np.array_equal(x0[k], x2[k]), k
+        i, s_ = k
+        ro, ci = inst[i][0], inst[i][1]
+        n = len(ro) - 1
+        U = sp.csr_matrix((seq[i][s_][0], ci, ro), shape=(n, n))
+        A = U + sp.triu(U, 1).T
+        b = inst[i][3]
+        assert np.linalg.norm(A @ x2[k] - b) / np.linalg.norm(b) <= 1e-8, k

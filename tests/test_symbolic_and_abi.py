@@ -255,9 +255,12 @@ def test_serving_loop_mechanics_without_gpu():
     cfg = _lib.KktServeConfig()
     L.kkt_serve_default_config(ctypes.byref(cfg))
     assert (cfg.budget_bytes, cfg.nworkers, cfg.max_iter, cfg.tol_exp) == (200 << 30, 12, 30, 10)
-    bad = _lib.KktServeConfig.from_buffer_copy(cfg)
-    bad.nworkers = 0
-    assert not L.kkt_serve_create(ctypes.byref(bad))
+    assert (cfg.group_runners, cfg.group_max) == (2, 64)
+    for field, value in (("nworkers", 0), ("group_runners", -1), ("group_runners", 9), ("group_max", 0),
+                         ("group_max", 129)):
+        bad = _lib.KktServeConfig.from_buffer_copy(cfg)
+        setattr(bad, field, value)
+        assert not L.kkt_serve_create(ctypes.byref(bad)), field
     cfg.nworkers = 2
     s = L.kkt_serve_create(ctypes.byref(cfg))
     assert s
@@ -285,6 +288,7 @@ def test_serving_loop_mechanics_without_gpu():
         st = _lib.KktServeStats()
         L.kkt_serve_get_stats(s, ctypes.byref(st))
         assert (st.submitted, st.completed, st.errors, st.admissions) == (3, 3, 3, 0)
+        assert (st.group_calls, st.group_requests) == (0, 0)  # no device: no group object
     finally:
         L.kkt_serve_destroy(s)
     L.kkt_serve_destroy(None)
