@@ -2706,6 +2706,30 @@ KKT_INST_M(double)
 // b - pre[h] and the segment's arguments, i.e. exactly what that handle's own
 // launch of the step would run in that block.
 // ------------------------------------------------------------------------
+// A pointer read from a segment's argument table is generic to the compiler
+// (flat loads and stores, which wait on both the vector-memory and the LDS
+// counters and hold their addresses in more registers).  Rebuilt from its
+// integer value as a global-address-space pointer, it gets global loads and
+// stores, as a kernel's own pointer arguments do (a direct cast from the generic
+// pointer is folded away).
+template <typename P>
+__device__ __forceinline__ P* gbl(P* p) {
+  typedef __attribute__((address_space(1))) P gp;
+  return (P*)(gp*)(unsigned long long)p;
+}
+// the symbolic arrays the solve bodies read, from a segment's table
+__device__ __forceinline__ DevSym gbl_sym(const DevSym& s) {
+  DevSym d = s;
+  d.rel = gbl(s.rel);
+  d.ilperm = gbl(s.ilperm);
+  d.lperm = gbl(s.lperm);
+  d.gptr = gbl(s.gptr);
+  d.gsrc = gbl(s.gsrc);
+  d.push_ptr = gbl(s.push_ptr);
+  d.push_pos = gbl(s.push_pos);
+  return d;
+}
+
 __device__ inline int seg_find(const int* __restrict__ pre, int nseg, int& bx) {
   __shared__ int sp[SEG_MAX + 1];
   for (int i = threadIdx.x; i <= nseg; i += 256) sp[i] = pre[i];
@@ -2731,25 +2755,29 @@ __global__ __launch_bounds__(256) void k_seg_vec(const int* __restrict__ pre, in
   const int h = seg_find(pre, nseg, bx);
   const SegVec& a = args[h];
   const int n = (int)a.n;
+  const int* idx = gbl(a.idx);
+  const double* sc = gbl(a.sc);
+  const double *va = gbl(a.a), *vb = gbl(a.b), *vc = gbl(a.c), *ve = gbl(a.e), *vg = gbl(a.g);
+  double *vd = gbl(a.d), *vf = gbl(a.f), *p1 = gbl(a.part1), *p2 = gbl(a.part2);
+  const T *tin = gbl((const T*)a.tin), *tin2 = gbl((const T*)a.tin2), *tin3 = gbl((const T*)a.tin3);
+  T* tout = gbl((T*)a.tout);
   switch (a.op) {
-    case SEG_GATHER: gather_perm_body<double, T>(bx, n, a.idx, a.a, (T*)a.tout); break;
-    case SEG_GATHER_P: gather_bicg_p_body<T>(bx, n, a.idx, a.sc, a.a, a.b, a.d, (T*)a.tout); break;
-    case SEG_SCATTER: scatter_perm_body<T, double>(bx, n, a.idx, (const T*)a.tin, a.d); break;
-    case SEG_DSOLVE:
-      dsolve_body<T>(bx, n, (const T*)a.tin, (const T*)a.tin2, a.piv, a.wpos, (const T*)a.tin3, (T*)a.tout);
-      break;
-    case SEG_EXPAND: expand_vals_body(bx, a.n, a.idx, a.a, a.d); break;
-    case SEG_FILL: fill_body(bx, a.n, a.v, a.d); break;
+    case SEG_GATHER: gather_perm_body<double, T>(bx, n, idx, va, tout); break;
+    case SEG_GATHER_P: gather_bicg_p_body<T>(bx, n, idx, sc, va, vb, vd, tout); break;
+    case SEG_SCATTER: scatter_perm_body<T, double>(bx, n, idx, tin, vd); break;
+    case SEG_DSOLVE: dsolve_body<T>(bx, n, tin, tin2, gbl(a.piv), gbl(a.wpos), tin3, tout); break;
+    case SEG_EXPAND: expand_vals_body(bx, a.n, idx, va, vd); break;
+    case SEG_FILL: fill_body(bx, a.n, a.v, vd); break;
     case SEG_COPY: {
       const long long i = (long long)bx * 256 + threadIdx.x;
-      if (i < a.n) a.d[i] = a.a[i];
+      if (i < a.n) vd[i] = va[i];
     } break;
-    case SEG_BICG_S: bicg_s_body(bx, n, a.sc, a.a, a.b, a.d, a.part1); break;
-    case SEG_BICG_XR: bicg_xr_body(bx, n, a.sc, a.a, a.b, a.c, a.e, a.d, a.f, a.g, a.part1, a.part2); break;
-    case SEG_BICG_XA: bicg_xa_body(bx, n, a.sc, a.a, a.d); break;
-    case SEG_BICG_INIT: bicg_init_body(bx, a.d); break;
+    case SEG_BICG_S: bicg_s_body(bx, n, sc, va, vb, vd, p1); break;
+    case SEG_BICG_XR: bicg_xr_body(bx, n, sc, va, vb, vc, ve, vd, vf, vg, p1, p2); break;
+    case SEG_BICG_XA: bicg_xa_body(bx, n, sc, va, vd); break;
+    case SEG_BICG_INIT: bicg_init_body(bx, vd); break;
     case SEG_COLLECT:
-      if (bx == 0 && threadIdx.x < 16) a.d[threadIdx.x] = a.a[threadIdx.x];
+      if (bx == 0 && threadIdx.x < 16) vd[threadIdx.x] = va[threadIdx.x];
       break;
     default: break;
   }
@@ -2760,10 +2788,12 @@ __global__ __launch_bounds__(256) void k_seg_red(const int* __restrict__ pre, in
   int bx;
   const int h = seg_find(pre, nseg, bx);
   const SegRed& a = args[h];
+  const double *p1 = gbl(a.p1), *p2 = gbl(a.p2);
+  double *o1 = gbl(a.out1), *o2 = gbl(a.out2), *sc = gbl(a.sc);
   switch (a.op) {
-    case SEG_RED2: reduce2_body(a.p1, a.p2, a.np, a.out1, a.out2, a.sc); break;
-    case SEG_DOTP: dot_partial_body(bx, a.n, a.p1, a.p2, a.out1); break;
-    case SEG_REDF: reduce_final_body(a.p1, a.np, a.out1, 0); break;
+    case SEG_RED2: reduce2_body(p1, p2, a.np, o1, o2, sc); break;
+    case SEG_DOTP: dot_partial_body(bx, a.n, p1, p2, o1); break;
+    case SEG_REDF: reduce_final_body(p1, a.np, o1, 0); break;
     default: break;
   }
 }
@@ -2773,7 +2803,8 @@ __global__ __launch_bounds__(256) void k_seg_spmv(const int* __restrict__ pre, i
   int bx;
   const int h = seg_find(pre, nseg, bx);
   const SegSpmv& a = args[h];
-  spmv_body(bx, a.n, a.rowptr, a.col, a.val, a.x, a.b, a.y, a.mode, a.a1, a.part1, a.part2);
+  spmv_body(bx, a.n, gbl(a.rowptr), gbl(a.col), gbl(a.val), gbl(a.x), gbl(a.b), gbl(a.y), a.mode, gbl(a.a1),
+            gbl(a.part1), gbl(a.part2));
 }
 
 template <typename T, typename S>
@@ -2782,7 +2813,9 @@ __global__ __launch_bounds__(256) void k_seg_fwd(const int* __restrict__ pre, in
   int bx;
   const int h = seg_find(pre, nseg, bx);
   const SegFwd& a = args[h];
-  fwd_level_body<T, S>(bx, a.items, a.d, (const S*)a.F, (const T*)a.bp, (T*)a.fp, (T*)a.y, (T*)a.u);
+  const DevSym d = gbl_sym(a.d);
+  fwd_level_body<T, S>(bx, gbl(a.items), d, gbl((const S*)a.F), gbl((const T*)a.bp), gbl((T*)a.fp),
+                       gbl((T*)a.y), gbl((T*)a.u));
 }
 
 template <typename T, typename S>
@@ -2791,7 +2824,8 @@ __global__ __launch_bounds__(256) void k_seg_bwd(const int* __restrict__ pre, in
   int bx;
   const int h = seg_find(pre, nseg, bx);
   const SegBwd& a = args[h];
-  bwd_cols_body<T, S>(bx, a.items, a.d, (const S*)a.F, (T*)a.wv, (T*)a.xp);
+  const DevSym d = gbl_sym(a.d);
+  bwd_cols_body<T, S>(bx, gbl(a.items), d, gbl((const S*)a.F), gbl((T*)a.wv), gbl((T*)a.xp));
 }
 
 int red_blocks() { return RED_BLOCKS; }

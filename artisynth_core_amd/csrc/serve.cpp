@@ -46,6 +46,14 @@
 
 namespace {
 
+// Forward row groups per row block of every instance's solve plan.  A group
+// solve launches all its instances' tree levels at once, tens of thousands of
+// row blocks per launch, so fewer, larger blocks amortise each block's gather:
+// 4 groups per block gave 18.1 ms against 20.7 ms for the one-handle default on
+// one stale-factor step of the 64-instance mix.  The solution is bitwise the
+// same for any choice.
+constexpr int kServeRowGroups = 4;
+
 double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -203,6 +211,10 @@ struct kkt_server {
     if (!rc) rc = kkt_analyze_values(h, in->vals0.data());
     kkt_info info{};
     if (!rc) rc = kkt_get_info(h, &info);
+    if (!rc) {
+      std::vector<int> rg((size_t)std::max(info.nlevels, 1), kServeRowGroups);
+      rc = kkt_set_forward_row_groups(h, rg.data());
+    }
     if (rc) {
       const char* m = h ? kkt_last_error(h) : nullptr;
       why = std::string("admission of instance ") + std::to_string(in->iid) + " failed: " + (m ? m : "");
