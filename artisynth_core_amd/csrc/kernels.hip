@@ -1115,8 +1115,9 @@ __device__ inline void fwd_small_front(const SolveItem& it, const DevSym& d, con
   // row p of G from the lower copy (column l, row p): consecutive threads read
   // consecutive entries (a column past the row's length re-reads column 0 and is
   // dropped by a select).  The first HOIST loads issue before the gather, the
-  // rest after it (FP64: half, so the kernel keeps more waves resident).
-  constexpr int HOIST = sizeof(T) == 8 ? FWD_SMALL_K / 2 : FWD_SMALL_K;
+  // rest after it: half of them, so the forward kernel's registers stay at 64
+  // (FP32) and eight waves fit per SIMD.
+  constexpr int HOIST = FWD_SMALL_K / 2;
   const int len = (p < m) ? min(p, k) : 0;
   const S* rowp = F + it.poff + (p < m ? prow_off(p, k) : 0);
   T v[FWD_SMALL_K];
@@ -1253,8 +1254,11 @@ __device__ inline void fwd_level_body(int bx, const SolveItem* __restrict__ item
   const int k = it.k, m = it.m, first = it.first;
   // the first PF chunks of this lane's part of its row in group 0 (the seg_dot
   // order: chunks sl, sl + seg, ..., 16 bytes of S each), loaded before the
-  // gather; the chunks are added in ascending order whatever the grouping
-  constexpr int W = 16 / sizeof(S), GC = 4, PF = 3;
+  // gather, then GC chunks per loop pass; the chunks are added in ascending
+  // order whatever the grouping.  GC = 2 (against 4) keeps the FP32 kernel at
+  // 64 registers: the same per-level time on one large system and 7 % less time
+  // when a group launch holds many systems' levels (more waves resident).
+  constexpr int W = 16 / sizeof(S), GC = 2, PF = 3;
   typedef typename V16<S>::v vt;
   typedef VecW<T, W> vw;
   vt x0[PF];
