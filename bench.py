@@ -3,11 +3,13 @@
 Workload (one GPU): the 87,567-DOF neo-Hookean hex beam 100x16x16 with 867
 attachment constraint rows (n = 88,434), a 20-step ConstrainedBackwardEuler
 sequence (h = 0.005, seeded tip loads).  One "step" = one KKT solve of that
-sequence with the reference's hybrid policy: the factor of the first step is
-reused by BiCGStab (stale-factor preconditioner, tolExp = 10) until BiCGStab
-needs more than 30 iterations, then the matrix is refactored and solved
-directly with refinement.  The FP32-MFMA + FP64-refinement path is the headline
-`value`; the same sequence with FP64 fronts is timed in the same run.
+sequence with the reference's hybrid policy (KKTSolver.factorAndSolve): the
+factor of an earlier step is reused by BiCGStab (stale-factor preconditioner,
+tolExp = 10) while estimateOptimalCount says the iterative steps still pay
+against a refactor (from the measured step times) and BiCGStab converges within
+30 iterations; otherwise the matrix is refactored and solved directly with
+refinement.  The FP32-MFMA + FP64-refinement path is the headline `value`; the
+same sequence with FP64 fronts is timed in the same run.
 
 Inputs (all 20 value arrays and right-hand sides) are resident in HBM before the
 timed region; the library is driven through its device-pointer entry points.
@@ -140,13 +142,54 @@ def spmv_bytes(n, nnz_upper, mode_resid=True):
     return float(full), float(upper), nnz_full
 
 
+class FactorAndSolvePolicy:
+    """KKTSolver.factorAndSolve's choice between the stale-factor iterative solve
+    and a factor + solve (KKTSolver.java:889-1014), on wall-clock step times:
+    after a factor, iterative steps are tried while their count stays below
+    estimateOptimalCount (the step count that minimises the average time per step
+    when iterative steps grow linearly more expensive after a factor); a failed
+    iterative attempt (more than max_iter iterations) also leads to a factor.
+    Direct times accumulate over the whole sequence, iterative times since the
+    last factor, as in the reference."""
+
+    def __init__(self):
+        self.direct_ms, self.direct_cnt = 0.0, 0
+        self.iter_ms, self.first_iter_ms, self.iter_cnt = 0.0, 0.0, 0
+
+    def estimate_optimal_count(self):
+        ft = self.direct_ms / self.direct_cnt
+        it0 = self.first_iter_ms
+        it_avg = self.iter_ms / self.iter_cnt
+        if it_avg >= ft:
+            return self.iter_cnt + 1
+        dt = 2.0 * (it_avg - it0) / self.iter_cnt
+        if dt <= 0:
+            return self.iter_cnt + 19
+        return int(np.ceil(np.sqrt(2.0 * (ft - it0 + dt) / dt)))
+
+    def try_iterative(self):
+        return self.direct_cnt > 0 and (self.iter_cnt == 0 or self.iter_cnt + 1 < self.estimate_optimal_count())
+
+    def iterative_done(self, ms):
+        self.iter_ms += ms
+        if self.iter_cnt == 0:
+            self.first_iter_ms = self.iter_ms
+        self.iter_cnt += 1
+
+    def direct_done(self, ms):
+        self.direct_ms += ms
+        self.direct_cnt += 1
+        self.iter_cnt, self.iter_ms = 0, 0.0
+
+
 def run_sequence(L, h, vals_d, b_d, x_d, steps, nsys, tol_exp=10, max_iter=30):
-    """Runs `steps` KKT solves over the systems 0..nsys-1 (cyclic) with the hybrid
-    policy of KKTSolver.factorAndSolve; returns per-step latencies (ms) and kinds."""
+    """Runs `steps` KKT solves over the systems 0..nsys-1 (cyclic) with
+    KKTSolver.factorAndSolve's policy (FactorAndSolvePolicy, a fresh one per
+    sequence); returns per-step latencies (ms), kinds and iteration counts."""
     import torch
     lat, kinds, iters = [], [], []
     tol = 10.0 ** (-tol_exp)
-    have_factor = False
+    pol = FactorAndSolvePolicy()
     for i in range(steps):
         k = i % nsys
         vp = vals_d[k].data_ptr()
@@ -154,19 +197,22 @@ def run_sequence(L, h, vals_d, b_d, x_d, steps, nsys, tol_exp=10, max_iter=30):
         xp = x_d[i % len(x_d)].data_ptr()
         t0 = time.perf_counter()
         rc = -1
-        if have_factor:
-            rc = L.kkt_iterative_solve_device(h, vp, bp, xp, tol, max_iter)
+        if pol.try_iterative():
+            rc = L.kkt_iterative_solve_device(h, vp, bp, xp, tol, max_iter)  # returns after its last copy
         if rc > 0:
+            pol.iterative_done((time.perf_counter() - t0) * 1e3)
             kinds.append("iterative")
             iters.append(rc)
         else:
+            t1 = time.perf_counter()
             st = L.kkt_factor_device(h, vp)
             if st != 0:
                 raise RuntimeError("factor failed: %s" % L.kkt_last_error(h))
             st = L.kkt_solve_device(h, bp, xp)
             if st != 0:
                 raise RuntimeError("solve failed: %s" % L.kkt_last_error(h))
-            have_factor = True
+            torch.cuda.synchronize()
+            pol.direct_done((time.perf_counter() - t1) * 1e3)
             kinds.append("direct")
             iters.append(0)
         lat.append((time.perf_counter() - t0) * 1e3)

@@ -18,9 +18,10 @@ Policy (KKTSolver.java:930-1014 with PardisoSolver.iterativeSolve,
 PardisoSolver.java:1697-1737): step 0 factors (phase 22) and solves (phase 33,
 one refinement step); each later step first runs phase 33 with iparm[3] = 102 on
 its new values (CGS preconditioned by the stale factor, pardisoMkl.cc:704-750)
-and refactors + solves when that returns <= 0 or needed more than MAX_ITER
-iterations -- the GPU sequence's rule (the stale factor is kept until the
-iterative solve exceeds 30 iterations); the failed attempt's time counts.
+while KKTSolver's estimateOptimalCount (KKTSolver.java:889-922, on this run's own
+step times) says the iterative steps still pay, and refactors + solves when it
+does not, or when CGS returns <= 0 or needed more than MAX_ITER iterations -- the
+GPU sequence's rule; the failed attempt's time counts.
 """
 import json
 import os
@@ -55,18 +56,36 @@ def main(inp, outp):
 
     def sequence(p):
         lat, kinds, iters, xs = [], [], [], []
-        have = False
+        # KKTSolver.factorAndSolve's state (KKTSolver.java:883-887, 930-1014)
+        direct_ms, direct_cnt, iter_ms, first_iter_ms, iter_cnt = 0.0, 0, 0.0, 0.0, 0
+
+        def optimal_count():  # estimateOptimalCount, KKTSolver.java:889-922
+            ft, it_avg = direct_ms / direct_cnt, iter_ms / iter_cnt
+            if it_avg >= ft:
+                return iter_cnt + 1
+            dt = 2.0 * (it_avg - first_iter_ms) / iter_cnt
+            if dt <= 0:
+                return iter_cnt + 19
+            return int(np.ceil(np.sqrt(2.0 * (ft - first_iter_ms + dt) / dt)))
+
         for k in range(nsteps):
             t0 = time.perf_counter()
             it = 0
-            if have:
+            if direct_cnt > 0 and (iter_cnt == 0 or iter_cnt + 1 < optimal_count()):
                 it, x = p.iterative_solve(V[k], B[k], 10)
             if 0 < it <= MAX_ITER:
                 kinds.append("i")
+                iter_ms += (time.perf_counter() - t0) * 1e3
+                if iter_cnt == 0:
+                    first_iter_ms = iter_ms
+                iter_cnt += 1
             else:
+                t1 = time.perf_counter()
                 p.factor(V[k])
                 x = p.solve(B[k])
-                have = True
+                direct_ms += (time.perf_counter() - t1) * 1e3
+                direct_cnt += 1
+                iter_cnt, iter_ms = 0, 0.0
                 kinds.append("D")
             lat.append((time.perf_counter() - t0) * 1e3)
             iters.append(int(it))

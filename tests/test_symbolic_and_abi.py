@@ -195,6 +195,42 @@ def test_estimate_optimal_count():
     assert k.estimateOptimalCount() == 3
 
 
+def test_bench_policy_matches_kktsolver_mirror():
+    # the bench sequence's step choice (bench.FactorAndSolvePolicy) follows
+    # KKTSolver.factorAndSolve (KKTSolver.java:930-1014): fed the same step times
+    # and outcomes, it chooses what the mirrored KKTSolver state would choose,
+    # step by step, including a failed iterative attempt that forces a factor
+    import bench
+    rng = np.random.default_rng(5)
+    pol = bench.FactorAndSolvePolicy()
+    k = KKTSolver.__new__(KKTSolver)
+    k.directTimeMsec, k.directCnt = 0.0, 0
+    k.iterativeTimeMsec, k.firstIterativeTimeMsec, k.iterativeCnt = 0.0, 0.0, 0
+    kinds = []
+    for step in range(60):
+        mirror = k.directCnt > 0 and (k.iterativeCnt == 0 or k.iterativeCnt + 1 < k.estimateOptimalCount())
+        assert pol.try_iterative() == mirror, step
+        converged = mirror and rng.uniform() > 0.1
+        if converged:
+            ms = 2.0 + 1.5 * k.iterativeCnt + rng.uniform(0, 0.5)  # cost grows after a factor
+            pol.iterative_done(ms)
+            k.iterativeTimeMsec += ms
+            if k.iterativeCnt == 0:
+                k.firstIterativeTimeMsec = k.iterativeTimeMsec
+            k.iterativeCnt += 1
+            kinds.append("i")
+        else:
+            ms = 16.0 + rng.uniform(0, 1.0)
+            pol.direct_done(ms)
+            k.directTimeMsec += ms
+            k.directCnt += 1
+            k.iterativeCnt, k.iterativeTimeMsec = 0, 0.0
+            kinds.append("D")
+    runs = "".join(kinds).split("D")[1:]
+    assert kinds[0] == "D" and max(len(r) for r in runs) < 19  # refactors before 19 iterative steps
+    assert sum(1 for r in runs if len(r) >= 3) >= 3  # and reuses factors over several steps
+
+
 def test_mooney_rivlin_tangent_is_the_stress_derivative():
     # the precision workload's material: A = dP/dF by central differences, symmetric
     # in (iJ)<->(kL), and linear elasticity (mu = 2 c10, lambda = kappa - 2mu/3) at F = I
