@@ -585,7 +585,8 @@ def main():
         keys = ("ninst", "nreq", "rate_target", "offered_rate_achieved", "workers", "budget_gb",
                 "throughput_req_s", "p50_ms", "p99_ms", "errors", "direct", "iterative",
                 "max_rel_residual_sample", "resident_gb", "evictions", "admissions_timed", "queue_max",
-                "concurrency", "total_dof", "inputs")
+                "concurrency", "total_dof", "inputs", "group_runners", "group_calls", "group_requests",
+                "mean_iterations")
         sl = serve_run(args.ninst, args.nreq, args.rate, args.workers, args.budget_gb, args.eps, emit=False,
                        group_runners=args.group_runners, group_max=args.group_max)
         line["serve"] = {k: sl[k] for k in keys}
@@ -1090,6 +1091,7 @@ def serve_run(ninst, nreq, rate, nworkers, budget_gb, eps=5e-3, emit=True, group
         x = xs[q].cpu().numpy()
         res.append(float(np.linalg.norm(A @ x - b) / np.linalg.norm(b)))
     kinds = [r.kind for r in reqs]
+    its = [r.iters for r in reqs if r.kind == "iterative"]
     offered = nreq / (t_sub - t_start)
     line = {"mode": "serve", "ninst": ninst, "nreq": nreq, "rate_target": rate,
             "offered_rate_achieved": offered, "workers": nworkers, "budget_gb": budget_gb,
@@ -1101,6 +1103,7 @@ def serve_run(ninst, nreq, rate, nworkers, budget_gb, eps=5e-3, emit=True, group
             "concurrency": (st["service_ms"] - st_start["service_ms"]) / (1e3 * (t_end - t_start)),
             "admissions_timed": st["admissions"] - st_start["admissions"],
             "group_runners": group_runners, "group_max": group_max,
+            "mean_iterations": float(np.mean(its)) if its else 0.0,
             "group_calls": st["group_calls"] - st_start["group_calls"],
             "group_requests": st["group_requests"] - st_start["group_requests"],
             "total_dof": int(sum(len(m[2]) - 1 for m in mix)), "inputs": "device (HBM), drift eps %g" % eps}
