@@ -68,7 +68,8 @@ class KktServeStats(ctypes.Structure):
                 ("budget_bytes", ctypes.c_longlong), ("bytes_resident", ctypes.c_longlong),
                 ("bytes_peak", ctypes.c_longlong), ("instances_resident", ctypes.c_int),
                 ("service_ms", ctypes.c_double), ("elapsed_ms", ctypes.c_double),
-                ("group_calls", ctypes.c_longlong), ("group_requests", ctypes.c_longlong)]
+                ("group_calls", ctypes.c_longlong), ("group_requests", ctypes.c_longlong),
+                ("releases", ctypes.c_longlong)]
 
 
 # (name, restype, argtypes) for every symbol of include/kkt_mi355x.h
@@ -117,6 +118,7 @@ SIGNATURES = [
     ("kkt_multiply", _I, [_P, _DP, _DP]),
     ("kkt_iterative_solve", _I, [_P, _DP, _DP, _DP, _D, _I]),
     ("kkt_iterative_solve_device", _I, [_P, _P, _P, _P, _D, _I]),
+    ("kkt_release_factor", _I, [_P]),
     ("kkt_group_create", _P, []),
     ("kkt_group_destroy", None, [_P]),
     ("kkt_group_iterative_solve_device", _I, [_P, _I, _PP, _PP, _PP, _PP, _D, _I, _IP]),

@@ -8,7 +8,10 @@
 // - Admission by an HBM budget: after the analysis (which holds only the
 //   structure) the handle reports what it will hold once factored; that much is
 //   reserved before the factor allocates its fronts, evicting idle resident
-//   instances least recently used first.  An instance larger than the whole
+//   instances least recently used first: an idle instance's factor is released
+//   first (its analysis stays, so its next request refactors without a new
+//   analysis), and analysis-only instances are destroyed when that is not
+//   enough.  An instance larger than the whole
 //   budget fails its requests; one that fits only after busy instances finish
 //   waits for them.  An FP64 refactor after stalled FP32 refinement is charged
 //   when it happens.
@@ -97,6 +100,7 @@ struct kkt_server {
   std::deque<Request*> fallback;    // group solves that did not converge (instance held)
   long long inflight = 0;           // taken, not yet done
   int group_live = 0;               // group runners holding a group object
+  int freeing = 0;                  // instances whose device memory is being freed
   std::map<long long, std::unique_ptr<Request>> reqs;
   std::vector<std::thread> workers;
   long long next_id = 1;
@@ -170,29 +174,71 @@ struct kkt_server {
     }
   }
 
-  bool evict_one_locked(const Instance* self) {
+  // Frees the device memory of one idle instance other than `self`, least
+  // recently used first: an instance's factor goes first (kkt_release_factor:
+  // its analysis stays resident, so coming back costs a factor, not a new
+  // analysis); when no idle instance holds a factor, an analysis-only instance
+  // is destroyed.  The victim is held busy and the lock released while the
+  // device memory is freed (stream synchronisation and hipFree), so submissions
+  // and other workers do not wait on it; while one free is in flight, other
+  // callers wait for its result instead of freeing more.  Returns false when
+  // no idle instance holds device memory.
+  bool free_one(const Instance* self, std::unique_lock<std::mutex>& lk) {
+    if (freeing > 0) {
+      cv.wait_for(lk, std::chrono::milliseconds(2));
+      return true;
+    }
     Instance* victim = nullptr;
     for (auto& kv : inst) {
       Instance* in = kv.second.get();
-      if (in == self || !in->h || in->busy) continue;
+      if (in == self || !in->h || in->busy || !in->factored) continue;
       if (!victim || in->last_used < victim->last_used) victim = in;
     }
+    bool whole = victim == nullptr;
+    if (whole) {
+      for (auto& kv : inst) {
+        Instance* in = kv.second.get();
+        if (in == self || !in->h || in->busy) continue;
+        if (!victim || in->last_used < victim->last_used) victim = in;
+      }
+    }
     if (!victim) return false;
-    kkt_destroy(victim->h);
-    victim->h = nullptr;
+    victim->busy = true;
+    freeing++;
+    kkt_solver* h = victim->h;
+    lk.unlock();
+    long long keep = 0;
+    if (!whole) {
+      kkt_info info{};
+      if (kkt_release_factor(h) == KKT_OK && kkt_get_info(h, &info) == KKT_OK) keep = info.device_bytes;
+      else whole = true;  // a failed release: drop the whole instance instead
+    }
+    if (whole) kkt_destroy(h);
+    lk.lock();
+    if (whole) {
+      victim->h = nullptr;
+      used -= victim->charged;
+      victim->charged = 0;
+      st.evictions++;
+    } else {
+      keep = std::min(victim->charged, std::max(0LL, keep));
+      used -= victim->charged - keep;
+      victim->charged = keep;
+      st.releases++;
+    }
     victim->factored = false;
-    used -= victim->charged;
-    victim->charged = 0;
-    st.evictions++;
+    victim->busy = false;
+    freeing--;
+    cv.notify_all();
     return true;
   }
 
-  // reserve `need` more bytes for instance `in` (evicting idle instances, or
-  // waiting for busy ones); false if it can never fit
+  // reserve `need` more bytes for instance `in` (freeing idle instances'
+  // memory, or waiting for busy ones); false if it can never fit
   bool reserve(Instance* in, long long need, std::unique_lock<std::mutex>& lk) {
     if (in->charged + need > cfg.budget_bytes) return false;
     while (used + need > cfg.budget_bytes) {
-      if (!evict_one_locked(in)) cv.wait_for(lk, std::chrono::milliseconds(20));
+      if (!free_one(in, lk)) cv.wait_for(lk, std::chrono::milliseconds(20));
     }
     used += need;
     in->charged += need;
@@ -236,6 +282,27 @@ struct kkt_server {
     in->h = h;
     in->factored = false;
     st.admissions++;
+    return KKT_OK;
+  }
+
+  // an instance whose factor was released: reserve the factor's memory again
+  // before the request's factor allocates it
+  int recharge(Instance* in, std::unique_lock<std::mutex>& lk, std::string& why) {
+    kkt_info info{};
+    lk.unlock();
+    const int rc = kkt_get_info(in->h, &info);
+    lk.lock();
+    if (rc) {
+      why = "instance " + std::to_string(in->iid) + ": kkt_get_info failed";
+      return rc;
+    }
+    const long long need = info.factored_bytes - in->charged;
+    if (need > 0 && !reserve(in, need, lk)) {
+      in->too_large = true;
+      why = "instance " + std::to_string(in->iid) + " needs " + std::to_string(info.factored_bytes) +
+            " bytes of device memory, more than the whole budget of " + std::to_string(cfg.budget_bytes);
+      return KKT_ERR_ALLOC;
+    }
     return KKT_OK;
   }
 
@@ -286,6 +353,8 @@ struct kkt_server {
         why = "instance " + std::to_string(in->iid) + " does not fit in the budget";
       } else if (!in->h) {
         rc = admit(in, lk, why);
+      } else if (!in->factored && !r->group_failed) {
+        rc = recharge(in, lk, why);
       }
       if (!rc) {
         lk.unlock();
@@ -305,7 +374,7 @@ struct kkt_server {
             used += extra;
             in->charged += extra;
             peak = std::max(peak, used);
-            while (used > cfg.budget_bytes && evict_one_locked(in)) {
+            while (used > cfg.budget_bytes && free_one(in, lk)) {
             }
           }
         }

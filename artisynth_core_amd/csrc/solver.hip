@@ -3454,6 +3454,20 @@ int kkt_iterative_solve(kkt_solver* h, const double* vals, const double* b, doub
   return it;
 }
 
+int kkt_release_factor(kkt_solver* h) {
+  CHECK_H(h);
+  if (h->state < kkt_solver::ANALYZED) return h->fail(KKT_ERR_STATE, "no analysis");
+  if (h->dev_ok && h->st && hipStreamSynchronize(h->st) != hipSuccess)
+    return h->fail(KKT_ERR_DEVICE, "handle stream failed");
+  // fronts, packed solve arrays, workspaces and solve vectors (and the graphs
+  // that name them); the analysis, its plan and the pattern stay resident
+  h->release_numeric();
+  h->fallback_active = false;
+  h->state = kkt_solver::ANALYZED;
+  h->ok();
+  return KKT_OK;
+}
+
 int kkt_set_ir_steps(kkt_solver* h, int nsteps) {
   CHECK_H(h);
   if (nsteps < 0) return h->fail(KKT_ERR_ARGUMENT, "negative refinement steps");

@@ -586,7 +586,7 @@ def main():
                 "throughput_req_s", "p50_ms", "p99_ms", "errors", "direct", "iterative",
                 "max_rel_residual_sample", "resident_gb", "evictions", "admissions_timed", "queue_max",
                 "concurrency", "total_dof", "inputs", "group_runners", "group_calls", "group_requests",
-                "mean_iterations")
+                "mean_iterations", "releases")
         sl = serve_run(args.ninst, args.nreq, args.rate, args.workers, args.budget_gb, args.eps, emit=False,
                        group_runners=args.group_runners, group_max=args.group_max)
         line["serve"] = {k: sl[k] for k in keys}
@@ -1100,6 +1100,7 @@ def serve_run(ninst, nreq, rate, nworkers, budget_gb, eps=5e-3, emit=True, group
             "iterative": kinds.count("iterative"), "max_rel_residual_sample": max(res),
             "resident_gb": st["bytes_resident"] / 2 ** 30, "peak_gb": st["bytes_peak"] / 2 ** 30,
             "admissions": st["admissions"], "evictions": st["evictions"], "queue_max": st["queue_max"],
+            "releases": st["releases"] - st_start["releases"],
             "concurrency": (st["service_ms"] - st_start["service_ms"]) / (1e3 * (t_end - t_start)),
             "admissions_timed": st["admissions"] - st_start["admissions"],
             "group_runners": group_runners, "group_max": group_max,
@@ -1108,8 +1109,9 @@ def serve_run(ninst, nreq, rate, nworkers, budget_gb, eps=5e-3, emit=True, group
             "group_requests": st["group_requests"] - st_start["group_requests"],
             "total_dof": int(sum(len(m[2]) - 1 for m in mix)), "inputs": "device (HBM), drift eps %g" % eps}
     log("serve: offered %.0f req/s (target %.0f), served %.0f req/s, p50 %.2f ms, p99 %.2f ms, direct %d, "
-        "evictions %d, group solves %d of %.1f requests on average, max resid %.2e" % (
-            offered, rate, line["throughput_req_s"], lat["p50_ms"], lat["p99_ms"], line["direct"], line["evictions"],
+        "releases %d, evictions %d, group solves %d of %.1f requests on average, max resid %.2e" % (
+            offered, rate, line["throughput_req_s"], lat["p50_ms"], lat["p99_ms"], line["direct"], line["releases"],
+            line["evictions"],
             line["group_calls"], line["group_requests"] / max(1, line["group_calls"]), max(res)))
     if emit:
         print(json.dumps(line), flush=True)

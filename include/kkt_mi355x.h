@@ -250,6 +250,13 @@ int kkt_group_iterative_solve_device(kkt_group* g, int nh, kkt_solver* const* hs
                                      const double* const* d_vals, const double* const* d_b,
                                      double* const* d_x, double tolRel, int maxIter, int* status);
 const char* kkt_group_last_error(kkt_group* g);
+/* Frees the factor's device memory (fronts, packed solve arrays, workspaces,
+ * solve vectors) and keeps the analysis, its plan and the pattern resident:
+ * the handle is back in the analyzed state, and the next factor allocates again
+ * and needs no new analysis (no counterpart in the reference; for callers that
+ * hold many systems under a device-memory budget, as the serving loop does when
+ * it evicts an idle instance). */
+int kkt_release_factor(kkt_solver* h);
 /* CuDssSolver.doSetIrSteps (cudssBridge.cc setIterativeRefinementSteps):
  * refinement steps after each FP64 solve (default 0; FP32 factors always
  * refine to the refinement tolerance). */
@@ -442,6 +449,8 @@ typedef struct kkt_serve_stats {
   double service_ms;                     /* sum of the requests' solve times */
   double elapsed_ms;                     /* since the server's creation */
   long long group_calls, group_requests; /* group solves run / requests they took */
+  long long releases;                    /* idle instances' factors released (analysis kept);
+                                            evictions counts instances destroyed */
 } kkt_serve_stats;
 /* The defaults listed above. */
 void kkt_serve_default_config(kkt_serve_config* cfg);

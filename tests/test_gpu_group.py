@@ -218,3 +218,37 @@ def test_serving_group_runners_equal_workers(gpu):
         A = U + sp.triu(U, 1).T
         b = inst[i][3]
         assert np.linalg.norm(A @ x2[k] - b) / np.linalg.norm(b) <= 1e-8, k
+
+
+@pytest.mark.parametrize("prec", [KKT_PREC_FP32_REFINE, KKT_PREC_FP64])
+def test_release_factor_keeps_the_analysis(gpu, prec):
+    # kkt_release_factor frees the factor's device memory and leaves the handle
+    # analyzed: solves are refused until the next factor, which needs no new
+    # analysis and gives bitwise the solution the first factor gave
+    _say("release factor, precision %d" % prec)
+    from artisynth_core_amd._lib import KktInfo
+    L = lib()
+    ro, ci, va, v1, b = _systems()[3]
+    h = _handle(L, ro, ci, va, prec)
+    try:
+        x1 = np.zeros(len(b))
+        x2 = np.zeros(len(b))
+        assert L.kkt_solve(h, nptr(b, ctypes.c_double), nptr(x1, ctypes.c_double)) == 0
+        before, after = KktInfo(), KktInfo()
+        assert L.kkt_get_info(h, ctypes.byref(before)) == 0
+        assert L.kkt_release_factor(h) == 0
+        assert L.kkt_get_info(h, ctypes.byref(after)) == 0
+        assert after.device_bytes < before.device_bytes - after.front_bytes // 2
+        assert after.nsuper == before.nsuper and after.nnzL == before.nnzL  # the analysis stays
+        assert L.kkt_solve(h, nptr(b, ctypes.c_double), nptr(x2, ctypes.c_double)) == -8  # not factored
+        assert L.kkt_factor(h, nptr(va, ctypes.c_double)) == 0, L.kkt_last_error(h)
+        assert L.kkt_solve(h, nptr(b, ctypes.c_double), nptr(x2, ctypes.c_double)) == 0
+        assert np.array_equal(x1, x2)
+        assert L.kkt_release_factor(h) == 0 and L.kkt_release_factor(h) == 0  # idempotent
+    finally:
+        L.kkt_destroy(h)
+    u = L.kkt_create()
+    try:
+        assert L.kkt_release_factor(u) == -8  # no analysis yet
+    finally:
+        L.kkt_destroy(u)
