@@ -122,6 +122,7 @@ SIGNATURES = [
     ("kkt_group_create", _P, []),
     ("kkt_group_destroy", None, [_P]),
     ("kkt_group_iterative_solve_device", _I, [_P, _I, _PP, _PP, _PP, _PP, _D, _I, _IP]),
+    ("kkt_group_iterative_solve", _I, [_P, _I, _PP, _PP, _PP, _PP, _D, _I, _IP]),
     ("kkt_group_last_error", ctypes.c_char_p, [_P]),
     ("kkt_set_ir_steps", _I, [_P, _I]),
     ("kkt_last_error", ctypes.c_char_p, [_P]),

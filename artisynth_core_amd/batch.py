@@ -129,6 +129,22 @@ class IterativeGroup:
             nptr(st, ctypes.c_int))
         return rc, st
 
+    def solve_host(self, handles, vals, b, x, tol_rel, max_iter):
+        """The same on host arrays (kkt_group_iterative_solve): vals, b, x are lists
+        of float64 numpy arrays; x[i] is written for every positive status."""
+        k = len(handles)
+        PA = ctypes.c_void_p * k
+        ins = [np.ascontiguousarray(v, np.float64) for v in vals]
+        bs = [np.ascontiguousarray(v, np.float64) for v in b]
+        for xi in x:
+            if xi.dtype != np.float64 or not xi.flags.c_contiguous:
+                raise ValueError("x arrays must be contiguous float64")
+        st = np.zeros(k, np.int32)
+        rc = self.L.kkt_group_iterative_solve(
+            self.g, k, PA(*handles), PA(*[a.ctypes.data for a in ins]), PA(*[a.ctypes.data for a in bs]),
+            PA(*[a.ctypes.data for a in x]), float(tol_rel), int(max_iter), nptr(st, ctypes.c_int))
+        return rc, st
+
     def last_error(self):
         m = self.L.kkt_group_last_error(self.g)
         return m.decode() if m else None

@@ -24,10 +24,11 @@
 // - A request is one KKT step with KKTSolver.factorAndSolve's policy
 //   (KKTSolver.java:930-1014): BiCGStab with the resident (stale) factor while it
 //   converges within max_iter, else a factor and a refined direct solve.
-// - Group solves (group_runners > 0): device-array requests of resident,
-//   factored, idle instances are taken together, up to group_max at a time, by
-//   a group runner thread and solved in one kkt_group_iterative_solve_device
-//   call (every BiCGStab step one launch for all of them); a request whose
+// - Group solves (group_runners > 0): requests of resident, factored, idle
+//   instances are taken together, up to group_max at a time, by a group runner
+//   thread and solved in one kkt_group_iterative_solve_device call for the
+//   device-array requests and one kkt_group_iterative_solve call for the
+//   host-array ones (every BiCGStab step one launch for all of them); a request whose
 //   group solve does not converge goes to a worker for the factor and direct
 //   solve, its instance still held.  Each instance's result is bitwise the one
 //   its own iterative solve gives, so the mix still does not change it.
@@ -119,9 +120,9 @@ struct kkt_server {
     return code;
   }
 
-  // a request the group runners take: device arrays, instance resident and factored
-  bool group_eligible(const Request* r, const Instance* in) const {
-    return group_live > 0 && r->device && in->h && in->factored && !in->too_large;
+  // a request the group runners take: its instance resident and factored
+  bool group_eligible(const Request*, const Instance* in) const {
+    return group_live > 0 && in->h && in->factored && !in->too_large;
   }
 
   // for a worker: a failed group solve's direct step first, else the oldest
@@ -425,6 +426,8 @@ struct kkt_server {
       });
       if (batch.empty()) break;  // stopping
       const int K = (int)batch.size();
+      // device-array requests first, then host-array ones: one group call each
+      std::stable_partition(batch.begin(), batch.end(), [](const Request* r) { return r->device; });
       hs.resize(K);
       vs.resize(K);
       bs.resize(K);
@@ -432,19 +435,25 @@ struct kkt_server {
       status.assign(K, 0);
       relres.assign(K, 0.0);
       const double ts = now_ms();
+      int nd = 0;
       for (int k = 0; k < K; k++) {
         Request* r = batch[k];
         r->res.start_ms = ts - t0;
         hs[k] = inst[r->iid]->h;
-        vs[k] = r->d_vals;
-        bs[k] = r->d_b;
+        vs[k] = r->device ? r->d_vals : r->vals.data();
+        bs[k] = r->device ? r->d_b : r->b.data();
         xs[k] = r->x;
+        nd += r->device ? 1 : 0;
       }
-      st.group_calls++;
+      st.group_calls += (nd > 0) + (nd < K);
       st.group_requests += K;
       lk.unlock();
-      (void)kkt_group_iterative_solve_device(g, K, hs.data(), vs.data(), bs.data(), xs.data(), tol, cfg.max_iter,
-                                             status.data());
+      if (nd > 0)
+        (void)kkt_group_iterative_solve_device(g, nd, hs.data(), vs.data(), bs.data(), xs.data(), tol, cfg.max_iter,
+                                               status.data());
+      if (nd < K)
+        (void)kkt_group_iterative_solve(g, K - nd, hs.data() + nd, vs.data() + nd, bs.data() + nd, xs.data() + nd,
+                                        tol, cfg.max_iter, status.data() + nd);
       for (int k = 0; k < K; k++) {
         kkt_info info{};
         if (kkt_get_info(hs[k], &info) == KKT_OK) relres[k] = info.last_rel_residual;
@@ -578,7 +587,10 @@ static long long submit(kkt_server* s, int iid, bool device, const double* vals,
   s->queue.push_back(rp);
   s->st.submitted++;
   s->st.queue_max = std::max<long long>(s->st.queue_max, (long long)s->queue.size());
-  s->cv.notify_one();
+  // workers and group runners wait on the same condition and each takes only
+  // its own kind of request: wake them all, or the one woken may not be able to
+  // take it
+  s->cv.notify_all();
   s->has_err = false;
   return rp->id;
 }

@@ -3435,6 +3435,45 @@ int kkt_group_iterative_solve_device(kkt_group* g, int nh, kkt_solver* const* hs
   return rc_all;
 }
 
+int kkt_group_iterative_solve(kkt_group* g, int nh, kkt_solver* const* hs, const double* const* vals,
+                              const double* const* b, double* const* x, double tolRel, int maxIter, int* status) {
+  if (!g) return KKT_ERR_ARGUMENT;
+  g->has_err = false;
+  if (nh < 1 || nh > KKT_GROUP_MAX) return g->fail(KKT_ERR_ARGUMENT, "group size must be 1..KKT_GROUP_MAX");
+  if (!hs || !vals || !b || !x || !status) return g->fail(KKT_ERR_ARGUMENT, "null argument");
+  // each handle's values and right-hand side go to its own device buffers (the
+  // value buffer and two solve vectors), as kkt_iterative_solve stages them;
+  // then the device-array group solve, and the solutions back
+  std::vector<const double*> dv(nh), db(nh);
+  std::vector<double*> dx(nh);
+  for (int i = 0; i < nh; i++) {
+    kkt_solver* h = hs[i];
+    if (!h || !vals[i] || !b[i] || !x[i]) return g->fail(KKT_ERR_ARGUMENT, "null handle or array");
+    if (h->state != kkt_solver::FACTORED)
+      return g->fail(KKT_ERR_STATE, "iterativeSolve called before factor (handle " + std::to_string(i) + ")");
+    for (int j = 0; j < i; j++)
+      if (hs[j] == h) return g->fail(KKT_ERR_ARGUMENT, "a handle appears twice in the group");
+    dv[i] = h->d_vals;
+    db[i] = h->vec(10);
+    dx[i] = h->vec(11);
+    if (hipMemcpyAsync(h->d_vals, vals[i], (size_t)h->nnz * sizeof(double), hipMemcpyHostToDevice, h->st) !=
+            hipSuccess ||
+        hipMemcpyAsync(h->vec(10), b[i], (size_t)h->n * sizeof(double), hipMemcpyHostToDevice, h->st) != hipSuccess)
+      return g->fail(KKT_ERR_COPY, "hipMemcpyAsync (vals, b) failed");
+  }
+  int rc = kkt_group_iterative_solve_device(g, nh, hs, dv.data(), db.data(), dx.data(), tolRel, maxIter, status);
+  for (int i = 0; i < nh; i++) {
+    if (status[i] <= 0) continue;
+    kkt_solver* h = hs[i];
+    if (hipMemcpyAsync(x[i], dx[i], (size_t)h->n * sizeof(double), hipMemcpyDeviceToHost, g->st) != hipSuccess) {
+      status[i] = h->fail(KKT_ERR_COPY, "hipMemcpyAsync (x) failed");
+      if (!rc) rc = g->fail(KKT_ERR_COPY, "hipMemcpyAsync (x) failed");
+    }
+  }
+  if (hipStreamSynchronize(g->st) != hipSuccess && !rc) rc = g->fail(KKT_ERR_DEVICE, "group stream failed");
+  return rc;
+}
+
 int kkt_iterative_solve(kkt_solver* h, const double* vals, const double* b, double* x, double tolRel,
                         int maxIter) {
   CHECK_H(h);

@@ -249,6 +249,12 @@ void kkt_group_destroy(kkt_group* g);
 int kkt_group_iterative_solve_device(kkt_group* g, int nh, kkt_solver* const* hs,
                                      const double* const* d_vals, const double* const* d_b,
                                      double* const* d_x, double tolRel, int maxIter, int* status);
+/* The same on host arrays: vals[i], b[i] are copied to handle i's own device
+ * buffers (as kkt_iterative_solve does) and x[i] is written on return for every
+ * handle whose status is positive; bitwise what kkt_iterative_solve gives. */
+int kkt_group_iterative_solve(kkt_group* g, int nh, kkt_solver* const* hs, const double* const* vals,
+                              const double* const* b, double* const* x, double tolRel, int maxIter,
+                              int* status);
 const char* kkt_group_last_error(kkt_group* g);
 /* Frees the factor's device memory (fronts, packed solve arrays, workspaces,
  * solve vectors) and keeps the analysis, its plan and the pattern resident:

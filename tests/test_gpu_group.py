@@ -252,3 +252,35 @@ def test_release_factor_keeps_the_analysis(gpu, prec):
         assert L.kkt_release_factor(u) == -8  # no analysis yet
     finally:
         L.kkt_destroy(u)
+
+
+def test_group_host_arrays_equal_single_handle_solves(gpu):
+    # kkt_group_iterative_solve (host arrays, staged through each handle's own
+    # device buffers) gives bitwise what kkt_iterative_solve gives per handle
+    _say("group iterative solve on host arrays vs single handles")
+    L = lib()
+    syss = _systems()
+    ha = [_handle(L, ro, ci, va, KKT_PREC_FP32_REFINE) for ro, ci, va, v1, b in syss]
+    hb = [_handle(L, ro, ci, va, KKT_PREC_FP32_REFINE) for ro, ci, va, v1, b in syss]
+    grp = IterativeGroup()
+    try:
+        xa = [np.zeros(len(s[4])) for s in syss]
+        xb = [np.full(len(s[4]), 7.0) for s in syss]
+        single = [L.kkt_iterative_solve(ha[q], nptr(s[3], ctypes.c_double), nptr(s[4], ctypes.c_double),
+                                        nptr(xa[q], ctypes.c_double), 1e-10, 30) for q, s in enumerate(syss)]
+        rc, st = grp.solve_host(hb, [s[3] for s in syss], [s[4] for s in syss], xb, 1e-10, 30)
+        assert rc == 0, grp.last_error()
+        assert list(st) == single and min(single) > 0, (list(st), single)
+        for q in range(len(syss)):
+            assert np.array_equal(xa[q], xb[q]), q
+        # an iteration limit: the handles that stop keep their x untouched, as the single call does
+        xc = [np.full(len(s[4]), 7.0) for s in syss]
+        rc, st = grp.solve_host(hb, [s[3] for s in syss], [s[4] for s in syss], xc, 1e-10, 2)
+        assert rc == 0
+        for q in range(len(syss)):
+            if st[q] <= 0:
+                assert np.all(xc[q] == 7.0)
+    finally:
+        grp.dispose()
+        for h in ha + hb:
+            L.kkt_destroy(h)
