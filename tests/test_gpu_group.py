@@ -284,3 +284,47 @@ def test_group_host_arrays_equal_single_handle_solves(gpu):
         grp.dispose()
         for h in ha + hb:
             L.kkt_destroy(h)
+
+
+def test_group_of_kkt_group_max_handles(gpu):
+    # a full group (KKT_GROUP_MAX = 128 handles, the reductions' launches then
+    # hold 256 segments, the tables' limit): every handle's solution and count
+    # bitwise its own iterative solve's
+    _say("group iterative solve with 128 handles")
+    L = lib()
+    dev = torch.device("cuda:0")
+    kinds = ("beam", "tet", "rigid")
+    hs, dv, db, dx, single, xs = [], [], [], [], [], []
+    try:
+        for q in range(128):
+            ro, ci, va, b = G.serving_instance(kinds[q % 3], 150 + 7 * q, 5000 + q)
+            ro, ci, va, b = (np.asarray(ro, np.int32), np.asarray(ci, np.int32), np.asarray(va, np.float64),
+                             np.asarray(b, np.float64))
+            v1 = va * (1.0 + 2e-3 * np.random.default_rng(q).uniform(-1, 1, va.shape))
+            hs.append(_handle(L, ro, ci, va, KKT_PREC_FP32_REFINE))
+            dv.append(torch.from_numpy(v1).to(dev))
+            db.append(torch.from_numpy(b).to(dev))
+            dx.append(torch.zeros(len(b), dtype=torch.float64, device=dev))
+        torch.cuda.synchronize()
+        for q in range(128):
+            single.append(L.kkt_iterative_solve_device(hs[q], dv[q].data_ptr(), db[q].data_ptr(), dx[q].data_ptr(),
+                                                       1e-10, 30))
+        torch.cuda.synchronize()
+        xs = [t.cpu().numpy() for t in dx]
+        for t in dx:
+            t.fill_(3.0)
+        torch.cuda.synchronize()
+        grp = IterativeGroup()
+        try:
+            rc, st = grp.solve(hs, [t.data_ptr() for t in dv], [t.data_ptr() for t in db], [t.data_ptr() for t in dx],
+                               1e-10, 30)
+            assert rc == 0, grp.last_error()
+        finally:
+            grp.dispose()
+        torch.cuda.synchronize()
+        assert list(st) == single and min(single) > 0
+        for q in range(128):
+            assert np.array_equal(dx[q].cpu().numpy(), xs[q]), q
+    finally:
+        for h in hs:
+            L.kkt_destroy(h)
