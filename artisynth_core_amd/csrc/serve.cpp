@@ -21,9 +21,14 @@
 //   on their streams.  An instance's requests run in submission order, one at a
 //   time, on its own handle, so its results do not depend on the mix (an
 //   evicted instance refactors on its next request).
-// - A request is one KKT step with KKTSolver.factorAndSolve's policy
-//   (KKTSolver.java:930-1014): BiCGStab with the resident (stale) factor while it
-//   converges within max_iter, else a factor and a refined direct solve.
+// - A request is one KKT step with the iteration-limit part of
+//   KKTSolver.factorAndSolve's policy (KKTSolver.java:930-1014): BiCGStab with
+//   the resident (stale) factor while it converges within max_iter, else a
+//   factor and a refined direct solve.  The policy's other part, refactoring
+//   early when estimateOptimalCount (KKTSolver.java:889-922) says so, reads one
+//   simulation's own step times; in a shared server a request's time includes
+//   the other instances' work (a group solve is shared by all its requests), so
+//   the loop does not apply it.
 // - Group solves (group_runners > 0): requests of resident, factored, idle
 //   instances are taken together, up to group_max at a time, by a group runner
 //   thread and solved in one kkt_group_iterative_solve_device call for the

@@ -12,10 +12,13 @@ its worker threads, admission and eviction are C++ in csrc/serve.cpp).
   whose instance is idle (continuous batching: arrivals start as soon as a
   worker and memory are free; instances' device work overlaps on their
   streams).  An instance's requests run in submission order, one at a time.
-- A request is one KKT step with KKTSolver.factorAndSolve's policy
-  (KKTSolver.java:930-1014): BiCGStab with the resident factor while it
-  converges within `max_iter`, else a refactor and a refined direct solve.
-- With `group_runners` > 0, device-array requests of resident, factored, idle
+- A request is one KKT step with the iteration-limit part of
+  KKTSolver.factorAndSolve's policy (KKTSolver.java:930-1014): BiCGStab with the
+  resident factor while it converges within `max_iter`, else a refactor and a
+  refined direct solve (the timing-based early refactor of estimateOptimalCount
+  is not applied: a request's time in a shared server includes other
+  instances' work).
+- With `group_runners` > 0, requests of resident, factored, idle
   instances are solved together (up to `group_max` per call) by group runner
   threads, each BiCGStab step one launch for all of them; a request that does
   not converge there is refactored and solved directly by a worker.  Results

@@ -424,9 +424,11 @@ const char* kkt_batch_last_error(kkt_batch* b);
  * each taking the oldest request whose instance is idle (continuous batching:
  * arrivals start as soon as a worker and memory are free, instances' device
  * work overlaps on their streams).  An instance's requests run in submission
- * order, one at a time, each a KKT step with KKTSolver.factorAndSolve's policy
- * (KKTSolver.java:930-1014): BiCGStab with the resident factor while it
- * converges within max_iter iterations, else a factor and a refined solve. */
+ * order, one at a time, each a KKT step with the iteration-limit part of
+ * KKTSolver.factorAndSolve's policy (KKTSolver.java:930-1014): BiCGStab with the
+ * resident factor while it converges within max_iter iterations, else a factor
+ * and a refined solve (no timing-based early refactor: a request's time in a
+ * shared server includes other instances' work). */
 typedef struct kkt_server kkt_server;
 typedef struct kkt_serve_config {
   long long budget_bytes; /* device memory the resident instances may hold (default 200 GiB) */
