@@ -576,7 +576,7 @@ def main():
         "roofline_factor": roof_factor, "roofline_factor_update": roof_update, "cpu_baseline": cpu,
     }
     if not args.no_mix:
-        # the other two north-star workloads, measured in the same run (after the
+        # the batching and serving workloads, measured in the same run (after the
         # headline sequence; their own timed regions)
         bl = batch_run(args.nbatch, args.steps, args.warmup, emit=False)
         line["batch"] = {k: bl[k] for k in ("nsys", "n_total", "batched_ms", "per_system_ms", "speedup",
@@ -1242,7 +1242,7 @@ def precision_run(dims, nsteps, warmup, with_oracle, max_super=2048):
 
 
 def cpu_baseline(beam, V, B, x64, x32):
-    """CPU baseline on the GPU box's host: MKL PARDISO with the reference's
+    """CPU baseline on the host cores of the machine running the bench: MKL PARDISO with the reference's
     settings, dlopened from the host's MKL install and driven through the same
     step sequence with KKTSolver.factorAndSolve's policy (oracle/pardiso_run.py,
     a child process: PARDISO fails in a process that has loaded the GPU

@@ -1,5 +1,5 @@
 """Writes the golden fixtures under tests/golden/ from data files the reference's
-own tests hold (run in the build container; the GPU box never reads the reference).
+own tests hold (run where the reference's files are; the GPU tests read only the fixtures).
 
 testMatrix_upper.npz: src/maspack/solvers/testMatrix.mat (PardisoSolverTest.java:
 608-688), a 2529x2529 symmetric matrix given as "[ (i j v) ... ]" followed by a

@@ -100,7 +100,8 @@ def test_beam_brute_force_and_inertia(nx):
     assert inf["nneg"] == beam.numG
 
 
-def test_beam_sizes_match_survey():
+def test_small_beam_sizes():
+    # the 8- and 26-element beams of the reference's CPU-runnable case: n and stored entries
     beam, vals, b = G.small_beam(8)
     assert (beam.n, beam.nnz) == (270, 5688)
     beam, vals, b = G.small_beam(26)
